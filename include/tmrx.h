@@ -1,0 +1,211 @@
+/*
+  tmrx -- C-ABI of the MI355X hex8 topology-optimisation solve path.
+
+  Plain C types only: opaque handles, int status codes (0 = success,
+  negative = error, message from tmrx_last_error()), caller-owned buffers or
+  borrowed pointers whose lifetime is stated per call.
+
+  Each entry point names the reference interface it replaces (file:line in
+  smdogroup/tmr).  The reference has no status codes: it prints to stderr and
+  returns NULL/0; here every failure is reported through the return value.
+*/
+#ifndef TMRX_H
+#define TMRX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tmrx_forest tmrx_forest;
+typedef struct tmrx_problem tmrx_problem;
+
+/* Same layout as TMROctant (src/TMROctant.h:48-53). */
+typedef struct {
+  int32_t block;
+  int32_t x, y, z;
+  int32_t tag;
+  int16_t level;
+  int16_t info;
+} tmrx_octant;
+
+/* Last error message of the calling thread ("" when none). */
+const char *tmrx_last_error(void);
+/* Library version string. */
+const char *tmrx_version(void);
+
+/* ---------------------------------------------------------------------------
+   Forest: TMROctForest (src/TMROctForest.h:51-181), one process.
+   ------------------------------------------------------------------------- */
+
+/* TMROctForest::TMROctForest(comm, mesh_order, interp_type)
+   (src/TMROctForest.h:51-52).  Only mesh_order 2 builds nodes. */
+int tmrx_forest_create(int mesh_order, int interp_type, tmrx_forest **out);
+/* TMREntity::decref (src/TMRBase.h:151-188): frees the forest. */
+void tmrx_forest_free(tmrx_forest *f);
+
+/* TMROctForest::setConnectivity (src/TMROctForest.cpp:558-590). */
+int tmrx_forest_set_connectivity(tmrx_forest *f, int num_nodes, const int *block_conn,
+                                 int num_blocks);
+/* TMROctForest::getConnectivity (src/TMROctForest.cpp:1633-1692).
+   Borrowed pointers valid until the next set_connectivity. */
+int tmrx_forest_get_connectivity(tmrx_forest *f, int *nblocks, int *nfaces, int *nedges,
+                                 int *nnodes, const int **block_conn,
+                                 const int **block_face_conn, const int **block_edge_conn,
+                                 const int **block_face_ids);
+/* Replaces TMROctForest::setTopology (src/TMROctForest.cpp:508-534): block
+   vertex coordinates (num_nodes x 3); every block is mapped trilinearly. */
+int tmrx_forest_set_node_points(tmrx_forest *f, const double *Xpts);
+/* Replaces the entity names of TMRTopology used by getNodesWithName /
+   getOctsWithName: kind 0 vertex, 1 edge, 2 face, 3 volume (block). */
+int tmrx_forest_set_name(tmrx_forest *f, int kind, int index, const char *name);
+
+/* TMROctForest::createTrees (src/TMROctForest.cpp:1744-1833). */
+int tmrx_forest_create_trees(tmrx_forest *f, int level);
+/* TMROctForest::refine (src/TMROctForest.cpp:2169-2329); refinement may be
+   NULL (refine everything by one level). */
+int tmrx_forest_refine(tmrx_forest *f, const int *refinement, int min_level, int max_level);
+/* TMROctForest::balance (src/TMROctForest.cpp:2917-3089). */
+int tmrx_forest_balance(tmrx_forest *f, int balance_corner);
+/* TMROctForest::coarsen (src/TMROctForest.cpp:2119-2164). New handle. */
+int tmrx_forest_coarsen(tmrx_forest *f, tmrx_forest **out);
+/* TMROctForest::duplicate (src/TMROctForest.cpp:2097-2109). New handle. */
+int tmrx_forest_duplicate(tmrx_forest *f, tmrx_forest **out);
+/* TMROctForest::repartition (src/TMROctForest.cpp:1922-2095): one process
+   owns everything, so this only validates the handle. */
+int tmrx_forest_repartition(tmrx_forest *f);
+/* TMROctForest::createNodes (src/TMROctForest.cpp:4064-4268). */
+int tmrx_forest_create_nodes(tmrx_forest *f);
+
+/* TMROctForest::getOctants (src/TMROctForest.cpp:1452-1456).  Borrowed,
+   valid until the forest is refined/balanced/freed. */
+int tmrx_forest_get_octants(tmrx_forest *f, const tmrx_octant **octs, int *n);
+/* TMROctForest::getNodeConn (src/TMROctForest.cpp:5686-5704).  Borrowed. */
+int tmrx_forest_get_node_conn(tmrx_forest *f, const int **conn, int *num_elements,
+                              int *num_owned_nodes);
+/* TMROctForest::getDepNodeConn (src/TMROctForest.cpp:5717-5729).
+   Returns the number of dependent nodes (>= 0) or an error (< 0). */
+int tmrx_forest_get_dep_node_conn(tmrx_forest *f, const int **ptr, const int **conn,
+                                  const double **weights);
+/* TMROctForest::getOwnedNodeRange (src/TMROctForest.cpp:1442-1447):
+   range[0] = 0, range[1] = number of independent nodes. */
+int tmrx_forest_get_owned_node_range(tmrx_forest *f, int range[2]);
+/* TMROctForest::getNodeNumbers (src/TMROctForest.cpp:1461-1466). Borrowed. */
+int tmrx_forest_get_node_numbers(tmrx_forest *f, const int **nums, int *n);
+/* TMROctForest::getPoints (src/TMROctForest.cpp:1476-1481): num_local x 3,
+   in the order of the sorted node numbers.  Borrowed. */
+int tmrx_forest_get_points(tmrx_forest *f, const double **X, int *n);
+/* TMROctForest::getNodesWithName (src/TMROctForest.cpp:5882-6203).
+   Writes up to cap entries, returns the full count (call with cap 0 to size). */
+int tmrx_forest_get_nodes_with_name(tmrx_forest *f, const char *name, int *out, int cap);
+/* TMROctForest::getOctsWithName (src/TMROctForest.cpp:5747-5862).  info is
+   the local face index, tag the element index. Same sizing rule. */
+int tmrx_forest_get_octs_with_name(tmrx_forest *f, const char *name, tmrx_octant *out,
+                                   int cap);
+/* TMROctForest::createInterpolation (src/TMROctForest.cpp:6611-6793) into
+   CSR instead of a TACSBVecInterp: rows are the fine independent nodes,
+   columns the coarse ones, sorted and merged as TMRIndexWeight::uniqueSort.
+   Borrowed arrays stored in the fine forest, valid until its next call. */
+int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int *nrows,
+                                     int *ncols, const int **rowp, const int **cols,
+                                     const double **weights);
+
+
+/* ---------------------------------------------------------------------------
+   Problem: the finite-element hierarchy and solver that TMR hands to TACS.
+   One handle owns every multigrid level on one GPU.
+   ------------------------------------------------------------------------- */
+
+/* TMRStiffnessProperties (src/topology/TMROctConstitutive.cpp:26-71) with a
+   single isotropic material (TACSMaterialProperties E, nu, ys).
+   penalty_type 0 = SIMP (rho^q + k0), 1 = RAMP. */
+typedef struct {
+  double E, nu, ys;
+  double q, k0, eps;
+  int penalty_type;
+} tmrx_material;
+
+typedef struct {
+  double rtol;      /* relative residual tolerance of the FP64 PCG */
+  double atol;      /* absolute residual tolerance */
+  int max_iters;
+  int precision;    /* 0: FP64 V-cycle, 1: FP32 V-cycle inside FP64 PCG */
+  int cheb_degree;  /* Chebyshev smoother degree */
+  double cheb_lower, cheb_upper; /* interval as fractions of lambda_max */
+  int eig_iters;    /* power iterations for lambda_max(D^-1 A) */
+  int monitor;      /* print the residual history to stderr */
+  int zero_guess;   /* start from u = 0 */
+} tmrx_solve_opts;
+
+typedef struct {
+  int iters;
+  int converged;
+  double rnorm0, rnorm;   /* |b| and the final recursive residual norm */
+  double setup_ms, solve_ms;
+} tmrx_solve_info;
+
+/* Defaults: rtol 1e-8, FP32 V-cycle, Chebyshev degree 2 on
+   [1/30, 1.1] lambda_max, 200 iterations. */
+void tmrx_solve_opts_default(tmrx_solve_opts *o);
+
+/* Replaces TopOptUtils.createTopoProblem's assembler/multigrid construction
+   (tmr/TopOptUtils.py:14-128): TMROctTACSCreator::createTACS per level
+   (src/TMR_TACSCreator.cpp:332-461) and TMR_CreateTACSMg
+   (src/TMR_RefinementTools.cpp:37-107).  forests[0] is the finest level;
+   each coarser forest must nest in the finer one.  Boundary conditions are
+   TMRBoundaryConditions entries (src/TMRBoundaryConditions.cpp:27-49): a
+   name and a dof bit mask (bit d clamps displacement component d; 7 = all),
+   applied on every level by getNodesWithName.  device = HIP device index. */
+int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat,
+                        int nbcs, const char **bc_names, const int *bc_dof_masks, int device,
+                        tmrx_problem **out);
+void tmrx_problem_free(tmrx_problem *p);
+/* Sizes of the finest level. */
+int tmrx_problem_get_sizes(tmrx_problem *p, int *ndof, int *nelem, int *nlevels);
+/* HIP stream (hipStream_t) used by every later call; NULL = default stream. */
+int tmrx_problem_set_stream(tmrx_problem *p, void *stream);
+
+/* Element densities on the finest level (one per element, element order of
+   the forest); on_device != 0 means rho is a device pointer.  Replaces
+   TMRTopoProblem::setDesignVars (src/topology/TMRTopoProblem.cpp:974) for
+   element-constant densities. */
+int tmrx_problem_set_design(tmrx_problem *p, const double *rho, int n, int on_device);
+/* Density-dependent multigrid setup (diagonals, eigenvalue bounds, coarse
+   Cholesky): mg->assembleJacobian + mg->factor
+   (src/topology/TMRTopoProblem.cpp:1000-1001). */
+int tmrx_problem_setup(tmrx_problem *p, const tmrx_solve_opts *opts);
+/* K(rho) u = f by FP64 flexible PCG with one multigrid V-cycle as the
+   preconditioner; replaces ksm->solve (src/topology/TMRTopoProblem.cpp:1003-1006).
+   f and u have ndof entries (3 per node, interleaved); on_device selects
+   device pointers.  Returns 0 when converged, 1 when not, < 0 on error. */
+int tmrx_problem_solve(tmrx_problem *p, const double *f, double *u, int on_device,
+                       const tmrx_solve_opts *opts, tmrx_solve_info *info);
+/* Relative true residual |f - K u| / |f| of the last solve (FP64 operator). */
+int tmrx_problem_true_residual(tmrx_problem *p, double *rel);
+/* Compliance f^T u of the last solve (src/topology/TMRTopoProblem.cpp:1034). */
+int tmrx_problem_compliance(tmrx_problem *p, double *c);
+/* d compliance / d rho_e for every element: the compliance branch of
+   TMRTopoProblem::evalObjConGradient (src/topology/TMRTopoProblem.cpp:1275-1280),
+   -psi^T dK/drho u with psi = u. */
+int tmrx_problem_compliance_grad(tmrx_problem *p, double *g, int on_device);
+/* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32). */
+int tmrx_problem_apply(tmrx_problem *p, const double *x, double *y, int on_device, int precision);
+/* Consistent nodal load of a constant traction tr[3] on the faces named
+   `name`: TopOptUtils.compute3DTractionLoad (tmr/TopOptUtils.py:241-267).
+   f has ndof entries (host). */
+int tmrx_problem_traction_load(tmrx_problem *p, const char *name, const double tr[3], double *f);
+
+/* Measurement hooks (no counterpart in the reference, which records no
+   timings): HIP events around every finest-level element-pass launch on the
+   problem's stream while enabled.  result: summed milliseconds, launch count
+   and a mask of the precisions seen (1 FP32, 2 FP64). */
+int tmrx_problem_kernel_timing(tmrx_problem *p, int enable);
+int tmrx_problem_kernel_timing_result(tmrx_problem *p, double *total_ms, int *count,
+                                      int *precision_mask);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMRX_H */

@@ -1,0 +1,107 @@
+"""GPU parity of the hex8 apply, the MG-PCG solve, compliance and its
+gradient against the FP64 CPU oracle (oracle/fe_oracle.c)."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tmr_amd import solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+BCS = [("fixed", 7)]
+Q, K0 = 3.0, 1e-6
+
+
+def _oracle(forest, rho):
+    orc = po.from_forest(forest, BCS)
+    orc.assemble(np.array([po.penalty(r, Q, K0) for r in rho]) if len(rho) < 200000 else
+                 rho ** Q + K0)
+    return orc
+
+
+def _load(prob):
+    return prob.traction_load("load", [0.0, 0.0, -1.0])
+
+
+@pytest.fixture(scope="module")
+def cube():
+    forests, rho, L = workloads.cube_l4()
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_design(rho)
+    return forests, rho, prob
+
+
+@pytest.fixture(scope="module")
+def adapt():
+    forests, rho, L = workloads.adaptive(seed=3, base_level=2, max_level=4, nlevels=3)
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_design(rho)
+    return forests, rho, prob
+
+
+@pytest.mark.parametrize("which", ["cube", "adapt"])
+def test_apply_parity(which, cube, adapt):
+    forests, rho, prob = cube if which == "cube" else adapt
+    print("apply parity", which, "nelem", prob.nelem, "ndof", prob.ndof, flush=True)
+    orc = _oracle(forests[0], rho)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(prob.ndof)
+    # identity rows on clamped dofs in both operators
+    y_ref = orc.matvec(x)
+    y64 = prob.apply(x, precision=0)
+    y32 = prob.apply(x, precision=1)
+    e64 = np.linalg.norm(y64 - y_ref) / np.linalg.norm(y_ref)
+    e32 = np.linalg.norm(y32 - y_ref) / np.linalg.norm(y_ref)
+    print("rel err fp64 %.3e fp32 %.3e" % (e64, e32), flush=True)
+    assert e64 < 1e-13
+    assert e32 < 1e-5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("which", ["cube", "adapt"])
+def test_solve_compliance_parity(which, precision, cube, adapt):
+    forests, rho, prob = cube if which == "cube" else adapt
+    print("solve parity", which, "precision", precision, flush=True)
+    f = _load(prob)
+    orc = _oracle(forests[0], rho)
+    u_ref = orc.solve(f, rtol=1e-13)
+    c_ref = float(f @ u_ref)
+    opts = solver.default_opts(rtol=1e-10, precision=precision)
+    u = prob.solve(f, opts=opts)
+    c = prob.compliance()
+    tr = prob.true_residual()
+    print("iters", prob.info.iters, "true res %.3e" % tr, "c %.12g ref %.12g" % (c, c_ref), flush=True)
+    assert prob.info.converged
+    assert tr < 1e-8
+    assert abs(c - c_ref) / abs(c_ref) < 1e-6
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-6
+
+
+def test_compliance_gradient_parity(cube):
+    forests, rho, prob = cube
+    f = _load(prob)
+    prob.solve(f, opts=solver.default_opts(rtol=1e-11, precision=1))
+    g = prob.compliance_grad()
+    orc = _oracle(forests[0], rho)
+    u_ref = orc.solve(f, rtol=1e-13)
+    e = orc.energy(u_ref)
+    g_ref = -np.array([po.dpenalty(r, Q) for r in rho]) * e
+    err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print("grad rel err %.3e" % err, flush=True)
+    assert err < 1e-6
+
+
+def test_uniform_density_invariant(cube):
+    """c(rho) = c(1) (1 + k0) / (rho^q + k0) for a uniform density."""
+    forests, rho, prob = cube
+    f = _load(prob)
+    opts = solver.default_opts(rtol=1e-11, precision=1)
+    prob.set_design(np.full(prob.nelem, 0.5))
+    prob.solve(f, opts=opts)
+    c_half = prob.compliance()
+    prob.set_design(np.full(prob.nelem, 1.0))
+    prob.solve(f, opts=opts)
+    c_one = prob.compliance()
+    prob.set_design(rho)
+    ratio = c_half / (c_one * (1 + K0) / (0.5 ** Q + K0))
+    assert abs(ratio - 1) < 1e-8

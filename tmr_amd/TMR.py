@@ -1,0 +1,272 @@
+"""Python mirror of the reference's TMR module for the hex8 solve path.
+
+Class and method names, argument order and return types follow the Cython
+binding tmr/TMR.pyx (OctForest :3393-3790, BoundaryConditions, createMg
+:4317-4368, StiffnessProperties :5041-5184, TopoProblem :5371-6000).  MPI
+communicators are accepted and ignored: one process owns the whole problem.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+UNIFORM_POINTS = 0
+GAUSS_LOBATTO_POINTS = 1
+BERNSTEIN_POINTS = 2
+MAX_LEVEL = 30
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _octants_to_array(ptr, n):
+    dt = np.dtype([("block", np.int32), ("x", np.int32), ("y", np.int32), ("z", np.int32),
+                   ("tag", np.int32), ("level", np.int16), ("info", np.int16)])
+    if n == 0:
+        return np.zeros(0, dtype=dt)
+    buf = (C.c_char * (24 * n)).from_address(C.addressof(ptr.contents))
+    return np.frombuffer(bytes(buf), dtype=dt).copy()
+
+
+class OctForest:
+    """TMR.OctForest (tmr/TMR.pyx:3393-3790) on the native forest."""
+
+    def __init__(self, comm=None, order=2, interp=GAUSS_LOBATTO_POINTS, _handle=None):
+        self.comm = comm
+        if _handle is not None:
+            self.ptr = _handle
+        else:
+            h = C.c_void_p()
+            check(lib().tmrx_forest_create(order, interp, C.byref(h)), "OctForest")
+            self.ptr = h
+        self._order = order
+        self._interp = interp
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().tmrx_forest_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+    # -- setup --------------------------------------------------------------
+    def getMeshOrder(self):
+        return self._order
+
+    def getInterpType(self):
+        return self._interp
+
+    def setMeshOrder(self, order, interp=GAUSS_LOBATTO_POINTS):
+        if order != 2:
+            raise ValueError("only mesh order 2 (trilinear hex8) is supported")
+        self._order, self._interp = order, interp
+
+    def setConnectivity(self, conn):
+        conn = np.ascontiguousarray(conn, dtype=np.intc)
+        if conn.ndim != 2 or conn.shape[1] != 8:
+            raise ValueError("Expected array of shape (n, 8)")
+        num_nodes = int(conn.max()) + 1
+        check(lib().tmrx_forest_set_connectivity(self.ptr, num_nodes, _iptr(conn), conn.shape[0]),
+              "setConnectivity")
+
+    def setNodePoints(self, X):
+        """Block vertex coordinates (num_nodes x 3); stands in for setTopology."""
+        X = np.ascontiguousarray(X, dtype=np.double)
+        check(lib().tmrx_forest_set_node_points(self.ptr, _dptr(X)), "setNodePoints")
+        self._Xv = X.copy()
+
+    def getElementExtents(self):
+        """Edge lengths (nelem x 3) of every element of a forest of box blocks."""
+        octs = self.getOctants()
+        bc = self.getConnectivity()["block_conn"]
+        X = self._Xv
+        ext = X[bc[:, 7]] - X[bc[:, 0]]
+        return ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
+
+    def setName(self, kind, index, name):
+        kinds = {"vertex": 0, "edge": 1, "face": 2, "volume": 3, "block": 3}
+        k = kinds[kind] if isinstance(kind, str) else int(kind)
+        check(lib().tmrx_forest_set_name(self.ptr, k, int(index), name.encode()), "setName")
+
+    def getConnectivity(self):
+        nb, nf, ne, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        bc, bfc, bec, bfi = (C.POINTER(C.c_int)() for _ in range(4))
+        check(lib().tmrx_forest_get_connectivity(self.ptr, C.byref(nb), C.byref(nf), C.byref(ne),
+                                                 C.byref(nn), C.byref(bc), C.byref(bfc),
+                                                 C.byref(bec), C.byref(bfi)), "getConnectivity")
+        b = nb.value
+        return dict(nblocks=b, nfaces=nf.value, nedges=ne.value, nnodes=nn.value,
+                    block_conn=np.ctypeslib.as_array(bc, (8 * b,)).reshape(b, 8).copy(),
+                    block_face_conn=np.ctypeslib.as_array(bfc, (6 * b,)).reshape(b, 6).copy(),
+                    block_edge_conn=np.ctypeslib.as_array(bec, (12 * b,)).reshape(b, 12).copy(),
+                    block_face_ids=np.ctypeslib.as_array(bfi, (6 * b,)).reshape(b, 6).copy())
+
+    def repartition(self, max_rank=-1):
+        check(lib().tmrx_forest_repartition(self.ptr), "repartition")
+
+    def createTrees(self, depth=0):
+        check(lib().tmrx_forest_create_trees(self.ptr, int(depth)), "createTrees")
+
+    def refine(self, refine=None, min_lev=0, max_lev=MAX_LEVEL):
+        if refine is not None:
+            refine = np.ascontiguousarray(refine, dtype=np.intc)
+            n = len(self.getOctants())
+            if refine.shape != (n,):
+                raise ValueError("refine must have one entry per octant")
+            check(lib().tmrx_forest_refine(self.ptr, _iptr(refine), int(min_lev), int(max_lev)), "refine")
+        else:
+            check(lib().tmrx_forest_refine(self.ptr, None, int(min_lev), int(max_lev)), "refine")
+
+    def duplicate(self):
+        h = C.c_void_p()
+        check(lib().tmrx_forest_duplicate(self.ptr, C.byref(h)), "duplicate")
+        f = OctForest(self.comm, self._order, self._interp, _handle=h)
+        f._Xv = getattr(self, "_Xv", None)
+        return f
+
+    def coarsen(self):
+        h = C.c_void_p()
+        check(lib().tmrx_forest_coarsen(self.ptr, C.byref(h)), "coarsen")
+        f = OctForest(self.comm, self._order, self._interp, _handle=h)
+        f._Xv = getattr(self, "_Xv", None)
+        return f
+
+    def balance(self, btype):
+        check(lib().tmrx_forest_balance(self.ptr, int(btype)), "balance")
+
+    def createNodes(self):
+        check(lib().tmrx_forest_create_nodes(self.ptr), "createNodes")
+
+    # -- queries --------------------------------------------------------------
+    def getOctants(self):
+        p = C.POINTER(_lib.Octant)()
+        n = C.c_int()
+        check(lib().tmrx_forest_get_octants(self.ptr, C.byref(p), C.byref(n)), "getOctants")
+        return _octants_to_array(p, n.value)
+
+    def getOctsWithName(self, aname):
+        name = aname.encode() if aname is not None else None
+        n = check(lib().tmrx_forest_get_octs_with_name(self.ptr, name, None, 0), "getOctsWithName")
+        buf = (_lib.Octant * max(n, 1))()
+        check(lib().tmrx_forest_get_octs_with_name(self.ptr, name, buf, n), "getOctsWithName")
+        return _octants_to_array(C.cast(buf, C.POINTER(_lib.Octant)), n)
+
+    def getNodesWithName(self, aname):
+        name = aname.encode() if aname is not None else None
+        n = check(lib().tmrx_forest_get_nodes_with_name(self.ptr, name, None, 0), "getNodesWithName")
+        out = np.zeros(max(n, 1), dtype=np.intc)
+        check(lib().tmrx_forest_get_nodes_with_name(self.ptr, name, _iptr(out), n), "getNodesWithName")
+        return out[:n].copy()
+
+    def getNodeRange(self):
+        r = np.zeros(2, dtype=np.intc)
+        check(lib().tmrx_forest_get_owned_node_range(self.ptr, _iptr(r)), "getNodeRange")
+        return r
+
+    def getMeshConn(self):
+        p = C.POINTER(C.c_int)()
+        ne, no = C.c_int(), C.c_int()
+        check(lib().tmrx_forest_get_node_conn(self.ptr, C.byref(p), C.byref(ne), C.byref(no)),
+              "getMeshConn")
+        if ne.value == 0:
+            return np.zeros((0, 8), dtype=np.intc)
+        return np.ctypeslib.as_array(p, (8 * ne.value,)).reshape(ne.value, 8).copy()
+
+    def getDepNodeConn(self):
+        p, c = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        w = C.POINTER(C.c_double)()
+        nd = check(lib().tmrx_forest_get_dep_node_conn(self.ptr, C.byref(p), C.byref(c), C.byref(w)),
+                   "getDepNodeConn")
+        ptr = np.ctypeslib.as_array(p, (nd + 1,)).copy()
+        nnz = int(ptr[-1])
+        if nnz == 0:
+            return ptr, np.zeros(0, np.intc), np.zeros(0)
+        return (ptr, np.ctypeslib.as_array(c, (nnz,)).copy(),
+                np.ctypeslib.as_array(w, (nnz,)).copy())
+
+    def getNodeNumbers(self):
+        p = C.POINTER(C.c_int)()
+        n = C.c_int()
+        check(lib().tmrx_forest_get_node_numbers(self.ptr, C.byref(p), C.byref(n)), "getNodeNumbers")
+        return np.ctypeslib.as_array(p, (n.value,)).copy()
+
+    def getPoints(self):
+        p = C.POINTER(C.c_double)()
+        n = C.c_int()
+        check(lib().tmrx_forest_get_points(self.ptr, C.byref(p), C.byref(n)), "getPoints")
+        return np.ctypeslib.as_array(p, (3 * n.value,)).reshape(n.value, 3).copy()
+
+    def createInterpolation(self, forest, vec=None):
+        """CSR of the prolongation from `forest` (coarse) to this forest.
+
+        Returns (rowp, cols, weights); if `vec` is given (an object with a
+        `setCSR(rowp, cols, weights, ncols)` method) it is filled instead.
+        """
+        if forest.ptr.value == self.ptr.value:
+            raise ValueError("Cannot interpolate between the same object")
+        nr, nc = C.c_int(), C.c_int()
+        rp, cl = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        w = C.POINTER(C.c_double)()
+        check(lib().tmrx_forest_create_interpolation(self.ptr, forest.ptr, C.byref(nr), C.byref(nc),
+                                                     C.byref(rp), C.byref(cl), C.byref(w)),
+              "createInterpolation")
+        rowp = np.ctypeslib.as_array(rp, (nr.value + 1,)).copy()
+        nnz = int(rowp[-1])
+        cols = np.ctypeslib.as_array(cl, (nnz,)).copy() if nnz else np.zeros(0, np.intc)
+        wts = np.ctypeslib.as_array(w, (nnz,)).copy() if nnz else np.zeros(0)
+        if vec is not None:
+            vec.setCSR(rowp, cols, wts, nc.value)
+            return vec
+        return rowp, cols, wts
+
+
+def box_connectivity(nx, ny, nz, L=(1.0, 1.0, 1.0)):
+    """Block connectivity and vertex points of an nx x ny x nz lattice of box
+    blocks covering [0,Lx]x[0,Ly]x[0,Lz]; corner ordering i + 2j + 4k."""
+    vid = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
+    conn = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                conn.append([vid(i + a, j + b, k + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)])
+    X = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
+    for k in range(nz + 1):
+        for j in range(ny + 1):
+            for i in range(nx + 1):
+                X[vid(i, j, k)] = (L[0] * i / nx, L[1] * j / ny, L[2] * k / nz)
+    return np.array(conn, dtype=np.intc), X
+
+
+def createBoxForest(nx, ny, nz, L=(1.0, 1.0, 1.0), face_names=None, comm=None):
+    """Forest on an nx x ny x nz lattice of box blocks with named outer faces.
+
+    face_names maps 'x-', 'x+', 'y-', 'y+', 'z-', 'z+' to names; every block
+    face on that side of the box receives it.  This replaces the CAD topology
+    (TMRTopology) the reference uses to attach names.
+    """
+    conn, X = box_connectivity(nx, ny, nz, L)
+    f = OctForest(comm)
+    f.setConnectivity(conn)
+    f.setNodePoints(X)
+    if face_names:
+        info = f.getConnectivity()
+        bfc = info["block_face_conn"]
+        sides = {"x-": 0, "x+": 1, "y-": 2, "y+": 3, "z-": 4, "z+": 5}
+        dims = (nx, ny, nz)
+        for side, name in face_names.items():
+            fi = sides[side]
+            a, hi = fi // 2, fi % 2
+            for b in range(len(conn)):
+                ijk = (b % nx, (b // nx) % ny, b // (nx * ny))
+                if ijk[a] == (dims[a] - 1 if hi else 0):
+                    f.setName("face", int(bfc[b, fi]), name)
+    return f

@@ -1,0 +1,213 @@
+// C-ABI wrappers for the solver (declarations in include/tmrx.h).
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tmrx.h"
+#include "problem.h"
+
+namespace tmrx {
+void set_error(const std::string &s);
+}
+using tmrx::set_error;
+tmrx::Forest *tmrx_forest_impl(tmrx_forest *f);
+
+struct tmrx_problem {
+  tmrx::Problem *p;
+  int device;
+};
+
+static tmrx::SolveOpts to_opts(const tmrx_solve_opts *o) {
+  tmrx::SolveOpts s;
+  if (!o) return s;
+  s.rtol = o->rtol;
+  s.atol = o->atol;
+  s.max_iters = o->max_iters;
+  s.precision = o->precision;
+  s.cheb_degree = o->cheb_degree;
+  s.cheb_lower = o->cheb_lower;
+  s.cheb_upper = o->cheb_upper;
+  s.eig_iters = o->eig_iters;
+  s.monitor = o->monitor;
+  s.zero_guess = o->zero_guess;
+  return s;
+}
+
+#define CHECK_P(h)                         \
+  do {                                     \
+    if (!(h) || !(h)->p) {                 \
+      set_error("null problem handle");    \
+      return -1;                           \
+    }                                      \
+    (void)hipSetDevice((h)->device);       \
+  } while (0)
+
+static int fail(tmrx_problem *h) {
+  set_error(h->p->last_error);
+  return -1;
+}
+
+extern "C" {
+
+void tmrx_solve_opts_default(tmrx_solve_opts *o) {
+  if (!o) return;
+  tmrx::SolveOpts s;
+  o->rtol = s.rtol;
+  o->atol = s.atol;
+  o->max_iters = s.max_iters;
+  o->precision = s.precision;
+  o->cheb_degree = s.cheb_degree;
+  o->cheb_lower = s.cheb_lower;
+  o->cheb_upper = s.cheb_upper;
+  o->eig_iters = s.eig_iters;
+  o->monitor = s.monitor;
+  o->zero_guess = s.zero_guess;
+}
+
+int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat, int nbcs,
+                        const char **bc_names, const int *bc_dof_masks, int device,
+                        tmrx_problem **out) {
+  if (!out || !forests || nlevels < 1) {
+    set_error("tmrx_problem_create: invalid arguments");
+    return -1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    set_error("tmrx_problem_create: no HIP device available");
+    return -1;
+  }
+  if (device < 0 || device >= ndev) {
+    set_error("tmrx_problem_create: device index out of range");
+    return -1;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    set_error("tmrx_problem_create: hipSetDevice failed");
+    return -1;
+  }
+  std::vector<tmrx::Forest *> fs(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    fs[l] = tmrx_forest_impl(forests[l]);
+    if (!fs[l]) {
+      set_error("tmrx_problem_create: null forest");
+      return -1;
+    }
+  }
+  tmrx::Material m;
+  if (mat) {
+    m.E = mat->E;
+    m.nu = mat->nu;
+    m.ys = mat->ys;
+    m.q = mat->q;
+    m.k0 = mat->k0;
+    m.eps = mat->eps;
+    m.penalty_type = mat->penalty_type;
+  }
+  if (m.penalty_type == 0 && m.q < 1.0) m.q = 1.0;  // TMRStiffnessProperties clamp
+  auto *p = new tmrx::Problem();
+  if (p->init(nlevels, fs.data(), m, nbcs, bc_names, bc_dof_masks)) {
+    set_error(p->last_error);
+    delete p;
+    return -1;
+  }
+  *out = new tmrx_problem{p, device};
+  return 0;
+}
+
+void tmrx_problem_free(tmrx_problem *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  delete h->p;
+  delete h;
+}
+
+int tmrx_problem_get_sizes(tmrx_problem *h, int *ndof, int *nelem, int *nlevels) {
+  CHECK_P(h);
+  if (ndof) *ndof = h->p->num_dof();
+  if (nelem) *nelem = h->p->num_elements();
+  if (nlevels) *nlevels = h->p->num_levels();
+  return 0;
+}
+
+int tmrx_problem_set_stream(tmrx_problem *h, void *stream) {
+  CHECK_P(h);
+  h->p->set_stream((hipStream_t)stream);
+  return 0;
+}
+
+int tmrx_problem_set_design(tmrx_problem *h, const double *rho, int n, int on_device) {
+  CHECK_P(h);
+  if (!rho) {
+    set_error("tmrx_problem_set_design: null densities");
+    return -1;
+  }
+  return h->p->set_design(rho, n, on_device != 0) ? fail(h) : 0;
+}
+
+int tmrx_problem_setup(tmrx_problem *h, const tmrx_solve_opts *opts) {
+  CHECK_P(h);
+  return h->p->setup(to_opts(opts)) ? fail(h) : 0;
+}
+
+int tmrx_problem_solve(tmrx_problem *h, const double *f, double *u, int on_device,
+                       const tmrx_solve_opts *opts, tmrx_solve_info *info) {
+  CHECK_P(h);
+  tmrx::SolveInfo si;
+  int r = h->p->solve(f, u, on_device != 0, to_opts(opts), &si);
+  if (info) {
+    info->iters = si.iters;
+    info->converged = si.converged;
+    info->rnorm0 = si.rnorm0;
+    info->rnorm = si.rnorm;
+    info->setup_ms = si.setup_ms;
+    info->solve_ms = si.solve_ms;
+  }
+  if (r < 0) return fail(h);
+  if (r > 0) set_error(h->p->last_error);
+  return r;
+}
+
+int tmrx_problem_true_residual(tmrx_problem *h, double *rel) {
+  CHECK_P(h);
+  if (rel) *rel = h->p->last_true_residual;
+  return 0;
+}
+
+int tmrx_problem_compliance(tmrx_problem *h, double *c) {
+  CHECK_P(h);
+  return h->p->compliance(c) ? fail(h) : 0;
+}
+
+int tmrx_problem_compliance_grad(tmrx_problem *h, double *g, int on_device) {
+  CHECK_P(h);
+  return h->p->compliance_grad(g, on_device != 0) ? fail(h) : 0;
+}
+
+int tmrx_problem_apply(tmrx_problem *h, const double *x, double *y, int on_device,
+                       int precision) {
+  CHECK_P(h);
+  return h->p->apply(x, y, on_device != 0, precision) ? fail(h) : 0;
+}
+
+int tmrx_problem_traction_load(tmrx_problem *h, const char *name, const double tr[3],
+                               double *f) {
+  CHECK_P(h);
+  if (!name || !tr || !f) {
+    set_error("tmrx_problem_traction_load: invalid arguments");
+    return -1;
+  }
+  return h->p->traction_load(name, tr, f) ? fail(h) : 0;
+}
+
+int tmrx_problem_kernel_timing(tmrx_problem *h, int enable) {
+  CHECK_P(h);
+  return h->p->kernel_timing(enable);
+}
+
+int tmrx_problem_kernel_timing_result(tmrx_problem *h, double *total_ms, int *count,
+                                      int *precision_mask) {
+  CHECK_P(h);
+  return h->p->kernel_timing_result(total_ms, count, precision_mask) ? fail(h) : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// HIP kernels of the hex8 solve path (gfx950) and their host launchers.
+//
+// Data layout in HBM (one multigrid level):
+//   * vectors: node-major, 3 dofs interleaved (u_x, u_y, u_z) per independent
+//     node, nodes in forest order (Morton order of the node positions);
+//   * elements are processed in batches of up to 256 consecutive (Morton
+//     order) elements of one stiffness class; a batch stages the distinct
+//     nodes it touches ("slots") in LDS.  Slot ids are global:
+//     wg_node_ptr[b] .. wg_node_ptr[b+1] belong to batch b;
+//   * element -> slot connectivity is 8 x uint16 local indices per element;
+//   * the element pass writes one partial sum per slot; the assembly pass
+//     gathers, for every independent node, its slots (weight 1) and the slots
+//     of the hanging nodes that depend on it (hanging-node weight), so there
+//     are no atomics and the summation order is fixed.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace tmrx {
+
+constexpr int WG_ELEMS = 256;
+constexpr int MAX_COLORS = 32;
+
+struct ElemBatches {
+  int n_wg = 0;
+  int n_elem = 0;
+  int ncolors = 0;
+  int max_wg_nodes = 0;
+  const int *wg_node_ptr = nullptr;  // [n_wg + 1]
+  const int *wg_nodes = nullptr;     // [nslots]: >= 0 node, < 0 -(hanging + 1)
+  const int *wg_elem_ptr = nullptr;  // [n_wg + 1]
+  const int *wg_class = nullptr;     // [n_wg]
+  const uint16_t *lconn = nullptr;   // [n_elem * 8]
+  const uint8_t *color = nullptr;    // [n_elem]
+  const int *dep_ptr = nullptr;      // hanging-node constraints
+  const int *dep_conn = nullptr;
+  const float *dep_w = nullptr;
+  const uint8_t *bcmask = nullptr;   // [n_nodes], bit d set = dof d clamped
+};
+
+struct AsmCSR {
+  int n_nodes = 0;
+  const int *ptr = nullptr;   // [n_nodes + 1]
+  const int *slot = nullptr;  // [nnz]
+  const float *w = nullptr;   // [nnz]
+};
+
+struct TransferCSR {  // rows = output nodes
+  int n_rows = 0;
+  const int *ptr = nullptr;
+  const int *col = nullptr;
+  const float *w = nullptr;
+};
+
+// y-side operation performed by the assembly pass
+enum AsmMode {
+  ASM_STORE = 0,   // y = A x
+  ASM_RESID = 1,   // y = b - A x
+  ASM_CHEB = 2,    // r = b - A x; d = a d + c Dinv r; x += d
+  ASM_DIAG = 3,    // Dinv = 1 / diag(A)   (partials hold element diagonals)
+  ASM_DOT = 4,     // y = A x and block partials of x . y
+};
+
+struct AsmArgs {
+  const void *b = nullptr;
+  void *y = nullptr;
+  void *x = nullptr;       // input of the product (read) or iterate (CHEB)
+  void *d = nullptr;       // Chebyshev direction
+  const void *dinv = nullptr;
+  double ca = 0.0, cc = 0.0;
+  double *partials = nullptr;
+};
+
+template <class T>
+void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, const T *x,
+                       T *partial, hipStream_t st);
+template <class T>
+void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *partial,
+                      hipStream_t st);
+template <class T>
+void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, int mode,
+                     const AsmArgs &args, hipStream_t st);
+int assemble_num_blocks(int n_nodes);
+
+template <class T>
+void launch_prolong_add(const TransferCSR &P, const uint8_t *bcmask_f, const T *xc, T *xf,
+                        hipStream_t st);
+template <class T>
+void launch_restrict(const TransferCSR &R, const uint8_t *bcmask_c, const T *rf, T *rc,
+                     hipStream_t st);
+template <class T>
+void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
+                       hipStream_t st);
+template <class T>
+void launch_dense_solve(int n, const double *Ainv, const T *b, T *x, hipStream_t st);
+
+template <class TI, class TO>
+void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);
+template <class T>
+void launch_fill(int64_t n, T v, T *a, hipStream_t st);
+
+// element stiffness scales for a level: s = penalty(rho) * 2^-level
+void launch_simp_scale(int n, const double *rho, const int16_t *level, double q, double k0,
+                       int penalty_type, double *s64, float *s32, hipStream_t st);
+// coarse element scales: volume-weighted mean of the material factor of the
+// fine elements inside each coarse element (CSR over coarse elements)
+void launch_restrict_scale(int nc, const int *ptr, const int *child, const float *vw,
+                           const double *s_fine, const int16_t *lev_fine,
+                           const int16_t *lev_coarse, double *s64, float *s32, hipStream_t st);
+
+// FP64 PCG helpers (scalars live on the device)
+enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_N };
+int reduce_blocks(int64_t n);
+void launch_pcg_update_xr(int64_t ndof, double *x, double *r, const double *p, const double *q,
+                          const double *sc, double *partials, float *r32, hipStream_t st);
+void launch_dot2(int64_t ndof, const double *z, const double *r, const double *q,
+                 double *partials, hipStream_t st);
+void launch_dot1(int64_t ndof, const double *a, const double *b, double *partials,
+                 hipStream_t st);
+void launch_pcg_update_p(int64_t ndof, double *p, const double *z, const double *sc,
+                         hipStream_t st);
+// finalize: mode 0 rz, 1 pq->alpha, 2 rr, 3 (zr, zq)->beta, 4 bb
+void launch_finalize(int nparts, const double *partials, int mode, double *sc, hipStream_t st);
+
+// per-element compliance density and its design derivative (element mode)
+void launch_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                        const float *dep_w, const uint8_t *bcmask, const int *cls,
+                        const double *K0, const double *u, double *energy, hipStream_t st);
+// SIMP/RAMP derivative (src/topology/TMROctConstitutive.cpp:597-619) times
+// -2^-level times the element energy
+void launch_compliance_grad(int n, const double *rho, const int16_t *lev, double q, int ptype,
+                            const double *energy, double *g, hipStream_t st);
+
+}  // namespace tmrx

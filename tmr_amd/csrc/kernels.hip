@@ -1,0 +1,657 @@
+// HIP kernels for gfx950 (wave64).  See kernels.h for the data layout.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace tmrx {
+
+#define TMRX_LAUNCH_CHECK() (void)hipGetLastError()
+
+// ---------------------------------------------------------------------------
+// element pass: stage batch nodes in LDS, y_e = s_e K0 x_e, colored LDS
+// accumulation, one partial per slot.
+template <class T>
+__device__ __forceinline__ void load_node(int e, const T *__restrict__ x,
+                                          const int *__restrict__ dep_ptr,
+                                          const int *__restrict__ dep_conn,
+                                          const float *__restrict__ dep_w,
+                                          const uint8_t *__restrict__ bcmask, T v[3]) {
+  if (e >= 0) {
+    const uint8_t m = bcmask[e];
+    v[0] = (m & 1) ? T(0) : x[3 * e];
+    v[1] = (m & 2) ? T(0) : x[3 * e + 1];
+    v[2] = (m & 4) ? T(0) : x[3 * e + 2];
+  } else {
+    const int d = -e - 1;
+    v[0] = v[1] = v[2] = T(0);
+    for (int j = dep_ptr[d]; j < dep_ptr[d + 1]; j++) {
+      const int p = dep_conn[j];
+      const T w = (T)dep_w[j];
+      const uint8_t m = bcmask[p];
+      if (!(m & 1)) v[0] += w * x[3 * p];
+      if (!(m & 2)) v[1] += w * x[3 * p + 1];
+      if (!(m & 4)) v[2] += w * x[3 * p + 2];
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(WG_ELEMS) void k_elem_apply(
+    const int *__restrict__ wg_node_ptr, const int *__restrict__ wg_nodes,
+    const int *__restrict__ wg_elem_ptr, const int *__restrict__ wg_class,
+    const uint4 *__restrict__ lconn, const uint8_t *__restrict__ color, int ncolors,
+    const int *__restrict__ dep_ptr, const int *__restrict__ dep_conn,
+    const float *__restrict__ dep_w, const uint8_t *__restrict__ bcmask,
+    const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
+    T *__restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wg = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nb = wg_node_ptr[wg];
+  const int nn = wg_node_ptr[wg + 1] - nb;
+  T *xs = reinterpret_cast<T *>(smem_raw);
+  T *ys = xs + 3 * nn;
+  for (int i = tid; i < nn; i += WG_ELEMS) {
+    T v[3];
+    load_node<T>(wg_nodes[nb + i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
+    xs[3 * i] = v[0];
+    xs[3 * i + 1] = v[1];
+    xs[3 * i + 2] = v[2];
+    ys[3 * i] = ys[3 * i + 1] = ys[3 * i + 2] = T(0);
+  }
+  __syncthreads();
+  const int e0 = wg_elem_ptr[wg];
+  const int ne = wg_elem_ptr[wg + 1] - e0;
+  const int cls = __builtin_amdgcn_readfirstlane(wg_class[wg]);
+  const T *__restrict__ K = K0 + 576 * cls;
+  T ye[24];
+  int lc[8];
+  int col = -1;
+  if (tid < ne) {
+    const uint4 q = lconn[e0 + tid];
+    lc[0] = q.x & 0xffff; lc[1] = q.x >> 16;
+    lc[2] = q.y & 0xffff; lc[3] = q.y >> 16;
+    lc[4] = q.z & 0xffff; lc[5] = q.z >> 16;
+    lc[6] = q.w & 0xffff; lc[7] = q.w >> 16;
+    col = color[e0 + tid];
+    const T s = s_elem[e0 + tid];
+    T xe[24];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      xe[3 * a] = xs[3 * lc[a]];
+      xe[3 * a + 1] = xs[3 * lc[a] + 1];
+      xe[3 * a + 2] = xs[3 * lc[a] + 2];
+    }
+#pragma unroll
+    for (int i = 0; i < 24; i++) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < 24; j++) acc += K[24 * i + j] * xe[j];
+      ye[i] = s * acc;
+    }
+  }
+  for (int c = 0; c < ncolors; c++) {
+    if (col == c) {
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        ys[3 * lc[a]] += ye[3 * a];
+        ys[3 * lc[a] + 1] += ye[3 * a + 1];
+        ys[3 * lc[a] + 2] += ye[3 * a + 2];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 3 * nn; i += WG_ELEMS) partial[3 * nb + i] = ys[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(WG_ELEMS) void k_elem_diag(
+    const int *__restrict__ wg_node_ptr, const int *__restrict__ wg_elem_ptr,
+    const int *__restrict__ wg_class, const uint4 *__restrict__ lconn,
+    const uint8_t *__restrict__ color, int ncolors, const T *__restrict__ K0,
+    const T *__restrict__ s_elem, T *__restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wg = blockIdx.x, tid = threadIdx.x;
+  const int nb = wg_node_ptr[wg], nn = wg_node_ptr[wg + 1] - nb;
+  T *ys = reinterpret_cast<T *>(smem_raw);
+  for (int i = tid; i < 3 * nn; i += WG_ELEMS) ys[i] = T(0);
+  __syncthreads();
+  const int e0 = wg_elem_ptr[wg], ne = wg_elem_ptr[wg + 1] - e0;
+  const int cls = __builtin_amdgcn_readfirstlane(wg_class[wg]);
+  const T *__restrict__ K = K0 + 576 * cls;
+  int lc[8];
+  int col = -1;
+  T s = T(0);
+  if (tid < ne) {
+    const uint4 q = lconn[e0 + tid];
+    lc[0] = q.x & 0xffff; lc[1] = q.x >> 16;
+    lc[2] = q.y & 0xffff; lc[3] = q.y >> 16;
+    lc[4] = q.z & 0xffff; lc[5] = q.z >> 16;
+    lc[6] = q.w & 0xffff; lc[7] = q.w >> 16;
+    col = color[e0 + tid];
+    s = s_elem[e0 + tid];
+  }
+  for (int c = 0; c < ncolors; c++) {
+    if (col == c) {
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) ys[3 * lc[a] + d] += s * K[(3 * a + d) * 25];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 3 * nn; i += WG_ELEMS) partial[3 * nb + i] = ys[i];
+}
+
+template <class T>
+void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, const T *x,
+                       T *partial, hipStream_t st) {
+  if (eb.n_wg == 0) return;
+  size_t sh = 2 * 3 * (size_t)eb.max_wg_nodes * sizeof(T);
+  hipLaunchKernelGGL(k_elem_apply<T>, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
+                     eb.wg_nodes, eb.wg_elem_ptr, eb.wg_class,
+                     reinterpret_cast<const uint4 *>(eb.lconn), eb.color, eb.ncolors,
+                     eb.dep_ptr, eb.dep_conn, eb.dep_w, eb.bcmask, K0, s_elem, x, partial);
+  TMRX_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *partial,
+                      hipStream_t st) {
+  if (eb.n_wg == 0) return;
+  size_t sh = 3 * (size_t)eb.max_wg_nodes * sizeof(T);
+  hipLaunchKernelGGL(k_elem_diag<T>, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
+                     eb.wg_elem_ptr, eb.wg_class, reinterpret_cast<const uint4 *>(eb.lconn),
+                     eb.color, eb.ncolors, K0, s_elem, partial);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// assembly pass: node-parallel weighted gather of slot partials + epilogue
+constexpr int NODE_BLOCK = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w];
+  }
+  return s;
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(NODE_BLOCK) void k_assemble(
+    int n, const int *__restrict__ ptr, const int *__restrict__ slot, const float *__restrict__ w,
+    const uint8_t *__restrict__ bcmask, const T *__restrict__ partial, const T *__restrict__ b,
+    T *__restrict__ y, T *__restrict__ x, T *__restrict__ d, const T *__restrict__ dinv,
+    double ca, double cc, double *__restrict__ partials) {
+  __shared__ double red[NODE_BLOCK / 64];
+  const int i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  double dot = 0.0;
+  if (i < n) {
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+    for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+      const int s = slot[j];
+      T ww = (T)w[j];
+      if (MODE == ASM_DIAG) ww = ww * ww;
+      a0 += ww * partial[3 * s];
+      a1 += ww * partial[3 * s + 1];
+      a2 += ww * partial[3 * s + 2];
+    }
+    const uint8_t m = bcmask[i];
+    const T acc[3] = {a0, a1, a2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int k = 3 * i + c;
+      const bool bc = (m >> c) & 1;
+      if (MODE == ASM_STORE) {
+        y[k] = bc ? x[k] : acc[c];
+      } else if (MODE == ASM_DOT) {
+        const T v = bc ? x[k] : acc[c];
+        y[k] = v;
+        dot += (double)x[k] * (double)v;
+      } else if (MODE == ASM_RESID) {
+        y[k] = bc ? T(0) : b[k] - acc[c];
+      } else if (MODE == ASM_CHEB) {
+        if (bc) {
+          d[k] = T(0);
+          x[k] = T(0);
+        } else {
+          const T r = b[k] - acc[c];
+          const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
+          d[k] = dn;
+          x[k] += dn;
+        }
+      } else if (MODE == ASM_DIAG) {
+        y[k] = bc ? T(1) : T(1) / acc[c];
+      }
+    }
+  }
+  if (MODE == ASM_DOT) {
+    const double s = block_sum(dot, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  }
+}
+
+int assemble_num_blocks(int n_nodes) { return (n_nodes + NODE_BLOCK - 1) / NODE_BLOCK; }
+
+template <class T>
+void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, int mode,
+                     const AsmArgs &g, hipStream_t st) {
+  const int nb = assemble_num_blocks(a.n_nodes);
+  if (nb == 0) return;
+  const T *b = (const T *)g.b;
+  T *y = (T *)g.y, *x = (T *)g.x, *d = (T *)g.d;
+  const T *dinv = (const T *)g.dinv;
+#define TMRX_ASM(M)                                                                           \
+  hipLaunchKernelGGL((k_assemble<T, M>), dim3(nb), dim3(NODE_BLOCK), 0, st, a.n_nodes, a.ptr, \
+                     a.slot, a.w, bcmask, partial, b, y, x, d, dinv, g.ca, g.cc, g.partials)
+  switch (mode) {
+    case ASM_STORE: TMRX_ASM(ASM_STORE); break;
+    case ASM_RESID: TMRX_ASM(ASM_RESID); break;
+    case ASM_CHEB: TMRX_ASM(ASM_CHEB); break;
+    case ASM_DIAG: TMRX_ASM(ASM_DIAG); break;
+    case ASM_DOT: TMRX_ASM(ASM_DOT); break;
+  }
+#undef TMRX_ASM
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// grid transfer: prolongation (add into fine), restriction (coarse = P^T r)
+template <class T>
+__global__ void k_prolong_add(int n, const int *__restrict__ ptr, const int *__restrict__ col,
+                              const float *__restrict__ w, const uint8_t *__restrict__ mask,
+                              const T *__restrict__ xc, T *__restrict__ xf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a0 = T(0), a1 = T(0), a2 = T(0);
+  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+    const int c = col[j];
+    const T ww = (T)w[j];
+    a0 += ww * xc[3 * c];
+    a1 += ww * xc[3 * c + 1];
+    a2 += ww * xc[3 * c + 2];
+  }
+  const uint8_t m = mask[i];
+  if (!(m & 1)) xf[3 * i] += a0;
+  if (!(m & 2)) xf[3 * i + 1] += a1;
+  if (!(m & 4)) xf[3 * i + 2] += a2;
+}
+
+template <class T>
+__global__ void k_restrict(int n, const int *__restrict__ ptr, const int *__restrict__ col,
+                           const float *__restrict__ w, const uint8_t *__restrict__ mask,
+                           const T *__restrict__ rf, T *__restrict__ rc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a0 = T(0), a1 = T(0), a2 = T(0);
+  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+    const int c = col[j];
+    const T ww = (T)w[j];
+    a0 += ww * rf[3 * c];
+    a1 += ww * rf[3 * c + 1];
+    a2 += ww * rf[3 * c + 2];
+  }
+  const uint8_t m = mask[i];
+  rc[3 * i] = (m & 1) ? T(0) : a0;
+  rc[3 * i + 1] = (m & 2) ? T(0) : a1;
+  rc[3 * i + 2] = (m & 4) ? T(0) : a2;
+}
+
+template <class T>
+void launch_prolong_add(const TransferCSR &P, const uint8_t *mask_f, const T *xc, T *xf,
+                        hipStream_t st) {
+  if (P.n_rows == 0) return;
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3((P.n_rows + 255) / 256), dim3(256), 0, st, P.n_rows,
+                     P.ptr, P.col, P.w, mask_f, xc, xf);
+  TMRX_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T *rc,
+                     hipStream_t st) {
+  if (R.n_rows == 0) return;
+  hipLaunchKernelGGL(k_restrict<T>, dim3((R.n_rows + 255) / 256), dim3(256), 0, st, R.n_rows,
+                     R.ptr, R.col, R.w, mask_c, rf, rc);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev first step from a zero guess: d = c Dinv b, x = d
+template <class T>
+__global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict__ dinv, double c,
+                             T *__restrict__ d, T *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T v = (T)c * dinv[i] * b[i];
+  d[i] = v;
+  x[i] = v;
+}
+
+template <class T>
+void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_cheb_first<T>, dim3((ndof + 255) / 256), dim3(256), 0, st, ndof, b, dinv,
+                     c, d, x);
+  TMRX_LAUNCH_CHECK();
+}
+
+// coarsest level: x = Ainv b (dense FP64, symmetric; one wave per row)
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__restrict__ Ainv,
+                                                     const T *__restrict__ b, T *__restrict__ x) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  const double *a = Ainv + (size_t)row * n;
+  for (int j = lane; j < n; j += 64) s += a[j] * (double)b[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[row] = (T)s;
+}
+
+template <class T>
+void launch_dense_solve(int n, const double *Ainv, const T *b, T *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, Ainv, b, x);
+  TMRX_LAUNCH_CHECK();
+}
+
+template <class TI, class TO>
+__global__ void k_convert(int64_t n, const TI *__restrict__ a, TO *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (TO)a[i];
+}
+template <class TI, class TO>
+void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert<TI, TO>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     a, b);
+  TMRX_LAUNCH_CHECK();
+}
+
+template <class T>
+__global__ void k_fill(int64_t n, T v, T *__restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+template <class T>
+void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, v, a);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// SIMP / RAMP penalty (src/topology/TMROctConstitutive.cpp:498-551) times
+// the element size factor 2^-level.
+__global__ void k_simp_scale(int n, const double *__restrict__ rho, const int16_t *__restrict__ lev,
+                             double q, double k0, int ptype, double *__restrict__ s64,
+                             float *__restrict__ s32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  const double pen = (ptype == 0) ? pow(r, q) + k0 : r / (1.0 + q * (1.0 - r)) + k0;
+  const double s = ldexp(pen, -(int)lev[i]);
+  s64[i] = s;
+  s32[i] = (float)s;
+}
+
+void launch_simp_scale(int n, const double *rho, const int16_t *level, double q, double k0,
+                       int ptype, double *s64, float *s32, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_simp_scale, dim3((n + 255) / 256), dim3(256), 0, st, n, rho, level, q, k0,
+                     ptype, s64, s32);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_restrict_scale(int nc, const int *__restrict__ ptr, const int *__restrict__ child,
+                                 const float *__restrict__ vw, const double *__restrict__ sf,
+                                 const int16_t *__restrict__ lf, const int16_t *__restrict__ lc,
+                                 double *__restrict__ s64, float *__restrict__ s32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  double m = 0.0;
+  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+    const int c = child[j];
+    m += (double)vw[j] * ldexp(sf[c], (int)lf[c]);  // material factor of the child
+  }
+  const double s = ldexp(m, -(int)lc[i]);
+  s64[i] = s;
+  s32[i] = (float)s;
+}
+
+void launch_restrict_scale(int nc, const int *ptr, const int *child, const float *vw,
+                           const double *s_fine, const int16_t *lev_fine, const int16_t *lev_coarse,
+                           double *s64, float *s32, hipStream_t st) {
+  if (nc == 0) return;
+  hipLaunchKernelGGL(k_restrict_scale, dim3((nc + 255) / 256), dim3(256), 0, st, nc, ptr, child, vw,
+                     s_fine, lev_fine, lev_coarse, s64, s32);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 PCG vector kernels with block partial reductions (fixed grid, so the
+// reduction order, and the result, is the same on every run)
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAX_BLOCKS = 2048;
+
+int reduce_blocks(int64_t n) {
+  int64_t b = (n + RED_BLOCK - 1) / RED_BLOCK;
+  return (int)(b < RED_MAX_BLOCKS ? (b < 1 ? 1 : b) : RED_MAX_BLOCKS);
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_pcg_update_xr(int64_t n, double *__restrict__ x,
+                                                             double *__restrict__ r,
+                                                             const double *__restrict__ p,
+                                                             const double *__restrict__ q,
+                                                             const double *__restrict__ sc,
+                                                             double *__restrict__ partials,
+                                                             float *__restrict__ r32) {
+  __shared__ double red[RED_BLOCK / 64];
+  const double alpha = sc[SC_ALPHA];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RED_BLOCK) {
+    x[i] += alpha * p[i];
+    const double rn = r[i] - alpha * q[i];
+    r[i] = rn;
+    if (r32) r32[i] = (float)rn;
+    acc += rn * rn;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void launch_pcg_update_xr(int64_t ndof, double *x, double *r, const double *p, const double *q,
+                          const double *sc, double *partials, float *r32, hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_update_xr, dim3(reduce_blocks(ndof)), dim3(RED_BLOCK), 0, st, ndof, x, r,
+                     p, q, sc, partials, r32);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_dot2(int64_t n, const double *__restrict__ z,
+                                                    const double *__restrict__ r,
+                                                    const double *__restrict__ q,
+                                                    double *__restrict__ partials) {
+  __shared__ double red[RED_BLOCK / 64];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RED_BLOCK) {
+    a += z[i] * r[i];
+    b += z[i] * q[i];
+  }
+  const double sa = block_sum(a, red);
+  __syncthreads();
+  const double sb = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sa;
+    partials[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+void launch_dot2(int64_t ndof, const double *z, const double *r, const double *q,
+                 double *partials, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot2, dim3(reduce_blocks(ndof)), dim3(RED_BLOCK), 0, st, ndof, z, r, q,
+                     partials);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(RED_BLOCK) void k_dot1(int64_t n, const double *__restrict__ a,
+                                                    const double *__restrict__ b,
+                                                    double *__restrict__ partials) {
+  __shared__ double red[RED_BLOCK / 64];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RED_BLOCK)
+    s += a[i] * b[i];
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+void launch_dot1(int64_t ndof, const double *a, const double *b, double *partials,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(k_dot1, dim3(reduce_blocks(ndof)), dim3(RED_BLOCK), 0, st, ndof, a, b,
+                     partials);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_pcg_update_p(int64_t n, double *__restrict__ p, const double *__restrict__ z,
+                               const double *__restrict__ sc) {
+  const double beta = sc[SC_BETA];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+void launch_pcg_update_p(int64_t ndof, double *p, const double *z, const double *sc,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_update_p, dim3(reduce_blocks(ndof)), dim3(RED_BLOCK), 0, st, ndof, p, z,
+                     sc);
+  TMRX_LAUNCH_CHECK();
+}
+
+// one block sums the partials (fixed order) and updates the PCG scalars
+__global__ __launch_bounds__(256) void k_finalize(int np, const double *__restrict__ part,
+                                                  int mode, double *__restrict__ sc) {
+  __shared__ double red[4];
+  const int stride = (mode == 3) ? 2 : 1;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    a += part[stride * i];
+    if (mode == 3) b += part[2 * i + 1];
+  }
+  const double sa = block_sum(a, red);
+  __syncthreads();
+  const double sb = block_sum(b, red);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case 0: sc[SC_RZ] = sa; break;
+    case 1:
+      sc[SC_PQ] = sa;
+      sc[SC_ALPHA] = sc[SC_RZ] / sa;
+      break;
+    case 2: sc[SC_RR] = sa; break;
+    case 3:  // flexible (Polak-Ribiere) beta = z_{k+1}.(r_{k+1}-r_k) / z_k.r_k
+      sc[SC_RZNEW] = sa;
+      sc[SC_ZQ] = sb;
+      sc[SC_BETA] = -sc[SC_ALPHA] * sb / sc[SC_RZ];
+      sc[SC_RZ] = sa;
+      break;
+    case 4: sc[SC_BB] = sa; break;
+  }
+}
+
+void launch_finalize(int nparts, const double *partials, int mode, double *sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nparts, partials, mode, sc);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// element strain energy u_e^T K0 u_e (unscaled) for the adjoint / compliance
+// sensitivities (src/topology/TMROctConstitutive.cpp:557-654 restricted to
+// element-constant density).
+__global__ void k_elem_energy(int ne, const int *__restrict__ conn, const int *__restrict__ dep_ptr,
+                              const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+                              const uint8_t *__restrict__ bcmask, const int *__restrict__ cls,
+                              const double *__restrict__ K0, const double *__restrict__ u,
+                              double *__restrict__ energy) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double ue[24];
+  for (int a = 0; a < 8; a++) {
+    double v[3];
+    load_node<double>(conn[8 * e + a], u, dep_ptr, dep_conn, dep_w, bcmask, v);
+    ue[3 * a] = v[0];
+    ue[3 * a + 1] = v[1];
+    ue[3 * a + 2] = v[2];
+  }
+  const double *K = K0 + 576 * cls[e];
+  double s = 0.0;
+  for (int i = 0; i < 24; i++) {
+    double t = 0.0;
+    for (int j = 0; j < 24; j++) t += K[24 * i + j] * ue[j];
+    s += ue[i] * t;
+  }
+  energy[e] = s;
+}
+
+void launch_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                        const float *dep_w, const uint8_t *bcmask, const int *cls,
+                        const double *K0, const double *u, double *energy, hipStream_t st) {
+  if (nelem == 0) return;
+  hipLaunchKernelGGL(k_elem_energy, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
+                     dep_ptr, dep_conn, dep_w, bcmask, cls, K0, u, energy);
+  TMRX_LAUNCH_CHECK();
+}
+
+// d compliance / d rho_e = -dpenalty(rho_e) 2^-level u_e^T K0 u_e
+__global__ void k_compliance_grad(int n, const double *__restrict__ rho, const int16_t *__restrict__ lev,
+                                  double q, int ptype, const double *__restrict__ energy,
+                                  double *__restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  double dp;
+  if (ptype == 0) {
+    dp = (q > 1.0) ? q * pow(r, q - 1.0) : 1.0;
+  } else {
+    const double t = 1.0 + q * (1.0 - r);
+    dp = (q + 1.0) / (t * t);
+  }
+  g[i] = -ldexp(dp, -(int)lev[i]) * energy[i];
+}
+
+void launch_compliance_grad(int n, const double *rho, const int16_t *lev, double q, int ptype,
+                            const double *energy, double *g, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_compliance_grad, dim3((n + 255) / 256), dim3(256), 0, st, n, rho, lev, q,
+                     ptype, energy, g);
+  TMRX_LAUNCH_CHECK();
+}
+
+// explicit instantiations
+#define TMRX_INST(T)                                                                            \
+  template void launch_elem_apply<T>(const ElemBatches &, const T *, const T *, const T *, T *, \
+                                     hipStream_t);                                              \
+  template void launch_elem_diag<T>(const ElemBatches &, const T *, const T *, T *,             \
+                                    hipStream_t);                                               \
+  template void launch_assemble<T>(const AsmCSR &, const uint8_t *, const T *, int,             \
+                                   const AsmArgs &, hipStream_t);                               \
+  template void launch_prolong_add<T>(const TransferCSR &, const uint8_t *, const T *, T *,     \
+                                      hipStream_t);                                             \
+  template void launch_restrict<T>(const TransferCSR &, const uint8_t *, const T *, T *,        \
+                                   hipStream_t);                                                \
+  template void launch_cheb_first<T>(int, const T *, const T *, double, T *, T *,               \
+                                     hipStream_t);                                              \
+  template void launch_dense_solve<T>(int, const double *, const T *, T *, hipStream_t);        \
+  template void launch_fill<T>(int64_t, T, T *, hipStream_t);
+TMRX_INST(float)
+TMRX_INST(double)
+template void launch_convert<float, double>(int64_t, const float *, double *, hipStream_t);
+template void launch_convert<double, float>(int64_t, const double *, float *, hipStream_t);
+
+}  // namespace tmrx

@@ -1,0 +1,849 @@
+// Multigrid-preconditioned FP64 flexible PCG on one GPU.  See problem.h.
+#include "problem.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "fe.h"
+
+namespace tmrx {
+
+#define HIPCHK(x)                                                       \
+  do {                                                                  \
+    hipError_t e_ = (x);                                                \
+    if (e_ != hipSuccess) {                                             \
+      last_error = std::string("HIP error: ") + hipGetErrorString(e_) + \
+                   " at " + #x;                                         \
+      return -1;                                                        \
+    }                                                                   \
+  } while (0)
+
+Problem::~Problem() {
+  for (auto &e : ev_pool_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+int Problem::kernel_timing(int enable) {
+  timing_ = enable != 0;
+  ev_used_ = 0;
+  ev_prec_mask_ = 0;
+  return 0;
+}
+
+int Problem::kernel_timing_result(double *total_ms, int *count, int *mask) {
+  HIPCHK(hipStreamSynchronize(st_));
+  double t = 0.0;
+  for (size_t i = 0; i < ev_used_; i++) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, ev_pool_[i].first, ev_pool_[i].second));
+    t += ms;
+  }
+  if (total_ms) *total_ms = t;
+  if (count) *count = (int)ev_used_;
+  if (mask) *mask = ev_prec_mask_;
+  return 0;
+}
+
+template <>
+LevelVecs<float> &Problem::vecs<float>(int l) { return levels_[l].v32; }
+template <>
+LevelVecs<double> &Problem::vecs<double>(int l) { return levels_[l].v64; }
+template <>
+const float *Problem::K0<float>() { return K0f_.p; }
+template <>
+const double *Problem::K0<double>() { return K0d_.p; }
+template <>
+const float *Problem::scale<float>(int l) { return levels_[l].s32.p; }
+template <>
+const double *Problem::scale<double>(int l) { return levels_[l].s64.p; }
+
+// ---------------------------------------------------------------------------
+int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
+                  const char **names, const int *dof_masks) {
+  if (nlevels < 1) {
+    last_error = "problem: need at least one level";
+    return -1;
+  }
+  mat_ = mat;
+  levels_.clear();
+  levels_.resize(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    Forest *f = forests[l];
+    if (!f) {
+      last_error = "problem: null forest";
+      return -1;
+    }
+    if (!f->blocks_are_boxes()) {
+      last_error = "problem: every block must be an axis-aligned box with points set";
+      return -1;
+    }
+    if (f->create_nodes()) {
+      last_error = "problem: level " + std::to_string(l) + ": " + f->last_error;
+      return -1;
+    }
+    levels_[l].forest = f;
+  }
+  // stiffness classes: blocks with the same box extents share K0
+  Forest *f0 = forests[0];
+  const BlockConn &bd = f0->conn_data();
+  std::vector<std::array<double, 3>> ext;
+  std::vector<int> block_cls(bd.num_blocks);
+  for (int b = 0; b < bd.num_blocks; b++) {
+    Oct root{b, 0, 0, 0, 0, 0, 0};
+    double h[3];
+    f0->element_extent(root, h);
+    int c = -1;
+    for (size_t k = 0; k < ext.size(); k++) {
+      bool eq = true;
+      for (int a = 0; a < 3; a++)
+        if (std::fabs(ext[k][a] - h[a]) > 1e-12 * std::fabs(h[a])) eq = false;
+      if (eq) {
+        c = (int)k;
+        break;
+      }
+    }
+    if (c < 0) {
+      c = (int)ext.size();
+      ext.push_back({h[0], h[1], h[2]});
+    }
+    block_cls[b] = c;
+  }
+  nclasses_ = (int)ext.size();
+  double C[36];
+  isotropic_C(mat.E, mat.nu, C);
+  K0h_.assign(576 * (size_t)nclasses_, 0.0);
+  for (int c = 0; c < nclasses_; c++)
+    hex8_box_stiffness(ext[c][0], ext[c][1], ext[c][2], C, &K0h_[576 * c]);
+  std::vector<float> K0f(K0h_.begin(), K0h_.end());
+  if (K0d_.upload(K0h_) || K0f_.upload(K0f)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  for (int l = 0; l < nlevels; l++) {
+    Level &L = levels_[l];
+    const auto &octs = L.forest->octants();
+    L.cls.resize(octs.size());
+    for (size_t e = 0; e < octs.size(); e++) L.cls[e] = block_cls[octs[e].block];
+    if (build_level(l, nbcs, names, dof_masks)) return -1;
+  }
+  for (int l = 0; l + 1 < nlevels; l++)
+    if (build_transfer(l)) return -1;
+  const int64_t n = levels_[0].ndof;
+  if (f_.alloc(n) || u_.alloc(n) || r_.alloc(n) || p_.alloc(n) || q_.alloc(n) || z_.alloc(n) ||
+      parts_.alloc((size_t)std::max(2 * 2048, assemble_num_blocks(levels_[0].n_nodes)) + 16) ||
+      sc_.alloc(SC_N) || rho_.alloc(levels_[0].n_elem) ||
+      energy_.alloc(levels_[0].n_elem)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  HIPCHK(hipMemset(u_.p, 0, n * sizeof(double)));
+  return 0;
+}
+
+int Problem::build_level(int l, int nbcs, const char **names, const int *masks) {
+  Level &L = levels_[l];
+  Forest *f = L.forest;
+  const auto &octs = f->octants();
+  L.n_elem = f->num_elements();
+  L.n_nodes = f->num_owned_nodes();
+  L.n_dep = f->num_dep_nodes();
+  L.ndof = 3 * (int64_t)L.n_nodes;
+  L.conn = f->conn();
+  L.lev.resize(L.n_elem);
+  for (int e = 0; e < L.n_elem; e++) L.lev[e] = octs[e].level;
+  // boundary conditions by name (TMROctTACSCreator::setBoundaryConditions,
+  // src/TMR_TACSCreator.cpp:468-500); hanging nodes follow their parents
+  L.bcmask.assign(L.n_nodes, 0);
+  for (int k = 0; k < nbcs; k++) {
+    std::vector<int> nodes;
+    f->nodes_with_name(names[k], nodes);
+    const uint8_t m = (uint8_t)((masks ? masks[k] : 7) & 7);
+    for (int n : nodes)
+      if (n >= 0) L.bcmask[n] |= m;
+  }
+  // element batches
+  const int ne = L.n_elem;
+  const int next = L.n_nodes + L.n_dep;
+  std::vector<int> stamp(next, -1), local(next, 0);
+  std::vector<int> wg_node_ptr{0}, wg_nodes, wg_elem_ptr{0}, wg_class;
+  std::vector<uint16_t> lconn(8 * (size_t)ne);
+  std::vector<uint8_t> color(ne);
+  std::vector<uint32_t> nodecol;
+  int max_nodes = 0, ncolors = 0;
+  int e = 0, b = 0;
+  while (e < ne) {
+    const int start = e, c = L.cls[e];
+    const int nb0 = (int)wg_nodes.size();
+    while (e < ne && e - start < WG_ELEMS && L.cls[e] == c) {
+      for (int a = 0; a < 8; a++) {
+        const int g = L.conn[8 * e + a];
+        const int x = g >= 0 ? g : L.n_nodes + (-g - 1);
+        if (stamp[x] != b) {
+          stamp[x] = b;
+          local[x] = (int)wg_nodes.size() - nb0;
+          wg_nodes.push_back(g);
+        }
+        lconn[8 * (size_t)e + a] = (uint16_t)local[x];
+      }
+      e++;
+    }
+    const int nn = (int)wg_nodes.size() - nb0;
+    max_nodes = std::max(max_nodes, nn);
+    nodecol.assign(nn, 0u);
+    for (int k = start; k < e; k++) {
+      uint32_t used = 0;
+      for (int a = 0; a < 8; a++) used |= nodecol[lconn[8 * (size_t)k + a]];
+      int col = 0;
+      while (used & (1u << col)) col++;
+      if (col >= MAX_COLORS) {
+        last_error = "problem: element batch needs too many colors";
+        return -1;
+      }
+      color[k] = (uint8_t)col;
+      ncolors = std::max(ncolors, col + 1);
+      for (int a = 0; a < 8; a++) nodecol[lconn[8 * (size_t)k + a]] |= 1u << col;
+    }
+    wg_node_ptr.push_back((int)wg_nodes.size());
+    wg_elem_ptr.push_back(e);
+    wg_class.push_back(c);
+    b++;
+  }
+  // assembly CSR: node <- its slots (w = 1) and the slots of the hanging
+  // nodes it supports (w = hanging-node weight), in increasing slot order
+  const auto &dptr = f->dep_ptr();
+  const auto &dconn = f->dep_conn();
+  const auto &dw = f->dep_weights();
+  const int nslots = (int)wg_nodes.size();
+  std::vector<int> aptr(L.n_nodes + 1, 0);
+  for (int s = 0; s < nslots; s++) {
+    const int g = wg_nodes[s];
+    if (g >= 0) {
+      aptr[g + 1]++;
+    } else {
+      const int d = -g - 1;
+      for (int j = dptr[d]; j < dptr[d + 1]; j++) aptr[dconn[j] + 1]++;
+    }
+  }
+  for (int i = 0; i < L.n_nodes; i++) aptr[i + 1] += aptr[i];
+  std::vector<int> aslot(aptr[L.n_nodes]);
+  std::vector<float> aw(aptr[L.n_nodes]);
+  {
+    std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+    for (int s = 0; s < nslots; s++) {
+      const int g = wg_nodes[s];
+      if (g >= 0) {
+        aslot[fill[g]] = s;
+        aw[fill[g]++] = 1.0f;
+      } else {
+        const int d = -g - 1;
+        for (int j = dptr[d]; j < dptr[d + 1]; j++) {
+          const int p = dconn[j];
+          aslot[fill[p]] = s;
+          aw[fill[p]++] = (float)dw[j];
+        }
+      }
+    }
+  }
+  std::vector<int> dp(dptr.begin(), dptr.end());
+  if (dp.empty()) dp.push_back(0);
+  std::vector<float> dwf(dw.begin(), dw.end());
+  if (L.wg_node_ptr.upload(wg_node_ptr) || L.wg_nodes.upload(wg_nodes) ||
+      L.wg_elem_ptr.upload(wg_elem_ptr) || L.wg_class.upload(wg_class) ||
+      L.lconn.upload(lconn) || L.color.upload(color) || L.d_bcmask.upload(L.bcmask) ||
+      L.dep_ptr.upload(dp) || L.dep_conn.upload(dconn) || L.dep_w.upload(dwf) ||
+      L.asm_ptr.upload(aptr) || L.asm_slot.upload(aslot) || L.asm_w.upload(aw) ||
+      L.d_conn.upload(L.conn) || L.d_cls.upload(L.cls) || L.d_lev.upload(L.lev) ||
+      L.s64.alloc(ne) || L.s32.alloc(ne)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  ElemBatches &eb = L.eb;
+  eb.n_wg = b;
+  eb.n_elem = ne;
+  eb.ncolors = ncolors;
+  eb.max_wg_nodes = max_nodes;
+  eb.wg_node_ptr = L.wg_node_ptr.p;
+  eb.wg_nodes = L.wg_nodes.p;
+  eb.wg_elem_ptr = L.wg_elem_ptr.p;
+  eb.wg_class = L.wg_class.p;
+  eb.lconn = L.lconn.p;
+  eb.color = L.color.p;
+  eb.dep_ptr = L.dep_ptr.p;
+  eb.dep_conn = L.dep_conn.p;
+  eb.dep_w = L.dep_w.p;
+  eb.bcmask = L.d_bcmask.p;
+  L.asmc.n_nodes = L.n_nodes;
+  L.asmc.ptr = L.asm_ptr.p;
+  L.asmc.slot = L.asm_slot.p;
+  L.asmc.w = L.asm_w.p;
+  return 0;
+}
+
+int Problem::build_transfer(int l) {
+  Level &F = levels_[l];
+  Level &Cc = levels_[l + 1];
+  InterpCSR P;
+  if (F.forest->create_interpolation(Cc.forest, P)) {
+    last_error = "problem: interpolation " + std::to_string(l) + ": " + F.forest->last_error;
+    return -1;
+  }
+  if (P.nrows != F.n_nodes || P.ncols != Cc.n_nodes) {
+    last_error = "problem: interpolation size mismatch";
+    return -1;
+  }
+  std::vector<float> pw(P.w.begin(), P.w.end());
+  // transpose (restriction), rows in coarse order, entries in fine order
+  std::vector<int> rptr(Cc.n_nodes + 1, 0);
+  for (int c : P.cols) rptr[c + 1]++;
+  for (int i = 0; i < Cc.n_nodes; i++) rptr[i + 1] += rptr[i];
+  std::vector<int> rcol(P.cols.size());
+  std::vector<float> rw(P.cols.size());
+  {
+    std::vector<int> fill(rptr.begin(), rptr.end() - 1);
+    for (int r = 0; r < P.nrows; r++)
+      for (int j = P.rowp[r]; j < P.rowp[r + 1]; j++) {
+        const int c = P.cols[j];
+        rcol[fill[c]] = r;
+        rw[fill[c]++] = (float)P.w[j];
+      }
+  }
+  // element containment: each fine element lies in one coarse element
+  const auto &fo = F.forest->octants();
+  std::vector<int> owner(F.n_elem);
+  std::vector<int> cnt(Cc.n_elem + 1, 0);
+  for (int e = 0; e < F.n_elem; e++) {
+    const int E = Cc.forest->locate(fo[e]);
+    if (E < 0 || Cc.lev[E] > F.lev[e]) {
+      last_error = "problem: coarse forest does not cover the fine forest";
+      return -1;
+    }
+    owner[e] = E;
+    cnt[E + 1]++;
+  }
+  for (int i = 0; i < Cc.n_elem; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> child(F.n_elem);
+  std::vector<float> vw(F.n_elem);
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < F.n_elem; e++) {
+      const int E = owner[e];
+      child[fill[E]] = e;
+      vw[fill[E]++] = (float)std::ldexp(1.0, -3 * (F.lev[e] - Cc.lev[E]));
+    }
+  }
+  if (F.P_ptr.upload(P.rowp) || F.P_col.upload(P.cols) || F.P_w.upload(pw) ||
+      F.R_ptr.upload(rptr) || F.R_col.upload(rcol) || F.R_w.upload(rw) ||
+      F.sc_ptr.upload(cnt) || F.sc_child.upload(child) || F.sc_w.upload(vw)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  F.P = {F.n_nodes, F.P_ptr.p, F.P_col.p, F.P_w.p};
+  F.R = {Cc.n_nodes, F.R_ptr.p, F.R_col.p, F.R_w.p};
+  return 0;
+}
+
+int Problem::ensure_vecs(int l, int precision) {
+  Level &L = levels_[l];
+  const size_t n = (size_t)L.ndof;
+  const size_t ns = 3 * (size_t)L.wg_nodes.n;
+  if (precision == 1) {
+    auto &v = L.v32;
+    if (v.ready) return 0;
+    if (v.b.alloc(n) || v.x.alloc(n) || v.r.alloc(n) || v.d.alloc(n) || v.dinv.alloc(n) ||
+        v.partial.alloc(ns))
+      return -1;
+    HIPCHK(hipMemset(v.d.p, 0, n * sizeof(float)));
+    HIPCHK(hipMemset(v.x.p, 0, n * sizeof(float)));
+    v.ready = true;
+  } else {
+    auto &v = L.v64;
+    if (v.ready) return 0;
+    if (v.b.alloc(n) || v.x.alloc(n) || v.r.alloc(n) || v.d.alloc(n) || v.dinv.alloc(n) ||
+        v.partial.alloc(ns))
+      return -1;
+    HIPCHK(hipMemset(v.d.p, 0, n * sizeof(double)));
+    HIPCHK(hipMemset(v.x.p, 0, n * sizeof(double)));
+    v.ready = true;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+int Problem::set_design(const double *rho, int n, bool on_device) {
+  if (levels_.empty()) {
+    last_error = "problem: not initialised";
+    return -1;
+  }
+  if (n != levels_[0].n_elem) {
+    last_error = "set_design: expected one density per finest-level element";
+    return -1;
+  }
+  HIPCHK(hipMemcpyAsync(rho_.p, rho, n * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  launch_simp_scale(n, rho_.p, levels_[0].d_lev.p, mat_.q, mat_.k0, mat_.penalty_type,
+                    levels_[0].s64.p, levels_[0].s32.p, st_);
+  for (size_t l = 0; l + 1 < levels_.size(); l++) {
+    Level &F = levels_[l], &Cc = levels_[l + 1];
+    launch_restrict_scale(Cc.n_elem, F.sc_ptr.p, F.sc_child.p, F.sc_w.p, F.s64.p, F.d_lev.p,
+                          Cc.d_lev.p, Cc.s64.p, Cc.s32.p, st_);
+  }
+  have_design_ = true;
+  have_setup_ = false;
+  return 0;
+}
+
+template <class T>
+void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
+  Level &L = levels_[l];
+  LevelVecs<T> &v = vecs<T>(l);
+  std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+  if (timing_ && l == 0) {
+    if (ev_used_ == ev_pool_.size()) {
+      std::pair<hipEvent_t, hipEvent_t> e;
+      if (hipEventCreate(&e.first) == hipSuccess && hipEventCreate(&e.second) == hipSuccess)
+        ev_pool_.push_back(e);
+    }
+    if (ev_used_ < ev_pool_.size()) ev = &ev_pool_[ev_used_++];
+    ev_prec_mask_ |= (sizeof(T) == 4) ? 1 : 2;
+  }
+  if (ev) (void)hipEventRecord(ev->first, st_);
+  launch_elem_apply<T>(L.eb, K0<T>(), scale<T>(l), x, v.partial.p, st_);
+  if (ev) (void)hipEventRecord(ev->second, st_);
+  AsmArgs g = a;
+  if (!g.x) g.x = (void *)x;
+  launch_assemble<T>(L.asmc, L.d_bcmask.p, v.partial.p, mode, g, st_);
+}
+
+int Problem::estimate_eigs(int l, const SolveOpts &o) {
+  // power iteration on Dinv A (FP64), lambda_max = |Dinv A v| / |v|
+  Level &L = levels_[l];
+  auto &v = L.v64;
+  const int64_t n = L.ndof;
+  std::vector<double> h(n);
+  uint64_t s = 0x9e3779b97f4a7c15ULL ^ (uint64_t)(l + 1);
+  for (int64_t i = 0; i < n; i++) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    h[i] = ((L.bcmask[i / 3] >> (i % 3)) & 1) ? 0.0 : 0.5 + (double)(s >> 11) * 0x1.0p-53;
+  }
+  HIPCHK(hipMemcpy(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  double lam = 1.0;
+  std::vector<double> dinvh(n);
+  HIPCHK(hipMemcpy(dinvh.data(), v.dinv.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  for (int it = 0; it < std::max(1, o.eig_iters); it++) {
+    AsmArgs a;
+    a.y = v.r.p;
+    apply_level<double>(l, v.x.p, ASM_STORE, a);
+    HIPCHK(hipMemcpyAsync(h.data(), v.r.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    std::vector<double> xv(n);
+    HIPCHK(hipMemcpy(xv.data(), v.x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    double nw = 0.0, nv = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+      h[i] *= dinvh[i];
+      nw += h[i] * h[i];
+      nv += xv[i] * xv[i];
+    }
+    lam = std::sqrt(nw / nv);
+    const double sc = 1.0 / std::sqrt(nw);
+    for (int64_t i = 0; i < n; i++) h[i] *= sc;
+    HIPCHK(hipMemcpy(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  L.lam_max = lam;
+  return 0;
+}
+
+int Problem::coarse_factor() {
+  // dense K on the coarsest level (identity rows for clamped dofs), Cholesky
+  // in FP64, explicit inverse for a one-GEMV coarse solve
+  Level &L = levels_.back();
+  const int n = (int)L.ndof;
+  std::vector<double> s(L.n_elem);
+  HIPCHK(hipMemcpy(s.data(), L.s64.p, L.n_elem * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> A((size_t)n * n, 0.0);
+  const auto &dptr = L.forest->dep_ptr();
+  const auto &dconn = L.forest->dep_conn();
+  const auto &dw = L.forest->dep_weights();
+  for (int e = 0; e < L.n_elem; e++) {
+    const double *K = &K0h_[576 * L.cls[e]];
+    std::vector<std::pair<int, double>> ex[8];
+    for (int a = 0; a < 8; a++) {
+      const int g = L.conn[8 * e + a];
+      if (g >= 0)
+        ex[a].push_back({g, 1.0});
+      else
+        for (int j = dptr[-g - 1]; j < dptr[-g]; j++) ex[a].push_back({dconn[j], dw[j]});
+    }
+    for (int a = 0; a < 8; a++)
+      for (auto &pa : ex[a])
+        for (int bb = 0; bb < 8; bb++)
+          for (auto &pb : ex[bb]) {
+            const double w = s[e] * pa.second * pb.second;
+            for (int i = 0; i < 3; i++) {
+              if ((L.bcmask[pa.first] >> i) & 1) continue;
+              for (int j = 0; j < 3; j++) {
+                if ((L.bcmask[pb.first] >> j) & 1) continue;
+                A[(size_t)(3 * pa.first + i) * n + 3 * pb.first + j] +=
+                    w * K[(3 * a + i) * 24 + 3 * bb + j];
+              }
+            }
+          }
+  }
+  for (int i = 0; i < n; i++)
+    if ((L.bcmask[i / 3] >> (i % 3)) & 1) A[(size_t)i * n + i] = 1.0;
+  // Cholesky A = L L^T (lower, row-major, in place)
+  for (int j = 0; j < n; j++) {
+    double *Aj = &A[(size_t)j * n];
+    double d = Aj[j];
+    for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
+    if (!(d > 0.0)) {
+      last_error = "coarse Cholesky: matrix is not positive definite (missing boundary conditions?)";
+      return -1;
+    }
+    d = std::sqrt(d);
+    Aj[j] = d;
+    const double inv = 1.0 / d;
+    for (int i = j + 1; i < n; i++) {
+      double *Ai = &A[(size_t)i * n];
+      double t = Ai[j];
+      for (int k = 0; k < j; k++) t -= Ai[k] * Aj[k];
+      Ai[j] = t * inv;
+    }
+  }
+  // Linv (lower), then Ainv = Linv^T Linv
+  std::vector<double> Li((size_t)n * n, 0.0);
+  for (int c = 0; c < n; c++) {
+    Li[(size_t)c * n + c] = 1.0 / A[(size_t)c * n + c];
+    for (int i = c + 1; i < n; i++) {
+      double t = 0.0;
+      const double *Ai = &A[(size_t)i * n];
+      for (int k = c; k < i; k++) t -= Ai[k] * Li[(size_t)k * n + c];
+      Li[(size_t)i * n + c] = t / Ai[i];
+    }
+  }
+  std::vector<double> Ai((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j <= i; j++) {
+      double t = 0.0;
+      for (int k = i; k < n; k++) t += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
+      Ai[(size_t)i * n + j] = t;
+      Ai[(size_t)j * n + i] = t;
+    }
+  n_coarse_dof_ = n;
+  if (Ainv_.upload(Ai)) {
+    last_error = "coarse: device allocation failed";
+    return -1;
+  }
+  return 0;
+}
+
+int Problem::setup(const SolveOpts &o) {
+  if (!have_design_) {
+    last_error = "setup: set the design first";
+    return -1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const int prec = o.precision ? 1 : 0;
+  for (size_t l = 0; l < levels_.size(); l++) {
+    if (ensure_vecs((int)l, 0) || (prec == 1 && ensure_vecs((int)l, 1))) {
+      last_error = "setup: device allocation failed";
+      return -1;
+    }
+  }
+  for (size_t l = 0; l + 1 < levels_.size(); l++) {
+    Level &L = levels_[l];
+    // Jacobi diagonal in both precisions
+    launch_elem_diag<double>(L.eb, K0d_.p, L.s64.p, L.v64.partial.p, st_);
+    AsmArgs a;
+    a.y = L.v64.dinv.p;
+    launch_assemble<double>(L.asmc, L.d_bcmask.p, L.v64.partial.p, ASM_DIAG, a, st_);
+    if (prec == 1) launch_convert<double, float>(L.ndof, L.v64.dinv.p, L.v32.dinv.p, st_);
+    if (estimate_eigs((int)l, o)) return -1;
+  }
+  if (coarse_factor()) return -1;
+  HIPCHK(hipStreamSynchronize(st_));
+  have_setup_ = true;
+  setup_precision_ = prec;
+  last_setup_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+// Chebyshev smoother (degree k) on Dinv A with eigenvalue bounds
+// [lower, upper] * lambda_max (TMR's TACSChebyshevSmoother interval,
+// src/TMR_RefinementTools.cpp:46-47)
+template <class T>
+void Problem::smooth(int l, const T *b, T *x, bool zero_guess, const SolveOpts &o) {
+  Level &L = levels_[l];
+  LevelVecs<T> &v = vecs<T>(l);
+  const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rho_prev = 1.0 / sigma;
+  for (int k = 0; k < o.cheb_degree; k++) {
+    double ca, cc;
+    if (k == 0) {
+      ca = 0.0;
+      cc = 1.0 / theta;
+    } else {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      ca = rho * rho_prev;
+      cc = 2.0 * rho / delta;
+      rho_prev = rho;
+    }
+    if (k == 0 && zero_guess) {
+      launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, cc, v.d.p, x, st_);
+      continue;
+    }
+    AsmArgs a;
+    a.b = b;
+    a.x = x;
+    a.d = v.d.p;
+    a.dinv = v.dinv.p;
+    a.ca = ca;
+    a.cc = cc;
+    apply_level<T>(l, x, ASM_CHEB, a);
+  }
+}
+
+template <class T>
+void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
+  const int last = (int)levels_.size() - 1;
+  if (l == last) {
+    launch_dense_solve<T>(n_coarse_dof_, Ainv_.p, b, x, st_);
+    return;
+  }
+  Level &L = levels_[l];
+  LevelVecs<T> &v = vecs<T>(l);
+  LevelVecs<T> &c = vecs<T>(l + 1);
+  smooth<T>(l, b, x, true, o);
+  AsmArgs a;
+  a.b = b;
+  a.y = v.r.p;
+  apply_level<T>(l, x, ASM_RESID, a);
+  launch_restrict<T>(L.R, levels_[l + 1].d_bcmask.p, v.r.p, c.b.p, st_);
+  vcycle<T>(l + 1, c.b.p, c.x.p, o);
+  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, x, st_);
+  smooth<T>(l, b, x, false, o);
+}
+
+int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &o,
+                   SolveInfo *info) {
+  if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
+    if (setup(o)) return -1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  Level &L0 = levels_[0];
+  const int64_t n = L0.ndof;
+  const int nrb = reduce_blocks(n);
+  if (f) {
+    HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  }
+  HIPCHK(hipMemsetAsync(u_.p, 0, n * sizeof(double), st_));
+  // clamped rows of the right-hand side are zero (TACSAssembler::applyBCs):
+  // r = f - A*0 through the assembly pass, which zeroes clamped rows
+  {
+    launch_fill<double>(3 * (int64_t)L0.wg_nodes.n, 0.0, L0.v64.partial.p, st_);
+    AsmArgs a;
+    a.b = f_.p;
+    a.y = r_.p;
+    a.x = u_.p;
+    launch_assemble<double>(L0.asmc, L0.d_bcmask.p, L0.v64.partial.p, ASM_RESID, a, st_);
+  }
+  HIPCHK(hipMemcpyAsync(f_.p, r_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  launch_dot1(n, r_.p, r_.p, parts_.p, st_);
+  launch_finalize(nrb, parts_.p, 4, sc_.p, st_);
+  const bool mixed = o.precision != 0;
+  auto precond = [&]() {
+    if (mixed) {
+      vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o);
+      launch_convert<float, double>(n, L0.v32.x.p, z_.p, st_);
+    } else {
+      vcycle<double>(0, r_.p, z_.p, o);
+    }
+  };
+  if (mixed) launch_convert<double, float>(n, r_.p, L0.v32.b.p, st_);
+  precond();
+  launch_dot1(n, r_.p, z_.p, parts_.p, st_);
+  launch_finalize(nrb, parts_.p, 0, sc_.p, st_);
+  HIPCHK(hipMemcpyAsync(p_.p, z_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  double hsc[SC_N];
+  HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  const double bnorm = std::sqrt(hsc[SC_BB]);
+  double rnorm = bnorm;
+  int it = 0;
+  bool conv = bnorm == 0.0;
+  const int nab = assemble_num_blocks(L0.n_nodes);
+  while (!conv && it < o.max_iters) {
+    it++;
+    AsmArgs a;
+    a.y = q_.p;
+    a.partials = parts_.p;
+    apply_level<double>(0, p_.p, ASM_DOT, a);
+    launch_finalize(nab, parts_.p, 1, sc_.p, st_);
+    launch_pcg_update_xr(n, u_.p, r_.p, p_.p, q_.p, sc_.p, parts_.p, mixed ? L0.v32.b.p : nullptr,
+                         st_);
+    launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
+    HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    rnorm = std::sqrt(std::max(hsc[SC_RR], 0.0));
+    if (o.monitor) fprintf(stderr, "[tmrx] pcg %3d  |r|/|b| = %.3e\n", it, rnorm / bnorm);
+    if (!(rnorm == rnorm)) break;
+    if (rnorm <= o.rtol * bnorm || rnorm <= o.atol) {
+      conv = true;
+      break;
+    }
+    precond();
+    launch_dot2(n, z_.p, r_.p, q_.p, parts_.p, st_);
+    launch_finalize(nrb, parts_.p, 3, sc_.p, st_);
+    launch_pcg_update_p(n, p_.p, z_.p, sc_.p, st_);
+  }
+  // true residual with the FP64 operator
+  {
+    AsmArgs a;
+    a.b = f_.p;
+    a.y = q_.p;
+    apply_level<double>(0, u_.p, ASM_RESID, a);
+    launch_dot1(n, q_.p, q_.p, parts_.p, st_);
+    launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
+    HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    last_true_residual = bnorm > 0 ? std::sqrt(std::max(hsc[SC_RR], 0.0)) / bnorm : 0.0;
+  }
+  if (u) {
+    HIPCHK(hipMemcpyAsync(u, u_.p, n * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+  }
+  have_solution_ = true;
+  if (info) {
+    info->iters = it;
+    info->converged = conv ? 1 : 0;
+    info->rnorm0 = bnorm;
+    info->rnorm = rnorm;
+    info->setup_ms = last_setup_ms_;
+    info->solve_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (!conv) {
+    last_error = "solve: PCG did not converge to the requested tolerance";
+    return 1;
+  }
+  return 0;
+}
+
+int Problem::compliance(double *c) {
+  if (!have_solution_) {
+    last_error = "compliance: solve first";
+    return -1;
+  }
+  const int64_t n = levels_[0].ndof;
+  launch_dot1(n, f_.p, u_.p, parts_.p, st_);
+  launch_finalize(reduce_blocks(n), parts_.p, 4, sc_.p, st_);
+  double h[SC_N];
+  HIPCHK(hipMemcpyAsync(h, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  *c = h[SC_BB];
+  return 0;
+}
+
+int Problem::compliance_grad(double *g, bool on_device) {
+  if (!have_solution_) {
+    last_error = "compliance_grad: solve first";
+    return -1;
+  }
+  Level &L = levels_[0];
+  launch_elem_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+                     L.d_cls.p, K0d_.p, u_.p, energy_.p, st_);
+  double *out = g;
+  DevArray<double> tmp;
+  if (!on_device) {
+    if (tmp.alloc(L.n_elem)) {
+      last_error = "compliance_grad: allocation failed";
+      return -1;
+    }
+    out = tmp.p;
+  }
+  launch_compliance_grad(L.n_elem, rho_.p, L.d_lev.p, mat_.q, mat_.penalty_type, energy_.p, out,
+                         st_);
+  if (!on_device)
+    HIPCHK(hipMemcpyAsync(g, out, L.n_elem * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+int Problem::apply(const double *x, double *y, bool on_device, int precision) {
+  if (!have_design_) {
+    last_error = "apply: set the design first";
+    return -1;
+  }
+  Level &L = levels_[0];
+  const int64_t n = L.ndof;
+  if (ensure_vecs(0, 0) || (precision && ensure_vecs(0, 1))) {
+    last_error = "apply: allocation failed";
+    return -1;
+  }
+  HIPCHK(hipMemcpyAsync(p_.p, x, n * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  AsmArgs a;
+  if (precision) {
+    launch_convert<double, float>(n, p_.p, L.v32.b.p, st_);
+    a.y = L.v32.r.p;
+    apply_level<float>(0, L.v32.b.p, ASM_STORE, a);
+    launch_convert<float, double>(n, L.v32.r.p, q_.p, st_);
+  } else {
+    a.y = q_.p;
+    apply_level<double>(0, p_.p, ASM_STORE, a);
+  }
+  HIPCHK(hipMemcpyAsync(y, q_.p, n * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+// Consistent load of a constant traction on the faces named `name`
+// (TopOptUtils.compute3DTractionLoad, tmr/TopOptUtils.py:241-267, with the
+// bilinear face integral of a box face: t * area / 4 per face node; hanging
+// nodes pass their share to their parents).
+int Problem::traction_load(const char *name, const double tr[3], double *f) {
+  Level &L = levels_[0];
+  std::vector<Oct> octs;
+  L.forest->octs_with_name(name, octs);
+  std::memset(f, 0, sizeof(double) * L.ndof);
+  const auto &dptr = L.forest->dep_ptr();
+  const auto &dconn = L.forest->dep_conn();
+  const auto &dw = L.forest->dep_weights();
+  for (const Oct &o : octs) {
+    const int face = o.info;
+    double h[3];
+    L.forest->element_extent(o, h);
+    const int ax = face / 2;
+    const double area = h[(ax + 1) % 3] * h[(ax + 2) % 3];
+    const int *c = &L.conn[8 * o.tag];
+    for (int k = 0; k < 8; k++) {
+      const int bit = (ax == 0) ? (k & 1) : (ax == 1) ? ((k >> 1) & 1) : (k >> 2);
+      if (bit != face % 2) continue;
+      const int g = c[k];
+      if (g >= 0) {
+        for (int d = 0; d < 3; d++) f[3 * g + d] += 0.25 * area * tr[d];
+      } else {
+        for (int j = dptr[-g - 1]; j < dptr[-g]; j++)
+          for (int d = 0; d < 3; d++) f[3 * dconn[j] + d] += dw[j] * 0.25 * area * tr[d];
+      }
+    }
+  }
+  return 0;
+}
+
+template void Problem::vcycle<float>(int, const float *, float *, const SolveOpts &);
+template void Problem::vcycle<double>(int, const double *, double *, const SolveOpts &);
+
+}  // namespace tmrx

@@ -1,0 +1,197 @@
+// Multigrid hierarchy + FP64 flexible PCG on one GPU.
+//
+// Replaces the TACS objects the reference drives from TMR: TACSAssembler
+// (src/TMR_TACSCreator.cpp:332-461), TACSMg with Chebyshev smoothing and a
+// coarse direct solve (src/TMR_RefinementTools.cpp:37-107), and the Krylov
+// solve, compliance and gradient of TMRTopoProblem
+// (src/topology/TMRTopoProblem.cpp:967-1060, 1216-1295).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "forest.h"
+#include "kernels.h"
+
+namespace tmrx {
+
+template <class T>
+struct DevArray {
+  T *p = nullptr;
+  size_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  DevArray(DevArray &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevArray &operator=(DevArray &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return -1;
+    }
+    n = count;
+    return 0;
+  }
+  int upload(const T *h, size_t count) {
+    if (alloc(count)) return -1;
+    if (count && hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    return 0;
+  }
+  int upload(const std::vector<T> &v) { return upload(v.data(), v.size()); }
+};
+
+struct Material {
+  double E = 1.0, nu = 0.3, ys = 1.0;
+  double q = 3.0, k0 = 1e-6, eps = 0.3;
+  int penalty_type = 0;  // 0 SIMP, 1 RAMP
+};
+
+struct SolveOpts {
+  double rtol = 1e-8;
+  double atol = 1e-30;
+  int max_iters = 200;
+  int precision = 1;  // 0: FP64 V-cycle, 1: FP32 V-cycle inside FP64 PCG
+  int cheb_degree = 2;
+  double cheb_lower = 1.0 / 30.0;
+  double cheb_upper = 1.1;
+  int eig_iters = 10;
+  int monitor = 0;
+  int zero_guess = 1;
+};
+
+struct SolveInfo {
+  int iters = 0;
+  int converged = 0;
+  double rnorm0 = 0.0;
+  double rnorm = 0.0;
+  double setup_ms = 0.0;
+  double solve_ms = 0.0;
+};
+
+template <class T>
+struct LevelVecs {
+  DevArray<T> b, x, r, d, dinv, partial, s;
+  bool ready = false;
+};
+
+struct Level {
+  Forest *forest = nullptr;
+  int n_nodes = 0, n_dep = 0, n_elem = 0;
+  int64_t ndof = 0;
+  // host copies
+  std::vector<int16_t> lev;
+  std::vector<int> cls;
+  std::vector<int> conn;
+  std::vector<uint8_t> bcmask;
+  // device topology
+  DevArray<int> wg_node_ptr, wg_nodes, wg_elem_ptr, wg_class;
+  DevArray<uint16_t> lconn;
+  DevArray<uint8_t> color, d_bcmask;
+  DevArray<int> dep_ptr, dep_conn, asm_ptr, asm_slot, d_conn, d_cls;
+  DevArray<float> dep_w, asm_w;
+  DevArray<int16_t> d_lev;
+  ElemBatches eb;
+  AsmCSR asmc;
+  // transfer to the next coarser level (P: rows = this level's nodes)
+  DevArray<int> P_ptr, P_col, R_ptr, R_col;
+  DevArray<float> P_w, R_w;
+  TransferCSR P, R;
+  // element-scale restriction from this level into the next coarser one
+  // (CSR over the coarse elements)
+  DevArray<int> sc_ptr, sc_child;
+  DevArray<float> sc_w;
+  // stiffness scales
+  DevArray<double> s64;
+  DevArray<float> s32;
+  double lam_max = 0.0;
+  LevelVecs<float> v32;
+  LevelVecs<double> v64;
+};
+
+class Problem {
+ public:
+  Problem() = default;
+  ~Problem();
+  int init(int nlevels, Forest **forests, const Material &mat, int nbcs, const char **names,
+           const int *dof_masks);
+  int set_design(const double *rho, int n, bool on_device);
+  int setup(const SolveOpts &o);
+  int solve(const double *f, double *u, bool on_device, const SolveOpts &o, SolveInfo *info);
+  int compliance(double *c);
+  int compliance_grad(double *g, bool on_device);
+  int apply(const double *x, double *y, bool on_device, int precision);
+  int traction_load(const char *name, const double tr[3], double *f);
+  int num_dof() const { return levels_.empty() ? 0 : (int)levels_[0].ndof; }
+  int num_elements() const { return levels_.empty() ? 0 : levels_[0].n_elem; }
+  int num_levels() const { return (int)levels_.size(); }
+  const Level &level(int i) const { return levels_[i]; }
+  void set_stream(hipStream_t s) { st_ = s; }
+  // HIP-event timing of every finest-level element-pass launch
+  int kernel_timing(int enable);
+  int kernel_timing_result(double *total_ms, int *count, int *precision_mask);
+  double last_true_residual = 0.0;
+  std::string last_error;
+
+ private:
+  int build_level(int l, int nbcs, const char **names, const int *masks);
+  int build_transfer(int l);
+  int ensure_vecs(int l, int precision);
+  template <class T>
+  LevelVecs<T> &vecs(int l);
+  template <class T>
+  const T *K0();
+  template <class T>
+  const T *scale(int l);
+  template <class T>
+  void apply_level(int l, const T *x, int mode, const AsmArgs &a);
+  template <class T>
+  void smooth(int l, const T *b, T *x, bool zero_guess, const SolveOpts &o);
+  template <class T>
+  void vcycle(int l, const T *b, T *x, const SolveOpts &o);
+  int coarse_factor();
+  int estimate_eigs(int l, const SolveOpts &o);
+
+  std::vector<Level> levels_;
+  Material mat_;
+  int nclasses_ = 0;
+  std::vector<double> K0h_;  // per class, full-block element
+  DevArray<double> K0d_;
+  DevArray<float> K0f_;
+  DevArray<double> rho_;
+  DevArray<double> Ainv_;
+  int n_coarse_dof_ = 0;
+  // PCG state
+  DevArray<double> f_, u_, r_, p_, q_, z_, parts_, sc_;
+  DevArray<double> energy_;
+  bool have_design_ = false, have_setup_ = false, have_solution_ = false;
+  int setup_precision_ = -1;
+  double last_setup_ms_ = 0.0;
+  hipStream_t st_ = nullptr;
+  bool timing_ = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+  size_t ev_used_ = 0;
+  int ev_prec_mask_ = 0;
+};
+
+}  // namespace tmrx

@@ -1,0 +1,139 @@
+"""Python handle of the native problem (multigrid hierarchy + FP64 PCG on one
+GPU).  Arrays may be numpy (host) or torch tensors on the GPU (device
+pointers are passed straight through the C-ABI)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+SIMP = 0
+RAMP = 1
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, dtype=np.double):
+    """(pointer, on_device) for a numpy array or a contiguous torch tensor."""
+    if _is_torch(a):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr()), 1 if a.is_cuda else 0
+    return C.c_void_p(a.ctypes.data), 0
+
+
+def default_opts(**kw):
+    o = _lib.SolveOpts()
+    lib().tmrx_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Problem:
+    """Finite-element hierarchy on forests[0] (finest) ... forests[-1]."""
+
+    def __init__(self, forests, E=1.0, nu=0.3, ys=1.0, q=3.0, k0=1e-6, eps=0.3,
+                 penalty_type=SIMP, bcs=(), device=0):
+        self.forests = list(forests)
+        m = _lib.Material(E, nu, ys, q, k0, eps, penalty_type)
+        names = [b[0].encode() for b in bcs]
+        masks = [int(b[1]) if len(b) > 1 else 7 for b in bcs]
+        carr = (C.c_char_p * max(1, len(names)))(*names)
+        marr = (C.c_int * max(1, len(masks)))(*masks)
+        farr = (C.c_void_p * len(self.forests))(*[f.ptr.value for f in self.forests])
+        h = C.c_void_p()
+        check(lib().tmrx_problem_create(len(self.forests), farr, C.byref(m), len(names), carr, marr,
+                                        int(device), C.byref(h)), "Problem")
+        self.ptr = h
+        nd, ne, nl = C.c_int(), C.c_int(), C.c_int()
+        check(lib().tmrx_problem_get_sizes(self.ptr, C.byref(nd), C.byref(ne), C.byref(nl)))
+        self.ndof, self.nelem, self.nlevels = nd.value, ne.value, nl.value
+        self.material = dict(E=E, nu=nu, ys=ys, q=q, k0=k0, eps=eps, penalty_type=penalty_type)
+        self.info = _lib.SolveInfo()
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().tmrx_problem_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        check(lib().tmrx_problem_set_stream(self.ptr, C.c_void_p(stream_ptr)))
+
+    def set_design(self, rho):
+        if not _is_torch(rho):
+            rho = np.ascontiguousarray(rho, dtype=np.double)
+        p, dev = _ptr(rho)
+        check(lib().tmrx_problem_set_design(self.ptr, p, int(self.nelem), dev), "set_design")
+
+    def setup(self, opts=None):
+        opts = opts or default_opts()
+        check(lib().tmrx_problem_setup(self.ptr, C.byref(opts)), "setup")
+
+    def solve(self, f, u=None, opts=None, allow_unconverged=False):
+        opts = opts or default_opts()
+        if u is None:
+            if _is_torch(f):
+                import torch
+                u = torch.empty_like(f)
+            else:
+                f = np.ascontiguousarray(f, dtype=np.double)
+                u = np.zeros(self.ndof)
+        fp, dev = _ptr(f)
+        up, _ = _ptr(u)
+        r = lib().tmrx_problem_solve(self.ptr, fp, up, dev, C.byref(opts), C.byref(self.info))
+        check(r, "solve")
+        if r == 1 and not allow_unconverged:
+            raise _lib.TmrxError("solve: PCG did not converge (%d iterations, |r|/|b| = %.3e)"
+                                 % (self.info.iters, self.info.rnorm / max(self.info.rnorm0, 1e-300)))
+        return u
+
+    def true_residual(self):
+        v = C.c_double()
+        check(lib().tmrx_problem_true_residual(self.ptr, C.byref(v)))
+        return v.value
+
+    def compliance(self):
+        v = C.c_double()
+        check(lib().tmrx_problem_compliance(self.ptr, C.byref(v)), "compliance")
+        return v.value
+
+    def compliance_grad(self, g=None):
+        if g is None:
+            g = np.zeros(self.nelem)
+        p, dev = _ptr(g)
+        check(lib().tmrx_problem_compliance_grad(self.ptr, p, dev), "compliance_grad")
+        return g
+
+    def apply(self, x, precision=0, y=None):
+        if not _is_torch(x):
+            x = np.ascontiguousarray(x, dtype=np.double)
+        if y is None:
+            y = np.zeros(self.ndof) if not _is_torch(x) else x.new_empty(x.shape)
+        xp, dev = _ptr(x)
+        yp, _ = _ptr(y)
+        check(lib().tmrx_problem_apply(self.ptr, xp, yp, dev, int(precision)), "apply")
+        return y
+
+    def kernel_timing(self, enable=True):
+        check(lib().tmrx_problem_kernel_timing(self.ptr, int(enable)))
+
+    def kernel_timing_result(self):
+        t, n, m = C.c_double(), C.c_int(), C.c_int()
+        check(lib().tmrx_problem_kernel_timing_result(self.ptr, C.byref(t), C.byref(n), C.byref(m)))
+        return t.value, n.value, m.value
+
+    def traction_load(self, name, tr):
+        f = np.zeros(self.ndof)
+        t = np.ascontiguousarray(tr, dtype=np.double)
+        check(lib().tmrx_problem_traction_load(self.ptr, name.encode(), t.ctypes.data_as(_lib.DP),
+                                               f.ctypes.data_as(_lib.DP)), "traction_load")
+        return f

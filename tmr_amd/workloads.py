@@ -1,0 +1,142 @@
+"""Synthetic workloads of the cantilever SIMP problems (seeded, no data files).
+
+* cube_l4     : unit cube, 1 block, uniform level 4 (4,096 hex), rho = 0.5,
+                3 multigrid levels;
+* box_256     : 2 x 1 x 1 box, 2 blocks at level 7 (256 x 128 x 128 =
+                4,194,304 hex, 12,830,211 dofs), seeded smoothed densities,
+                6 multigrid levels (coarsest 8 x 4 x 4);
+* adaptive    : 2 x 1 x 1 box of 2 x 2 x 2 blocks, per-octant levels from a
+                seeded Gaussian-sum density field, 2:1 balanced.
+All clamp the face x = 0 and load the face x = Lx with a unit traction -z.
+"""
+import numpy as np
+
+from . import TMR
+
+FACES = {"x-": "fixed", "x+": "load"}
+
+
+def hierarchy(forest, nlevels):
+    """Finest-first list of forests: coarsen + balance(1) per level
+    (TopOptUtils.createTopoProblem, tmr/TopOptUtils.py:69-99)."""
+    forest.balance(1)
+    forests = [forest]
+    for _ in range(nlevels - 1):
+        c = forests[-1].coarsen()
+        c.balance(1)
+        forests.append(c)
+    for f in forests:
+        f.createNodes()
+    return forests
+
+
+def smoothed_density(shape, seed, passes=3, lo=1e-3, hi=1.0):
+    """Uniform [lo, hi] densities, then `passes` passes of a 27-point box mean
+    (boundary-truncated), clipped to [lo, hi].  shape = (nz, ny, nx)."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(lo, hi, size=shape)
+    for _ in range(passes):
+        s = np.zeros_like(r)
+        c = np.zeros_like(r)
+        ones = np.ones_like(r)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    sl_dst, sl_src = [], []
+                    for d, n in zip((dz, dy, dx), shape):
+                        if d < 0:
+                            sl_dst.append(slice(0, n + d)); sl_src.append(slice(-d, n))
+                        elif d > 0:
+                            sl_dst.append(slice(d, n)); sl_src.append(slice(0, n - d))
+                        else:
+                            sl_dst.append(slice(0, n)); sl_src.append(slice(0, n))
+                    s[tuple(sl_dst)] += r[tuple(sl_src)]
+                    c[tuple(sl_dst)] += ones[tuple(sl_src)]
+        r = s / c
+    return np.clip(r, lo, hi)
+
+
+def element_density_from_grid(forest, grid, L):
+    """Sample a (nz, ny, nx) cell grid over the box [0,L] at element centres."""
+    X = forest._Xv
+    octs = forest.getOctants()
+    bc = forest.getConnectivity()["block_conn"]
+    x0 = X[bc[octs["block"], 0]]
+    ext = X[bc[:, 7]] - X[bc[:, 0]]
+    h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
+    u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
+    c = x0 + u * ext[octs["block"]] + 0.5 * h
+    nz, ny, nx = grid.shape
+    i = np.minimum((c[:, 0] / L[0] * nx).astype(np.int64), nx - 1)
+    j = np.minimum((c[:, 1] / L[1] * ny).astype(np.int64), ny - 1)
+    k = np.minimum((c[:, 2] / L[2] * nz).astype(np.int64), nz - 1)
+    return grid[k, j, i]
+
+
+def cube_l4(nlevels=3):
+    f = TMR.createBoxForest(1, 1, 1, (1.0, 1.0, 1.0), face_names=FACES)
+    f.createTrees(4)
+    forests = hierarchy(f, nlevels)
+    rho = np.full(len(forests[0].getOctants()), 0.5)
+    return forests, rho, (1.0, 1.0, 1.0)
+
+
+def box(level=7, nlevels=6, seed=0, passes=3):
+    L = (2.0, 1.0, 1.0)
+    f = TMR.createBoxForest(2, 1, 1, L, face_names=FACES)
+    f.createTrees(level)
+    forests = hierarchy(f, nlevels)
+    n = 1 << level
+    grid = smoothed_density((n, n, 2 * n), seed, passes)
+    rho = element_density_from_grid(forests[0], grid, L)
+    return forests, rho, L
+
+
+def gaussian_field(pts, seed, L=(2.0, 1.0, 1.0), ngauss=32):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 1, size=(ngauss, 3)) * np.array(L)
+    sig = rng.uniform(0.05, 0.2, size=ngauss)
+    amp = rng.uniform(0.0, 1.0, size=ngauss)
+    d2 = ((pts[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+    return np.clip((amp[None, :] * np.exp(-0.5 * d2 / sig[None, :] ** 2)).sum(1), 1e-3, 1.0)
+
+
+def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), max_sweeps=None):
+    """Adaptive forest on 2x2x2 blocks: octants where the seeded density has a
+    gradient above its median are refined towards a seeded level in
+    [base_level, max_level]; balance(1) after each sweep."""
+    rng = np.random.default_rng(seed + 1000)
+    f = TMR.createBoxForest(2, 2, 2, L, face_names=FACES)
+    f.createTrees(base_level)
+    sweeps = max_level - base_level if max_sweeps is None else max_sweeps
+    for _ in range(sweeps):
+        octs = f.getOctants()
+        X = f._Xv
+        bc = f.getConnectivity()["block_conn"]
+        ext = X[bc[:, 7]] - X[bc[:, 0]]
+        h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
+        u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
+        c = X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h
+        eps = 0.25 * h
+        g = np.zeros(len(octs))
+        for a in range(3):
+            dp = c.copy(); dm = c.copy()
+            dp[:, a] += eps[:, a]; dm[:, a] -= eps[:, a]
+            g += ((gaussian_field(dp, seed, L) - gaussian_field(dm, seed, L)) / (2 * eps[:, a])) ** 2
+        g = np.sqrt(g)
+        target = rng.integers(base_level, max_level + 1, size=len(octs))
+        ref = ((g > np.median(g)) & (octs["level"] < target)).astype(np.intc)
+        if not ref.any():
+            break
+        f.refine(ref, 0, max_level)
+        f.balance(1)
+    forests = hierarchy(f, nlevels)
+    octs = forests[0].getOctants()
+    X = forests[0]._Xv
+    bc = forests[0].getConnectivity()["block_conn"]
+    ext = X[bc[:, 7]] - X[bc[:, 0]]
+    h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
+    u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
+    cen = X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h
+    rho = gaussian_field(cen, seed, L)
+    return forests, rho, L
