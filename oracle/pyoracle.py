@@ -129,3 +129,111 @@ def from_forest(forest, bcs, E=1.0, nu=0.3):
     ptr, dc, dw = forest.getDepNodeConn()
     return Oracle(forest.getMeshConn(), ptr, dc, dw, forest.getElementExtents(),
                   bcmask_from_forest(forest, bcs), E, nu)
+
+
+# ---------------------------------------------------------------------------
+# forest oracle (geometric, box lattices)
+OCT_DT = np.dtype([("block", np.int32), ("x", np.int32), ("y", np.int32), ("z", np.int32),
+                   ("tag", np.int32), ("level", np.int16), ("info", np.int16)])
+
+
+def _flib():
+    L = lib()
+    if not hasattr(L, "_forest_sigs"):
+        P, I = C.c_void_p, C.c_int
+        L.orc_sort_octants.restype = I
+        L.orc_sort_octants.argtypes = [P, I, I]
+        L.orc_balance.restype = I
+        L.orc_balance.argtypes = [I, I, I, P, I, I]
+        L.orc_get_leaves.restype = I
+        L.orc_get_leaves.argtypes = [P]
+        L.orc_is_refined.restype = I
+        L.orc_is_refined.argtypes = [P]
+        L.orc_create_nodes.restype = I
+        L.orc_node_sizes.argtypes = [P, P, P, P]
+        L.orc_get_nodes.argtypes = [P, P, P, P, P]
+        L.orc_nodes_on_box_face.restype = I
+        L.orc_nodes_on_box_face.argtypes = [I, I, P]
+        L.orc_save_mesh.argtypes = [I]
+        L.orc_interp.restype = I
+        L.orc_get_interp.argtypes = [P, P, P]
+        L._forest_sigs = True
+    return L
+
+
+def sort_octants(octs, use_node=False):
+    a = np.ascontiguousarray(octs, dtype=OCT_DT).copy()
+    n = _flib().orc_sort_octants(_p(a), len(a), int(use_node))
+    return a[:n]
+
+
+class OracleMesh:
+    """Balanced leaves + nodes of a box-lattice forest, by brute force."""
+
+    def __init__(self, dims, octs, balance_corner=1):
+        L = _flib()
+        a = np.ascontiguousarray(octs, dtype=OCT_DT)
+        n = L.orc_balance(dims[0], dims[1], dims[2], _p(a), len(a), int(balance_corner))
+        self.leaves = np.zeros(n, dtype=OCT_DT)
+        L.orc_get_leaves(_p(self.leaves))
+        self.dims = dims
+
+    def is_refined(self, o):
+        a = np.ascontiguousarray(np.array([o], dtype=OCT_DT))
+        return bool(_flib().orc_is_refined(_p(a)))
+
+    def create_nodes(self):
+        L = _flib()
+        self.nindep = L.orc_create_nodes()
+        nn, ni, nd, ndc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        L.orc_node_sizes(C.byref(nn), C.byref(ni), C.byref(nd), C.byref(ndc))
+        self.nnodes, self.ndep = nn.value, nd.value
+        self.conn = np.zeros((len(self.leaves), 8), np.intc)
+        self.dep_ptr = np.zeros(self.ndep + 1, np.intc)
+        self.dep_conn = np.zeros(max(ndc.value, 1), np.intc)
+        self.dep_w = np.zeros(max(ndc.value, 1))
+        self.gxyz = np.zeros((self.nnodes, 3), np.int64)
+        L.orc_get_nodes(_p(self.conn), _p(self.dep_ptr), _p(self.dep_conn), _p(self.dep_w),
+                        _p(self.gxyz))
+        self.dep_conn = self.dep_conn[:ndc.value]
+        self.dep_w = self.dep_w[:ndc.value]
+        return self
+
+    def nodes_on_box_face(self, axis, side):
+        out = np.zeros(self.nnodes, np.intc)
+        n = _flib().orc_nodes_on_box_face(axis, side, _p(out))
+        return out[:n]
+
+    def save(self, slot):
+        _flib().orc_save_mesh(slot)
+
+
+def interpolation(fine_dims, fine_octs, coarse_octs, balance_corner=1):
+    """(rowp, cols, w) of the prolongation coarse -> fine."""
+    fm = OracleMesh(fine_dims, fine_octs, balance_corner).create_nodes()
+    fm.save(0)
+    cm = OracleMesh(fine_dims, coarse_octs, balance_corner).create_nodes()
+    cm.save(1)
+    L = _flib()
+    nnz = L.orc_interp()
+    rowp = np.zeros(fm.nindep + 1, np.intc)
+    cols = np.zeros(max(nnz, 1), np.intc)
+    w = np.zeros(max(nnz, 1))
+    L.orc_get_interp(_p(rowp), _p(cols), _p(w))
+    return fm, cm, (rowp, cols[:nnz], w[:nnz])
+
+
+def ref_octant_lib():
+    """The reference's own octant sort built in oracle/_ref (None if absent)."""
+    path = os.path.join(HERE, "_ref", "libtmr_octant_ref.so")
+    if not os.path.exists(path):
+        return None
+    R = C.CDLL(path)
+    R.ref_sort_octants.restype = C.c_int
+    R.ref_sort_octants.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    R.ref_child_id.restype = C.c_int
+    R.ref_child_id.argtypes = [C.c_void_p]
+    R.ref_compare.restype = C.c_int
+    R.ref_compare.argtypes = [C.c_void_p, C.c_void_p]
+    R.ref_sizeof_octant.restype = C.c_int
+    return R

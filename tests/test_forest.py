@@ -1,0 +1,127 @@
+"""The native forest (tmr_amd/csrc/forest.cpp, host code, no GPU) against
+the geometric oracle: octants after balance, element connectivity, hanging
+node maps and weights, interpolation CSR, named nodes -- bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tmr_amd import TMR, _lib
+
+KEY = ["block", "x", "y", "z", "level"]
+
+
+def build(dims, level, seed=None, maxlev=5, corner=1, frac=1.0):
+    L = (float(dims[0]), float(dims[1]), float(dims[2]))
+    f = TMR.createBoxForest(*dims, L=L, face_names={"x-": "fixed", "x+": "load", "z+": "top"})
+    f.createTrees(level)
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        n = len(f.getOctants())
+        r = (rng.integers(0, 3, n) * (rng.random(n) < frac)).astype(np.intc)
+        f.refine(r, 0, maxlev)
+    pre = f.getOctants()
+    f.balance(corner)
+    f.createNodes()
+    return f, pre
+
+
+@pytest.mark.parametrize("dims,level", [((1, 1, 1), 3), ((2, 1, 1), 3), ((2, 2, 2), 2), ((3, 1, 2), 2)])
+def test_uniform_matches_oracle(dims, level):
+    f, pre = build(dims, level)
+    m = po.OracleMesh(dims, pre).create_nodes()
+    assert np.array_equal(m.leaves[KEY], f.getOctants()[KEY])
+    assert np.array_equal(m.conn, f.getMeshConn())
+    assert f.getNodeRange()[1] == m.nindep
+    assert np.all(f.getOctants()["tag"] == np.arange(len(m.leaves)))
+
+
+@pytest.mark.parametrize("seed,corner", [(0, 1), (1, 1), (2, 1), (3, 0), (4, 0), (5, 1)])
+def test_adaptive_maps_match_oracle(seed, corner):
+    dims = (2, 2, 2)
+    f, pre = build(dims, 1, seed=seed, maxlev=5, corner=corner)
+    m = po.OracleMesh(dims, pre, corner).create_nodes()
+    assert np.array_equal(m.leaves[KEY], f.getOctants()[KEY])
+    assert np.array_equal(m.conn, f.getMeshConn())
+    ptr, dc, dw = f.getDepNodeConn()
+    assert m.ndep == len(ptr) - 1 and m.ndep > 0
+    assert np.array_equal(m.dep_ptr, ptr)
+    assert np.array_equal(m.dep_conn, dc)
+    assert np.array_equal(m.dep_w, dw)
+
+
+@pytest.mark.parametrize("seed", [0, 7])
+def test_interpolation_matches_oracle(seed):
+    dims = (2, 2, 2)
+    f, pre = build(dims, 1, seed=seed, maxlev=5)
+    c = f.coarsen()
+    cpre = c.getOctants()
+    c.balance(1)
+    c.createNodes()
+    rp, cl, w = f.createInterpolation(c)
+    fm, cm, (orp, ocl, ow) = po.interpolation(dims, pre, cpre)
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(cl, ocl)
+    assert np.array_equal(w, ow)
+
+
+def test_named_nodes_match_oracle():
+    dims = (2, 2, 2)
+    f, pre = build(dims, 1, seed=3, maxlev=4)
+    m = po.OracleMesh(dims, pre).create_nodes()
+    assert np.array_equal(f.getNodesWithName("fixed"), m.nodes_on_box_face(0, 0))
+    assert np.array_equal(f.getNodesWithName("load"), m.nodes_on_box_face(0, 1))
+    assert np.array_equal(f.getNodesWithName("top"), m.nodes_on_box_face(2, 1))
+    # hanging nodes on a named face are reported with their negative numbers
+    # (TMROctForest::getNodesWithName copies conn entries, :6098-6167)
+    assert np.any(f.getNodesWithName("fixed") < 0) == np.any(m.nodes_on_box_face(0, 0) < 0)
+
+
+def test_octs_with_name_face_index_and_tag():
+    f, _ = build((2, 1, 1), 2)
+    octs = f.getOctsWithName("load")
+    assert len(octs) == 4 * 4  # one block face at level 2
+    assert np.all(octs["info"] == 1)
+    allo = f.getOctants()
+    assert np.all(allo["x"][octs["tag"]] + (1 << 28) == (1 << 30))
+    assert np.all(allo["block"][octs["tag"]] == 1)
+
+
+def test_points_are_lattice_coordinates():
+    f, _ = build((2, 1, 1), 2)
+    X = f.getPoints()
+    nums = f.getNodeNumbers()
+    assert len(X) == len(nums)
+    conn = f.getMeshConn()
+    ext = f.getElementExtents()
+    assert np.allclose(ext, 0.25)
+    # every element's corner 7 minus corner 0 is its diagonal
+    nd = len(f.getDepNodeConn()[0]) - 1
+    idx = lambda n: n + nd if n >= 0 else nd - 1 - (-n - 1)
+    for e in range(0, len(conn), 7):
+        d = X[idx(conn[e, 7])] - X[idx(conn[e, 0])]
+        assert np.allclose(d, ext[e])
+
+
+def test_coarsen_takes_parents_of_zero_children():
+    f, _ = build((1, 1, 1), 3)
+    c = f.coarsen()
+    o = c.getOctants()
+    assert len(o) == 64 and np.all(o["level"] == 2)
+
+
+def test_errors_are_reported():
+    f = TMR.OctForest()
+    with pytest.raises(_lib.TmrxError):
+        f.createTrees(2)  # no connectivity
+    g = TMR.createBoxForest(1, 1, 1)
+    g.createTrees(1)
+    with pytest.raises(ValueError):
+        g.refine(np.zeros(3, np.intc))
+    # an unbalanced forest cannot number its nodes
+    r = np.zeros(8, np.intc)
+    r[0] = 3
+    g.refine(r, 0, 6)
+    with pytest.raises(_lib.TmrxError):
+        g.createNodes()
+    with pytest.raises(ValueError):
+        g.createInterpolation(g)

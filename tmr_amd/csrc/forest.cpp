@@ -1028,6 +1028,16 @@ int Forest::create_nodes() {
     last_error = "create_nodes: only mesh order 2 (trilinear hex8) is supported";
     return -1;
   }
+  {
+    // the leaves must tile the blocks exactly (refine() leaves holes and
+    // coarsen() overlaps until balance() completes the forest)
+    u128 vol = 0;
+    for (const Oct &o : octs_) vol += (u128)1 << (3 * (MAX_LEVEL - o.level));
+    if (vol != (u128)bd_.num_blocks << (3 * MAX_LEVEL)) {
+      last_error = "create_nodes: the octants do not tile the blocks; call balance() first";
+      return -1;
+    }
+  }
   compute_dep_faces_and_edges();
   const size_t ne = octs_.size();
 

@@ -120,7 +120,10 @@ void launch_dot1(int64_t ndof, const double *a, const double *b, double *partial
                  hipStream_t st);
 void launch_pcg_update_p(int64_t ndof, double *p, const double *z, const double *sc,
                          hipStream_t st);
-// finalize: mode 0 rz, 1 pq->alpha, 2 rr, 3 (zr, zq)->beta, 4 bb
+void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
+                      double *partials, hipStream_t st);
+// finalize: mode 0 rz, 1 pq->alpha, 2 rr, 3 (zr, zq)->beta, 4 bb,
+// 5 (|y|^2, |x|^2) of the power iteration
 void launch_finalize(int nparts, const double *partials, int mode, double *sc, hipStream_t st);
 
 // per-element compliance density and its design derivative (element mode)

@@ -7,6 +7,14 @@ namespace tmrx {
 
 #define TMRX_LAUNCH_CHECK() (void)hipGetLastError()
 
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks a
+// contiguous range of logical blocks (neighbouring Morton batches share nodes,
+// which then hit the same L2).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b & 7, i = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // ---------------------------------------------------------------------------
 // element pass: stage batch nodes in LDS, y_e = s_e K0 x_e, colored LDS
 // accumulation, one partial per slot.
@@ -45,7 +53,7 @@ __global__ __launch_bounds__(WG_ELEMS) void k_elem_apply(
     const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
     T *__restrict__ partial) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int wg = blockIdx.x;
+  const int wg = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int nb = wg_node_ptr[wg];
   const int nn = wg_node_ptr[wg + 1] - nb;
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_assemble(
     T *__restrict__ y, T *__restrict__ x, T *__restrict__ d, const T *__restrict__ dinv,
     double ca, double cc, double *__restrict__ partials) {
   __shared__ double red[NODE_BLOCK / 64];
-  const int i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * NODE_BLOCK + threadIdx.x;
   double dot = 0.0;
   if (i < n) {
     T a0 = T(0), a1 = T(0), a2 = T(0);
@@ -263,10 +271,12 @@ void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, i
 // ---------------------------------------------------------------------------
 // grid transfer: prolongation (add into fine), restriction (coarse = P^T r)
 template <class T>
-__global__ void k_prolong_add(int n, const int *__restrict__ ptr, const int *__restrict__ col,
-                              const float *__restrict__ w, const uint8_t *__restrict__ mask,
-                              const T *__restrict__ xc, T *__restrict__ xf) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restrict__ ptr,
+                                                     const int *__restrict__ col,
+                                                     const float *__restrict__ w,
+                                                     const uint8_t *__restrict__ mask,
+                                                     const T *__restrict__ xc, T *__restrict__ xf) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   for (int j = ptr[i]; j < ptr[i + 1]; j++) {
@@ -282,24 +292,38 @@ __global__ void k_prolong_add(int n, const int *__restrict__ ptr, const int *__r
   if (!(m & 4)) xf[3 * i + 2] += a2;
 }
 
+// 4 lanes per coarse row: rows gather ~27 scattered fine nodes, so spread
+// the loads over lanes and reduce with shuffles (fixed order).
 template <class T>
-__global__ void k_restrict(int n, const int *__restrict__ ptr, const int *__restrict__ col,
-                           const float *__restrict__ w, const uint8_t *__restrict__ mask,
-                           const T *__restrict__ rf, T *__restrict__ rc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__ ptr,
+                                                  const int *__restrict__ col,
+                                                  const float *__restrict__ w,
+                                                  const uint8_t *__restrict__ mask,
+                                                  const T *__restrict__ rf, T *__restrict__ rc) {
+  const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int i = g >> 2, sub = g & 3;
   T a0 = T(0), a1 = T(0), a2 = T(0);
-  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
-    const int c = col[j];
-    const T ww = (T)w[j];
-    a0 += ww * rf[3 * c];
-    a1 += ww * rf[3 * c + 1];
-    a2 += ww * rf[3 * c + 2];
+  if (i < n) {
+    for (int j = ptr[i] + sub; j < ptr[i + 1]; j += 4) {
+      const int c = col[j];
+      const T ww = (T)w[j];
+      a0 += ww * rf[3 * c];
+      a1 += ww * rf[3 * c + 1];
+      a2 += ww * rf[3 * c + 2];
+    }
   }
-  const uint8_t m = mask[i];
-  rc[3 * i] = (m & 1) ? T(0) : a0;
-  rc[3 * i + 1] = (m & 2) ? T(0) : a1;
-  rc[3 * i + 2] = (m & 4) ? T(0) : a2;
+  a0 += __shfl_xor(a0, 1, 64);
+  a1 += __shfl_xor(a1, 1, 64);
+  a2 += __shfl_xor(a2, 1, 64);
+  a0 += __shfl_xor(a0, 2, 64);
+  a1 += __shfl_xor(a1, 2, 64);
+  a2 += __shfl_xor(a2, 2, 64);
+  if (i < n && sub == 0) {
+    const uint8_t m = mask[i];
+    rc[3 * i] = (m & 1) ? T(0) : a0;
+    rc[3 * i + 1] = (m & 2) ? T(0) : a1;
+    rc[3 * i + 2] = (m & 4) ? T(0) : a2;
+  }
 }
 
 template <class T>
@@ -315,8 +339,9 @@ template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T *rc,
                      hipStream_t st) {
   if (R.n_rows == 0) return;
-  hipLaunchKernelGGL(k_restrict<T>, dim3((R.n_rows + 255) / 256), dim3(256), 0, st, R.n_rows,
-                     R.ptr, R.col, R.w, mask_c, rf, rc);
+  const int64_t threads = 4 * (int64_t)R.n_rows;
+  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
+                     R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -538,11 +563,11 @@ void launch_pcg_update_p(int64_t ndof, double *p, const double *z, const double 
 __global__ __launch_bounds__(256) void k_finalize(int np, const double *__restrict__ part,
                                                   int mode, double *__restrict__ sc) {
   __shared__ double red[4];
-  const int stride = (mode == 3) ? 2 : 1;
+  const int stride = (mode == 3 || mode == 5) ? 2 : 1;
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < np; i += 256) {
     a += part[stride * i];
-    if (mode == 3) b += part[2 * i + 1];
+    if (mode == 3 || mode == 5) b += part[2 * i + 1];
   }
   const double sa = block_sum(a, red);
   __syncthreads();
@@ -562,6 +587,10 @@ __global__ __launch_bounds__(256) void k_finalize(int np, const double *__restri
       sc[SC_RZ] = sa;
       break;
     case 4: sc[SC_BB] = sa; break;
+    case 5:  // power iteration: |Dinv A x|^2 and |x|^2
+      sc[SC_ZQ] = sa;
+      sc[SC_RZNEW] = sb;
+      break;
   }
 }
 
@@ -630,6 +659,37 @@ void launch_compliance_grad(int n, const double *rho, const int16_t *lev, double
   if (n == 0) return;
   hipLaunchKernelGGL(k_compliance_grad, dim3((n + 255) / 256), dim3(256), 0, st, n, rho, lev, q,
                      ptype, energy, g);
+  TMRX_LAUNCH_CHECK();
+}
+
+// power iteration step for lambda_max(Dinv A): y <- Dinv y, partials of
+// y.y and x.x
+__global__ __launch_bounds__(RED_BLOCK) void k_dinv_dots(int64_t n, const double *__restrict__ dinv,
+                                                         double *__restrict__ y,
+                                                         const double *__restrict__ x,
+                                                         double *__restrict__ partials) {
+  __shared__ double red[RED_BLOCK / 64];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RED_BLOCK) {
+    const double v = dinv[i] * y[i];
+    y[i] = v;
+    a += v * v;
+    b += x[i] * x[i];
+  }
+  const double sa = block_sum(a, red);
+  __syncthreads();
+  const double sb = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sa;
+    partials[2 * blockIdx.x + 1] = sb;
+  }
+}
+
+void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
+                      double *partials, hipStream_t st) {
+  hipLaunchKernelGGL(k_dinv_dots, dim3(reduce_blocks(n)), dim3(RED_BLOCK), 0, st, n, dinv, y, x,
+                     partials);
   TMRX_LAUNCH_CHECK();
 }
 

@@ -421,7 +421,9 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
 }
 
 int Problem::estimate_eigs(int l, const SolveOpts &o) {
-  // power iteration on Dinv A (FP64), lambda_max = |Dinv A v| / |v|
+  // power iteration on Dinv A (FP64) from a fixed pseudo-random start,
+  // lambda_max ~ |Dinv A x_k| / |x_k| after eig_iters steps (x_{k+1} =
+  // Dinv A x_k, no renormalisation: the growth is bounded by lambda^k)
   Level &L = levels_[l];
   auto &v = L.v64;
   const int64_t n = L.ndof;
@@ -433,30 +435,21 @@ int Problem::estimate_eigs(int l, const SolveOpts &o) {
     s ^= s << 17;
     h[i] = ((L.bcmask[i / 3] >> (i % 3)) & 1) ? 0.0 : 0.5 + (double)(s >> 11) * 0x1.0p-53;
   }
-  HIPCHK(hipMemcpy(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
-  double lam = 1.0;
-  std::vector<double> dinvh(n);
-  HIPCHK(hipMemcpy(dinvh.data(), v.dinv.p, n * sizeof(double), hipMemcpyDeviceToHost));
-  for (int it = 0; it < std::max(1, o.eig_iters); it++) {
-    AsmArgs a;
-    a.y = v.r.p;
-    apply_level<double>(l, v.x.p, ASM_STORE, a);
-    HIPCHK(hipMemcpyAsync(h.data(), v.r.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipStreamSynchronize(st_));
-    std::vector<double> xv(n);
-    HIPCHK(hipMemcpy(xv.data(), v.x.p, n * sizeof(double), hipMemcpyDeviceToHost));
-    double nw = 0.0, nv = 0.0;
-    for (int64_t i = 0; i < n; i++) {
-      h[i] *= dinvh[i];
-      nw += h[i] * h[i];
-      nv += xv[i] * xv[i];
-    }
-    lam = std::sqrt(nw / nv);
-    const double sc = 1.0 / std::sqrt(nw);
-    for (int64_t i = 0; i < n; i++) h[i] *= sc;
-    HIPCHK(hipMemcpy(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice, st_));
+  double *a = v.x.p, *b = v.r.p;
+  const int iters = std::max(1, o.eig_iters);
+  for (int it = 0; it < iters; it++) {
+    AsmArgs g;
+    g.y = b;
+    apply_level<double>(l, a, ASM_STORE, g);
+    launch_dinv_dots(n, v.dinv.p, b, a, parts_.p, st_);
+    std::swap(a, b);
   }
-  L.lam_max = lam;
+  launch_finalize(reduce_blocks(n), parts_.p, 5, sc_.p, st_);
+  double hsc[SC_N];
+  HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  L.lam_max = std::sqrt(hsc[SC_ZQ] / hsc[SC_RZNEW]);
   return 0;
 }
 
