@@ -39,6 +39,44 @@ struct ElemBatches {
   const uint8_t *bcmask = nullptr;   // [n_nodes], bit d set = dof d clamped
 };
 
+// Owner tiles: tile t owns the contiguous independent nodes
+// [node0[t], node0[t+1]) and holds every element touching them (or touching a
+// hanging node one of them supports).  LDS slots: owned nodes, then the
+// hanging nodes whose sums the tile needs ("accumulated"), then halo nodes.
+constexpr int TILE_NODES = 512;
+constexpr int TILE_THREADS = 512;
+
+struct TileElem {       // 24 bytes
+  uint16_t lc[8];       // LDS slot of each corner
+  int32_t elem;         // global element index (stiffness scale)
+  uint8_t mask;         // corners whose slot is accumulated
+  uint8_t color;        // conflict-free accumulation phase
+  uint8_t cls;          // stiffness class
+  uint8_t pad;
+};
+static_assert(sizeof(TileElem) == 24, "tile element record is 24 bytes");
+
+struct Tiles {
+  int n_tiles = 0;
+  bool single_class = true;
+  int max_slots = 0, max_acc = 0;
+  int ncolors = 0;
+  const int *node0 = nullptr;     // [n_tiles + 1]
+  const int *ext_ptr = nullptr;   // [n_tiles + 1] non-owned slots
+  const int *ext = nullptr;       // global node (>= 0) or -(hanging + 1)
+  const int *nacc_dep = nullptr;  // [n_tiles] accumulated hanging slots
+  const int *el_ptr = nullptr;    // [n_tiles + 1]
+  const TileElem *el = nullptr;
+  const int *comb_ptr = nullptr;  // [n_nodes + 1] owned node <- hanging slots
+  const int *comb_slot = nullptr; // local slot in the owning tile
+  const float *comb_w = nullptr;
+  const int *dep_ptr = nullptr;
+  const int *dep_conn = nullptr;
+  const float *dep_w = nullptr;
+  const uint8_t *bcmask = nullptr;
+};
+
+
 struct AsmCSR {
   int n_nodes = 0;
   const int *ptr = nullptr;   // [n_nodes + 1]
@@ -66,12 +104,17 @@ struct AsmArgs {
   const void *b = nullptr;
   void *y = nullptr;
   void *x = nullptr;       // input of the product (read) or iterate (CHEB)
+  void *x_out = nullptr;   // CHEB iterate output of the tile kernel
   void *d = nullptr;       // Chebyshev direction
   const void *dinv = nullptr;
   double ca = 0.0, cc = 0.0;
   double *partials = nullptr;
 };
 
+// single-pass apply with fused epilogue (modes as AsmMode, except ASM_DIAG)
+template <class T>
+void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
+                       const AsmArgs &args, hipStream_t st);
 template <class T>
 void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, const T *x,
                        T *partial, hipStream_t st);

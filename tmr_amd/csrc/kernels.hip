@@ -693,6 +693,172 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
   TMRX_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// owner-tile apply: one pass, final values of the owned nodes, fused epilogue
+template <class T, int MODE, bool SINGLE_CLASS>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
+    const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
+    const int *__restrict__ nacc_dep, const int *__restrict__ el_ptr,
+    const uint2 *__restrict__ el, int ncolors, const int *__restrict__ comb_ptr,
+    const int *__restrict__ comb_slot, const float *__restrict__ comb_w,
+    const int *__restrict__ dep_ptr, const int *__restrict__ dep_conn,
+    const float *__restrict__ dep_w, const uint8_t *__restrict__ bcmask,
+    const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
+    const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
+    const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[TILE_THREADS / 64];
+  const int t = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int n0 = node0[t], nown = node0[t + 1] - n0;
+  const int e0 = ext_ptr[t], next = ext_ptr[t + 1] - e0;
+  const int nslot = nown + next;
+  const int nacc = nown + nacc_dep[t];
+  T *xs = reinterpret_cast<T *>(smem_raw);
+  T *ys = xs + 3 * nslot;
+  for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
+    const int node = n0 + i / 3, c = i - 3 * (i / 3);
+    xs[i] = ((bcmask[node] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
+  }
+  for (int i = tid; i < next; i += TILE_THREADS) {
+    T v[3];
+    load_node<T>(ext[e0 + i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
+    xs[3 * (nown + i)] = v[0];
+    xs[3 * (nown + i) + 1] = v[1];
+    xs[3 * (nown + i) + 2] = v[2];
+  }
+  for (int i = tid; i < 3 * nacc; i += TILE_THREADS) ys[i] = T(0);
+  __syncthreads();
+  const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
+  for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
+    const int k = r0 + tid;
+    T ye[24];
+    int lc[8];
+    int mask = 0, col = -1;
+    if (k < nel) {
+      const uint2 w0 = el[3 * (size_t)(q0 + k)];
+      const uint2 w1 = el[3 * (size_t)(q0 + k) + 1];
+      const uint2 w2 = el[3 * (size_t)(q0 + k) + 2];
+      lc[0] = w0.x & 0xffff; lc[1] = w0.x >> 16;
+      lc[2] = w0.y & 0xffff; lc[3] = w0.y >> 16;
+      lc[4] = w1.x & 0xffff; lc[5] = w1.x >> 16;
+      lc[6] = w1.y & 0xffff; lc[7] = w1.y >> 16;
+      const int elem = (int)w2.x;
+      mask = w2.y & 0xff;
+      col = (w2.y >> 8) & 0xff;
+      const int cls = SINGLE_CLASS ? 0 : (int)((w2.y >> 16) & 0xff);
+      const T *__restrict__ K = K0 + 576 * cls;
+      const T sc = s_elem[elem];
+      T xe[24];
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        xe[3 * a] = xs[3 * lc[a]];
+        xe[3 * a + 1] = xs[3 * lc[a] + 1];
+        xe[3 * a + 2] = xs[3 * lc[a] + 2];
+      }
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        if (mask & (1 << a)) {
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < 24; j++) acc += K[(3 * a + r) * 24 + j] * xe[j];
+            ye[3 * a + r] = sc * acc;
+          }
+        }
+      }
+    }
+    for (int c = 0; c < ncolors; c++) {
+      if (col == c) {
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          if (mask & (1 << a)) {
+            ys[3 * lc[a]] += ye[3 * a];
+            ys[3 * lc[a] + 1] += ye[3 * a + 1];
+            ys[3 * lc[a] + 2] += ye[3 * a + 2];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double dot = 0.0;
+  for (int i = tid; i < nown; i += TILE_THREADS) {
+    const int node = n0 + i;
+    T a[3] = {ys[3 * i], ys[3 * i + 1], ys[3 * i + 2]};
+    for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
+      const int sl = comb_slot[j];
+      const T w = (T)comb_w[j];
+      a[0] += w * ys[3 * sl];
+      a[1] += w * ys[3 * sl + 1];
+      a[2] += w * ys[3 * sl + 2];
+    }
+    const uint8_t m = bcmask[node];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)node + c;
+      const bool bc = (m >> c) & 1;
+      if (MODE == ASM_STORE) {
+        y[k] = bc ? x[k] : a[c];
+      } else if (MODE == ASM_DOT) {
+        const T xv = bc ? x[k] : xs[3 * i + c];
+        const T v = bc ? xv : a[c];
+        y[k] = v;
+        dot += (double)xv * (double)v;
+      } else if (MODE == ASM_RESID) {
+        y[k] = bc ? T(0) : b[k] - a[c];
+      } else if (MODE == ASM_CHEB) {
+        if (bc) {
+          d[k] = T(0);
+          x_out[k] = T(0);
+        } else {
+          const T r = b[k] - a[c];
+          const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
+          d[k] = dn;
+          x_out[k] = xs[3 * i + c] + dn;
+        }
+      }
+    }
+  }
+  if (MODE == ASM_DOT) {
+    const double sum = block_sum(dot, red);
+    if (threadIdx.x == 0) partials[t] = sum;
+  }
+}
+
+template <class T>
+void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
+                       const AsmArgs &g, hipStream_t st) {
+  if (tl.n_tiles == 0) return;
+  const size_t sh = 3 * (size_t)(tl.max_slots + tl.max_acc) * sizeof(T);
+  const T *b = (const T *)g.b;
+  T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
+  const T *dinv = (const T *)g.dinv;
+  const bool single = tl.single_class;
+#define TMRX_TILE(M, S)                                                                         \
+  hipLaunchKernelGGL((k_tile_apply<T, M, S>), dim3(tl.n_tiles), dim3(TILE_THREADS), sh, st,     \
+                     tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                      \
+                     reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,           \
+                     tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0, \
+                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials)
+#define TMRX_TILE2(M) \
+  if (single) {       \
+    TMRX_TILE(M, true); \
+  } else {            \
+    TMRX_TILE(M, false); \
+  }
+  switch (mode) {
+    case ASM_STORE: TMRX_TILE2(ASM_STORE); break;
+    case ASM_RESID: TMRX_TILE2(ASM_RESID); break;
+    case ASM_CHEB: TMRX_TILE2(ASM_CHEB); break;
+    case ASM_DOT: TMRX_TILE2(ASM_DOT); break;
+  }
+#undef TMRX_TILE2
+#undef TMRX_TILE
+  TMRX_LAUNCH_CHECK();
+}
+
 // explicit instantiations
 #define TMRX_INST(T)                                                                            \
   template void launch_elem_apply<T>(const ElemBatches &, const T *, const T *, const T *, T *, \
@@ -708,7 +874,9 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
   template void launch_cheb_first<T>(int, const T *, const T *, double, T *, T *,               \
                                      hipStream_t);                                              \
   template void launch_dense_solve<T>(int, const double *, const T *, T *, hipStream_t);        \
-  template void launch_fill<T>(int64_t, T, T *, hipStream_t);
+  template void launch_fill<T>(int64_t, T, T *, hipStream_t);                                 \
+  template void launch_tile_apply<T>(const Tiles &, const T *, const T *, const T *, int,       \
+                                     const AsmArgs &, hipStream_t);
 TMRX_INST(float)
 TMRX_INST(double)
 template void launch_convert<float, double>(int64_t, const float *, double *, hipStream_t);

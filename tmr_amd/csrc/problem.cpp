@@ -131,12 +131,14 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     L.cls.resize(octs.size());
     for (size_t e = 0; e < octs.size(); e++) L.cls[e] = block_cls[octs[e].block];
     if (build_level(l, nbcs, names, dof_masks)) return -1;
+    if (build_tiles(l)) return -1;
   }
   for (int l = 0; l + 1 < nlevels; l++)
     if (build_transfer(l)) return -1;
   const int64_t n = levels_[0].ndof;
   if (f_.alloc(n) || u_.alloc(n) || r_.alloc(n) || p_.alloc(n) || q_.alloc(n) || z_.alloc(n) ||
-      parts_.alloc((size_t)std::max(2 * 2048, assemble_num_blocks(levels_[0].n_nodes)) + 16) ||
+      parts_.alloc((size_t)std::max(std::max(2 * 2048, assemble_num_blocks(levels_[0].n_nodes)),
+                                    levels_[0].tiles.n_tiles) + 16) ||
       sc_.alloc(SC_N) || rho_.alloc(levels_[0].n_elem) ||
       energy_.alloc(levels_[0].n_elem)) {
     last_error = "problem: device allocation failed";
@@ -285,6 +287,176 @@ int Problem::build_level(int l, int nbcs, const char **names, const int *masks) 
   return 0;
 }
 
+// Owner tiles: contiguous ranges of TILE_NODES independent nodes; every
+// element touching an owned node, or a hanging node with an owned parent, is
+// staged with its corners in LDS slots [owned | accumulated hanging | halo].
+int Problem::build_tiles(int l) {
+  Level &L = levels_[l];
+  Forest *f = L.forest;
+  const int n = L.n_nodes, nd = L.n_dep, ne = L.n_elem;
+  const auto &dptr = f->dep_ptr();
+  const auto &dconn = f->dep_conn();
+  const auto &dw = f->dep_weights();
+  // incidence: ext node (g >= 0 -> g, dep d -> n + d) -> elements
+  std::vector<int> iptr(n + nd + 1, 0), iel;
+  auto xid = [&](int g) { return g >= 0 ? g : n + (-g - 1); };
+  for (int e = 0; e < ne; e++)
+    for (int a = 0; a < 8; a++) iptr[xid(L.conn[8 * e + a]) + 1]++;
+  for (int i = 0; i < n + nd; i++) iptr[i + 1] += iptr[i];
+  iel.resize(iptr[n + nd]);
+  {
+    std::vector<int> fill(iptr.begin(), iptr.end() - 1);
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < 8; a++) iel[fill[xid(L.conn[8 * e + a])]++] = e;
+  }
+  // parent -> hanging nodes
+  std::vector<int> pptr(n + 1, 0), pdep;
+  std::vector<float> pw;
+  for (int j = 0; j < (nd ? dptr[nd] : 0); j++) pptr[dconn[j] + 1]++;
+  for (int i = 0; i < n; i++) pptr[i + 1] += pptr[i];
+  pdep.resize(pptr[n]);
+  pw.resize(pptr[n]);
+  {
+    std::vector<int> fill(pptr.begin(), pptr.end() - 1);
+    for (int d = 0; d < nd; d++)
+      for (int j = dptr[d]; j < dptr[d + 1]; j++) {
+        pdep[fill[dconn[j]]] = d;
+        pw[fill[dconn[j]]++] = (float)dw[j];
+      }
+  }
+  const int ntiles = (n + TILE_NODES - 1) / TILE_NODES;
+  std::vector<int> node0(ntiles + 1), ext_ptr{0}, ext, nacc(ntiles), el_ptr{0};
+  std::vector<TileElem> els;
+  std::vector<int> comb_ptr(n + 1, 0), comb_slot;
+  std::vector<float> comb_w;
+  std::vector<int> stamp_n(n + nd, -1), slot(n + nd, 0), stamp_e(ne, -1);
+  std::vector<uint32_t> col;
+  int max_slots = 0, max_acc = 0, ncolors = 0;
+  bool single = true;
+  for (int e = 0; e < ne; e++)
+    if (L.cls[e] != L.cls[0]) single = false;
+  std::vector<int> tel, rdeps;
+  for (int t = 0; t < ntiles; t++) {
+    const int n0 = t * TILE_NODES, n1 = std::min(n, n0 + TILE_NODES), nown = n1 - n0;
+    node0[t] = n0;
+    rdeps.clear();
+    tel.clear();
+    for (int p = n0; p < n1; p++) {
+      stamp_n[p] = t;
+      slot[p] = p - n0;
+      for (int j = pptr[p]; j < pptr[p + 1]; j++) {
+        const int x = n + pdep[j];
+        if (stamp_n[x] != t) {
+          stamp_n[x] = t;
+          slot[x] = nown + (int)rdeps.size();
+          rdeps.push_back(pdep[j]);
+        }
+      }
+    }
+    const int na = nown + (int)rdeps.size();
+    auto add_elems = [&](int x) {
+      for (int j = iptr[x]; j < iptr[x + 1]; j++)
+        if (stamp_e[iel[j]] != t) {
+          stamp_e[iel[j]] = t;
+          tel.push_back(iel[j]);
+        }
+    };
+    for (int p = n0; p < n1; p++) add_elems(p);
+    for (int d : rdeps) add_elems(n + d);
+    std::sort(tel.begin(), tel.end());
+    for (int d : rdeps) ext.push_back(-(d + 1));
+    int ns = na;
+    const size_t el0 = els.size();
+    for (int e : tel) {
+      TileElem te{};
+      te.elem = e;
+      te.cls = (uint8_t)L.cls[e];
+      for (int a = 0; a < 8; a++) {
+        const int g = L.conn[8 * e + a];
+        const int x = xid(g);
+        if (stamp_n[x] != t) {
+          stamp_n[x] = t;
+          slot[x] = ns++;
+          ext.push_back(g);
+        }
+        if (slot[x] >= 65536) {
+          last_error = "problem: tile slot index overflow";
+          return -1;
+        }
+        te.lc[a] = (uint16_t)slot[x];
+        if (slot[x] < na) te.mask |= (uint8_t)(1u << a);
+      }
+      els.push_back(te);
+    }
+    // group equal corner masks (fewer divergent row blocks per wave), then
+    // colour greedily over the accumulated slots
+    std::stable_sort(els.begin() + el0, els.end(), [](const TileElem &a, const TileElem &b) {
+      return a.mask > b.mask;
+    });
+    col.assign(na, 0u);
+    for (size_t k = el0; k < els.size(); k++) {
+      uint32_t used = 0;
+      for (int a = 0; a < 8; a++)
+        if (els[k].mask & (1u << a)) used |= col[els[k].lc[a]];
+      int c = 0;
+      while (used & (1u << c)) c++;
+      if (c >= MAX_COLORS) {
+        last_error = "problem: tile needs too many colours";
+        return -1;
+      }
+      els[k].color = (uint8_t)c;
+      ncolors = std::max(ncolors, c + 1);
+      for (int a = 0; a < 8; a++)
+        if (els[k].mask & (1u << a)) col[els[k].lc[a]] |= 1u << c;
+    }
+    nacc[t] = (int)rdeps.size();
+    ext_ptr.push_back((int)ext.size());
+    el_ptr.push_back((int)els.size());
+    max_slots = std::max(max_slots, ns);
+    max_acc = std::max(max_acc, na);
+    for (int p = n0; p < n1; p++) {
+      for (int j = pptr[p]; j < pptr[p + 1]; j++) {
+        comb_slot.push_back(slot[n + pdep[j]]);
+        comb_w.push_back(pw[j]);
+      }
+      comb_ptr[p + 1] = (int)comb_slot.size();
+    }
+  }
+  node0[ntiles] = n;
+  if (comb_slot.empty()) {
+    comb_slot.push_back(0);
+    comb_w.push_back(0.0f);
+  }
+  if (ext.empty()) ext.push_back(0);
+  if (L.t_node0.upload(node0) || L.t_ext_ptr.upload(ext_ptr) || L.t_ext.upload(ext) ||
+      L.t_nacc.upload(nacc) || L.t_el_ptr.upload(el_ptr) || L.t_el.upload(els) ||
+      L.t_comb_ptr.upload(comb_ptr) || L.t_comb_slot.upload(comb_slot) ||
+      L.t_comb_w.upload(comb_w)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  Tiles &T = L.tiles;
+  T.n_tiles = ntiles;
+  T.single_class = single;
+  T.max_slots = max_slots;
+  T.max_acc = max_acc;
+  T.ncolors = ncolors;
+  T.node0 = L.t_node0.p;
+  T.ext_ptr = L.t_ext_ptr.p;
+  T.ext = L.t_ext.p;
+  T.nacc_dep = L.t_nacc.p;
+  T.el_ptr = L.t_el_ptr.p;
+  T.el = L.t_el.p;
+  T.comb_ptr = L.t_comb_ptr.p;
+  T.comb_slot = L.t_comb_slot.p;
+  T.comb_w = L.t_comb_w.p;
+  T.dep_ptr = L.dep_ptr.p;
+  T.dep_conn = L.dep_conn.p;
+  T.dep_w = L.dep_w.p;
+  T.bcmask = L.d_bcmask.p;
+  return 0;
+}
+
 int Problem::build_transfer(int l) {
   Level &F = levels_[l];
   Level &Cc = levels_[l + 1];
@@ -355,20 +527,22 @@ int Problem::ensure_vecs(int l, int precision) {
   if (precision == 1) {
     auto &v = L.v32;
     if (v.ready) return 0;
-    if (v.b.alloc(n) || v.x.alloc(n) || v.r.alloc(n) || v.d.alloc(n) || v.dinv.alloc(n) ||
-        v.partial.alloc(ns))
+    if (v.b.alloc(n) || v.x.alloc(n) || v.x2.alloc(n) || v.r.alloc(n) || v.d.alloc(n) ||
+        v.dinv.alloc(n) || v.partial.alloc(ns))
       return -1;
     HIPCHK(hipMemset(v.d.p, 0, n * sizeof(float)));
     HIPCHK(hipMemset(v.x.p, 0, n * sizeof(float)));
+    HIPCHK(hipMemset(v.x2.p, 0, n * sizeof(float)));
     v.ready = true;
   } else {
     auto &v = L.v64;
     if (v.ready) return 0;
-    if (v.b.alloc(n) || v.x.alloc(n) || v.r.alloc(n) || v.d.alloc(n) || v.dinv.alloc(n) ||
-        v.partial.alloc(ns))
+    if (v.b.alloc(n) || v.x.alloc(n) || v.x2.alloc(n) || v.r.alloc(n) || v.d.alloc(n) ||
+        v.dinv.alloc(n) || v.partial.alloc(ns))
       return -1;
     HIPCHK(hipMemset(v.d.p, 0, n * sizeof(double)));
     HIPCHK(hipMemset(v.x.p, 0, n * sizeof(double)));
+    HIPCHK(hipMemset(v.x2.p, 0, n * sizeof(double)));
     v.ready = true;
   }
   return 0;
@@ -401,7 +575,6 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
 template <class T>
 void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
   Level &L = levels_[l];
-  LevelVecs<T> &v = vecs<T>(l);
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   if (timing_ && l == 0) {
     if (ev_used_ == ev_pool_.size()) {
@@ -412,12 +585,11 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
     if (ev_used_ < ev_pool_.size()) ev = &ev_pool_[ev_used_++];
     ev_prec_mask_ |= (sizeof(T) == 4) ? 1 : 2;
   }
-  if (ev) (void)hipEventRecord(ev->first, st_);
-  launch_elem_apply<T>(L.eb, K0<T>(), scale<T>(l), x, v.partial.p, st_);
-  if (ev) (void)hipEventRecord(ev->second, st_);
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
-  launch_assemble<T>(L.asmc, L.d_bcmask.p, v.partial.p, mode, g, st_);
+  if (ev) (void)hipEventRecord(ev->first, st_);
+  launch_tile_apply<T>(L.tiles, K0<T>(), scale<T>(l), x, mode, g, st_);
+  if (ev) (void)hipEventRecord(ev->second, st_);
 }
 
 int Problem::estimate_eigs(int l, const SolveOpts &o) {
@@ -570,39 +742,31 @@ int Problem::setup(const SolveOpts &o) {
 
 // Chebyshev smoother (degree k) on Dinv A with eigenvalue bounds
 // [lower, upper] * lambda_max (TMR's TACSChebyshevSmoother interval,
-// src/TMR_RefinementTools.cpp:46-47)
+// src/TMR_RefinementTools.cpp:46-47).  Step k reads x_in and writes x_out
+// (tiles read halo values of x, so the update cannot be in place).
 template <class T>
-void Problem::smooth(int l, const T *b, T *x, bool zero_guess, const SolveOpts &o) {
+void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o) {
   Level &L = levels_[l];
   LevelVecs<T> &v = vecs<T>(l);
   const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
   const double sigma = theta / delta;
-  double rho_prev = 1.0 / sigma;
-  for (int k = 0; k < o.cheb_degree; k++) {
-    double ca, cc;
-    if (k == 0) {
-      ca = 0.0;
-      cc = 1.0 / theta;
-    } else {
-      const double rho = 1.0 / (2.0 * sigma - rho_prev);
-      ca = rho * rho_prev;
-      cc = 2.0 * rho / delta;
-      rho_prev = rho;
-    }
-    if (k == 0 && zero_guess) {
-      launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, cc, v.d.p, x, st_);
-      continue;
-    }
-    AsmArgs a;
-    a.b = b;
-    a.x = x;
-    a.d = v.d.p;
-    a.dinv = v.dinv.p;
-    a.ca = ca;
-    a.cc = cc;
-    apply_level<T>(l, x, ASM_CHEB, a);
+  double rho_prev = 1.0 / sigma, ca = 0.0, cc = 1.0 / theta;
+  for (int j = 1; j <= k; j++) {
+    const double rho = 1.0 / (2.0 * sigma - rho_prev);
+    ca = rho * rho_prev;
+    cc = 2.0 * rho / delta;
+    rho_prev = rho;
   }
+  AsmArgs a;
+  a.b = b;
+  a.x = (void *)x_in;
+  a.x_out = x_out;
+  a.d = v.d.p;
+  a.dinv = v.dinv.p;
+  a.ca = ca;
+  a.cc = cc;
+  apply_level<T>(l, x_in, ASM_CHEB, a);
 }
 
 template <class T>
@@ -615,15 +779,30 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
   Level &L = levels_[l];
   LevelVecs<T> &v = vecs<T>(l);
   LevelVecs<T> &c = vecs<T>(l + 1);
-  smooth<T>(l, b, x, true, o);
+  const int deg = std::max(1, o.cheb_degree);
+  // 2 deg - 1 ping-pong updates in total (odd): start in the spare buffer so
+  // the last post-smoothing update lands in x
+  T *cur = v.x2.p, *oth = x;
+  if (cur == x) std::swap(cur, oth);
+  {
+    const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
+    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), v.d.p, cur, st_);
+  }
+  for (int k = 1; k < deg; k++) {
+    cheb_step<T>(l, b, cur, oth, k, o);
+    std::swap(cur, oth);
+  }
   AsmArgs a;
   a.b = b;
   a.y = v.r.p;
-  apply_level<T>(l, x, ASM_RESID, a);
+  apply_level<T>(l, cur, ASM_RESID, a);
   launch_restrict<T>(L.R, levels_[l + 1].d_bcmask.p, v.r.p, c.b.p, st_);
   vcycle<T>(l + 1, c.b.p, c.x.p, o);
-  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, x, st_);
-  smooth<T>(l, b, x, false, o);
+  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, cur, st_);
+  for (int k = 0; k < deg; k++) {
+    cheb_step<T>(l, b, cur, oth, k, o);
+    std::swap(cur, oth);
+  }
 }
 
 int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &o,

@@ -91,7 +91,7 @@ struct SolveInfo {
 
 template <class T>
 struct LevelVecs {
-  DevArray<T> b, x, r, d, dinv, partial, s;
+  DevArray<T> b, x, x2, r, d, dinv, partial, s;
   bool ready = false;
 };
 
@@ -113,6 +113,11 @@ struct Level {
   DevArray<int16_t> d_lev;
   ElemBatches eb;
   AsmCSR asmc;
+  // owner tiles (single-pass apply)
+  DevArray<int> t_node0, t_ext_ptr, t_ext, t_nacc, t_el_ptr, t_comb_ptr, t_comb_slot;
+  DevArray<TileElem> t_el;
+  DevArray<float> t_comb_w;
+  Tiles tiles;
   // transfer to the next coarser level (P: rows = this level's nodes)
   DevArray<int> P_ptr, P_col, R_ptr, R_col;
   DevArray<float> P_w, R_w;
@@ -156,6 +161,7 @@ class Problem {
  private:
   int build_level(int l, int nbcs, const char **names, const int *masks);
   int build_transfer(int l);
+  int build_tiles(int l);
   int ensure_vecs(int l, int precision);
   template <class T>
   LevelVecs<T> &vecs(int l);
@@ -166,7 +172,7 @@ class Problem {
   template <class T>
   void apply_level(int l, const T *x, int mode, const AsmArgs &a);
   template <class T>
-  void smooth(int l, const T *b, T *x, bool zero_guess, const SolveOpts &o);
+  void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o);
   template <class T>
   void vcycle(int l, const T *b, T *x, const SolveOpts &o);
   int coarse_factor();
