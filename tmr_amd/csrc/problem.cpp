@@ -853,7 +853,7 @@ int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &
   double rnorm = bnorm;
   int it = 0;
   bool conv = bnorm == 0.0;
-  const int nab = assemble_num_blocks(L0.n_nodes);
+  const int nab = L0.tiles.n_tiles;  // one p.q partial per owner tile
   while (!conv && it < o.max_iters) {
     it++;
     AsmArgs a;
