@@ -44,7 +44,7 @@ struct ElemBatches {
 // hanging node one of them supports).  LDS slots: owned nodes, then the
 // hanging nodes whose sums the tile needs ("accumulated"), then halo nodes.
 constexpr int TILE_NODES = 512;
-constexpr int TILE_THREADS = 512;
+constexpr int TILE_THREADS = 256;
 
 struct TileElem {       // 24 bytes
   uint16_t lc[8];       // LDS slot of each corner

@@ -769,7 +769,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
         }
       }
     }
-    for (int c = 0; c < ncolors; c++) {
+    // elements are sorted by colour: this round only spans the colours of
+    // its first and last element, and most waves hold a single colour
+    const int c_lo = (int)((el[3 * (size_t)(q0 + r0) + 2].y >> 8) & 0xff);
+    const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + TILE_THREADS, nel) - 1) + 2].y >> 8) & 0xff);
+    for (int c = c_lo; c <= c_hi; c++) {
       if (col == c) {
 #pragma unroll
         for (int a = 0; a < 8; a++) {

@@ -409,6 +409,10 @@ int Problem::build_tiles(int l) {
       for (int a = 0; a < 8; a++)
         if (els[k].mask & (1u << a)) col[els[k].lc[a]] |= 1u << c;
     }
+    // scatter phases follow the colours: sort by colour, then corner mask
+    std::stable_sort(els.begin() + el0, els.end(), [](const TileElem &a, const TileElem &b) {
+      return a.color != b.color ? a.color < b.color : a.mask > b.mask;
+    });
     nacc[t] = (int)rdeps.size();
     ext_ptr.push_back((int)ext.size());
     el_ptr.push_back((int)els.size());
