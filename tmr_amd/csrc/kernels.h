@@ -13,7 +13,7 @@
 //     of the hanging nodes that depend on it (hanging-node weight), so there
 //     are no atomics and the summation order is fixed.
 #pragma once
-#include <hip/hip_runtime_api.h>
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
@@ -45,6 +45,14 @@ struct ElemBatches {
 // hanging nodes whose sums the tile needs ("accumulated"), then halo nodes.
 constexpr int TILE_NODES = 512;
 constexpr int TILE_THREADS = 256;
+
+// LDS position of tile slot s: an XOR of higher bits into the low three bits
+// (a permutation inside each aligned group of 8), so that the corner slots of
+// same-colour elements -- Morton strides of 8, 64, 512 -- spread over banks.
+__host__ __device__ inline int tile_pos(int s) {
+  return s ^ (((s >> 3) ^ (s >> 6) ^ (s >> 9)) & 7);
+}
+inline int round8(int n) { return (n + 7) & ~7; }
 
 struct TileElem {       // 24 bytes
   uint16_t lc[8];       // LDS slot of each corner

@@ -715,19 +715,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   const int nslot = nown + next;
   const int nacc = nown + nacc_dep[t];
   T *xs = reinterpret_cast<T *>(smem_raw);
-  T *ys = xs + 3 * nslot;
+  T *ys = xs + 3 * ((nslot + 7) & ~7);
   for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
-    const int node = n0 + i / 3, c = i - 3 * (i / 3);
-    xs[i] = ((bcmask[node] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
+    const int j = i / 3, c = i - 3 * j;
+    xs[3 * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
   }
   for (int i = tid; i < next; i += TILE_THREADS) {
     T v[3];
     load_node<T>(ext[e0 + i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
-    xs[3 * (nown + i)] = v[0];
-    xs[3 * (nown + i) + 1] = v[1];
-    xs[3 * (nown + i) + 2] = v[2];
+    const int q = 3 * tile_pos(nown + i);
+    xs[q] = v[0];
+    xs[q + 1] = v[1];
+    xs[q + 2] = v[2];
   }
-  for (int i = tid; i < 3 * nacc; i += TILE_THREADS) ys[i] = T(0);
+  for (int i = tid; i < 3 * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
   __syncthreads();
   const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
   for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
@@ -790,7 +791,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   double dot = 0.0;
   for (int i = tid; i < nown; i += TILE_THREADS) {
     const int node = n0 + i;
-    T a[3] = {ys[3 * i], ys[3 * i + 1], ys[3 * i + 2]};
+    const int pi = 3 * tile_pos(i);
+    T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
     for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
       const int sl = comb_slot[j];
       const T w = (T)comb_w[j];
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       if (MODE == ASM_STORE) {
         y[k] = bc ? x[k] : a[c];
       } else if (MODE == ASM_DOT) {
-        const T xv = bc ? x[k] : xs[3 * i + c];
+        const T xv = bc ? x[k] : xs[pi + c];
         const T v = bc ? xv : a[c];
         y[k] = v;
         dot += (double)xv * (double)v;
@@ -820,7 +822,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
           const T r = b[k] - a[c];
           const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
           d[k] = dn;
-          x_out[k] = xs[3 * i + c] + dn;
+          x_out[k] = xs[pi + c] + dn;
         }
       }
     }
@@ -835,7 +837,7 @@ template <class T>
 void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
                        const AsmArgs &g, hipStream_t st) {
   if (tl.n_tiles == 0) return;
-  const size_t sh = 3 * (size_t)(tl.max_slots + tl.max_acc) * sizeof(T);
+  const size_t sh = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(T);
   const T *b = (const T *)g.b;
   T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
   const T *dinv = (const T *)g.dinv;

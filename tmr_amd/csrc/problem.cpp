@@ -379,11 +379,11 @@ int Problem::build_tiles(int l) {
           slot[x] = ns++;
           ext.push_back(g);
         }
-        if (slot[x] >= 65536) {
+        if (slot[x] >= 65528) {
           last_error = "problem: tile slot index overflow";
           return -1;
         }
-        te.lc[a] = (uint16_t)slot[x];
+        te.lc[a] = (uint16_t)tile_pos(slot[x]);
         if (slot[x] < na) te.mask |= (uint8_t)(1u << a);
       }
       els.push_back(te);
@@ -393,7 +393,7 @@ int Problem::build_tiles(int l) {
     std::stable_sort(els.begin() + el0, els.end(), [](const TileElem &a, const TileElem &b) {
       return a.mask > b.mask;
     });
-    col.assign(na, 0u);
+    col.assign(round8(na), 0u);
     for (size_t k = el0; k < els.size(); k++) {
       uint32_t used = 0;
       for (int a = 0; a < 8; a++)
@@ -420,7 +420,7 @@ int Problem::build_tiles(int l) {
     max_acc = std::max(max_acc, na);
     for (int p = n0; p < n1; p++) {
       for (int j = pptr[p]; j < pptr[p + 1]; j++) {
-        comb_slot.push_back(slot[n + pdep[j]]);
+        comb_slot.push_back(tile_pos(slot[n + pdep[j]]));
         comb_w.push_back(pw[j]);
       }
       comb_ptr[p + 1] = (int)comb_slot.size();
