@@ -189,6 +189,20 @@ int tmrx_problem_compliance(tmrx_problem *p, double *c);
    TMRTopoProblem::evalObjConGradient (src/topology/TMRTopoProblem.cpp:1275-1280),
    -psi^T dK/drho u with psi = u. */
 int tmrx_problem_compliance_grad(tmrx_problem *p, double *g, int on_device);
+/* p-norm (exponent P >= 2) of the relaxed von Mises failure of the last
+   solution, (sum_e sum_g (V_e/8) fail_eg^P)^(1/P) with
+   fail = rho/(eps(1-rho)+rho) * vonMises(C0 B_g u_e)/ys
+   (TMROctConstitutive::evalFailure, src/topology/TMROctConstitutive.cpp:1056-1104,
+   aggregated like TACS's "pnorm-continuous" KSFailure used in
+   examples/cylinder/cylinder.py:1040-1042). */
+int tmrx_problem_stress_pnorm(tmrx_problem *p, double P, double *value);
+/* Its gradient w.r.t. the element densities: explicit relaxation term
+   (TMROctConstitutive::addFailureDVSens, :1159-1215) minus psi^T dK/drho u,
+   psi from the adjoint solve K psi = df/du -- the general-functional branch
+   of TMRTopoProblem::evalObjConGradient (src/topology/TMRTopoProblem.cpp:1253-1271).
+   Returns 1 if the adjoint solve did not converge. */
+int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_device,
+                                   const tmrx_solve_opts *opts, tmrx_solve_info *info);
 /* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32). */
 int tmrx_problem_apply(tmrx_problem *p, const double *x, double *y, int on_device, int precision);
 /* Consistent nodal load of a constant traction tr[3] on the faces named

@@ -352,3 +352,140 @@ void orc_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *
     energy[e] = s;
   }
 }
+
+/* ---- p-norm of the relaxed von Mises failure ----------------------------
+   f = (sum_e sum_g (V_e/8) fail_eg^P)^(1/P), fail = r(rho) vm(C e_g) / ys,
+   r = rho / (eps (1 - rho) + rho) (TMROctConstitutive::evalFailure,
+   src/topology/TMROctConstitutive.cpp:1056-1104; aggregation as TACS's
+   "pnorm-continuous" KSFailure).  Returns f; when dfdu / dfdrho are not
+   NULL they receive df/du (assembled through the hanging-node weights,
+   clamped dofs zero) and the explicit df/drho_e. */
+static void gather_ue(const int *conn, int e, const int *dep_ptr, const int *dep_conn,
+                      const double *dep_w, const uint8_t *bcmask, const double *u, double ue[24]) {
+  for (int a = 0; a < 8; a++) {
+    int g = conn[8 * e + a];
+    for (int i = 0; i < 3; i++) {
+      double v = 0.0;
+      if (g >= 0) {
+        v = ((bcmask[g] >> i) & 1) ? 0.0 : u[3 * g + i];
+      } else {
+        int d = -g - 1;
+        for (int k = dep_ptr[d]; k < dep_ptr[d + 1]; k++) {
+          int p = dep_conn[k];
+          if (!((bcmask[p] >> i) & 1)) v += dep_w[k] * u[3 * p + i];
+        }
+      }
+      ue[3 * a + i] = v;
+    }
+  }
+}
+
+double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                        const int *dep_conn, const double *dep_w, const double *hdim,
+                        const uint8_t *bcmask, double E, double nu, double ys, double eps,
+                        double P, const double *rho, const double *u, double *dfdu,
+                        double *dfdrho) {
+  double C[6][6];
+  iso_C(E, nu, C);
+  const double gp = 1.0 / sqrt(3.0);
+  double S = 0.0;
+  /* first pass: the sum */
+  for (int e = 0; e < nelem; e++) {
+    double ue[24], h[3] = {hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2]};
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
+    double V = h[0] * h[1] * h[2];
+    double r = rho[e] / (eps * (1.0 - rho[e]) + rho[e]);
+    for (int q = 0; q < 8; q++) {
+      double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
+      double ep[6], s[6];
+      strain_at(h, p, ue, ep);
+      for (int i = 0; i < 6; i++) {
+        s[i] = 0.0;
+        for (int j = 0; j < 6; j++) s[i] += C[i][j] * ep[j];
+      }
+      double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
+                              (s[2] - s[0]) * (s[2] - s[0])) +
+                       3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])) / ys;
+      S += 0.125 * V * pow(r * vm, P);
+    }
+  }
+  double f = pow(S, 1.0 / P);
+  if (!dfdu && !dfdrho) return f;
+  double coef = pow(S, 1.0 / P - 1.0);
+  if (dfdu) memset(dfdu, 0, sizeof(double) * 3 * (size_t)nnodes);
+  for (int e = 0; e < nelem; e++) {
+    double ue[24], h[3] = {hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2]};
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
+    double V = h[0] * h[1] * h[2];
+    double den = eps * (1.0 - rho[e]) + rho[e];
+    double r = rho[e] / den, dr = eps / (den * den);
+    double ge[24] = {0}, sumvmP = 0.0;
+    for (int q = 0; q < 8; q++) {
+      double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
+      double ep[6], s[6];
+      strain_at(h, p, ue, ep);
+      for (int i = 0; i < 6; i++) {
+        s[i] = 0.0;
+        for (int j = 0; j < 6; j++) s[i] += C[i][j] * ep[j];
+      }
+      double J = 0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
+                        (s[2] - s[0]) * (s[2] - s[0])) +
+                 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+      double vm = sqrt(J) / ys;
+      sumvmP += pow(vm, P);
+      /* d fail^P / d s = P fail^(P-1) r dvm/ds, dvm/ds = dJ/ds / (2 ys^2 vm) */
+      double dJ[6] = {2 * s[0] - s[1] - s[2], 2 * s[1] - s[0] - s[2], 2 * s[2] - s[0] - s[1],
+                      6 * s[3], 6 * s[4], 6 * s[5]};
+      double w = coef * 0.125 * V * pow(r, P) * pow(vm, P - 2.0) / (2.0 * ys * ys);
+      /* chain through s = C eps(u): dfdu_c = sum_i w dJ_i (C eps(e_c))_i */
+      double unit[24];
+      for (int c = 0; c < 24; c++) {
+        memset(unit, 0, sizeof(unit));
+        unit[c] = 1.0;
+        double ec[6];
+        strain_at(h, p, unit, ec);
+        double t = 0.0;
+        for (int i = 0; i < 6; i++) {
+          double sc = 0.0;
+          for (int j = 0; j < 6; j++) sc += C[i][j] * ec[j];
+          t += dJ[i] * sc;
+        }
+        ge[c] += w * t;
+      }
+    }
+    if (dfdrho) dfdrho[e] = coef * 0.125 * V * pow(r, P - 1.0) * dr * sumvmP;
+    if (dfdu) {
+      for (int a = 0; a < 8; a++) {
+        int g = conn[8 * e + a];
+        for (int i = 0; i < 3; i++) {
+          if (g >= 0) {
+            if (!((bcmask[g] >> i) & 1)) dfdu[3 * g + i] += ge[3 * a + i];
+          } else {
+            int d = -g - 1;
+            for (int k = dep_ptr[d]; k < dep_ptr[d + 1]; k++) {
+              int p = dep_conn[k];
+              if (!((bcmask[p] >> i) & 1)) dfdu[3 * p + i] += dep_w[k] * ge[3 * a + i];
+            }
+          }
+        }
+      }
+    }
+  }
+  return f;
+}
+
+/* psi_e^T K_e u_e per element (unpenalised) */
+void orc_elem_energy2(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                      const double *dep_w, const double *hdim, const uint8_t *bcmask, double E,
+                      double nu, const double *u, const double *v, double *out) {
+  double K[576], ue[24], ve[24];
+  for (int e = 0; e < nelem; e++) {
+    orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, K);
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, v, ve);
+    double s = 0.0;
+    for (int i = 0; i < 24; i++)
+      for (int j = 0; j < 24; j++) s += ve[i] * K[24 * i + j] * ue[j];
+    out[e] = s;
+  }
+}

@@ -29,6 +29,7 @@ def lib():
         L.orc_solve.restype = I
         L.orc_solve.argtypes = [P, P, D, I, P]
         L.orc_elem_energy.argtypes = [I, P, P, P, P, P, P, D, D, P, P]
+        L.orc_elem_energy2.argtypes = [I, P, P, P, P, P, P, D, D, P, P, P]
         _L = L
     return _L
 
@@ -104,6 +105,15 @@ class Oracle:
         if it < 0:
             raise RuntimeError("oracle CG did not converge")
         return x
+
+    def energy2(self, u, v):
+        u = np.ascontiguousarray(u, dtype=np.double)
+        v = np.ascontiguousarray(v, dtype=np.double)
+        e = np.zeros(self.nelem)
+        lib().orc_elem_energy2(self.nelem, _p(self.conn), _p(self.dep_ptr), _p(self.dep_conn),
+                               _p(self.dep_w), _p(self.hdim), _p(self.bcmask), self.E, self.nu,
+                               _p(u), _p(v), _p(e))
+        return e
 
     def energy(self, u):
         u = np.ascontiguousarray(u, dtype=np.double)
@@ -237,3 +247,38 @@ def ref_octant_lib():
     R.ref_compare.argtypes = [C.c_void_p, C.c_void_p]
     R.ref_sizeof_octant.restype = C.c_int
     return R
+
+
+def _stress_sigs():
+    L = lib()
+    if not hasattr(L, "_stress_sigs"):
+        D, I, P = C.c_double, C.c_int, C.c_void_p
+        L.orc_stress_pnorm.restype = D
+        L.orc_stress_pnorm.argtypes = [I, I, P, P, P, P, P, P, D, D, D, D, D, P, P, P, P]
+        L._stress_sigs = True
+    return L
+
+
+def stress_pnorm(orc, rho, u, P=8.0, ys=1.0, eps=0.3, grad=False):
+    """Value (and, with grad=True, df/du and the explicit df/drho) of the
+    p-norm relaxed von Mises functional on an Oracle mesh."""
+    L = _stress_sigs()
+    rho = np.ascontiguousarray(rho, dtype=np.double)
+    u = np.ascontiguousarray(u, dtype=np.double)
+    dfdu = np.zeros(3 * orc.nnodes) if grad else None
+    dfdr = np.zeros(orc.nelem) if grad else None
+    f = L.orc_stress_pnorm(orc.nnodes, orc.nelem, _p(orc.conn), _p(orc.dep_ptr), _p(orc.dep_conn),
+                           _p(orc.dep_w), _p(orc.hdim), _p(orc.bcmask), orc.E, orc.nu, ys, eps, P,
+                           _p(rho), _p(u), _p(dfdu) if grad else None, _p(dfdr) if grad else None)
+    return (f, dfdu, dfdr) if grad else f
+
+
+def stress_gradient(orc, rho, b, q=3.0, k0=1e-6, P=8.0, ys=1.0, eps=0.3, rtol=1e-13):
+    """Full adjoint gradient d f / d rho_e of the p-norm functional (the
+    oracle's own CG solves for u and psi)."""
+    orc.assemble(np.array([penalty(r, q, k0) for r in rho]))
+    u = orc.solve(b, rtol=rtol)
+    f, dfdu, dfdr = stress_pnorm(orc, rho, u, P, ys, eps, grad=True)
+    psi = orc.solve(dfdu, rtol=rtol)
+    prod = orc.energy2(u, psi)
+    return f, dfdr - np.array([dpenalty(r, q) for r in rho]) * prod, u, psi

@@ -118,3 +118,42 @@ def test_compliance_gradient_central_differences():
             cs.append(b @ orc.solve(b, rtol=1e-14))
         fd = (cs[0] - cs[1]) / (2 * h)
         assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-9 * abs(cs[0]))
+
+
+def test_von_mises_closed_form_uniaxial():
+    """Uniaxial strain state: vm of C eps by hand."""
+    f, orc, rho, s, b = _small_problem(level=1, seed=2)
+    # u = strain field eps_xx = 1e-3 everywhere (linear in x), no clamps used
+    orc2 = po.Oracle(orc.conn, orc.dep_ptr, orc.dep_conn, orc.dep_w, orc.hdim,
+                     np.zeros_like(orc.bcmask), E, NU)
+    X = f.getPoints()
+    nd = len(f.getDepNodeConn()[0]) - 1
+    u = np.zeros(3 * orc.nnodes)
+    for n in range(orc.nnodes):
+        u[3 * n] = 1e-3 * X[n + nd, 0]
+    rho1 = np.ones(orc.nelem)
+    val = po.stress_pnorm(orc2, rho1, u, P=8.0, ys=1.0, eps=0.3)
+    s0, s1 = (LAM + 2 * MU) * 1e-3, LAM * 1e-3
+    vm = abs(s0 - s1)  # sqrt(0.5*(2 (s0-s1)^2)) = |s0 - s1|
+    vol = orc.hdim.reshape(-1, 3).prod(1).sum()
+    assert val == pytest.approx(vm * vol ** (1 / 8.0), rel=1e-12)
+
+
+def test_stress_gradient_central_differences():
+    f, orc, rho, s, b = _small_problem(level=1, seed=4)
+    b[:] = 0
+    for n in f.getNodesWithName("load"):
+        if n >= 0:
+            b[3 * n + 2] = -1.0
+    val, g, u, psi = po.stress_gradient(orc, rho, b)
+    h = 1e-6
+    rng = np.random.default_rng(2)
+    for k in rng.choice(len(rho), 6, replace=False):
+        vals = []
+        for sgn in (+1, -1):
+            r2 = rho.copy(); r2[k] += sgn * h
+            orc.assemble(np.array([po.penalty(r, 3.0, 1e-6) for r in r2]))
+            u2 = orc.solve(b, rtol=1e-14)
+            vals.append(po.stress_pnorm(orc, r2, u2))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-8 * abs(val))

@@ -83,6 +83,8 @@ _SIGS = {
     "tmrx_problem_compliance_grad": (I, [P, P, I]),
     "tmrx_problem_apply": (I, [P, P, P, I, I]),
     "tmrx_problem_traction_load": (I, [P, S, DP, DP]),
+    "tmrx_problem_stress_pnorm": (I, [P, D, DP]),
+    "tmrx_problem_stress_pnorm_grad": (I, [P, D, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "tmrx_problem_kernel_timing": (I, [P, I]),
     "tmrx_problem_kernel_timing_result": (I, [P, DP, IP, IP]),
 }

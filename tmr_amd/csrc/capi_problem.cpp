@@ -199,6 +199,37 @@ int tmrx_problem_traction_load(tmrx_problem *h, const char *name, const double t
   return h->p->traction_load(name, tr, f) ? fail(h) : 0;
 }
 
+int tmrx_problem_stress_pnorm(tmrx_problem *h, double P, double *value) {
+  CHECK_P(h);
+  if (!value) {
+    set_error("tmrx_problem_stress_pnorm: null output");
+    return -1;
+  }
+  return h->p->stress_pnorm(P, value) ? fail(h) : 0;
+}
+
+int tmrx_problem_stress_pnorm_grad(tmrx_problem *h, double P, double *g, int on_device,
+                                   const tmrx_solve_opts *opts, tmrx_solve_info *info) {
+  CHECK_P(h);
+  if (!g) {
+    set_error("tmrx_problem_stress_pnorm_grad: null output");
+    return -1;
+  }
+  tmrx::SolveInfo si;
+  int r = h->p->stress_pnorm_grad(P, g, on_device != 0, to_opts(opts), &si);
+  if (info) {
+    info->iters = si.iters;
+    info->converged = si.converged;
+    info->rnorm0 = si.rnorm0;
+    info->rnorm = si.rnorm;
+    info->setup_ms = si.setup_ms;
+    info->solve_ms = si.solve_ms;
+  }
+  if (r < 0) return fail(h);
+  if (r > 0) set_error(h->p->last_error);
+  return r;
+}
+
 int tmrx_problem_kernel_timing(tmrx_problem *h, int enable) {
   CHECK_P(h);
   return h->p->kernel_timing(enable);

@@ -106,6 +106,7 @@ enum AsmMode {
   ASM_CHEB = 2,    // r = b - A x; d = a d + c Dinv r; x += d
   ASM_DIAG = 3,    // Dinv = 1 / diag(A)   (partials hold element diagonals)
   ASM_DOT = 4,     // y = A x and block partials of x . y
+  ASM_GATHER = 5,  // y = sum of the slot partials (clamped rows zero): Z^T g
 };
 
 struct AsmArgs {
@@ -181,6 +182,35 @@ void launch_finalize(int nparts, const double *partials, int mode, double *sc, h
 void launch_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                         const float *dep_w, const uint8_t *bcmask, const int *cls,
                         const double *K0, const double *u, double *energy, hipStream_t st);
+// p-norm of the relaxed von Mises failure (TMROctConstitutive::evalFailure,
+// src/topology/TMROctConstitutive.cpp:1056-1104; TACS "pnorm-continuous"
+// aggregation): S = sum_e sum_g (V_e / 8) (r(rho_e) vm(C0 B_g u_e) / ys)^P.
+// CB: per class, the 8 Gauss-point stress operators C0 B_g (6 x 24) of the
+// full-block element; vol: per class block volume.
+struct StressArgs {
+  double P = 8.0, eps = 0.3, ys = 1.0;
+  const double *CB = nullptr;
+  const double *vol = nullptr;
+};
+void launch_stress_sum(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                       const float *dep_w, const uint8_t *bcmask, const int *cls,
+                       const int16_t *lev, const double *rho, const double *u,
+                       const StressArgs &sa, double *partials, hipStream_t st);
+// element dfdu into slot partials (element batches) and the explicit
+// d f / d rho_e; coef = S^(1/P - 1) read from sc[SC_ZQ]
+void launch_stress_grad(const ElemBatches &eb, const int16_t *lev, const int *cls,
+                        const double *rho, const double *u, const StressArgs &sa,
+                        const double *sc, double *partial, double *dfdrho, hipStream_t st);
+// psi_e^T K0 u_e per element (adjoint-residual product without the factor)
+void launch_elem_energy2(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                         const float *dep_w, const uint8_t *bcmask, const int *cls,
+                         const double *K0, const double *u, const double *v, double *out,
+                         hipStream_t st);
+// g = dfdrho_explicit - dpenalty(rho) 2^-level * prod
+void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q, int ptype,
+                         const double *prod, const double *explicit_part, double *g,
+                         hipStream_t st);
+
 // SIMP/RAMP derivative (src/topology/TMROctConstitutive.cpp:597-619) times
 // -2^-level times the element energy
 void launch_compliance_grad(int n, const double *rho, const int16_t *lev, double q, int ptype,

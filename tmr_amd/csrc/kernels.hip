@@ -235,6 +235,8 @@ __global__ __launch_bounds__(NODE_BLOCK) void k_assemble(
         }
       } else if (MODE == ASM_DIAG) {
         y[k] = bc ? T(1) : T(1) / acc[c];
+      } else if (MODE == ASM_GATHER) {
+        y[k] = bc ? T(0) : acc[c];
       }
     }
   }
@@ -263,6 +265,7 @@ void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, i
     case ASM_CHEB: TMRX_ASM(ASM_CHEB); break;
     case ASM_DIAG: TMRX_ASM(ASM_DIAG); break;
     case ASM_DOT: TMRX_ASM(ASM_DOT); break;
+    case ASM_GATHER: TMRX_ASM(ASM_GATHER); break;
   }
 #undef TMRX_ASM
   TMRX_LAUNCH_CHECK();
@@ -862,6 +865,230 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   }
 #undef TMRX_TILE2
 #undef TMRX_TILE
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// p-norm von Mises functional and its derivatives (FP64)
+__device__ __forceinline__ void vm_terms(const double s[6], double ys, double *J, double dJ[6]) {
+  const double a = s[0] - s[1], b = s[1] - s[2], c = s[0] - s[2];
+  *J = 0.5 * (a * a + b * b + c * c) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  dJ[0] = 2.0 * s[0] - s[1] - s[2];
+  dJ[1] = 2.0 * s[1] - s[0] - s[2];
+  dJ[2] = 2.0 * s[2] - s[0] - s[1];
+  dJ[3] = 6.0 * s[3];
+  dJ[4] = 6.0 * s[4];
+  dJ[5] = 6.0 * s[5];
+  (void)ys;
+}
+
+__global__ __launch_bounds__(256) void k_stress_sum(
+    int ne, const int *__restrict__ conn, const int *__restrict__ dep_ptr,
+    const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+    const uint8_t *__restrict__ bcmask, const int *__restrict__ cls,
+    const int16_t *__restrict__ lev, const double *__restrict__ rho, const double *__restrict__ u,
+    double P, double eps, double ys, const double *__restrict__ CB, const double *__restrict__ vol,
+    double *__restrict__ partials) {
+  __shared__ double red[4];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  double acc = 0.0;
+  if (e < ne) {
+    double ue[24];
+    for (int a = 0; a < 8; a++) {
+      double v[3];
+      load_node<double>(conn[8 * e + a], u, dep_ptr, dep_conn, dep_w, bcmask, v);
+      ue[3 * a] = v[0];
+      ue[3 * a + 1] = v[1];
+      ue[3 * a + 2] = v[2];
+    }
+    const int c = cls[e], l = lev[e];
+    const double hs = ldexp(1.0, l);  // B scales as 1/h
+    const double Ve = ldexp(vol[c], -3 * l);
+    const double r = rho[e] / (eps * (1.0 - rho[e]) + rho[e]);
+    const double *G = CB + 8 * 144 * (size_t)c;
+    for (int g = 0; g < 8; g++) {
+      double sg[6];
+      for (int i = 0; i < 6; i++) {
+        double t = 0.0;
+        for (int j = 0; j < 24; j++) t += G[144 * g + 24 * i + j] * ue[j];
+        sg[i] = hs * t;
+      }
+      double J, dJ[6];
+      vm_terms(sg, ys, &J, dJ);
+      const double fail = r * sqrt(J) / ys;
+      acc += 0.125 * Ve * pow(fail, P);
+    }
+  }
+  const double sum = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+}
+
+void launch_stress_sum(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                       const float *dep_w, const uint8_t *bcmask, const int *cls,
+                       const int16_t *lev, const double *rho, const double *u,
+                       const StressArgs &sa, double *partials, hipStream_t st) {
+  if (nelem == 0) return;
+  hipLaunchKernelGGL(k_stress_sum, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
+                     dep_ptr, dep_conn, dep_w, bcmask, cls, lev, rho, u, sa.P, sa.eps, sa.ys,
+                     sa.CB, sa.vol, partials);
+  TMRX_LAUNCH_CHECK();
+}
+
+// dfdu: per element g_e = sum_g coef (V/8) r^P vm^(P-2) / (2 ys^2) dJ^T (C0 B_g),
+// staged per batch like the apply; explicit dfdrho_e = coef (V/8) r^(P-1) r'
+// sum_g vm^P.  vm = sqrt(J) / ys.
+__global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
+    const int *__restrict__ wg_node_ptr, const int *__restrict__ wg_nodes,
+    const int *__restrict__ wg_elem_ptr, const uint4 *__restrict__ lconn,
+    const uint8_t *__restrict__ color, int ncolors, const int *__restrict__ dep_ptr,
+    const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+    const uint8_t *__restrict__ bcmask, const int16_t *__restrict__ lev,
+    const int *__restrict__ cls, const double *__restrict__ rho, const double *__restrict__ u,
+    double P, double eps, double ys, const double *__restrict__ CB,
+    const double *__restrict__ vol, const double *__restrict__ sc,
+    double *__restrict__ partial, double *__restrict__ dfdrho) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wg = blockIdx.x, tid = threadIdx.x;
+  const int nb = wg_node_ptr[wg], nn = wg_node_ptr[wg + 1] - nb;
+  double *xs = reinterpret_cast<double *>(smem_raw);
+  double *ys_acc = xs + 3 * nn;
+  for (int i = tid; i < nn; i += WG_ELEMS) {
+    double v[3];
+    load_node<double>(wg_nodes[nb + i], u, dep_ptr, dep_conn, dep_w, bcmask, v);
+    xs[3 * i] = v[0];
+    xs[3 * i + 1] = v[1];
+    xs[3 * i + 2] = v[2];
+    ys_acc[3 * i] = ys_acc[3 * i + 1] = ys_acc[3 * i + 2] = 0.0;
+  }
+  __syncthreads();
+  const int e0 = wg_elem_ptr[wg], ne = wg_elem_ptr[wg + 1] - e0;
+  const double coef = sc[SC_ZQ];
+  double ge[24];
+  int lc[8];
+  int col = -1;
+  if (tid < ne) {
+    const int e = e0 + tid;
+    const uint4 q = lconn[e];
+    lc[0] = q.x & 0xffff; lc[1] = q.x >> 16;
+    lc[2] = q.y & 0xffff; lc[3] = q.y >> 16;
+    lc[4] = q.z & 0xffff; lc[5] = q.z >> 16;
+    lc[6] = q.w & 0xffff; lc[7] = q.w >> 16;
+    col = color[e];
+    double ue[24];
+    for (int a = 0; a < 8; a++)
+      for (int d = 0; d < 3; d++) ue[3 * a + d] = xs[3 * lc[a] + d];
+    for (int j = 0; j < 24; j++) ge[j] = 0.0;
+    const int c = cls[e], l = lev[e];
+    const double hs = ldexp(1.0, l);
+    const double Ve = ldexp(vol[c], -3 * l);
+    const double rh = rho[e];
+    const double den = eps * (1.0 - rh) + rh;
+    const double r = rh / den, dr = eps / (den * den);
+    const double *G = CB + 8 * 144 * (size_t)c;
+    double sum_vmP = 0.0;
+    for (int g = 0; g < 8; g++) {
+      double sg[6];
+      for (int i = 0; i < 6; i++) {
+        double t = 0.0;
+        for (int j = 0; j < 24; j++) t += G[144 * g + 24 * i + j] * ue[j];
+        sg[i] = hs * t;
+      }
+      double J, dJ[6];
+      vm_terms(sg, ys, &J, dJ);
+      const double vm = sqrt(J) / ys;
+      sum_vmP += pow(vm, P);
+      // coef (V/8) r^P vm^(P-2) / (2 ys^2) * dJ^T CB_g (with the 1/h factor)
+      const double w = coef * 0.125 * Ve * pow(r, P) * pow(vm, P - 2.0) / (2.0 * ys * ys) * hs;
+      for (int i = 0; i < 6; i++) {
+        const double wi = w * dJ[i];
+        for (int j = 0; j < 24; j++) ge[j] += wi * G[144 * g + 24 * i + j];
+      }
+    }
+    dfdrho[e] = coef * 0.125 * Ve * pow(r, P - 1.0) * dr * sum_vmP;
+  }
+  for (int c = 0; c < ncolors; c++) {
+    if (col == c) {
+      for (int a = 0; a < 8; a++)
+        for (int d = 0; d < 3; d++) ys_acc[3 * lc[a] + d] += ge[3 * a + d];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < 3 * nn; i += WG_ELEMS) partial[3 * nb + i] = ys_acc[i];
+}
+
+void launch_stress_grad(const ElemBatches &eb, const int16_t *lev, const int *cls,
+                        const double *rho, const double *u, const StressArgs &sa,
+                        const double *sc, double *partial, double *dfdrho, hipStream_t st) {
+  if (eb.n_wg == 0) return;
+  const size_t sh = 2 * 3 * (size_t)eb.max_wg_nodes * sizeof(double);
+  hipLaunchKernelGGL(k_stress_grad, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
+                     eb.wg_nodes, eb.wg_elem_ptr, reinterpret_cast<const uint4 *>(eb.lconn),
+                     eb.color, eb.ncolors, eb.dep_ptr, eb.dep_conn, eb.dep_w, eb.bcmask, lev,
+                     cls, rho, u, sa.P, sa.eps, sa.ys, sa.CB, sa.vol, sc, partial, dfdrho);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_elem_energy2(int ne, const int *__restrict__ conn, const int *__restrict__ dep_ptr,
+                               const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+                               const uint8_t *__restrict__ bcmask, const int *__restrict__ cls,
+                               const double *__restrict__ K0, const double *__restrict__ u,
+                               const double *__restrict__ v, double *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double ue[24], ve[24];
+  for (int a = 0; a < 8; a++) {
+    double t[3];
+    load_node<double>(conn[8 * e + a], u, dep_ptr, dep_conn, dep_w, bcmask, t);
+    ue[3 * a] = t[0];
+    ue[3 * a + 1] = t[1];
+    ue[3 * a + 2] = t[2];
+    load_node<double>(conn[8 * e + a], v, dep_ptr, dep_conn, dep_w, bcmask, t);
+    ve[3 * a] = t[0];
+    ve[3 * a + 1] = t[1];
+    ve[3 * a + 2] = t[2];
+  }
+  const double *K = K0 + 576 * cls[e];
+  double s = 0.0;
+  for (int i = 0; i < 24; i++) {
+    double t = 0.0;
+    for (int j = 0; j < 24; j++) t += K[24 * i + j] * ue[j];
+    s += ve[i] * t;
+  }
+  out[e] = s;
+}
+
+void launch_elem_energy2(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                         const float *dep_w, const uint8_t *bcmask, const int *cls,
+                         const double *K0, const double *u, const double *v, double *out,
+                         hipStream_t st) {
+  if (nelem == 0) return;
+  hipLaunchKernelGGL(k_elem_energy2, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
+                     dep_ptr, dep_conn, dep_w, bcmask, cls, K0, u, v, out);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_adjoint_grad(int n, const double *__restrict__ rho, const int16_t *__restrict__ lev,
+                               double q, int ptype, const double *__restrict__ prod,
+                               const double *__restrict__ ex, double *__restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  double dp;
+  if (ptype == 0) {
+    dp = (q > 1.0) ? q * pow(r, q - 1.0) : 1.0;
+  } else {
+    const double t = 1.0 + q * (1.0 - r);
+    dp = (q + 1.0) / (t * t);
+  }
+  g[i] = ex[i] - ldexp(dp, -(int)lev[i]) * prod[i];
+}
+
+void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q, int ptype,
+                         const double *prod, const double *explicit_part, double *g,
+                         hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_adjoint_grad, dim3((n + 255) / 256), dim3(256), 0, st, n, rho, lev, q, ptype,
+                     prod, explicit_part, g);
   TMRX_LAUNCH_CHECK();
 }
 

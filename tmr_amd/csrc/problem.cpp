@@ -120,6 +120,30 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
   K0h_.assign(576 * (size_t)nclasses_, 0.0);
   for (int c = 0; c < nclasses_; c++)
     hex8_box_stiffness(ext[c][0], ext[c][1], ext[c][2], C, &K0h_[576 * c]);
+  // Gauss-point stress operators C0 B_g (6 x 24, 8 points) and volumes of the
+  // full-block element of every class (p-norm stress functional)
+  {
+    std::vector<double> cb(8 * 144 * (size_t)nclasses_), vol(nclasses_);
+    const double gp = 1.0 / std::sqrt(3.0);
+    for (int c = 0; c < nclasses_; c++) {
+      vol[c] = ext[c][0] * ext[c][1] * ext[c][2];
+      for (int g = 0; g < 8; g++) {
+        const double pt[3] = {(g & 1) ? gp : -gp, (g & 2) ? gp : -gp, (g & 4) ? gp : -gp};
+        double B[144], N[8];
+        hex8_box_B(ext[c][0], ext[c][1], ext[c][2], pt, B, N);
+        for (int i = 0; i < 6; i++)
+          for (int j = 0; j < 24; j++) {
+            double t = 0.0;
+            for (int k = 0; k < 6; k++) t += C[6 * i + k] * B[24 * k + j];
+            cb[144 * (8 * (size_t)c + g) + 24 * i + j] = t;
+          }
+      }
+    }
+    if (CB_.upload(cb) || cvol_.upload(vol)) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
+  }
   std::vector<float> K0f(K0h_.begin(), K0h_.end());
   if (K0d_.upload(K0h_) || K0f_.upload(K0f)) {
     last_error = "problem: device allocation failed";
@@ -138,9 +162,11 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
   const int64_t n = levels_[0].ndof;
   if (f_.alloc(n) || u_.alloc(n) || r_.alloc(n) || p_.alloc(n) || q_.alloc(n) || z_.alloc(n) ||
       parts_.alloc((size_t)std::max(std::max(2 * 2048, assemble_num_blocks(levels_[0].n_nodes)),
-                                    levels_[0].tiles.n_tiles) + 16) ||
+                                    std::max(levels_[0].tiles.n_tiles,
+                                             (levels_[0].n_elem + 255) / 256)) + 16) ||
       sc_.alloc(SC_N) || rho_.alloc(levels_[0].n_elem) ||
-      energy_.alloc(levels_[0].n_elem)) {
+      energy_.alloc(levels_[0].n_elem) || psi_.alloc(n) || adjf_.alloc(n) ||
+      dfdrho_.alloc(levels_[0].n_elem)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -811,6 +837,29 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
 
 int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &o,
                    SolveInfo *info) {
+  const int64_t n = levels_.empty() ? 0 : levels_[0].ndof;
+  if (!have_design_) {
+    last_error = "solve: set the design first";
+    return -1;
+  }
+  if (f) {
+    HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  }
+  int r = solve_core(f_.p, u_.p, o, info);
+  if (r < 0) return r;
+  if (u) {
+    HIPCHK(hipMemcpyAsync(u, u_.p, n * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+  }
+  have_solution_ = true;
+  return r;
+}
+
+// FP64 flexible PCG on K x = rhs from x = 0; rhs is overwritten with its
+// clamped rows zeroed.  Uses the PCG work vectors r_, p_, q_, z_.
+int Problem::solve_core(double *rhs, double *xsol, const SolveOpts &o, SolveInfo *info) {
   if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
     if (setup(o)) return -1;
   }
@@ -818,22 +867,18 @@ int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &
   Level &L0 = levels_[0];
   const int64_t n = L0.ndof;
   const int nrb = reduce_blocks(n);
-  if (f) {
-    HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double),
-                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
-  }
-  HIPCHK(hipMemsetAsync(u_.p, 0, n * sizeof(double), st_));
+  HIPCHK(hipMemsetAsync(xsol, 0, n * sizeof(double), st_));
   // clamped rows of the right-hand side are zero (TACSAssembler::applyBCs):
-  // r = f - A*0 through the assembly pass, which zeroes clamped rows
+  // r = rhs - A*0 through the assembly pass, which zeroes clamped rows
   {
     launch_fill<double>(3 * (int64_t)L0.wg_nodes.n, 0.0, L0.v64.partial.p, st_);
     AsmArgs a;
-    a.b = f_.p;
+    a.b = rhs;
     a.y = r_.p;
-    a.x = u_.p;
+    a.x = xsol;
     launch_assemble<double>(L0.asmc, L0.d_bcmask.p, L0.v64.partial.p, ASM_RESID, a, st_);
   }
-  HIPCHK(hipMemcpyAsync(f_.p, r_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  HIPCHK(hipMemcpyAsync(rhs, r_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
   launch_dot1(n, r_.p, r_.p, parts_.p, st_);
   launch_finalize(nrb, parts_.p, 4, sc_.p, st_);
   const bool mixed = o.precision != 0;
@@ -865,7 +910,7 @@ int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &
     a.partials = parts_.p;
     apply_level<double>(0, p_.p, ASM_DOT, a);
     launch_finalize(nab, parts_.p, 1, sc_.p, st_);
-    launch_pcg_update_xr(n, u_.p, r_.p, p_.p, q_.p, sc_.p, parts_.p, mixed ? L0.v32.b.p : nullptr,
+    launch_pcg_update_xr(n, xsol, r_.p, p_.p, q_.p, sc_.p, parts_.p, mixed ? L0.v32.b.p : nullptr,
                          st_);
     launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
     HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -885,21 +930,15 @@ int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &
   // true residual with the FP64 operator
   {
     AsmArgs a;
-    a.b = f_.p;
+    a.b = rhs;
     a.y = q_.p;
-    apply_level<double>(0, u_.p, ASM_RESID, a);
+    apply_level<double>(0, xsol, ASM_RESID, a);
     launch_dot1(n, q_.p, q_.p, parts_.p, st_);
     launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
     HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
     HIPCHK(hipStreamSynchronize(st_));
     last_true_residual = bnorm > 0 ? std::sqrt(std::max(hsc[SC_RR], 0.0)) / bnorm : 0.0;
   }
-  if (u) {
-    HIPCHK(hipMemcpyAsync(u, u_.p, n * sizeof(double),
-                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipStreamSynchronize(st_));
-  }
-  have_solution_ = true;
   if (info) {
     info->iters = it;
     info->converged = conv ? 1 : 0;
@@ -950,6 +989,83 @@ int Problem::compliance_grad(double *g, bool on_device) {
   }
   launch_compliance_grad(L.n_elem, rho_.p, L.d_lev.p, mat_.q, mat_.penalty_type, energy_.p, out,
                          st_);
+  if (!on_device)
+    HIPCHK(hipMemcpyAsync(g, out, L.n_elem * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+StressArgs Problem_stress_args(const Material &m, double P, const double *CB, const double *vol) {
+  StressArgs sa;
+  sa.P = P;
+  sa.eps = m.eps;
+  sa.ys = m.ys;
+  sa.CB = CB;
+  sa.vol = vol;
+  return sa;
+}
+
+int Problem::stress_pnorm(double P, double *f) {
+  if (!have_solution_) {
+    last_error = "stress_pnorm: solve first";
+    return -1;
+  }
+  if (!(P >= 2.0)) {
+    last_error = "stress_pnorm: the exponent must be >= 2";
+    return -1;
+  }
+  Level &L = levels_[0];
+  const StressArgs sa = Problem_stress_args(mat_, P, CB_.p, cvol_.p);
+  const int nb = (L.n_elem + 255) / 256;
+  if ((size_t)nb > parts_.n) {
+    last_error = "stress_pnorm: reduction buffer too small";
+    return -1;
+  }
+  launch_stress_sum(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+                    L.d_cls.p, L.d_lev.p, rho_.p, u_.p, sa, parts_.p, st_);
+  launch_finalize(nb, parts_.p, 4, sc_.p, st_);
+  double h[SC_N];
+  HIPCHK(hipMemcpyAsync(h, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  *f = std::pow(h[SC_BB], 1.0 / P);
+  last_stress_sum_ = h[SC_BB];
+  return 0;
+}
+
+int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveOpts &o,
+                               SolveInfo *info) {
+  double fval;
+  if (stress_pnorm(P, &fval)) return -1;
+  Level &L = levels_[0];
+  const StressArgs sa = Problem_stress_args(mat_, P, CB_.p, cvol_.p);
+  // coef = S^(1/P - 1) into the scalar slot the gradient kernel reads
+  const double coef = std::pow(last_stress_sum_, 1.0 / P - 1.0);
+  HIPCHK(hipMemcpyAsync(sc_.p + SC_ZQ, &coef, sizeof(double), hipMemcpyHostToDevice, st_));
+  launch_stress_grad(L.eb, L.d_lev.p, L.d_cls.p, rho_.p, u_.p, sa, sc_.p, L.v64.partial.p,
+                     dfdrho_.p, st_);
+  AsmArgs a;
+  a.y = adjf_.p;
+  launch_assemble<double>(L.asmc, L.d_bcmask.p, L.v64.partial.p, ASM_GATHER, a, st_);
+  HIPCHK(hipStreamSynchronize(st_));
+  // adjoint K psi = df/du (K symmetric); u_ and f_ are left untouched
+  const double tr = last_true_residual;
+  int r = solve_core(adjf_.p, psi_.p, o, info);
+  last_adjoint_residual = last_true_residual;
+  last_true_residual = tr;
+  if (r != 0) return r < 0 ? -1 : 1;
+  launch_elem_energy2(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+                      L.d_cls.p, K0d_.p, u_.p, psi_.p, energy_.p, st_);
+  double *out = g;
+  DevArray<double> tmp;
+  if (!on_device) {
+    if (tmp.alloc(L.n_elem)) {
+      last_error = "stress_pnorm_grad: allocation failed";
+      return -1;
+    }
+    out = tmp.p;
+  }
+  launch_adjoint_grad(L.n_elem, rho_.p, L.d_lev.p, mat_.q, mat_.penalty_type, energy_.p,
+                      dfdrho_.p, out, st_);
   if (!on_device)
     HIPCHK(hipMemcpyAsync(g, out, L.n_elem * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));

@@ -145,6 +145,11 @@ class Problem {
   int solve(const double *f, double *u, bool on_device, const SolveOpts &o, SolveInfo *info);
   int compliance(double *c);
   int compliance_grad(double *g, bool on_device);
+  // p-norm (exponent P) of the relaxed von Mises failure of the last solve,
+  // and its gradient w.r.t. the element densities through an adjoint solve
+  int stress_pnorm(double P, double *f);
+  int stress_pnorm_grad(double P, double *g, bool on_device, const SolveOpts &o,
+                        SolveInfo *info);
   int apply(const double *x, double *y, bool on_device, int precision);
   int traction_load(const char *name, const double tr[3], double *f);
   int num_dof() const { return levels_.empty() ? 0 : (int)levels_[0].ndof; }
@@ -156,6 +161,7 @@ class Problem {
   int kernel_timing(int enable);
   int kernel_timing_result(double *total_ms, int *count, int *precision_mask);
   double last_true_residual = 0.0;
+  double last_adjoint_residual = 0.0;
   std::string last_error;
 
  private:
@@ -176,6 +182,7 @@ class Problem {
   template <class T>
   void vcycle(int l, const T *b, T *x, const SolveOpts &o);
   int coarse_factor();
+  int solve_core(double *rhs, double *x, const SolveOpts &o, SolveInfo *info);
   int estimate_eigs(int l, const SolveOpts &o);
 
   std::vector<Level> levels_;
@@ -189,10 +196,11 @@ class Problem {
   int n_coarse_dof_ = 0;
   // PCG state
   DevArray<double> f_, u_, r_, p_, q_, z_, parts_, sc_;
-  DevArray<double> energy_;
+  DevArray<double> energy_, psi_, adjf_, dfdrho_, CB_, cvol_;
   bool have_design_ = false, have_setup_ = false, have_solution_ = false;
   int setup_precision_ = -1;
   double last_setup_ms_ = 0.0;
+  double last_stress_sum_ = 0.0;
   hipStream_t st_ = nullptr;
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;

@@ -113,6 +113,25 @@ class Problem:
         check(lib().tmrx_problem_compliance_grad(self.ptr, p, dev), "compliance_grad")
         return g
 
+    def stress_pnorm(self, P=8.0):
+        v = C.c_double()
+        check(lib().tmrx_problem_stress_pnorm(self.ptr, float(P), C.byref(v)), "stress_pnorm")
+        return v.value
+
+    def stress_pnorm_grad(self, P=8.0, g=None, opts=None):
+        opts = opts or default_opts()
+        if g is None:
+            g = np.zeros(self.nelem)
+        p, dev = _ptr(g)
+        info = _lib.SolveInfo()
+        r = lib().tmrx_problem_stress_pnorm_grad(self.ptr, float(P), p, dev, C.byref(opts),
+                                                 C.byref(info))
+        check(r, "stress_pnorm_grad")
+        self.adjoint_info = info
+        if r == 1:
+            raise _lib.TmrxError("stress_pnorm_grad: the adjoint solve did not converge")
+        return g
+
     def apply(self, x, precision=0, y=None):
         if not _is_torch(x):
             x = np.ascontiguousarray(x, dtype=np.double)
