@@ -111,6 +111,13 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
                                      int *ncols, const int **rowp, const int **cols,
                                      const double **weights);
 
+/* TMROctTACSTopoCreator's element densities from a filter forest
+   (src/topology/TMR_TACSTopoCreator.cpp:55-186): rows = elements of f, columns
+   = independent nodes of the filter; rho_e = sum_j w_ej x_j.  Borrowed arrays
+   stored in f (shared with tmrx_forest_create_interpolation's). */
+int tmrx_forest_element_center_weights(tmrx_forest *f, tmrx_forest *filter, int *nrows,
+                                       int *ncols, const int **rowp, const int **cols,
+                                       const double **weights);
 
 /* ---------------------------------------------------------------------------
    Problem: the finite-element hierarchy and solver that TMR hands to TACS.
@@ -183,6 +190,11 @@ int tmrx_problem_solve(tmrx_problem *p, const double *f, double *u, int on_devic
                        const tmrx_solve_opts *opts, tmrx_solve_info *info);
 /* Relative true residual |f - K u| / |f| of the last solve (FP64 operator). */
 int tmrx_problem_true_residual(tmrx_problem *p, double *rel);
+/* Make a stored load case (load f, displacement u; ndof each) the state read by
+   the compliance / stress / gradient calls, as TMRTopoProblem does with
+   assembler->setVariables(vars[i]) before each load case's gradient
+   (src/topology/TMRTopoProblem.cpp:1275-1277). */
+int tmrx_problem_set_state(tmrx_problem *p, const double *f, const double *u, int on_device);
 /* Compliance f^T u of the last solve (src/topology/TMRTopoProblem.cpp:1034). */
 int tmrx_problem_compliance(tmrx_problem *p, double *c);
 /* d compliance / d rho_e for every element: the compliance branch of

@@ -125,3 +125,46 @@ def test_errors_are_reported():
         g.createNodes()
     with pytest.raises(ValueError):
         g.createInterpolation(g)
+
+
+@pytest.mark.parametrize("seed", [0, 4])
+def test_element_center_weights_reproduce_linear_fields(seed):
+    """TMROctTACSTopoCreator's element densities (src/topology/
+    TMR_TACSTopoCreator.cpp:55-186) are the trilinear filter field at each
+    element centre: rows form a partition of unity and reproduce any linear
+    field sampled at the filter's independent nodes (hanging-node weights
+    are exact for linear fields)."""
+    dims = (2, 2, 2)
+    f, _ = build(dims, 1, seed=seed, maxlev=5)
+    c = f.coarsen()
+    c.balance(1)
+    c.createNodes()
+    rp, cl, w, nc = f.createElementCenterWeights(c)
+    assert len(rp) - 1 == len(f.getOctants()) and nc == c.getNodeRange()[1]
+    assert np.all(np.diff(rp) >= 1) and np.all((cl >= 0) & (cl < nc))
+    for e in range(len(rp) - 1):  # sorted, unique columns per row
+        assert np.all(np.diff(cl[rp[e]:rp[e + 1]]) > 0)
+    assert np.allclose(np.add.reduceat(w, rp[:-1]), 1.0, rtol=0, atol=1e-14)
+    ndc = len(c.getDepNodeConn()[0]) - 1
+    Xc = c.getPoints()[ndc:]
+    ndf = len(f.getDepNodeConn()[0]) - 1
+    Xf = f.getPoints()
+    conn = f.getMeshConn()
+    idx = np.where(conn >= 0, conn + ndf, ndf - 1 - (-conn - 1))
+    centres = Xf[idx].mean(axis=1)
+    a = np.array([0.3, -1.7, 2.2])
+    lin = Xc @ a + 0.5
+    rho = np.zeros(len(rp) - 1)
+    np.add.at(rho, np.repeat(np.arange(len(rp) - 1), np.diff(rp)), w * lin[cl])
+    assert np.allclose(rho, centres @ a + 0.5, rtol=0, atol=1e-12)
+
+
+def test_element_center_weights_same_forest():
+    """With the analysis forest as its own filter, each element centre takes
+    1/8 of each corner (hanging corners expanded through their parents)."""
+    f, _ = build((1, 1, 1), 2)
+    g = f.duplicate()
+    g.createNodes()
+    rp, cl, w, nc = f.createElementCenterWeights(g)
+    assert np.all(np.diff(rp) == 8) and np.all(w == 0.125)
+    assert np.array_equal(cl.reshape(-1, 8), np.sort(f.getMeshConn(), axis=1))

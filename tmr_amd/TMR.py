@@ -228,6 +228,24 @@ class OctForest:
             return vec
         return rowp, cols, wts
 
+    def createElementCenterWeights(self, filt):
+        """Element-centre density weights from the filter forest `filt`
+        (TMROctTACSTopoCreator::computeWeights, src/topology/
+        TMR_TACSTopoCreator.cpp:55-186): returns CSR (rowp, cols, weights)
+        with one row per element of this forest and columns = independent
+        filter nodes, so that rho_e = sum_j w_ej x_j."""
+        nr, nc = C.c_int(), C.c_int()
+        rp, cl = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        w = C.POINTER(C.c_double)()
+        check(lib().tmrx_forest_element_center_weights(self.ptr, filt.ptr, C.byref(nr), C.byref(nc),
+                                                       C.byref(rp), C.byref(cl), C.byref(w)),
+              "createElementCenterWeights")
+        rowp = np.ctypeslib.as_array(rp, (nr.value + 1,)).copy()
+        nnz = int(rowp[-1])
+        cols = np.ctypeslib.as_array(cl, (nnz,)).copy() if nnz else np.zeros(0, np.intc)
+        wts = np.ctypeslib.as_array(w, (nnz,)).copy() if nnz else np.zeros(0)
+        return rowp, cols, wts, nc.value
+
 
 def box_connectivity(nx, ny, nz, L=(1.0, 1.0, 1.0)):
     """Block connectivity and vertex points of an nx x ny x nz lattice of box
@@ -270,3 +288,9 @@ def createBoxForest(nx, ny, nz, L=(1.0, 1.0, 1.0), face_names=None, comm=None):
                 if ijk[a] == (dims[a] - 1 if hi else 0):
                     f.setName("face", int(bfc[b, fi]), name)
     return f
+
+
+# topology-optimisation classes (tmr/TMR.pyx:3877-6005) live in topo.py
+from .topo import (BoundaryConditions, ConformFilter, LagrangeFilter, MaterialProperties,  # noqa: E402,F401
+                   OctConformTopoCreator, OctConstitutive, OctTopoCreator, StiffnessProperties,
+                   TopoProblem, createMg, RAMP_PENALTY, SIMP_PENALTY)

@@ -69,6 +69,7 @@ _SIGS = {
     "tmrx_forest_get_nodes_with_name": (I, [P, S, IP, I]),
     "tmrx_forest_get_octs_with_name": (I, [P, S, C.POINTER(Octant), I]),
     "tmrx_forest_create_interpolation": (I, [P, P, IP, IP, CIP, CIP, CDP]),
+    "tmrx_forest_element_center_weights": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "tmrx_problem_create": (I, [I, C.POINTER(P), C.POINTER(Material), I, C.POINTER(S), IP, I,
                                 C.POINTER(P)]),
@@ -79,6 +80,7 @@ _SIGS = {
     "tmrx_problem_setup": (I, [P, C.POINTER(SolveOpts)]),
     "tmrx_problem_solve": (I, [P, P, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "tmrx_problem_true_residual": (I, [P, DP]),
+    "tmrx_problem_set_state": (I, [P, P, P, I]),
     "tmrx_problem_compliance": (I, [P, DP]),
     "tmrx_problem_compliance_grad": (I, [P, P, I]),
     "tmrx_problem_apply": (I, [P, P, P, I, I]),

@@ -231,6 +231,21 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
   return 0;
 }
 
+int tmrx_forest_element_center_weights(tmrx_forest *f, tmrx_forest *filter, int *nrows,
+                                       int *ncols, const int **rowp, const int **cols,
+                                       const double **weights) {
+  CHECK_HANDLE(f);
+  CHECK_HANDLE(filter);
+  if (f->f->create_nodes() != 0) return fail_from(f);
+  if (f->f->element_center_weights(filter->f, f->P)) return fail_from(f);
+  if (nrows) *nrows = f->P.nrows;
+  if (ncols) *ncols = f->P.ncols;
+  if (rowp) *rowp = f->P.rowp.data();
+  if (cols) *cols = f->P.cols.data();
+  if (weights) *weights = f->P.w.data();
+  return 0;
+}
+
 }  // extern "C"
 
 // accessor for the solver side

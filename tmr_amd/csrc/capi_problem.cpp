@@ -173,6 +173,11 @@ int tmrx_problem_true_residual(tmrx_problem *h, double *rel) {
   return 0;
 }
 
+int tmrx_problem_set_state(tmrx_problem *h, const double *f, const double *u, int on_device) {
+  CHECK_P(h);
+  return h->p->set_state(f, u, on_device != 0) ? fail(h) : 0;
+}
+
 int tmrx_problem_compliance(tmrx_problem *h, double *c) {
   CHECK_P(h);
   return h->p->compliance(c) ? fail(h) : 0;

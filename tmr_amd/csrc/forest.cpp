@@ -1493,6 +1493,71 @@ const Oct *Forest::find_enclosing(const Oct &node) const {
   return nullptr;
 }
 
+// TMROctTACSTopoCreator::createElements/computeWeights
+// (src/topology/TMR_TACSTopoCreator.cpp:55-186): the density of an element is
+// the filter field at the element centre -- the filter octant containing the
+// centre, its trilinear shape functions there, filter hanging nodes expanded,
+// merged and sorted by column.  (The reference copies a fixed 8 entries per
+// element after the merge; here every merged entry is kept.)
+int Forest::element_center_weights(Forest *filter, InterpCSR &W) {
+  if (filter->create_nodes() != 0) {
+    last_error = "element_center_weights: filter: " + filter->last_error;
+    return -1;
+  }
+  const auto &fo = filter->octs_;
+  const auto &fconn = filter->conn_;
+  const auto &fdp = filter->dep_ptr_;
+  const auto &fdc = filter->dep_conn_;
+  const auto &fdw = filter->dep_w_;
+  W.nrows = (int)octs_.size();
+  W.ncols = filter->num_owned_;
+  W.rowp.assign(W.nrows + 1, 0);
+  W.cols.clear();
+  W.w.clear();
+  std::vector<std::pair<int, double>> wts;
+  for (size_t i = 0; i < octs_.size(); i++) {
+    const Oct &o = octs_[i];
+    const int32_t h = 1 << (MAX_LEVEL - o.level);
+    Oct c{o.block, o.x + h / 2, o.y + h / 2, o.z + h / 2, 0, (int16_t)MAX_LEVEL, 0};
+    if (o.level == MAX_LEVEL) c = o;
+    const int t = filter->locate(c);
+    if (t < 0) {
+      last_error = "element_center_weights: element centre outside the filter forest";
+      return -1;
+    }
+    const Oct &f = fo[t];
+    const int32_t hf = 1 << (MAX_LEVEL - f.level);
+    const double pc[3] = {-1.0 + 2.0 * ((c.x - f.x) / (double)hf),
+                          -1.0 + 2.0 * ((c.y - f.y) / (double)hf),
+                          -1.0 + 2.0 * ((c.z - f.z) / (double)hf)};
+    wts.clear();
+    const int *cc = &fconn[8 * t];
+    for (int k = 0; k < 8; k++) {
+      double N = 1.0;
+      for (int a = 0; a < 3; a++) N *= ((k >> a) & 1) ? 0.5 * (1.0 + pc[a]) : 0.5 * (1.0 - pc[a]);
+      const int g = cc[k];
+      if (g >= 0) {
+        wts.push_back({g, N});
+      } else {
+        for (int j = fdp[-g - 1]; j < fdp[-g]; j++) wts.push_back({fdc[j], N * fdw[j]});
+      }
+    }
+    std::sort(wts.begin(), wts.end(),
+              [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
+    size_t start = W.cols.size();
+    for (auto &p : wts) {
+      if (W.cols.size() > start && W.cols.back() == p.first)
+        W.w.back() += p.second;
+      else {
+        W.cols.push_back(p.first);
+        W.w.push_back(p.second);
+      }
+    }
+    W.rowp[i + 1] = (int)W.cols.size();
+  }
+  return 0;
+}
+
 // createInterpolation + computeElemInterp (src/TMROctForest.cpp:6396-6793)
 int Forest::create_interpolation(Forest *coarse, InterpCSR &P) {
   if (coarse == this) {

@@ -100,6 +100,9 @@ class Forest {
   int create_interpolation(Forest *coarse, InterpCSR &P);
   // index of the octant whose region contains the lower corner of o, or -1
   int locate(const Oct &o) const;
+  // element-centre interpolation weights from a filter forest: rows =
+  // elements of this forest, columns = independent filter nodes
+  int element_center_weights(Forest *filter, InterpCSR &W);
 
   // physical size of an element along each axis (box blocks only)
   void element_extent(const Oct &o, double h[3]) const;

@@ -413,6 +413,23 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
   TMRX_LAUNCH_CHECK();
 }
 
+__global__ void k_zero_clamped(int n_nodes, const uint8_t *__restrict__ bcmask,
+                               double *__restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  const uint8_t m = bcmask[i];
+  if (m & 1) v[3 * (int64_t)i] = 0.0;
+  if (m & 2) v[3 * (int64_t)i + 1] = 0.0;
+  if (m & 4) v[3 * (int64_t)i + 2] = 0.0;
+}
+
+void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st) {
+  if (n_nodes == 0) return;
+  hipLaunchKernelGGL(k_zero_clamped, dim3((n_nodes + 255) / 256), dim3(256), 0, st, n_nodes,
+                     bcmask, v);
+  TMRX_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // SIMP / RAMP penalty (src/topology/TMROctConstitutive.cpp:498-551) times
 // the element size factor 2^-level.

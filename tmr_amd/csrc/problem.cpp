@@ -955,6 +955,29 @@ int Problem::solve_core(double *rhs, double *xsol, const SolveOpts &o, SolveInfo
   return 0;
 }
 
+// Installs a stored load case (load f, displacement u) as the state that
+// compliance, stress and gradient calls read; the clamped rows of f are
+// zeroed as solve() does.
+int Problem::set_state(const double *f, const double *u, bool on_device) {
+  if (!have_design_) {
+    last_error = "set_state: set the design first";
+    return -1;
+  }
+  if (!f || !u) {
+    last_error = "set_state: null load or displacement";
+    return -1;
+  }
+  Level &L = levels_[0];
+  const int64_t n = L.ndof;
+  const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double), k, st_));
+  HIPCHK(hipMemcpyAsync(u_.p, u, n * sizeof(double), k, st_));
+  launch_zero_clamped(L.n_nodes, L.d_bcmask.p, f_.p, st_);
+  HIPCHK(hipStreamSynchronize(st_));
+  have_solution_ = true;
+  return 0;
+}
+
 int Problem::compliance(double *c) {
   if (!have_solution_) {
     last_error = "compliance: solve first";

@@ -143,6 +143,7 @@ class Problem {
   int set_design(const double *rho, int n, bool on_device);
   int setup(const SolveOpts &o);
   int solve(const double *f, double *u, bool on_device, const SolveOpts &o, SolveInfo *info);
+  int set_state(const double *f, const double *u, bool on_device);
   int compliance(double *c);
   int compliance_grad(double *g, bool on_device);
   // p-norm (exponent P) of the relaxed von Mises failure of the last solve,

@@ -101,6 +101,17 @@ class Problem:
         check(lib().tmrx_problem_true_residual(self.ptr, C.byref(v)))
         return v.value
 
+    def set_state(self, f, u):
+        """Install a stored load case (f, u) as the current state."""
+        if not _is_torch(f):
+            f = np.ascontiguousarray(f, dtype=np.double)
+            u = np.ascontiguousarray(u, dtype=np.double)
+        fp, dev = _ptr(f)
+        up, dev2 = _ptr(u)
+        if dev != dev2:
+            raise ValueError("f and u must both be host or both device arrays")
+        check(lib().tmrx_problem_set_state(self.ptr, fp, up, dev), "set_state")
+
     def compliance(self):
         v = C.c_double()
         check(lib().tmrx_problem_compliance(self.ptr, C.byref(v)), "compliance")

@@ -1,0 +1,551 @@
+"""TMR's topology-optimisation surface over the native GPU problem.
+
+Mirrors the Cython classes a TMR user builds a problem from -- same names,
+argument meaning and error behaviour -- with the TACS objects they wrap
+replaced by the native multigrid-preconditioned PCG (libtmrx.so):
+
+- ``BoundaryConditions``   tmr/TMR.pyx:3877-3930, src/TMRBoundaryConditions.cpp:68-116
+- ``MaterialProperties``   TACS ``constitutive.MaterialProperties(rho, E, nu, ys)``
+- ``StiffnessProperties``  tmr/TMR.pyx:5041-5130 (defaults q=5, eps=0.3,
+  k0=1e-6, RAMP), src/topology/TMROctConstitutive.cpp:26-71
+- ``OctConstitutive``      tmr/TMR.pyx:5201-5211
+- ``OctTopoCreator``       tmr/TMR.pyx:4238-4261, element densities from a
+  filter forest (src/topology/TMR_TACSTopoCreator.cpp:55-371)
+- ``createMg``             tmr/TMR.pyx:4317-4368, src/TMR_RefinementTools.cpp:37-107
+- ``LagrangeFilter``       tmr/TMR.pyx:4776-4815 (design space = filter nodes)
+- ``TopoProblem``          tmr/TMR.pyx:5371-6005, src/topology/TMRTopoProblem.cpp
+
+Design layout: the design vector lives on the independent nodes of the
+finest filter forest; element densities are ``rho = W x`` with W the
+element-centre weights, and every gradient is returned in design space as
+``W^T dF/drho`` (the filter transpose of TMRTopoProblem.cpp:1282).
+
+Vectors are numpy arrays (TACS ``Vec`` / ParOpt ``PVec`` in the reference).
+"""
+import numpy as np
+
+from . import solver
+
+SIMP_PENALTY = solver.SIMP
+RAMP_PENALTY = solver.RAMP
+NATURAL_ORDER = 0
+MULTICOLOR_ORDER = 1
+
+
+class BoundaryConditions:
+    """Named Dirichlet conditions (TMRBoundaryConditions).  ``bc_nums=None``
+    constrains every displacement component of the named nodes (TACS
+    ``addBCs`` with no variable list).  Only homogeneous values are built:
+    the operator keeps identity rows on clamped dofs with zero right-hand
+    side."""
+
+    def __init__(self):
+        self._bcs = []
+
+    def getNumBoundaryConditions(self):
+        return len(self._bcs)
+
+    def addBoundaryCondition(self, aname, bc_nums=None, bc_vals=None):
+        if bc_nums is not None and bc_vals is not None and len(bc_nums) != len(bc_vals):
+            raise ValueError("Boundary condition lists must be the same length")
+        if bc_vals is not None and any(float(v) != 0.0 for v in bc_vals):
+            raise NotImplementedError("non-zero boundary values are not supported")
+        if bc_nums is None or len(bc_nums) == 0:
+            mask = 7
+        else:
+            mask = 0
+            for n in bc_nums:
+                n = int(n)
+                if n < 0 or n > 2:
+                    raise ValueError("boundary condition variable %d out of range [0, 3)" % n)
+                mask |= 1 << n
+        self._bcs.append((aname, mask))
+
+    def as_list(self):
+        """(name, dof bitmask) pairs, bit i = displacement component i."""
+        return [b for b in self._bcs if b[0] is not None]
+
+
+class MaterialProperties:
+    """Isotropic material (TACS MaterialProperties: rho, E, nu, ys)."""
+
+    def __init__(self, rho=1.0, E=1.0, nu=0.3, ys=1.0, **kwargs):
+        self.rho, self.E, self.nu, self.ys = float(rho), float(E), float(nu), float(ys)
+
+    def getDensity(self):
+        return self.rho
+
+
+class StiffnessProperties:
+    """SIMP/RAMP stiffness parametrisation (TMRStiffnessProperties), with the
+    keyword aliases of tmr/TMR.pyx:5047-5110."""
+
+    def __init__(self, props, **kwargs):
+        if isinstance(props, (list, tuple)):
+            if len(props) != 1:
+                raise NotImplementedError("multi-material design is not supported")
+            props = props[0]
+        self.props = props
+        self.nmats = 1
+        self.q = float(kwargs.get("stiffness_penalty_value", kwargs.get("q", 5.0)))
+        self.eps = float(kwargs.get("stress_relax_value", kwargs.get("eps", 0.3)))
+        self.k0 = float(kwargs.get("stiffness_offset", kwargs.get("k0", 1e-6)))
+        pt = kwargs.get("penalty_type", "RAMP")
+        self.penalty_type = SIMP_PENALTY if pt in ("simp", "SIMP", SIMP_PENALTY) else RAMP_PENALTY
+        self.ksWeight = float(kwargs.get("ksWeight", 30.0))
+        self.qmass = float(kwargs.get("mass_penalty_value", kwargs.get("qmass", 0.0)))
+        if kwargs.get("use_project"):
+            raise NotImplementedError("projected densities are not supported")
+
+    def getDesignVarsPerNode(self):
+        return 1
+
+    def design_lower_bound(self):
+        """TMROctConstitutive::getDesignVarRange (TMROctConstitutive.cpp:188-211)."""
+        return 1e-3 if self.penalty_type == SIMP_PENALTY else 0.0
+
+
+class OctConstitutive:
+    """Constitutive object of a topology element (TMROctConstitutive)."""
+
+    def __init__(self, props=None, forest=None):
+        if props is None:
+            raise ValueError("OctConstitutive needs StiffnessProperties")
+        self.props = props
+        self.forest = forest
+
+
+def _props_of(obj):
+    """StiffnessProperties behind what a createElement callback returned."""
+    seen = 0
+    while obj is not None and not isinstance(obj, StiffnessProperties) and seen < 4:
+        obj = getattr(obj, "props", None) or getattr(obj, "con", None) or \
+            getattr(obj, "constitutive", None)
+        seen += 1
+    if not isinstance(obj, StiffnessProperties):
+        raise TypeError("createElement must return an OctConstitutive, StiffnessProperties or an "
+                        "object whose .props/.con leads to one")
+    return obj
+
+
+class Assembler:
+    """One mesh level: the forest, its boundary conditions, the stiffness
+    parametrisation and the element-centre weights from the filter (what
+    TMROctTACSCreator::createTACS, TMR_TACSCreator.cpp:332-461, flattens into
+    a TACSAssembler)."""
+
+    def __init__(self, forest, filt, bcs, props, weights):
+        self.forest, self.filter, self.bcs, self.props = forest, filt, bcs, props
+        self.rowp, self.cols, self.w, self.ndv = weights
+        self.nelem = len(self.rowp) - 1
+        self.nnodes = forest.getNodeRange()[1]
+        self.ndof = 3 * self.nnodes
+        ext = forest.getElementExtents()
+        self.volumes = np.prod(ext, axis=1)
+        self._rows = np.repeat(np.arange(self.nelem), np.diff(self.rowp))
+        self.problem = None  # the native problem, once createMg has built one
+
+    def getNumElements(self):
+        return self.nelem
+
+    def getNumOwnedNodes(self):
+        return self.nnodes
+
+    def getVarsPerNode(self):
+        return 3
+
+    def createVec(self):
+        return np.zeros(self.ndof)
+
+    def createDesignVec(self):
+        return np.zeros(self.ndv)
+
+    def getDesignVars(self, x=None):
+        """Initial design x = 0.95 (TMROctConstitutive.cpp:104-109)."""
+        if x is None:
+            x = self.createDesignVec()
+        x[:] = 0.95
+        return x
+
+    def getDesignVarRange(self, lb=None, ub=None):
+        lb = self.createDesignVec() if lb is None else lb
+        ub = self.createDesignVec() if ub is None else ub
+        lb[:] = self.props.design_lower_bound()
+        ub[:] = 1.0
+        return lb, ub
+
+    def element_densities(self, x):
+        """rho_e = sum_j w_ej x_j."""
+        x = np.asarray(x, dtype=np.double)
+        if x.shape != (self.ndv,):
+            raise ValueError("design vector has %s entries, expected %d" % (x.shape, self.ndv))
+        rho = np.zeros(self.nelem)
+        np.add.at(rho, self._rows, self.w * x[self.cols])
+        return rho
+
+    def weights_transpose(self, g_rho):
+        """W^T g: element sensitivities to design space."""
+        g = np.zeros(self.ndv)
+        np.add.at(g, self.cols, self.w * np.asarray(g_rho)[self._rows])
+        return g
+
+
+class OctTopoCreator:
+    """Creates a level's Assembler with element densities interpolated from
+    the filter forest (TMROctTACSTopoCreator).  A subclass may define
+    ``createElement(order, octant, index, weights)``; its return value (an
+    OctConstitutive, or an element holding one through ``.props``/``.con``)
+    names the StiffnessProperties every element of the level uses.
+    Without ``createElement`` the ``props`` keyword is used."""
+
+    def __init__(self, bcs, filt, design_vars_per_node=1, props=None, *args, **kwargs):
+        if design_vars_per_node != 1:
+            raise NotImplementedError("one design variable per node only")
+        self.bcs, self.filt, self.props = bcs, filt, props
+
+    def getFilter(self):
+        return self.filt
+
+    def createTACS(self, forest, ordering=NATURAL_ORDER):
+        forest.createNodes()
+        weights = forest.createElementCenterWeights(self.filt)
+        props = self.props
+        if hasattr(self, "createElement"):
+            octs = forest.getOctants()
+            rp, cl, w, _ = weights
+            props = _props_of(self.createElement(forest.getMeshOrder(), octs[0],
+                                                 list(cl[rp[0]:rp[1]]), list(w[rp[0]:rp[1]])))
+        if props is None:
+            raise ValueError("no StiffnessProperties: pass props= or define createElement")
+        return Assembler(forest, self.filt, self.bcs, props, weights)
+
+
+class OctConformTopoCreator:
+    """Nodal design variables with per-Gauss-point SIMP
+    (TMROctConformTACSTopoCreator) -- not built: the native operator scales
+    whole elements by one density."""
+
+    def __init__(self, *args, **kwargs):
+        raise NotImplementedError("OctConformTopoCreator: nodal (per-Gauss-point) densities are "
+                                  "not supported; use OctTopoCreator (element densities)")
+
+
+class Mg:
+    """The multigrid-preconditioned solver of a level hierarchy (TACSMg from
+    TMR_CreateTACSMg).  Smoothing is Chebyshev on the Jacobi-scaled operator
+    and the coarsest level is a direct (Cholesky) solve whatever the flags;
+    ``use_galerkin`` (algebraic coarse operators) is not built."""
+
+    def __init__(self, assemblers, forests, omega=1.0, use_galerkin=False,
+                 use_coarse_direct_solve=True, use_chebyshev_smoother=False, device=0):
+        if use_galerkin:
+            raise NotImplementedError("Galerkin coarse operators are not supported; "
+                                      "levels are rediscretised")
+        if len(assemblers) != len(forests) or len(assemblers) < 1:
+            raise ValueError("createMg: list length mismatch")
+        a0 = assemblers[0]
+        p, m = a0.props, a0.props.props
+        self.assemblers, self.forests = list(assemblers), list(forests)
+        self.problem = solver.Problem(self.forests, E=m.E, nu=m.nu, ys=m.ys, q=p.q, k0=p.k0,
+                                      eps=p.eps, penalty_type=p.penalty_type,
+                                      bcs=a0.bcs.as_list(), device=device)
+        self.nlevels = len(assemblers)
+        a0.problem = self.problem
+        self.opts = solver.default_opts()
+
+    def getAssembler(self):
+        return self.assemblers[0]
+
+
+def createMg(assemblers, forests, omega=1.0, use_galerkin=False, use_coarse_direct_solve=True,
+             use_chebyshev_smoother=False):
+    return Mg(assemblers, forests, omega, use_galerkin, use_coarse_direct_solve,
+              use_chebyshev_smoother)
+
+
+class TopoFilter:
+    def __init__(self, assemblers, filters):
+        self.assemblers, self.filters = list(assemblers), list(filters)
+        self.x = None
+
+    def getFilter(self):
+        return self.filters[0]
+
+    def getAssembler(self):
+        return self.assemblers[0]
+
+    def setDesignVars(self, x):
+        self.x = np.array(x, dtype=np.double)
+
+    def getDesignVars(self, x=None):
+        if x is None:
+            return None if self.x is None else self.x.copy()
+        x[:] = self.x
+        return x
+
+    def applyFilter(self, vec_in, vec_out=None):
+        """Element densities of the finest level from a design vector."""
+        rho = self.assemblers[0].element_densities(vec_in)
+        if vec_out is not None:
+            vec_out[:] = rho
+            return vec_out
+        return rho
+
+    def addValues(self, g_rho):
+        """Filter transpose of an element sensitivity."""
+        return self.assemblers[0].weights_transpose(g_rho)
+
+
+class LagrangeFilter(TopoFilter):
+    """Design variables on the finest filter forest's nodes
+    (TMRLagrangeFilter with OctTopoCreator levels)."""
+
+
+class ConformFilter(TopoFilter):
+    def __init__(self, assemblers, filters):
+        raise NotImplementedError("ConformFilter: nodal densities are not supported")
+
+
+# ---------------------------------------------------------------------------
+# functions of the state (TACS Compliance / StructuralMass / stress failure)
+class Compliance:
+    """u . f of a load case."""
+
+
+class StructuralMass:
+    """sum_e V_e rho_e density (TMROctConstitutive::evalDensity with an
+    element-constant rho, TMROctConstitutive.cpp:217-279)."""
+
+
+class PNormFailure:
+    """Relaxed von Mises p-norm (f = (sum_e sum_g V_e/8 fail^P)^(1/P),
+    fail = rho/(eps(1-rho)+rho) vm(C0 B u)/ys)."""
+
+    def __init__(self, P=8.0):
+        self.P = float(P)
+
+
+class TopoProblem:
+    """The optimisation problem (TMRTopoProblem): load cases, a weighted
+    compliance or function objective, constraints ``scale*(f + offset)``,
+    linear constraints and callbacks, evaluated through the native solver."""
+
+    def __init__(self, fltr, pc=None, gmres_subspace=50, rtol=1e-9):
+        if pc is None or not isinstance(pc, Mg):
+            raise ValueError("TopoProblem requires a Mg preconditioner")
+        self.filter, self.mg = fltr, pc
+        self.assembler = fltr.getAssembler()
+        self.problem = pc.problem
+        self.rtol = float(rtol)
+        self.forces, self.vars = [], []
+        self.obj_weights, self.obj_funcs = None, None
+        self.cons = []  # (case, funcs, offset, scale)
+        self.lin_vecs, self.lin_offset = [], []
+        self.objcb = self.objgradcb = None
+        self.concb = []  # (ncon, confunc, congradfunc)
+        self.xinit = self.xlb = self.xub = None
+        self.prefix, self.iter_count = "", 0
+        self.initialized = False
+        self.last_info = []
+
+    # --- configuration ----------------------------------------------------
+    def getAssembler(self):
+        return self.assembler
+
+    def getFilter(self):
+        return self.filter.getFilter()
+
+    def getTopoFilter(self):
+        return self.filter
+
+    def getMg(self):
+        return self.mg
+
+    def setLoadCases(self, forces):
+        self.forces = [None if f is None else np.array(f, dtype=np.double) for f in forces]
+        for f in self.forces:
+            if f is not None and f.shape != (self.assembler.ndof,):
+                raise ValueError("load vector has %s entries, expected %d" % (f.shape,
+                                                                           self.assembler.ndof))
+        self.vars = [np.zeros(self.assembler.ndof) for _ in forces]
+        if self.obj_weights is None:
+            self.obj_weights = np.ones(len(forces))
+
+    def getNumLoadCases(self):
+        return len(self.forces)
+
+    def setObjective(self, weights, funcs=None):
+        if len(weights) != self.getNumLoadCases():
+            raise ValueError("Incorrect number of weights")
+        self.obj_weights = np.array(weights, dtype=np.double)
+        self.obj_funcs = list(funcs) if funcs else None
+
+    def addConstraints(self, case, funcs, offset, scale):
+        if case < 0 or case >= self.getNumLoadCases():
+            raise ValueError("Load case out of expected range")
+        if len(funcs) != len(offset) or len(funcs) != len(scale):
+            raise ValueError("Expected equal function, offset and scale counts")
+        self.cons.append((case, list(funcs), [float(o) for o in offset], [float(s) for s in scale]))
+
+    def addLinearConstraints(self, vecs, offset):
+        if len(vecs) != len(offset):
+            raise ValueError("Expected equal vector and offset counts")
+        self.lin_vecs += [np.array(v, dtype=np.double) for v in vecs]
+        self.lin_offset += [float(o) for o in offset]
+
+    def addObjectiveCallback(self, objfunc, objgradfunc):
+        self.objcb, self.objgradcb = objfunc, objgradfunc
+
+    def addConstraintCallback(self, ncon, nineq, confunc, congradfunc):
+        self.concb.append((int(ncon), confunc, congradfunc))
+
+    def setPrefix(self, prefix):
+        self.prefix = prefix
+
+    def setIterationCounter(self, count):
+        self.iter_count = int(count)
+
+    def setInitDesignVars(self, x, lb=None, ub=None):
+        self.xinit = np.array(x, dtype=np.double)
+        self.xlb = None if lb is None else np.array(lb, dtype=np.double)
+        self.xub = None if ub is None else np.array(ub, dtype=np.double)
+
+    def initialize(self):
+        self.initialized = True
+
+    def getNumConstraints(self):
+        n = len(self.lin_vecs) + sum(len(c[1]) for c in self.cons)
+        return n + sum(c[0] for c in self.concb)
+
+    # --- ParOpt problem interface --------------------------------------------
+    def createDesignVec(self):
+        return self.assembler.createDesignVec()
+
+    def getVarsAndBounds(self, x=None, lb=None, ub=None):
+        """TMRTopoProblem::getVarsAndBounds (TMRTopoProblem.cpp:911-950)."""
+        a = self.assembler
+        x = a.createDesignVec() if x is None else x
+        if self.xinit is not None:
+            x[:] = self.xinit
+        else:
+            a.getDesignVars(x)
+        lb0, ub0 = a.getDesignVarRange()
+        lb = lb0 if lb is None else lb
+        ub = ub0 if ub is None else ub
+        lb[:] = self.xlb if self.xlb is not None else lb0
+        ub[:] = self.xub if self.xub is not None else ub0
+        return x, lb, ub
+
+    def _eval_func(self, func, case):
+        p = self.problem
+        if isinstance(func, Compliance):
+            return float(np.dot(self.vars[case], self.forces[case]))
+        if isinstance(func, StructuralMass):
+            return float(np.dot(self.assembler.volumes, self.rho)) * \
+                self.assembler.props.props.getDensity()
+        if isinstance(func, PNormFailure):
+            self._restore(case)
+            return p.stress_pnorm(func.P)
+        raise TypeError("unsupported function %r" % (func,))
+
+    def _eval_func_grad(self, func, case):
+        """d func / d rho (element sensitivities) for load case `case`."""
+        p = self.problem
+        if isinstance(func, Compliance):
+            self._restore(case)
+            return p.compliance_grad()
+        if isinstance(func, StructuralMass):
+            return self.assembler.volumes * self.assembler.props.props.getDensity()
+        if isinstance(func, PNormFailure):
+            self._restore(case)
+            return p.stress_pnorm_grad(func.P, opts=self._opts())
+        raise TypeError("unsupported function %r" % (func,))
+
+    def _opts(self):
+        return solver.default_opts(rtol=self.rtol, precision=1)
+
+    def _restore(self, case):
+        """Make load case `case` the native problem's current state."""
+        if self._current == case:
+            return
+        self.problem.set_state(self.forces[case], self.vars[case])
+        self._current = case
+
+    def setDesignVars(self, x):
+        """TMRTopoProblem::setDesignVars (TMRTopoProblem.cpp:955-962)."""
+        self.filter.setDesignVars(x)
+        self.rho = self.assembler.element_densities(x)
+        self.problem.set_design(self.rho)
+        self._current = None
+
+    def evalObjCon(self, ncon, x):
+        """TMRTopoProblem::evalObjCon (TMRTopoProblem.cpp:967-1060): returns
+        (fail, fobj, cons)."""
+        self.setDesignVars(x)
+        fobj = 0.0
+        cons = np.zeros(ncon)
+        k = 0
+        for v, o in zip(self.lin_vecs, self.lin_offset):
+            cons[k] = o + float(np.dot(v, x))
+            k += 1
+        if self.objcb is not None:
+            fobj = float(self.objcb(self.filter, self.mg))
+        fail = 0
+        self.last_info = []
+        if self.forces:
+            self.problem.setup(self._opts())
+            for i, f in enumerate(self.forces):
+                if f is None:
+                    continue
+                r = self.problem.solve(f, opts=self._opts(), allow_unconverged=True)
+                self.vars[i] = np.array(r)
+                self._current = i
+                info = self.problem.info
+                self.last_info.append((info.iters, info.converged))
+                if not info.converged:
+                    fail = 1
+                if self.obj_funcs and self.obj_funcs[i] is not None:
+                    fobj += self.obj_weights[i] * self._eval_func(self.obj_funcs[i], i)
+                elif not self.obj_funcs:
+                    fobj += self.obj_weights[i] * float(np.dot(self.vars[i], f))
+                for case, funcs, offs, scales in self.cons:
+                    if case != i:
+                        continue
+                    for fn, o, s in zip(funcs, offs, scales):
+                        cons[k] = s * (self._eval_func(fn, i) + o)
+                        k += 1
+        for ncb, cf, _ in self.concb:
+            vals = cf(self.filter, self.mg)
+            cons[k:k + ncb] = vals
+            k += ncb
+        self.x = np.array(x, dtype=np.double)
+        return fail, fobj, cons
+
+    def evalObjConGradient(self, x, g, A):
+        """TMRTopoProblem::evalObjConGradient (TMRTopoProblem.cpp:1216-1295):
+        fills the design-space gradient g and the constraint gradients A."""
+        a = self.assembler
+        g[:] = 0.0
+        if self.objgradcb is not None:
+            self.objgradcb(self.filter, self.mg, g)
+        grho = np.zeros(a.nelem)
+        for i, f in enumerate(self.forces):
+            if f is None or self.obj_weights[i] == 0.0:
+                continue
+            fn = self.obj_funcs[i] if self.obj_funcs else Compliance()
+            if fn is None:
+                continue
+            grho += self.obj_weights[i] * self._eval_func_grad(fn, i)
+        g += a.weights_transpose(grho)
+        k = 0
+        for v in self.lin_vecs:
+            A[k][:] = v
+            k += 1
+        for case, funcs, offs, scales in self.cons:
+            for fn, s in zip(funcs, scales):
+                A[k][:] = s * a.weights_transpose(self._eval_func_grad(fn, case))
+                k += 1
+        for ncb, _, cgf in self.concb:
+            cgf(self.filter, self.mg, A[k:k + ncb])
+            k += ncb
+        return 0
