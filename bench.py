@@ -21,8 +21,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-BYTES_PER_ELEM_FP32 = 60.5   # x + y per node (~1.02 nodes/elem), rho, 8-int connectivity
+# algorithmic bytes of one finest-level Chebyshev step launch in FP32: per node
+# x, b, d, dinv read and d, x_out written (6 x 3 floats) plus the clamp mask
+# byte; per element the SIMP scale (float) and the 8-int connectivity
+CHEB_BYTES_PER_NODE = 6 * 3 * 4 + 1
+CHEB_BYTES_PER_ELEM = 4 + 8 * 4
 FLOPS_PER_ELEM = 1152.0      # 2 * 24 * 24
+TIME_CHEB = 1 << 2           # launch-kind mask: Chebyshev update
 
 
 def log(*a):
@@ -73,6 +78,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-level", type=int, default=5)
+    ap.add_argument("--fp32-valu", action="store_true",
+                    help="FP32 operator on the vector ALUs instead of the matrix cores")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -97,7 +104,8 @@ def main():
     prob.set_design(rho)
 
     def run(precision, steps, warmup, timing=False):
-        opts = solver.default_opts(rtol=1e-8, precision=precision, max_iters=500)
+        opts = solver.default_opts(rtol=1e-8, precision=precision, max_iters=500,
+                                   fp32_valu=int(args.fp32_valu))
         torch.cuda.synchronize()
         ts = time.perf_counter()
         prob.setup(opts)
@@ -106,7 +114,7 @@ def main():
         for _ in range(warmup):
             prob.solve(f, u, opts=opts)
         if timing:
-            prob.kernel_timing(True)
+            prob.kernel_timing(TIME_CHEB)
         torch.cuda.synchronize()
         ts = time.perf_counter()
         iters = []
@@ -135,7 +143,9 @@ def main():
     mdof = ndof * sum(mixed["iters"]) / mixed["seconds"] / 1e6
     tot_ms, cnt, _ = mixed["kt"]
     avg_s = tot_ms / max(cnt, 1) / 1e3
-    achieved = BYTES_PER_ELEM_FP32 * prob.nelem / avg_s / 1e9
+    nnodes = ndof // 3
+    bytes_launch = CHEB_BYTES_PER_NODE * nnodes + CHEB_BYTES_PER_ELEM * prob.nelem
+    achieved = bytes_launch / avg_s / 1e9
     out = {
         "metric": "solves/s + MDOF*iter/s: mixed-prec vs FP64 MG-PCG to 1e-8, 4.2M-hex SIMP",
         "value": solves_s,
@@ -160,7 +170,9 @@ def main():
         "setup_ms": mixed["setup_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_elem_apply<float> (finest level)",
+                     "kernel": ("k_tile_apply<float, ASM_CHEB>" if args.fp32_valu else
+                                "k_tile_apply_mfma<ASM_CHEB>") + " (finest-level Chebyshev step)",
+                     "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt,
                      "mfma_or_valu_tflops": FLOPS_PER_ELEM * prob.nelem / avg_s / 1e12,
                      "fp32_peak_tflops": FP32_PEAK_TFLOPS},

@@ -143,6 +143,7 @@ typedef struct {
   int eig_iters;    /* power iterations for lambda_max(D^-1 A) */
   int monitor;      /* print the residual history to stderr */
   int zero_guess;   /* start from u = 0 */
+  int fp32_valu;    /* FP32 operator on the vector ALUs instead of the matrix cores */
 } tmrx_solve_opts;
 
 typedef struct {
@@ -215,7 +216,8 @@ int tmrx_problem_stress_pnorm(tmrx_problem *p, double P, double *value);
    Returns 1 if the adjoint solve did not converge. */
 int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_device,
                                    const tmrx_solve_opts *opts, tmrx_solve_info *info);
-/* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32). */
+/* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32 on the matrix
+   cores, 2 FP32 on the vector ALUs). */
 int tmrx_problem_apply(tmrx_problem *p, const double *x, double *y, int on_device, int precision);
 /* Consistent nodal load of a constant traction tr[3] on the faces named
    `name`: TopOptUtils.compute3DTractionLoad (tmr/TopOptUtils.py:241-267).
@@ -223,9 +225,11 @@ int tmrx_problem_apply(tmrx_problem *p, const double *x, double *y, int on_devic
 int tmrx_problem_traction_load(tmrx_problem *p, const char *name, const double tr[3], double *f);
 
 /* Measurement hooks (no counterpart in the reference, which records no
-   timings): HIP events around every finest-level element-pass launch on the
-   problem's stream while enabled.  result: summed milliseconds, launch count
-   and a mask of the precisions seen (1 FP32, 2 FP64). */
+   timings): HIP events around the finest-level operator launches on the
+   problem's stream.  enable = 0 stops; otherwise it is a mask of the launch
+   kinds to time: 1 plain product, 2 residual b - A x, 4 Chebyshev update,
+   16 product with dot, -1 all.  result: summed milliseconds, launch count and
+   a mask of the precisions seen (1 FP32, 2 FP64). */
 int tmrx_problem_kernel_timing(tmrx_problem *p, int enable);
 int tmrx_problem_kernel_timing_result(tmrx_problem *p, double *total_ms, int *count,
                                       int *precision_mask);

@@ -26,7 +26,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("max_iters", C.c_int),
                 ("precision", C.c_int), ("cheb_degree", C.c_int),
                 ("cheb_lower", C.c_double), ("cheb_upper", C.c_double),
-                ("eig_iters", C.c_int), ("monitor", C.c_int), ("zero_guess", C.c_int)]
+                ("eig_iters", C.c_int), ("monitor", C.c_int), ("zero_guess", C.c_int),
+                ("fp32_valu", C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -31,6 +31,7 @@ static tmrx::SolveOpts to_opts(const tmrx_solve_opts *o) {
   s.eig_iters = o->eig_iters;
   s.monitor = o->monitor;
   s.zero_guess = o->zero_guess;
+  s.fp32_valu = o->fp32_valu;
   return s;
 }
 
@@ -63,6 +64,7 @@ void tmrx_solve_opts_default(tmrx_solve_opts *o) {
   o->eig_iters = s.eig_iters;
   o->monitor = s.monitor;
   o->zero_guess = s.zero_guess;
+  o->fp32_valu = s.fp32_valu;
 }
 
 int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat, int nbcs,

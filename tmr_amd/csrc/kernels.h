@@ -82,6 +82,13 @@ struct Tiles {
   const int *dep_conn = nullptr;
   const float *dep_w = nullptr;
   const uint8_t *bcmask = nullptr;
+  // matrix-core (FP32) form: records in batches of 16 of one colour and
+  // class (padded with mask-0 records), batch offsets per tile and colour,
+  // and K0 of every class split into bf16 hi/lo MFMA A fragments
+  const int *mb_ptr = nullptr;    // [n_tiles * (MAX_COLORS + 1)]
+  const TileElem *mel = nullptr;  // [16 * batches]
+  const uint4 *afrag = nullptr;   // [n_classes][hi, lo][row block][64 lanes]
+  bool use_mfma = false;
 };
 
 

@@ -715,6 +715,85 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 
 // ---------------------------------------------------------------------------
 // owner-tile apply: one pass, final values of the owned nodes, fused epilogue
+
+// stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
+// hanging-node interpolation, clamped dofs zero) and zero the accumulators
+template <class T>
+__device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
+                                           int nacc, const int *__restrict__ dep_ptr,
+                                           const int *__restrict__ dep_conn,
+                                           const float *__restrict__ dep_w,
+                                           const uint8_t *__restrict__ bcmask,
+                                           const T *__restrict__ x, T *xs, T *ys) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
+    const int j = i / 3, c = i - 3 * j;
+    xs[3 * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
+  }
+  for (int i = tid; i < next; i += TILE_THREADS) {
+    T v[3];
+    load_node<T>(ext_l[i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
+    const int q = 3 * tile_pos(nown + i);
+    xs[q] = v[0];
+    xs[q + 1] = v[1];
+    xs[q + 2] = v[2];
+  }
+  for (int i = tid; i < 3 * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
+}
+
+// owned rows: add the hanging-node sums, then the mode's epilogue
+template <class T, int MODE>
+__device__ __forceinline__ void tile_finish(
+    int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
+    const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
+    const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out,
+    T *__restrict__ d, const T *__restrict__ dinv, double ca, double cc,
+    double *__restrict__ partials, double *red) {
+  double dot = 0.0;
+  for (int i = threadIdx.x; i < nown; i += TILE_THREADS) {
+    const int node = n0 + i;
+    const int pi = 3 * tile_pos(i);
+    T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
+    for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
+      const int sl = comb_slot[j];
+      const T w = (T)comb_w[j];
+      a[0] += w * ys[3 * sl];
+      a[1] += w * ys[3 * sl + 1];
+      a[2] += w * ys[3 * sl + 2];
+    }
+    const uint8_t m = bcmask[node];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)node + c;
+      const bool bc = (m >> c) & 1;
+      if (MODE == ASM_STORE) {
+        y[k] = bc ? x[k] : a[c];
+      } else if (MODE == ASM_DOT) {
+        const T xv = bc ? x[k] : xs[pi + c];
+        const T v = bc ? xv : a[c];
+        y[k] = v;
+        dot += (double)xv * (double)v;
+      } else if (MODE == ASM_RESID) {
+        y[k] = bc ? T(0) : b[k] - a[c];
+      } else if (MODE == ASM_CHEB) {
+        if (bc) {
+          d[k] = T(0);
+          x_out[k] = T(0);
+        } else {
+          const T r = b[k] - a[c];
+          const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
+          d[k] = dn;
+          x_out[k] = xs[pi + c] + dn;
+        }
+      }
+    }
+  }
+  if (MODE == ASM_DOT) {
+    const double sum = block_sum(dot, red);
+    if (threadIdx.x == 0) partials[t] = sum;
+  }
+}
+
 template <class T, int MODE, bool SINGLE_CLASS>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
@@ -736,19 +815,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   const int nacc = nown + nacc_dep[t];
   T *xs = reinterpret_cast<T *>(smem_raw);
   T *ys = xs + 3 * ((nslot + 7) & ~7);
-  for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
-    const int j = i / 3, c = i - 3 * j;
-    xs[3 * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
-  }
-  for (int i = tid; i < next; i += TILE_THREADS) {
-    T v[3];
-    load_node<T>(ext[e0 + i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
-    const int q = 3 * tile_pos(nown + i);
-    xs[q] = v[0];
-    xs[q + 1] = v[1];
-    xs[q + 2] = v[2];
-  }
-  for (int i = tid; i < 3 * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
+  tile_stage<T>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
   __syncthreads();
   const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
   for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
@@ -808,49 +875,163 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       __syncthreads();
     }
   }
-  double dot = 0.0;
-  for (int i = tid; i < nown; i += TILE_THREADS) {
-    const int node = n0 + i;
-    const int pi = 3 * tile_pos(i);
-    T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
-    for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
-      const int sl = comb_slot[j];
-      const T w = (T)comb_w[j];
-      a[0] += w * ys[3 * sl];
-      a[1] += w * ys[3 * sl + 1];
-      a[2] += w * ys[3 * sl + 2];
-    }
-    const uint8_t m = bcmask[node];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const size_t k = 3 * (size_t)node + c;
-      const bool bc = (m >> c) & 1;
-      if (MODE == ASM_STORE) {
-        y[k] = bc ? x[k] : a[c];
-      } else if (MODE == ASM_DOT) {
-        const T xv = bc ? x[k] : xs[pi + c];
-        const T v = bc ? xv : a[c];
-        y[k] = v;
-        dot += (double)xv * (double)v;
-      } else if (MODE == ASM_RESID) {
-        y[k] = bc ? T(0) : b[k] - a[c];
-      } else if (MODE == ASM_CHEB) {
-        if (bc) {
-          d[k] = T(0);
-          x_out[k] = T(0);
-        } else {
-          const T r = b[k] - a[c];
-          const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
-          d[k] = dn;
-          x_out[k] = xs[pi + c] + dn;
-        }
+  tile_finish<T, MODE>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y, x_out,
+                       d, dinv, ca, cc, partials, red);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 owner-tile apply on the matrix cores.  A wave takes 16 elements of one
+// colour and class at a time and forms Y (24 x 16) = K0 (24 x 24) X (24 x 16)
+// with v_mfma_f32_16x16x32_bf16: rows padded to two 16-row blocks, K padded
+// to 32, each FP32 operand split into bf16 hi + lo and the product taken as
+// hi*hi + hi*lo + lo*hi (about 16 significant bits per product, FP32
+// accumulation).  Lane l holds element n = l % 16 and the K (input dof) /
+// M (output row) group g = l / 16: B[k][n] for k = 8g .. 8g+7 and
+// D[4g + v][n] for v = 0..3 of each row block.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// corner / component of input dof 8g + v (v compile-time, g = lane group)
+template <int V>
+__device__ __forceinline__ int kslot(int g, const int lc[8]) {
+  constexpr int c0 = V / 3, c1 = (8 + V) / 3, c2 = (16 + V) / 3;
+  constexpr int p0 = V % 3, p1 = (8 + V) % 3, p2 = (16 + V) % 3;
+  return g == 0 ? 3 * lc[c0] + p0 : g == 1 ? 3 * lc[c1] + p1 : 3 * lc[c2] + p2;
+}
+
+// add one output row of an element: row 16 BLK + 4g + V of lane group g
+template <int V, int BLK>
+__device__ __forceinline__ void rscatter(int g, const int lc[8], int mask, float val, float *ys) {
+  constexpr int r0 = 16 * BLK + V, r1 = r0 + 4, r2 = r0 + 8, r3 = r0 + 12;
+  int cr, comp;
+  if (BLK == 0) {
+    cr = g == 0 ? r0 / 3 : g == 1 ? r1 / 3 : g == 2 ? r2 / 3 : r3 / 3;
+    comp = g == 0 ? r0 % 3 : g == 1 ? r1 % 3 : g == 2 ? r2 % 3 : r3 % 3;
+  } else {
+    cr = g == 0 ? r0 / 3 : r1 / 3;
+    comp = g == 0 ? r0 % 3 : r1 % 3;
+    if (g >= 2) return;
+  }
+  const int sl = g == 0 ? lc[r0 / 3] : g == 1 ? lc[r1 / 3] : g == 2 ? lc[(r2 < 24 ? r2 : 0) / 3]
+                                                                    : lc[(r3 < 24 ? r3 : 0) / 3];
+  if (mask & (1 << cr)) ys[3 * sl + comp] += val;
+}
+
+template <int MODE, bool SINGLE_CLASS>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
+    const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
+    const int *__restrict__ nacc_dep, const int *__restrict__ mb_ptr,
+    const uint2 *__restrict__ mel, int ncolors, const uint4 *__restrict__ afrag,
+    const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
+    const float *__restrict__ comb_w, const int *__restrict__ dep_ptr,
+    const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+    const uint8_t *__restrict__ bcmask, const float *__restrict__ s_elem,
+    const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
+    float *__restrict__ x_out, float *__restrict__ d, const float *__restrict__ dinv, double ca,
+    double cc, double *__restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[TILE_THREADS / 64];
+  const int t = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int en = lane & 15, g = lane >> 4;
+  const int n0 = node0[t], nown = node0[t + 1] - n0;
+  const int e0 = ext_ptr[t], next = ext_ptr[t + 1] - e0;
+  const int nslot = nown + next;
+  const int nacc = nown + nacc_dep[t];
+  float *xs = reinterpret_cast<float *>(smem_raw);
+  float *ys = xs + 3 * ((nslot + 7) & ~7);
+  tile_stage<float>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  // A fragments of K0: [class][hi, lo][row block][lane]
+  union Frag {
+    uint4 u;
+    bf16x8_t v;
+  };
+  Frag ahi0, ahi1, alo0, alo1;
+  if (SINGLE_CLASS) {
+    ahi0.u = afrag[0 * 64 + lane];
+    ahi1.u = afrag[1 * 64 + lane];
+    alo0.u = afrag[2 * 64 + lane];
+    alo1.u = afrag[3 * 64 + lane];
+  }
+  __syncthreads();
+  const int *cp = mb_ptr + (size_t)t * (MAX_COLORS + 1);
+  for (int c = 0; c < ncolors; c++) {
+    const int b0 = cp[c], b1 = cp[c + 1];
+    for (int bt = b0 + wave; bt < b1; bt += TILE_THREADS / 64) {
+      const size_t rec = 3 * ((size_t)bt * 16 + en);
+      const uint2 w0 = mel[rec], w1 = mel[rec + 1], w2 = mel[rec + 2];
+      int lc[8];
+      lc[0] = w0.x & 0xffff; lc[1] = w0.x >> 16;
+      lc[2] = w0.y & 0xffff; lc[3] = w0.y >> 16;
+      lc[4] = w1.x & 0xffff; lc[5] = w1.x >> 16;
+      lc[6] = w1.y & 0xffff; lc[7] = w1.y >> 16;
+      const int elem = (int)w2.x;
+      const int mask = w2.y & 0xff;
+      if (!SINGLE_CLASS) {
+        const int cls = __builtin_amdgcn_readfirstlane((int)((w2.y >> 16) & 0xff));
+        const uint4 *af = afrag + (size_t)cls * 256;
+        ahi0.u = af[0 * 64 + lane];
+        ahi1.u = af[1 * 64 + lane];
+        alo0.u = af[2 * 64 + lane];
+        alo1.u = af[3 * 64 + lane];
       }
+      // B fragment: x of dofs 8g .. 8g+7 of element en (none for g = 3)
+      float xv[8];
+      if (g < 3) {
+        xv[0] = xs[kslot<0>(g, lc)];
+        xv[1] = xs[kslot<1>(g, lc)];
+        xv[2] = xs[kslot<2>(g, lc)];
+        xv[3] = xs[kslot<3>(g, lc)];
+        xv[4] = xs[kslot<4>(g, lc)];
+        xv[5] = xs[kslot<5>(g, lc)];
+        xv[6] = xs[kslot<6>(g, lc)];
+        xv[7] = xs[kslot<7>(g, lc)];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 8; v++) xv[v] = 0.0f;
+      }
+      Frag bhi, blo;
+      uint32_t hw[4], lw[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t h0 = bf16_bits(xv[2 * v]), h1 = bf16_bits(xv[2 * v + 1]);
+        const uint32_t l0 = bf16_bits(xv[2 * v] - __uint_as_float(h0 << 16));
+        const uint32_t l1 = bf16_bits(xv[2 * v + 1] - __uint_as_float(h1 << 16));
+        hw[v] = h0 | (h1 << 16);
+        lw[v] = l0 | (l1 << 16);
+      }
+      bhi.u = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+      blo.u = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+      f32x4_t acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo0.v, bhi.v, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo1.v, bhi.v, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi0.v, blo.v, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi1.v, blo.v, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi0.v, bhi.v, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi1.v, bhi.v, acc1, 0, 0, 0);
+      const float sc = s_elem[elem];
+      // scatter rows 4g + v (block 0) and 16 + 4g + v (block 1, g < 2):
+      // row r -> corner r / 3, component r % 3; only the tile's corners
+      rscatter<0, 0>(g, lc, mask, sc * acc0[0], ys);
+      rscatter<1, 0>(g, lc, mask, sc * acc0[1], ys);
+      rscatter<2, 0>(g, lc, mask, sc * acc0[2], ys);
+      rscatter<3, 0>(g, lc, mask, sc * acc0[3], ys);
+      rscatter<0, 1>(g, lc, mask, sc * acc1[0], ys);
+      rscatter<1, 1>(g, lc, mask, sc * acc1[1], ys);
+      rscatter<2, 1>(g, lc, mask, sc * acc1[2], ys);
+      rscatter<3, 1>(g, lc, mask, sc * acc1[3], ys);
     }
+    __syncthreads();
   }
-  if (MODE == ASM_DOT) {
-    const double sum = block_sum(dot, red);
-    if (threadIdx.x == 0) partials[t] = sum;
-  }
+  tile_finish<float, MODE>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
+                           x_out, d, dinv, ca, cc, partials, red);
 }
 
 template <class T>
@@ -862,6 +1043,33 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
   const T *dinv = (const T *)g.dinv;
   const bool single = tl.single_class;
+  if constexpr (sizeof(T) == 4) {
+    if (tl.use_mfma) {
+#define TMRX_MTILE(M, S)                                                                         \
+  hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), dim3(tl.n_tiles), dim3(TILE_THREADS), sh, st,    \
+                     tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.mb_ptr,                       \
+                     reinterpret_cast<const uint2 *>(tl.mel), tl.ncolors, tl.afrag, tl.comb_ptr, \
+                     tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,      \
+                     (const float *)s_elem, (const float *)x, (const float *)b, (float *)y,      \
+                     (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.partials)
+#define TMRX_MTILE2(M)    \
+  if (single) {           \
+    TMRX_MTILE(M, true);  \
+  } else {                \
+    TMRX_MTILE(M, false); \
+  }
+      switch (mode) {
+        case ASM_STORE: TMRX_MTILE2(ASM_STORE); break;
+        case ASM_RESID: TMRX_MTILE2(ASM_RESID); break;
+        case ASM_CHEB: TMRX_MTILE2(ASM_CHEB); break;
+        case ASM_DOT: TMRX_MTILE2(ASM_DOT); break;
+      }
+#undef TMRX_MTILE2
+#undef TMRX_MTILE
+      TMRX_LAUNCH_CHECK();
+      return;
+    }
+  }
 #define TMRX_TILE(M, S)                                                                         \
   hipLaunchKernelGGL((k_tile_apply<T, M, S>), dim3(tl.n_tiles), dim3(TILE_THREADS), sh, st,     \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                      \

@@ -30,7 +30,7 @@ Problem::~Problem() {
 }
 
 int Problem::kernel_timing(int enable) {
-  timing_ = enable != 0;
+  timing_ = enable;
   ev_used_ = 0;
   ev_prec_mask_ = 0;
   return 0;
@@ -145,7 +145,30 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     }
   }
   std::vector<float> K0f(K0h_.begin(), K0h_.end());
-  if (K0d_.upload(K0h_) || K0f_.upload(K0f)) {
+  // MFMA A fragments of the FP32 K0: [class][hi, lo][row block][lane][8 bf16],
+  // lane l holds rows 16 blk + l % 16, columns 8 (l / 16) + v, zero padded
+  std::vector<uint16_t> af(2048 * (size_t)nclasses_, 0);
+  auto bf16 = [](float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+  };
+  for (int c = 0; c < nclasses_; c++)
+    for (int blk = 0; blk < 2; blk++)
+      for (int l = 0; l < 64; l++)
+        for (int v = 0; v < 8; v++) {
+          const int row = 16 * blk + l % 16, col = 8 * (l / 16) + v;
+          const float k = (row < 24 && col < 24) ? K0f[576 * c + 24 * row + col] : 0.0f;
+          const uint16_t hi = bf16(k);
+          uint32_t hb = (uint32_t)hi << 16;
+          float hf;
+          std::memcpy(&hf, &hb, 4);
+          const uint16_t lo = bf16(k - hf);
+          af[2048 * (size_t)c + 512 * (0 + blk) + 8 * l + v] = hi;
+          af[2048 * (size_t)c + 512 * (2 + blk) + 8 * l + v] = lo;
+        }
+  if (K0d_.upload(K0h_) || K0f_.upload(K0f) || afrag_.upload(af)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -352,7 +375,8 @@ int Problem::build_tiles(int l) {
   }
   const int ntiles = (n + TILE_NODES - 1) / TILE_NODES;
   std::vector<int> node0(ntiles + 1), ext_ptr{0}, ext, nacc(ntiles), el_ptr{0};
-  std::vector<TileElem> els;
+  std::vector<TileElem> els, mel;
+  std::vector<int> mb_ptr((size_t)ntiles * (MAX_COLORS + 1), 0);
   std::vector<int> comb_ptr(n + 1, 0), comb_slot;
   std::vector<float> comb_w;
   std::vector<int> stamp_n(n + nd, -1), slot(n + nd, 0), stamp_e(ne, -1);
@@ -439,6 +463,29 @@ int Problem::build_tiles(int l) {
     std::stable_sort(els.begin() + el0, els.end(), [](const TileElem &a, const TileElem &b) {
       return a.color != b.color ? a.color < b.color : a.mask > b.mask;
     });
+    // matrix-core batches: (colour, class) groups padded to 16 records
+    {
+      std::vector<TileElem> grp(els.begin() + el0, els.end());
+      std::stable_sort(grp.begin(), grp.end(), [](const TileElem &a, const TileElem &b) {
+        return a.color != b.color ? a.color < b.color : a.cls < b.cls;
+      });
+      int *cp = &mb_ptr[(size_t)t * (MAX_COLORS + 1)];
+      size_t k = 0;
+      for (int c = 0; c <= MAX_COLORS; c++) {
+        cp[c] = (int)(mel.size() / 16);
+        while (k < grp.size() && grp[k].color == c) {
+          const uint8_t cl = grp[k].cls;
+          size_t k1 = k;
+          while (k1 < grp.size() && grp[k1].color == c && grp[k1].cls == cl) mel.push_back(grp[k1++]);
+          TileElem pad{};
+          pad.elem = grp[k].elem;
+          pad.color = (uint8_t)c;
+          pad.cls = cl;
+          while (mel.size() % 16) mel.push_back(pad);
+          k = k1;
+        }
+      }
+    }
     nacc[t] = (int)rdeps.size();
     ext_ptr.push_back((int)ext.size());
     el_ptr.push_back((int)els.size());
@@ -461,7 +508,7 @@ int Problem::build_tiles(int l) {
   if (L.t_node0.upload(node0) || L.t_ext_ptr.upload(ext_ptr) || L.t_ext.upload(ext) ||
       L.t_nacc.upload(nacc) || L.t_el_ptr.upload(el_ptr) || L.t_el.upload(els) ||
       L.t_comb_ptr.upload(comb_ptr) || L.t_comb_slot.upload(comb_slot) ||
-      L.t_comb_w.upload(comb_w)) {
+      L.t_comb_w.upload(comb_w) || L.t_mb_ptr.upload(mb_ptr) || L.t_mel.upload(mel)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -484,6 +531,10 @@ int Problem::build_tiles(int l) {
   T.dep_conn = L.dep_conn.p;
   T.dep_w = L.dep_w.p;
   T.bcmask = L.d_bcmask.p;
+  T.mb_ptr = L.t_mb_ptr.p;
+  T.mel = L.t_mel.p;
+  T.afrag = reinterpret_cast<const uint4 *>(afrag_.p);
+  T.use_mfma = true;
   return 0;
 }
 
@@ -606,7 +657,7 @@ template <class T>
 void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
   Level &L = levels_[l];
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
-  if (timing_ && l == 0) {
+  if (l == 0 && ((timing_ >> mode) & 1)) {
     if (ev_used_ == ev_pool_.size()) {
       std::pair<hipEvent_t, hipEvent_t> e;
       if (hipEventCreate(&e.first) == hipSuccess && hipEventCreate(&e.second) == hipSuccess)
@@ -617,8 +668,10 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
   }
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
+  Tiles tl = L.tiles;
+  tl.use_mfma = tl.use_mfma && !fp32_valu_;
   if (ev) (void)hipEventRecord(ev->first, st_);
-  launch_tile_apply<T>(L.tiles, K0<T>(), scale<T>(l), x, mode, g, st_);
+  launch_tile_apply<T>(tl, K0<T>(), scale<T>(l), x, mode, g, st_);
   if (ev) (void)hipEventRecord(ev->second, st_);
 }
 
@@ -863,6 +916,7 @@ int Problem::solve_core(double *rhs, double *xsol, const SolveOpts &o, SolveInfo
   if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
     if (setup(o)) return -1;
   }
+  fp32_valu_ = o.fp32_valu != 0;
   auto t0 = std::chrono::steady_clock::now();
   Level &L0 = levels_[0];
   const int64_t n = L0.ndof;
@@ -1110,9 +1164,12 @@ int Problem::apply(const double *x, double *y, bool on_device, int precision) {
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
   AsmArgs a;
   if (precision) {
+    const bool keep = fp32_valu_;
+    fp32_valu_ = precision == 2;
     launch_convert<double, float>(n, p_.p, L.v32.b.p, st_);
     a.y = L.v32.r.p;
     apply_level<float>(0, L.v32.b.p, ASM_STORE, a);
+    fp32_valu_ = keep;
     launch_convert<float, double>(n, L.v32.r.p, q_.p, st_);
   } else {
     a.y = q_.p;

@@ -78,6 +78,7 @@ struct SolveOpts {
   int eig_iters = 10;
   int monitor = 0;
   int zero_guess = 1;
+  int fp32_valu = 0;  // FP32 operator on the vector ALUs (else matrix cores)
 };
 
 struct SolveInfo {
@@ -114,6 +115,8 @@ struct Level {
   ElemBatches eb;
   AsmCSR asmc;
   // owner tiles (single-pass apply)
+  DevArray<int> t_mb_ptr;
+  DevArray<TileElem> t_mel;
   DevArray<int> t_node0, t_ext_ptr, t_ext, t_nacc, t_el_ptr, t_comb_ptr, t_comb_slot;
   DevArray<TileElem> t_el;
   DevArray<float> t_comb_w;
@@ -192,6 +195,7 @@ class Problem {
   std::vector<double> K0h_;  // per class, full-block element
   DevArray<double> K0d_;
   DevArray<float> K0f_;
+  DevArray<uint16_t> afrag_;  // bf16 hi/lo MFMA fragments of K0f_
   DevArray<double> rho_;
   DevArray<double> Ainv_;
   int n_coarse_dof_ = 0;
@@ -203,7 +207,8 @@ class Problem {
   double last_setup_ms_ = 0.0;
   double last_stress_sum_ = 0.0;
   hipStream_t st_ = nullptr;
-  bool timing_ = false;
+  bool fp32_valu_ = false;  // FP32 operator on the vector ALUs
+  int timing_ = 0;  // mask of the timed launch modes (bit m = AsmMode m)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_used_ = 0;
   int ev_prec_mask_ = 0;

@@ -153,8 +153,11 @@ class Problem:
         check(lib().tmrx_problem_apply(self.ptr, xp, yp, dev, int(precision)), "apply")
         return y
 
-    def kernel_timing(self, enable=True):
-        check(lib().tmrx_problem_kernel_timing(self.ptr, int(enable)))
+    def kernel_timing(self, mask=-1):
+        """Time the finest-level operator launches whose kind is in `mask`
+        (1 product, 2 residual, 4 Chebyshev update, 16 product with dot, -1
+        all); False/0 stops."""
+        check(lib().tmrx_problem_kernel_timing(self.ptr, int(mask)))
 
     def kernel_timing_result(self):
         t, n, m = C.c_double(), C.c_int(), C.c_int()
