@@ -130,7 +130,7 @@ struct AsmArgs {
 // single-pass apply with fused epilogue (modes as AsmMode, except ASM_DIAG)
 template <class T>
 void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
-                       const AsmArgs &args, hipStream_t st);
+                       const AsmArgs &args, hipStream_t st, int nrhs = 1, int64_t vs = 0);
 template <class T>
 void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, const T *x,
                        T *partial, hipStream_t st);
@@ -144,15 +144,24 @@ int assemble_num_blocks(int n_nodes);
 
 template <class T>
 void launch_prolong_add(const TransferCSR &P, const uint8_t *bcmask_f, const T *xc, T *xf,
-                        hipStream_t st);
+                        hipStream_t st, int nrhs = 1, int64_t vsc = 0, int64_t vsf = 0);
 template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *bcmask_c, const T *rf, T *rc,
-                     hipStream_t st);
+                     hipStream_t st, int nrhs = 1, int64_t vsf = 0, int64_t vsc = 0);
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
-                       hipStream_t st);
+                       hipStream_t st, int nrhs = 1);
+// coarsest-level solve: block-diagonal explicit inverse (one block per
+// independent component), per row: first column, block size, row offset
+struct CoarseBlocks {
+  int n = 0;
+  const double *Ainv = nullptr;
+  const int *col0 = nullptr;
+  const int *nb = nullptr;
+  const int64_t *aoff = nullptr;
+};
 template <class T>
-void launch_dense_solve(int n, const double *Ainv, const T *b, T *x, hipStream_t st);
+void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs = 1);
 
 template <class TI, class TO>
 void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);

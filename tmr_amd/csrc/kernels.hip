@@ -278,9 +278,12 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
                                                      const int *__restrict__ col,
                                                      const float *__restrict__ w,
                                                      const uint8_t *__restrict__ mask,
-                                                     const T *__restrict__ xc, T *__restrict__ xf) {
+                                                     const T *__restrict__ xc, T *__restrict__ xf,
+                                                     int64_t vsc, int64_t vsf) {
   const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
+  xc += (int64_t)blockIdx.y * vsc;
+  xf += (int64_t)blockIdx.y * vsf;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   for (int j = ptr[i]; j < ptr[i + 1]; j++) {
     const int c = col[j];
@@ -302,8 +305,11 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
                                                   const int *__restrict__ col,
                                                   const float *__restrict__ w,
                                                   const uint8_t *__restrict__ mask,
-                                                  const T *__restrict__ rf, T *__restrict__ rc) {
+                                                  const T *__restrict__ rf, T *__restrict__ rc,
+                                                  int64_t vsf, int64_t vsc) {
   const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  rf += (int64_t)blockIdx.y * vsf;
+  rc += (int64_t)blockIdx.y * vsc;
   const int i = g >> 2, sub = g & 3;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   if (i < n) {
@@ -331,20 +337,20 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
 
 template <class T>
 void launch_prolong_add(const TransferCSR &P, const uint8_t *mask_f, const T *xc, T *xf,
-                        hipStream_t st) {
+                        hipStream_t st, int nrhs, int64_t vsc, int64_t vsf) {
   if (P.n_rows == 0) return;
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3((P.n_rows + 255) / 256), dim3(256), 0, st, P.n_rows,
-                     P.ptr, P.col, P.w, mask_f, xc, xf);
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3((P.n_rows + 255) / 256, nrhs), dim3(256), 0, st,
+                     P.n_rows, P.ptr, P.col, P.w, mask_f, xc, xf, vsc, vsf);
   TMRX_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T *rc,
-                     hipStream_t st) {
+                     hipStream_t st, int nrhs, int64_t vsf, int64_t vsc) {
   if (R.n_rows == 0) return;
   const int64_t threads = 4 * (int64_t)R.n_rows;
-  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
-                     R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc);
+  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256), nrhs), dim3(256), 0,
+                     st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -352,9 +358,13 @@ void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T
 // Chebyshev first step from a zero guess: d = c Dinv b, x = d
 template <class T>
 __global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict__ dinv, double c,
-                             T *__restrict__ d, T *__restrict__ x) {
+                             T *__restrict__ d, T *__restrict__ x, int64_t vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int64_t off = (int64_t)blockIdx.y * vs;
+  b += off;
+  d += off;
+  x += off;
   const T v = (T)c * dinv[i] * b[i];
   d[i] = v;
   x[i] = v;
@@ -362,29 +372,41 @@ __global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict
 
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_cheb_first<T>, dim3((ndof + 255) / 256), dim3(256), 0, st, ndof, b, dinv,
-                     c, d, x);
+                       hipStream_t st, int nrhs) {
+  hipLaunchKernelGGL(k_cheb_first<T>, dim3((ndof + 255) / 256, nrhs), dim3(256), 0, st, ndof, b,
+                     dinv, c, d, x, (int64_t)ndof);
   TMRX_LAUNCH_CHECK();
 }
 
-// coarsest level: x = Ainv b (dense FP64, symmetric; one wave per row)
+// coarsest level: x = Ainv b with Ainv block diagonal (one dense FP64
+// symmetric block per independent component; one wave per row).  Row i of
+// the coarse vector belongs to the block whose columns start at col0[i]
+// (nb[i] of them) and whose row i is stored at Ainv + aoff[i].
 template <class T>
 __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__restrict__ Ainv,
+                                                     const int *__restrict__ col0,
+                                                     const int *__restrict__ nb,
+                                                     const int64_t *__restrict__ aoff,
                                                      const T *__restrict__ b, T *__restrict__ x) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
+  b += (int64_t)blockIdx.y * n;
+  x += (int64_t)blockIdx.y * n;
   double s = 0.0;
-  const double *a = Ainv + (size_t)row * n;
-  for (int j = lane; j < n; j += 64) s += a[j] * (double)b[j];
+  const double *a = Ainv + aoff[row];
+  const T *bb = b + col0[row];
+  const int m = nb[row];
+  for (int j = lane; j < m; j += 64) s += a[j] * (double)bb[j];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) x[row] = (T)s;
 }
 
 template <class T>
-void launch_dense_solve(int n, const double *Ainv, const T *b, T *x, hipStream_t st) {
-  hipLaunchKernelGGL(k_dense_solve<T>, dim3((n + 3) / 4), dim3(256), 0, st, n, Ainv, b, x);
+void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs) {
+  if (cb.n == 0) return;
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3((cb.n + 3) / 4, nrhs), dim3(256), 0, st, cb.n,
+                     cb.Ainv, cb.col0, cb.nb, cb.aoff, b, x);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -804,11 +826,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const float *__restrict__ dep_w, const uint8_t *__restrict__ bcmask,
     const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
     const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
-    const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials) {
+    const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials, int64_t vs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
+  {  // right-hand side blockIdx.y of a batch of vectors, stride vs
+    const int64_t off = (int64_t)blockIdx.y * vs;
+    x += off;
+    if (b) b += off;
+    if (y) y += off;
+    if (x_out) x_out += off;
+    if (d) d += off;
+    if (partials) partials += (int64_t)blockIdx.y * gridDim.x;
+  }
   const int n0 = node0[t], nown = node0[t + 1] - n0;
   const int e0 = ext_ptr[t], next = ext_ptr[t + 1] - e0;
   const int nslot = nown + next;
@@ -934,11 +965,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const uint8_t *__restrict__ bcmask, const float *__restrict__ s_elem,
     const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
     float *__restrict__ x_out, float *__restrict__ d, const float *__restrict__ dinv, double ca,
-    double cc, double *__restrict__ partials) {
+    double cc, double *__restrict__ partials, int64_t vs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
+  {  // right-hand side blockIdx.y of a batch of vectors, stride vs
+    const int64_t off = (int64_t)blockIdx.y * vs;
+    x += off;
+    if (b) b += off;
+    if (y) y += off;
+    if (x_out) x_out += off;
+    if (d) d += off;
+    if (partials) partials += (int64_t)blockIdx.y * gridDim.x;
+  }
   const int lane = tid & 63, wave = tid >> 6;
   const int en = lane & 15, g = lane >> 4;
   const int n0 = node0[t], nown = node0[t + 1] - n0;
@@ -1036,8 +1076,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
 
 template <class T>
 void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
-                       const AsmArgs &g, hipStream_t st) {
+                       const AsmArgs &g, hipStream_t st, int nrhs, int64_t vs) {
   if (tl.n_tiles == 0) return;
+  const dim3 grid(tl.n_tiles, nrhs);
   const size_t sh = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(T);
   const T *b = (const T *)g.b;
   T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
@@ -1046,12 +1087,12 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   if constexpr (sizeof(T) == 4) {
     if (tl.use_mfma) {
 #define TMRX_MTILE(M, S)                                                                         \
-  hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), dim3(tl.n_tiles), dim3(TILE_THREADS), sh, st,    \
+  hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), grid, dim3(TILE_THREADS), sh, st,                \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.mb_ptr,                       \
                      reinterpret_cast<const uint2 *>(tl.mel), tl.ncolors, tl.afrag, tl.comb_ptr, \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,      \
                      (const float *)s_elem, (const float *)x, (const float *)b, (float *)y,      \
-                     (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.partials)
+                     (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.partials, vs)
 #define TMRX_MTILE2(M)    \
   if (single) {           \
     TMRX_MTILE(M, true);  \
@@ -1071,11 +1112,11 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
     }
   }
 #define TMRX_TILE(M, S)                                                                         \
-  hipLaunchKernelGGL((k_tile_apply<T, M, S>), dim3(tl.n_tiles), dim3(TILE_THREADS), sh, st,     \
+  hipLaunchKernelGGL((k_tile_apply<T, M, S>), grid, dim3(TILE_THREADS), sh, st,                  \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                      \
                      reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,           \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0, \
-                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials)
+                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials, vs)
 #define TMRX_TILE2(M) \
   if (single) {       \
     TMRX_TILE(M, true); \
@@ -1326,15 +1367,15 @@ void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q,
   template void launch_assemble<T>(const AsmCSR &, const uint8_t *, const T *, int,             \
                                    const AsmArgs &, hipStream_t);                               \
   template void launch_prolong_add<T>(const TransferCSR &, const uint8_t *, const T *, T *,     \
-                                      hipStream_t);                                             \
+                                      hipStream_t, int, int64_t, int64_t);                      \
   template void launch_restrict<T>(const TransferCSR &, const uint8_t *, const T *, T *,        \
-                                   hipStream_t);                                                \
+                                   hipStream_t, int, int64_t, int64_t);                         \
   template void launch_cheb_first<T>(int, const T *, const T *, double, T *, T *,               \
-                                     hipStream_t);                                              \
-  template void launch_dense_solve<T>(int, const double *, const T *, T *, hipStream_t);        \
+                                     hipStream_t, int);                                         \
+  template void launch_dense_solve<T>(const CoarseBlocks &, const T *, T *, hipStream_t, int);  \
   template void launch_fill<T>(int64_t, T, T *, hipStream_t);                                 \
   template void launch_tile_apply<T>(const Tiles &, const T *, const T *, const T *, int,       \
-                                     const AsmArgs &, hipStream_t);
+                                     const AsmArgs &, hipStream_t, int, int64_t);
 TMRX_INST(float)
 TMRX_INST(double)
 template void launch_convert<float, double>(int64_t, const float *, double *, hipStream_t);

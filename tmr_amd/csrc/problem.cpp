@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "fe.h"
@@ -89,6 +90,7 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     }
     levels_[l].forest = f;
   }
+  if (find_components()) return -1;
   // stiffness classes: blocks with the same box extents share K0
   Forest *f0 = forests[0];
   const BlockConn &bd = f0->conn_data();
@@ -708,87 +710,184 @@ int Problem::estimate_eigs(int l, const SolveOpts &o) {
   return 0;
 }
 
+// Blocks that share no vertex, directly or through other blocks, are
+// independent problems solved side by side.  Each component must be a
+// contiguous range of block indices, so that its nodes (numbered block by
+// block) are a contiguous range on every level.
+int Problem::find_components() {
+  const BlockConn &bd = levels_[0].forest->conn_data();
+  const int nb = bd.num_blocks;
+  std::vector<int> parent(nb);
+  for (int b = 0; b < nb; b++) parent[b] = b;
+  std::function<int(int)> root = [&](int b) {
+    while (parent[b] != b) b = parent[b] = parent[parent[b]];
+    return b;
+  };
+  std::vector<int> vfirst(bd.num_nodes, -1);
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 8; k++) {
+      const int v = bd.block_conn[8 * b + k];
+      if (vfirst[v] < 0) {
+        vfirst[v] = b;
+      } else {
+        const int ra = root(vfirst[v]), rb = root(b);
+        if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+      }
+    }
+  seg_block0_.assign(1, 0);
+  std::vector<int> comp(nb);
+  for (int b = 0; b < nb; b++) comp[b] = root(b);
+  for (int b = 1; b < nb; b++) {
+    if (comp[b] != comp[b - 1]) {
+      for (int c : seg_block0_)
+        if (comp[c] == comp[b]) {
+          last_error = "problem: each independent component must be a contiguous block range";
+          return -1;
+        }
+      seg_block0_.push_back(b);
+    }
+  }
+  seg_block0_.push_back(nb);
+  nseg_ = (int)seg_block0_.size() - 1;
+  std::vector<int> bseg(nb);
+  for (int s = 0; s < nseg_; s++)
+    for (int b = seg_block0_[s]; b < seg_block0_[s + 1]; b++) bseg[b] = s;
+  for (auto &L : levels_) {
+    Forest *f = L.forest;
+    const auto &octs = f->octants();
+    const auto &conn = f->conn();
+    const int n = f->num_owned_nodes();
+    std::vector<int> lo(nseg_, n), hi(nseg_, -1);
+    for (size_t e = 0; e < octs.size(); e++) {
+      const int s = bseg[octs[e].block];
+      for (int a = 0; a < 8; a++) {
+        const int g = conn[8 * e + a];
+        if (g >= 0) {
+          lo[s] = std::min(lo[s], g);
+          hi[s] = std::max(hi[s], g);
+        }
+      }
+    }
+    L.seg_node0.assign(nseg_ + 1, 0);
+    for (int s = 0; s < nseg_; s++) {
+      if (lo[s] != (s ? hi[s - 1] + 1 : 0)) {
+        last_error = "problem: component node ranges are not contiguous";
+        return -1;
+      }
+      L.seg_node0[s] = lo[s];
+    }
+    L.seg_node0[nseg_] = n;
+    if (hi[nseg_ - 1] != n - 1) {
+      last_error = "problem: component node ranges do not cover the nodes";
+      return -1;
+    }
+  }
+  return 0;
+}
+
 int Problem::coarse_factor() {
-  // dense K on the coarsest level (identity rows for clamped dofs), Cholesky
-  // in FP64, explicit inverse for a one-GEMV coarse solve
+  // dense K on the coarsest level (identity rows for clamped dofs), one
+  // block per independent component, Cholesky in FP64 and the explicit
+  // inverse for a one-GEMV coarse solve
   Level &L = levels_.back();
   const int n = (int)L.ndof;
   std::vector<double> s(L.n_elem);
   HIPCHK(hipMemcpy(s.data(), L.s64.p, L.n_elem * sizeof(double), hipMemcpyDeviceToHost));
-  std::vector<double> A((size_t)n * n, 0.0);
   const auto &dptr = L.forest->dep_ptr();
   const auto &dconn = L.forest->dep_conn();
   const auto &dw = L.forest->dep_weights();
-  for (int e = 0; e < L.n_elem; e++) {
-    const double *K = &K0h_[576 * L.cls[e]];
-    std::vector<std::pair<int, double>> ex[8];
-    for (int a = 0; a < 8; a++) {
-      const int g = L.conn[8 * e + a];
-      if (g >= 0)
-        ex[a].push_back({g, 1.0});
-      else
-        for (int j = dptr[-g - 1]; j < dptr[-g]; j++) ex[a].push_back({dconn[j], dw[j]});
-    }
-    for (int a = 0; a < 8; a++)
-      for (auto &pa : ex[a])
-        for (int bb = 0; bb < 8; bb++)
-          for (auto &pb : ex[bb]) {
-            const double w = s[e] * pa.second * pb.second;
-            for (int i = 0; i < 3; i++) {
-              if ((L.bcmask[pa.first] >> i) & 1) continue;
-              for (int j = 0; j < 3; j++) {
-                if ((L.bcmask[pb.first] >> j) & 1) continue;
-                A[(size_t)(3 * pa.first + i) * n + 3 * pb.first + j] +=
-                    w * K[(3 * a + i) * 24 + 3 * bb + j];
+  const auto &octs = L.forest->octants();
+  std::vector<int> bseg(seg_block0_.back());
+  for (int sg = 0; sg < nseg_; sg++)
+    for (int b = seg_block0_[sg]; b < seg_block0_[sg + 1]; b++) bseg[b] = sg;
+  std::vector<double> Aall;
+  std::vector<int> col0(n), nb(n);
+  std::vector<int64_t> aoff(n);
+  for (int sg = 0; sg < nseg_; sg++) {
+    const int d0 = 3 * L.seg_node0[sg], m = 3 * (L.seg_node0[sg + 1] - L.seg_node0[sg]);
+    std::vector<double> A((size_t)m * m, 0.0);
+    for (int e = 0; e < L.n_elem; e++) {
+      if (bseg[octs[e].block] != sg) continue;
+      const double *K = &K0h_[576 * L.cls[e]];
+      std::vector<std::pair<int, double>> ex[8];
+      for (int a = 0; a < 8; a++) {
+        const int g = L.conn[8 * e + a];
+        if (g >= 0)
+          ex[a].push_back({g, 1.0});
+        else
+          for (int j = dptr[-g - 1]; j < dptr[-g]; j++) ex[a].push_back({dconn[j], dw[j]});
+      }
+      for (int a = 0; a < 8; a++)
+        for (auto &pa : ex[a])
+          for (int bb = 0; bb < 8; bb++)
+            for (auto &pb : ex[bb]) {
+              const double w = s[e] * pa.second * pb.second;
+              for (int i = 0; i < 3; i++) {
+                if ((L.bcmask[pa.first] >> i) & 1) continue;
+                for (int j = 0; j < 3; j++) {
+                  if ((L.bcmask[pb.first] >> j) & 1) continue;
+                  A[(size_t)(3 * pa.first + i - d0) * m + 3 * pb.first + j - d0] +=
+                      w * K[(3 * a + i) * 24 + 3 * bb + j];
+                }
               }
             }
-          }
+    }
+    for (int i = 0; i < m; i++)
+      if ((L.bcmask[(d0 + i) / 3] >> ((d0 + i) % 3)) & 1) A[(size_t)i * m + i] = 1.0;
+    // Cholesky A = L L^T (lower, row-major, in place)
+    for (int j = 0; j < m; j++) {
+      double *Aj = &A[(size_t)j * m];
+      double d = Aj[j];
+      for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
+      if (!(d > 0.0)) {
+        last_error = "coarse Cholesky: matrix of component " + std::to_string(sg) +
+                     " is not positive definite (missing boundary conditions?)";
+        return -1;
+      }
+      d = std::sqrt(d);
+      Aj[j] = d;
+      const double inv = 1.0 / d;
+      for (int i = j + 1; i < m; i++) {
+        double *Ai = &A[(size_t)i * m];
+        double t = Ai[j];
+        for (int k = 0; k < j; k++) t -= Ai[k] * Aj[k];
+        Ai[j] = t * inv;
+      }
+    }
+    // Linv (lower), then Ainv = Linv^T Linv
+    std::vector<double> Li((size_t)m * m, 0.0);
+    for (int c = 0; c < m; c++) {
+      Li[(size_t)c * m + c] = 1.0 / A[(size_t)c * m + c];
+      for (int i = c + 1; i < m; i++) {
+        double t = 0.0;
+        for (int k = c; k < i; k++) t -= A[(size_t)i * m + k] * Li[(size_t)k * m + c];
+        Li[(size_t)i * m + c] = t / A[(size_t)i * m + i];
+      }
+    }
+    const size_t base = Aall.size();
+    Aall.resize(base + (size_t)m * m);
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j <= i; j++) {
+        double t = 0.0;
+        for (int k = i; k < m; k++) t += Li[(size_t)k * m + i] * Li[(size_t)k * m + j];
+        Aall[base + (size_t)i * m + j] = t;
+        Aall[base + (size_t)j * m + i] = t;
+      }
+    for (int i = 0; i < m; i++) {
+      col0[d0 + i] = d0;
+      nb[d0 + i] = m;
+      aoff[d0 + i] = (int64_t)(base + (size_t)i * m);
+    }
   }
-  for (int i = 0; i < n; i++)
-    if ((L.bcmask[i / 3] >> (i % 3)) & 1) A[(size_t)i * n + i] = 1.0;
-  // Cholesky A = L L^T (lower, row-major, in place)
-  for (int j = 0; j < n; j++) {
-    double *Aj = &A[(size_t)j * n];
-    double d = Aj[j];
-    for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
-    if (!(d > 0.0)) {
-      last_error = "coarse Cholesky: matrix is not positive definite (missing boundary conditions?)";
-      return -1;
-    }
-    d = std::sqrt(d);
-    Aj[j] = d;
-    const double inv = 1.0 / d;
-    for (int i = j + 1; i < n; i++) {
-      double *Ai = &A[(size_t)i * n];
-      double t = Ai[j];
-      for (int k = 0; k < j; k++) t -= Ai[k] * Aj[k];
-      Ai[j] = t * inv;
-    }
-  }
-  // Linv (lower), then Ainv = Linv^T Linv
-  std::vector<double> Li((size_t)n * n, 0.0);
-  for (int c = 0; c < n; c++) {
-    Li[(size_t)c * n + c] = 1.0 / A[(size_t)c * n + c];
-    for (int i = c + 1; i < n; i++) {
-      double t = 0.0;
-      const double *Ai = &A[(size_t)i * n];
-      for (int k = c; k < i; k++) t -= Ai[k] * Li[(size_t)k * n + c];
-      Li[(size_t)i * n + c] = t / Ai[i];
-    }
-  }
-  std::vector<double> Ai((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j <= i; j++) {
-      double t = 0.0;
-      for (int k = i; k < n; k++) t += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
-      Ai[(size_t)i * n + j] = t;
-      Ai[(size_t)j * n + i] = t;
-    }
-  n_coarse_dof_ = n;
-  if (Ainv_.upload(Ai)) {
+  if (Ainv_.upload(Aall) || cb_col0_.upload(col0) || cb_nb_.upload(nb) || cb_aoff_.upload(aoff)) {
     last_error = "coarse: device allocation failed";
     return -1;
   }
+  coarse_.n = n;
+  coarse_.Ainv = Ainv_.p;
+  coarse_.col0 = cb_col0_.p;
+  coarse_.nb = cb_nb_.p;
+  coarse_.aoff = cb_aoff_.p;
   return 0;
 }
 
@@ -856,7 +955,7 @@ template <class T>
 void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
   const int last = (int)levels_.size() - 1;
   if (l == last) {
-    launch_dense_solve<T>(n_coarse_dof_, Ainv_.p, b, x, st_);
+    launch_dense_solve<T>(coarse_, b, x, st_);
     return;
   }
   Level &L = levels_[l];

@@ -100,6 +100,9 @@ struct Level {
   Forest *forest = nullptr;
   int n_nodes = 0, n_dep = 0, n_elem = 0;
   int64_t ndof = 0;
+  // independent components (separate problems sharing the hierarchy):
+  // nodes [seg_node0[s], seg_node0[s+1]) belong to component s
+  std::vector<int> seg_node0;
   // host copies
   std::vector<int16_t> lev;
   std::vector<int> cls;
@@ -186,6 +189,7 @@ class Problem {
   template <class T>
   void vcycle(int l, const T *b, T *x, const SolveOpts &o);
   int coarse_factor();
+  int find_components();
   int solve_core(double *rhs, double *x, const SolveOpts &o, SolveInfo *info);
   int estimate_eigs(int l, const SolveOpts &o);
 
@@ -198,7 +202,11 @@ class Problem {
   DevArray<uint16_t> afrag_;  // bf16 hi/lo MFMA fragments of K0f_
   DevArray<double> rho_;
   DevArray<double> Ainv_;
-  int n_coarse_dof_ = 0;
+  DevArray<int> cb_col0_, cb_nb_;
+  DevArray<int64_t> cb_aoff_;
+  CoarseBlocks coarse_;
+  int nseg_ = 1;                 // independent components
+  std::vector<int> seg_block0_;  // [nseg + 1] block ranges of the components
   // PCG state
   DevArray<double> f_, u_, r_, p_, q_, z_, parts_, sc_;
   DevArray<double> energy_, psi_, adjf_, dfdrho_, CB_, cvol_;
