@@ -40,25 +40,33 @@ def test_stress_value_and_gradient_match_oracle(adapt, precision):
 
 
 def test_stress_gradient_central_differences(adapt):
+    """Central differences of the p-norm along a random density direction
+    (h = 1e-5: per-element differences at h = 1e-6 drown in the solve
+    tolerance, f ~ 5e3), and at the element of largest sensitivity."""
     forests, rho, prob = adapt
     f = prob.traction_load("load", [0.0, 0.0, -1.0])
     opts = solver.default_opts(rtol=1e-12, precision=0)
     prob.set_design(rho)
     prob.solve(f, opts=opts)
     g = prob.stress_pnorm_grad(8.0, opts=opts)
-    rng = np.random.default_rng(3)
-    h = 1e-6
-    for k in rng.choice(prob.nelem, 4, replace=False):
-        vals = []
-        for sgn in (1, -1):
-            r2 = rho.copy()
-            r2[k] += sgn * h
-            prob.set_design(r2)
-            prob.solve(f, opts=opts)
-            vals.append(prob.stress_pnorm(8.0))
-        fd = (vals[0] - vals[1]) / (2 * h)
-        print("elem", k, "adjoint %.8e fd %.8e" % (g[k], fd), flush=True)
-        assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(vals[0]))
+
+    def fval(r2):
+        prob.set_design(r2)
+        prob.solve(f, opts=opts)
+        return prob.stress_pnorm(8.0)
+
+    d = np.random.default_rng(11).standard_normal(prob.nelem)
+    h = 1e-5
+    fd = (fval(rho + h * d) - fval(rho - h * d)) / (2 * h)
+    print("directional: adjoint %.10e fd %.10e" % (g @ d, fd), flush=True)
+    assert abs(fd - g @ d) / abs(g @ d) < 1e-5
+    k = int(np.argmax(np.abs(g)))
+    e = np.zeros(prob.nelem)
+    e[k] = 1.0
+    h = 1e-4
+    fd = (fval(rho + h * e) - fval(rho - h * e)) / (2 * h)
+    print("elem", k, "adjoint %.8e fd %.8e" % (g[k], fd), flush=True)
+    assert abs(fd - g[k]) / abs(g[k]) < 1e-3
     prob.set_design(rho)
 
 

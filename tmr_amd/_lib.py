@@ -101,6 +101,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libtmrx.so not found at %s; run tmr_amd.build.build()" % LIB_PATH)
+        # One HIP runtime per process: torch ships its own libamdhip64 under
+        # the same soname, and a second copy loaded after ours cannot open
+        # the GPU.  Loading torch first makes this library bind to torch's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)

@@ -22,6 +22,12 @@ namespace tmrx {
 constexpr int WG_ELEMS = 256;
 constexpr int MAX_COLORS = 32;
 
+// launch errors: noted after every launch, reported by the next
+// synchronising call of the problem
+void note_launch(hipError_t e);
+hipError_t take_launch_error();
+#define TMRX_LAUNCH_CHECK() ::tmrx::note_launch(hipGetLastError())
+
 struct ElemBatches {
   int n_wg = 0;
   int n_elem = 0;
@@ -180,7 +186,7 @@ void launch_restrict_scale(int nc, const int *ptr, const int *child, const float
                            const int16_t *lev_coarse, double *s64, float *s32, hipStream_t st);
 
 // FP64 PCG helpers (scalars live on the device)
-enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_N };
+enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_TRR, SC_N };
 int reduce_blocks(int64_t n);
 void launch_pcg_update_xr(int64_t ndof, double *x, double *r, const double *p, const double *q,
                           const double *sc, double *partials, float *r32, hipStream_t st);
@@ -195,6 +201,39 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // finalize: mode 0 rz, 1 pq->alpha, 2 rr, 3 (zr, zq)->beta, 4 bb,
 // 5 (|y|^2, |x|^2) of the power iteration
 void launch_finalize(int nparts, const double *partials, int mode, double *sc, hipStream_t st);
+
+// Batched PCG over nrhs right-hand sides (vectors [nrhs][n]) whose dofs split
+// into independent segments (components) with their own scalars
+// sc[(rhs * nseg + seg) * SC_N + k], running flags act[rhs * nseg + seg] and
+// fixed reduction blocks that never straddle a segment.
+struct SegPlan {
+  int nrhs = 1, nseg = 1, nblk = 0, ntiles = 0;
+  int64_t n = 0;                 // dofs per right-hand side
+  const int *blk_lo = nullptr;   // [nblk + 1] block b = dofs [blk_lo[b], blk_lo[b+1])
+  const int *blk_seg = nullptr;  // [nblk]
+  const int *seg_blk = nullptr;  // [nseg + 1]
+  const int *seg_tile = nullptr; // [nseg + 1] finest-level owner tiles of each segment
+};
+enum { FIN_BB = 0, FIN_RZ, FIN_PQ, FIN_RR, FIN_BETA, FIN_TRUE };
+void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const int *act,
+                   double *partials, hipStream_t st);
+void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
+                        const double *q, float *r32, const double *sc, const int *act,
+                        double *partials, hipStream_t st);
+void launch_b_dot2(const SegPlan &pl, const double *z, const double *r, const double *q,
+                   const int *act, double *partials, hipStream_t st);
+void launch_b_update_p(const SegPlan &pl, double *p, const double *z, const double *sc,
+                       const int *act, hipStream_t st);
+void launch_b_copy(const SegPlan &pl, const double *a, double *b, const int *act,
+                   hipStream_t st);
+// FIN_BB also sets the stop threshold max(rtol2 |b|^2, atol2) and the flags;
+// FIN_RR stops a segment at the threshold (itconv = iteration, negative on
+// breakdown); iter is the device iteration counter advanced by launch_b_tick
+void launch_b_finalize(const SegPlan &pl, int mode, const double *partials, double *sc, int *act,
+                       double *thr2, const int *iter, int *itconv, double rtol2, double atol2,
+                       hipStream_t st);
+// advance the iteration counter and count the running segments into *nact
+void launch_b_tick(int *iter, const int *act, int nidx, int *nact, hipStream_t st);
 
 // per-element compliance density and its design derivative (element mode)
 void launch_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,

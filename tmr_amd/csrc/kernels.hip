@@ -5,7 +5,16 @@
 
 namespace tmrx {
 
-#define TMRX_LAUNCH_CHECK() (void)hipGetLastError()
+// the first failed launch is kept until a synchronising call collects it
+static hipError_t g_launch_error = hipSuccess;
+void note_launch(hipError_t e) {
+  if (e != hipSuccess && g_launch_error == hipSuccess) g_launch_error = e;
+}
+hipError_t take_launch_error() {
+  const hipError_t e = g_launch_error;
+  g_launch_error = hipSuccess;
+  return e;
+}
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks a
 // contiguous range of logical blocks (neighbouring Morton batches share nodes,

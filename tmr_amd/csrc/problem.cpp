@@ -16,6 +16,7 @@ namespace tmrx {
 #define HIPCHK(x)                                                       \
   do {                                                                  \
     hipError_t e_ = (x);                                                \
+    if (e_ == hipSuccess) e_ = take_launch_error();                     \
     if (e_ != hipSuccess) {                                             \
       last_error = std::string("HIP error: ") + hipGetErrorString(e_) + \
                    " at " + #x;                                         \
