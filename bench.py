@@ -68,6 +68,121 @@ def cpu_baseline(level=5, iters=None):
                       % (level, conn.shape[0], ndof, it, dt)}
 
 
+def bench_multirhs(args):
+    """16 load cases on a 2,097,152-hex cube (1 block, level 7): one batched
+    solve of all cases against 16 single solves."""
+    import torch
+    from tmr_amd import TMR, solver, workloads
+    nrhs = args.nrhs
+    f0 = TMR.createBoxForest(1, 1, 1, L=(1.0, 1.0, 1.0), face_names=workloads.FACES)
+    f0.createTrees(args.level)
+    forests = workloads.hierarchy(f0, args.nlevels)
+    n = 1 << args.level
+    rho = workloads.element_density_from_grid(forests[0], workloads.smoothed_density((n, n, n), 0),
+                                              (1.0, 1.0, 1.0))
+    prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
+    prob.set_stream(torch.cuda.current_stream().cuda_stream)
+    prob.set_design(rho)
+    if not args.no_autotune:
+        prob.autotune((256, 512, 1024), reps=3)
+    rng = np.random.default_rng(0)
+    F = np.stack([prob.traction_load("load", rng.standard_normal(3)) for _ in range(nrhs)])
+    Fd = torch.from_numpy(F).cuda()
+    Ud = torch.zeros_like(Fd)
+    opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=500, no_graph=int(args.no_graph))
+    prob.setup(opts)
+    prob.solve_multi(Fd, Ud, opts=opts)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        prob.solve_multi(Fd, Ud, opts=opts)
+    torch.cuda.synchronize()
+    dt_b = (time.perf_counter() - ts) / args.steps
+    iters_b = [i.iters for i in prob.infos]
+    u1 = torch.zeros_like(Fd[0])
+    prob.solve(Fd[0], u1, opts=opts)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    iters_s = []
+    for k in range(nrhs):
+        prob.solve(Fd[k], u1, opts=opts)
+        iters_s.append(prob.info.iters)
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - ts
+    ndof = prob.ndof
+    out = {"metric": "load-case solves/s, %d load cases batched vs one by one" % nrhs,
+           "value": nrhs / dt_b, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 1, "ms_per_step": 1e3 * dt_b, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
+           "config": {"workload": "cube 128^3 hex8 (1 block, level %d), %d traction load cases, "
+                                  "%d-level MG, PCG to 1e-8" % (args.level, nrhs, prob.nlevels),
+                      "elements": prob.nelem, "dofs": ndof},
+           "sequential_solves_per_s": nrhs / dt_s, "speedup_batched": dt_s / dt_b,
+           "iters_batched": iters_b, "iters_sequential": iters_s,
+           "mdof_iter_per_s": ndof * sum(iters_b) / dt_b / 1e6}
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """96 independent cantilevers of edge 8..48 elements solved side by side
+    in one forest, against the same problems one by one (a sample)."""
+    import torch
+    from tmr_amd import solver, workloads
+    t0 = time.perf_counter()
+    forests, rho = workloads.cube_batch(args.count, level=3, nlevels=4)
+    prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
+    log("batch of %d problems built in %.1f s: %d elements, %d dofs"
+        % (prob.num_components(), time.perf_counter() - t0, prob.nelem, prob.ndof))
+    prob.set_stream(torch.cuda.current_stream().cuda_stream)
+    prob.set_design(rho)
+    if not args.no_autotune:
+        prob.autotune((256, 512, 1024), reps=3)
+    f = torch.from_numpy(prob.traction_load("load", [0.0, 0.0, -1.0])).cuda()
+    u = torch.zeros_like(f)
+    opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=500, no_graph=int(args.no_graph))
+    prob.setup(opts)
+    prob.solve(f, u, opts=opts)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        prob.solve(f, u, opts=opts)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - ts) / args.steps
+    nc = prob.num_components()
+    its = [prob.component_info(k)[0] for k in range(nc)]
+    # the same problems one at a time (first 6: one of each size)
+    seq = 0.0
+    for k in range(min(6, nc)):
+        d = 1 + (k % 6)
+        fk, rk = workloads.box_batch([(d, d, d)], 3, 4, seed=k)
+        pk = solver.Problem(fk, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
+        pk.set_stream(torch.cuda.current_stream().cuda_stream)
+        pk.set_design(rk)
+        gk = torch.from_numpy(pk.traction_load("load", [0.0, 0.0, -1.0])).cuda()
+        uk = torch.zeros_like(gk)
+        pk.setup(opts)
+        pk.solve(gk, uk, opts=opts)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        pk.solve(gk, uk, opts=opts)
+        torch.cuda.synchronize()
+        seq += time.perf_counter() - ts
+    seq_all = seq * nc / min(6, nc)
+    out = {"metric": "independent problems solved/s, %d uneven cantilevers in one launch per "
+                     "stage vs one by one" % nc,
+           "value": nc / dt, "unit": "problems/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 1, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
+           "config": {"workload": "%d cubes of edge 8..48 hex8 (k^3 blocks at level 3, k = 1..6), "
+                                  "4-level MG, PCG to 1e-8 per problem" % nc,
+                      "elements": prob.nelem, "dofs": prob.ndof},
+           "sequential_problems_per_s": nc / seq_all,
+           "speedup_batched": seq_all / dt,
+           "sequential_sample": "the 6 sizes solved alone once each, scaled to %d problems" % nc,
+           "iters_min_max": [min(its), max(its)]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +195,24 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=5)
     ap.add_argument("--fp32-valu", action="store_true",
                     help="FP32 operator on the vector ALUs instead of the matrix cores")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="enqueue every PCG iteration instead of replaying HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default owner-tile size")
+    ap.add_argument("--workload", default="cantilever", choices=["cantilever", "multirhs", "batch"],
+                    help="cantilever: the 4.2M-hex headline; multirhs: 16 load cases on a "
+                         "2.1M-hex cube; batch: 96 uneven independent cantilevers")
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--count", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "multirhs":
+        return bench_multirhs(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import torch
     from tmr_amd import solver, workloads
@@ -102,10 +230,16 @@ def main():
     f = torch.from_numpy(f_host).cuda()
     u = torch.zeros_like(f)
     prob.set_design(rho)
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = prob.autotune((256, 512, 1024), reps=3)
+        log("autotuned owner tiles: %d nodes (ms per 4 FP32 + 1 FP64 products: %s) in %.1f s"
+            % (tune[0], tune[1], time.perf_counter() - t0))
 
     def run(precision, steps, warmup, timing=False):
         opts = solver.default_opts(rtol=1e-8, precision=precision, max_iters=500,
-                                   fp32_valu=int(args.fp32_valu))
+                                   fp32_valu=int(args.fp32_valu), no_graph=int(args.no_graph))
         torch.cuda.synchronize()
         ts = time.perf_counter()
         prob.setup(opts)
@@ -113,8 +247,6 @@ def main():
         setup_ms = 1e3 * (time.perf_counter() - ts)
         for _ in range(warmup):
             prob.solve(f, u, opts=opts)
-        if timing:
-            prob.kernel_timing(TIME_CHEB)
         torch.cuda.synchronize()
         ts = time.perf_counter()
         iters = []
@@ -123,11 +255,16 @@ def main():
             iters.append(prob.info.iters)
         torch.cuda.synchronize()
         dt = time.perf_counter() - ts
-        kt = prob.kernel_timing_result() if timing else None
+        res = dict(seconds=dt, iters=iters, setup_ms=setup_ms,
+                   true_res=prob.true_residual(), compliance=prob.compliance())
         if timing:
+            # kernel durations from one more (untimed, ungraphed) solve: HIP
+            # events around every finest-level Chebyshev-step launch
+            prob.kernel_timing(TIME_CHEB)
+            prob.solve(f, u, opts=opts)
+            res["kt"] = prob.kernel_timing_result()
             prob.kernel_timing(False)
-        return dict(seconds=dt, iters=iters, setup_ms=setup_ms, kt=kt,
-                    true_res=prob.true_residual(), compliance=prob.compliance())
+        return res
 
     mixed = run(1, args.steps, args.warmup, timing=True)
     log("mixed: %.3f s for %d solves, iters %s, true residual %.2e"
@@ -168,6 +305,8 @@ def main():
         "true_residual": mixed["true_res"],
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
+        "tile_nodes": tune[0] if tune else 512,
+        "hip_graphs": not args.no_graph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": ("k_tile_apply<float, ASM_CHEB>" if args.fp32_valu else

@@ -144,6 +144,7 @@ typedef struct {
   int monitor;      /* print the residual history to stderr */
   int zero_guess;   /* start from u = 0 */
   int fp32_valu;    /* FP32 operator on the vector ALUs instead of the matrix cores */
+  int no_graph;     /* enqueue each PCG iteration instead of replaying a captured HIP graph */
 } tmrx_solve_opts;
 
 typedef struct {
@@ -189,6 +190,32 @@ int tmrx_problem_setup(tmrx_problem *p, const tmrx_solve_opts *opts);
    device pointers.  Returns 0 when converged, 1 when not, < 0 on error. */
 int tmrx_problem_solve(tmrx_problem *p, const double *f, double *u, int on_device,
                        const tmrx_solve_opts *opts, tmrx_solve_info *info);
+/* nrhs load cases with one operator: the loop of TMRTopoProblem::evalObjCon
+   over load cases (src/topology/TMRTopoProblem.cpp:1003-1009, one ksm->solve
+   per case) as one batched PCG.  f and u are [nrhs][ndof]; infos[nrhs].
+   Every stage is one launch over all cases; each case stops at its own
+   tolerance.  Load case 0 becomes the state read by compliance and the
+   gradients.  Returns 1 if any case did not converge. */
+int tmrx_problem_solve_multi(tmrx_problem *p, int nrhs, const double *f, double *u,
+                             int on_device, const tmrx_solve_opts *opts,
+                             tmrx_solve_info *infos);
+/* Independent components: forests whose blocks fall into groups sharing no
+   vertex hold several problems that are solved side by side in every
+   launch, each with its own PCG scalars and stopping test (uneven problem
+   batches; TMR would run one TMRTopoProblem per problem).  Per component of
+   the last solve: iterations, converged flag and the true relative residual
+   |f - K u| / |f| over its dofs. */
+int tmrx_problem_num_components(tmrx_problem *p, int *ncomp);
+/* Owner-tile autotuner (no counterpart in the reference): rebuilds the
+   operator tiles for each candidate size (independent nodes per tile,
+   64..4096), times the finest-level operator reps times (FP32 matrix-core
+   product x4 + FP64 product, one V-cycle's and one PCG iteration's share)
+   and keeps the fastest.  ms[n] receives the time per candidate (may be
+   NULL).  Needs a design; invalidates the current solution. */
+int tmrx_problem_autotune(tmrx_problem *p, const int *tile_sizes, int n, int reps, int *chosen,
+                          double *ms);
+int tmrx_problem_component_info(tmrx_problem *p, int rhs, int comp, int *iters, int *converged,
+                                double *rel_residual);
 /* Relative true residual |f - K u| / |f| of the last solve (FP64 operator). */
 int tmrx_problem_true_residual(tmrx_problem *p, double *rel);
 /* Make a stored load case (load f, displacement u; ndof each) the state read by

@@ -27,7 +27,7 @@ class SolveOpts(C.Structure):
                 ("precision", C.c_int), ("cheb_degree", C.c_int),
                 ("cheb_lower", C.c_double), ("cheb_upper", C.c_double),
                 ("eig_iters", C.c_int), ("monitor", C.c_int), ("zero_guess", C.c_int),
-                ("fp32_valu", C.c_int)]
+                ("fp32_valu", C.c_int), ("no_graph", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -80,6 +80,10 @@ _SIGS = {
     "tmrx_problem_set_design": (I, [P, P, I, I]),
     "tmrx_problem_setup": (I, [P, C.POINTER(SolveOpts)]),
     "tmrx_problem_solve": (I, [P, P, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "tmrx_problem_solve_multi": (I, [P, I, P, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
+    "tmrx_problem_num_components": (I, [P, IP]),
+    "tmrx_problem_component_info": (I, [P, I, I, IP, IP, C.POINTER(C.c_double)]),
+    "tmrx_problem_autotune": (I, [P, IP, I, I, IP, C.POINTER(C.c_double)]),
     "tmrx_problem_true_residual": (I, [P, DP]),
     "tmrx_problem_set_state": (I, [P, P, P, I]),
     "tmrx_problem_compliance": (I, [P, DP]),

@@ -32,6 +32,7 @@ static tmrx::SolveOpts to_opts(const tmrx_solve_opts *o) {
   s.monitor = o->monitor;
   s.zero_guess = o->zero_guess;
   s.fp32_valu = o->fp32_valu;
+  s.no_graph = o->no_graph;
   return s;
 }
 
@@ -65,6 +66,7 @@ void tmrx_solve_opts_default(tmrx_solve_opts *o) {
   o->monitor = s.monitor;
   o->zero_guess = s.zero_guess;
   o->fp32_valu = s.fp32_valu;
+  o->no_graph = s.no_graph;
 }
 
 int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat, int nbcs,
@@ -167,6 +169,51 @@ int tmrx_problem_solve(tmrx_problem *h, const double *f, double *u, int on_devic
   if (r < 0) return fail(h);
   if (r > 0) set_error(h->p->last_error);
   return r;
+}
+
+int tmrx_problem_solve_multi(tmrx_problem *h, int nrhs, const double *f, double *u, int on_device,
+                             const tmrx_solve_opts *opts, tmrx_solve_info *infos) {
+  CHECK_P(h);
+  if (nrhs < 1) {
+    set_error("tmrx_problem_solve_multi: nrhs must be positive");
+    return -1;
+  }
+  std::vector<tmrx::SolveInfo> si(nrhs);
+  int r = h->p->solve_multi(nrhs, f, u, on_device != 0, to_opts(opts), si.data());
+  if (infos)
+    for (int k = 0; k < nrhs; k++) {
+      infos[k].iters = si[k].iters;
+      infos[k].converged = si[k].converged;
+      infos[k].rnorm0 = si[k].rnorm0;
+      infos[k].rnorm = si[k].rnorm;
+      infos[k].setup_ms = si[k].setup_ms;
+      infos[k].solve_ms = si[k].solve_ms;
+    }
+  if (r < 0) return fail(h);
+  if (r > 0) set_error(h->p->last_error);
+  return r;
+}
+
+int tmrx_problem_autotune(tmrx_problem *h, const int *tile_sizes, int n, int reps, int *chosen,
+                          double *ms) {
+  CHECK_P(h);
+  return h->p->autotune(tile_sizes, n, reps, chosen, ms) ? fail(h) : 0;
+}
+
+int tmrx_problem_num_components(tmrx_problem *h, int *ncomp) {
+  CHECK_P(h);
+  if (ncomp) *ncomp = h->p->num_segments();
+  return 0;
+}
+
+int tmrx_problem_component_info(tmrx_problem *h, int rhs, int comp, int *iters, int *converged,
+                                double *rel_residual) {
+  CHECK_P(h);
+  if (h->p->segment_info(rhs, comp, iters, converged, rel_residual)) {
+    set_error("tmrx_problem_component_info: no such right-hand side / component in the last solve");
+    return -1;
+  }
+  return 0;
 }
 
 int tmrx_problem_true_residual(tmrx_problem *h, double *rel) {

@@ -174,7 +174,8 @@ void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);
 template <class T>
 void launch_fill(int64_t n, T v, T *a, hipStream_t st);
 // zero the clamped components of a node-major 3-dof vector
-void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st);
+void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st,
+                         int nrhs = 1, int64_t vs = 0);
 
 // element stiffness scales for a level: s = penalty(rho) * 2^-level
 void launch_simp_scale(int n, const double *rho, const int16_t *level, double q, double k0,

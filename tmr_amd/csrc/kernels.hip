@@ -445,19 +445,21 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
 }
 
 __global__ void k_zero_clamped(int n_nodes, const uint8_t *__restrict__ bcmask,
-                               double *__restrict__ v) {
+                               double *__restrict__ v, int64_t vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_nodes) return;
+  v += (int64_t)blockIdx.y * vs;
   const uint8_t m = bcmask[i];
   if (m & 1) v[3 * (int64_t)i] = 0.0;
   if (m & 2) v[3 * (int64_t)i + 1] = 0.0;
   if (m & 4) v[3 * (int64_t)i + 2] = 0.0;
 }
 
-void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st) {
+void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st, int nrhs,
+                         int64_t vs) {
   if (n_nodes == 0) return;
-  hipLaunchKernelGGL(k_zero_clamped, dim3((n_nodes + 255) / 256), dim3(256), 0, st, n_nodes,
-                     bcmask, v);
+  hipLaunchKernelGGL(k_zero_clamped, dim3((n_nodes + 255) / 256, nrhs), dim3(256), 0, st, n_nodes,
+                     bcmask, v, vs);
   TMRX_LAUNCH_CHECK();
 }
 

@@ -24,7 +24,21 @@ namespace tmrx {
     }                                                                   \
   } while (0)
 
+void Problem::drop_graphs() {
+  for (int k = 0; k < 2; k++)
+    if (gexec_[k]) {
+      (void)hipGraphExecDestroy(gexec_[k]);
+      gexec_[k] = nullptr;
+    }
+  g_gen_ = -1;
+}
+
 Problem::~Problem() {
+  drop_graphs();
+  for (int k = 0; k < 2; k++)
+    if (gev_[k]) (void)hipEventDestroy(gev_[k]);
+  if (cap_st_) (void)hipStreamDestroy(cap_st_);
+  if (nact_host_) (void)hipHostFree(nact_host_);
   for (auto &e : ev_pool_) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -186,13 +200,9 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
   for (int l = 0; l + 1 < nlevels; l++)
     if (build_transfer(l)) return -1;
   const int64_t n = levels_[0].ndof;
-  if (f_.alloc(n) || u_.alloc(n) || r_.alloc(n) || p_.alloc(n) || q_.alloc(n) || z_.alloc(n) ||
-      parts_.alloc((size_t)std::max(std::max(2 * 2048, assemble_num_blocks(levels_[0].n_nodes)),
-                                    std::max(levels_[0].tiles.n_tiles,
-                                             (levels_[0].n_elem + 255) / 256)) + 16) ||
-      sc_.alloc(SC_N) || rho_.alloc(levels_[0].n_elem) ||
-      energy_.alloc(levels_[0].n_elem) || psi_.alloc(n) || adjf_.alloc(n) ||
-      dfdrho_.alloc(levels_[0].n_elem)) {
+  if (build_plan()) return -1;
+  if (ensure_batch(1) || rho_.alloc(levels_[0].n_elem) || energy_.alloc(levels_[0].n_elem) ||
+      psi_.alloc(n) || adjf_.alloc(n) || dfdrho_.alloc(levels_[0].n_elem)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -339,7 +349,7 @@ int Problem::build_level(int l, int nbcs, const char **names, const int *masks) 
   return 0;
 }
 
-// Owner tiles: contiguous ranges of TILE_NODES independent nodes; every
+// Owner tiles: contiguous ranges of L.tile_nodes independent nodes; every
 // element touching an owned node, or a hanging node with an owned parent, is
 // staged with its corners in LDS slots [owned | accumulated hanging | halo].
 int Problem::build_tiles(int l) {
@@ -376,7 +386,15 @@ int Problem::build_tiles(int l) {
         pw[fill[dconn[j]]++] = (float)dw[j];
       }
   }
-  const int ntiles = (n + TILE_NODES - 1) / TILE_NODES;
+  // tiles never straddle two independent components
+  std::vector<int> tstart;
+  L.seg_tile.assign(1, 0);
+  for (size_t sg = 0; sg + 1 < L.seg_node0.size(); sg++) {
+    for (int a = L.seg_node0[sg]; a < L.seg_node0[sg + 1]; a += L.tile_nodes) tstart.push_back(a);
+    L.seg_tile.push_back((int)tstart.size());
+  }
+  tstart.push_back(n);
+  const int ntiles = (int)tstart.size() - 1;
   std::vector<int> node0(ntiles + 1), ext_ptr{0}, ext, nacc(ntiles), el_ptr{0};
   std::vector<TileElem> els, mel;
   std::vector<int> mb_ptr((size_t)ntiles * (MAX_COLORS + 1), 0);
@@ -390,7 +408,7 @@ int Problem::build_tiles(int l) {
     if (L.cls[e] != L.cls[0]) single = false;
   std::vector<int> tel, rdeps;
   for (int t = 0; t < ntiles; t++) {
-    const int n0 = t * TILE_NODES, n1 = std::min(n, n0 + TILE_NODES), nown = n1 - n0;
+    const int n0 = tstart[t], n1 = std::min(tstart[t + 1], n0 + L.tile_nodes), nown = n1 - n0;
     node0[t] = n0;
     rdeps.clear();
     tel.clear();
@@ -604,31 +622,85 @@ int Problem::build_transfer(int l) {
   return 0;
 }
 
-int Problem::ensure_vecs(int l, int precision) {
+int Problem::ensure_vecs(int l, int precision, int nrhs) {
   Level &L = levels_[l];
   const size_t n = (size_t)L.ndof;
   const size_t ns = 3 * (size_t)L.wg_nodes.n;
-  if (precision == 1) {
-    auto &v = L.v32;
-    if (v.ready) return 0;
-    if (v.b.alloc(n) || v.x.alloc(n) || v.x2.alloc(n) || v.r.alloc(n) || v.d.alloc(n) ||
-        v.dinv.alloc(n) || v.partial.alloc(ns))
+  auto grow = [&](auto &v) -> int {
+    using T = std::remove_reference_t<decltype(*v.b.p)>;
+    if (!v.ready) {
+      if (v.dinv.alloc(n) || v.partial.alloc(ns)) return -1;
+      v.ready = true;
+      v.cap = 0;
+    }
+    if (v.cap >= nrhs) return 0;
+    const size_t m = n * (size_t)nrhs;
+    if (v.b.alloc(m) || v.x.alloc(m) || v.x2.alloc(m) || v.r.alloc(m) || v.d.alloc(m)) return -1;
+    if (hipMemset(v.d.p, 0, m * sizeof(T)) != hipSuccess ||
+        hipMemset(v.x.p, 0, m * sizeof(T)) != hipSuccess ||
+        hipMemset(v.x2.p, 0, m * sizeof(T)) != hipSuccess)
       return -1;
-    HIPCHK(hipMemset(v.d.p, 0, n * sizeof(float)));
-    HIPCHK(hipMemset(v.x.p, 0, n * sizeof(float)));
-    HIPCHK(hipMemset(v.x2.p, 0, n * sizeof(float)));
-    v.ready = true;
-  } else {
-    auto &v = L.v64;
-    if (v.ready) return 0;
-    if (v.b.alloc(n) || v.x.alloc(n) || v.x2.alloc(n) || v.r.alloc(n) || v.d.alloc(n) ||
-        v.dinv.alloc(n) || v.partial.alloc(ns))
-      return -1;
-    HIPCHK(hipMemset(v.d.p, 0, n * sizeof(double)));
-    HIPCHK(hipMemset(v.x.p, 0, n * sizeof(double)));
-    HIPCHK(hipMemset(v.x2.p, 0, n * sizeof(double)));
-    v.ready = true;
+    v.cap = nrhs;
+    gen_++;
+    return 0;
+  };
+  return precision == 1 ? grow(L.v32) : grow(L.v64);
+}
+
+// PCG vectors, scalars and partials for nrhs right-hand sides
+int Problem::ensure_batch(int nrhs) {
+  if (nrhs <= batch_cap_) return 0;
+  const Level &L0 = levels_[0];
+  const size_t n = (size_t)L0.ndof, m = n * (size_t)nrhs;
+  const size_t nidx = (size_t)nrhs * nseg_;
+  const size_t np = std::max({(size_t)2 * 2048, (size_t)assemble_num_blocks(L0.n_nodes),
+                              (size_t)L0.tiles.n_tiles * nrhs, (size_t)2 * plan_.nblk * nrhs,
+                              (size_t)(L0.n_elem + 255) / 256}) + 16;
+  if (f_.alloc(m) || u_.alloc(m) || r_.alloc(m) || p_.alloc(m) || q_.alloc(m) || z_.alloc(m) ||
+      parts_.alloc(np) || sc_.alloc(std::max(nidx * SC_N, (size_t)SC_N)) || act_.alloc(nidx) ||
+      itconv_.alloc(nidx) || thr_.alloc(nidx) || iter_.alloc(1) || nact_.alloc(1)) {
+    last_error = "problem: device allocation failed";
+    return -1;
   }
+  HIPCHK(hipMemset(u_.p, 0, m * sizeof(double)));
+  HIPCHK(hipMemset(f_.p, 0, m * sizeof(double)));
+  if (!nact_host_) HIPCHK(hipHostMalloc((void **)&nact_host_, 2 * sizeof(int)));
+  have_solution_ = false;
+  batch_cap_ = nrhs;
+  gen_++;
+  return 0;
+}
+
+// fixed reduction blocks of the finest-level vector, none straddling a
+// component
+int Problem::build_plan() {
+  const Level &L = levels_[0];
+  constexpr int CHUNK = 4096;  // dofs per reduction block
+  std::vector<int> lo, bseg, sblk{0};
+  for (int sg = 0; sg < nseg_; sg++) {
+    const int d0 = 3 * L.seg_node0[sg], d1 = 3 * L.seg_node0[sg + 1];
+    for (int a = d0; a < d1; a += CHUNK) {
+      lo.push_back(a);
+      bseg.push_back(sg);
+    }
+    sblk.push_back((int)lo.size());
+  }
+  lo.push_back((int)L.ndof);
+  if (bseg.empty()) bseg.push_back(0);
+  if (pl_blk_lo_.upload(lo) || pl_blk_seg_.upload(bseg) || pl_seg_blk_.upload(sblk) ||
+      pl_seg_tile_.upload(L.seg_tile)) {
+    last_error = "problem: device allocation failed";
+    return -1;
+  }
+  plan_.nrhs = 1;
+  plan_.nseg = nseg_;
+  plan_.nblk = (int)lo.size() - 1;
+  plan_.ntiles = L.tiles.n_tiles;
+  plan_.n = L.ndof;
+  plan_.blk_lo = pl_blk_lo_.p;
+  plan_.blk_seg = pl_blk_seg_.p;
+  plan_.seg_blk = pl_seg_blk_.p;
+  plan_.seg_tile = pl_seg_tile_.p;
   return 0;
 }
 
@@ -657,7 +729,7 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
 }
 
 template <class T>
-void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
+void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrhs) {
   Level &L = levels_[l];
   std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
   if (l == 0 && ((timing_ >> mode) & 1)) {
@@ -674,7 +746,7 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a) {
   Tiles tl = L.tiles;
   tl.use_mfma = tl.use_mfma && !fp32_valu_;
   if (ev) (void)hipEventRecord(ev->first, st_);
-  launch_tile_apply<T>(tl, K0<T>(), scale<T>(l), x, mode, g, st_);
+  launch_tile_apply<T>(tl, K0<T>(), scale<T>(l), x, mode, g, st_, nrhs, L.ndof);
   if (ev) (void)hipEventRecord(ev->second, st_);
 }
 
@@ -919,6 +991,7 @@ int Problem::setup(const SolveOpts &o) {
   HIPCHK(hipStreamSynchronize(st_));
   have_setup_ = true;
   setup_precision_ = prec;
+  gen_++;
   last_setup_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
@@ -928,7 +1001,8 @@ int Problem::setup(const SolveOpts &o) {
 // src/TMR_RefinementTools.cpp:46-47).  Step k reads x_in and writes x_out
 // (tiles read halo values of x, so the update cannot be in place).
 template <class T>
-void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o) {
+void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
+                        int nrhs) {
   Level &L = levels_[l];
   LevelVecs<T> &v = vecs<T>(l);
   const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
@@ -949,17 +1023,18 @@ void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const
   a.dinv = v.dinv.p;
   a.ca = ca;
   a.cc = cc;
-  apply_level<T>(l, x_in, ASM_CHEB, a);
+  apply_level<T>(l, x_in, ASM_CHEB, a, nrhs);
 }
 
 template <class T>
-void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
+void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   const int last = (int)levels_.size() - 1;
   if (l == last) {
-    launch_dense_solve<T>(coarse_, b, x, st_);
+    launch_dense_solve<T>(coarse_, b, x, st_, nrhs);
     return;
   }
   Level &L = levels_[l];
+  Level &Lc = levels_[l + 1];
   LevelVecs<T> &v = vecs<T>(l);
   LevelVecs<T> &c = vecs<T>(l + 1);
   const int deg = std::max(1, o.cheb_degree);
@@ -969,40 +1044,65 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o) {
   if (cur == x) std::swap(cur, oth);
   {
     const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
-    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), v.d.p, cur, st_);
+    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), v.d.p, cur, st_, nrhs);
   }
   for (int k = 1; k < deg; k++) {
-    cheb_step<T>(l, b, cur, oth, k, o);
+    cheb_step<T>(l, b, cur, oth, k, o, nrhs);
     std::swap(cur, oth);
   }
   AsmArgs a;
   a.b = b;
   a.y = v.r.p;
-  apply_level<T>(l, cur, ASM_RESID, a);
-  launch_restrict<T>(L.R, levels_[l + 1].d_bcmask.p, v.r.p, c.b.p, st_);
-  vcycle<T>(l + 1, c.b.p, c.x.p, o);
-  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, cur, st_);
+  apply_level<T>(l, cur, ASM_RESID, a, nrhs);
+  launch_restrict<T>(L.R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof);
+  vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs);
+  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
   for (int k = 0; k < deg; k++) {
-    cheb_step<T>(l, b, cur, oth, k, o);
+    cheb_step<T>(l, b, cur, oth, k, o, nrhs);
     std::swap(cur, oth);
+  }
+}
+
+// z = M r for every right-hand side: one V-cycle, FP32 (mixed) or FP64
+void Problem::precond(int nrhs, const SolveOpts &o) {
+  Level &L0 = levels_[0];
+  const int64_t m = L0.ndof * nrhs;
+  if (o.precision != 0) {
+    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs);
+    launch_convert<float, double>(m, L0.v32.x.p, z_.p, st_);
+  } else {
+    vcycle<double>(0, r_.p, z_.p, o, nrhs);
   }
 }
 
 int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &o,
                    SolveInfo *info) {
-  const int64_t n = levels_.empty() ? 0 : levels_[0].ndof;
-  if (!have_design_) {
+  return solve_multi(1, f, u, on_device, o, info);
+}
+
+int Problem::solve_multi(int nrhs, const double *f, double *u, bool on_device, const SolveOpts &o,
+                         SolveInfo *infos) {
+  if (levels_.empty() || !have_design_) {
     last_error = "solve: set the design first";
     return -1;
   }
-  if (f) {
-    HIPCHK(hipMemcpyAsync(f_.p, f, n * sizeof(double),
-                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  if (nrhs < 1) {
+    last_error = "solve: need at least one right-hand side";
+    return -1;
   }
-  int r = solve_core(f_.p, u_.p, o, info);
+  if (!f && nrhs > 1) {
+    last_error = "solve: null load vectors";
+    return -1;
+  }
+  if (ensure_batch(nrhs)) return -1;
+  const int64_t m = levels_[0].ndof * nrhs;
+  if (f)
+    HIPCHK(hipMemcpyAsync(f_.p, f, m * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  const int r = solve_batch(nrhs, f_.p, u_.p, o, infos);
   if (r < 0) return r;
   if (u) {
-    HIPCHK(hipMemcpyAsync(u, u_.p, n * sizeof(double),
+    HIPCHK(hipMemcpyAsync(u, u_.p, m * sizeof(double),
                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
     HIPCHK(hipStreamSynchronize(st_));
   }
@@ -1010,102 +1110,191 @@ int Problem::solve(const double *f, double *u, bool on_device, const SolveOpts &
   return r;
 }
 
-// FP64 flexible PCG on K x = rhs from x = 0; rhs is overwritten with its
-// clamped rows zeroed.  Uses the PCG work vectors r_, p_, q_, z_.
-int Problem::solve_core(double *rhs, double *xsol, const SolveOpts &o, SolveInfo *info) {
+// one flexible-PCG iteration for every running (right-hand side, component):
+// q = A p, alpha, x and r, stop test, z = M r, beta, p; then the running
+// count into pinned slot `slot` and the iteration counter advanced.  Kernels
+// and one copy only, so that it can be captured.
+void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const SolveOpts &o,
+                                int slot) {
+  const bool mixed = o.precision != 0;
+  float *r32 = mixed ? levels_[0].v32.b.p : nullptr;
+  const int nidx = nrhs * nseg_;
+  const double rtol2 = o.rtol * o.rtol, atol2 = o.atol * o.atol;
+  AsmArgs a;
+  a.y = q_.p;
+  a.partials = parts_.p;
+  apply_level<double>(0, p_.p, ASM_DOT, a, nrhs);
+  launch_b_finalize(pl, FIN_PQ, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
+                    st_);
+  launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_);
+  launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
+                    st_);
+  precond(nrhs, o);
+  launch_b_dot2(pl, z_.p, r_.p, q_.p, act_.p, parts_.p, st_);
+  launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
+                    atol2, st_);
+  launch_b_update_p(pl, p_.p, z_.p, sc_.p, act_.p, st_);
+  launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
+  (void)hipMemcpyAsync(nact_host_ + slot, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_);
+}
+
+int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o,
+                         SolveInfo *infos) {
   if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
     if (setup(o)) return -1;
   }
+  const bool mixed = o.precision != 0;
+  for (size_t l = 0; l < levels_.size(); l++)
+    if (ensure_vecs((int)l, mixed ? 1 : 0, nrhs)) {
+      last_error = "solve: device allocation failed";
+      return -1;
+    }
+  if (ensure_batch(nrhs)) return -1;
   fp32_valu_ = o.fp32_valu != 0;
   auto t0 = std::chrono::steady_clock::now();
   Level &L0 = levels_[0];
-  const int64_t n = L0.ndof;
-  const int nrb = reduce_blocks(n);
-  HIPCHK(hipMemsetAsync(xsol, 0, n * sizeof(double), st_));
-  // clamped rows of the right-hand side are zero (TACSAssembler::applyBCs):
-  // r = rhs - A*0 through the assembly pass, which zeroes clamped rows
-  {
-    launch_fill<double>(3 * (int64_t)L0.wg_nodes.n, 0.0, L0.v64.partial.p, st_);
-    AsmArgs a;
-    a.b = rhs;
-    a.y = r_.p;
-    a.x = xsol;
-    launch_assemble<double>(L0.asmc, L0.d_bcmask.p, L0.v64.partial.p, ASM_RESID, a, st_);
-  }
-  HIPCHK(hipMemcpyAsync(rhs, r_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
-  launch_dot1(n, r_.p, r_.p, parts_.p, st_);
-  launch_finalize(nrb, parts_.p, 4, sc_.p, st_);
-  const bool mixed = o.precision != 0;
-  auto precond = [&]() {
-    if (mixed) {
-      vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o);
-      launch_convert<float, double>(n, L0.v32.x.p, z_.p, st_);
-    } else {
-      vcycle<double>(0, r_.p, z_.p, o);
-    }
-  };
-  if (mixed) launch_convert<double, float>(n, r_.p, L0.v32.b.p, st_);
-  precond();
-  launch_dot1(n, r_.p, z_.p, parts_.p, st_);
-  launch_finalize(nrb, parts_.p, 0, sc_.p, st_);
-  HIPCHK(hipMemcpyAsync(p_.p, z_.p, n * sizeof(double), hipMemcpyDeviceToDevice, st_));
-  double hsc[SC_N];
-  HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  const int64_t n = L0.ndof, m = n * nrhs;
+  SegPlan pl = plan_;
+  pl.nrhs = nrhs;
+  const int nidx = nrhs * nseg_;
+  const double rtol2 = o.rtol * o.rtol, atol2 = o.atol * o.atol;
+  float *r32 = mixed ? L0.v32.b.p : nullptr;
+  // x = 0; clamped rows of the right-hand side are zero (TACSAssembler::applyBCs)
+  launch_fill<double>(m, 0.0, xsol, st_);
+  launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, rhs, st_, nrhs, n);
+  HIPCHK(hipMemcpyAsync(r_.p, rhs, m * sizeof(double), hipMemcpyDeviceToDevice, st_));
+  launch_b_dot1(pl, r_.p, r_.p, nullptr, parts_.p, st_);
+  launch_b_finalize(pl, FIN_BB, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
+                    st_);
+  if (mixed) launch_convert<double, float>(m, r_.p, r32, st_);
+  precond(nrhs, o);
+  launch_b_dot1(pl, r_.p, z_.p, act_.p, parts_.p, st_);
+  launch_b_finalize(pl, FIN_RZ, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
+                    st_);
+  launch_b_copy(pl, z_.p, p_.p, nullptr, st_);
+  HIPCHK(hipMemsetAsync(iter_.p, 0, sizeof(int), st_));
+  // count the running segments; iteration counter to 1
+  launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
+  HIPCHK(hipMemcpyAsync(nact_host_, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
-  const double bnorm = std::sqrt(hsc[SC_BB]);
-  double rnorm = bnorm;
-  int it = 0;
-  bool conv = bnorm == 0.0;
-  const int nab = L0.tiles.n_tiles;  // one p.q partial per owner tile
-  while (!conv && it < o.max_iters) {
-    it++;
-    AsmArgs a;
-    a.y = q_.p;
-    a.partials = parts_.p;
-    apply_level<double>(0, p_.p, ASM_DOT, a);
-    launch_finalize(nab, parts_.p, 1, sc_.p, st_);
-    launch_pcg_update_xr(n, xsol, r_.p, p_.p, q_.p, sc_.p, parts_.p, mixed ? L0.v32.b.p : nullptr,
-                         st_);
-    launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
-    HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipStreamSynchronize(st_));
-    rnorm = std::sqrt(std::max(hsc[SC_RR], 0.0));
-    if (o.monitor) fprintf(stderr, "[tmrx] pcg %3d  |r|/|b| = %.3e\n", it, rnorm / bnorm);
-    if (!(rnorm == rnorm)) break;
-    if (rnorm <= o.rtol * bnorm || rnorm <= o.atol) {
-      conv = true;
-      break;
+  int running = nact_host_[0];
+  // iterations: replayed graphs (or enqueued bodies), the stop count of
+  // iteration k read while iteration k + 1 runs
+  const bool graphs = !o.no_graph && !o.monitor && timing_ == 0;
+  if (graphs && running > 0) {
+    const bool same = g_gen_ == gen_ && g_nrhs_ == nrhs && g_xsol_ == xsol &&
+                      g_opts_.precision == o.precision &&
+                      g_opts_.fp32_valu == o.fp32_valu && g_opts_.cheb_degree == o.cheb_degree &&
+                      g_opts_.cheb_lower == o.cheb_lower && g_opts_.cheb_upper == o.cheb_upper &&
+                      g_opts_.rtol == o.rtol && g_opts_.atol == o.atol;
+    if (!same || !gexec_[0] || !gexec_[1]) {
+      drop_graphs();
+      if (!cap_st_) HIPCHK(hipStreamCreateWithFlags(&cap_st_, hipStreamNonBlocking));
+      hipStream_t keep = st_;
+      for (int k = 0; k < 2; k++) {
+        hipGraph_t g = nullptr;
+        st_ = cap_st_;
+        hipError_t e = hipStreamBeginCapture(cap_st_, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+          enqueue_iteration(pl, nrhs, xsol, o, k);
+          e = hipStreamEndCapture(cap_st_, &g);
+        }
+        st_ = keep;
+        if (e == hipSuccess) e = take_launch_error();
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec_[k], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          drop_graphs();
+          last_error = std::string("solve: graph capture failed: ") + hipGetErrorString(e);
+          return -1;
+        }
+      }
+      g_gen_ = gen_;
+      g_nrhs_ = nrhs;
+      g_xsol_ = xsol;
+      g_opts_ = o;
     }
-    precond();
-    launch_dot2(n, z_.p, r_.p, q_.p, parts_.p, st_);
-    launch_finalize(nrb, parts_.p, 3, sc_.p, st_);
-    launch_pcg_update_p(n, p_.p, z_.p, sc_.p, st_);
   }
-  // true residual with the FP64 operator
+  for (int k = 0; k < 2; k++)
+    if (!gev_[k]) HIPCHK(hipEventCreateWithFlags(&gev_[k], hipEventDisableTiming));
+  int launched = 0;
+  while (running > 0 && launched < o.max_iters) {
+    const int slot = launched & 1;
+    if (graphs)
+      HIPCHK(hipGraphLaunch(gexec_[slot], st_));
+    else
+      enqueue_iteration(pl, nrhs, xsol, o, slot);
+    HIPCHK(hipEventRecord(gev_[slot], st_));
+    launched++;
+    if (o.monitor || launched >= 2) {
+      const int prev = o.monitor ? slot : (slot ^ 1);
+      HIPCHK(hipEventSynchronize(gev_[prev]));
+      running = nact_host_[prev];
+      if (o.monitor) fprintf(stderr, "[tmrx] pcg %3d  running %d of %d\n", launched, running, nidx);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st_));
+  // true residual with the FP64 operator, per component
   {
     AsmArgs a;
     a.b = rhs;
     a.y = q_.p;
-    apply_level<double>(0, xsol, ASM_RESID, a);
-    launch_dot1(n, q_.p, q_.p, parts_.p, st_);
-    launch_finalize(nrb, parts_.p, 2, sc_.p, st_);
-    HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipStreamSynchronize(st_));
-    last_true_residual = bnorm > 0 ? std::sqrt(std::max(hsc[SC_RR], 0.0)) / bnorm : 0.0;
+    apply_level<double>(0, xsol, ASM_RESID, a, nrhs);
+    launch_b_dot1(pl, q_.p, q_.p, nullptr, parts_.p, st_);
+    launch_b_finalize(pl, FIN_TRUE, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
+                      atol2, st_);
   }
-  if (info) {
-    info->iters = it;
-    info->converged = conv ? 1 : 0;
-    info->rnorm0 = bnorm;
-    info->rnorm = rnorm;
-    info->setup_ms = last_setup_ms_;
-    info->solve_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  seg_sc_.resize((size_t)nidx * SC_N);
+  seg_itc_.resize(nidx);
+  seg_act_.resize(nidx);
+  HIPCHK(hipMemcpyAsync(seg_sc_.data(), sc_.p, seg_sc_.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(seg_itc_.data(), itconv_.p, nidx * sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(seg_act_.data(), act_.p, nidx * sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  last_nrhs_ = nrhs;
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  bool all_conv = true;
+  for (int r = 0; r < nrhs; r++) {
+    double bb = 0, rr = 0, trr = 0;
+    int iters = 0;
+    bool conv = true;
+    for (int sg = 0; sg < nseg_; sg++) {
+      const double *h = &seg_sc_[(size_t)(r * nseg_ + sg) * SC_N];
+      const int itc = seg_itc_[r * nseg_ + sg];
+      bb += h[SC_BB];
+      rr += h[SC_RR];
+      trr += h[SC_TRR];
+      iters = std::max(iters, std::abs(itc));
+      if (seg_act_[r * nseg_ + sg] || itc < 0) conv = false;
+    }
+    if (r == 0) last_true_residual = bb > 0 ? std::sqrt(trr / bb) : 0.0;
+    if (infos) {
+      SolveInfo &in = infos[r];
+      in.iters = iters;
+      in.converged = conv ? 1 : 0;
+      in.rnorm0 = std::sqrt(bb);
+      in.rnorm = std::sqrt(std::max(rr, 0.0));
+      in.setup_ms = last_setup_ms_;
+      in.solve_ms = ms;
+    }
+    all_conv = all_conv && conv;
   }
-  if (!conv) {
+  if (!all_conv) {
     last_error = "solve: PCG did not converge to the requested tolerance";
     return 1;
   }
+  return 0;
+}
+
+int Problem::segment_info(int rhs, int seg, int *iters, int *converged, double *rel_res) const {
+  if (rhs < 0 || rhs >= last_nrhs_ || seg < 0 || seg >= nseg_) return -1;
+  const int k = rhs * nseg_ + seg;
+  const double *h = &seg_sc_[(size_t)k * SC_N];
+  if (iters) *iters = std::abs(seg_itc_[k]);
+  if (converged) *converged = (!seg_act_[k] && seg_itc_[k] >= 0) ? 1 : 0;
+  if (rel_res) *rel_res = h[SC_BB] > 0 ? std::sqrt(h[SC_TRR] / h[SC_BB]) : 0.0;
   return 0;
 }
 
@@ -1226,7 +1415,7 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   HIPCHK(hipStreamSynchronize(st_));
   // adjoint K psi = df/du (K symmetric); u_ and f_ are left untouched
   const double tr = last_true_residual;
-  int r = solve_core(adjf_.p, psi_.p, o, info);
+  int r = solve_batch(1, adjf_.p, psi_.p, o, info);
   last_adjoint_residual = last_true_residual;
   last_true_residual = tr;
   if (r != 0) return r < 0 ? -1 : 1;
@@ -1315,7 +1504,112 @@ int Problem::traction_load(const char *name, const double tr[3], double *f) {
   return 0;
 }
 
-template void Problem::vcycle<float>(int, const float *, float *, const SolveOpts &);
-template void Problem::vcycle<double>(int, const double *, double *, const SolveOpts &);
+// Owner-tile size autotuner: for each candidate size, rebuild the tiles of
+// every level and time the finest-level operator (FP32 product on the matrix
+// cores 4x, as in a V-cycle's smoothing and residual, plus one FP64 product,
+// as in a PCG iteration); keep the fastest.  Needs a design.
+int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double *ms_out) {
+  if (!have_design_) {
+    last_error = "autotune: set the design first";
+    return -1;
+  }
+  if (ncand < 1 || !cands) {
+    last_error = "autotune: no candidate tile sizes";
+    return -1;
+  }
+  for (int c = 0; c < ncand; c++)
+    if (cands[c] < 64 || cands[c] > 4096) {
+      last_error = "autotune: tile sizes must lie in [64, 4096]";
+      return -1;
+    }
+  reps = std::max(1, reps);
+  if (ensure_vecs(0, 0) || ensure_vecs(0, 1)) {
+    last_error = "autotune: allocation failed";
+    return -1;
+  }
+  Level &L0 = levels_[0];
+  const int64_t n = L0.ndof;
+  launch_fill<float>(n, 1.0f, L0.v32.b.p, st_);
+  launch_fill<double>(n, 1.0, L0.v64.b.p, st_);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const bool keep_valu = fp32_valu_;
+  fp32_valu_ = false;
+  int dev = 0, max_lds = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int best = -1;
+  double best_ms = 0.0;
+  for (int c = 0; c < ncand; c++) {
+    bool fits = true;
+    for (size_t l = 0; l < levels_.size(); l++) {
+      levels_[l].tile_nodes = cands[c];
+      if (build_tiles((int)l)) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return -1;
+      }
+      const Tiles &tl = levels_[l].tiles;
+      const size_t lds = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
+      if (lds + 64 > (size_t)max_lds) fits = false;
+    }
+    if (!fits) {  // FP64 tiles would not fit in LDS
+      if (ms_out) ms_out[c] = -1.0;
+      continue;
+    }
+    AsmArgs a32, a64;
+    a32.y = L0.v32.r.p;
+    a64.y = L0.v64.r.p;
+    apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);  // warm
+    apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
+    (void)hipEventRecord(e0, st_);
+    for (int r = 0; r < reps; r++) {
+      for (int k = 0; k < 4; k++) apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);
+      apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
+    }
+    (void)hipEventRecord(e1, st_);
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    const double per = ms / reps;
+    if (ms_out) ms_out[c] = per;
+    if (best < 0 || per < best_ms) {
+      best = c;
+      best_ms = per;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  fp32_valu_ = keep_valu;
+  if (best < 0) {
+    last_error = "autotune: no candidate tile size fits in LDS";
+    best = -1;
+    for (size_t l = 0; l < levels_.size(); l++) {
+      levels_[l].tile_nodes = TILE_NODES;
+      if (build_tiles((int)l)) return -1;
+    }
+    if (build_plan()) return -1;
+    const int cap0 = std::max(1, batch_cap_);
+    batch_cap_ = 0;
+    if (ensure_batch(cap0)) return -1;
+    return -1;
+  }
+  for (size_t l = 0; l < levels_.size(); l++) {
+    levels_[l].tile_nodes = cands[best];
+    if (build_tiles((int)l)) return -1;
+  }
+  // tiles moved: reduction plan, partial buffers and captured graphs follow
+  if (build_plan()) return -1;
+  const int cap = std::max(1, batch_cap_);
+  batch_cap_ = 0;
+  if (ensure_batch(cap)) return -1;
+  gen_++;
+  if (chosen) *chosen = cands[best];
+  return 0;
+}
+
+template void Problem::vcycle<float>(int, const float *, float *, const SolveOpts &, int);
+template void Problem::vcycle<double>(int, const double *, double *, const SolveOpts &, int);
 
 }  // namespace tmrx

@@ -79,6 +79,7 @@ struct SolveOpts {
   int monitor = 0;
   int zero_guess = 1;
   int fp32_valu = 0;  // FP32 operator on the vector ALUs (else matrix cores)
+  int no_graph = 0;   // enqueue every PCG iteration instead of replaying a HIP graph
 };
 
 struct SolveInfo {
@@ -92,8 +93,9 @@ struct SolveInfo {
 
 template <class T>
 struct LevelVecs {
-  DevArray<T> b, x, x2, r, d, dinv, partial, s;
+  DevArray<T> b, x, x2, r, d, dinv, partial, s;  // b, x, x2, r, d: [cap][ndof]
   bool ready = false;
+  int cap = 0;  // right-hand sides the per-rhs vectors hold
 };
 
 struct Level {
@@ -101,8 +103,10 @@ struct Level {
   int n_nodes = 0, n_dep = 0, n_elem = 0;
   int64_t ndof = 0;
   // independent components (separate problems sharing the hierarchy):
-  // nodes [seg_node0[s], seg_node0[s+1]) belong to component s
-  std::vector<int> seg_node0;
+  // nodes [seg_node0[s], seg_node0[s+1]) belong to component s, owner
+  // tiles [seg_tile[s], seg_tile[s+1]) hold them
+  std::vector<int> seg_node0, seg_tile;
+  int tile_nodes = TILE_NODES;  // owner-tile size (autotuned)
   // host copies
   std::vector<int16_t> lev;
   std::vector<int> cls;
@@ -149,6 +153,17 @@ class Problem {
   int set_design(const double *rho, int n, bool on_device);
   int setup(const SolveOpts &o);
   int solve(const double *f, double *u, bool on_device, const SolveOpts &o, SolveInfo *info);
+  // nrhs load cases at once: f, u are [nrhs][ndof]; infos[nrhs].  Load case
+  // 0 becomes the current state.
+  int solve_multi(int nrhs, const double *f, double *u, bool on_device, const SolveOpts &o,
+                  SolveInfo *infos);
+  // per independent component of the last solve: iterations, converged,
+  // true relative residual |f - K u| / |f| over the component's dofs
+  int segment_info(int rhs, int seg, int *iters, int *converged, double *rel_res) const;
+  int num_segments() const { return nseg_; }
+  // owner-tile size autotuner (ms per candidate, chosen size)
+  int autotune(const int *cands, int ncand, int reps, int *chosen, double *ms);
+  int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
   int set_state(const double *f, const double *u, bool on_device);
   int compliance(double *c);
   int compliance_grad(double *g, bool on_device);
@@ -175,7 +190,9 @@ class Problem {
   int build_level(int l, int nbcs, const char **names, const int *masks);
   int build_transfer(int l);
   int build_tiles(int l);
-  int ensure_vecs(int l, int precision);
+  int ensure_vecs(int l, int precision, int nrhs = 1);
+  int ensure_batch(int nrhs);
+  int build_plan();
   template <class T>
   LevelVecs<T> &vecs(int l);
   template <class T>
@@ -183,14 +200,18 @@ class Problem {
   template <class T>
   const T *scale(int l);
   template <class T>
-  void apply_level(int l, const T *x, int mode, const AsmArgs &a);
+  void apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrhs = 1);
   template <class T>
-  void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o);
+  void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
+                 int nrhs = 1);
   template <class T>
-  void vcycle(int l, const T *b, T *x, const SolveOpts &o);
+  void vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs = 1);
+  void precond(int nrhs, const SolveOpts &o);
   int coarse_factor();
   int find_components();
-  int solve_core(double *rhs, double *x, const SolveOpts &o, SolveInfo *info);
+  // batched FP64 flexible PCG from x = 0 on rhs ([nrhs][ndof], clamped rows
+  // zeroed in place); work vectors r_, p_, q_, z_
+  int solve_batch(int nrhs, double *rhs, double *x, const SolveOpts &o, SolveInfo *infos);
   int estimate_eigs(int l, const SolveOpts &o);
 
   std::vector<Level> levels_;
@@ -207,8 +228,28 @@ class Problem {
   CoarseBlocks coarse_;
   int nseg_ = 1;                 // independent components
   std::vector<int> seg_block0_;  // [nseg + 1] block ranges of the components
-  // PCG state
+  // PCG state: f_, u_ hold the current load case (first of a batch)
   DevArray<double> f_, u_, r_, p_, q_, z_, parts_, sc_;
+  int batch_cap_ = 0;  // right-hand sides the PCG vectors hold
+  SegPlan plan_;
+  DevArray<int> pl_blk_lo_, pl_blk_seg_, pl_seg_blk_, pl_seg_tile_;
+  DevArray<int> act_, itconv_, iter_, nact_;
+  DevArray<double> thr_;
+  int *nact_host_ = nullptr;  // pinned, two slots (iterations alternate)
+  // PCG iteration bodies captured as HIP graphs (two: they write alternate
+  // stop-count slots), valid for one (nrhs, options, buffers, setup)
+  hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+  hipEvent_t gev_[2] = {nullptr, nullptr};
+  hipStream_t cap_st_ = nullptr;
+  int g_nrhs_ = 0, g_gen_ = -1;
+  double *g_xsol_ = nullptr;
+  SolveOpts g_opts_;
+  int gen_ = 0;  // bumped whenever buffers move or setup reruns
+  void drop_graphs();
+  void enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const SolveOpts &o, int slot);
+  int last_nrhs_ = 0;
+  std::vector<double> seg_sc_;
+  std::vector<int> seg_itc_, seg_act_;
   DevArray<double> energy_, psi_, adjf_, dfdrho_, CB_, cvol_;
   bool have_design_ = false, have_setup_ = false, have_solution_ = false;
   int setup_precision_ = -1;

@@ -96,6 +96,59 @@ class Problem:
                                  % (self.info.iters, self.info.rnorm / max(self.info.rnorm0, 1e-300)))
         return u
 
+    def solve_multi(self, F, U=None, opts=None, allow_unconverged=False):
+        """Solve for every load case (rows of F, shape [nrhs, ndof]) in one
+        batched PCG; returns U of the same shape; per-case infos in
+        self.infos.  Load case 0 becomes the current state."""
+        opts = opts or default_opts()
+        if _is_torch(F):
+            import torch
+            if F.dim() != 2:
+                raise ValueError("F must be [nrhs, ndof]")
+            U = torch.empty_like(F) if U is None else U
+        else:
+            F = np.ascontiguousarray(F, dtype=np.double)
+            if F.ndim != 2:
+                raise ValueError("F must be [nrhs, ndof]")
+            U = np.zeros_like(F) if U is None else U
+        nrhs = F.shape[0]
+        if F.shape[1] != self.ndof:
+            raise ValueError("F has %d columns, expected %d" % (F.shape[1], self.ndof))
+        infos = (_lib.SolveInfo * nrhs)()
+        fp, dev = _ptr(F)
+        up, _ = _ptr(U)
+        r = lib().tmrx_problem_solve_multi(self.ptr, nrhs, fp, up, dev, C.byref(opts), infos)
+        check(r, "solve_multi")
+        self.infos = list(infos)
+        self.info = self.infos[0]
+        if r == 1 and not allow_unconverged:
+            bad = [k for k, i in enumerate(self.infos) if not i.converged]
+            raise _lib.TmrxError("solve_multi: load cases %s did not converge" % bad)
+        return U
+
+    def num_components(self):
+        n = C.c_int()
+        check(lib().tmrx_problem_num_components(self.ptr, C.byref(n)))
+        return n.value
+
+    def component_info(self, comp, rhs=0):
+        """(iterations, converged, true relative residual) of one
+        independent component in the last solve."""
+        it, cv, rr = C.c_int(), C.c_int(), C.c_double()
+        check(lib().tmrx_problem_component_info(self.ptr, int(rhs), int(comp), C.byref(it),
+                                                C.byref(cv), C.byref(rr)), "component_info")
+        return it.value, bool(cv.value), rr.value
+
+    def autotune(self, tile_sizes=(256, 512, 1024), reps=5):
+        """Pick the owner-tile size with the fastest finest-level operator;
+        returns (chosen size, {size: ms})."""
+        n = len(tile_sizes)
+        ts = (C.c_int * n)(*tile_sizes)
+        ms = (C.c_double * n)()
+        ch = C.c_int()
+        check(lib().tmrx_problem_autotune(self.ptr, ts, n, int(reps), C.byref(ch), ms), "autotune")
+        return ch.value, {int(t): (m if m >= 0 else None) for t, m in zip(tile_sizes, ms)}
+
     def true_residual(self):
         v = C.c_double()
         check(lib().tmrx_problem_true_residual(self.ptr, C.byref(v)))

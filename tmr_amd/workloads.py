@@ -140,3 +140,59 @@ def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), ma
     cen = X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h
     rho = gaussian_field(cen, seed, L)
     return forests, rho, L
+
+
+def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
+    """Independent cantilevers solved side by side: one forest holding a
+    disjoint box lattice per problem (dims in blocks of edge h, each block at
+    `level`), placed along x with a gap.  Every problem clamps its own x-
+    face ("fixed") and is loaded on its x+ face ("load").  Returns (forests,
+    rho) with seeded smoothed densities per problem."""
+    conns, Xs, names = [], [], []
+    off_v, off_b, x0 = 0, 0, 0.0
+    for (nx, ny, nz) in dims_list:
+        conn, X = TMR.box_connectivity(nx, ny, nz, L=(nx * h, ny * h, nz * h))
+        X = X.copy()
+        X[:, 0] += x0
+        conns.append(conn + off_v)
+        Xs.append(X)
+        for b in range(len(conn)):
+            i = b % nx
+            if i == 0:
+                names.append((off_b + b, 0, "fixed"))
+            if i == nx - 1:
+                names.append((off_b + b, 1, "load"))
+        off_v += len(X)
+        off_b += len(conn)
+        x0 += nx * h + gap
+    f = TMR.OctForest()
+    f.setConnectivity(np.vstack(conns).astype(np.intc))
+    f.setNodePoints(np.vstack(Xs))
+    bfc = f.getConnectivity()["block_face_conn"]
+    for b, fi, name in names:
+        f.setName("face", int(bfc[b, fi]), name)
+    f.createTrees(level)
+    forests = hierarchy(f, nlevels)
+    octs = forests[0].getOctants()
+    n = 1 << level
+    rho = np.empty(len(octs))
+    hh = 1 << (30 - level)
+    off_b = 0
+    for k, (nx, ny, nz) in enumerate(dims_list):
+        nb = nx * ny * nz
+        sel = (octs["block"] >= off_b) & (octs["block"] < off_b + nb)
+        grid = smoothed_density((nz * n, ny * n, nx * n), seed + k, passes=passes)
+        b = octs["block"][sel] - off_b
+        ix = (b % nx) * n + octs["x"][sel] // hh
+        iy = ((b // nx) % ny) * n + octs["y"][sel] // hh
+        iz = (b // (nx * ny)) * n + octs["z"][sel] // hh
+        rho[sel] = grid[iz, iy, ix]
+        off_b += nb
+    return forests, rho
+
+
+def cube_batch(count=96, level=3, nlevels=4, seed=0):
+    """`count` cubes of edge 8k elements (k = 1..6 in turn: k x k x k blocks
+    at level 3), the uneven-size batch of independent problems."""
+    ks = [1 + (i % 6) for i in range(count)]
+    return box_batch([(k, k, k) for k in ks], level, nlevels, seed=seed)
