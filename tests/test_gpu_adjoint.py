@@ -90,4 +90,6 @@ def test_autograd_functions(adapt):
     prob.set_design(rho)
     prob.solve(f, opts=solver.default_opts(rtol=1e-10))
     assert float(s) == pytest.approx(prob.stress_pnorm(8.0), rel=1e-9)
-    assert np.allclose(g_s, prob.stress_pnorm_grad(8.0), rtol=1e-6, atol=1e-12)
+    # same options as the autograd path (rtol 1e-10 for both solves)
+    g_ref = prob.stress_pnorm_grad(8.0, opts=solver.default_opts(rtol=1e-10))
+    assert np.allclose(g_s, g_ref, rtol=1e-12, atol=0)

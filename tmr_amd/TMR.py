@@ -291,6 +291,7 @@ def createBoxForest(nx, ny, nz, L=(1.0, 1.0, 1.0), face_names=None, comm=None):
 
 
 # topology-optimisation classes (tmr/TMR.pyx:3877-6005) live in topo.py
-from .topo import (BoundaryConditions, ConformFilter, LagrangeFilter, MaterialProperties,  # noqa: E402,F401
-                   OctConformTopoCreator, OctConstitutive, OctTopoCreator, StiffnessProperties,
-                   TopoProblem, createMg, RAMP_PENALTY, SIMP_PENALTY)
+from .topo import (BoundaryConditions, Compliance, ConformFilter, LagrangeFilter,  # noqa: E402,F401
+                   MaterialProperties, OctConformTopoCreator, OctConstitutive, OctTopoCreator,
+                   PNormFailure, StiffnessProperties, StructuralMass, TopoProblem, createMg,
+                   RAMP_PENALTY, SIMP_PENALTY)

@@ -494,16 +494,22 @@ class TopoProblem:
         self.last_info = []
         if self.forces:
             self.problem.setup(self._opts())
+            # every load case in one batched solve (TMRTopoProblem.cpp:1003-1009
+            # solves them one after another)
+            cases = [i for i, f in enumerate(self.forces) if f is not None]
+            if cases:
+                U = self.problem.solve_multi(np.stack([self.forces[i] for i in cases]),
+                                             opts=self._opts(), allow_unconverged=True)
+                for j, i in enumerate(cases):
+                    self.vars[i] = np.array(U[j])
+                    info = self.problem.infos[j]
+                    self.last_info.append((info.iters, info.converged))
+                    if not info.converged:
+                        fail = 1
+                self._current = cases[0]
             for i, f in enumerate(self.forces):
                 if f is None:
                     continue
-                r = self.problem.solve(f, opts=self._opts(), allow_unconverged=True)
-                self.vars[i] = np.array(r)
-                self._current = i
-                info = self.problem.info
-                self.last_info.append((info.iters, info.converged))
-                if not info.converged:
-                    fail = 1
                 if self.obj_funcs and self.obj_funcs[i] is not None:
                     fobj += self.obj_weights[i] * self._eval_func(self.obj_funcs[i], i)
                 elif not self.obj_funcs:
