@@ -74,6 +74,7 @@ struct Tiles {
   int n_tiles = 0;
   bool single_class = true;
   int max_slots = 0, max_acc = 0;
+  int max_rec = 0, max_mrec = 0;  // records per tile (plain, matrix-core padded)
   int ncolors = 0;
   const int *node0 = nullptr;     // [n_tiles + 1]
   const int *ext_ptr = nullptr;   // [n_tiles + 1] non-owned slots

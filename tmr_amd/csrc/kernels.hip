@@ -750,8 +750,9 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // owner-tile apply: one pass, final values of the owned nodes, fused epilogue
 
 // stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
-// hanging-node interpolation, clamped dofs zero) and zero the accumulators
-template <class T>
+// hanging-node interpolation, clamped dofs zero) and zero the accumulators;
+// S floats per node (S = 4 pads each node to 16 bytes with a zero)
+template <class T, int S>
 __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
                                            int nacc, const int *__restrict__ dep_ptr,
                                            const int *__restrict__ dep_conn,
@@ -761,21 +762,23 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
   const int tid = threadIdx.x;
   for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
     const int j = i / 3, c = i - 3 * j;
-    xs[3 * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
+    xs[S * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
   }
   for (int i = tid; i < next; i += TILE_THREADS) {
     T v[3];
     load_node<T>(ext_l[i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
-    const int q = 3 * tile_pos(nown + i);
+    const int q = S * tile_pos(nown + i);
     xs[q] = v[0];
     xs[q + 1] = v[1];
     xs[q + 2] = v[2];
   }
-  for (int i = tid; i < 3 * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
+  if (S == 4)
+    for (int i = tid; i < ((nown + next + 7) & ~7); i += TILE_THREADS) xs[4 * i + 3] = T(0);
+  for (int i = tid; i < S * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
 }
 
 // owned rows: add the hanging-node sums, then the mode's epilogue
-template <class T, int MODE>
+template <class T, int MODE, int S>
 __device__ __forceinline__ void tile_finish(
     int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
     const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
@@ -785,14 +788,14 @@ __device__ __forceinline__ void tile_finish(
   double dot = 0.0;
   for (int i = threadIdx.x; i < nown; i += TILE_THREADS) {
     const int node = n0 + i;
-    const int pi = 3 * tile_pos(i);
+    const int pi = S * tile_pos(i);
     T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
     for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
       const int sl = comb_slot[j];
       const T w = (T)comb_w[j];
-      a[0] += w * ys[3 * sl];
-      a[1] += w * ys[3 * sl + 1];
-      a[2] += w * ys[3 * sl + 2];
+      a[0] += w * ys[S * sl];
+      a[1] += w * ys[S * sl + 1];
+      a[2] += w * ys[S * sl + 2];
     }
     const uint8_t m = bcmask[node];
 #pragma unroll
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   const int nacc = nown + nacc_dep[t];
   T *xs = reinterpret_cast<T *>(smem_raw);
   T *ys = xs + 3 * ((nslot + 7) & ~7);
-  tile_stage<T>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  tile_stage<T, 3>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
   __syncthreads();
   const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
   for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
@@ -917,59 +920,52 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       __syncthreads();
     }
   }
-  tile_finish<T, MODE>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y, x_out,
-                       d, dinv, ca, cc, partials, red);
+  tile_finish<T, MODE, 3>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
+                          x_out, d, dinv, ca, cc, partials, red);
 }
 
 // ---------------------------------------------------------------------------
 // FP32 owner-tile apply on the matrix cores.  A wave takes 16 elements of one
 // colour and class at a time and forms Y (24 x 16) = K0 (24 x 24) X (24 x 16)
-// with v_mfma_f32_16x16x32_bf16: rows padded to two 16-row blocks, K padded
-// to 32, each FP32 operand split into bf16 hi + lo and the product taken as
-// hi*hi + hi*lo + lo*hi (about 16 significant bits per product, FP32
-// accumulation).  Lane l holds element n = l % 16 and the K (input dof) /
-// M (output row) group g = l / 16: B[k][n] for k = 8g .. 8g+7 and
-// D[4g + v][n] for v = 0..3 of each row block.
+// with v_mfma_f32_16x16x32_bf16.  The dof orderings are permuted so that lane
+// group g = l / 16 of element n = l % 16 owns corners 2g and 2g + 1: K index
+// 8g + 4c + j is component j of corner 2g + c, and row 4g + j of row block b
+// is component j of corner 2g + b (j = 3 pads, zero in A).  LDS keeps 4
+// floats per node, so a corner is one 16-byte read and one 16-byte update.
+// Every FP32 operand is split into three bf16 parts (hi + mid + lo, 24
+// significant bits) and the six products down to 2^-16 relative are summed in
+// FP32: the product keeps FP32 accuracy, which the coarse-grid correction
+// needs (residuals of smooth errors cancel to a small fraction of |K||x|).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+union Frag {
+  uint4 u;
+  bf16x8_t v;
+};
 
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
-
-// corner / component of input dof 8g + v (v compile-time, g = lane group)
-template <int V>
-__device__ __forceinline__ int kslot(int g, const int lc[8]) {
-  constexpr int c0 = V / 3, c1 = (8 + V) / 3, c2 = (16 + V) / 3;
-  constexpr int p0 = V % 3, p1 = (8 + V) % 3, p2 = (16 + V) % 3;
-  return g == 0 ? 3 * lc[c0] + p0 : g == 1 ? 3 * lc[c1] + p1 : 3 * lc[c2] + p2;
-}
-
-// add one output row of an element: row 16 BLK + 4g + V of lane group g
-template <int V, int BLK>
-__device__ __forceinline__ void rscatter(int g, const int lc[8], int mask, float val, float *ys) {
-  constexpr int r0 = 16 * BLK + V, r1 = r0 + 4, r2 = r0 + 8, r3 = r0 + 12;
-  int cr, comp;
-  if (BLK == 0) {
-    cr = g == 0 ? r0 / 3 : g == 1 ? r1 / 3 : g == 2 ? r2 / 3 : r3 / 3;
-    comp = g == 0 ? r0 % 3 : g == 1 ? r1 % 3 : g == 2 ? r2 % 3 : r3 % 3;
-  } else {
-    cr = g == 0 ? r0 / 3 : r1 / 3;
-    comp = g == 0 ? r0 % 3 : r1 % 3;
-    if (g >= 2) return;
-  }
-  const int sl = g == 0 ? lc[r0 / 3] : g == 1 ? lc[r1 / 3] : g == 2 ? lc[(r2 < 24 ? r2 : 0) / 3]
-                                                                    : lc[(r3 < 24 ? r3 : 0) / 3];
-  if (mask & (1 << cr)) ys[3 * sl + comp] += val;
+// three bf16 parts of a pair of floats, packed two per word
+__device__ __forceinline__ void split3(float a, float b, uint32_t &h, uint32_t &m, uint32_t &l) {
+  union P {
+    __bf16 e[2];
+    uint32_t u;
+  } ph, pm, pl;
+  ph.e[0] = (__bf16)a;
+  ph.e[1] = (__bf16)b;
+  const float ra = a - (float)ph.e[0], rb = b - (float)ph.e[1];
+  pm.e[0] = (__bf16)ra;
+  pm.e[1] = (__bf16)rb;
+  pl.e[0] = (__bf16)(ra - (float)pm.e[0]);
+  pl.e[1] = (__bf16)(rb - (float)pm.e[1]);
+  h = ph.u;
+  m = pm.u;
+  l = pl.u;
 }
 
 template <int MODE, bool SINGLE_CLASS>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
     const int *__restrict__ nacc_dep, const int *__restrict__ mb_ptr,
-    const uint2 *__restrict__ mel, int ncolors, const uint4 *__restrict__ afrag,
+    const uint32_t *__restrict__ mel, int ncolors, const uint4 *__restrict__ afrag,
     const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
     const float *__restrict__ comb_w, const int *__restrict__ dep_ptr,
     const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
@@ -997,92 +993,91 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
   const int nslot = nown + next;
   const int nacc = nown + nacc_dep[t];
   float *xs = reinterpret_cast<float *>(smem_raw);
-  float *ys = xs + 3 * ((nslot + 7) & ~7);
-  tile_stage<float>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
-  // A fragments of K0: [class][hi, lo][row block][lane]
-  union Frag {
-    uint4 u;
-    bf16x8_t v;
-  };
-  Frag ahi0, ahi1, alo0, alo1;
+  float *ys = xs + 4 * ((nslot + 7) & ~7);
+  const float4 *xs4 = reinterpret_cast<const float4 *>(xs);
+  float4 *ys4 = reinterpret_cast<float4 *>(ys);
+  const int *cp = mb_ptr + (size_t)t * (MAX_COLORS + 1);
+  const int rb0 = 16 * cp[0], nrec = 16 * cp[MAX_COLORS] - rb0;
+  // the tile's element records follow the accumulators; the element index
+  // word is replaced by the element's stiffness scale
+  uint32_t *recs = reinterpret_cast<uint32_t *>(ys + 4 * ((nacc + 7) & ~7));
+  for (int i = tid; i < 6 * nrec; i += TILE_THREADS) {
+    uint32_t w = mel[6 * (size_t)rb0 + i];
+    if (i % 6 == 4) w = __float_as_uint(s_elem[w]);
+    recs[i] = w;
+  }
+  tile_stage<float, 4>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  // A fragments of K0: [class][part hi, mid, lo][row block][lane]
+  Frag a[3][2];
   if (SINGLE_CLASS) {
-    ahi0.u = afrag[0 * 64 + lane];
-    ahi1.u = afrag[1 * 64 + lane];
-    alo0.u = afrag[2 * 64 + lane];
-    alo1.u = afrag[3 * 64 + lane];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+      a[p][0].u = afrag[(2 * p) * 64 + lane];
+      a[p][1].u = afrag[(2 * p + 1) * 64 + lane];
+    }
   }
   __syncthreads();
-  const int *cp = mb_ptr + (size_t)t * (MAX_COLORS + 1);
   for (int c = 0; c < ncolors; c++) {
     const int b0 = cp[c], b1 = cp[c + 1];
     for (int bt = b0 + wave; bt < b1; bt += TILE_THREADS / 64) {
-      const size_t rec = 3 * ((size_t)bt * 16 + en);
-      const uint2 w0 = mel[rec], w1 = mel[rec + 1], w2 = mel[rec + 2];
-      int lc[8];
-      lc[0] = w0.x & 0xffff; lc[1] = w0.x >> 16;
-      lc[2] = w0.y & 0xffff; lc[3] = w0.y >> 16;
-      lc[4] = w1.x & 0xffff; lc[5] = w1.x >> 16;
-      lc[6] = w1.y & 0xffff; lc[7] = w1.y >> 16;
-      const int elem = (int)w2.x;
-      const int mask = w2.y & 0xff;
+      // record words: 0-3 corner positions (two per word), 4 stiffness
+      // scale, 5 mask | colour << 8 | class << 16
+      const uint32_t *rec = recs + 6 * (16 * bt - rb0 + en);
+      const uint32_t pair = rec[g];
+      const float sc = __uint_as_float(rec[4]);
+      const uint32_t meta = rec[5];
       if (!SINGLE_CLASS) {
-        const int cls = __builtin_amdgcn_readfirstlane((int)((w2.y >> 16) & 0xff));
-        const uint4 *af = afrag + (size_t)cls * 256;
-        ahi0.u = af[0 * 64 + lane];
-        ahi1.u = af[1 * 64 + lane];
-        alo0.u = af[2 * 64 + lane];
-        alo1.u = af[3 * 64 + lane];
-      }
-      // B fragment: x of dofs 8g .. 8g+7 of element en (none for g = 3)
-      float xv[8];
-      if (g < 3) {
-        xv[0] = xs[kslot<0>(g, lc)];
-        xv[1] = xs[kslot<1>(g, lc)];
-        xv[2] = xs[kslot<2>(g, lc)];
-        xv[3] = xs[kslot<3>(g, lc)];
-        xv[4] = xs[kslot<4>(g, lc)];
-        xv[5] = xs[kslot<5>(g, lc)];
-        xv[6] = xs[kslot<6>(g, lc)];
-        xv[7] = xs[kslot<7>(g, lc)];
-      } else {
+        const int cls = __builtin_amdgcn_readfirstlane((int)((meta >> 16) & 0xff));
+        const uint4 *af = afrag + (size_t)cls * 384;
 #pragma unroll
-        for (int v = 0; v < 8; v++) xv[v] = 0.0f;
+        for (int p = 0; p < 3; p++) {
+          a[p][0].u = af[(2 * p) * 64 + lane];
+          a[p][1].u = af[(2 * p + 1) * 64 + lane];
+        }
       }
-      Frag bhi, blo;
-      uint32_t hw[4], lw[4];
+      const int p0 = pair & 0xffff, p1 = pair >> 16;
+      const float4 v0 = xs4[p0], v1 = xs4[p1];
+      Frag bh, bm, bl;
+      uint32_t h[4], m[4], l[4];
+      split3(v0.x, v0.y, h[0], m[0], l[0]);
+      split3(v0.z, 0.0f, h[1], m[1], l[1]);
+      split3(v1.x, v1.y, h[2], m[2], l[2]);
+      split3(v1.z, 0.0f, h[3], m[3], l[3]);
+      bh.u = make_uint4(h[0], h[1], h[2], h[3]);
+      bm.u = make_uint4(m[0], m[1], m[2], m[3]);
+      bl.u = make_uint4(l[0], l[1], l[2], l[3]);
+      f32x4_t acc[2];
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const uint32_t h0 = bf16_bits(xv[2 * v]), h1 = bf16_bits(xv[2 * v + 1]);
-        const uint32_t l0 = bf16_bits(xv[2 * v] - __uint_as_float(h0 << 16));
-        const uint32_t l1 = bf16_bits(xv[2 * v + 1] - __uint_as_float(h1 << 16));
-        hw[v] = h0 | (h1 << 16);
-        lw[v] = l0 | (l1 << 16);
+      for (int blk = 0; blk < 2; blk++) {
+        f32x4_t z = {0.0f, 0.0f, 0.0f, 0.0f};
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, bh.v, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bm.v, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bl.v, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bh.v, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bm.v, z, 0, 0, 0);
+        acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bh.v, z, 0, 0, 0);
       }
-      bhi.u = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-      blo.u = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-      f32x4_t acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo0.v, bhi.v, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo1.v, bhi.v, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi0.v, blo.v, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi1.v, blo.v, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi0.v, bhi.v, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi1.v, bhi.v, acc1, 0, 0, 0);
-      const float sc = s_elem[elem];
-      // scatter rows 4g + v (block 0) and 16 + 4g + v (block 1, g < 2):
-      // row r -> corner r / 3, component r % 3; only the tile's corners
-      rscatter<0, 0>(g, lc, mask, sc * acc0[0], ys);
-      rscatter<1, 0>(g, lc, mask, sc * acc0[1], ys);
-      rscatter<2, 0>(g, lc, mask, sc * acc0[2], ys);
-      rscatter<3, 0>(g, lc, mask, sc * acc0[3], ys);
-      rscatter<0, 1>(g, lc, mask, sc * acc1[0], ys);
-      rscatter<1, 1>(g, lc, mask, sc * acc1[1], ys);
-      rscatter<2, 1>(g, lc, mask, sc * acc1[2], ys);
-      rscatter<3, 1>(g, lc, mask, sc * acc1[3], ys);
+      const int mask = meta & 0xff;
+      // rows of block b are the components of corner 2g + b
+      if (mask & (1 << (2 * g))) {
+        float4 w = ys4[p0];
+        w.x += sc * acc[0][0];
+        w.y += sc * acc[0][1];
+        w.z += sc * acc[0][2];
+        ys4[p0] = w;
+      }
+      if (mask & (2 << (2 * g))) {
+        float4 w = ys4[p1];
+        w.x += sc * acc[1][0];
+        w.y += sc * acc[1][1];
+        w.z += sc * acc[1][2];
+        ys4[p1] = w;
+      }
     }
     __syncthreads();
   }
-  tile_finish<float, MODE>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
-                           x_out, d, dinv, ca, cc, partials, red);
+  tile_finish<float, MODE, 4>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
+                              x_out, d, dinv, ca, cc, partials, red);
 }
 
 template <class T>
@@ -1097,10 +1092,12 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   const bool single = tl.single_class;
   if constexpr (sizeof(T) == 4) {
     if (tl.use_mfma) {
+      const size_t sh4 = 4 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(float) +
+                         24 * (size_t)tl.max_mrec;
 #define TMRX_MTILE(M, S)                                                                         \
-  hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), grid, dim3(TILE_THREADS), sh, st,                \
+  hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), grid, dim3(TILE_THREADS), sh4, st,               \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.mb_ptr,                       \
-                     reinterpret_cast<const uint2 *>(tl.mel), tl.ncolors, tl.afrag, tl.comb_ptr, \
+                     reinterpret_cast<const uint32_t *>(tl.mel), tl.ncolors, tl.afrag, tl.comb_ptr, \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,      \
                      (const float *)s_elem, (const float *)x, (const float *)b, (float *)y,      \
                      (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.partials, vs)

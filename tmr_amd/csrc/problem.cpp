@@ -162,28 +162,43 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     }
   }
   std::vector<float> K0f(K0h_.begin(), K0h_.end());
-  // MFMA A fragments of the FP32 K0: [class][hi, lo][row block][lane][8 bf16],
-  // lane l holds rows 16 blk + l % 16, columns 8 (l / 16) + v, zero padded
-  std::vector<uint16_t> af(2048 * (size_t)nclasses_, 0);
+  // MFMA A fragments of the FP32 K0 for the matrix-core tile apply:
+  // [class][part hi, mid, lo][row block][lane][8 bf16].  Lane l holds row
+  // l % 16 of the block and K columns 8 (l / 16) .. +7.  Row 4q + j of block
+  // b is component j of corner 2q + b; column 8q + 4c + j is component j of
+  // corner 2q + c; j = 3 is zero padding.
+  std::vector<uint16_t> af(3072 * (size_t)nclasses_, 0);
   auto bf16 = [](float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
   };
+  auto bf16f = [](uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
   for (int c = 0; c < nclasses_; c++)
     for (int blk = 0; blk < 2; blk++)
       for (int l = 0; l < 64; l++)
         for (int v = 0; v < 8; v++) {
-          const int row = 16 * blk + l % 16, col = 8 * (l / 16) + v;
-          const float k = (row < 24 && col < 24) ? K0f[576 * c + 24 * row + col] : 0.0f;
+          const int i = l % 16, q = i / 4, j = i % 4;
+          const int gk = l / 16, cc = v / 4, jj = v % 4;
+          float k = 0.0f;
+          if (j < 3 && jj < 3) {
+            const int row = 3 * (2 * q + blk) + j, col = 3 * (2 * gk + cc) + jj;
+            k = K0f[576 * c + 24 * row + col];
+          }
           const uint16_t hi = bf16(k);
-          uint32_t hb = (uint32_t)hi << 16;
-          float hf;
-          std::memcpy(&hf, &hb, 4);
-          const uint16_t lo = bf16(k - hf);
-          af[2048 * (size_t)c + 512 * (0 + blk) + 8 * l + v] = hi;
-          af[2048 * (size_t)c + 512 * (2 + blk) + 8 * l + v] = lo;
+          const float r = k - bf16f(hi);
+          const uint16_t mid = bf16(r);
+          const uint16_t lo = bf16(r - bf16f(mid));
+          const size_t base = 3072 * (size_t)c + 8 * (size_t)l + v;
+          af[base + 512 * (0 + blk)] = hi;
+          af[base + 512 * (2 + blk)] = mid;
+          af[base + 512 * (4 + blk)] = lo;
         }
   if (K0d_.upload(K0h_) || K0f_.upload(K0f) || afrag_.upload(af)) {
     last_error = "problem: device allocation failed";
@@ -402,7 +417,7 @@ int Problem::build_tiles(int l) {
   std::vector<float> comb_w;
   std::vector<int> stamp_n(n + nd, -1), slot(n + nd, 0), stamp_e(ne, -1);
   std::vector<uint32_t> col;
-  int max_slots = 0, max_acc = 0, ncolors = 0;
+  int max_slots = 0, max_acc = 0, ncolors = 0, max_mrec = 0, max_rec = 0;
   bool single = true;
   for (int e = 0; e < ne; e++)
     if (L.cls[e] != L.cls[0]) single = false;
@@ -512,6 +527,11 @@ int Problem::build_tiles(int l) {
     el_ptr.push_back((int)els.size());
     max_slots = std::max(max_slots, ns);
     max_acc = std::max(max_acc, na);
+    max_rec = std::max(max_rec, (int)(els.size() - el0));
+    {
+      const int *cp = &mb_ptr[(size_t)t * (MAX_COLORS + 1)];
+      max_mrec = std::max(max_mrec, 16 * (cp[MAX_COLORS] - cp[0]));
+    }
     for (int p = n0; p < n1; p++) {
       for (int j = pptr[p]; j < pptr[p + 1]; j++) {
         comb_slot.push_back(tile_pos(slot[n + pdep[j]]));
@@ -537,6 +557,8 @@ int Problem::build_tiles(int l) {
   T.n_tiles = ntiles;
   T.single_class = single;
   T.max_slots = max_slots;
+  T.max_mrec = max_mrec;
+  T.max_rec = max_rec;
   T.max_acc = max_acc;
   T.ncolors = ncolors;
   T.node0 = L.t_node0.p;
@@ -1551,10 +1573,12 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
         return -1;
       }
       const Tiles &tl = levels_[l].tiles;
-      const size_t lds = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
-      if (lds + 64 > (size_t)max_lds) fits = false;
+      const size_t lds64 = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
+      const size_t lds32 = 4 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(float) +
+                           24 * (size_t)tl.max_mrec;
+      if (std::max(lds64, lds32) + 64 > (size_t)max_lds) fits = false;
     }
-    if (!fits) {  // FP64 tiles would not fit in LDS
+    if (!fits) {  // the tiles would not fit in LDS
       if (ms_out) ms_out[c] = -1.0;
       continue;
     }
