@@ -50,6 +50,7 @@ struct ElemBatches {
 // hanging node one of them supports).  LDS slots: owned nodes, then the
 // hanging nodes whose sums the tile needs ("accumulated"), then halo nodes.
 constexpr int TILE_NODES = 512;
+constexpr int MFMA_WAVES = 4;  // waves of a matrix-core tile, each owning a share of its nodes
 constexpr int TILE_THREADS = 256;
 
 // LDS position of tile slot s: an XOR of higher bits into the low three bits
@@ -89,11 +90,14 @@ struct Tiles {
   const int *dep_conn = nullptr;
   const float *dep_w = nullptr;
   const uint8_t *bcmask = nullptr;
-  // matrix-core (FP32) form: records in batches of 16 of one colour and
-  // class (padded with mask-0 records), batch offsets per tile and colour,
-  // and K0 of every class split into bf16 hi/lo MFMA A fragments
-  const int *mb_ptr = nullptr;    // [n_tiles * (MAX_COLORS + 1)]
-  const TileElem *mel = nullptr;  // [16 * batches]
+  // matrix-core (FP32) form: per tile and wave, records of the elements
+  // touching the wave's share of the nodes, in batches of 16 of one colour
+  // and class (padded with mask-0 records); K0 of every class split into
+  // bf16 hi/mid/lo MFMA A fragments
+  const int *mb_ptr = nullptr;    // [n_tiles * (MFMA_WAVES + 1)] batch offsets
+  TileElem *mel = nullptr;        // [16 * batches]; elem word holds the FP32 scale
+  const int *mel_elem = nullptr;  // [16 * batches] element of each record
+  int n_mrec = 0;
   const uint4 *afrag = nullptr;   // [n_classes][hi, lo][row block][64 lanes]
   bool use_mfma = false;
 };
@@ -178,6 +182,9 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st);
 void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st,
                          int nrhs = 1, int64_t vs = 0);
 
+// word 4 of every matrix-core record <- s32[its element]
+void launch_rec_scale(int nrec, const int *rec_elem, const float *s32, uint32_t *rec_words,
+                      hipStream_t st);
 // element stiffness scales for a level: s = penalty(rho) * 2^-level
 void launch_simp_scale(int n, const double *rho, const int16_t *level, double q, double k0,
                        int penalty_type, double *s64, float *s32, hipStream_t st);

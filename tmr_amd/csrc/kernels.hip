@@ -463,6 +463,22 @@ void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStrea
   TMRX_LAUNCH_CHECK();
 }
 
+// matrix-core records carry their element's FP32 stiffness scale in word 4
+__global__ void k_rec_scale(int nrec, const int *__restrict__ rec_elem,
+                            const float *__restrict__ s32, uint32_t *__restrict__ rec_words) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  rec_words[6 * (size_t)i + 4] = __float_as_uint(s32[rec_elem[i]]);
+}
+
+void launch_rec_scale(int nrec, const int *rec_elem, const float *s32, uint32_t *rec_words,
+                      hipStream_t st) {
+  if (nrec == 0) return;
+  hipLaunchKernelGGL(k_rec_scale, dim3((nrec + 255) / 256), dim3(256), 0, st, nrec, rec_elem, s32,
+                     rec_words);
+  TMRX_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // SIMP / RAMP penalty (src/topology/TMROctConstitutive.cpp:498-551) times
 // the element size factor 2^-level.
@@ -752,7 +768,7 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
 // hanging-node interpolation, clamped dofs zero) and zero the accumulators;
 // S floats per node (S = 4 pads each node to 16 bytes with a zero)
-template <class T, int S>
+template <class T, int S, int NT = TILE_THREADS>
 __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
                                            int nacc, const int *__restrict__ dep_ptr,
                                            const int *__restrict__ dep_conn,
@@ -760,11 +776,28 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
                                            const uint8_t *__restrict__ bcmask,
                                            const T *__restrict__ x, T *xs, T *ys) {
   const int tid = threadIdx.x;
-  for (int i = tid; i < 3 * nown; i += TILE_THREADS) {
-    const int j = i / 3, c = i - 3 * j;
-    xs[S * tile_pos(j) + c] = ((bcmask[n0 + j] >> c) & 1) ? T(0) : x[3 * (size_t)n0 + i];
+  const int n3 = 3 * nown;
+  for (int base = 0; base < n3; base += 4 * NT) {  // four loads in flight per thread
+    T v[4];
+    uint8_t mk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = base + k * NT + tid;
+      if (i < n3) {
+        v[k] = x[3 * (size_t)n0 + i];
+        mk[k] = bcmask[n0 + i / 3];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = base + k * NT + tid;
+      if (i < n3) {
+        const int j = i / 3, c = i - 3 * j;
+        xs[S * tile_pos(j) + c] = ((mk[k] >> c) & 1) ? T(0) : v[k];
+      }
+    }
   }
-  for (int i = tid; i < next; i += TILE_THREADS) {
+  for (int i = tid; i < next; i += NT) {
     T v[3];
     load_node<T>(ext_l[i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
     const int q = S * tile_pos(nown + i);
@@ -773,22 +806,52 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
     xs[q + 2] = v[2];
   }
   if (S == 4)
-    for (int i = tid; i < ((nown + next + 7) & ~7); i += TILE_THREADS) xs[4 * i + 3] = T(0);
-  for (int i = tid; i < S * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = T(0);
+    for (int i = tid; i < ((nown + next + 7) & ~7); i += NT) xs[4 * i + 3] = T(0);
+  for (int i = tid; i < S * ((nacc + 7) & ~7); i += NT) ys[i] = T(0);
 }
 
 // owned rows: add the hanging-node sums, then the mode's epilogue
-template <class T, int MODE, int S>
+// x of the bf16 planes (hi, mid, lo: 4 bf16 per node each, planes pstride
+// nodes apart): hi + mid + lo restores the FP32 value exactly
+__device__ __forceinline__ float plane_get(uint2 w, int c) {
+  const uint32_t u = c == 0 ? (w.x << 16) : c == 1 ? (w.x & 0xffff0000u) : (w.y << 16);
+  return __uint_as_float(u);
+}
+
+template <class T, int MODE, int S, int NT = TILE_THREADS, bool SPLIT = false>
 __device__ __forceinline__ void tile_finish(
     int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
     const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
     const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out,
     T *__restrict__ d, const T *__restrict__ dinv, double ca, double cc,
-    double *__restrict__ partials, double *red) {
+    double *__restrict__ partials, double *red, int pstride = 0) {
+  auto xat = [&](int pos, int c) -> T {
+    if constexpr (SPLIT) {
+      const uint2 *ph = reinterpret_cast<const uint2 *>(xs);
+      return (plane_get(ph[pos], c) + plane_get(ph[pos + pstride], c)) +
+             plane_get(ph[pos + 2 * pstride], c);
+    } else {
+      return xs[S * pos + c];
+    }
+  };
   double dot = 0.0;
-  for (int i = threadIdx.x; i < nown; i += TILE_THREADS) {
+  for (int i = threadIdx.x; i < nown; i += NT) {
     const int node = n0 + i;
     const int pi = S * tile_pos(i);
+    // the epilogue's vector loads go out before the hanging-node sums
+    T bv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)}, wv[3] = {T(0), T(0), T(0)};
+    T xv[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const size_t k = 3 * (size_t)node + c;
+      if (MODE == ASM_RESID || MODE == ASM_CHEB) bv[c] = b[k];
+      if (MODE == ASM_CHEB) {
+        dv[c] = d[k];
+        wv[c] = dinv[k];
+      }
+      if (MODE == ASM_STORE || MODE == ASM_DOT) xv[c] = x[k];
+    }
+    const uint8_t m = bcmask[node];
     T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
     for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
       const int sl = comb_slot[j];
@@ -797,29 +860,28 @@ __device__ __forceinline__ void tile_finish(
       a[1] += w * ys[S * sl + 1];
       a[2] += w * ys[S * sl + 2];
     }
-    const uint8_t m = bcmask[node];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const size_t k = 3 * (size_t)node + c;
       const bool bc = (m >> c) & 1;
       if (MODE == ASM_STORE) {
-        y[k] = bc ? x[k] : a[c];
+        y[k] = bc ? xv[c] : a[c];
       } else if (MODE == ASM_DOT) {
-        const T xv = bc ? x[k] : xs[pi + c];
-        const T v = bc ? xv : a[c];
+        const T xx = bc ? xv[c] : xat(pi / S, c);
+        const T v = bc ? xx : a[c];
         y[k] = v;
-        dot += (double)xv * (double)v;
+        dot += (double)xx * (double)v;
       } else if (MODE == ASM_RESID) {
-        y[k] = bc ? T(0) : b[k] - a[c];
+        y[k] = bc ? T(0) : bv[c] - a[c];
       } else if (MODE == ASM_CHEB) {
         if (bc) {
           d[k] = T(0);
           x_out[k] = T(0);
         } else {
-          const T r = b[k] - a[c];
-          const T dn = (T)ca * d[k] + (T)cc * dinv[k] * r;
+          const T r = bv[c] - a[c];
+          const T dn = (T)ca * dv[c] + (T)cc * wv[c] * r;
           d[k] = dn;
-          x_out[k] = xs[pi + c] + dn;
+          x_out[k] = xat(pi / S, c) + dn;
         }
       }
     }
@@ -961,6 +1023,60 @@ __device__ __forceinline__ void split3(float a, float b, uint32_t &h, uint32_t &
   l = pl.u;
 }
 
+// Every wave owns a contiguous share of the tile's nodes (and the hanging
+// nodes whose first owned parent it owns) and works through the batches of
+// the elements touching them, in order and with the rows of its own corners
+// only: no two waves write the same accumulator, so the element phase needs
+// no barriers, and records are fetched three batches ahead and x one batch
+// ahead of the batch on the matrix cores.
+// stage x as three bf16 planes (hi, mid, lo; 4 bf16 per node: x, y, z, 0),
+// each np nodes long: the matrix-core operands are read ready-split
+__device__ __forceinline__ void put_planes(uint2 *ph, int np, int pos, float vx, float vy,
+                                           float vz) {
+  uint32_t h0, m0, l0, h1, m1, l1;
+  split3(vx, vy, h0, m0, l0);
+  split3(vz, 0.0f, h1, m1, l1);
+  ph[pos] = make_uint2(h0, h1);
+  ph[pos + np] = make_uint2(m0, m1);
+  ph[pos + 2 * np] = make_uint2(l0, l1);
+}
+
+__device__ __forceinline__ void tile_stage_planes(int n0, int nown, const int *__restrict__ ext_l,
+                                                  int next, int nacc, int np,
+                                                  const int *__restrict__ dep_ptr,
+                                                  const int *__restrict__ dep_conn,
+                                                  const float *__restrict__ dep_w,
+                                                  const uint8_t *__restrict__ bcmask,
+                                                  const float *__restrict__ x, uint2 *ph,
+                                                  float *ys) {
+  const int tid = threadIdx.x;
+  for (int base = 0; base < nown; base += 2 * TILE_THREADS) {  // two nodes in flight
+    float v[2][3];
+    uint8_t mk[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int j = min(base + k * TILE_THREADS + tid, nown - 1);
+      mk[k] = bcmask[n0 + j];
+      v[k][0] = x[3 * (size_t)(n0 + j)];
+      v[k][1] = x[3 * (size_t)(n0 + j) + 1];
+      v[k][2] = x[3 * (size_t)(n0 + j) + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int j = base + k * TILE_THREADS + tid;
+      if (j < nown)
+        put_planes(ph, np, tile_pos(j), (mk[k] & 1) ? 0.0f : v[k][0], (mk[k] & 2) ? 0.0f : v[k][1],
+                   (mk[k] & 4) ? 0.0f : v[k][2]);
+    }
+  }
+  for (int i = tid; i < next; i += TILE_THREADS) {
+    float v[3];
+    load_node<float>(ext_l[i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
+    put_planes(ph, np, tile_pos(nown + i), v[0], v[1], v[2]);
+  }
+  for (int i = tid; i < 4 * ((nacc + 7) & ~7); i += TILE_THREADS) ys[i] = 0.0f;
+}
+
 template <int MODE, bool SINGLE_CLASS>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
@@ -976,7 +1092,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   {  // right-hand side blockIdx.y of a batch of vectors, stride vs
     const int64_t off = (int64_t)blockIdx.y * vs;
     x += off;
@@ -986,27 +1102,33 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     if (d) d += off;
     if (partials) partials += (int64_t)blockIdx.y * gridDim.x;
   }
-  const int lane = tid & 63, wave = tid >> 6;
   const int en = lane & 15, g = lane >> 4;
   const int n0 = node0[t], nown = node0[t + 1] - n0;
   const int e0 = ext_ptr[t], next = ext_ptr[t + 1] - e0;
   const int nslot = nown + next;
   const int nacc = nown + nacc_dep[t];
-  float *xs = reinterpret_cast<float *>(smem_raw);
-  float *ys = xs + 4 * ((nslot + 7) & ~7);
-  const float4 *xs4 = reinterpret_cast<const float4 *>(xs);
+  // LDS: x as three bf16 planes of np nodes (8 bytes per node each), then
+  // the FP32 accumulators (16 bytes per node)
+  const int np = (nslot + 7) & ~7;
+  uint2 *ph = reinterpret_cast<uint2 *>(smem_raw);
+  float *ys = reinterpret_cast<float *>(ph + 3 * np);
   float4 *ys4 = reinterpret_cast<float4 *>(ys);
-  const int *cp = mb_ptr + (size_t)t * (MAX_COLORS + 1);
-  const int rb0 = 16 * cp[0], nrec = 16 * cp[MAX_COLORS] - rb0;
-  // the tile's element records follow the accumulators; the element index
-  // word is replaced by the element's stiffness scale
-  uint32_t *recs = reinterpret_cast<uint32_t *>(ys + 4 * ((nacc + 7) & ~7));
-  for (int i = tid; i < 6 * nrec; i += TILE_THREADS) {
-    uint32_t w = mel[6 * (size_t)rb0 + i];
-    if (i % 6 == 4) w = __float_as_uint(s_elem[w]);
-    recs[i] = w;
-  }
-  tile_stage<float, 4>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  const int *cp = mb_ptr + (size_t)t * (MFMA_WAVES + 1);
+  const int bb0 = cp[wv], bb1 = cp[wv + 1];
+  // record words: 0-3 corner positions (two per word), 4 FP32 stiffness
+  // scale, 5 mask | colour << 8 | class << 16
+  auto load_rec = [&](int bt, uint32_t &pr, float &sc, uint32_t &mt) {
+    const uint32_t *rec = mel + 6 * ((size_t)bt * 16 + en);
+    pr = rec[g];
+    sc = __uint_as_float(rec[4]);
+    mt = rec[5];
+  };
+  uint32_t pc = 0, mc = 0, p1 = 0, m1 = 0, p2 = 0, m2 = 0, p3 = 0, m3 = 0;
+  float sc = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  if (bb0 < bb1) load_rec(bb0, pc, sc, mc);
+  if (bb0 + 1 < bb1) load_rec(bb0 + 1, p1, s1, m1);
+  if (bb0 + 2 < bb1) load_rec(bb0 + 2, p2, s2, m2);
+  tile_stage_planes(n0, nown, ext + e0, next, nacc, np, dep_ptr, dep_conn, dep_w, bcmask, x, ph, ys);
   // A fragments of K0: [class][part hi, mid, lo][row block][lane]
   Frag a[3][2];
   if (SINGLE_CLASS) {
@@ -1015,69 +1137,95 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
       a[p][0].u = afrag[(2 * p) * 64 + lane];
       a[p][1].u = afrag[(2 * p + 1) * 64 + lane];
     }
+    // keep them in registers: without this the compiler reloads them from
+    // memory in every batch to save registers
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+        asm volatile("" : "+v"(a[p][q].u.x), "+v"(a[p][q].u.y), "+v"(a[p][q].u.z),
+                     "+v"(a[p][q].u.w));
   }
   __syncthreads();
-  for (int c = 0; c < ncolors; c++) {
-    const int b0 = cp[c], b1 = cp[c + 1];
-    for (int bt = b0 + wave; bt < b1; bt += TILE_THREADS / 64) {
-      // record words: 0-3 corner positions (two per word), 4 stiffness
-      // scale, 5 mask | colour << 8 | class << 16
-      const uint32_t *rec = recs + 6 * (16 * bt - rb0 + en);
-      const uint32_t pair = rec[g];
-      const float sc = __uint_as_float(rec[4]);
-      const uint32_t meta = rec[5];
-      if (!SINGLE_CLASS) {
-        const int cls = __builtin_amdgcn_readfirstlane((int)((meta >> 16) & 0xff));
-        const uint4 *af = afrag + (size_t)cls * 384;
+  // B fragments of a batch: the planes of its two corners per lane
+  auto load_b = [&](uint32_t pr, Frag &h, Frag &m, Frag &l) {
+    const int q0 = pr & 0xffff, q1 = pr >> 16;
+    const uint2 a0 = ph[q0], a1 = ph[q1], b0 = ph[q0 + np], b1 = ph[q1 + np];
+    const uint2 c0 = ph[q0 + 2 * np], c1 = ph[q1 + 2 * np];
+    h.u = make_uint4(a0.x, a0.y, a1.x, a1.y);
+    m.u = make_uint4(b0.x, b0.y, b1.x, b1.y);
+    l.u = make_uint4(c0.x, c0.y, c1.x, c1.y);
+  };
+  Frag bh, bm, bl;
+  bh.u = bm.u = bl.u = make_uint4(0u, 0u, 0u, 0u);
+  if (bb0 < bb1) load_b(pc, bh, bm, bl);
+  for (int bt = bb0; bt < bb1; bt++) {
+    if (bt + 3 < bb1) load_rec(bt + 3, p3, s3, m3);
+    Frag nh = bh, nm = bm, nl = bl;
+    if (bt + 1 < bb1) load_b(p1, nh, nm, nl);  // next batch (x is read-only here)
+    if (!SINGLE_CLASS) {
+      const int cls = __builtin_amdgcn_readfirstlane((int)((mc >> 16) & 0xff));
+      const uint4 *af = afrag + (size_t)cls * 384;
 #pragma unroll
-        for (int p = 0; p < 3; p++) {
-          a[p][0].u = af[(2 * p) * 64 + lane];
-          a[p][1].u = af[(2 * p + 1) * 64 + lane];
-        }
-      }
-      const int p0 = pair & 0xffff, p1 = pair >> 16;
-      const float4 v0 = xs4[p0], v1 = xs4[p1];
-      Frag bh, bm, bl;
-      uint32_t h[4], m[4], l[4];
-      split3(v0.x, v0.y, h[0], m[0], l[0]);
-      split3(v0.z, 0.0f, h[1], m[1], l[1]);
-      split3(v1.x, v1.y, h[2], m[2], l[2]);
-      split3(v1.z, 0.0f, h[3], m[3], l[3]);
-      bh.u = make_uint4(h[0], h[1], h[2], h[3]);
-      bm.u = make_uint4(m[0], m[1], m[2], m[3]);
-      bl.u = make_uint4(l[0], l[1], l[2], l[3]);
-      f32x4_t acc[2];
-#pragma unroll
-      for (int blk = 0; blk < 2; blk++) {
-        f32x4_t z = {0.0f, 0.0f, 0.0f, 0.0f};
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, bh.v, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bm.v, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bl.v, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bh.v, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bm.v, z, 0, 0, 0);
-        acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bh.v, z, 0, 0, 0);
-      }
-      const int mask = meta & 0xff;
-      // rows of block b are the components of corner 2g + b
-      if (mask & (1 << (2 * g))) {
-        float4 w = ys4[p0];
-        w.x += sc * acc[0][0];
-        w.y += sc * acc[0][1];
-        w.z += sc * acc[0][2];
-        ys4[p0] = w;
-      }
-      if (mask & (2 << (2 * g))) {
-        float4 w = ys4[p1];
-        w.x += sc * acc[1][0];
-        w.y += sc * acc[1][1];
-        w.z += sc * acc[1][2];
-        ys4[p1] = w;
+      for (int p = 0; p < 3; p++) {
+        a[p][0].u = af[(2 * p) * 64 + lane];
+        a[p][1].u = af[(2 * p + 1) * 64 + lane];
       }
     }
-    __syncthreads();
+    // four independent accumulation chains (two per row block: the small
+    // cross terms, then the leading terms) so the matrix core pipelines
+    f32x4_t acc[2], sm[2];
+    const f32x4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++) {
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, bh.v, zero, 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bh.v, zero, 0, 0, 0);
+    }
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++) {
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bm.v, sm[blk], 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bm.v, acc[blk], 0, 0, 0);
+    }
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++) {
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bl.v, sm[blk], 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bh.v, acc[blk], 0, 0, 0);
+    }
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++) acc[blk] += sm[blk];
+    const int mask = mc & 0xff;
+    // rows of block b are the components of corner 2g + b
+    if (mask & (1 << (2 * g))) {
+      float4 w = ys4[pc & 0xffff];
+      w.x += sc * acc[0][0];
+      w.y += sc * acc[0][1];
+      w.z += sc * acc[0][2];
+      ys4[pc & 0xffff] = w;
+    }
+    if (mask & (2 << (2 * g))) {
+      float4 w = ys4[pc >> 16];
+      w.x += sc * acc[1][0];
+      w.y += sc * acc[1][1];
+      w.z += sc * acc[1][2];
+      ys4[pc >> 16] = w;
+    }
+    pc = p1;
+    sc = s1;
+    mc = m1;
+    bh = nh;
+    bm = nm;
+    bl = nl;
+    p1 = p2;
+    s1 = s2;
+    m1 = m2;
+    p2 = p3;
+    s2 = s3;
+    m2 = m3;
   }
-  tile_finish<float, MODE, 4>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
-                              x_out, d, dinv, ca, cc, partials, red);
+  __syncthreads();
+  tile_finish<float, MODE, 4, TILE_THREADS, true>(
+      t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, reinterpret_cast<const float *>(ph), ys, x,
+      b, y, x_out, d, dinv, ca, cc, partials, red, np);
 }
 
 template <class T>
@@ -1092,8 +1240,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   const bool single = tl.single_class;
   if constexpr (sizeof(T) == 4) {
     if (tl.use_mfma) {
-      const size_t sh4 = 4 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(float) +
-                         24 * (size_t)tl.max_mrec;
+      const size_t sh4 = 24 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
 #define TMRX_MTILE(M, S)                                                                         \
   hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), grid, dim3(TILE_THREADS), sh4, st,               \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.mb_ptr,                       \

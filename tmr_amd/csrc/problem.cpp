@@ -412,7 +412,7 @@ int Problem::build_tiles(int l) {
   const int ntiles = (int)tstart.size() - 1;
   std::vector<int> node0(ntiles + 1), ext_ptr{0}, ext, nacc(ntiles), el_ptr{0};
   std::vector<TileElem> els, mel;
-  std::vector<int> mb_ptr((size_t)ntiles * (MAX_COLORS + 1), 0);
+  std::vector<int> mb_ptr((size_t)ntiles * (MFMA_WAVES + 1), 0);
   std::vector<int> comb_ptr(n + 1, 0), comb_slot;
   std::vector<float> comb_w;
   std::vector<int> stamp_n(n + nd, -1), slot(n + nd, 0), stamp_e(ne, -1);
@@ -421,12 +421,14 @@ int Problem::build_tiles(int l) {
   bool single = true;
   for (int e = 0; e < ne; e++)
     if (L.cls[e] != L.cls[0]) single = false;
-  std::vector<int> tel, rdeps;
+  std::vector<int> tel, rdeps, rdep_wave;
   for (int t = 0; t < ntiles; t++) {
     const int n0 = tstart[t], n1 = std::min(tstart[t + 1], n0 + L.tile_nodes), nown = n1 - n0;
     node0[t] = n0;
     rdeps.clear();
+    rdep_wave.clear();
     tel.clear();
+    const int qw = (nown + MFMA_WAVES - 1) / MFMA_WAVES;  // owned nodes per wave
     for (int p = n0; p < n1; p++) {
       stamp_n[p] = t;
       slot[p] = p - n0;
@@ -436,6 +438,7 @@ int Problem::build_tiles(int l) {
           stamp_n[x] = t;
           slot[x] = nown + (int)rdeps.size();
           rdeps.push_back(pdep[j]);
+          rdep_wave.push_back((p - n0) / qw);  // the wave of its first owned parent
         }
       }
     }
@@ -499,28 +502,61 @@ int Problem::build_tiles(int l) {
     std::stable_sort(els.begin() + el0, els.end(), [](const TileElem &a, const TileElem &b) {
       return a.color != b.color ? a.color < b.color : a.mask > b.mask;
     });
-    // matrix-core batches: (colour, class) groups padded to 16 records
+    // matrix-core batches: each of the MFMA_WAVES waves owns a contiguous
+    // share of the owned nodes (and the hanging nodes whose first owned
+    // parent it owns) and takes every element touching them, with the rows
+    // of those corners only; per wave, records are coloured over its
+    // corners and (colour, class) groups are padded to 16 records
     {
-      std::vector<TileElem> grp(els.begin() + el0, els.end());
-      std::stable_sort(grp.begin(), grp.end(), [](const TileElem &a, const TileElem &b) {
-        return a.color != b.color ? a.color < b.color : a.cls < b.cls;
-      });
-      int *cp = &mb_ptr[(size_t)t * (MAX_COLORS + 1)];
-      size_t k = 0;
-      for (int c = 0; c <= MAX_COLORS; c++) {
-        cp[c] = (int)(mel.size() / 16);
-        while (k < grp.size() && grp[k].color == c) {
-          const uint8_t cl = grp[k].cls;
+      auto wave_of = [&](int pos) {
+        const int sl = tile_pos(pos);  // tile_pos is an involution
+        return sl < nown ? sl / qw : (sl < na ? rdep_wave[sl - nown] : -1);
+      };
+      int *cp = &mb_ptr[(size_t)t * (MFMA_WAVES + 1)];
+      std::vector<TileElem> grp;
+      for (int w = 0; w < MFMA_WAVES; w++) {
+        cp[w] = (int)(mel.size() / 16);
+        grp.clear();
+        for (size_t k = el0; k < els.size(); k++) {
+          TileElem r = els[k];
+          uint8_t m = 0;
+          for (int a = 0; a < 8; a++)
+            if ((r.mask & (1u << a)) && wave_of(r.lc[a]) == w) m |= (uint8_t)(1u << a);
+          if (!m) continue;
+          r.mask = m;
+          grp.push_back(r);
+        }
+        col.assign(round8(na), 0u);
+        for (auto &r : grp) {
+          uint32_t used = 0;
+          for (int a = 0; a < 8; a++)
+            if (r.mask & (1u << a)) used |= col[r.lc[a]];
+          int c = 0;
+          while (used & (1u << c)) c++;
+          if (c >= MAX_COLORS) {
+            last_error = "problem: tile needs too many colours";
+            return -1;
+          }
+          r.color = (uint8_t)c;
+          for (int a = 0; a < 8; a++)
+            if (r.mask & (1u << a)) col[r.lc[a]] |= 1u << c;
+        }
+        std::stable_sort(grp.begin(), grp.end(), [](const TileElem &a, const TileElem &b) {
+          return a.color != b.color ? a.color < b.color : a.cls < b.cls;
+        });
+        for (size_t k = 0; k < grp.size();) {
+          const uint8_t c = grp[k].color, cl = grp[k].cls;
           size_t k1 = k;
           while (k1 < grp.size() && grp[k1].color == c && grp[k1].cls == cl) mel.push_back(grp[k1++]);
           TileElem pad{};
           pad.elem = grp[k].elem;
-          pad.color = (uint8_t)c;
+          pad.color = c;
           pad.cls = cl;
           while (mel.size() % 16) mel.push_back(pad);
           k = k1;
         }
       }
+      cp[MFMA_WAVES] = (int)(mel.size() / 16);
     }
     nacc[t] = (int)rdeps.size();
     ext_ptr.push_back((int)ext.size());
@@ -529,8 +565,8 @@ int Problem::build_tiles(int l) {
     max_acc = std::max(max_acc, na);
     max_rec = std::max(max_rec, (int)(els.size() - el0));
     {
-      const int *cp = &mb_ptr[(size_t)t * (MAX_COLORS + 1)];
-      max_mrec = std::max(max_mrec, 16 * (cp[MAX_COLORS] - cp[0]));
+      const int *cp = &mb_ptr[(size_t)t * (MFMA_WAVES + 1)];
+      max_mrec = std::max(max_mrec, 16 * (cp[MFMA_WAVES] - cp[0]));
     }
     for (int p = n0; p < n1; p++) {
       for (int j = pptr[p]; j < pptr[p + 1]; j++) {
@@ -552,6 +588,14 @@ int Problem::build_tiles(int l) {
       L.t_comb_w.upload(comb_w) || L.t_mb_ptr.upload(mb_ptr) || L.t_mel.upload(mel)) {
     last_error = "problem: device allocation failed";
     return -1;
+  }
+  {
+    std::vector<int> me(mel.size());
+    for (size_t k = 0; k < mel.size(); k++) me[k] = mel[k].elem;
+    if (L.t_mel_elem.upload(me)) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
   }
   Tiles &T = L.tiles;
   T.n_tiles = ntiles;
@@ -576,6 +620,10 @@ int Problem::build_tiles(int l) {
   T.bcmask = L.d_bcmask.p;
   T.mb_ptr = L.t_mb_ptr.p;
   T.mel = L.t_mel.p;
+  T.mel_elem = L.t_mel_elem.p;
+  T.n_mrec = (int)mel.size();
+  if (have_design_)
+    launch_rec_scale(T.n_mrec, T.mel_elem, L.s32.p, reinterpret_cast<uint32_t *>(T.mel), st_);
   T.afrag = reinterpret_cast<const uint4 *>(afrag_.p);
   T.use_mfma = true;
   return 0;
@@ -745,6 +793,9 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
     launch_restrict_scale(Cc.n_elem, F.sc_ptr.p, F.sc_child.p, F.sc_w.p, F.s64.p, F.d_lev.p,
                           Cc.d_lev.p, Cc.s64.p, Cc.s32.p, st_);
   }
+  for (auto &L : levels_)
+    launch_rec_scale(L.tiles.n_mrec, L.tiles.mel_elem, L.s32.p,
+                     reinterpret_cast<uint32_t *>(L.tiles.mel), st_);
   have_design_ = true;
   have_setup_ = false;
   return 0;
@@ -1574,8 +1625,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       }
       const Tiles &tl = levels_[l].tiles;
       const size_t lds64 = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
-      const size_t lds32 = 4 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(float) +
-                           24 * (size_t)tl.max_mrec;
+      const size_t lds32 = 24 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
       if (std::max(lds64, lds32) + 64 > (size_t)max_lds) fits = false;
     }
     if (!fits) {  // the tiles would not fit in LDS

@@ -124,6 +124,7 @@ struct Level {
   // owner tiles (single-pass apply)
   DevArray<int> t_mb_ptr;
   DevArray<TileElem> t_mel;
+  DevArray<int> t_mel_elem;
   DevArray<int> t_node0, t_ext_ptr, t_ext, t_nacc, t_el_ptr, t_comb_ptr, t_comb_slot;
   DevArray<TileElem> t_el;
   DevArray<float> t_comb_w;
