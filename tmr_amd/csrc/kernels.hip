@@ -1114,20 +1114,25 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
   float *ys = reinterpret_cast<float *>(ph + 3 * np);
   float4 *ys4 = reinterpret_cast<float4 *>(ys);
   const int *cp = mb_ptr + (size_t)t * (MFMA_WAVES + 1);
-  const int bb0 = cp[wv], bb1 = cp[wv + 1];
+  // wave-uniform batch range: keeps the loop control scalar
+  const int bb0 = __builtin_amdgcn_readfirstlane(cp[wv]);
+  const int bb1 = __builtin_amdgcn_readfirstlane(cp[wv + 1]);
   // record words: 0-3 corner positions (two per word), 4 FP32 stiffness
   // scale, 5 mask | colour << 8 | class << 16
-  auto load_rec = [&](int bt, uint32_t &pr, float &sc, uint32_t &mt) {
-    const uint32_t *rec = mel + 6 * ((size_t)bt * 16 + en);
-    pr = rec[g];
-    sc = __uint_as_float(rec[4]);
-    mt = rec[5];
+  struct Rec {
+    uint32_t p, m;
+    float s;
   };
-  uint32_t pc = 0, mc = 0, p1 = 0, m1 = 0, p2 = 0, m2 = 0, p3 = 0, m3 = 0;
-  float sc = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-  if (bb0 < bb1) load_rec(bb0, pc, sc, mc);
-  if (bb0 + 1 < bb1) load_rec(bb0 + 1, p1, s1, m1);
-  if (bb0 + 2 < bb1) load_rec(bb0 + 2, p2, s2, m2);
+  auto load_rec = [&](int bt, Rec &r) {
+    const uint32_t *rec = mel + 6 * ((size_t)bt * 16 + en);
+    r.p = rec[g];
+    r.s = __uint_as_float(rec[4]);
+    r.m = rec[5];
+  };
+  Rec r0{0u, 0u, 0.0f}, r1 = r0, r2 = r0, r3 = r0;
+  if (bb0 < bb1) load_rec(bb0, r0);
+  if (bb0 + 1 < bb1) load_rec(bb0 + 1, r1);
+  if (bb0 + 2 < bb1) load_rec(bb0 + 2, r2);
   tile_stage_planes(n0, nown, ext + e0, next, nacc, np, dep_ptr, dep_conn, dep_w, bcmask, x, ph, ys);
   // A fragments of K0: [class][part hi, mid, lo][row block][lane]
   Frag a[3][2];
@@ -1148,23 +1153,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
   }
   __syncthreads();
   // B fragments of a batch: the planes of its two corners per lane
-  auto load_b = [&](uint32_t pr, Frag &h, Frag &m, Frag &l) {
+  auto load_b = [&](uint32_t pr, Frag *B) {
     const int q0 = pr & 0xffff, q1 = pr >> 16;
     const uint2 a0 = ph[q0], a1 = ph[q1], b0 = ph[q0 + np], b1 = ph[q1 + np];
     const uint2 c0 = ph[q0 + 2 * np], c1 = ph[q1 + 2 * np];
-    h.u = make_uint4(a0.x, a0.y, a1.x, a1.y);
-    m.u = make_uint4(b0.x, b0.y, b1.x, b1.y);
-    l.u = make_uint4(c0.x, c0.y, c1.x, c1.y);
+    B[0].u = make_uint4(a0.x, a0.y, a1.x, a1.y);
+    B[1].u = make_uint4(b0.x, b0.y, b1.x, b1.y);
+    B[2].u = make_uint4(c0.x, c0.y, c1.x, c1.y);
   };
-  Frag bh, bm, bl;
-  bh.u = bm.u = bl.u = make_uint4(0u, 0u, 0u, 0u);
-  if (bb0 < bb1) load_b(pc, bh, bm, bl);
-  for (int bt = bb0; bt < bb1; bt++) {
-    if (bt + 3 < bb1) load_rec(bt + 3, p3, s3, m3);
-    Frag nh = bh, nm = bm, nl = bl;
-    if (bt + 1 < bb1) load_b(p1, nh, nm, nl);  // next batch (x is read-only here)
+  auto compute = [&](const Rec &cur, const Frag *B) {
     if (!SINGLE_CLASS) {
-      const int cls = __builtin_amdgcn_readfirstlane((int)((mc >> 16) & 0xff));
+      const int cls = __builtin_amdgcn_readfirstlane((int)((cur.m >> 16) & 0xff));
       const uint4 *af = afrag + (size_t)cls * 384;
 #pragma unroll
       for (int p = 0; p < 3; p++) {
@@ -1178,50 +1177,57 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const f32x4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, bh.v, zero, 0, 0, 0);
-      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bh.v, zero, 0, 0, 0);
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, B[0].v, zero, 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[0].v, zero, 0, 0, 0);
     }
 #pragma unroll
     for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, bm.v, sm[blk], 0, 0, 0);
-      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bm.v, acc[blk], 0, 0, 0);
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[1].v, sm[blk], 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[1].v, acc[blk], 0, 0, 0);
     }
 #pragma unroll
     for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bl.v, sm[blk], 0, 0, 0);
-      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, bh.v, acc[blk], 0, 0, 0);
+      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[2].v, sm[blk], 0, 0, 0);
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[0].v, acc[blk], 0, 0, 0);
     }
 #pragma unroll
     for (int blk = 0; blk < 2; blk++) acc[blk] += sm[blk];
-    const int mask = mc & 0xff;
+    const int mask = cur.m & 0xff;
     // rows of block b are the components of corner 2g + b
     if (mask & (1 << (2 * g))) {
-      float4 w = ys4[pc & 0xffff];
-      w.x += sc * acc[0][0];
-      w.y += sc * acc[0][1];
-      w.z += sc * acc[0][2];
-      ys4[pc & 0xffff] = w;
+      float4 w = ys4[cur.p & 0xffff];
+      w.x += cur.s * acc[0][0];
+      w.y += cur.s * acc[0][1];
+      w.z += cur.s * acc[0][2];
+      ys4[cur.p & 0xffff] = w;
     }
     if (mask & (2 << (2 * g))) {
-      float4 w = ys4[pc >> 16];
-      w.x += sc * acc[1][0];
-      w.y += sc * acc[1][1];
-      w.z += sc * acc[1][2];
-      ys4[pc >> 16] = w;
+      float4 w = ys4[cur.p >> 16];
+      w.x += cur.s * acc[1][0];
+      w.y += cur.s * acc[1][1];
+      w.z += cur.s * acc[1][2];
+      ys4[cur.p >> 16] = w;
     }
-    pc = p1;
-    sc = s1;
-    mc = m1;
-    bh = nh;
-    bm = nm;
-    bl = nl;
-    p1 = p2;
-    s1 = s2;
-    m1 = m2;
-    p2 = p3;
-    s2 = s3;
-    m2 = m3;
+  };
+  // ring of four records (fetched three batches ahead) and two sets of B
+  // fragments (read one batch ahead), unrolled so that nothing is copied
+  Frag B0[3], B1[3];
+  if (bb0 < bb1) load_b(r0.p, B0);
+  auto step = [&](int bt, const Rec &cur, const Rec &nxt, Rec &fill, const Frag *Bc, Frag *Bn) {
+    if (bt + 3 < bb1) load_rec(bt + 3, fill);
+    if (bt + 1 < bb1) load_b(nxt.p, Bn);  // x is read-only during this phase
+    compute(cur, Bc);
+  };
+  int bt = bb0;
+  for (; bt + 4 <= bb1; bt += 4) {
+    step(bt, r0, r1, r3, B0, B1);
+    step(bt + 1, r1, r2, r0, B1, B0);
+    step(bt + 2, r2, r3, r1, B0, B1);
+    step(bt + 3, r3, r0, r2, B1, B0);
   }
+  if (bt < bb1) step(bt, r0, r1, r3, B0, B1);
+  if (bt + 1 < bb1) step(bt + 1, r1, r2, r0, B1, B0);
+  if (bt + 2 < bb1) step(bt + 2, r2, r3, r1, B0, B1);
   __syncthreads();
   tile_finish<float, MODE, 4, TILE_THREADS, true>(
       t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, reinterpret_cast<const float *>(ph), ys, x,
