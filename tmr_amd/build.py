@@ -19,6 +19,8 @@ HIP_FLAGS = [
     # no packed-FP32 instructions in generated device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-result",
+    # matrix-core results in ordinary VGPRs (no accumulator-register copies)
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]

@@ -1171,27 +1171,28 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
         a[p][1].u = af[(2 * p + 1) * 64 + lane];
       }
     }
-    // four independent accumulation chains (two per row block: the small
-    // cross terms, then the leading terms) so the matrix core pipelines
-    f32x4_t acc[2], sm[2];
+    // one accumulation chain per row block, small terms first; the two
+    // chains interleave on the matrix core
+    f32x4_t acc[2];
     const f32x4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, B[0].v, zero, 0, 0, 0);
-      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[0].v, zero, 0, 0, 0);
-    }
+    for (int blk = 0; blk < 2; blk++)
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][blk].v, B[0].v, zero, 0, 0, 0);
 #pragma unroll
-    for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[1].v, sm[blk], 0, 0, 0);
+    for (int blk = 0; blk < 2; blk++)
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[1].v, acc[blk], 0, 0, 0);
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[2].v, acc[blk], 0, 0, 0);
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+      acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][blk].v, B[0].v, acc[blk], 0, 0, 0);
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
       acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[1].v, acc[blk], 0, 0, 0);
-    }
 #pragma unroll
-    for (int blk = 0; blk < 2; blk++) {
-      sm[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[2].v, sm[blk], 0, 0, 0);
+    for (int blk = 0; blk < 2; blk++)
       acc[blk] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][blk].v, B[0].v, acc[blk], 0, 0, 0);
-    }
-#pragma unroll
-    for (int blk = 0; blk < 2; blk++) acc[blk] += sm[blk];
     const int mask = cur.m & 0xff;
     // rows of block b are the components of corner 2g + b
     if (mask & (1 << (2 * g))) {
