@@ -305,11 +305,13 @@ def main():
         "true_residual": mixed["true_res"],
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
-        "tile_nodes": tune[0] if tune else 512,
+        "tile_nodes": prob.tuning()[0],
+        "fp32_operator": "vector ALUs" if (args.fp32_valu or not prob.tuning()[1]) else "matrix cores",
         "hip_graphs": not args.no_graph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": ("k_tile_apply<float, ASM_CHEB>" if args.fp32_valu else
+                     "kernel": ("k_tile_apply<float, ASM_CHEB>"
+                                if (args.fp32_valu or not prob.tuning()[1]) else
                                 "k_tile_apply_mfma<ASM_CHEB>") + " (finest-level Chebyshev step)",
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt,

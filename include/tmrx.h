@@ -206,11 +206,16 @@ int tmrx_problem_solve_multi(tmrx_problem *p, int nrhs, const double *f, double 
    the last solve: iterations, converged flag and the true relative residual
    |f - K u| / |f| over its dofs. */
 int tmrx_problem_num_components(tmrx_problem *p, int *ncomp);
-/* Owner-tile autotuner (no counterpart in the reference): rebuilds the
-   operator tiles for each candidate size (independent nodes per tile,
-   64..4096), times the finest-level operator reps times (FP32 matrix-core
-   product x4 + FP64 product, one V-cycle's and one PCG iteration's share)
-   and keeps the fastest.  ms[n] receives the time per candidate (may be
+/* The tuned choices: finest-level tile size and whether the FP32 operator
+   runs on the matrix cores (1) or the vector ALUs (0). */
+int tmrx_problem_tuning(tmrx_problem *p, int *tile_nodes, int *fp32_matrix_cores);
+/* Operator autotuner (no counterpart in the reference): rebuilds the owner
+   tiles for each candidate size (independent nodes per finest-level tile,
+   64..4096; coarse levels shrink theirs to keep >= 2048 tiles), times the
+   finest-level FP32 operator on the matrix cores and on the vector ALUs and
+   the FP64 operator, and keeps the size and FP32 form with the least
+   4 x FP32 + 1 x FP64 time (one V-cycle's and one PCG iteration's share).
+   ms[n] receives that time per candidate (-1: does not fit in LDS; may be
    NULL).  Needs a design; invalidates the current solution. */
 int tmrx_problem_autotune(tmrx_problem *p, const int *tile_sizes, int n, int reps, int *chosen,
                           double *ms);

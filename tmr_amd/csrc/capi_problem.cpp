@@ -200,6 +200,13 @@ int tmrx_problem_autotune(tmrx_problem *h, const int *tile_sizes, int n, int rep
   return h->p->autotune(tile_sizes, n, reps, chosen, ms) ? fail(h) : 0;
 }
 
+int tmrx_problem_tuning(tmrx_problem *h, int *tile_nodes, int *fp32_matrix_cores) {
+  CHECK_P(h);
+  if (tile_nodes) *tile_nodes = h->p->tile_nodes();
+  if (fp32_matrix_cores) *fp32_matrix_cores = h->p->fp32_matrix_cores() ? 1 : 0;
+  return 0;
+}
+
 int tmrx_problem_num_components(tmrx_problem *h, int *ncomp) {
   CHECK_P(h);
   if (ncomp) *ncomp = h->p->num_segments();

@@ -229,9 +229,11 @@ void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const in
 void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
                         const double *q, float *r32, const double *sc, const int *act,
                         double *partials, hipStream_t st);
-void launch_b_dot2(const SegPlan &pl, const double *z, const double *r, const double *q,
+template <class TZ>
+void launch_b_dot2(const SegPlan &pl, const TZ *z, const double *r, const double *q,
                    const int *act, double *partials, hipStream_t st);
-void launch_b_update_p(const SegPlan &pl, double *p, const double *z, const double *sc,
+template <class TZ>
+void launch_b_update_p(const SegPlan &pl, double *p, const TZ *z, const double *sc,
                        const int *act, hipStream_t st);
 void launch_b_copy(const SegPlan &pl, const double *a, double *b, const int *act,
                    hipStream_t st);

@@ -74,8 +74,9 @@ __global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restri
   if (threadIdx.x == 0) partials[(int64_t)rhs * pl.nblk + blk] = t;
 }
 
-// partials z.r and z.q
-__global__ __launch_bounds__(BB) void k_b_dot2(SegPlan pl, const double *__restrict__ z,
+// partials z.r and z.q (z in FP64, or straight from the FP32 V-cycle)
+template <class TZ>
+__global__ __launch_bounds__(BB) void k_b_dot2(SegPlan pl, const TZ *__restrict__ z,
                                                const double *__restrict__ r,
                                                const double *__restrict__ q,
                                                const int *__restrict__ act,
@@ -87,8 +88,9 @@ __global__ __launch_bounds__(BB) void k_b_dot2(SegPlan pl, const double *__restr
   double a = 0.0, c = 0.0;
   if (act[sidx])
     for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB) {
-      a += z[off + i] * r[off + i];
-      c += z[off + i] * q[off + i];
+      const double zv = (double)z[off + i];
+      a += zv * r[off + i];
+      c += zv * q[off + i];
     }
   const double sa = bsum(a, red);
   const double sb = bsum(c, red);
@@ -99,8 +101,9 @@ __global__ __launch_bounds__(BB) void k_b_dot2(SegPlan pl, const double *__restr
 }
 
 // p = z + beta p
+template <class TZ>
 __global__ __launch_bounds__(BB) void k_b_update_p(SegPlan pl, double *__restrict__ p,
-                                                   const double *__restrict__ z,
+                                                   const TZ *__restrict__ z,
                                                    const double *__restrict__ sc,
                                                    const int *__restrict__ act) {
   const int blk = blockIdx.x, rhs = blockIdx.y;
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(BB) void k_b_update_p(SegPlan pl, double *__restric
   const double beta = sc[(int64_t)sidx * SC_N + SC_BETA];
   const int64_t off = (int64_t)rhs * pl.n;
   for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB)
-    p[off + i] = z[off + i] + beta * p[off + i];
+    p[off + i] = (double)z[off + i] + beta * p[off + i];
 }
 
 // copy a into b for running segments (b = a everywhere when act is null)
@@ -230,17 +233,28 @@ void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p
                      sc, act, partials);
   TMRX_LAUNCH_CHECK();
 }
-void launch_b_dot2(const SegPlan &pl, const double *z, const double *r, const double *q,
+template <class TZ>
+void launch_b_dot2(const SegPlan &pl, const TZ *z, const double *r, const double *q,
                    const int *act, double *partials, hipStream_t st) {
-  hipLaunchKernelGGL(k_b_dot2, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, z, r, q, act,
+  hipLaunchKernelGGL(k_b_dot2<TZ>, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, z, r, q, act,
                      partials);
   TMRX_LAUNCH_CHECK();
 }
-void launch_b_update_p(const SegPlan &pl, double *p, const double *z, const double *sc,
+template <class TZ>
+void launch_b_update_p(const SegPlan &pl, double *p, const TZ *z, const double *sc,
                        const int *act, hipStream_t st) {
-  hipLaunchKernelGGL(k_b_update_p, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, p, z, sc, act);
+  hipLaunchKernelGGL(k_b_update_p<TZ>, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, p, z, sc,
+                     act);
   TMRX_LAUNCH_CHECK();
 }
+template void launch_b_dot2<float>(const SegPlan &, const float *, const double *, const double *,
+                                   const int *, double *, hipStream_t);
+template void launch_b_dot2<double>(const SegPlan &, const double *, const double *,
+                                    const double *, const int *, double *, hipStream_t);
+template void launch_b_update_p<float>(const SegPlan &, double *, const float *, const double *,
+                                       const int *, hipStream_t);
+template void launch_b_update_p<double>(const SegPlan &, double *, const double *,
+                                        const double *, const int *, hipStream_t);
 void launch_b_copy(const SegPlan &pl, const double *a, double *b, const int *act,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_b_copy, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, a, b, act);

@@ -24,6 +24,15 @@ namespace tmrx {
     }                                                                   \
   } while (0)
 
+// Tile size of a coarse level: the finest level's size, reduced (down to 64
+// nodes) until the level has at least 2048 tiles, so that small levels still
+// spread over the chip and each tile's serial latency stays short.
+static int coarse_tile_nodes(int fine_tile, int n_nodes) {
+  int t = fine_tile;
+  while (t > 64 && (int64_t)n_nodes < 2048LL * t) t /= 2;
+  return t;
+}
+
 void Problem::drop_graphs() {
   for (int k = 0; k < 2; k++)
     if (gexec_[k]) {
@@ -210,6 +219,7 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     L.cls.resize(octs.size());
     for (size_t e = 0; e < octs.size(); e++) L.cls[e] = block_cls[octs[e].block];
     if (build_level(l, nbcs, names, dof_masks)) return -1;
+    if (l > 0) L.tile_nodes = coarse_tile_nodes(levels_[0].tile_nodes, L.n_nodes);
     if (build_tiles(l)) return -1;
   }
   for (int l = 0; l + 1 < nlevels; l++)
@@ -817,7 +827,7 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
   Tiles tl = L.tiles;
-  tl.use_mfma = tl.use_mfma && !fp32_valu_;
+  tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_;
   if (ev) (void)hipEventRecord(ev->first, st_);
   launch_tile_apply<T>(tl, K0<T>(), scale<T>(l), x, mode, g, st_, nrhs, L.ndof);
   if (ev) (void)hipEventRecord(ev->second, st_);
@@ -1202,11 +1212,22 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
-  precond(nrhs, o);
-  launch_b_dot2(pl, z_.p, r_.p, q_.p, act_.p, parts_.p, st_);
-  launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
-                    atol2, st_);
-  launch_b_update_p(pl, p_.p, z_.p, sc_.p, act_.p, st_);
+  // z = M r; in mixed precision the FP64 updates read the FP32 result
+  // directly (no FP64 copy of z)
+  if (mixed) {
+    Level &L0 = levels_[0];
+    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs);
+    launch_b_dot2<float>(pl, L0.v32.x.p, r_.p, q_.p, act_.p, parts_.p, st_);
+    launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
+                      atol2, st_);
+    launch_b_update_p<float>(pl, p_.p, L0.v32.x.p, sc_.p, act_.p, st_);
+  } else {
+    precond(nrhs, o);
+    launch_b_dot2<double>(pl, z_.p, r_.p, q_.p, act_.p, parts_.p, st_);
+    launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
+                      atol2, st_);
+    launch_b_update_p<double>(pl, p_.p, z_.p, sc_.p, act_.p, st_);
+  }
   launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
   (void)hipMemcpyAsync(nact_host_ + slot, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_);
 }
@@ -1613,11 +1634,12 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   int best = -1;
+  bool best_mfma = true;
   double best_ms = 0.0;
   for (int c = 0; c < ncand; c++) {
     bool fits = true;
     for (size_t l = 0; l < levels_.size(); l++) {
-      levels_[l].tile_nodes = cands[c];
+      levels_[l].tile_nodes = l == 0 ? cands[c] : coarse_tile_nodes(cands[c], levels_[l].n_nodes);
       if (build_tiles((int)l)) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1635,22 +1657,39 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
     AsmArgs a32, a64;
     a32.y = L0.v32.r.p;
     a64.y = L0.v64.r.p;
-    apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);  // warm
-    apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
-    (void)hipEventRecord(e0, st_);
-    for (int r = 0; r < reps; r++) {
-      for (int k = 0; k < 4; k++) apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);
-      apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
+    // ms per launch of one operator form
+    auto time_op = [&](int form) -> double {  // 0 FP32 matrix cores, 1 FP32 VALU, 2 FP64
+      fp32_valu_ = form == 1;
+      auto once = [&]() {
+        if (form == 2)
+          apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
+        else
+          apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);
+      };
+      once();  // warm
+      (void)hipEventRecord(e0, st_);
+      for (int r = 0; r < reps; r++) once();
+      (void)hipEventRecord(e1, st_);
+      if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+      return ms / reps;
+    };
+    const double tm = time_op(0), tv = time_op(1), t64 = time_op(2);
+    if (tm < 0 || tv < 0 || t64 < 0) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      HIPCHK(hipGetLastError());
+      last_error = "autotune: timing failed";
+      return -1;
     }
-    (void)hipEventRecord(e1, st_);
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    const double per = ms / reps;
+    const bool mfma = tm <= tv;
+    const double per = 4.0 * (mfma ? tm : tv) + t64;
     if (ms_out) ms_out[c] = per;
     if (best < 0 || per < best_ms) {
       best = c;
       best_ms = per;
+      best_mfma = mfma;
     }
   }
   (void)hipEventDestroy(e0);
@@ -1660,7 +1699,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
     last_error = "autotune: no candidate tile size fits in LDS";
     best = -1;
     for (size_t l = 0; l < levels_.size(); l++) {
-      levels_[l].tile_nodes = TILE_NODES;
+      levels_[l].tile_nodes = l == 0 ? TILE_NODES : coarse_tile_nodes(TILE_NODES, levels_[l].n_nodes);
       if (build_tiles((int)l)) return -1;
     }
     if (build_plan()) return -1;
@@ -1670,7 +1709,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
     return -1;
   }
   for (size_t l = 0; l < levels_.size(); l++) {
-    levels_[l].tile_nodes = cands[best];
+    levels_[l].tile_nodes = l == 0 ? cands[best] : coarse_tile_nodes(cands[best], levels_[l].n_nodes);
     if (build_tiles((int)l)) return -1;
   }
   // tiles moved: reduction plan, partial buffers and captured graphs follow
@@ -1679,6 +1718,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   batch_cap_ = 0;
   if (ensure_batch(cap)) return -1;
   gen_++;
+  fp32_mfma_ = best_mfma;
   if (chosen) *chosen = cands[best];
   return 0;
 }

@@ -165,6 +165,7 @@ class Problem {
   // owner-tile size autotuner (ms per candidate, chosen size)
   int autotune(const int *cands, int ncand, int reps, int *chosen, double *ms);
   int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
+  bool fp32_matrix_cores() const { return fp32_mfma_; }
   int set_state(const double *f, const double *u, bool on_device);
   int compliance(double *c);
   int compliance_grad(double *g, bool on_device);
@@ -258,6 +259,7 @@ class Problem {
   double last_stress_sum_ = 0.0;
   hipStream_t st_ = nullptr;
   bool fp32_valu_ = false;  // FP32 operator on the vector ALUs
+  bool fp32_mfma_ = true;   // autotuned: FP32 operator on the matrix cores
   int timing_ = 0;  // mask of the timed launch modes (bit m = AsmMode m)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_used_ = 0;

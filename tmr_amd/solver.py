@@ -149,6 +149,12 @@ class Problem:
         check(lib().tmrx_problem_autotune(self.ptr, ts, n, int(reps), C.byref(ch), ms), "autotune")
         return ch.value, {int(t): (m if m >= 0 else None) for t, m in zip(tile_sizes, ms)}
 
+    def tuning(self):
+        """(finest-level tile size, FP32 operator on the matrix cores?)."""
+        t, m = C.c_int(), C.c_int()
+        check(lib().tmrx_problem_tuning(self.ptr, C.byref(t), C.byref(m)))
+        return t.value, bool(m.value)
+
     def true_residual(self):
         v = C.c_double()
         check(lib().tmrx_problem_true_residual(self.ptr, C.byref(v)))
