@@ -203,6 +203,9 @@ def main():
                     help="cantilever: the 4.2M-hex headline; multirhs: 16 load cases on a "
                          "2.1M-hex cube; batch: 96 uneven independent cantilevers")
     ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
+                    help="after the solves, copy MIB MiB device to device (a known-size "
+                         "reference for memory-traffic counters)")
     ap.add_argument("--count", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -325,6 +328,14 @@ def main():
                        "compliance": fp64["compliance"], "setup_ms": fp64["setup_ms"]}
         out["speedup_mixed_vs_fp64"] = solves_s / out["fp64"]["solves_per_s"]
         out["compliance_rel_diff_mixed_vs_fp64"] = abs(mixed["compliance"] - fp64["compliance"]) / abs(fp64["compliance"])
+    if args.calibration_copy > 0:
+        n = args.calibration_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = 4 * n
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_level)
