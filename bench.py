@@ -28,6 +28,15 @@ CHEB_BYTES_PER_NODE = 6 * 3 * 4 + 1
 CHEB_BYTES_PER_ELEM = 4 + 8 * 4
 FLOPS_PER_ELEM = 1152.0      # 2 * 24 * 24
 TIME_CHEB = 1 << 2           # launch-kind mask: Chebyshev update
+# HBM traffic of the finest-level Chebyshev launch on the default workload
+# (level 7), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per
+# dispatch, mean over the launches at that level), keyed by (FP32 form, owner
+# tile nodes).  The counters are scaled by what they read for a 1 GiB
+# device-to-device copy in the same runs (bench.py --calibration-copy 1024):
+# FETCH_SIZE 524,316 KiB and WRITE_SIZE 1,048,576 KiB for 1,048,576 KiB moved.
+COPY_FETCH_FACTOR = 1048576.0 / 524316.0
+COPY_WRITE_FACTOR = 1048576.0 / 1048576.0
+MEASURED_TRAFFIC_KIB = {("valu", 512): (218442.8, 100237.4)}
 
 
 def log(*a):
@@ -286,6 +295,11 @@ def main():
     nnodes = ndof // 3
     bytes_launch = CHEB_BYTES_PER_NODE * nnodes + CHEB_BYTES_PER_ELEM * prob.nelem
     achieved = bytes_launch / avg_s / 1e9
+    form = "valu" if (args.fp32_valu or not prob.tuning()[1]) else "mfma"
+    traffic = None
+    pmc = MEASURED_TRAFFIC_KIB.get((form, prob.tuning()[0])) if args.level == 7 else None
+    if pmc is not None:
+        traffic = 1024.0 * (pmc[0] * COPY_FETCH_FACTOR + pmc[1] * COPY_WRITE_FACTOR)
     out = {
         "metric": "solves/s + MDOF*iter/s: mixed-prec vs FP64 MG-PCG to 1e-8, 4.2M-hex SIMP",
         "value": solves_s,
@@ -309,12 +323,11 @@ def main():
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
         "tile_nodes": prob.tuning()[0],
-        "fp32_operator": "vector ALUs" if (args.fp32_valu or not prob.tuning()[1]) else "matrix cores",
+        "fp32_operator": "vector ALUs" if form == "valu" else "matrix cores",
         "hip_graphs": not args.no_graph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": ("k_tile_apply<float, ASM_CHEB>"
-                                if (args.fp32_valu or not prob.tuning()[1]) else
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": ("k_tile_apply<float, ASM_CHEB>" if form == "valu" else
                                 "k_tile_apply_mfma<ASM_CHEB>") + " (finest-level Chebyshev step)",
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt,
