@@ -180,6 +180,25 @@ int tmrx_problem_set_stream(tmrx_problem *p, void *stream);
    TMRTopoProblem::setDesignVars (src/topology/TMRTopoProblem.cpp:974) for
    element-constant densities. */
 int tmrx_problem_set_design(tmrx_problem *p, const double *rho, int n, int on_device);
+/* Nodal design variables: x[n] with one value per independent node of the
+   finest forest (n = its node count); hanging nodes take the interpolated
+   value.  Replaces the design path of TMROctConformTACSTopoCreator
+   (src/topology/TMR_TACSTopoCreator.cpp:714-777: element i's design numbers
+   are its nodes) with TMROctConstitutive::setDesignVars
+   (src/topology/TMROctConstitutive.cpp:142-153): the density at each of the
+   8 Gauss points is the trilinear interpolation of the corner values and the
+   penalty (SIMP rho^q + k0, RAMP rho/(1+q(1-rho)) + k0) applies per point
+   (TMROctConstitutive::evalTangentStiffness, :498-551).  Coarse levels get
+   x_c = (P^T x_f) / (P^T 1), TMRConformFilter::setDesignVars
+   (src/topology/TMRConformFilter.cpp:154-167) with TACSBVecInterp's
+   multWeightTranspose.  After this call gradients are per node. */
+int tmrx_problem_set_design_nodal(tmrx_problem *p, const double *x, int n, int on_device);
+/* Design mode of the last set_design* call (0 element densities, 1 nodal)
+   and the length of a design gradient (elements or finest-level nodes). */
+int tmrx_problem_design_info(tmrx_problem *p, int *mode, int *n);
+/* Nodal design values on multigrid level `level` (0 = finest), x[nodes of
+   that level]: TMRConformFilter's per-level design vectors x[k]. */
+int tmrx_problem_level_design(tmrx_problem *p, int level, double *x);
 /* Density-dependent multigrid setup (diagonals, eigenvalue bounds, coarse
    Cholesky): mg->assembleJacobian + mg->factor
    (src/topology/TMRTopoProblem.cpp:1000-1001). */
@@ -232,7 +251,9 @@ int tmrx_problem_set_state(tmrx_problem *p, const double *f, const double *u, in
 int tmrx_problem_compliance(tmrx_problem *p, double *c);
 /* d compliance / d rho_e for every element: the compliance branch of
    TMRTopoProblem::evalObjConGradient (src/topology/TMRTopoProblem.cpp:1275-1280),
-   -psi^T dK/drho u with psi = u. */
+   -psi^T dK/drho u with psi = u.  With a nodal design, d compliance / d x_j
+   per finest-level node (TMROctConstitutive::addStressDVSens,
+   src/topology/TMROctConstitutive.cpp:557-654). */
 int tmrx_problem_compliance_grad(tmrx_problem *p, double *g, int on_device);
 /* p-norm (exponent P >= 2) of the relaxed von Mises failure of the last
    solution, (sum_e sum_g (V_e/8) fail_eg^P)^(1/P) with
@@ -245,7 +266,8 @@ int tmrx_problem_stress_pnorm(tmrx_problem *p, double P, double *value);
    (TMROctConstitutive::addFailureDVSens, :1159-1215) minus psi^T dK/drho u,
    psi from the adjoint solve K psi = df/du -- the general-functional branch
    of TMRTopoProblem::evalObjConGradient (src/topology/TMRTopoProblem.cpp:1253-1271).
-   Returns 1 if the adjoint solve did not converge. */
+   Returns 1 if the adjoint solve did not converge.  With a nodal design the
+   relaxation uses the Gauss-point density and g is per finest-level node. */
 int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_device,
                                    const tmrx_solve_opts *opts, tmrx_solve_info *info);
 /* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32 on the matrix

@@ -72,34 +72,54 @@ static void strain_at(const double h[3], const double p[3], const double *ue, do
   e[5] = G[0][1] + G[1][0];
 }
 
-/* K column by column: K[:,c] = sum_g w detJ B^T C (B e_c), using the strain
-   of unit displacement vectors rather than a stored B matrix */
-void orc_hex8_stiffness(double hx, double hy, double hz, double E, double nu, double *K) {
+/* Contribution of Gauss point q (bit d of q set: coordinate d at +1/sqrt(3))
+   to K, column by column: K_q[:,c] = w detJ B_q^T C (B_q e_c), using the
+   strain of unit displacement vectors rather than a stored B matrix */
+void orc_hex8_stiffness_gp(double hx, double hy, double hz, double E, double nu, int q,
+                           double *K) {
   double C[6][6];
   iso_C(E, nu, C);
   const double h[3] = {hx, hy, hz};
   const double gp = 1.0 / sqrt(3.0);
   const double detJ = hx * hy * hz / 8.0;
-  memset(K, 0, sizeof(double) * 576);
   double unit[24];
-  for (int q = 0; q < 8; q++) {
-    double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
-    double eps[24][6];
+  double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
+  double eps[24][6];
+  for (int c = 0; c < 24; c++) {
+    memset(unit, 0, sizeof(unit));
+    unit[c] = 1.0;
+    strain_at(h, p, unit, eps[c]);
+  }
+  for (int r = 0; r < 24; r++)
     for (int c = 0; c < 24; c++) {
-      memset(unit, 0, sizeof(unit));
-      unit[c] = 1.0;
-      strain_at(h, p, unit, eps[c]);
-    }
-    for (int r = 0; r < 24; r++)
-      for (int c = 0; c < 24; c++) {
-        double s = 0.0;
-        for (int i = 0; i < 6; i++) {
-          double sig = 0.0;
-          for (int j = 0; j < 6; j++) sig += C[i][j] * eps[c][j];
-          s += eps[r][i] * sig;
-        }
-        K[24 * r + c] += detJ * s;
+      double s = 0.0;
+      for (int i = 0; i < 6; i++) {
+        double sig = 0.0;
+        for (int j = 0; j < 6; j++) sig += C[i][j] * eps[c][j];
+        s += eps[r][i] * sig;
       }
+      K[24 * r + c] = detJ * s;
+    }
+}
+
+/* K = sum over the 8 Gauss points */
+void orc_hex8_stiffness(double hx, double hy, double hz, double E, double nu, double *K) {
+  double Kq[576];
+  memset(K, 0, sizeof(double) * 576);
+  for (int q = 0; q < 8; q++) {
+    orc_hex8_stiffness_gp(hx, hy, hz, E, nu, q, Kq);
+    for (int i = 0; i < 576; i++) K[i] += Kq[i];
+  }
+}
+
+/* element stiffness of a nodal design: sum_q s[q] K_q */
+static void hex8_stiffness_scaled(const double h[3], double E, double nu, const double *sq,
+                                  double *K) {
+  double Kq[576];
+  memset(K, 0, sizeof(double) * 576);
+  for (int q = 0; q < 8; q++) {
+    orc_hex8_stiffness_gp(h[0], h[1], h[2], E, nu, q, Kq);
+    for (int i = 0; i < 576; i++) K[i] += sq[q] * Kq[i];
   }
 }
 
@@ -154,9 +174,28 @@ void orc_free(void) {
     bcmask[nnodes] (bit d = dof d clamped), E, nu.
   Returns the number of stored entries.
 */
+static long assemble_impl(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                          const int *dep_conn, const double *dep_w, const double *hdim,
+                          const double *s, int per_gp, const uint8_t *bcmask, double E, double nu);
+
 long orc_assemble(int nnodes, int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                   const double *dep_w, const double *hdim, const double *s,
                   const uint8_t *bcmask, double E, double nu) {
+  return assemble_impl(nnodes, nelem, conn, dep_ptr, dep_conn, dep_w, hdim, s, 0, bcmask, E, nu);
+}
+
+/* Nodal design (TMROctConstitutive with per-Gauss-point penalty,
+   src/topology/TMROctConstitutive.cpp:498-551): K_e = sum_q sgp[8 e + q] K_q */
+long orc_assemble_gp(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                     const int *dep_conn, const double *dep_w, const double *hdim,
+                     const double *sgp, const uint8_t *bcmask, double E, double nu) {
+  return assemble_impl(nnodes, nelem, conn, dep_ptr, dep_conn, dep_w, hdim, sgp, 1, bcmask, E,
+                       nu);
+}
+
+static long assemble_impl(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                          const int *dep_conn, const double *dep_w, const double *hdim,
+                          const double *s, int per_gp, const uint8_t *bcmask, double E, double nu) {
   orc_free();
   long cap = 0;
   for (int e = 0; e < nelem; e++) {
@@ -172,7 +211,13 @@ long orc_assemble(int nnodes, int nelem, const int *conn, const int *dep_ptr, co
   long nt = 0;
   double K[576];
   for (int e = 0; e < nelem; e++) {
-    orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, K);
+    double se = 1.0;
+    if (per_gp) {
+      hex8_stiffness_scaled(&hdim[3 * e], E, nu, &s[8 * (size_t)e], K);
+    } else {
+      orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, K);
+      se = s[e];
+    }
     int idx[8][4];
     double wt[8][4];
     int cnt[8];
@@ -196,7 +241,7 @@ long orc_assemble(int nnodes, int nelem, const int *conn, const int *dep_ptr, co
         for (int b = 0; b < 8; b++)
           for (int kb = 0; kb < cnt[b]; kb++) {
             int p = idx[a][ka], q = idx[b][kb];
-            double w = s[e] * wt[a][ka] * wt[b][kb];
+            double w = se * wt[a][ka] * wt[b][kb];
             for (int i = 0; i < 3; i++) {
               if ((bcmask[p] >> i) & 1) continue;
               for (int j = 0; j < 3; j++) {
@@ -380,11 +425,36 @@ static void gather_ue(const int *conn, int e, const int *dep_ptr, const int *dep
   }
 }
 
+static double stress_impl(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                          const int *dep_conn, const double *dep_w, const double *hdim,
+                          const uint8_t *bcmask, double E, double nu, double ys, double eps,
+                          double P, const double *rho, int per_gp, const double *u, double *dfdu,
+                          double *dfdrho);
+
 double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_ptr,
                         const int *dep_conn, const double *dep_w, const double *hdim,
                         const uint8_t *bcmask, double E, double nu, double ys, double eps,
                         double P, const double *rho, const double *u, double *dfdu,
                         double *dfdrho) {
+  return stress_impl(nnodes, nelem, conn, dep_ptr, dep_conn, dep_w, hdim, bcmask, E, nu, ys, eps,
+                     P, rho, 0, u, dfdu, dfdrho);
+}
+
+/* nodal design: rho[8 e + q] at the Gauss points, dfdrho per Gauss point */
+double orc_stress_pnorm_gp(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                           const int *dep_conn, const double *dep_w, const double *hdim,
+                           const uint8_t *bcmask, double E, double nu, double ys, double eps,
+                           double P, const double *rho_gp, const double *u, double *dfdu,
+                           double *dfdrho_gp) {
+  return stress_impl(nnodes, nelem, conn, dep_ptr, dep_conn, dep_w, hdim, bcmask, E, nu, ys, eps,
+                     P, rho_gp, 1, u, dfdu, dfdrho_gp);
+}
+
+static double stress_impl(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                          const int *dep_conn, const double *dep_w, const double *hdim,
+                          const uint8_t *bcmask, double E, double nu, double ys, double eps,
+                          double P, const double *rho, int per_gp, const double *u, double *dfdu,
+                          double *dfdrho) {
   double C[6][6];
   iso_C(E, nu, C);
   const double gp = 1.0 / sqrt(3.0);
@@ -394,8 +464,9 @@ double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_p
     double ue[24], h[3] = {hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2]};
     gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
     double V = h[0] * h[1] * h[2];
-    double r = rho[e] / (eps * (1.0 - rho[e]) + rho[e]);
     for (int q = 0; q < 8; q++) {
+      double rq = per_gp ? rho[8 * (size_t)e + q] : rho[e];
+      double r = rq / (eps * (1.0 - rq) + rq);
       double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
       double ep[6], s[6];
       strain_at(h, p, ue, ep);
@@ -417,10 +488,11 @@ double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_p
     double ue[24], h[3] = {hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2]};
     gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
     double V = h[0] * h[1] * h[2];
-    double den = eps * (1.0 - rho[e]) + rho[e];
-    double r = rho[e] / den, dr = eps / (den * den);
     double ge[24] = {0}, sumvmP = 0.0;
     for (int q = 0; q < 8; q++) {
+      double rq = per_gp ? rho[8 * (size_t)e + q] : rho[e];
+      double den = eps * (1.0 - rq) + rq;
+      double r = rq / den, dr = eps / (den * den);
       double p[3] = {(q & 1) ? gp : -gp, (q & 2) ? gp : -gp, (q & 4) ? gp : -gp};
       double ep[6], s[6];
       strain_at(h, p, ue, ep);
@@ -433,6 +505,7 @@ double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_p
                  3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
       double vm = sqrt(J) / ys;
       sumvmP += pow(vm, P);
+      if (per_gp && dfdrho) dfdrho[8 * (size_t)e + q] = coef * 0.125 * V * pow(r, P - 1.0) * dr * pow(vm, P);
       /* d fail^P / d s = P fail^(P-1) r dvm/ds, dvm/ds = dJ/ds / (2 ys^2 vm) */
       double dJ[6] = {2 * s[0] - s[1] - s[2], 2 * s[1] - s[0] - s[2], 2 * s[2] - s[0] - s[1],
                       6 * s[3], 6 * s[4], 6 * s[5]};
@@ -453,7 +526,11 @@ double orc_stress_pnorm(int nnodes, int nelem, const int *conn, const int *dep_p
         ge[c] += w * t;
       }
     }
-    if (dfdrho) dfdrho[e] = coef * 0.125 * V * pow(r, P - 1.0) * dr * sumvmP;
+    if (!per_gp && dfdrho) {
+      double den = eps * (1.0 - rho[e]) + rho[e];
+      double r = rho[e] / den, dr = eps / (den * den);
+      dfdrho[e] = coef * 0.125 * V * pow(r, P - 1.0) * dr * sumvmP;
+    }
     if (dfdu) {
       for (int a = 0; a < 8; a++) {
         int g = conn[8 * e + a];
@@ -487,5 +564,24 @@ void orc_elem_energy2(int nelem, const int *conn, const int *dep_ptr, const int 
     for (int i = 0; i < 24; i++)
       for (int j = 0; j < 24; j++) s += ve[i] * K[24 * i + j] * ue[j];
     out[e] = s;
+  }
+}
+
+/* v_e^T K_q u_e at every Gauss point q of every element (unpenalised),
+   out[8 e + q] */
+void orc_elem_energy_gp(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                        const double *dep_w, const double *hdim, const uint8_t *bcmask, double E,
+                        double nu, const double *u, const double *v, double *out) {
+  double K[576], ue[24], ve[24];
+  for (int e = 0; e < nelem; e++) {
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, u, ue);
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, v, ve);
+    for (int q = 0; q < 8; q++) {
+      orc_hex8_stiffness_gp(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, q, K);
+      double t = 0.0;
+      for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) t += ve[i] * K[24 * i + j] * ue[j];
+      out[8 * (size_t)e + q] = t;
+    }
   }
 }

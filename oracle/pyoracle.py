@@ -30,6 +30,12 @@ def lib():
         L.orc_solve.argtypes = [P, P, D, I, P]
         L.orc_elem_energy.argtypes = [I, P, P, P, P, P, P, D, D, P, P]
         L.orc_elem_energy2.argtypes = [I, P, P, P, P, P, P, D, D, P, P, P]
+        L.orc_hex8_stiffness_gp.argtypes = [D, D, D, D, D, I, P]
+        L.orc_assemble_gp.restype = C.c_long
+        L.orc_assemble_gp.argtypes = [I, I, P, P, P, P, P, P, P, D, D]
+        L.orc_elem_energy_gp.argtypes = [I, P, P, P, P, P, P, D, D, P, P, P]
+        L.orc_stress_pnorm_gp.restype = D
+        L.orc_stress_pnorm_gp.argtypes = [I, I, P, P, P, P, P, P, D, D, D, D, D, P, P, P, P]
         _L = L
     return _L
 
@@ -41,6 +47,13 @@ def _p(a):
 def hex8_stiffness(hx, hy, hz, E=1.0, nu=0.3):
     K = np.zeros((24, 24))
     lib().orc_hex8_stiffness(hx, hy, hz, E, nu, _p(K))
+    return K
+
+
+def hex8_stiffness_gp(hx, hy, hz, q, E=1.0, nu=0.3):
+    """Contribution of Gauss point q (bit d set: coordinate d at +1/sqrt(3))."""
+    K = np.zeros((24, 24))
+    lib().orc_hex8_stiffness_gp(hx, hy, hz, E, nu, int(q), _p(K))
     return K
 
 
@@ -115,6 +128,61 @@ class Oracle:
                                _p(u), _p(v), _p(e))
         return e
 
+    def assemble_gp(self, sgp):
+        """Nodal design: K_e = sum_q sgp[e, q] K_q (src/topology/TMROctConstitutive.cpp:498-551)."""
+        sgp = np.ascontiguousarray(sgp, dtype=np.double).reshape(-1)
+        if sgp.size != 8 * self.nelem:
+            raise ValueError("need 8 scales per element")
+        nnz = lib().orc_assemble_gp(self.nnodes, self.nelem, _p(self.conn), _p(self.dep_ptr),
+                                    _p(self.dep_conn), _p(self.dep_w), _p(self.hdim), _p(sgp),
+                                    _p(self.bcmask), self.E, self.nu)
+        self.nnz = nnz
+        return nnz
+
+    def energy_gp(self, u, v):
+        """v_e^T K_q u_e per element and Gauss point, shape (nelem, 8)."""
+        u = np.ascontiguousarray(u, dtype=np.double)
+        v = np.ascontiguousarray(v, dtype=np.double)
+        e = np.zeros(8 * self.nelem)
+        lib().orc_elem_energy_gp(self.nelem, _p(self.conn), _p(self.dep_ptr), _p(self.dep_conn),
+                                 _p(self.dep_w), _p(self.hdim), _p(self.bcmask), self.E, self.nu,
+                                 _p(u), _p(v), _p(e))
+        return e.reshape(-1, 8)
+
+    def corner_values(self, x):
+        """Nodal design values at every element corner (hanging corners
+        interpolated from their parents), shape (nelem, 8)."""
+        x = np.asarray(x, dtype=np.double)
+        c = self.conn.reshape(-1, 8)
+        out = np.zeros(c.shape)
+        ind = c >= 0
+        out[ind] = x[c[ind]]
+        for e, a in zip(*np.nonzero(~ind)):
+            d = -c[e, a] - 1
+            lo, hi = self.dep_ptr[d], self.dep_ptr[d + 1]
+            out[e, a] = float(np.dot(self.dep_w[lo:hi], x[self.dep_conn[lo:hi]]))
+        return out
+
+    def gp_density(self, x):
+        """rho at the 8 Gauss points: trilinear interpolation of the corner
+        values (TMROctConstitutive::evalDesignFieldValue,
+        src/topology/TMROctConstitutive.cpp:1470-1499), shape (nelem, 8)."""
+        return self.corner_values(x) @ gauss_shape().T
+
+    def gather_corners(self, t):
+        """sum over elements of per-corner values t (nelem, 8) onto the
+        independent nodes (hanging corners to their parents with the
+        hanging weights)."""
+        c = self.conn.reshape(-1, 8)
+        g = np.zeros(self.nnodes)
+        ind = c >= 0
+        np.add.at(g, c[ind], t[ind])
+        for e, a in zip(*np.nonzero(~ind)):
+            d = -c[e, a] - 1
+            lo, hi = self.dep_ptr[d], self.dep_ptr[d + 1]
+            np.add.at(g, self.dep_conn[lo:hi], self.dep_w[lo:hi] * t[e, a])
+        return g
+
     def energy(self, u):
         u = np.ascontiguousarray(u, dtype=np.double)
         e = np.zeros(self.nelem)
@@ -122,6 +190,51 @@ class Oracle:
                               _p(self.dep_w), _p(self.hdim), _p(self.bcmask), self.E, self.nu,
                               _p(u), _p(e))
         return e
+
+
+def gauss_shape():
+    """N[q, a]: trilinear shape function of corner a (x fastest) at Gauss
+    point q (bit d of q set: coordinate d at +1/sqrt(3))."""
+    gp = 1.0 / np.sqrt(3.0)
+    N = np.zeros((8, 8))
+    for q in range(8):
+        pt = [gp if (q >> d) & 1 else -gp for d in range(3)]
+        for a in range(8):
+            v = 1.0
+            for d in range(3):
+                s = 1.0 if (a >> d) & 1 else -1.0
+                v *= 0.5 * (1.0 + s * pt[d])
+            N[q, a] = v
+    return N
+
+
+def weight_transpose(rowp, cols, w, xf, ncoarse):
+    """TACSBVecInterp::multWeightTranspose as TMRConformFilter::setDesignVars
+    uses it (src/topology/TMRConformFilter.cpp:154-167): with P the
+    prolongation (rows fine nodes, columns coarse nodes),
+    x_c = (P^T x_f) / (P^T 1)."""
+    rowp = np.asarray(rowp)
+    rows = np.repeat(np.arange(len(rowp) - 1), np.diff(rowp))
+    num = np.zeros(ncoarse)
+    den = np.zeros(ncoarse)
+    np.add.at(num, cols, w * np.asarray(xf)[rows])
+    np.add.at(den, cols, w)
+    return num / den
+
+
+def stress_pnorm_gp(orc, rho_gp, u, P=8.0, ys=1.0, eps=0.3, grad=True):
+    """Nodal-design p-norm: the relaxation uses the Gauss-point densities;
+    returns (f, dfdu, dfdrho_gp (nelem, 8))."""
+    L = _stress_sigs()
+    rho_gp = np.ascontiguousarray(rho_gp, dtype=np.double).reshape(-1)
+    u = np.ascontiguousarray(u, dtype=np.double)
+    dfdu = np.zeros(3 * orc.nnodes)
+    dr = np.zeros(8 * orc.nelem)
+    f = L.orc_stress_pnorm_gp(orc.nnodes, orc.nelem, _p(orc.conn), _p(orc.dep_ptr),
+                              _p(orc.dep_conn), _p(orc.dep_w), _p(orc.hdim), _p(orc.bcmask),
+                              orc.E, orc.nu, ys, eps, P, _p(rho_gp), _p(u),
+                              _p(dfdu) if grad else None, _p(dr) if grad else None)
+    return f, dfdu, dr.reshape(-1, 8)
 
 
 def bcmask_from_forest(forest, bcs):
@@ -282,3 +395,51 @@ def stress_gradient(orc, rho, b, q=3.0, k0=1e-6, P=8.0, ys=1.0, eps=0.3, rtol=1e
     psi = orc.solve(dfdu, rtol=rtol)
     prod = orc.energy2(u, psi)
     return f, dfdr - np.array([dpenalty(r, q) for r in rho]) * prod, u, psi
+
+
+# ---------------------------------------------------------------------------
+# nodal design variables (TMROctConformTACSTopoCreator + TMROctConstitutive)
+def _pen(r, q, k0, ptype):
+    r = np.asarray(r, dtype=np.double)
+    return r ** q + k0 if ptype == 0 else r / (1.0 + q * (1.0 - r)) + k0
+
+
+def _dpen(r, q, ptype):
+    r = np.asarray(r, dtype=np.double)
+    if ptype == 0:
+        return q * r ** (q - 1.0) if q > 1.0 else np.ones_like(r)
+    t = 1.0 + q * (1.0 - r)
+    return (q + 1.0) / (t * t)
+
+
+def nodal_assemble(orc, x, q=3.0, k0=1e-6, ptype=0):
+    """Assemble K(x) of a nodal design: per Gauss point penalty(rho_q) K_q
+    (TMROctConstitutive::evalTangentStiffness, src/topology/TMROctConstitutive.cpp:498-551)."""
+    rg = orc.gp_density(x)
+    orc.assemble_gp(_pen(rg, q, k0, ptype))
+    return rg
+
+
+def nodal_compliance_gradient(orc, x, b, q=3.0, k0=1e-6, ptype=0, rtol=1e-13):
+    """c = b^T u and d c / d x_j = -sum_e sum_q N_a(q) p'(rho_q) u_e^T K_q u_e
+    over the corners a of the elements at node j (hanging corners to their
+    parents): TMROctConstitutive::addStressDVSens with psi = u
+    (src/topology/TMROctConstitutive.cpp:557-654)."""
+    rg = nodal_assemble(orc, x, q, k0, ptype)
+    u = orc.solve(b, rtol=rtol)
+    e = orc.energy_gp(u, u)
+    t = (-_dpen(rg, q, ptype) * e) @ gauss_shape()
+    return float(b @ u), orc.gather_corners(t), u
+
+
+def nodal_stress_gradient(orc, x, b, q=3.0, k0=1e-6, ptype=0, P=8.0, ys=1.0, eps=0.3,
+                          rtol=1e-13):
+    """p-norm value and its gradient per node for a nodal design (relaxation
+    at the Gauss-point densities; explicit term plus -psi^T dK/dx u)."""
+    rg = nodal_assemble(orc, x, q, k0, ptype)
+    u = orc.solve(b, rtol=rtol)
+    f, dfdu, dfdr = stress_pnorm_gp(orc, rg, u, P, ys, eps, grad=True)
+    psi = orc.solve(dfdu, rtol=rtol)
+    e = orc.energy_gp(u, psi)
+    t = (dfdr - _dpen(rg, q, ptype) * e) @ gauss_shape()
+    return f, orc.gather_corners(t), u, psi

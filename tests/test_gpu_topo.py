@@ -115,3 +115,64 @@ def test_traction_load_matches_native():
     mask = po.bcmask_from_forest(a.forest, [("fixed", 7)])
     fn.reshape(-1, 3)[mask == 7] = 0.0
     assert np.allclose(fn, forces[0], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("penalty", ["SIMP", "RAMP"])
+def test_conform_filter_nodal_design(penalty):
+    """createTopoProblem(..., "conform") with OctConformTopoCreator levels:
+    nodal design variables on the analysis forests, per-Gauss-point penalty,
+    weighted compliance and mass with gradients per node."""
+    forest = TMR.createBoxForest(2, 1, 1, L=(2.0, 1.0, 1.0), face_names=FACES)
+    forest.createTrees(2)
+    rng = np.random.default_rng(12)
+    forest.refine(rng.integers(0, 2, len(forest.getOctants())).astype(np.intc), 0, 4)
+    mat = TMR.MaterialProperties(rho=2.5, E=1.0, nu=0.3, ys=1.0)
+    props = TMR.StiffnessProperties(mat, q=Q, k0=K0, eps=0.3, penalty_type=penalty)
+    ptype = props.penalty_type
+    bcs = TMR.BoundaryConditions()
+    bcs.addBoundaryCondition("fixed")
+
+    def callback(f):
+        c = topo.OctConformTopoCreator(bcs, f, props=props)
+        return c, c.getFilter()
+
+    prob = TopOptUtils.createTopoProblem(forest, callback, "conform", nlevels=3)
+    a = prob.getAssembler()
+    assert a.nodal and a.ndv == a.forest.getNodeRange()[1]
+    f1 = TopOptUtils.compute3DTractionLoad("load", a.forest, a, [0.0, 0.0, -1.0])
+    f2 = TopOptUtils.compute3DTractionLoad("top", a.forest, a, [0.0, 0.3, -0.5])
+    prob.setLoadCases([f1, f2])
+    w = [1.0, 0.5]
+    prob.setObjective(w)
+    vol = a.volumes.sum()
+    prob.addConstraints(0, [topo.StructuralMass()], [-0.4 * vol * 2.5], [-1.0])
+    prob.rtol = 1e-11
+    x0, lb, ub = prob.getVarsAndBounds()
+    x = np.clip(0.3 + 0.7 * rng.random(len(x0)), lb, ub)
+    print("conform", penalty, "nelem", a.nelem, "ndv", a.ndv, flush=True)
+    fail, fobj, cons = prob.evalObjCon(1, x)
+    assert fail == 0
+    orc = po.from_forest(a.forest, [("fixed", 7)])
+    refs = [po.nodal_compliance_gradient(orc, x, f, Q, K0, ptype) for f in (f1, f2)]
+    fref = sum(wi * r[0] for wi, r in zip(w, refs))
+    print("fobj %.12g ref %.12g" % (fobj, fref), prob.last_info, flush=True)
+    assert abs(fobj - fref) / abs(fref) < 1e-6
+    # mass: exact integral of the trilinear density
+    rg = orc.gp_density(x)
+    mass = 2.5 * float(np.sum(a.volumes[:, None] / 8.0 * rg))
+    assert np.isclose(cons[0], -(mass - 0.4 * vol * 2.5), rtol=1e-12)
+    g = np.zeros(a.ndv)
+    A = [np.zeros(a.ndv)]
+    assert prob.evalObjConGradient(x, g, A) == 0
+    gref = sum(wi * r[1] for wi, r in zip(w, refs))
+    err = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print("grad rel err %.3e" % err, flush=True)
+    assert err < 1e-6
+    assert np.allclose(A[0], -2.5 * a.mass_weights(), rtol=1e-12)
+    d = rng.standard_normal(len(x))
+    h = 1e-4
+    fp = prob.evalObjCon(1, x + h * d)[1]
+    fm = prob.evalObjCon(1, x - h * d)[1]
+    fd = (fp - fm) / (2 * h)
+    print("fd %.10g g.d %.10g" % (fd, g @ d), flush=True)
+    assert abs(fd - g @ d) / abs(fd) < 1e-5

@@ -88,8 +88,49 @@ def test_creator_assembler_weights_and_callback():
     assert np.isclose(a.volumes.sum(), 2.0)
     with pytest.raises(ValueError):
         a.element_densities(np.zeros(a.ndv + 1))
+
+
+def test_conform_creator_nodal_assembler():
+    """OctConformTopoCreator: element i's design variables are its nodes
+    (TMR_TACSTopoCreator.cpp:714-777); mass weights integrate the trilinear
+    density exactly."""
+    forest = _forest(seed=2)
+    props = TMR.StiffnessProperties(TMR.MaterialProperties(rho=2.0), penalty_type="SIMP")
+    bcs = TMR.BoundaryConditions()
+    bcs.addBoundaryCondition("fixed")
+
+    class Creator(TMR.OctConformTopoCreator):
+        calls = 0
+
+        def createElement(self, order, octant, index, filtr):
+            Creator.calls += 1
+            assert order == 2 and len(index) == 8 and filtr is self.getFilter()
+            return TMR.OctConstitutive(props, filtr)
+
+    a = Creator(bcs, forest).createTACS(forest)
+    assert Creator.calls == 1 and a.nodal and a.props is props
+    assert a.ndv == forest.getNodeRange()[1]
+    lb, ub = a.getDesignVarRange()
+    assert np.all(lb == 1e-3) and np.all(ub == 1.0)
+    # a constant density c integrates to c * volume; a linear field exactly
+    w = a.mass_weights()
+    assert np.isclose(w.sum(), a.volumes.sum(), rtol=1e-14)
+    nums = np.asarray(forest.getNodeNumbers())
+    X = forest.getPoints()[nums >= 0]   # points in sorted node-number order
+    assert np.array_equal(nums[nums >= 0], np.arange(a.ndv))
+    lin = 0.5 + 0.25 * X[:, 0] - 0.125 * X[:, 2]
+    # exact integral of 0.5 + 0.25 x - 0.125 z over [0,2]x[0,1]x[0,1]
+    assert np.isclose(w @ lin, 0.5 * 2 + 0.25 * 2.0 - 0.125 * 1.0, rtol=1e-13)
+    filt = TMR.ConformFilter([a], [forest])
+    x = np.linspace(0.2, 0.9, a.ndv)
+    assert np.array_equal(filt.applyFilter(x), x)
+    assert np.array_equal(filt.addValues(x), x)
+    with pytest.raises(ValueError):
+        a.element_densities(x)
     with pytest.raises(NotImplementedError):
-        TMR.OctConformTopoCreator(bcs, forest)
+        TMR.OctConformTopoCreator(bcs, forest, design_vars_per_node=2)
+    with pytest.raises(ValueError):
+        TMR.ConformFilter([a], [])
 
 
 @pytest.mark.parametrize("seed", [None, 2])

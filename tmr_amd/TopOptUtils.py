@@ -20,8 +20,10 @@ def createTopoProblem(forest, callback, filter_type, nlevels=2, repartition=True
     filter per level from ``creator, filter = callback(forest)``, the
     multigrid object and the TopoProblem (TopOptUtils.py:14-128).
 
-    Only order-2 meshes and the "lagrange" filter (element densities
-    interpolated from the filter forest) are built."""
+    Only order-2 meshes and the "lagrange" (element densities interpolated
+    from the filter forest, OctTopoCreator) and "conform" (nodal design
+    variables on the analysis forests, OctConformTopoCreator) filters are
+    built."""
     if design_vars_per_node != 1:
         raise NotImplementedError("one design variable per node only")
     if scale_coordinate_factor != 1.0:

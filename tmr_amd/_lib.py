@@ -78,6 +78,9 @@ _SIGS = {
     "tmrx_problem_get_sizes": (I, [P, IP, IP, IP]),
     "tmrx_problem_set_stream": (I, [P, P]),
     "tmrx_problem_set_design": (I, [P, P, I, I]),
+    "tmrx_problem_set_design_nodal": (I, [P, P, I, I]),
+    "tmrx_problem_design_info": (I, [P, IP, IP]),
+    "tmrx_problem_level_design": (I, [P, I, DP]),
     "tmrx_problem_setup": (I, [P, C.POINTER(SolveOpts)]),
     "tmrx_problem_solve": (I, [P, P, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "tmrx_problem_solve_multi": (I, [P, I, P, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),

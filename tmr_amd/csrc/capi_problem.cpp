@@ -148,6 +148,31 @@ int tmrx_problem_set_design(tmrx_problem *h, const double *rho, int n, int on_de
   return h->p->set_design(rho, n, on_device != 0) ? fail(h) : 0;
 }
 
+int tmrx_problem_set_design_nodal(tmrx_problem *h, const double *x, int n, int on_device) {
+  CHECK_P(h);
+  if (!x) {
+    set_error("tmrx_problem_set_design_nodal: null design vector");
+    return -1;
+  }
+  return h->p->set_design_nodal(x, n, on_device != 0) ? fail(h) : 0;
+}
+
+int tmrx_problem_design_info(tmrx_problem *h, int *mode, int *n) {
+  CHECK_P(h);
+  if (mode) *mode = h->p->design_mode();
+  if (n) *n = h->p->design_size();
+  return 0;
+}
+
+int tmrx_problem_level_design(tmrx_problem *h, int level, double *x) {
+  CHECK_P(h);
+  if (!x) {
+    set_error("tmrx_problem_level_design: null output");
+    return -1;
+  }
+  return h->p->level_design(level, x) ? fail(h) : 0;
+}
+
 int tmrx_problem_setup(tmrx_problem *h, const tmrx_solve_opts *opts) {
   CHECK_P(h);
   return h->p->setup(to_opts(opts)) ? fail(h) : 0;

@@ -44,26 +44,45 @@ void hex8_box_B(double hx, double hy, double hz, const double pt[3], double B[14
   }
 }
 
-void hex8_box_stiffness(double hx, double hy, double hz, const double C[36], double K[576]) {
+void hex8_box_stiffness_gp(double hx, double hy, double hz, const double C[36], int q,
+                           double K[576]) {
   std::memset(K, 0, 576 * sizeof(double));
   const double g = 1.0 / std::sqrt(3.0);
   const double detJ = hx * hy * hz / 8.0;
   double B[144], CB[144];
+  const double pt[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
+  hex8_box_B(hx, hy, hz, pt, B, nullptr);
+  for (int i = 0; i < 6; i++)
+    for (int k = 0; k < 24; k++) {
+      double s = 0.0;
+      for (int j = 0; j < 6; j++) s += C[6 * i + j] * B[24 * j + k];
+      CB[24 * i + k] = s;
+    }
+  for (int r = 0; r < 24; r++)
+    for (int c = 0; c < 24; c++) {
+      double s = 0.0;
+      for (int i = 0; i < 6; i++) s += B[24 * i + r] * CB[24 * i + c];
+      K[24 * r + c] = detJ * s;
+    }
+}
+
+void hex8_box_stiffness(double hx, double hy, double hz, const double C[36], double K[576]) {
+  std::memset(K, 0, 576 * sizeof(double));
+  double Kq[576];
   for (int q = 0; q < 8; q++) {
-    double pt[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
+    hex8_box_stiffness_gp(hx, hy, hz, C, q, Kq);
+    for (int i = 0; i < 576; i++) K[i] += Kq[i];
+  }
+}
+
+void hex8_box_grad_table(double hx, double hy, double hz, double G[192]) {
+  const double g = 1.0 / std::sqrt(3.0);
+  double B[144];
+  for (int q = 0; q < 8; q++) {
+    const double pt[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
     hex8_box_B(hx, hy, hz, pt, B, nullptr);
-    for (int i = 0; i < 6; i++)
-      for (int k = 0; k < 24; k++) {
-        double s = 0.0;
-        for (int j = 0; j < 6; j++) s += C[6 * i + j] * B[24 * j + k];
-        CB[24 * i + k] = s;
-      }
-    for (int r = 0; r < 24; r++)
-      for (int c = 0; c < 24; c++) {
-        double s = 0.0;
-        for (int i = 0; i < 6; i++) s += B[24 * i + r] * CB[24 * i + c];
-        K[24 * r + c] += detJ * s;
-      }
+    for (int a = 0; a < 8; a++)
+      for (int d = 0; d < 3; d++) G[24 * q + 3 * a + d] = B[24 * d + 3 * a + d];
   }
 }
 

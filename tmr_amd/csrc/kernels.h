@@ -100,7 +100,15 @@ struct Tiles {
   int n_mrec = 0;
   const uint4 *afrag = nullptr;   // [n_classes][hi, lo][row block][64 lanes]
   bool use_mfma = false;
+  // nodal design: the element stiffness is sum_g s_eg K_g over the 8 Gauss
+  // points (K0 holds the gradient tables, s_elem 8 scales per element)
+  bool nodal = false;
 };
+
+// Per stiffness class, the nodal-design element tables: dN_a/dx_d at the 8
+// Gauss points of the full-block element (G[24 g + 3 a + d], 192 entries),
+// then lambda and mu times the Gauss weight |J| (= volume / 8), then 2 pads.
+constexpr int NODAL_TABLE = 196;
 
 
 struct AsmCSR {
@@ -147,7 +155,7 @@ void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, cons
                        T *partial, hipStream_t st);
 template <class T>
 void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *partial,
-                      hipStream_t st);
+                      hipStream_t st, bool nodal = false);
 template <class T>
 void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, int mode,
                      const AsmArgs &args, hipStream_t st);
@@ -193,6 +201,32 @@ void launch_simp_scale(int n, const double *rho, const int16_t *level, double q,
 void launch_restrict_scale(int nc, const int *ptr, const int *child, const float *vw,
                            const double *s_fine, const int16_t *lev_fine,
                            const int16_t *lev_coarse, double *s64, float *s32, hipStream_t st);
+
+// nodal design (TMROctConstitutive with one design variable per node,
+// src/topology/TMROctConstitutive.cpp:498-551): rho at the 8 Gauss points of
+// every element from its corner values (hanging corners interpolated from
+// their parents), s = penalty(rho) * 2^-level per point; rho_gp (may be null)
+// receives the point densities
+void launch_nodal_scale(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                        const float *dep_w, const double *xn, const int16_t *level, double q,
+                        double k0, int penalty_type, double *rho_gp, double *s64, float *s32,
+                        hipStream_t st);
+// design vector to the next coarser level: x_c = (R x_f) / (R 1), R = P^T
+// (TACSBVecInterp::multWeightTranspose as used by TMRConformFilter::setDesignVars,
+// src/topology/TMRConformFilter.cpp:154-167)
+void launch_weight_restrict(const TransferCSR &R, const double *xf, double *xc, hipStream_t st);
+// v_e^T K_g u_e at every Gauss point of every element (full-block tables)
+void launch_nodal_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                         const float *dep_w, const uint8_t *bcmask, const int *cls,
+                         const double *G, const double *u, const double *v, double *out,
+                         hipStream_t st);
+// nodal design gradient into slot partials (element batches, component 0):
+// t_a = sum_g N_a(g) (ex_g - dpenalty(rho_g) 2^-level e_g), ex may be null
+void launch_nodal_grad(const ElemBatches &eb, const int16_t *lev, const double *rho_gp, double q,
+                       int penalty_type, const double *e_gp, const double *ex_gp,
+                       double *partial, hipStream_t st);
+// out[i] = in[3 i]
+void launch_take_comp0(int n, const double *in, double *out, hipStream_t st);
 
 // FP64 PCG helpers (scalars live on the device)
 enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_TRR, SC_N };
@@ -260,15 +294,19 @@ struct StressArgs {
   const double *CB = nullptr;
   const double *vol = nullptr;
 };
+// rho_gp (nodal design, may be null): densities at the Gauss points, used
+// in place of the element densities rho
 void launch_stress_sum(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                        const float *dep_w, const uint8_t *bcmask, const int *cls,
-                       const int16_t *lev, const double *rho, const double *u,
-                       const StressArgs &sa, double *partials, hipStream_t st);
+                       const int16_t *lev, const double *rho, const double *rho_gp,
+                       const double *u, const StressArgs &sa, double *partials, hipStream_t st);
 // element dfdu into slot partials (element batches) and the explicit
-// d f / d rho_e; coef = S^(1/P - 1) read from sc[SC_ZQ]
+// d f / d rho_e (or, with rho_gp, d f / d rho_g per Gauss point);
+// coef = S^(1/P - 1) read from sc[SC_ZQ]
 void launch_stress_grad(const ElemBatches &eb, const int16_t *lev, const int *cls,
-                        const double *rho, const double *u, const StressArgs &sa,
-                        const double *sc, double *partial, double *dfdrho, hipStream_t st);
+                        const double *rho, const double *rho_gp, const double *u,
+                        const StressArgs &sa, const double *sc, double *partial,
+                        double *dfdrho, hipStream_t st);
 // psi_e^T K0 u_e per element (adjoint-residual product without the factor)
 void launch_elem_energy2(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                          const float *dep_w, const uint8_t *bcmask, const int *cls,

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(WG_ELEMS) void k_elem_apply(
   for (int i = tid; i < 3 * nn; i += WG_ELEMS) partial[3 * nb + i] = ys[i];
 }
 
-template <class T>
+template <class T, bool NODAL>
 __global__ __launch_bounds__(WG_ELEMS) void k_elem_diag(
     const int *__restrict__ wg_node_ptr, const int *__restrict__ wg_elem_ptr,
     const int *__restrict__ wg_class, const uint4 *__restrict__ lconn,
@@ -135,10 +135,9 @@ __global__ __launch_bounds__(WG_ELEMS) void k_elem_diag(
   __syncthreads();
   const int e0 = wg_elem_ptr[wg], ne = wg_elem_ptr[wg + 1] - e0;
   const int cls = __builtin_amdgcn_readfirstlane(wg_class[wg]);
-  const T *__restrict__ K = K0 + 576 * cls;
   int lc[8];
   int col = -1;
-  T s = T(0);
+  T de[24];
   if (tid < ne) {
     const uint4 q = lconn[e0 + tid];
     lc[0] = q.x & 0xffff; lc[1] = q.x >> 16;
@@ -146,14 +145,38 @@ __global__ __launch_bounds__(WG_ELEMS) void k_elem_diag(
     lc[4] = q.z & 0xffff; lc[5] = q.z >> 16;
     lc[6] = q.w & 0xffff; lc[7] = q.w >> 16;
     col = color[e0 + tid];
-    s = s_elem[e0 + tid];
+    if constexpr (NODAL) {
+      // diag K_g = |J| w [(lambda + mu) G_d^2 + mu |G|^2] per Gauss point
+      const T *__restrict__ Gt = K0 + NODAL_TABLE * cls;
+      const T lam = Gt[192], mu = Gt[193];
+      const T *__restrict__ sg = s_elem + 8 * (size_t)(e0 + tid);
+#pragma unroll
+      for (int i = 0; i < 24; i++) de[i] = T(0);
+#pragma unroll
+      for (int g = 0; g < 8; g++) {
+        const T s = sg[g];
+#pragma unroll
+        for (int a = 0; a < 8; a++) {
+          const T g0 = Gt[24 * g + 3 * a], g1 = Gt[24 * g + 3 * a + 1], g2 = Gt[24 * g + 3 * a + 2];
+          const T n2 = mu * (g0 * g0 + g1 * g1 + g2 * g2);
+          de[3 * a] += s * ((lam + mu) * g0 * g0 + n2);
+          de[3 * a + 1] += s * ((lam + mu) * g1 * g1 + n2);
+          de[3 * a + 2] += s * ((lam + mu) * g2 * g2 + n2);
+        }
+      }
+    } else {
+      const T *__restrict__ K = K0 + 576 * cls;
+      const T s = s_elem[e0 + tid];
+#pragma unroll
+      for (int i = 0; i < 24; i++) de[i] = s * K[i * 25];
+    }
   }
   for (int c = 0; c < ncolors; c++) {
     if (col == c) {
 #pragma unroll
       for (int a = 0; a < 8; a++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) ys[3 * lc[a] + d] += s * K[(3 * a + d) * 25];
+        for (int d = 0; d < 3; d++) ys[3 * lc[a] + d] += de[3 * a + d];
     }
     __syncthreads();
   }
@@ -174,12 +197,19 @@ void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, cons
 
 template <class T>
 void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *partial,
-                      hipStream_t st) {
+                      hipStream_t st, bool nodal) {
   if (eb.n_wg == 0) return;
   size_t sh = 3 * (size_t)eb.max_wg_nodes * sizeof(T);
-  hipLaunchKernelGGL(k_elem_diag<T>, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
-                     eb.wg_elem_ptr, eb.wg_class, reinterpret_cast<const uint4 *>(eb.lconn),
-                     eb.color, eb.ncolors, K0, s_elem, partial);
+  if (nodal)
+    hipLaunchKernelGGL((k_elem_diag<T, true>), dim3(eb.n_wg), dim3(WG_ELEMS), sh, st,
+                       eb.wg_node_ptr, eb.wg_elem_ptr, eb.wg_class,
+                       reinterpret_cast<const uint4 *>(eb.lconn), eb.color, eb.ncolors, K0, s_elem,
+                       partial);
+  else
+    hipLaunchKernelGGL((k_elem_diag<T, false>), dim3(eb.n_wg), dim3(WG_ELEMS), sh, st,
+                       eb.wg_node_ptr, eb.wg_elem_ptr, eb.wg_class,
+                       reinterpret_cast<const uint4 *>(eb.lconn), eb.color, eb.ncolors, K0, s_elem,
+                       partial);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -524,6 +554,220 @@ void launch_restrict_scale(int nc, const int *ptr, const int *child, const float
   if (nc == 0) return;
   hipLaunchKernelGGL(k_restrict_scale, dim3((nc + 255) / 256), dim3(256), 0, st, nc, ptr, child, vw,
                      s_fine, lev_fine, lev_coarse, s64, s32);
+  TMRX_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// nodal design variables (one per node, TMROctConstitutive with nvars = 1)
+
+// trilinear shape function of corner a (bits x, y, z) at Gauss point g (bit
+// d of g set: coordinate d at +1/sqrt(3)): each factor is (1 +- 1/sqrt(3))/2
+__device__ __forceinline__ double gauss_N(int a, int g) {
+  const double gp = 0.57735026918962576451;  // 1/sqrt(3)
+  const double hi = 0.5 * (1.0 + gp), lo = 0.5 * (1.0 - gp);
+  const int same = ~(a ^ g);
+  return ((same & 1) ? hi : lo) * ((same & 2) ? hi : lo) * ((same & 4) ? hi : lo);
+}
+
+__device__ __forceinline__ double node_value(int g, const double *__restrict__ xn,
+                                             const int *__restrict__ dep_ptr,
+                                             const int *__restrict__ dep_conn,
+                                             const float *__restrict__ dep_w) {
+  if (g >= 0) return xn[g];
+  const int d = -g - 1;
+  double v = 0.0;
+  for (int j = dep_ptr[d]; j < dep_ptr[d + 1]; j++) v += (double)dep_w[j] * xn[dep_conn[j]];
+  return v;
+}
+
+__device__ __forceinline__ double penalty_of(double r, double q, double k0, int ptype) {
+  return (ptype == 0) ? pow(r, q) + k0 : r / (1.0 + q * (1.0 - r)) + k0;
+}
+
+__device__ __forceinline__ double dpenalty_of(double r, double q, int ptype) {
+  if (ptype == 0) return (q > 1.0) ? q * pow(r, q - 1.0) : 1.0;
+  const double t = 1.0 + q * (1.0 - r);
+  return (q + 1.0) / (t * t);
+}
+
+__global__ void k_nodal_scale(int ne, const int *__restrict__ conn, const int *__restrict__ dep_ptr,
+                              const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+                              const double *__restrict__ xn, const int16_t *__restrict__ lev,
+                              double q, double k0, int ptype, double *__restrict__ rho_gp,
+                              double *__restrict__ s64, float *__restrict__ s32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double xa[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) xa[a] = node_value(conn[8 * (size_t)e + a], xn, dep_ptr, dep_conn, dep_w);
+  const int l = lev[e];
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    double r = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) r += gauss_N(a, g) * xa[a];
+    const double sc = ldexp(penalty_of(r, q, k0, ptype), -l);
+    const size_t k = 8 * (size_t)e + g;
+    if (rho_gp) rho_gp[k] = r;
+    s64[k] = sc;
+    s32[k] = (float)sc;
+  }
+}
+
+void launch_nodal_scale(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                        const float *dep_w, const double *xn, const int16_t *level, double q,
+                        double k0, int penalty_type, double *rho_gp, double *s64, float *s32,
+                        hipStream_t st) {
+  if (nelem == 0) return;
+  hipLaunchKernelGGL(k_nodal_scale, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
+                     dep_ptr, dep_conn, dep_w, xn, level, q, k0, penalty_type, rho_gp, s64, s32);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_weight_restrict(int nc, const int *__restrict__ ptr, const int *__restrict__ col,
+                                  const float *__restrict__ w, const double *__restrict__ xf,
+                                  double *__restrict__ xc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  double s = 0.0, ws = 0.0;
+  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+    const double wj = (double)w[j];
+    s += wj * xf[col[j]];
+    ws += wj;
+  }
+  xc[i] = s / ws;
+}
+
+void launch_weight_restrict(const TransferCSR &R, const double *xf, double *xc, hipStream_t st) {
+  if (R.n_rows == 0) return;
+  hipLaunchKernelGGL(k_weight_restrict, dim3((R.n_rows + 255) / 256), dim3(256), 0, st, R.n_rows,
+                     R.ptr, R.col, R.w, xf, xc);
+  TMRX_LAUNCH_CHECK();
+}
+
+// grad u at Gauss point g of an element (H[3 c + d] = du_c / dx_d)
+__device__ __forceinline__ void grad_at(const double *__restrict__ G, const double ue[24],
+                                        double H[9]) {
+  for (int i = 0; i < 9; i++) H[i] = 0.0;
+  for (int a = 0; a < 8; a++)
+    for (int c = 0; c < 3; c++)
+      for (int d = 0; d < 3; d++) H[3 * c + d] += ue[3 * a + c] * G[3 * a + d];
+}
+
+__global__ void k_nodal_energy(int ne, const int *__restrict__ conn,
+                               const int *__restrict__ dep_ptr, const int *__restrict__ dep_conn,
+                               const float *__restrict__ dep_w, const uint8_t *__restrict__ bcmask,
+                               const int *__restrict__ cls, const double *__restrict__ Gtab,
+                               const double *__restrict__ u, const double *__restrict__ v,
+                               double *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double ue[24], ve[24];
+  for (int a = 0; a < 8; a++) {
+    double t[3];
+    load_node<double>(conn[8 * (size_t)e + a], u, dep_ptr, dep_conn, dep_w, bcmask, t);
+    ue[3 * a] = t[0];
+    ue[3 * a + 1] = t[1];
+    ue[3 * a + 2] = t[2];
+    load_node<double>(conn[8 * (size_t)e + a], v, dep_ptr, dep_conn, dep_w, bcmask, t);
+    ve[3 * a] = t[0];
+    ve[3 * a + 1] = t[1];
+    ve[3 * a + 2] = t[2];
+  }
+  const double *Gt = Gtab + NODAL_TABLE * (size_t)cls[e];
+  const double lam = Gt[192], mu = Gt[193];
+  for (int g = 0; g < 8; g++) {
+    double Hu[9], Hv[9];
+    grad_at(Gt + 24 * g, ue, Hu);
+    grad_at(Gt + 24 * g, ve, Hv);
+    // eps_v : C eps_u with symmetric strains eps = (H + H^T) / 2
+    double se = 0.0;
+    for (int c = 0; c < 3; c++)
+      for (int d = 0; d < 3; d++)
+        se += 0.25 * (Hu[3 * c + d] + Hu[3 * d + c]) * (Hv[3 * c + d] + Hv[3 * d + c]);
+    const double tru = Hu[0] + Hu[4] + Hu[8], trv = Hv[0] + Hv[4] + Hv[8];
+    out[8 * (size_t)e + g] = lam * tru * trv + 2.0 * mu * se;
+  }
+}
+
+void launch_nodal_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
+                         const float *dep_w, const uint8_t *bcmask, const int *cls,
+                         const double *G, const double *u, const double *v, double *out,
+                         hipStream_t st) {
+  if (nelem == 0) return;
+  hipLaunchKernelGGL(k_nodal_energy, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
+                     dep_ptr, dep_conn, dep_w, bcmask, cls, G, u, v, out);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(WG_ELEMS) void k_nodal_grad(
+    const int *__restrict__ wg_node_ptr, const int *__restrict__ wg_elem_ptr,
+    const uint4 *__restrict__ lconn, const uint8_t *__restrict__ color, int ncolors,
+    const int16_t *__restrict__ lev, const double *__restrict__ rho_gp, double q, int ptype,
+    const double *__restrict__ e_gp, const double *__restrict__ ex_gp,
+    double *__restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wg = blockIdx.x, tid = threadIdx.x;
+  const int nb = wg_node_ptr[wg], nn = wg_node_ptr[wg + 1] - nb;
+  double *acc = reinterpret_cast<double *>(smem_raw);
+  for (int i = tid; i < nn; i += WG_ELEMS) acc[i] = 0.0;
+  __syncthreads();
+  const int e0 = wg_elem_ptr[wg], ne = wg_elem_ptr[wg + 1] - e0;
+  int lc[8];
+  int col = -1;
+  double t[8];
+  if (tid < ne) {
+    const int e = e0 + tid;
+    const uint4 qq = lconn[e];
+    lc[0] = qq.x & 0xffff; lc[1] = qq.x >> 16;
+    lc[2] = qq.y & 0xffff; lc[3] = qq.y >> 16;
+    lc[4] = qq.z & 0xffff; lc[5] = qq.z >> 16;
+    lc[6] = qq.w & 0xffff; lc[7] = qq.w >> 16;
+    col = color[e];
+    double wgp[8];
+    for (int g = 0; g < 8; g++) {
+      const size_t k = 8 * (size_t)e + g;
+      const double dp = ldexp(dpenalty_of(rho_gp[k], q, ptype), -(int)lev[e]);
+      wgp[g] = (ex_gp ? ex_gp[k] : 0.0) - dp * e_gp[k];
+    }
+    for (int a = 0; a < 8; a++) {
+      double v = 0.0;
+      for (int g = 0; g < 8; g++) v += gauss_N(a, g) * wgp[g];
+      t[a] = v;
+    }
+  }
+  for (int c = 0; c < ncolors; c++) {
+    if (col == c) {
+      for (int a = 0; a < 8; a++) acc[lc[a]] += t[a];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < nn; i += WG_ELEMS) {
+    partial[3 * (size_t)(nb + i)] = acc[i];
+    partial[3 * (size_t)(nb + i) + 1] = 0.0;
+    partial[3 * (size_t)(nb + i) + 2] = 0.0;
+  }
+}
+
+void launch_nodal_grad(const ElemBatches &eb, const int16_t *lev, const double *rho_gp, double q,
+                       int penalty_type, const double *e_gp, const double *ex_gp,
+                       double *partial, hipStream_t st) {
+  if (eb.n_wg == 0) return;
+  const size_t sh = (size_t)eb.max_wg_nodes * sizeof(double);
+  hipLaunchKernelGGL(k_nodal_grad, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
+                     eb.wg_elem_ptr, reinterpret_cast<const uint4 *>(eb.lconn), eb.color,
+                     eb.ncolors, lev, rho_gp, q, penalty_type, e_gp, ex_gp, partial);
+  TMRX_LAUNCH_CHECK();
+}
+
+__global__ void k_take_comp0(int n, const double *__restrict__ in, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[3 * (size_t)i];
+}
+
+void launch_take_comp0(int n, const double *in, double *out, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_take_comp0, dim3((n + 255) / 256), dim3(256), 0, st, n, in, out);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -892,7 +1136,49 @@ __device__ __forceinline__ void tile_finish(
   }
 }
 
-template <class T, int MODE, bool SINGLE_CLASS>
+// Nodal-design element product y_e = sum_g s_g K_g x_e in gradient form: at
+// every Gauss point H = grad u, sigma = s_g (lambda tr(H) I + mu (H + H^T))
+// (lambda, mu carry the Gauss weight |J|), y_a += sigma grad N_a; only the
+// rows of the corners in `mask` are formed.
+template <class T>
+__device__ __forceinline__ void nodal_elem(const T *__restrict__ Gt, const T *__restrict__ sg,
+                                           const T xe[24], int mask, T ye[24]) {
+  const T lam = Gt[192], mu = Gt[193];
+#pragma unroll
+  for (int i = 0; i < 24; i++) ye[i] = T(0);
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    const T *__restrict__ G = Gt + 24 * g;
+    T H[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) H[i] = T(0);
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) H[3 * c + d] += xe[3 * a + c] * G[3 * a + d];
+    const T s = sg[g];
+    const T tr = lam * (H[0] + H[4] + H[8]);
+    T S[9];
+    S[0] = s * (tr + T(2) * mu * H[0]);
+    S[4] = s * (tr + T(2) * mu * H[4]);
+    S[8] = s * (tr + T(2) * mu * H[8]);
+    S[1] = S[3] = s * mu * (H[1] + H[3]);
+    S[2] = S[6] = s * mu * (H[2] + H[6]);
+    S[5] = S[7] = s * mu * (H[5] + H[7]);
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+      if (mask & (1 << a)) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          ye[3 * a + c] += S[3 * c] * G[3 * a] + S[3 * c + 1] * G[3 * a + 1] +
+                           S[3 * c + 2] * G[3 * a + 2];
+      }
+  }
+}
+
+template <class T, int MODE, bool SINGLE_CLASS, bool NODAL>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
     const int *__restrict__ nacc_dep, const int *__restrict__ el_ptr,
@@ -942,8 +1228,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       mask = w2.y & 0xff;
       col = (w2.y >> 8) & 0xff;
       const int cls = SINGLE_CLASS ? 0 : (int)((w2.y >> 16) & 0xff);
-      const T *__restrict__ K = K0 + 576 * cls;
-      const T sc = s_elem[elem];
       T xe[24];
 #pragma unroll
       for (int a = 0; a < 8; a++) {
@@ -951,15 +1235,21 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
         xe[3 * a + 1] = xs[3 * lc[a] + 1];
         xe[3 * a + 2] = xs[3 * lc[a] + 2];
       }
+      if constexpr (NODAL) {
+        nodal_elem<T>(K0 + NODAL_TABLE * cls, s_elem + 8 * (size_t)elem, xe, mask, ye);
+      } else {
+        const T *__restrict__ K = K0 + 576 * cls;
+        const T sc = s_elem[elem];
 #pragma unroll
-      for (int a = 0; a < 8; a++) {
-        if (mask & (1 << a)) {
+        for (int a = 0; a < 8; a++) {
+          if (mask & (1 << a)) {
 #pragma unroll
-          for (int r = 0; r < 3; r++) {
-            T acc = T(0);
+            for (int r = 0; r < 3; r++) {
+              T acc = T(0);
 #pragma unroll
-            for (int j = 0; j < 24; j++) acc += K[(3 * a + r) * 24 + j] * xe[j];
-            ye[3 * a + r] = sc * acc;
+              for (int j = 0; j < 24; j++) acc += K[(3 * a + r) * 24 + j] * xe[j];
+              ye[3 * a + r] = sc * acc;
+            }
           }
         }
       }
@@ -1246,7 +1536,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   const T *dinv = (const T *)g.dinv;
   const bool single = tl.single_class;
   if constexpr (sizeof(T) == 4) {
-    if (tl.use_mfma) {
+    if (tl.use_mfma && !tl.nodal) {
       const size_t sh4 = 24 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
 #define TMRX_MTILE(M, S)                                                                         \
   hipLaunchKernelGGL((k_tile_apply_mfma<M, S>), grid, dim3(TILE_THREADS), sh4, st,               \
@@ -1274,7 +1564,14 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
     }
   }
 #define TMRX_TILE(M, S)                                                                         \
-  hipLaunchKernelGGL((k_tile_apply<T, M, S>), grid, dim3(TILE_THREADS), sh, st,                  \
+  if (tl.nodal)                                                                                 \
+    hipLaunchKernelGGL((k_tile_apply<T, M, S, true>), grid, dim3(TILE_THREADS), sh, st,          \
+                       tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                    \
+                       reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,         \
+                       tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,   \
+                       K0, s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials, vs);           \
+  else                                                                                          \
+  hipLaunchKernelGGL((k_tile_apply<T, M, S, false>), grid, dim3(TILE_THREADS), sh, st,           \
                      tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                      \
                      reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,           \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0, \
@@ -1314,8 +1611,9 @@ __global__ __launch_bounds__(256) void k_stress_sum(
     int ne, const int *__restrict__ conn, const int *__restrict__ dep_ptr,
     const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
     const uint8_t *__restrict__ bcmask, const int *__restrict__ cls,
-    const int16_t *__restrict__ lev, const double *__restrict__ rho, const double *__restrict__ u,
-    double P, double eps, double ys, const double *__restrict__ CB, const double *__restrict__ vol,
+    const int16_t *__restrict__ lev, const double *__restrict__ rho,
+    const double *__restrict__ rho_gp, const double *__restrict__ u, double P, double eps,
+    double ys, const double *__restrict__ CB, const double *__restrict__ vol,
     double *__restrict__ partials) {
   __shared__ double red[4];
   const int e = blockIdx.x * 256 + threadIdx.x;
@@ -1332,9 +1630,11 @@ __global__ __launch_bounds__(256) void k_stress_sum(
     const int c = cls[e], l = lev[e];
     const double hs = ldexp(1.0, l);  // B scales as 1/h
     const double Ve = ldexp(vol[c], -3 * l);
-    const double r = rho[e] / (eps * (1.0 - rho[e]) + rho[e]);
+    const double re = rho_gp ? 0.0 : rho[e];
     const double *G = CB + 8 * 144 * (size_t)c;
     for (int g = 0; g < 8; g++) {
+      const double rh = rho_gp ? rho_gp[8 * (size_t)e + g] : re;
+      const double r = rh / (eps * (1.0 - rh) + rh);
       double sg[6];
       for (int i = 0; i < 6; i++) {
         double t = 0.0;
@@ -1353,12 +1653,12 @@ __global__ __launch_bounds__(256) void k_stress_sum(
 
 void launch_stress_sum(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                        const float *dep_w, const uint8_t *bcmask, const int *cls,
-                       const int16_t *lev, const double *rho, const double *u,
-                       const StressArgs &sa, double *partials, hipStream_t st) {
+                       const int16_t *lev, const double *rho, const double *rho_gp,
+                       const double *u, const StressArgs &sa, double *partials, hipStream_t st) {
   if (nelem == 0) return;
   hipLaunchKernelGGL(k_stress_sum, dim3((nelem + 255) / 256), dim3(256), 0, st, nelem, conn,
-                     dep_ptr, dep_conn, dep_w, bcmask, cls, lev, rho, u, sa.P, sa.eps, sa.ys,
-                     sa.CB, sa.vol, partials);
+                     dep_ptr, dep_conn, dep_w, bcmask, cls, lev, rho, rho_gp, u, sa.P, sa.eps,
+                     sa.ys, sa.CB, sa.vol, partials);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -1371,10 +1671,10 @@ __global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
     const uint8_t *__restrict__ color, int ncolors, const int *__restrict__ dep_ptr,
     const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
     const uint8_t *__restrict__ bcmask, const int16_t *__restrict__ lev,
-    const int *__restrict__ cls, const double *__restrict__ rho, const double *__restrict__ u,
-    double P, double eps, double ys, const double *__restrict__ CB,
-    const double *__restrict__ vol, const double *__restrict__ sc,
-    double *__restrict__ partial, double *__restrict__ dfdrho) {
+    const int *__restrict__ cls, const double *__restrict__ rho,
+    const double *__restrict__ rho_gp, const double *__restrict__ u, double P, double eps,
+    double ys, const double *__restrict__ CB, const double *__restrict__ vol,
+    const double *__restrict__ sc, double *__restrict__ partial, double *__restrict__ dfdrho) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int wg = blockIdx.x, tid = threadIdx.x;
   const int nb = wg_node_ptr[wg], nn = wg_node_ptr[wg + 1] - nb;
@@ -1409,12 +1709,13 @@ __global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
     const int c = cls[e], l = lev[e];
     const double hs = ldexp(1.0, l);
     const double Ve = ldexp(vol[c], -3 * l);
-    const double rh = rho[e];
-    const double den = eps * (1.0 - rh) + rh;
-    const double r = rh / den, dr = eps / (den * den);
+    const double re = rho_gp ? 0.0 : rho[e];
     const double *G = CB + 8 * 144 * (size_t)c;
     double sum_vmP = 0.0;
     for (int g = 0; g < 8; g++) {
+      const double rh = rho_gp ? rho_gp[8 * (size_t)e + g] : re;
+      const double den = eps * (1.0 - rh) + rh;
+      const double r = rh / den, dr = eps / (den * den);
       double sg[6];
       for (int i = 0; i < 6; i++) {
         double t = 0.0;
@@ -1425,6 +1726,7 @@ __global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
       vm_terms(sg, ys, &J, dJ);
       const double vm = sqrt(J) / ys;
       sum_vmP += pow(vm, P);
+      if (rho_gp) dfdrho[8 * (size_t)e + g] = coef * 0.125 * Ve * pow(r, P - 1.0) * dr * pow(vm, P);
       // coef (V/8) r^P vm^(P-2) / (2 ys^2) * dJ^T CB_g (with the 1/h factor)
       const double w = coef * 0.125 * Ve * pow(r, P) * pow(vm, P - 2.0) / (2.0 * ys * ys) * hs;
       for (int i = 0; i < 6; i++) {
@@ -1432,7 +1734,11 @@ __global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
         for (int j = 0; j < 24; j++) ge[j] += wi * G[144 * g + 24 * i + j];
       }
     }
-    dfdrho[e] = coef * 0.125 * Ve * pow(r, P - 1.0) * dr * sum_vmP;
+    if (!rho_gp) {
+      const double den = eps * (1.0 - re) + re;
+      const double r = re / den, dr = eps / (den * den);
+      dfdrho[e] = coef * 0.125 * Ve * pow(r, P - 1.0) * dr * sum_vmP;
+    }
   }
   for (int c = 0; c < ncolors; c++) {
     if (col == c) {
@@ -1445,14 +1751,16 @@ __global__ __launch_bounds__(WG_ELEMS) void k_stress_grad(
 }
 
 void launch_stress_grad(const ElemBatches &eb, const int16_t *lev, const int *cls,
-                        const double *rho, const double *u, const StressArgs &sa,
-                        const double *sc, double *partial, double *dfdrho, hipStream_t st) {
+                        const double *rho, const double *rho_gp, const double *u,
+                        const StressArgs &sa, const double *sc, double *partial,
+                        double *dfdrho, hipStream_t st) {
   if (eb.n_wg == 0) return;
   const size_t sh = 2 * 3 * (size_t)eb.max_wg_nodes * sizeof(double);
   hipLaunchKernelGGL(k_stress_grad, dim3(eb.n_wg), dim3(WG_ELEMS), sh, st, eb.wg_node_ptr,
                      eb.wg_nodes, eb.wg_elem_ptr, reinterpret_cast<const uint4 *>(eb.lconn),
                      eb.color, eb.ncolors, eb.dep_ptr, eb.dep_conn, eb.dep_w, eb.bcmask, lev,
-                     cls, rho, u, sa.P, sa.eps, sa.ys, sa.CB, sa.vol, sc, partial, dfdrho);
+                     cls, rho, rho_gp, u, sa.P, sa.eps, sa.ys, sa.CB, sa.vol, sc, partial,
+                     dfdrho);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -1525,7 +1833,7 @@ void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q,
   template void launch_elem_apply<T>(const ElemBatches &, const T *, const T *, const T *, T *, \
                                      hipStream_t);                                              \
   template void launch_elem_diag<T>(const ElemBatches &, const T *, const T *, T *,             \
-                                    hipStream_t);                                               \
+                                    hipStream_t, bool);                                         \
   template void launch_assemble<T>(const AsmCSR &, const uint8_t *, const T *, int,             \
                                    const AsmArgs &, hipStream_t);                               \
   template void launch_prolong_add<T>(const TransferCSR &, const uint8_t *, const T *, T *,     \

@@ -87,6 +87,14 @@ template <>
 const float *Problem::scale<float>(int l) { return levels_[l].s32.p; }
 template <>
 const double *Problem::scale<double>(int l) { return levels_[l].s64.p; }
+template <>
+const float *Problem::gtab<float>() { return G32_.p; }
+template <>
+const double *Problem::gtab<double>() { return G64_.p; }
+template <>
+const float *Problem::gscale<float>(int l) { return levels_[l].sg32.p; }
+template <>
+const double *Problem::gscale<double>(int l) { return levels_[l].sg64.p; }
 
 // ---------------------------------------------------------------------------
 int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
@@ -146,6 +154,28 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
   K0h_.assign(576 * (size_t)nclasses_, 0.0);
   for (int c = 0; c < nclasses_; c++)
     hex8_box_stiffness(ext[c][0], ext[c][1], ext[c][2], C, &K0h_[576 * c]);
+  // nodal design: Gauss-point stiffness (host, coarse factor) and gradient
+  // tables with lambda |J|, mu |J| (device)
+  {
+    Kgh_.assign(8 * 576 * (size_t)nclasses_, 0.0);
+    std::vector<double> gt(NODAL_TABLE * (size_t)nclasses_, 0.0);
+    const double lam = mat.E * mat.nu / ((1.0 + mat.nu) * (1.0 - 2.0 * mat.nu));
+    const double mu = mat.E / (2.0 * (1.0 + mat.nu));
+    for (int c = 0; c < nclasses_; c++) {
+      for (int q = 0; q < 8; q++)
+        hex8_box_stiffness_gp(ext[c][0], ext[c][1], ext[c][2], C, q, &Kgh_[576 * (8 * (size_t)c + q)]);
+      double *g = &gt[NODAL_TABLE * (size_t)c];
+      hex8_box_grad_table(ext[c][0], ext[c][1], ext[c][2], g);
+      const double wj = ext[c][0] * ext[c][1] * ext[c][2] / 8.0;
+      g[192] = lam * wj;
+      g[193] = mu * wj;
+    }
+    std::vector<float> gtf(gt.begin(), gt.end());
+    if (G64_.upload(gt) || G32_.upload(gtf)) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
+  }
   // Gauss-point stress operators C0 B_g (6 x 24, 8 points) and volumes of the
   // full-block element of every class (p-norm stress functional)
   {
@@ -796,6 +826,7 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
   }
   HIPCHK(hipMemcpyAsync(rho_.p, rho, n * sizeof(double),
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  design_mode_ = 0;
   launch_simp_scale(n, rho_.p, levels_[0].d_lev.p, mat_.q, mat_.k0, mat_.penalty_type,
                     levels_[0].s64.p, levels_[0].s32.p, st_);
   for (size_t l = 0; l + 1 < levels_.size(); l++) {
@@ -808,6 +839,80 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
                      reinterpret_cast<uint32_t *>(L.tiles.mel), st_);
   have_design_ = true;
   have_setup_ = false;
+  return 0;
+}
+
+// Nodal design variables (TMROctConformTACSTopoCreator: the design forest is
+// the analysis forest, src/topology/TMR_TACSTopoCreator.cpp:714-777).  Every
+// level gets its design by the weighted restriction of the next finer one
+// (TMRConformFilter::setDesignVars, src/topology/TMRConformFilter.cpp:154-167)
+// and per-Gauss-point stiffness scales (TMROctConstitutive::evalTangentStiffness,
+// src/topology/TMROctConstitutive.cpp:498-551).
+int Problem::set_design_nodal(const double *x, int n, bool on_device) {
+  if (levels_.empty()) {
+    last_error = "problem: not initialised";
+    return -1;
+  }
+  if (n != levels_[0].n_nodes) {
+    last_error = "set_design_nodal: expected one design value per finest-level node";
+    return -1;
+  }
+  for (auto &L : levels_) {
+    if (L.xn.n < (size_t)L.n_nodes && L.xn.alloc(L.n_nodes)) goto oom;
+    if (L.sg64.n < 8 * (size_t)L.n_elem && (L.sg64.alloc(8 * (size_t)L.n_elem) ||
+                                            L.sg32.alloc(8 * (size_t)L.n_elem)))
+      goto oom;
+  }
+  {
+    const size_t ng = 8 * (size_t)levels_[0].n_elem;
+    if ((rho_gp_.n < ng && rho_gp_.alloc(ng)) || (egp_.n < ng && egp_.alloc(ng)) ||
+        (exgp_.n < ng && exgp_.alloc(ng)) ||
+        (gtmp_.n < (size_t)levels_[0].ndof && gtmp_.alloc(levels_[0].ndof)))
+      goto oom;
+    if (nomask_.n < (size_t)levels_[0].n_nodes) {
+      std::vector<uint8_t> z(levels_[0].n_nodes, 0);
+      if (nomask_.upload(z)) goto oom;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(levels_[0].xn.p, x, n * sizeof(double),
+                        on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  for (size_t l = 0; l + 1 < levels_.size(); l++)
+    launch_weight_restrict(levels_[l].R, levels_[l].xn.p, levels_[l + 1].xn.p, st_);
+  for (size_t l = 0; l < levels_.size(); l++) {
+    Level &L = levels_[l];
+    launch_nodal_scale(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.xn.p,
+                       L.d_lev.p, mat_.q, mat_.k0, mat_.penalty_type,
+                       l == 0 ? rho_gp_.p : nullptr, L.sg64.p, L.sg32.p, st_);
+  }
+  design_mode_ = 1;
+  have_design_ = true;
+  have_setup_ = false;
+  return 0;
+oom:
+  last_error = "set_design_nodal: device allocation failed";
+  return -1;
+}
+
+int Problem::design_size() const {
+  if (levels_.empty()) return 0;
+  return design_mode_ ? levels_[0].n_nodes : levels_[0].n_elem;
+}
+
+int Problem::level_design(int l, double *x) const {
+  if (design_mode_ != 1 || !have_design_) {
+    last_error = "level_design: no nodal design set";
+    return -1;
+  }
+  if (l < 0 || l >= (int)levels_.size()) {
+    last_error = "level_design: level out of range";
+    return -1;
+  }
+  const Level &L = levels_[l];
+  if (hipStreamSynchronize(st_) != hipSuccess ||
+      hipMemcpy(x, L.xn.p, L.n_nodes * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    last_error = "level_design: copy failed";
+    return -1;
+  }
   return 0;
 }
 
@@ -827,9 +932,11 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
   Tiles tl = L.tiles;
-  tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_;
+  tl.nodal = design_mode_ == 1;
+  tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_ && !tl.nodal;
   if (ev) (void)hipEventRecord(ev->first, st_);
-  launch_tile_apply<T>(tl, K0<T>(), scale<T>(l), x, mode, g, st_, nrhs, L.ndof);
+  launch_tile_apply<T>(tl, tl.nodal ? gtab<T>() : K0<T>(), tl.nodal ? gscale<T>(l) : scale<T>(l),
+                       x, mode, g, st_, nrhs, L.ndof);
   if (ev) (void)hipEventRecord(ev->second, st_);
 }
 
@@ -947,8 +1054,11 @@ int Problem::coarse_factor() {
   // inverse for a one-GEMV coarse solve
   Level &L = levels_.back();
   const int n = (int)L.ndof;
-  std::vector<double> s(L.n_elem);
-  HIPCHK(hipMemcpy(s.data(), L.s64.p, L.n_elem * sizeof(double), hipMemcpyDeviceToHost));
+  const bool nodal = design_mode_ == 1;
+  std::vector<double> s(nodal ? 8 * (size_t)L.n_elem : L.n_elem);
+  HIPCHK(hipMemcpy(s.data(), nodal ? L.sg64.p : L.s64.p, s.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  std::vector<double> Ke(576);
   const auto &dptr = L.forest->dep_ptr();
   const auto &dconn = L.forest->dep_conn();
   const auto &dw = L.forest->dep_weights();
@@ -965,6 +1075,17 @@ int Problem::coarse_factor() {
     for (int e = 0; e < L.n_elem; e++) {
       if (bseg[octs[e].block] != sg) continue;
       const double *K = &K0h_[576 * L.cls[e]];
+      double se = s[e];
+      if (nodal) {  // sum_g s_eg K_g, the element scale folded in
+        const double *Kg = &Kgh_[576 * 8 * (size_t)L.cls[e]];
+        for (int i = 0; i < 576; i++) {
+          double t = 0.0;
+          for (int g = 0; g < 8; g++) t += s[8 * (size_t)e + g] * Kg[576 * g + i];
+          Ke[i] = t;
+        }
+        K = Ke.data();
+        se = 1.0;
+      }
       std::vector<std::pair<int, double>> ex[8];
       for (int a = 0; a < 8; a++) {
         const int g = L.conn[8 * e + a];
@@ -977,7 +1098,7 @@ int Problem::coarse_factor() {
         for (auto &pa : ex[a])
           for (int bb = 0; bb < 8; bb++)
             for (auto &pb : ex[bb]) {
-              const double w = s[e] * pa.second * pb.second;
+              const double w = se * pa.second * pb.second;
               for (int i = 0; i < 3; i++) {
                 if ((L.bcmask[pa.first] >> i) & 1) continue;
                 for (int j = 0; j < 3; j++) {
@@ -1063,7 +1184,10 @@ int Problem::setup(const SolveOpts &o) {
   for (size_t l = 0; l + 1 < levels_.size(); l++) {
     Level &L = levels_[l];
     // Jacobi diagonal in both precisions
-    launch_elem_diag<double>(L.eb, K0d_.p, L.s64.p, L.v64.partial.p, st_);
+    if (design_mode_ == 1)
+      launch_elem_diag<double>(L.eb, G64_.p, L.sg64.p, L.v64.partial.p, st_, true);
+    else
+      launch_elem_diag<double>(L.eb, K0d_.p, L.s64.p, L.v64.partial.p, st_);
     AsmArgs a;
     a.y = L.v64.dinv.p;
     launch_assemble<double>(L.asmc, L.d_bcmask.p, L.v64.partial.p, ASM_DIAG, a, st_);
@@ -1436,6 +1560,11 @@ int Problem::compliance_grad(double *g, bool on_device) {
     return -1;
   }
   Level &L = levels_[0];
+  if (design_mode_ == 1) {
+    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+                        L.d_cls.p, G64_.p, u_.p, u_.p, egp_.p, st_);
+    return nodal_gather(nullptr, g, on_device);
+  }
   launch_elem_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
                      L.d_cls.p, K0d_.p, u_.p, energy_.p, st_);
   double *out = g;
@@ -1451,6 +1580,38 @@ int Problem::compliance_grad(double *g, bool on_device) {
                          st_);
   if (!on_device)
     HIPCHK(hipMemcpyAsync(g, out, L.n_elem * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+// Nodal design gradient: per element corner sum_g N_a(g) (ex_g - p'(rho_g)
+// 2^-level e_g) with e_g = psi_e^T K_g u_e in egp_ (the nodal branch of
+// TMROctConstitutive::addStressDVSens, src/topology/TMROctConstitutive.cpp:557-654),
+// gathered onto the independent nodes (hanging corners pass their share to
+// their parents, the transpose of the design dependent-node interpolation)
+int Problem::nodal_gather(const double *ex_gp, double *g, bool on_device) {
+  Level &L = levels_[0];
+  if (ensure_vecs(0, 0)) {
+    last_error = "nodal gradient: allocation failed";
+    return -1;
+  }
+  launch_nodal_grad(L.eb, L.d_lev.p, rho_gp_.p, mat_.q, mat_.penalty_type, egp_.p, ex_gp,
+                    L.v64.partial.p, st_);
+  AsmArgs a;
+  a.y = gtmp_.p;
+  launch_assemble<double>(L.asmc, nomask_.p, L.v64.partial.p, ASM_GATHER, a, st_);
+  double *out = g;
+  DevArray<double> tmp;
+  if (!on_device) {
+    if (tmp.alloc(L.n_nodes)) {
+      last_error = "nodal gradient: allocation failed";
+      return -1;
+    }
+    out = tmp.p;
+  }
+  launch_take_comp0(L.n_nodes, gtmp_.p, out, st_);
+  if (!on_device)
+    HIPCHK(hipMemcpyAsync(g, out, L.n_nodes * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   return 0;
 }
@@ -1482,7 +1643,8 @@ int Problem::stress_pnorm(double P, double *f) {
     return -1;
   }
   launch_stress_sum(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
-                    L.d_cls.p, L.d_lev.p, rho_.p, u_.p, sa, parts_.p, st_);
+                    L.d_cls.p, L.d_lev.p, rho_.p, design_mode_ == 1 ? rho_gp_.p : nullptr, u_.p,
+                    sa, parts_.p, st_);
   launch_finalize(nb, parts_.p, 4, sc_.p, st_);
   double h[SC_N];
   HIPCHK(hipMemcpyAsync(h, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
@@ -1501,8 +1663,9 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   // coef = S^(1/P - 1) into the scalar slot the gradient kernel reads
   const double coef = std::pow(last_stress_sum_, 1.0 / P - 1.0);
   HIPCHK(hipMemcpyAsync(sc_.p + SC_ZQ, &coef, sizeof(double), hipMemcpyHostToDevice, st_));
-  launch_stress_grad(L.eb, L.d_lev.p, L.d_cls.p, rho_.p, u_.p, sa, sc_.p, L.v64.partial.p,
-                     dfdrho_.p, st_);
+  const bool nodal = design_mode_ == 1;
+  launch_stress_grad(L.eb, L.d_lev.p, L.d_cls.p, rho_.p, nodal ? rho_gp_.p : nullptr, u_.p, sa,
+                     sc_.p, L.v64.partial.p, nodal ? exgp_.p : dfdrho_.p, st_);
   AsmArgs a;
   a.y = adjf_.p;
   launch_assemble<double>(L.asmc, L.d_bcmask.p, L.v64.partial.p, ASM_GATHER, a, st_);
@@ -1513,6 +1676,11 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   last_adjoint_residual = last_true_residual;
   last_true_residual = tr;
   if (r != 0) return r < 0 ? -1 : 1;
+  if (nodal) {
+    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+                        L.d_cls.p, G64_.p, u_.p, psi_.p, egp_.p, st_);
+    return nodal_gather(exgp_.p, g, on_device);
+  }
   launch_elem_energy2(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
                       L.d_cls.p, K0d_.p, u_.p, psi_.p, energy_.p, st_);
   double *out = g;
@@ -1675,7 +1843,9 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
       return ms / reps;
     };
-    const double tm = time_op(0), tv = time_op(1), t64 = time_op(2);
+    const double tv = time_op(1), t64 = time_op(2);
+    // the matrix-core form covers element densities only
+    const double tm = design_mode_ == 1 ? tv + 1.0 : time_op(0);
     if (tm < 0 || tv < 0 || t64 < 0) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

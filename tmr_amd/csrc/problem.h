@@ -140,6 +140,9 @@ struct Level {
   // stiffness scales
   DevArray<double> s64;
   DevArray<float> s32;
+  // nodal design: design value per independent node, scale per Gauss point
+  DevArray<double> xn, sg64;
+  DevArray<float> sg32;
   double lam_max = 0.0;
   LevelVecs<float> v32;
   LevelVecs<double> v64;
@@ -152,6 +155,14 @@ class Problem {
   int init(int nlevels, Forest **forests, const Material &mat, int nbcs, const char **names,
            const int *dof_masks);
   int set_design(const double *rho, int n, bool on_device);
+  // one design value per independent node of the finest level; coarse levels
+  // take the weighted restriction (R x) / (R 1)
+  int set_design_nodal(const double *x, int n, bool on_device);
+  // design mode (0 element densities, 1 nodal) and the length of a gradient
+  int design_mode() const { return design_mode_; }
+  int design_size() const;
+  // nodal design values of level l (nodal mode)
+  int level_design(int l, double *x) const;
   int setup(const SolveOpts &o);
   int solve(const double *f, double *u, bool on_device, const SolveOpts &o, SolveInfo *info);
   // nrhs load cases at once: f, u are [nrhs][ndof]; infos[nrhs].  Load case
@@ -186,7 +197,7 @@ class Problem {
   int kernel_timing_result(double *total_ms, int *count, int *precision_mask);
   double last_true_residual = 0.0;
   double last_adjoint_residual = 0.0;
-  std::string last_error;
+  mutable std::string last_error;
 
  private:
   int build_level(int l, int nbcs, const char **names, const int *masks);
@@ -201,6 +212,11 @@ class Problem {
   const T *K0();
   template <class T>
   const T *scale(int l);
+  template <class T>
+  const T *gtab();
+  template <class T>
+  const T *gscale(int l);
+  int nodal_gather(const double *ex_gp, double *g, bool on_device);
   template <class T>
   void apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrhs = 1);
   template <class T>
@@ -223,6 +239,13 @@ class Problem {
   DevArray<double> K0d_;
   DevArray<float> K0f_;
   DevArray<uint16_t> afrag_;  // bf16 hi/lo MFMA fragments of K0f_
+  // nodal design: per-class gradient tables, per-Gauss-point stiffness
+  std::vector<double> Kgh_;  // [class][8][576]
+  DevArray<double> G64_;
+  DevArray<float> G32_;
+  int design_mode_ = 0;      // 0 element densities, 1 nodal
+  DevArray<double> rho_gp_, egp_, exgp_, gtmp_;
+  DevArray<uint8_t> nomask_;
   DevArray<double> rho_;
   DevArray<double> Ainv_;
   DevArray<int> cb_col0_, cb_nb_;

@@ -74,6 +74,31 @@ class Problem:
         p, dev = _ptr(rho)
         check(lib().tmrx_problem_set_design(self.ptr, p, int(self.nelem), dev), "set_design")
 
+    def set_design_nodal(self, x):
+        """One design value per independent node of the finest forest
+        (TMROctConformTACSTopoCreator + TMRConformFilter); afterwards the
+        gradients are per node."""
+        if not _is_torch(x):
+            x = np.ascontiguousarray(x, dtype=np.double)
+        p, dev = _ptr(x)
+        n = len(x)
+        check(lib().tmrx_problem_set_design_nodal(self.ptr, p, int(n), dev), "set_design_nodal")
+
+    def design_info(self):
+        """(mode, gradient length): mode 0 element densities, 1 nodal."""
+        m, n = C.c_int(), C.c_int()
+        check(lib().tmrx_problem_design_info(self.ptr, C.byref(m), C.byref(n)), "design_info")
+        return m.value, n.value
+
+    def level_design(self, level):
+        """Nodal design values on multigrid level `level` (0 = finest)."""
+        n = self.forests[level].getNodeRange()[1]
+        x = np.zeros(n)
+        check(lib().tmrx_problem_level_design(self.ptr, int(level),
+                                              x.ctypes.data_as(C.POINTER(C.c_double))),
+              "level_design")
+        return x
+
     def setup(self, opts=None):
         opts = opts or default_opts()
         check(lib().tmrx_problem_setup(self.ptr, C.byref(opts)), "setup")
@@ -178,7 +203,7 @@ class Problem:
 
     def compliance_grad(self, g=None):
         if g is None:
-            g = np.zeros(self.nelem)
+            g = np.zeros(self.design_info()[1])
         p, dev = _ptr(g)
         check(lib().tmrx_problem_compliance_grad(self.ptr, p, dev), "compliance_grad")
         return g
@@ -191,7 +216,7 @@ class Problem:
     def stress_pnorm_grad(self, P=8.0, g=None, opts=None):
         opts = opts or default_opts()
         if g is None:
-            g = np.zeros(self.nelem)
+            g = np.zeros(self.design_info()[1])
         p, dev = _ptr(g)
         info = _lib.SolveInfo()
         r = lib().tmrx_problem_stress_pnorm_grad(self.ptr, float(P), p, dev, C.byref(opts),

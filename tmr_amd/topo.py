@@ -11,14 +11,20 @@ replaced by the native multigrid-preconditioned PCG (libtmrx.so):
 - ``OctConstitutive``      tmr/TMR.pyx:5201-5211
 - ``OctTopoCreator``       tmr/TMR.pyx:4238-4261, element densities from a
   filter forest (src/topology/TMR_TACSTopoCreator.cpp:55-371)
+- ``OctConformTopoCreator`` tmr/TMR.pyx:4289-4315, nodal design variables on
+  the analysis forest (src/topology/TMR_TACSTopoCreator.cpp:714-777)
 - ``createMg``             tmr/TMR.pyx:4317-4368, src/TMR_RefinementTools.cpp:37-107
 - ``LagrangeFilter``       tmr/TMR.pyx:4776-4815 (design space = filter nodes)
+- ``ConformFilter``        tmr/TMR.pyx:4817-4852, src/topology/TMRConformFilter.cpp
 - ``TopoProblem``          tmr/TMR.pyx:5371-6005, src/topology/TMRTopoProblem.cpp
 
 Design layout: the design vector lives on the independent nodes of the
-finest filter forest; element densities are ``rho = W x`` with W the
-element-centre weights, and every gradient is returned in design space as
-``W^T dF/drho`` (the filter transpose of TMRTopoProblem.cpp:1282).
+finest filter forest.  With ``OctTopoCreator`` element densities are
+``rho = W x`` with W the element-centre weights, and every gradient is
+returned in design space as ``W^T dF/drho`` (the filter transpose of
+TMRTopoProblem.cpp:1282).  With ``OctConformTopoCreator`` the design
+variables are the analysis forest's own nodal values, interpolated to every
+Gauss point (nodal mode of the native problem), and gradients are per node.
 
 Vectors are numpy arrays (TACS ``Vec`` / ParOpt ``PVec`` in the reference).
 """
@@ -134,16 +140,50 @@ class Assembler:
     TMROctTACSCreator::createTACS, TMR_TACSCreator.cpp:332-461, flattens into
     a TACSAssembler)."""
 
-    def __init__(self, forest, filt, bcs, props, weights):
+    def __init__(self, forest, filt, bcs, props, weights=None):
         self.forest, self.filter, self.bcs, self.props = forest, filt, bcs, props
-        self.rowp, self.cols, self.w, self.ndv = weights
-        self.nelem = len(self.rowp) - 1
         self.nnodes = forest.getNodeRange()[1]
         self.ndof = 3 * self.nnodes
         ext = forest.getElementExtents()
         self.volumes = np.prod(ext, axis=1)
-        self._rows = np.repeat(np.arange(self.nelem), np.diff(self.rowp))
+        self.nelem = len(self.volumes)
+        self.nodal = weights is None
+        if self.nodal:
+            # design variable numbers of element i are its nodes
+            # (TMROctConformTACSTopoCreator::createElements)
+            self.ndv = self.nnodes
+            self._mass_w = self._corner_gather(np.repeat(self.volumes[:, None] / 8.0, 8, axis=1))
+        else:
+            self.rowp, self.cols, self.w, self.ndv = weights
+            self._rows = np.repeat(np.arange(self.nelem), np.diff(self.rowp))
         self.problem = None  # the native problem, once createMg has built one
+
+    def _corner_gather(self, t):
+        """sum_e t[e, a] onto the independent nodes (hanging corners pass
+        their share to their parents with the hanging weights)."""
+        conn = np.asarray(self.forest.getMeshConn()).reshape(-1, 8)
+        ptr, dc, dw = self.forest.getDepNodeConn()
+        g = np.zeros(self.nnodes)
+        ind = conn >= 0
+        np.add.at(g, conn[ind], t[ind])
+        for e, a in zip(*np.nonzero(~ind)):
+            d = -conn[e, a] - 1
+            np.add.at(g, dc[ptr[d]:ptr[d + 1]], dw[ptr[d]:ptr[d + 1]] * t[e, a])
+        return g
+
+    def mass_weights(self):
+        """d(mass)/dx per design variable, density 1: element volumes through
+        the element-centre weights, or (nodal) sum_e V_e/8 over the corners at
+        each node -- the exact 2x2x2 integral of the trilinear density
+        (TMROctConstitutive::evalDensity, src/topology/TMROctConstitutive.cpp:217-248)."""
+        if self.nodal:
+            return self._mass_w
+        return self.weights_transpose(self.volumes)
+
+    def to_design_space(self, g):
+        """A native gradient in design space: W^T g for element densities,
+        unchanged for nodal designs (already per node)."""
+        return np.asarray(g) if self.nodal else self.weights_transpose(g)
 
     def getNumElements(self):
         return self.nelem
@@ -175,7 +215,9 @@ class Assembler:
         return lb, ub
 
     def element_densities(self, x):
-        """rho_e = sum_j w_ej x_j."""
+        """rho_e = sum_j w_ej x_j (element-density assemblers)."""
+        if self.nodal:
+            raise ValueError("a nodal-design assembler has no element densities")
         x = np.asarray(x, dtype=np.double)
         if x.shape != (self.ndv,):
             raise ValueError("design vector has %s entries, expected %d" % (x.shape, self.ndv))
@@ -221,13 +263,38 @@ class OctTopoCreator:
 
 
 class OctConformTopoCreator:
-    """Nodal design variables with per-Gauss-point SIMP
-    (TMROctConformTACSTopoCreator) -- not built: the native operator scales
-    whole elements by one density."""
+    """Nodal design variables on the analysis forest (TMROctConformTACSTopoCreator,
+    src/topology/TMR_TACSTopoCreator.cpp:714-777): element i's design
+    variables are its 8 nodes; the density at each Gauss point is their
+    trilinear interpolation and the penalty applies per point.  The filter is
+    the forest itself (``order < 0`` or the forest's own order); another
+    order is not built.  A subclass may define
+    ``createElement(order, octant, index, filter)`` returning what names the
+    StiffnessProperties, as with OctTopoCreator."""
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("OctConformTopoCreator: nodal (per-Gauss-point) densities are "
-                                  "not supported; use OctTopoCreator (element densities)")
+    def __init__(self, bcs, forest, design_vars_per_node=1, order=-1, interp=None, props=None,
+                 *args, **kwargs):
+        if design_vars_per_node != 1:
+            raise NotImplementedError("one design variable per node only")
+        if order >= 0 and order != forest.getMeshOrder():
+            raise NotImplementedError("a filter order other than the forest's is not supported")
+        self.bcs, self.forest, self.props = bcs, forest, props
+        forest.createNodes()
+
+    def getFilter(self):
+        return self.forest
+
+    def createTACS(self, forest, ordering=NATURAL_ORDER):
+        forest.createNodes()
+        props = self.props
+        if hasattr(self, "createElement"):
+            octs = forest.getOctants()
+            conn = np.asarray(forest.getMeshConn()).reshape(-1, 8)
+            props = _props_of(self.createElement(forest.getMeshOrder(), octs[0],
+                                                 list(conn[0]), forest))
+        if props is None:
+            raise ValueError("no StiffnessProperties: pass props= or define createElement")
+        return Assembler(forest, forest, self.bcs, props, None)
 
 
 class Mg:
@@ -284,16 +351,19 @@ class TopoFilter:
         return x
 
     def applyFilter(self, vec_in, vec_out=None):
-        """Element densities of the finest level from a design vector."""
-        rho = self.assemblers[0].element_densities(vec_in)
+        """Element densities of the finest level from a design vector (the
+        design vector itself for a nodal design)."""
+        a = self.assemblers[0]
+        rho = np.array(vec_in, dtype=np.double) if a.nodal else a.element_densities(vec_in)
         if vec_out is not None:
             vec_out[:] = rho
             return vec_out
         return rho
 
     def addValues(self, g_rho):
-        """Filter transpose of an element sensitivity."""
-        return self.assemblers[0].weights_transpose(g_rho)
+        """Filter transpose of an element sensitivity (identity for a nodal
+        design)."""
+        return self.assemblers[0].to_design_space(g_rho)
 
 
 class LagrangeFilter(TopoFilter):
@@ -302,8 +372,17 @@ class LagrangeFilter(TopoFilter):
 
 
 class ConformFilter(TopoFilter):
+    """Nodal design variables on the analysis forests (TMRConformFilter,
+    src/topology/TMRConformFilter.cpp): the design vector of level k+1 is the
+    weighted restriction (P^T x_k) / (P^T 1) of level k's, done natively by
+    the problem when the design is set."""
+
     def __init__(self, assemblers, filters):
-        raise NotImplementedError("ConformFilter: nodal densities are not supported")
+        if len(assemblers) != len(filters):
+            raise ValueError("ConformFilter must have equal number of objects in lists")
+        if not all(getattr(a, "nodal", False) for a in assemblers):
+            raise ValueError("ConformFilter needs OctConformTopoCreator assemblers")
+        super().__init__(assemblers, filters)
 
 
 # ---------------------------------------------------------------------------
@@ -313,8 +392,9 @@ class Compliance:
 
 
 class StructuralMass:
-    """sum_e V_e rho_e density (TMROctConstitutive::evalDensity with an
-    element-constant rho, TMROctConstitutive.cpp:217-279)."""
+    """Integral of rho times the material density (TMROctConstitutive::evalDensity,
+    TMROctConstitutive.cpp:217-279): sum_e V_e rho_e for element densities,
+    the exact integral of the trilinear nodal density otherwise."""
 
 
 class PNormFailure:
@@ -441,21 +521,26 @@ class TopoProblem:
         if isinstance(func, Compliance):
             return float(np.dot(self.vars[case], self.forces[case]))
         if isinstance(func, StructuralMass):
-            return float(np.dot(self.assembler.volumes, self.rho)) * \
-                self.assembler.props.props.getDensity()
+            a = self.assembler
+            if a.nodal:
+                return float(np.dot(a.mass_weights(), self.x_design)) * a.props.props.getDensity()
+            return float(np.dot(a.volumes, self.rho)) * a.props.props.getDensity()
         if isinstance(func, PNormFailure):
             self._restore(case)
             return p.stress_pnorm(func.P)
         raise TypeError("unsupported function %r" % (func,))
 
     def _eval_func_grad(self, func, case):
-        """d func / d rho (element sensitivities) for load case `case`."""
+        """Native gradient of func for load case `case`: per element
+        (element densities) or per node (nodal design)."""
         p = self.problem
         if isinstance(func, Compliance):
             self._restore(case)
             return p.compliance_grad()
         if isinstance(func, StructuralMass):
-            return self.assembler.volumes * self.assembler.props.props.getDensity()
+            a = self.assembler
+            d = a.props.props.getDensity()
+            return a.mass_weights() * d if a.nodal else a.volumes * d
         if isinstance(func, PNormFailure):
             self._restore(case)
             return p.stress_pnorm_grad(func.P, opts=self._opts())
@@ -474,8 +559,13 @@ class TopoProblem:
     def setDesignVars(self, x):
         """TMRTopoProblem::setDesignVars (TMRTopoProblem.cpp:955-962)."""
         self.filter.setDesignVars(x)
-        self.rho = self.assembler.element_densities(x)
-        self.problem.set_design(self.rho)
+        self.x_design = np.array(x, dtype=np.double)
+        if self.assembler.nodal:
+            self.rho = None
+            self.problem.set_design_nodal(self.x_design)
+        else:
+            self.rho = self.assembler.element_densities(x)
+            self.problem.set_design(self.rho)
         self._current = None
 
     def evalObjCon(self, ncon, x):
@@ -534,7 +624,7 @@ class TopoProblem:
         g[:] = 0.0
         if self.objgradcb is not None:
             self.objgradcb(self.filter, self.mg, g)
-        grho = np.zeros(a.nelem)
+        grho = np.zeros(a.ndv if a.nodal else a.nelem)
         for i, f in enumerate(self.forces):
             if f is None or self.obj_weights[i] == 0.0:
                 continue
@@ -542,14 +632,14 @@ class TopoProblem:
             if fn is None:
                 continue
             grho += self.obj_weights[i] * self._eval_func_grad(fn, i)
-        g += a.weights_transpose(grho)
+        g += a.to_design_space(grho)
         k = 0
         for v in self.lin_vecs:
             A[k][:] = v
             k += 1
         for case, funcs, offs, scales in self.cons:
             for fn, s in zip(funcs, scales):
-                A[k][:] = s * a.weights_transpose(self._eval_func_grad(fn, case))
+                A[k][:] = s * a.to_design_space(self._eval_func_grad(fn, case))
                 k += 1
         for ncb, _, cgf in self.concb:
             cgf(self.filter, self.mg, A[k:k + ncb])
