@@ -109,6 +109,11 @@ struct Tiles {
 // Gauss points of the full-block element (G[24 g + 3 a + d], 192 entries),
 // then lambda and mu times the Gauss weight |J| (= volume / 8), then 2 pads.
 constexpr int NODAL_TABLE = 196;
+// Per stiffness class, the element stiffness in the reflection-symmetric
+// basis: 8 blocks of 3 x 3 (row-major), block s coupling the Walsh-Hadamard
+// coefficients X_0[s^1], X_1[s^2], X_2[s^4], scaled by 1/64.  The
+// element-density tile apply takes this table in place of K0.
+constexpr int REFL_TABLE = 72;
 
 
 struct AsmCSR {
@@ -146,7 +151,9 @@ struct AsmArgs {
   double *partials = nullptr;
 };
 
-// single-pass apply with fused epilogue (modes as AsmMode, except ASM_DIAG)
+// single-pass apply with fused epilogue (modes as AsmMode, except ASM_DIAG);
+// K0 is the REFL_TABLE table (element densities) or the NODAL_TABLE table
+// (tl.nodal) on the vector ALUs, unused by the matrix-core form
 template <class T>
 void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
                        const AsmArgs &args, hipStream_t st, int nrhs = 1, int64_t vs = 0);

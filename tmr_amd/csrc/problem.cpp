@@ -88,6 +88,10 @@ const float *Problem::scale<float>(int l) { return levels_[l].s32.p; }
 template <>
 const double *Problem::scale<double>(int l) { return levels_[l].s64.p; }
 template <>
+const float *Problem::rtab<float>() { return R32_.p; }
+template <>
+const double *Problem::rtab<double>() { return R64_.p; }
+template <>
 const float *Problem::gtab<float>() { return G32_.p; }
 template <>
 const double *Problem::gtab<double>() { return G64_.p; }
@@ -154,6 +158,42 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
   K0h_.assign(576 * (size_t)nclasses_, 0.0);
   for (int c = 0; c < nclasses_; c++)
     hex8_box_stiffness(ext[c][0], ext[c][1], ext[c][2], C, &K0h_[576 * c]);
+  // K0 in the reflection-symmetric basis: with H the 8-point Walsh-Hadamard
+  // matrix over the corners, (H K0_cc' H)[t][t'] vanishes unless
+  // t ^ (1 << c) == t' ^ (1 << c'); the surviving entries form one 3x3 block
+  // per s = t ^ (1 << c).  Checked here: a box element with an isotropic
+  // material is symmetric under the three coordinate reflections.
+  {
+    std::vector<double> rt(REFL_TABLE * (size_t)nclasses_, 0.0);
+    auto Hs = [](int t, int a) { return (__builtin_popcount(t & a) & 1) ? -1.0 : 1.0; };
+    for (int cl = 0; cl < nclasses_; cl++) {
+      const double *K = &K0h_[576 * (size_t)cl];
+      double kmax = 0.0, off = 0.0;
+      for (int i = 0; i < 576; i++) kmax = std::max(kmax, std::fabs(K[i]));
+      for (int c = 0; c < 3; c++)
+        for (int cp = 0; cp < 3; cp++)
+          for (int t = 0; t < 8; t++)
+            for (int tp = 0; tp < 8; tp++) {
+              double v = 0.0;
+              for (int a = 0; a < 8; a++)
+                for (int b = 0; b < 8; b++) v += Hs(t, a) * K[24 * (3 * a + c) + 3 * b + cp] * Hs(tp, b);
+              const int sv = t ^ (1 << c);
+              if (sv == (tp ^ (1 << cp)))
+                rt[REFL_TABLE * (size_t)cl + 9 * sv + 3 * c + cp] = v / 64.0;
+              else
+                off = std::max(off, std::fabs(v));
+            }
+      if (off > 1e-12 * 64.0 * kmax) {
+        last_error = "problem: element stiffness is not symmetric under the coordinate reflections";
+        return -1;
+      }
+    }
+    std::vector<float> rtf(rt.begin(), rt.end());
+    if (R64_.upload(rt) || R32_.upload(rtf)) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
+  }
   // nodal design: Gauss-point stiffness (host, coarse factor) and gradient
   // tables with lambda |J|, mu |J| (device)
   {
@@ -935,7 +975,7 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   tl.nodal = design_mode_ == 1;
   tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_ && !tl.nodal;
   if (ev) (void)hipEventRecord(ev->first, st_);
-  launch_tile_apply<T>(tl, tl.nodal ? gtab<T>() : K0<T>(), tl.nodal ? gscale<T>(l) : scale<T>(l),
+  launch_tile_apply<T>(tl, tl.nodal ? gtab<T>() : rtab<T>(), tl.nodal ? gscale<T>(l) : scale<T>(l),
                        x, mode, g, st_, nrhs, L.ndof);
   if (ev) (void)hipEventRecord(ev->second, st_);
 }

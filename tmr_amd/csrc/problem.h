@@ -215,6 +215,8 @@ class Problem {
   template <class T>
   const T *gtab();
   template <class T>
+  const T *rtab();
+  template <class T>
   const T *gscale(int l);
   int nodal_gather(const double *ex_gp, double *g, bool on_device);
   template <class T>
@@ -240,6 +242,8 @@ class Problem {
   DevArray<float> K0f_;
   DevArray<uint16_t> afrag_;  // bf16 hi/lo MFMA fragments of K0f_
   // nodal design: per-class gradient tables, per-Gauss-point stiffness
+  DevArray<double> R64_;  // reflection-block tables of K0, REFL_TABLE per class
+  DevArray<float> R32_;
   std::vector<double> Kgh_;  // [class][8][576]
   DevArray<double> G64_;
   DevArray<float> G32_;
