@@ -28,6 +28,7 @@ CHEB_BYTES_PER_NODE = 6 * 3 * 4 + 1
 CHEB_BYTES_PER_ELEM = 4 + 8 * 4
 FLOPS_PER_ELEM = 1152.0      # 2 * 24 * 24
 TIME_CHEB = 1 << 2           # launch-kind mask: Chebyshev update
+TILE_CANDIDATES = (256, 384, 512, 768, 1024)  # owner-tile sizes the autotuner tries
 # HBM traffic of the finest-level Chebyshev launch on the default workload
 # (level 7), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per
 # dispatch, mean over the launches at that level), keyed by (FP32 form, owner
@@ -93,7 +94,7 @@ def bench_multirhs(args):
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
     if not args.no_autotune:
-        prob.autotune((256, 512, 1024), reps=3)
+        prob.autotune(TILE_CANDIDATES, reps=3)
     rng = np.random.default_rng(0)
     F = np.stack([prob.traction_load("load", rng.standard_normal(3)) for _ in range(nrhs)])
     Fd = torch.from_numpy(F).cuda()
@@ -145,7 +146,7 @@ def bench_batch(args):
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
     if not args.no_autotune:
-        prob.autotune((256, 512, 1024), reps=3)
+        prob.autotune(TILE_CANDIDATES, reps=3)
     f = torch.from_numpy(prob.traction_load("load", [0.0, 0.0, -1.0])).cuda()
     u = torch.zeros_like(f)
     opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=500, no_graph=int(args.no_graph))
@@ -245,7 +246,7 @@ def main():
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        tune = prob.autotune((256, 512, 1024), reps=3)
+        tune = prob.autotune(TILE_CANDIDATES, reps=3)
         log("autotuned owner tiles: %d nodes (ms per 4 FP32 + 1 FP64 products: %s) in %.1f s"
             % (tune[0], tune[1], time.perf_counter() - t0))
 
@@ -314,8 +315,8 @@ def main():
         "dtype": "f64 PCG + f32 multigrid",
         "data": "synthetic (seeded smoothed densities, unit traction)",
         "config": {"workload": "box cantilever 256x128x128 hex8 (2 blocks, level %d), SIMP q=3 "
-                               "k0=1e-6, 6-level MG (Chebyshev deg 2), FP64 flexible PCG to 1e-8"
-                               % args.level,
+                               "k0=1e-6, 6-level MG (Chebyshev deg 2 on [lambda/6, 1.1 lambda]), "
+                               "FP64 flexible PCG to 1e-8" % args.level,
                    "elements": prob.nelem, "dofs": ndof, "mg_levels": prob.nlevels},
         "mdof_iter_per_s": mdof,
         "pcg_iters": mixed["iters"],

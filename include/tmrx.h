@@ -155,7 +155,8 @@ typedef struct {
 } tmrx_solve_info;
 
 /* Defaults: rtol 1e-8, FP32 V-cycle, Chebyshev degree 2 on
-   [1/30, 1.1] lambda_max, 200 iterations. */
+   [1/6, 1.1] lambda_max (TMR's smoother: degree 3 on [1/30, 1.1],
+   src/TMR_RefinementTools.cpp:46-47), 200 iterations. */
 void tmrx_solve_opts_default(tmrx_solve_opts *o);
 
 /* Replaces TopOptUtils.createTopoProblem's assembler/multigrid construction

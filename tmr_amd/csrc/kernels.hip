@@ -1233,6 +1233,43 @@ __device__ __forceinline__ void nodal_elem(const T *__restrict__ Gt, const T *__
   }
 }
 
+// LDS values per node of the vector-ALU tile apply.  Three: padding FP32
+// nodes to 16 bytes (one 128-bit access per corner) costs more in tiles per
+// CU than it saves in LDS instructions.
+template <class T>
+constexpr int tile_stride() {
+  return 3;
+}
+
+template <class T, int S>
+__device__ __forceinline__ void lds_get3(const T *p, T &a, T &b, T &c) {
+  if constexpr (S == 4 && sizeof(T) == 4) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    a = v.x;
+    b = v.y;
+    c = v.z;
+  } else {
+    a = p[0];
+    b = p[1];
+    c = p[2];
+  }
+}
+
+template <class T, int S>
+__device__ __forceinline__ void lds_add3(T *p, T a, T b, T c) {
+  if constexpr (S == 4 && sizeof(T) == 4) {
+    float4 v = *reinterpret_cast<float4 *>(p);
+    v.x += a;
+    v.y += b;
+    v.z += c;
+    *reinterpret_cast<float4 *>(p) = v;
+  } else {
+    p[0] += a;
+    p[1] += b;
+    p[2] += c;
+  }
+}
+
 template <class T, int MODE, bool SINGLE_CLASS, bool NODAL>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
@@ -1244,6 +1281,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
     const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
     const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials, int64_t vs) {
+  constexpr int S = tile_stride<T>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -1262,19 +1300,23 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   const int nslot = nown + next;
   const int nacc = nown + nacc_dep[t];
   T *xs = reinterpret_cast<T *>(smem_raw);
-  T *ys = xs + 3 * ((nslot + 7) & ~7);
-  tile_stage<T, 3>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
-  __syncthreads();
+  T *ys = xs + S * ((nslot + 7) & ~7);
   const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
+  // the first round's records go out before the staging loads
+  uint2 w0 = make_uint2(0, 0), w1 = w0, w2 = w0;
+  if (tid < nel) {
+    w0 = el[3 * (size_t)(q0 + tid)];
+    w1 = el[3 * (size_t)(q0 + tid) + 1];
+    w2 = el[3 * (size_t)(q0 + tid) + 2];
+  }
+  tile_stage<T, S>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  __syncthreads();
   for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
     const int k = r0 + tid;
     T ye[24];
     int lc[8];
     int mask = 0, col = -1;
     if (k < nel) {
-      const uint2 w0 = el[3 * (size_t)(q0 + k)];
-      const uint2 w1 = el[3 * (size_t)(q0 + k) + 1];
-      const uint2 w2 = el[3 * (size_t)(q0 + k) + 2];
       lc[0] = w0.x & 0xffff; lc[1] = w0.x >> 16;
       lc[2] = w0.y & 0xffff; lc[3] = w0.y >> 16;
       lc[4] = w1.x & 0xffff; lc[5] = w1.x >> 16;
@@ -1285,16 +1327,18 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       const int cls = SINGLE_CLASS ? 0 : (int)((w2.y >> 16) & 0xff);
       T xe[24];
 #pragma unroll
-      for (int a = 0; a < 8; a++) {
-        xe[3 * a] = xs[3 * lc[a]];
-        xe[3 * a + 1] = xs[3 * lc[a] + 1];
-        xe[3 * a + 2] = xs[3 * lc[a] + 2];
-      }
+      for (int a = 0; a < 8; a++) lds_get3<T, S>(xs + S * lc[a], xe[3 * a], xe[3 * a + 1], xe[3 * a + 2]);
       if constexpr (NODAL) {
         nodal_elem<T>(K0 + NODAL_TABLE * cls, s_elem + 8 * (size_t)elem, xe, mask, ye);
       } else {
         refl_elem<T>(K0 + REFL_TABLE * cls, s_elem[elem], xe, ye);
       }
+    }
+    // next round's records load while this round accumulates
+    if (k + TILE_THREADS < nel) {
+      w0 = el[3 * (size_t)(q0 + k + TILE_THREADS)];
+      w1 = el[3 * (size_t)(q0 + k + TILE_THREADS) + 1];
+      w2 = el[3 * (size_t)(q0 + k + TILE_THREADS) + 2];
     }
     // elements are sorted by colour: this round only spans the colours of
     // its first and last element, and most waves hold a single colour
@@ -1303,18 +1347,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     for (int c = c_lo; c <= c_hi; c++) {
       if (col == c) {
 #pragma unroll
-        for (int a = 0; a < 8; a++) {
-          if (mask & (1 << a)) {
-            ys[3 * lc[a]] += ye[3 * a];
-            ys[3 * lc[a] + 1] += ye[3 * a + 1];
-            ys[3 * lc[a] + 2] += ye[3 * a + 2];
-          }
-        }
+        for (int a = 0; a < 8; a++)
+          if (mask & (1 << a)) lds_add3<T, S>(ys + S * lc[a], ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
       }
       __syncthreads();
     }
   }
-  tile_finish<T, MODE, 3>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
+  tile_finish<T, MODE, S>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
                           x_out, d, dinv, ca, cc, partials, red);
 }
 
@@ -1572,7 +1611,8 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                        const AsmArgs &g, hipStream_t st, int nrhs, int64_t vs) {
   if (tl.n_tiles == 0) return;
   const dim3 grid(tl.n_tiles, nrhs);
-  const size_t sh = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(T);
+  const size_t sh =
+      (size_t)tile_stride<T>() * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(T);
   const T *b = (const T *)g.b;
   T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
   const T *dinv = (const T *)g.dinv;

@@ -164,7 +164,7 @@ class Problem:
                                                 C.byref(cv), C.byref(rr)), "component_info")
         return it.value, bool(cv.value), rr.value
 
-    def autotune(self, tile_sizes=(256, 512, 1024), reps=5):
+    def autotune(self, tile_sizes=(256, 384, 512, 768, 1024), reps=5):
         """Pick the owner-tile size with the fastest finest-level operator;
         returns (chosen size, {size: ms})."""
         n = len(tile_sizes)
