@@ -103,6 +103,8 @@ struct Tiles {
   // nodal design: the element stiffness is sum_g s_eg K_g over the 8 Gauss
   // points (K0 holds the gradient tables, s_elem 8 scales per element)
   bool nodal = false;
+  // threads per tile of the vector-ALU form (256 or 512)
+  int threads = TILE_THREADS;
 };
 
 // Per stiffness class, the nodal-design element tables: dN_a/dx_d at the 8

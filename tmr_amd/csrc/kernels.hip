@@ -1012,7 +1012,7 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
 // hanging-node interpolation, clamped dofs zero) and zero the accumulators;
 // S floats per node (S = 4 pads each node to 16 bytes with a zero)
-template <class T, int S, int NT = TILE_THREADS>
+template <class T, int S, int NT = TILE_THREADS, int SY = S>
 __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
                                            int nacc, const int *__restrict__ dep_ptr,
                                            const int *__restrict__ dep_conn,
@@ -1021,19 +1021,20 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
                                            const T *__restrict__ x, T *xs, T *ys) {
   const int tid = threadIdx.x;
   const int n3 = 3 * nown;
-  for (int base = 0; base < n3; base += 4 * NT) {  // four loads in flight per thread
-    T v[4];
-    uint8_t mk[4];
+  // owned values: eight loads in flight per thread, issued before any use
+  // (indices clamped into the tile's range, so the loads are unconditional)
+  constexpr int VO = 8;
+  for (int base = 0; base < n3; base += VO * NT) {
+    T v[VO];
+    uint8_t mk[VO];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int i = base + k * NT + tid;
-      if (i < n3) {
-        v[k] = x[3 * (size_t)n0 + i];
-        mk[k] = bcmask[n0 + i / 3];
-      }
+    for (int k = 0; k < VO; k++) {
+      const int i = min(base + k * NT + tid, n3 - 1);
+      v[k] = x[3 * (size_t)n0 + i];
+      mk[k] = bcmask[n0 + i / 3];
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < VO; k++) {
       const int i = base + k * NT + tid;
       if (i < n3) {
         const int j = i / 3, c = i - 3 * j;
@@ -1041,17 +1042,44 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
       }
     }
   }
-  for (int i = tid; i < next; i += NT) {
-    T v[3];
-    load_node<T>(ext_l[i], x, dep_ptr, dep_conn, dep_w, bcmask, v);
-    const int q = S * tile_pos(nown + i);
-    xs[q] = v[0];
-    xs[q + 1] = v[1];
-    xs[q + 2] = v[2];
+  // halo nodes: two per thread in flight; independent nodes load directly,
+  // hanging ones are interpolated from their parents afterwards
+  constexpr int VH = 2;
+  for (int i0 = tid; i0 < next; i0 += VH * NT) {
+    int g[VH];
+#pragma unroll
+    for (int k = 0; k < VH; k++) g[k] = ext_l[min(i0 + k * NT, next - 1)];
+    T v[VH][3];
+    uint8_t mk[VH];
+#pragma unroll
+    for (int k = 0; k < VH; k++) {
+      const int gg = g[k] >= 0 ? g[k] : 0;
+      v[k][0] = x[3 * (size_t)gg];
+      v[k][1] = x[3 * (size_t)gg + 1];
+      v[k][2] = x[3 * (size_t)gg + 2];
+      mk[k] = bcmask[gg];
+    }
+#pragma unroll
+    for (int k = 0; k < VH; k++) {
+      const int i = i0 + k * NT;
+      if (i < next) {
+        if (g[k] < 0) {
+          load_node<T>(g[k], x, dep_ptr, dep_conn, dep_w, bcmask, v[k]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+            if ((mk[k] >> c) & 1) v[k][c] = T(0);
+        }
+        const int q = S * tile_pos(nown + i);
+        xs[q] = v[k][0];
+        xs[q + 1] = v[k][1];
+        xs[q + 2] = v[k][2];
+      }
+    }
   }
   if (S == 4)
     for (int i = tid; i < ((nown + next + 7) & ~7); i += NT) xs[4 * i + 3] = T(0);
-  for (int i = tid; i < S * ((nacc + 7) & ~7); i += NT) ys[i] = T(0);
+  for (int i = tid; i < SY * ((nacc + 7) & ~7); i += NT) ys[i] = T(0);
 }
 
 // owned rows: add the hanging-node sums, then the mode's epilogue
@@ -1062,7 +1090,7 @@ __device__ __forceinline__ float plane_get(uint2 w, int c) {
   return __uint_as_float(u);
 }
 
-template <class T, int MODE, int S, int NT = TILE_THREADS, bool SPLIT = false>
+template <class T, int MODE, int S, int NT = TILE_THREADS, bool SPLIT = false, int SX = S>
 __device__ __forceinline__ void tile_finish(
     int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
     const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
@@ -1075,13 +1103,14 @@ __device__ __forceinline__ void tile_finish(
       return (plane_get(ph[pos], c) + plane_get(ph[pos + pstride], c)) +
              plane_get(ph[pos + 2 * pstride], c);
     } else {
-      return xs[S * pos + c];
+      return xs[SX * pos + c];
     }
   };
   double dot = 0.0;
   for (int i = threadIdx.x; i < nown; i += NT) {
     const int node = n0 + i;
-    const int pi = S * tile_pos(i);
+    const int ti = tile_pos(i);
+    const int pi = S * ti;
     // the epilogue's vector loads go out before the hanging-node sums
     T bv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)}, wv[3] = {T(0), T(0), T(0)};
     T xv[3] = {T(0), T(0), T(0)};
@@ -1111,7 +1140,7 @@ __device__ __forceinline__ void tile_finish(
       if (MODE == ASM_STORE) {
         y[k] = bc ? xv[c] : a[c];
       } else if (MODE == ASM_DOT) {
-        const T xx = bc ? xv[c] : xat(pi / S, c);
+        const T xx = bc ? xv[c] : xat(ti, c);
         const T v = bc ? xx : a[c];
         y[k] = v;
         dot += (double)xx * (double)v;
@@ -1125,7 +1154,7 @@ __device__ __forceinline__ void tile_finish(
           const T r = bv[c] - a[c];
           const T dn = (T)ca * dv[c] + (T)cc * wv[c] * r;
           d[k] = dn;
-          x_out[k] = xat(pi / S, c) + dn;
+          x_out[k] = xat(ti, c) + dn;
         }
       }
     }
@@ -1240,6 +1269,12 @@ template <class T>
 constexpr int tile_stride() {
   return 3;
 }
+// x slots: 3 values per node as well; 16-byte FP32 slots (one 128-bit read
+// per corner) measured slower for the tiles per CU they cost
+template <class T>
+constexpr int tile_xstride() {
+  return 3;
+}
 
 template <class T, int S>
 __device__ __forceinline__ void lds_get3(const T *p, T &a, T &b, T &c) {
@@ -1270,8 +1305,8 @@ __device__ __forceinline__ void lds_add3(T *p, T a, T b, T c) {
   }
 }
 
-template <class T, int MODE, bool SINGLE_CLASS, bool NODAL>
-__global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
+template <class T, int MODE, bool SINGLE_CLASS, bool NODAL, int NT>
+__global__ __launch_bounds__(NT) void k_tile_apply(
     const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
     const int *__restrict__ nacc_dep, const int *__restrict__ el_ptr,
     const uint2 *__restrict__ el, int ncolors, const int *__restrict__ comb_ptr,
@@ -1282,8 +1317,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
     const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials, int64_t vs) {
   constexpr int S = tile_stride<T>();
+  constexpr int SX = tile_xstride<T>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double red[TILE_THREADS / 64];
+  __shared__ double red[NT / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   {  // right-hand side blockIdx.y of a batch of vectors, stride vs
@@ -1300,7 +1336,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
   const int nslot = nown + next;
   const int nacc = nown + nacc_dep[t];
   T *xs = reinterpret_cast<T *>(smem_raw);
-  T *ys = xs + S * ((nslot + 7) & ~7);
+  T *ys = xs + SX * ((nslot + 7) & ~7);
   const int q0 = el_ptr[t], nel = el_ptr[t + 1] - q0;
   // the first round's records go out before the staging loads
   uint2 w0 = make_uint2(0, 0), w1 = w0, w2 = w0;
@@ -1309,9 +1345,10 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
     w1 = el[3 * (size_t)(q0 + tid) + 1];
     w2 = el[3 * (size_t)(q0 + tid) + 2];
   }
-  tile_stage<T, S>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w, bcmask, x, xs, ys);
+  tile_stage<T, SX, NT, S>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w,
+                                     bcmask, x, xs, ys);
   __syncthreads();
-  for (int r0 = 0; r0 < nel; r0 += TILE_THREADS) {
+  for (int r0 = 0; r0 < nel; r0 += NT) {
     const int k = r0 + tid;
     T ye[24];
     int lc[8];
@@ -1327,7 +1364,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       const int cls = SINGLE_CLASS ? 0 : (int)((w2.y >> 16) & 0xff);
       T xe[24];
 #pragma unroll
-      for (int a = 0; a < 8; a++) lds_get3<T, S>(xs + S * lc[a], xe[3 * a], xe[3 * a + 1], xe[3 * a + 2]);
+      for (int a = 0; a < 8; a++)
+        lds_get3<T, SX>(xs + SX * lc[a], xe[3 * a], xe[3 * a + 1], xe[3 * a + 2]);
       if constexpr (NODAL) {
         nodal_elem<T>(K0 + NODAL_TABLE * cls, s_elem + 8 * (size_t)elem, xe, mask, ye);
       } else {
@@ -1335,15 +1373,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       }
     }
     // next round's records load while this round accumulates
-    if (k + TILE_THREADS < nel) {
-      w0 = el[3 * (size_t)(q0 + k + TILE_THREADS)];
-      w1 = el[3 * (size_t)(q0 + k + TILE_THREADS) + 1];
-      w2 = el[3 * (size_t)(q0 + k + TILE_THREADS) + 2];
+    if (k + NT < nel) {
+      w0 = el[3 * (size_t)(q0 + k + NT)];
+      w1 = el[3 * (size_t)(q0 + k + NT) + 1];
+      w2 = el[3 * (size_t)(q0 + k + NT) + 2];
     }
     // elements are sorted by colour: this round only spans the colours of
     // its first and last element, and most waves hold a single colour
     const int c_lo = (int)((el[3 * (size_t)(q0 + r0) + 2].y >> 8) & 0xff);
-    const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + TILE_THREADS, nel) - 1) + 2].y >> 8) & 0xff);
+    const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + NT, nel) - 1) + 2].y >> 8) & 0xff);
     for (int c = c_lo; c <= c_hi; c++) {
       if (col == c) {
 #pragma unroll
@@ -1353,8 +1391,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply(
       __syncthreads();
     }
   }
-  tile_finish<T, MODE, S>(t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, xs, ys, x, b, y,
-                          x_out, d, dinv, ca, cc, partials, red);
+  tile_finish<T, MODE, S, NT, false, SX>(t, n0, nown, comb_ptr, comb_slot, comb_w,
+                                                   bcmask, xs, ys, x, b, y, x_out, d, dinv, ca,
+                                                   cc, partials, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1611,8 +1650,8 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                        const AsmArgs &g, hipStream_t st, int nrhs, int64_t vs) {
   if (tl.n_tiles == 0) return;
   const dim3 grid(tl.n_tiles, nrhs);
-  const size_t sh =
-      (size_t)tile_stride<T>() * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(T);
+  const size_t sh = ((size_t)tile_xstride<T>() * round8(tl.max_slots) +
+                     (size_t)tile_stride<T>() * round8(tl.max_acc)) * sizeof(T);
   const T *b = (const T *)g.b;
   T *y = (T *)g.y, *xo = (T *)g.x_out, *d = (T *)g.d;
   const T *dinv = (const T *)g.dinv;
@@ -1645,19 +1684,24 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
       return;
     }
   }
-#define TMRX_TILE(M, S)                                                                         \
-  if (tl.nodal)                                                                                 \
-    hipLaunchKernelGGL((k_tile_apply<T, M, S, true>), grid, dim3(TILE_THREADS), sh, st,          \
-                       tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                    \
-                       reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,         \
-                       tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,   \
-                       K0, s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials, vs);           \
-  else                                                                                          \
-  hipLaunchKernelGGL((k_tile_apply<T, M, S, false>), grid, dim3(TILE_THREADS), sh, st,           \
-                     tl.node0, tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                      \
-                     reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,           \
-                     tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0, \
+#define TMRX_TILE_NT(M, S, N, NTH)                                                              \
+  hipLaunchKernelGGL((k_tile_apply<T, M, S, N, NTH>), grid, dim3(NTH), sh, st, tl.node0,          \
+                     tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                                  \
+                     reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,             \
+                     tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0,   \
                      s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials, vs)
+#define TMRX_TILE(M, S)                    \
+  if (tl.threads == 512) {                 \
+    if (tl.nodal)                          \
+      TMRX_TILE_NT(M, S, true, 512);       \
+    else                                   \
+      TMRX_TILE_NT(M, S, false, 512);      \
+  } else {                                 \
+    if (tl.nodal)                          \
+      TMRX_TILE_NT(M, S, true, 256);       \
+    else                                   \
+      TMRX_TILE_NT(M, S, false, 256);      \
+  }
 #define TMRX_TILE2(M) \
   if (single) {       \
     TMRX_TILE(M, true); \
@@ -1672,6 +1716,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   }
 #undef TMRX_TILE2
 #undef TMRX_TILE
+#undef TMRX_TILE_NT
   TMRX_LAUNCH_CHECK();
 }
 

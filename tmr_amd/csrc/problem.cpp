@@ -679,6 +679,9 @@ int Problem::build_tiles(int l) {
   }
   Tiles &T = L.tiles;
   T.n_tiles = ntiles;
+  // large tiles take 512 threads on the vector ALUs, so that the LDS they
+  // need (4 tiles per CU) still fills the CU with waves
+  T.threads = L.tile_nodes >= 768 ? 512 : TILE_THREADS;
   T.single_class = single;
   T.max_slots = max_slots;
   T.max_mrec = max_mrec;
