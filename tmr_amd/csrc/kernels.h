@@ -142,6 +142,10 @@ enum AsmMode {
   ASM_GATHER = 5,  // y = sum of the slot partials (clamped rows zero): Z^T g
 };
 
+// Chebyshev-step flags: read the previous direction d, take d = x (the step
+// after a first step from a zero guess, where x = d), store the new d
+enum { CHEB_D_READ = 1, CHEB_D_IS_X = 2, CHEB_D_STORE = 4 };
+
 struct AsmArgs {
   const void *b = nullptr;
   void *y = nullptr;
@@ -150,6 +154,7 @@ struct AsmArgs {
   void *d = nullptr;       // Chebyshev direction
   const void *dinv = nullptr;
   double ca = 0.0, cc = 0.0;
+  int cflags = CHEB_D_READ | CHEB_D_STORE;  // tile apply, ASM_CHEB
   double *partials = nullptr;
 };
 
@@ -176,6 +181,7 @@ void launch_prolong_add(const TransferCSR &P, const uint8_t *bcmask_f, const T *
 template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *bcmask_c, const T *rf, T *rc,
                      hipStream_t st, int nrhs = 1, int64_t vsf = 0, int64_t vsc = 0);
+// x = c Dinv b (and d = x when d is not null)
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
                        hipStream_t st, int nrhs = 1);

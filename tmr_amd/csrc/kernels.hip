@@ -401,12 +401,9 @@ __global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t off = (int64_t)blockIdx.y * vs;
-  b += off;
-  d += off;
-  x += off;
-  const T v = (T)c * dinv[i] * b[i];
-  d[i] = v;
-  x[i] = v;
+  const T v = (T)c * dinv[i] * b[off + i];
+  if (d) d[off + i] = v;
+  x[off + i] = v;
 }
 
 template <class T>
@@ -1095,7 +1092,7 @@ __device__ __forceinline__ void tile_finish(
     int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
     const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
     const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out,
-    T *__restrict__ d, const T *__restrict__ dinv, double ca, double cc,
+    T *__restrict__ d, const T *__restrict__ dinv, double ca, double cc, int cflags,
     double *__restrict__ partials, double *red, int pstride = 0) {
   auto xat = [&](int pos, int c) -> T {
     if constexpr (SPLIT) {
@@ -1119,7 +1116,7 @@ __device__ __forceinline__ void tile_finish(
       const size_t k = 3 * (size_t)node + c;
       if (MODE == ASM_RESID || MODE == ASM_CHEB) bv[c] = b[k];
       if (MODE == ASM_CHEB) {
-        dv[c] = d[k];
+        if (cflags & CHEB_D_READ) dv[c] = d[k];
         wv[c] = dinv[k];
       }
       if (MODE == ASM_STORE || MODE == ASM_DOT) xv[c] = x[k];
@@ -1148,13 +1145,15 @@ __device__ __forceinline__ void tile_finish(
         y[k] = bc ? T(0) : bv[c] - a[c];
       } else if (MODE == ASM_CHEB) {
         if (bc) {
-          d[k] = T(0);
+          if (cflags & CHEB_D_STORE) d[k] = T(0);
           x_out[k] = T(0);
         } else {
           const T r = bv[c] - a[c];
-          const T dn = (T)ca * dv[c] + (T)cc * wv[c] * r;
-          d[k] = dn;
-          x_out[k] = xat(ti, c) + dn;
+          const T xc = xat(ti, c);
+          const T dp = (cflags & CHEB_D_IS_X) ? xc : dv[c];
+          const T dn = (T)ca * dp + (T)cc * wv[c] * r;
+          if (cflags & CHEB_D_STORE) d[k] = dn;
+          x_out[k] = xc + dn;
         }
       }
     }
@@ -1315,7 +1314,8 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
     const float *__restrict__ dep_w, const uint8_t *__restrict__ bcmask,
     const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
     const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
-    const T *__restrict__ dinv, double ca, double cc, double *__restrict__ partials, int64_t vs) {
+    const T *__restrict__ dinv, double ca, double cc, int cflags, double *__restrict__ partials,
+    int64_t vs) {
   constexpr int S = tile_stride<T>();
   constexpr int SX = tile_xstride<T>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1393,7 +1393,7 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
   }
   tile_finish<T, MODE, S, NT, false, SX>(t, n0, nown, comb_ptr, comb_slot, comb_w,
                                                    bcmask, xs, ys, x, b, y, x_out, d, dinv, ca,
-                                                   cc, partials, red);
+                                                   cc, cflags, partials, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1498,7 +1498,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const uint8_t *__restrict__ bcmask, const float *__restrict__ s_elem,
     const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
     float *__restrict__ x_out, float *__restrict__ d, const float *__restrict__ dinv, double ca,
-    double cc, double *__restrict__ partials, int64_t vs) {
+    double cc, int cflags, double *__restrict__ partials, int64_t vs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -1642,7 +1642,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
   __syncthreads();
   tile_finish<float, MODE, 4, TILE_THREADS, true>(
       t, n0, nown, comb_ptr, comb_slot, comb_w, bcmask, reinterpret_cast<const float *>(ph), ys, x,
-      b, y, x_out, d, dinv, ca, cc, partials, red, np);
+      b, y, x_out, d, dinv, ca, cc, cflags, partials, red, np);
 }
 
 template <class T>
@@ -1665,7 +1665,8 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                      reinterpret_cast<const uint32_t *>(tl.mel), tl.ncolors, tl.afrag, tl.comb_ptr, \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,      \
                      (const float *)s_elem, (const float *)x, (const float *)b, (float *)y,      \
-                     (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.partials, vs)
+                     (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.cflags,          \
+                     g.partials, vs)
 #define TMRX_MTILE2(M)    \
   if (single) {           \
     TMRX_MTILE(M, true);  \
@@ -1689,7 +1690,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                      tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                                  \
                      reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,             \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0,   \
-                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.partials, vs)
+                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.cflags, g.partials, vs)
 #define TMRX_TILE(M, S)                    \
   if (tl.threads == 512) {                 \
     if (tl.nodal)                          \

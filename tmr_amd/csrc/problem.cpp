@@ -1252,7 +1252,7 @@ int Problem::setup(const SolveOpts &o) {
 // (tiles read halo values of x, so the update cannot be in place).
 template <class T>
 void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
-                        int nrhs) {
+                        int nrhs, int cflags) {
   Level &L = levels_[l];
   LevelVecs<T> &v = vecs<T>(l);
   const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
@@ -1273,6 +1273,7 @@ void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const
   a.dinv = v.dinv.p;
   a.ca = ca;
   a.cc = cc;
+  a.cflags = k == 0 ? (cflags & CHEB_D_STORE) : cflags;  // step 0: no previous direction
   apply_level<T>(l, x_in, ASM_CHEB, a, nrhs);
 }
 
@@ -1292,12 +1293,15 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   // the last post-smoothing update lands in x
   T *cur = v.x2.p, *oth = x;
   if (cur == x) std::swap(cur, oth);
+  // the first step from a zero guess leaves d = x, so the second step reads
+  // x for d; the last step of each sweep stores no direction
   {
     const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
-    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), v.d.p, cur, st_, nrhs);
+    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), nullptr, cur, st_, nrhs);
   }
   for (int k = 1; k < deg; k++) {
-    cheb_step<T>(l, b, cur, oth, k, o, nrhs);
+    const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < deg ? CHEB_D_STORE : 0);
+    cheb_step<T>(l, b, cur, oth, k, o, nrhs, fl);
     std::swap(cur, oth);
   }
   AsmArgs a;
@@ -1308,7 +1312,7 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs);
   launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
   for (int k = 0; k < deg; k++) {
-    cheb_step<T>(l, b, cur, oth, k, o, nrhs);
+    cheb_step<T>(l, b, cur, oth, k, o, nrhs, CHEB_D_READ | (k + 1 < deg ? CHEB_D_STORE : 0));
     std::swap(cur, oth);
   }
 }

@@ -227,7 +227,7 @@ class Problem {
   void apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrhs = 1);
   template <class T>
   void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
-                 int nrhs = 1);
+                 int nrhs = 1, int cflags = CHEB_D_READ | CHEB_D_STORE);
   template <class T>
   void vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs = 1);
   void precond(int nrhs, const SolveOpts &o);
