@@ -21,10 +21,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# algorithmic bytes of one finest-level Chebyshev step launch in FP32: per node
-# x, b, d, dinv read and d, x_out written (6 x 3 floats) plus the clamp mask
-# byte; per element the SIMP scale (float) and the 8-int connectivity
-CHEB_BYTES_PER_NODE = 6 * 3 * 4 + 1
+# algorithmic bytes of a finest-level Chebyshev step launch in FP32, mean of
+# the three launches of a degree-2 V-cycle: per node x, b, dinv read and
+# x_out written (4 x 3 floats) plus the clamp-mask byte; the pre-smoothing
+# step (after the first step from zero, d = x) neither reads nor writes d,
+# the first post-smoothing step writes it, the last one reads it: 49, 61
+# and 61 bytes.  Per element the SIMP scale (float) and the 8-int connectivity.
+CHEB_BYTES_PER_NODE = (49 + 61 + 61) / 3.0
 CHEB_BYTES_PER_ELEM = 4 + 8 * 4
 FLOPS_PER_ELEM = 1152.0      # 2 * 24 * 24
 TIME_CHEB = 1 << 2           # launch-kind mask: Chebyshev update
@@ -34,10 +37,10 @@ TILE_CANDIDATES = (256, 384, 512, 768, 1024)  # owner-tile sizes the autotuner t
 # dispatch, mean over the launches at that level), keyed by (FP32 form, owner
 # tile nodes).  The counters are scaled by what they read for a 1 GiB
 # device-to-device copy in the same runs (bench.py --calibration-copy 1024):
-# FETCH_SIZE 524,316 KiB and WRITE_SIZE 1,048,576 KiB for 1,048,576 KiB moved.
-COPY_FETCH_FACTOR = 1048576.0 / 524316.0
+# FETCH_SIZE 524,318 KiB and WRITE_SIZE 1,048,576 KiB for 1,048,576 KiB moved.
+COPY_FETCH_FACTOR = 1048576.0 / 524318.25
 COPY_WRITE_FACTOR = 1048576.0 / 1048576.0
-MEASURED_TRAFFIC_KIB = {("valu", 512): (218442.8, 100237.4)}
+MEASURED_TRAFFIC_KIB = {("valu", 1024): (192855.5, 66826.3)}
 
 
 def log(*a):
@@ -45,8 +48,9 @@ def log(*a):
 
 
 def cpu_baseline(level=5, iters=None):
-    """FP64 oracle (assembled CSR + Jacobi PCG, 1 core) on the same box
-    cantilever at a smaller level; reports MDOF*iter/s."""
+    """FP64 oracle (assembled CSR + Jacobi PCG) on the same box cantilever at
+    a smaller level, on all host cores (OpenMP, at most 16 threads: the GPU
+    host's share) and on one core; reports MDOF*iter/s."""
     from oracle import pyoracle as po
     from tmr_amd import workloads
     forests, rho, L = workloads.box(level=level, nlevels=1, seed=0)
@@ -64,18 +68,26 @@ def cpu_baseline(level=5, iters=None):
             bit = [(k & 1), (k >> 1) & 1, (k >> 2)][ax]
             if bit == face % 2:
                 f[3 * conn[e][k] + 2] += -0.25 * area
-    t0 = time.perf_counter()
-    try:
-        orc.solve(f, rtol=1e-8, maxit=iters or 20000)
-        it = orc.iters
-    except RuntimeError:
-        it = iters
-    dt = time.perf_counter() - t0
     ndof = 3 * orc.nnodes
-    return {"value": ndof * it / dt / 1e6, "unit": "MDOF*iter/s", "cores": 1, "kind": "port",
-            "sample": "oracle FP64 Jacobi-PCG (assembled CSR) on the same box cantilever at "
-                      "level %d (%d hex, %d dofs), %d iterations to 1e-8 in %.1f s"
-                      % (level, conn.shape[0], ndof, it, dt)}
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    rates = {}
+    for th in (nthreads, 1):
+        po.set_threads(th)
+        t0 = time.perf_counter()
+        try:
+            orc.solve(f, rtol=1e-8, maxit=iters or 20000)
+            it = orc.iters
+        except RuntimeError:
+            it = iters
+        dt = time.perf_counter() - t0
+        rates[th] = (ndof * it / dt / 1e6, it, dt)
+    po.set_threads(1)
+    v, it, dt = rates[nthreads]
+    return {"value": v, "unit": "MDOF*iter/s", "cores": nthreads, "kind": "port",
+            "value_1core": rates[1][0],
+            "sample": "oracle FP64 Jacobi-PCG (assembled CSR, OpenMP) on the same box cantilever at "
+                      "level %d (%d hex, %d dofs), %d iterations to 1e-8 in %.2f s on %d threads "
+                      "(%.1f s on 1)" % (level, conn.shape[0], ndof, it, dt, nthreads, rates[1][2])}
 
 
 def bench_multirhs(args):
@@ -277,6 +289,13 @@ def main():
             prob.solve(f, u, opts=opts)
             res["kt"] = prob.kernel_timing_result()
             prob.kernel_timing(False)
+            # host buffers: f copied in and u copied out around each solve
+            u_host = np.zeros_like(f_host)
+            prob.solve(f_host, u_host, opts=opts)
+            ts = time.perf_counter()
+            for _ in range(steps):
+                prob.solve(f_host, u_host, opts=opts)
+            res["host_seconds"] = time.perf_counter() - ts
         return res
 
     mixed = run(1, args.steps, args.warmup, timing=True)
@@ -294,7 +313,7 @@ def main():
     tot_ms, cnt, _ = mixed["kt"]
     avg_s = tot_ms / max(cnt, 1) / 1e3
     nnodes = ndof // 3
-    bytes_launch = CHEB_BYTES_PER_NODE * nnodes + CHEB_BYTES_PER_ELEM * prob.nelem
+    bytes_launch = int(round(CHEB_BYTES_PER_NODE * nnodes + CHEB_BYTES_PER_ELEM * prob.nelem))
     achieved = bytes_launch / avg_s / 1e9
     form = "valu" if (args.fp32_valu or not prob.tuning()[1]) else "mfma"
     traffic = None
@@ -319,6 +338,7 @@ def main():
                                "FP64 flexible PCG to 1e-8" % args.level,
                    "elements": prob.nelem, "dofs": ndof, "mg_levels": prob.nlevels},
         "mdof_iter_per_s": mdof,
+        "host_buffer_solves_per_s": args.steps / mixed["host_seconds"],
         "pcg_iters": mixed["iters"],
         "true_residual": mixed["true_res"],
         "compliance": mixed["compliance"],

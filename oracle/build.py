@@ -21,7 +21,7 @@ def _run(cmd):
 def build_oracle():
     srcs = [os.path.join(HERE, s) for s in sorted(os.listdir(HERE)) if s.endswith(".c")]
     tmp = LIB + ".tmp"
-    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-o", tmp] + srcs + ["-lm"])
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-o", tmp] + srcs + ["-lm"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -293,8 +293,14 @@ int orc_get_csr(int *rowp, int *cols, double *vals) {
   return 0;
 }
 
+/* threads of the CG loops (OpenMP; 1 by default so that test results do not
+   depend on the host) */
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
 void orc_matvec(const double *x, double *y) {
   const csr_t *A = g_A;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int i = 0; i < A->n; i++) {
     double s = 0.0;
     for (int k = A->rowp[i]; k < A->rowp[i + 1]; k++) s += A->vals[k] * x[A->cols[k]];
@@ -333,9 +339,11 @@ int orc_solve(const double *b, double *x, double rtol, int maxit, double *relres
     it++;
     orc_matvec(p, q);
     double pq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : pq) num_threads(g_threads)
     for (int i = 0; i < n; i++) pq += p[i] * q[i];
     double alpha = rz / pq;
     double rr = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr) num_threads(g_threads)
     for (int i = 0; i < n; i++) {
       x[i] += alpha * p[i];
       r[i] -= alpha * q[i];
@@ -347,12 +355,14 @@ int orc_solve(const double *b, double *x, double rtol, int maxit, double *relres
       break;
     }
     double rzn = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rzn) num_threads(g_threads)
     for (int i = 0; i < n; i++) {
       z[i] = dinv[i] * r[i];
       rzn += r[i] * z[i];
     }
     double beta = rzn / rz;
     rz = rzn;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (int i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
   }
   /* true residual */

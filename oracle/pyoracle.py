@@ -31,6 +31,7 @@ def lib():
         L.orc_elem_energy.argtypes = [I, P, P, P, P, P, P, D, D, P, P]
         L.orc_elem_energy2.argtypes = [I, P, P, P, P, P, P, D, D, P, P, P]
         L.orc_hex8_stiffness_gp.argtypes = [D, D, D, D, D, I, P]
+        L.orc_set_threads.argtypes = [I]
         L.orc_assemble_gp.restype = C.c_long
         L.orc_assemble_gp.argtypes = [I, I, P, P, P, P, P, P, P, D, D]
         L.orc_elem_energy_gp.argtypes = [I, P, P, P, P, P, P, D, D, P, P, P]
@@ -48,6 +49,11 @@ def hex8_stiffness(hx, hy, hz, E=1.0, nu=0.3):
     K = np.zeros((24, 24))
     lib().orc_hex8_stiffness(hx, hy, hz, E, nu, _p(K))
     return K
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's CG (1 by default)."""
+    lib().orc_set_threads(int(n))
 
 
 def hex8_stiffness_gp(hx, hy, hz, q, E=1.0, nu=0.3):
