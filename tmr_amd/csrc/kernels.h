@@ -105,6 +105,9 @@ struct Tiles {
   bool nodal = false;
   // threads per tile of the vector-ALU form (256 or 512)
   int threads = TILE_THREADS;
+  // when set, launches return at once while *run == 0 (no segment of a
+  // batched solve is still running)
+  const int *run = nullptr;
 };
 
 // Per stiffness class, the nodal-design element tables: dN_a/dx_d at the 8
@@ -130,6 +133,7 @@ struct TransferCSR {  // rows = output nodes
   const int *ptr = nullptr;
   const int *col = nullptr;
   const float *w = nullptr;
+  const int *run = nullptr;  // as Tiles::run
 };
 
 // y-side operation performed by the assembly pass
@@ -184,7 +188,7 @@ void launch_restrict(const TransferCSR &R, const uint8_t *bcmask_c, const T *rf,
 // x = c Dinv b (and d = x when d is not null)
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
-                       hipStream_t st, int nrhs = 1);
+                       hipStream_t st, int nrhs = 1, const int *run = nullptr);
 // coarsest-level solve: block-diagonal explicit inverse (one block per
 // independent component), per row: first column, block size, row offset
 struct CoarseBlocks {
@@ -193,6 +197,7 @@ struct CoarseBlocks {
   const int *col0 = nullptr;
   const int *nb = nullptr;
   const int64_t *aoff = nullptr;
+  const int *run = nullptr;  // as Tiles::run
 };
 template <class T>
 void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs = 1);

@@ -318,7 +318,9 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
                                                      const float *__restrict__ w,
                                                      const uint8_t *__restrict__ mask,
                                                      const T *__restrict__ xc, T *__restrict__ xf,
-                                                     int64_t vsc, int64_t vsf) {
+                                                     int64_t vsc, int64_t vsf,
+                                                     const int *__restrict__ run) {
+  if (run && *run == 0) return;
   const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
   xc += (int64_t)blockIdx.y * vsc;
@@ -345,7 +347,9 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
                                                   const float *__restrict__ w,
                                                   const uint8_t *__restrict__ mask,
                                                   const T *__restrict__ rf, T *__restrict__ rc,
-                                                  int64_t vsf, int64_t vsc) {
+                                                  int64_t vsf, int64_t vsc,
+                                                  const int *__restrict__ run) {
+  if (run && *run == 0) return;
   const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   rf += (int64_t)blockIdx.y * vsf;
   rc += (int64_t)blockIdx.y * vsc;
@@ -379,7 +383,7 @@ void launch_prolong_add(const TransferCSR &P, const uint8_t *mask_f, const T *xc
                         hipStream_t st, int nrhs, int64_t vsc, int64_t vsf) {
   if (P.n_rows == 0) return;
   hipLaunchKernelGGL(k_prolong_add<T>, dim3((P.n_rows + 255) / 256, nrhs), dim3(256), 0, st,
-                     P.n_rows, P.ptr, P.col, P.w, mask_f, xc, xf, vsc, vsf);
+                     P.n_rows, P.ptr, P.col, P.w, mask_f, xc, xf, vsc, vsf, P.run);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -389,7 +393,7 @@ void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T
   if (R.n_rows == 0) return;
   const int64_t threads = 4 * (int64_t)R.n_rows;
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256), nrhs), dim3(256), 0,
-                     st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc);
+                     st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc, R.run);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -397,7 +401,9 @@ void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T
 // Chebyshev first step from a zero guess: d = c Dinv b, x = d
 template <class T>
 __global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict__ dinv, double c,
-                             T *__restrict__ d, T *__restrict__ x, int64_t vs) {
+                             T *__restrict__ d, T *__restrict__ x, int64_t vs,
+                             const int *__restrict__ run) {
+  if (run && *run == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t off = (int64_t)blockIdx.y * vs;
@@ -408,9 +414,9 @@ __global__ void k_cheb_first(int n, const T *__restrict__ b, const T *__restrict
 
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
-                       hipStream_t st, int nrhs) {
+                       hipStream_t st, int nrhs, const int *run) {
   hipLaunchKernelGGL(k_cheb_first<T>, dim3((ndof + 255) / 256, nrhs), dim3(256), 0, st, ndof, b,
-                     dinv, c, d, x, (int64_t)ndof);
+                     dinv, c, d, x, (int64_t)ndof, run);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -423,7 +429,9 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__rest
                                                      const int *__restrict__ col0,
                                                      const int *__restrict__ nb,
                                                      const int64_t *__restrict__ aoff,
-                                                     const T *__restrict__ b, T *__restrict__ x) {
+                                                     const T *__restrict__ b, T *__restrict__ x,
+                                                     const int *__restrict__ run) {
+  if (run && *run == 0) return;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
@@ -442,7 +450,7 @@ template <class T>
 void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs) {
   if (cb.n == 0) return;
   hipLaunchKernelGGL(k_dense_solve<T>, dim3((cb.n + 3) / 4, nrhs), dim3(256), 0, st, cb.n,
-                     cb.Ainv, cb.col0, cb.nb, cb.aoff, b, x);
+                     cb.Ainv, cb.col0, cb.nb, cb.aoff, b, x, cb.run);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -1315,7 +1323,8 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
     const T *__restrict__ K0, const T *__restrict__ s_elem, const T *__restrict__ x,
     const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out, T *__restrict__ d,
     const T *__restrict__ dinv, double ca, double cc, int cflags, double *__restrict__ partials,
-    int64_t vs) {
+    int64_t vs, const int *__restrict__ run) {
+  if (run && *run == 0) return;
   constexpr int S = tile_stride<T>();
   constexpr int SX = tile_xstride<T>();
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1498,7 +1507,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
     const uint8_t *__restrict__ bcmask, const float *__restrict__ s_elem,
     const float *__restrict__ x, const float *__restrict__ b, float *__restrict__ y,
     float *__restrict__ x_out, float *__restrict__ d, const float *__restrict__ dinv, double ca,
-    double cc, int cflags, double *__restrict__ partials, int64_t vs) {
+    double cc, int cflags, double *__restrict__ partials, int64_t vs,
+    const int *__restrict__ run) {
+  if (run && *run == 0) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[TILE_THREADS / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -1666,7 +1677,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,      \
                      (const float *)s_elem, (const float *)x, (const float *)b, (float *)y,      \
                      (float *)xo, (float *)d, (const float *)dinv, g.ca, g.cc, g.cflags,          \
-                     g.partials, vs)
+                     g.partials, vs, tl.run)
 #define TMRX_MTILE2(M)    \
   if (single) {           \
     TMRX_MTILE(M, true);  \
@@ -1690,7 +1701,7 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
                      tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                                  \
                      reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,             \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0,   \
-                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.cflags, g.partials, vs)
+                     s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.cflags, g.partials, vs, tl.run)
 #define TMRX_TILE(M, S)                    \
   if (tl.threads == 512) {                 \
     if (tl.nodal)                          \
@@ -1969,7 +1980,7 @@ void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q,
   template void launch_restrict<T>(const TransferCSR &, const uint8_t *, const T *, T *,        \
                                    hipStream_t, int, int64_t, int64_t);                         \
   template void launch_cheb_first<T>(int, const T *, const T *, double, T *, T *,               \
-                                     hipStream_t, int);                                         \
+                                     hipStream_t, int, const int *);                            \
   template void launch_dense_solve<T>(const CoarseBlocks &, const T *, T *, hipStream_t, int);  \
   template void launch_fill<T>(int64_t, T, T *, hipStream_t);                                 \
   template void launch_tile_apply<T>(const Tiles &, const T *, const T *, const T *, int,       \

@@ -975,6 +975,7 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
   Tiles tl = L.tiles;
+  tl.run = run_;
   tl.nodal = design_mode_ == 1;
   tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_ && !tl.nodal;
   if (ev) (void)hipEventRecord(ev->first, st_);
@@ -1281,7 +1282,9 @@ template <class T>
 void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   const int last = (int)levels_.size() - 1;
   if (l == last) {
-    launch_dense_solve<T>(coarse_, b, x, st_, nrhs);
+    CoarseBlocks cb = coarse_;
+    cb.run = run_;
+    launch_dense_solve<T>(cb, b, x, st_, nrhs);
     return;
   }
   Level &L = levels_[l];
@@ -1297,7 +1300,8 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   // x for d; the last step of each sweep stores no direction
   {
     const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
-    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), nullptr, cur, st_, nrhs);
+    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), nullptr, cur, st_, nrhs,
+                         run_);
   }
   for (int k = 1; k < deg; k++) {
     const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < deg ? CHEB_D_STORE : 0);
@@ -1308,9 +1312,11 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   a.b = b;
   a.y = v.r.p;
   apply_level<T>(l, cur, ASM_RESID, a, nrhs);
-  launch_restrict<T>(L.R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof);
+  TransferCSR R = L.R, P = L.P;
+  R.run = P.run = run_;
+  launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof);
   vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs);
-  launch_prolong_add<T>(L.P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
+  launch_prolong_add<T>(P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
   for (int k = 0; k < deg; k++) {
     cheb_step<T>(l, b, cur, oth, k, o, nrhs, CHEB_D_READ | (k + 1 < deg ? CHEB_D_STORE : 0));
     std::swap(cur, oth);
@@ -1374,6 +1380,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   float *r32 = mixed ? levels_[0].v32.b.p : nullptr;
   const int nidx = nrhs * nseg_;
   const double rtol2 = o.rtol * o.rtol, atol2 = o.atol * o.atol;
+  run_ = nact_.p;
   AsmArgs a;
   a.y = q_.p;
   a.partials = parts_.p;
@@ -1399,6 +1406,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
                       atol2, st_);
     launch_b_update_p<double>(pl, p_.p, z_.p, sc_.p, act_.p, st_);
   }
+  run_ = nullptr;
   launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
   (void)hipMemcpyAsync(nact_host_ + slot, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_);
 }

@@ -278,6 +278,10 @@ class Problem {
   double *g_xsol_ = nullptr;
   SolveOpts g_opts_;
   int gen_ = 0;  // bumped whenever buffers move or setup reruns
+  // while a PCG iteration is enqueued: the device count of running segments;
+  // the operator, transfer and smoother launches return at once when it is 0
+  // (the iteration launched after the one that converged)
+  const int *run_ = nullptr;
   void drop_graphs();
   void enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const SolveOpts &o, int slot);
   int last_nrhs_ = 0;
