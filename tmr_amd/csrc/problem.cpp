@@ -1390,6 +1390,9 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
+  // count the segments still running (and advance the iteration counter)
+  // before the preconditioner: an iteration that converged skips it
+  launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
   // z = M r; in mixed precision the FP64 updates read the FP32 result
   // directly (no FP64 copy of z)
   if (mixed) {
@@ -1407,7 +1410,6 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
     launch_b_update_p<double>(pl, p_.p, z_.p, sc_.p, act_.p, st_);
   }
   run_ = nullptr;
-  launch_b_tick(iter_.p, act_.p, nidx, nact_.p, st_);
   (void)hipMemcpyAsync(nact_host_ + slot, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_);
 }
 
