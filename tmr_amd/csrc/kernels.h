@@ -182,9 +182,12 @@ int assemble_num_blocks(int n_nodes);
 template <class T>
 void launch_prolong_add(const TransferCSR &P, const uint8_t *bcmask_f, const T *xc, T *xf,
                         hipStream_t st, int nrhs = 1, int64_t vsc = 0, int64_t vsf = 0);
+// rc = R rf (clamped rows zero); with x1, also the coarse level's first
+// Chebyshev step from zero, x1 = c0 Dinv_c rc
 template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *bcmask_c, const T *rf, T *rc,
-                     hipStream_t st, int nrhs = 1, int64_t vsf = 0, int64_t vsc = 0);
+                     hipStream_t st, int nrhs = 1, int64_t vsf = 0, int64_t vsc = 0,
+                     const T *dinv_c = nullptr, double c0 = 0.0, T *x1 = nullptr);
 // x = c Dinv b (and d = x when d is not null)
 template <class T>
 void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x,
@@ -280,9 +283,12 @@ struct SegPlan {
 enum { FIN_BB = 0, FIN_RZ, FIN_PQ, FIN_RR, FIN_BETA, FIN_TRUE };
 void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const int *act,
                    double *partials, hipStream_t st);
+// x += alpha p, r -= alpha q, the FP32 copy r32 of r and, with x1, the
+// finest level's first Chebyshev step from zero x1 = c0 Dinv32 r32
 void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
                         const double *q, float *r32, const double *sc, const int *act,
-                        double *partials, hipStream_t st);
+                        double *partials, hipStream_t st, const float *dinv32 = nullptr,
+                        double c0 = 0.0, float *x1 = nullptr);
 template <class TZ>
 void launch_b_dot2(const SegPlan &pl, const TZ *z, const double *r, const double *q,
                    const int *act, double *partials, hipStream_t st);

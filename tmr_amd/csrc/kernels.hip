@@ -348,11 +348,14 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
                                                   const uint8_t *__restrict__ mask,
                                                   const T *__restrict__ rf, T *__restrict__ rc,
                                                   int64_t vsf, int64_t vsc,
-                                                  const int *__restrict__ run) {
+                                                  const int *__restrict__ run,
+                                                  const T *__restrict__ dinv_c, double c0,
+                                                  T *__restrict__ x1) {
   if (run && *run == 0) return;
   const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   rf += (int64_t)blockIdx.y * vsf;
   rc += (int64_t)blockIdx.y * vsc;
+  if (x1) x1 += (int64_t)blockIdx.y * vsc;
   const int i = g >> 2, sub = g & 3;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   if (i < n) {
@@ -372,9 +375,15 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   a2 += __shfl_xor(a2, 2, 64);
   if (i < n && sub == 0) {
     const uint8_t m = mask[i];
-    rc[3 * i] = (m & 1) ? T(0) : a0;
-    rc[3 * i + 1] = (m & 2) ? T(0) : a1;
-    rc[3 * i + 2] = (m & 4) ? T(0) : a2;
+    const T v0 = (m & 1) ? T(0) : a0, v1 = (m & 2) ? T(0) : a1, v2 = (m & 4) ? T(0) : a2;
+    rc[3 * i] = v0;
+    rc[3 * i + 1] = v1;
+    rc[3 * i + 2] = v2;
+    if (x1) {  // first Chebyshev step of the coarse level from zero
+      x1[3 * i] = (T)c0 * dinv_c[3 * i] * v0;
+      x1[3 * i + 1] = (T)c0 * dinv_c[3 * i + 1] * v1;
+      x1[3 * i + 2] = (T)c0 * dinv_c[3 * i + 2] * v2;
+    }
   }
 }
 
@@ -389,11 +398,13 @@ void launch_prolong_add(const TransferCSR &P, const uint8_t *mask_f, const T *xc
 
 template <class T>
 void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T *rc,
-                     hipStream_t st, int nrhs, int64_t vsf, int64_t vsc) {
+                     hipStream_t st, int nrhs, int64_t vsf, int64_t vsc, const T *dinv_c,
+                     double c0, T *x1) {
   if (R.n_rows == 0) return;
   const int64_t threads = 4 * (int64_t)R.n_rows;
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256), nrhs), dim3(256), 0,
-                     st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc, R.run);
+                     st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc, R.run, dinv_c, c0,
+                     x1);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -1978,7 +1989,7 @@ void launch_adjoint_grad(int n, const double *rho, const int16_t *lev, double q,
   template void launch_prolong_add<T>(const TransferCSR &, const uint8_t *, const T *, T *,     \
                                       hipStream_t, int, int64_t, int64_t);                      \
   template void launch_restrict<T>(const TransferCSR &, const uint8_t *, const T *, T *,        \
-                                   hipStream_t, int, int64_t, int64_t);                         \
+                                   hipStream_t, int, int64_t, int64_t, const T *, double, T *); \
   template void launch_cheb_first<T>(int, const T *, const T *, double, T *, T *,               \
                                      hipStream_t, int, const int *);                            \
   template void launch_dense_solve<T>(const CoarseBlocks &, const T *, T *, hipStream_t, int);  \

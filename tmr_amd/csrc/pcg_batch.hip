@@ -50,7 +50,9 @@ __global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restri
                                                     float *__restrict__ r32,
                                                     const double *__restrict__ sc,
                                                     const int *__restrict__ act,
-                                                    double *__restrict__ partials) {
+                                                    double *__restrict__ partials,
+                                                    const float *__restrict__ dinv32, double c0,
+                                                    float *__restrict__ x1) {
   __shared__ double red[BB / 64];
   const int blk = blockIdx.x, rhs = blockIdx.y;
   const int sidx = rhs * pl.nseg + pl.blk_seg[blk];
@@ -63,12 +65,19 @@ __global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restri
       x[k] += alpha * p[k];
       const double rn = r[k] - alpha * q[k];
       r[k] = rn;
-      if (r32) r32[k] = (float)rn;
+      if (r32) {
+        const float rf = (float)rn;
+        r32[k] = rf;
+        if (x1) x1[k] = (float)c0 * dinv32[i] * rf;
+      }
       s += rn * rn;
     }
   } else if (r32) {
-    for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB)
-      r32[off + i] = (float)r[off + i];
+    for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB) {
+      const float rf = (float)r[off + i];
+      r32[off + i] = rf;
+      if (x1) x1[off + i] = (float)c0 * dinv32[i] * rf;
+    }
   }
   const double t = bsum(s, red);
   if (threadIdx.x == 0) partials[(int64_t)rhs * pl.nblk + blk] = t;
@@ -228,9 +237,10 @@ void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const in
 }
 void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
                         const double *q, float *r32, const double *sc, const int *act,
-                        double *partials, hipStream_t st) {
+                        double *partials, hipStream_t st, const float *dinv32, double c0,
+                        float *x1) {
   hipLaunchKernelGGL(k_b_update_xr, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, x, r, p, q, r32,
-                     sc, act, partials);
+                     sc, act, partials, dinv32, c0, x1);
   TMRX_LAUNCH_CHECK();
 }
 template <class TZ>

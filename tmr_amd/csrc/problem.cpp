@@ -1279,7 +1279,14 @@ void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const
 }
 
 template <class T>
-void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
+double Problem::cheb_c0(int l, const SolveOpts &o) const {
+  const Level &L = levels_[l];
+  const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
+  return 2.0 / (lmax + lmin);
+}
+
+template <class T>
+void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool first_done) {
   const int last = (int)levels_.size() - 1;
   if (l == last) {
     CoarseBlocks cb = coarse_;
@@ -1298,11 +1305,8 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   if (cur == x) std::swap(cur, oth);
   // the first step from a zero guess leaves d = x, so the second step reads
   // x for d; the last step of each sweep stores no direction
-  {
-    const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
-    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, 2.0 / (lmax + lmin), nullptr, cur, st_, nrhs,
-                         run_);
-  }
+  if (!first_done)
+    launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, cheb_c0<T>(l, o), nullptr, cur, st_, nrhs, run_);
   for (int k = 1; k < deg; k++) {
     const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < deg ? CHEB_D_STORE : 0);
     cheb_step<T>(l, b, cur, oth, k, o, nrhs, fl);
@@ -1314,8 +1318,14 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs) {
   apply_level<T>(l, cur, ASM_RESID, a, nrhs);
   TransferCSR R = L.R, P = L.P;
   R.run = P.run = run_;
-  launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof);
-  vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs);
+  // the coarse level's first smoothing step rides on the restriction (its
+  // spare buffer, where that level's vcycle starts); the coarsest level is
+  // solved directly
+  const bool fuse = l + 1 < last;
+  launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof,
+                     fuse ? c.dinv.p : nullptr, fuse ? cheb_c0<T>(l + 1, o) : 0.0,
+                     fuse ? c.x2.p : nullptr);
+  vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs, fuse);
   launch_prolong_add<T>(P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
   for (int k = 0; k < deg; k++) {
     cheb_step<T>(l, b, cur, oth, k, o, nrhs, CHEB_D_READ | (k + 1 < deg ? CHEB_D_STORE : 0));
@@ -1387,7 +1397,12 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   apply_level<double>(0, p_.p, ASM_DOT, a, nrhs);
   launch_b_finalize(pl, FIN_PQ, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
-  launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_);
+  // mixed precision: the finest level's first smoothing step rides on the
+  // residual update (into the spare buffer where the V-cycle starts)
+  Level &L0u = levels_[0];
+  launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_,
+                     mixed ? L0u.v32.dinv.p : nullptr, mixed ? cheb_c0<float>(0, o) : 0.0,
+                     mixed ? L0u.v32.x2.p : nullptr);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
   // count the segments still running (and advance the iteration counter)
@@ -1397,7 +1412,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   // directly (no FP64 copy of z)
   if (mixed) {
     Level &L0 = levels_[0];
-    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs);
+    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs, true);
     launch_b_dot2<float>(pl, L0.v32.x.p, r_.p, q_.p, act_.p, parts_.p, st_);
     launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
@@ -1950,7 +1965,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   return 0;
 }
 
-template void Problem::vcycle<float>(int, const float *, float *, const SolveOpts &, int);
-template void Problem::vcycle<double>(int, const double *, double *, const SolveOpts &, int);
+template void Problem::vcycle<float>(int, const float *, float *, const SolveOpts &, int, bool);
+template void Problem::vcycle<double>(int, const double *, double *, const SolveOpts &, int, bool);
 
 }  // namespace tmrx

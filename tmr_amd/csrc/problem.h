@@ -229,7 +229,11 @@ class Problem {
   void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
                  int nrhs = 1, int cflags = CHEB_D_READ | CHEB_D_STORE);
   template <class T>
-  void vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs = 1);
+  // first_done: the first smoothing step from zero (c0 Dinv b) is already
+  // in the level's spare buffer (written by the restriction or the PCG update)
+  void vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs = 1, bool first_done = false);
+  template <class T>
+  double cheb_c0(int l, const SolveOpts &o) const;
   void precond(int nrhs, const SolveOpts &o);
   int coarse_factor();
   int find_components();
