@@ -1390,7 +1390,8 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   float *r32 = mixed ? levels_[0].v32.b.p : nullptr;
   const int nidx = nrhs * nseg_;
   const double rtol2 = o.rtol * o.rtol, atol2 = o.atol * o.atol;
-  run_ = nact_.p;
+  // (not while launches are being timed: every timed launch does its work)
+  run_ = timing_ ? nullptr : nact_.p;
   AsmArgs a;
   a.y = q_.p;
   a.partials = parts_.p;
