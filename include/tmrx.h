@@ -200,6 +200,15 @@ int tmrx_problem_design_info(tmrx_problem *p, int *mode, int *n);
 /* Nodal design values on multigrid level `level` (0 = finest), x[nodes of
    that level]: TMRConformFilter's per-level design vectors x[k]. */
 int tmrx_problem_level_design(tmrx_problem *p, int level, double *x);
+/* Prescribed values of the clamped dofs: ubc[n] (n = finest-level dofs,
+   entries of free dofs ignored, all zero = homogeneous conditions).  The
+   values of TMRBoundaryConditions::addBoundaryCondition
+   (src/TMRBoundaryConditions.cpp:68-116) that TMROctTACSCreator::
+   setBoundaryConditions hands to TACSAssembler::addBCs
+   (src/TMR_TACSCreator.cpp:468-500).  Solutions take them on the clamped
+   dofs; the free dofs solve K_ff u_f = f_f - K_fc u_c.  Compliance and its
+   gradient then use the full u (-u^T dK/drho u, as TMR's compliance branch). */
+int tmrx_problem_set_bc_values(tmrx_problem *p, const double *ubc, int n, int on_device);
 /* Density-dependent multigrid setup (diagonals, eigenvalue bounds, coarse
    Cholesky): mg->assembleJacobian + mg->factor
    (src/topology/TMRTopoProblem.cpp:1000-1001). */

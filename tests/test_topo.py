@@ -36,10 +36,20 @@ def test_boundary_conditions():
     assert bcs.as_list() == [("fixed", 7), ("sym", 5), ("roller", 2)]
     with pytest.raises(ValueError):
         bcs.addBoundaryCondition("x", [0, 1], [0.0])
-    with pytest.raises(NotImplementedError):
-        bcs.addBoundaryCondition("x", [0], [1.0])
     with pytest.raises(ValueError):
         bcs.addBoundaryCondition("x", [3])
+    assert not bcs.has_values()
+    # prescribed values: one per listed component (tmr/TMR.pyx:3917-3927)
+    bcs.addBoundaryCondition("load", [2, 0], [0.25, -1.5])
+    assert bcs.has_values()
+    assert bcs.as_list()[-1] == ("load", 5)
+    forest = _forest()
+    u = bcs.values_vector(forest)
+    nodes = np.asarray(forest.getNodesWithName("load"))
+    nodes = nodes[nodes >= 0]
+    assert np.all(u[3 * nodes] == -1.5) and np.all(u[3 * nodes + 2] == 0.25)
+    assert np.all(u[3 * nodes + 1] == 0.0)
+    assert np.count_nonzero(u) == 2 * len(nodes)
 
 
 def test_stiffness_properties_defaults_and_aliases():

@@ -79,6 +79,7 @@ _SIGS = {
     "tmrx_problem_set_stream": (I, [P, P]),
     "tmrx_problem_set_design": (I, [P, P, I, I]),
     "tmrx_problem_set_design_nodal": (I, [P, P, I, I]),
+    "tmrx_problem_set_bc_values": (I, [P, P, I, I]),
     "tmrx_problem_design_info": (I, [P, IP, IP]),
     "tmrx_problem_level_design": (I, [P, I, DP]),
     "tmrx_problem_setup": (I, [P, C.POINTER(SolveOpts)]),

@@ -157,6 +157,15 @@ int tmrx_problem_set_design_nodal(tmrx_problem *h, const double *x, int n, int o
   return h->p->set_design_nodal(x, n, on_device != 0) ? fail(h) : 0;
 }
 
+int tmrx_problem_set_bc_values(tmrx_problem *h, const double *ubc, int n, int on_device) {
+  CHECK_P(h);
+  if (!ubc) {
+    set_error("tmrx_problem_set_bc_values: null values");
+    return -1;
+  }
+  return h->p->set_bc_values(ubc, n, on_device != 0) ? fail(h) : 0;
+}
+
 int tmrx_problem_design_info(tmrx_problem *h, int *mode, int *n) {
   CHECK_P(h);
   if (mode) *mode = h->p->design_mode();

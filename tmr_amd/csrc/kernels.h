@@ -209,6 +209,9 @@ template <class TI, class TO>
 void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);
 template <class T>
 void launch_fill(int64_t n, T v, T *a, hipStream_t st);
+// v[r][i] += alpha y[i] for every right-hand side r (vectors of n values)
+void launch_axpy_bcast(int64_t n, int nrhs, double alpha, const double *y, double *v,
+                       hipStream_t st);
 // zero the clamped components of a node-major 3-dof vector
 void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st,
                          int nrhs = 1, int64_t vs = 0);

@@ -501,6 +501,20 @@ __global__ void k_zero_clamped(int n_nodes, const uint8_t *__restrict__ bcmask,
   if (m & 4) v[3 * (int64_t)i + 2] = 0.0;
 }
 
+__global__ void k_axpy_bcast(int64_t n, double alpha, const double *__restrict__ y,
+                             double *__restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[(int64_t)blockIdx.y * n + i] += alpha * y[i];
+}
+
+void launch_axpy_bcast(int64_t n, int nrhs, double alpha, const double *y, double *v,
+                       hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_axpy_bcast, dim3((unsigned)((n + 255) / 256), nrhs), dim3(256), 0, st, n,
+                     alpha, y, v);
+  TMRX_LAUNCH_CHECK();
+}
+
 void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st, int nrhs,
                          int64_t vs) {
   if (n_nodes == 0) return;

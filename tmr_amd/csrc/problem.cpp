@@ -302,6 +302,50 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     return -1;
   }
   HIPCHK(hipMemset(u_.p, 0, n * sizeof(double)));
+  {
+    std::vector<uint8_t> z(levels_[0].n_nodes, 0);
+    if (nomask_.upload(z)) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
+  }
+  return 0;
+}
+
+// Prescribed values of the clamped dofs: TMRBoundaryConditions' bc_vals
+// (src/TMRBoundaryConditions.cpp:68-116) handed to TACSAssembler::addBCs by
+// TMROctTACSCreator::setBoundaryConditions (src/TMR_TACSCreator.cpp:468-500).
+// The solve lifts them: K_ff u_f = f_f - K_fc u_c, u_c = the values.
+int Problem::set_bc_values(const double *ubc, int n, bool on_device) {
+  if (levels_.empty()) {
+    last_error = "problem: not initialised";
+    return -1;
+  }
+  const Level &L = levels_[0];
+  if (n != L.ndof) {
+    last_error = "set_bc_values: expected one value per finest-level dof";
+    return -1;
+  }
+  std::vector<double> h(n);
+  if (on_device) {
+    HIPCHK(hipStreamSynchronize(st_));
+    HIPCHK(hipMemcpy(h.data(), ubc, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::copy(ubc, ubc + n, h.begin());
+  }
+  bool any = false;
+  for (int i = 0; i < L.n_nodes; i++)
+    for (int c = 0; c < 3; c++) {
+      double &v = h[3 * (size_t)i + c];
+      if (!((L.bcmask[i] >> c) & 1)) v = 0.0;
+      if (v != 0.0) any = true;
+    }
+  if (ubc_.upload(h)) {
+    last_error = "set_bc_values: device allocation failed";
+    return -1;
+  }
+  has_ubc_ = any;
+  have_solution_ = false;
   return 0;
 }
 
@@ -1369,7 +1413,7 @@ int Problem::solve_multi(int nrhs, const double *f, double *u, bool on_device, c
   if (f)
     HIPCHK(hipMemcpyAsync(f_.p, f, m * sizeof(double),
                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
-  const int r = solve_batch(nrhs, f_.p, u_.p, o, infos);
+  const int r = solve_batch(nrhs, f_.p, u_.p, o, infos, has_ubc_);
   if (r < 0) return r;
   if (u) {
     HIPCHK(hipMemcpyAsync(u, u_.p, m * sizeof(double),
@@ -1430,7 +1474,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
 }
 
 int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o,
-                         SolveInfo *infos) {
+                         SolveInfo *infos, bool lift) {
   if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
     if (setup(o)) return -1;
   }
@@ -1453,6 +1497,27 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
   // x = 0; clamped rows of the right-hand side are zero (TACSAssembler::applyBCs)
   launch_fill<double>(m, 0.0, xsol, st_);
   launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, rhs, st_, nrhs, n);
+  if (lift) {
+    // prescribed values: the PCG solves for the free dofs against
+    // f - K u_bc (K without the identity rows), the values are added after
+    if (lrhs_.n < (size_t)m && lrhs_.alloc(m)) {
+      last_error = "solve: device allocation failed";
+      return -1;
+    }
+    HIPCHK(hipMemcpyAsync(lrhs_.p, rhs, m * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    Tiles tl = L0.tiles;
+    tl.bcmask = nomask_.p;
+    tl.nodal = design_mode_ == 1;
+    tl.use_mfma = false;
+    AsmArgs a;
+    a.y = q_.p;
+    a.x = (void *)ubc_.p;
+    launch_tile_apply<double>(tl, tl.nodal ? G64_.p : R64_.p, tl.nodal ? L0.sg64.p : L0.s64.p,
+                              ubc_.p, ASM_STORE, a, st_, 1, n);
+    launch_axpy_bcast(n, nrhs, -1.0, q_.p, lrhs_.p, st_);
+    launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, lrhs_.p, st_, nrhs, n);
+    rhs = lrhs_.p;
+  }
   HIPCHK(hipMemcpyAsync(r_.p, rhs, m * sizeof(double), hipMemcpyDeviceToDevice, st_));
   launch_b_dot1(pl, r_.p, r_.p, nullptr, parts_.p, st_);
   launch_b_finalize(pl, FIN_BB, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
@@ -1535,6 +1600,7 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
     launch_b_finalize(pl, FIN_TRUE, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
   }
+  if (lift) launch_axpy_bcast(n, nrhs, 1.0, ubc_.p, xsol, st_);
   seg_sc_.resize((size_t)nidx * SC_N);
   seg_itc_.resize(nidx);
   seg_act_.resize(nidx);
@@ -1634,11 +1700,11 @@ int Problem::compliance_grad(double *g, bool on_device) {
   }
   Level &L = levels_[0];
   if (design_mode_ == 1) {
-    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, state_mask(),
                         L.d_cls.p, G64_.p, u_.p, u_.p, egp_.p, st_);
     return nodal_gather(nullptr, g, on_device);
   }
-  launch_elem_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+  launch_elem_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, state_mask(),
                      L.d_cls.p, K0d_.p, u_.p, energy_.p, st_);
   double *out = g;
   DevArray<double> tmp;
@@ -1715,7 +1781,7 @@ int Problem::stress_pnorm(double P, double *f) {
     last_error = "stress_pnorm: reduction buffer too small";
     return -1;
   }
-  launch_stress_sum(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+  launch_stress_sum(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, state_mask(),
                     L.d_cls.p, L.d_lev.p, rho_.p, design_mode_ == 1 ? rho_gp_.p : nullptr, u_.p,
                     sa, parts_.p, st_);
   launch_finalize(nb, parts_.p, 4, sc_.p, st_);
@@ -1737,7 +1803,9 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   const double coef = std::pow(last_stress_sum_, 1.0 / P - 1.0);
   HIPCHK(hipMemcpyAsync(sc_.p + SC_ZQ, &coef, sizeof(double), hipMemcpyHostToDevice, st_));
   const bool nodal = design_mode_ == 1;
-  launch_stress_grad(L.eb, L.d_lev.p, L.d_cls.p, rho_.p, nodal ? rho_gp_.p : nullptr, u_.p, sa,
+  ElemBatches eb = L.eb;
+  eb.bcmask = state_mask();
+  launch_stress_grad(eb, L.d_lev.p, L.d_cls.p, rho_.p, nodal ? rho_gp_.p : nullptr, u_.p, sa,
                      sc_.p, L.v64.partial.p, nodal ? exgp_.p : dfdrho_.p, st_);
   AsmArgs a;
   a.y = adjf_.p;
@@ -1750,11 +1818,11 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   last_true_residual = tr;
   if (r != 0) return r < 0 ? -1 : 1;
   if (nodal) {
-    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+    launch_nodal_energy(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, state_mask(),
                         L.d_cls.p, G64_.p, u_.p, psi_.p, egp_.p, st_);
     return nodal_gather(exgp_.p, g, on_device);
   }
-  launch_elem_energy2(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, L.d_bcmask.p,
+  launch_elem_energy2(L.n_elem, L.d_conn.p, L.dep_ptr.p, L.dep_conn.p, L.dep_w.p, state_mask(),
                       L.d_cls.p, K0d_.p, u_.p, psi_.p, energy_.p, st_);
   double *out = g;
   DevArray<double> tmp;

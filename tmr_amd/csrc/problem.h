@@ -182,6 +182,9 @@ class Problem {
   int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
   bool fp32_matrix_cores() const { return fp32_mfma_; }
   int set_state(const double *f, const double *u, bool on_device);
+  // prescribed values of the clamped dofs (ubc[ndof], entries of free dofs
+  // ignored); solutions take them, the free dofs solve K_ff u_f = f_f - K_fc u_c
+  int set_bc_values(const double *ubc, int n, bool on_device);
   int compliance(double *c);
   int compliance_grad(double *g, bool on_device);
   // p-norm (exponent P) of the relaxed von Mises failure of the last solve,
@@ -239,7 +242,8 @@ class Problem {
   int find_components();
   // batched FP64 flexible PCG from x = 0 on rhs ([nrhs][ndof], clamped rows
   // zeroed in place); work vectors r_, p_, q_, z_
-  int solve_batch(int nrhs, double *rhs, double *x, const SolveOpts &o, SolveInfo *infos);
+  int solve_batch(int nrhs, double *rhs, double *x, const SolveOpts &o, SolveInfo *infos,
+                  bool lift = false);
   int estimate_eigs(int l, const SolveOpts &o);
 
   std::vector<Level> levels_;
@@ -257,7 +261,11 @@ class Problem {
   DevArray<float> G32_;
   int design_mode_ = 0;      // 0 element densities, 1 nodal
   DevArray<double> rho_gp_, egp_, exgp_, gtmp_;
-  DevArray<uint8_t> nomask_;
+  DevArray<uint8_t> nomask_;  // all dofs free (full K, unmasked state loads)
+  // non-zero prescribed values on clamped dofs
+  DevArray<double> ubc_, lrhs_;
+  bool has_ubc_ = false;
+  const uint8_t *state_mask() const { return has_ubc_ ? nomask_.p : levels_[0].d_bcmask.p; }
   DevArray<double> rho_;
   DevArray<double> Ainv_;
   DevArray<int> cb_col0_, cb_nb_;

@@ -84,6 +84,14 @@ class Problem:
         n = len(x)
         check(lib().tmrx_problem_set_design_nodal(self.ptr, p, int(n), dev), "set_design_nodal")
 
+    def set_bc_values(self, ubc):
+        """Prescribed values of the clamped dofs (one entry per dof; entries of
+        free dofs are ignored)."""
+        if not _is_torch(ubc):
+            ubc = np.ascontiguousarray(ubc, dtype=np.double)
+        p, dev = _ptr(ubc)
+        check(lib().tmrx_problem_set_bc_values(self.ptr, p, int(self.ndof), dev), "set_bc_values")
+
     def design_info(self):
         """(mode, gradient length): mode 0 element densities, 1 nodal."""
         m, n = C.c_int(), C.c_int()

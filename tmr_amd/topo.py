@@ -41,9 +41,9 @@ MULTICOLOR_ORDER = 1
 class BoundaryConditions:
     """Named Dirichlet conditions (TMRBoundaryConditions).  ``bc_nums=None``
     constrains every displacement component of the named nodes (TACS
-    ``addBCs`` with no variable list).  Only homogeneous values are built:
-    the operator keeps identity rows on clamped dofs with zero right-hand
-    side."""
+    ``addBCs`` with no variable list), with value zero; ``bc_vals`` (one per
+    listed component, tmr/TMR.pyx:3898-3939) prescribes non-zero values, which
+    the native solve lifts: K_ff u_f = f_f - K_fc u_c."""
 
     def __init__(self):
         self._bcs = []
@@ -54,22 +54,40 @@ class BoundaryConditions:
     def addBoundaryCondition(self, aname, bc_nums=None, bc_vals=None):
         if bc_nums is not None and bc_vals is not None and len(bc_nums) != len(bc_vals):
             raise ValueError("Boundary condition lists must be the same length")
-        if bc_vals is not None and any(float(v) != 0.0 for v in bc_vals):
-            raise NotImplementedError("non-zero boundary values are not supported")
+        vals = [0.0, 0.0, 0.0]
         if bc_nums is None or len(bc_nums) == 0:
             mask = 7
         else:
             mask = 0
-            for n in bc_nums:
+            for k, n in enumerate(bc_nums):
                 n = int(n)
                 if n < 0 or n > 2:
                     raise ValueError("boundary condition variable %d out of range [0, 3)" % n)
                 mask |= 1 << n
-        self._bcs.append((aname, mask))
+                if bc_vals is not None:
+                    vals[n] = float(bc_vals[k])
+        self._bcs.append((aname, mask, vals))
 
     def as_list(self):
         """(name, dof bitmask) pairs, bit i = displacement component i."""
-        return [b for b in self._bcs if b[0] is not None]
+        return [(b[0], b[1]) for b in self._bcs if b[0] is not None]
+
+    def has_values(self):
+        return any(v != 0.0 for b in self._bcs for v in b[2])
+
+    def values_vector(self, forest):
+        """Prescribed value of every dof of the forest's independent nodes
+        (zero where nothing is prescribed; later conditions win)."""
+        u = np.zeros(3 * forest.getNodeRange()[1])
+        for name, mask, vals in self._bcs:
+            if name is None:
+                continue
+            nodes = np.asarray(forest.getNodesWithName(name))
+            nodes = nodes[nodes >= 0]
+            for d in range(3):
+                if mask & (1 << d):
+                    u[3 * nodes + d] = vals[d]
+        return u
 
 
 class MaterialProperties:
@@ -318,6 +336,8 @@ class Mg:
                                       bcs=a0.bcs.as_list(), device=device)
         self.nlevels = len(assemblers)
         a0.problem = self.problem
+        if a0.bcs.has_values():
+            self.problem.set_bc_values(a0.bcs.values_vector(self.forests[0]))
         self.opts = solver.default_opts()
 
     def getAssembler(self):
