@@ -145,6 +145,7 @@ typedef struct {
   int zero_guess;   /* start from u = 0 */
   int fp32_valu;    /* FP32 operator on the vector ALUs instead of the matrix cores */
   int no_graph;     /* enqueue each PCG iteration instead of replaying a captured HIP graph */
+  int cheb_pre_degree; /* pre-smoothing degree (0: cheb_degree); post-smoothing uses cheb_degree */
 } tmrx_solve_opts;
 
 typedef struct {

@@ -113,3 +113,18 @@ def test_uniform_density_invariant(cube):
     prob.set_design(rho)
     ratio = c_half / (c_one * (1 + K0) / (0.5 ** Q + K0))
     assert abs(ratio - 1) < 1e-8
+
+
+@pytest.mark.parametrize("pre,post", [(1, 2), (1, 3), (3, 2)])
+def test_asymmetric_cycles_converge(pre, post, adapt):
+    """Pre- and post-smoothing degrees differ: the first smoothing step lands
+    in the buffer that the cycle's parity requires."""
+    forests, rho, prob = adapt
+    f = _load(prob)
+    orc = _oracle(forests[0], rho)
+    c_ref = float(f @ orc.solve(f, rtol=1e-13))
+    opts = solver.default_opts(rtol=1e-10, precision=1, cheb_degree=post, cheb_pre_degree=pre)
+    prob.solve(f, opts=opts)
+    print("pre %d post %d iters %d" % (pre, post, prob.info.iters), flush=True)
+    assert prob.true_residual() < 1e-9
+    assert abs(prob.compliance() - c_ref) / abs(c_ref) < 1e-6

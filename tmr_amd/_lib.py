@@ -27,7 +27,8 @@ class SolveOpts(C.Structure):
                 ("precision", C.c_int), ("cheb_degree", C.c_int),
                 ("cheb_lower", C.c_double), ("cheb_upper", C.c_double),
                 ("eig_iters", C.c_int), ("monitor", C.c_int), ("zero_guess", C.c_int),
-                ("fp32_valu", C.c_int), ("no_graph", C.c_int)]
+                ("fp32_valu", C.c_int), ("no_graph", C.c_int),
+                ("cheb_pre_degree", C.c_int)]
 
 
 class SolveInfo(C.Structure):

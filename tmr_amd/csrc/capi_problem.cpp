@@ -33,6 +33,7 @@ static tmrx::SolveOpts to_opts(const tmrx_solve_opts *o) {
   s.zero_guess = o->zero_guess;
   s.fp32_valu = o->fp32_valu;
   s.no_graph = o->no_graph;
+  s.cheb_pre_degree = o->cheb_pre_degree;
   return s;
 }
 
@@ -67,6 +68,7 @@ void tmrx_solve_opts_default(tmrx_solve_opts *o) {
   o->zero_guess = s.zero_guess;
   o->fp32_valu = s.fp32_valu;
   o->no_graph = s.no_graph;
+  o->cheb_pre_degree = s.cheb_pre_degree;
 }
 
 int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat, int nbcs,

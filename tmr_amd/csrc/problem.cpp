@@ -1323,6 +1323,13 @@ void Problem::cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const
 }
 
 template <class T>
+T *Problem::first_target(int l, T *x, const SolveOpts &o) {
+  const int deg = std::max(1, o.cheb_degree);
+  const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : deg;
+  return ((pre - 1 + deg) & 1) ? vecs<T>(l).x2.p : x;
+}
+
+template <class T>
 double Problem::cheb_c0(int l, const SolveOpts &o) const {
   const Level &L = levels_[l];
   const double lmax = o.cheb_upper * L.lam_max, lmin = o.cheb_lower * L.lam_max;
@@ -1343,16 +1350,17 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool
   LevelVecs<T> &v = vecs<T>(l);
   LevelVecs<T> &c = vecs<T>(l + 1);
   const int deg = std::max(1, o.cheb_degree);
-  // 2 deg - 1 ping-pong updates in total (odd): start in the spare buffer so
-  // the last post-smoothing update lands in x
-  T *cur = v.x2.p, *oth = x;
-  if (cur == x) std::swap(cur, oth);
+  const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : deg;
+  // pre - 1 + deg ping-pong updates follow the first step: it goes where
+  // the last update then lands in x
+  T *cur = first_target<T>(l, x, o);
+  T *oth = cur == x ? v.x2.p : x;
   // the first step from a zero guess leaves d = x, so the second step reads
   // x for d; the last step of each sweep stores no direction
   if (!first_done)
     launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, cheb_c0<T>(l, o), nullptr, cur, st_, nrhs, run_);
-  for (int k = 1; k < deg; k++) {
-    const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < deg ? CHEB_D_STORE : 0);
+  for (int k = 1; k < pre; k++) {
+    const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < pre ? CHEB_D_STORE : 0);
     cheb_step<T>(l, b, cur, oth, k, o, nrhs, fl);
     std::swap(cur, oth);
   }
@@ -1368,7 +1376,7 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool
   const bool fuse = l + 1 < last;
   launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof,
                      fuse ? c.dinv.p : nullptr, fuse ? cheb_c0<T>(l + 1, o) : 0.0,
-                     fuse ? c.x2.p : nullptr);
+                     fuse ? first_target<T>(l + 1, c.x.p, o) : nullptr);
   vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs, fuse);
   launch_prolong_add<T>(P, L.d_bcmask.p, c.x.p, cur, st_, nrhs, Lc.ndof, L.ndof);
   for (int k = 0; k < deg; k++) {
@@ -1447,7 +1455,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   Level &L0u = levels_[0];
   launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_,
                      mixed ? L0u.v32.dinv.p : nullptr, mixed ? cheb_c0<float>(0, o) : 0.0,
-                     mixed ? L0u.v32.x2.p : nullptr);
+                     mixed ? first_target<float>(0, L0u.v32.x.p, o) : nullptr);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
   // count the segments still running (and advance the iteration counter)
@@ -1541,6 +1549,7 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
     const bool same = g_gen_ == gen_ && g_nrhs_ == nrhs && g_xsol_ == xsol &&
                       g_opts_.precision == o.precision &&
                       g_opts_.fp32_valu == o.fp32_valu && g_opts_.cheb_degree == o.cheb_degree &&
+                      g_opts_.cheb_pre_degree == o.cheb_pre_degree &&
                       g_opts_.cheb_lower == o.cheb_lower && g_opts_.cheb_upper == o.cheb_upper &&
                       g_opts_.rtol == o.rtol && g_opts_.atol == o.atol;
     if (!same || !gexec_[0] || !gexec_[1]) {

@@ -84,6 +84,7 @@ struct SolveOpts {
   int zero_guess = 1;
   int fp32_valu = 0;  // FP32 operator on the vector ALUs (else matrix cores)
   int no_graph = 0;   // enqueue every PCG iteration instead of replaying a HIP graph
+  int cheb_pre_degree = 0;  // pre-smoothing degree (0: cheb_degree)
 };
 
 struct SolveInfo {
@@ -237,6 +238,9 @@ class Problem {
   void vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs = 1, bool first_done = false);
   template <class T>
   double cheb_c0(int l, const SolveOpts &o) const;
+  // buffer of level l that the first smoothing step writes (x or its spare)
+  template <class T>
+  T *first_target(int l, T *x, const SolveOpts &o);
   void precond(int nrhs, const SolveOpts &o);
   int coarse_factor();
   int find_components();
