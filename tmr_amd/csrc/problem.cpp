@@ -2012,10 +2012,10 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       best_mfma = mfma;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   fp32_valu_ = keep_valu;
   if (best < 0) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     last_error = "autotune: no candidate tile size fits in LDS";
     best = -1;
     for (size_t l = 0; l < levels_.size(); l++) {
@@ -2032,6 +2032,50 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
     levels_[l].tile_nodes = l == 0 ? cands[best] : coarse_tile_nodes(cands[best], levels_[l].n_nodes);
     if (build_tiles((int)l)) return -1;
   }
+  // coarse levels large enough to fill the GPU: their own tile size, by the
+  // time of the FP32 product on the vector ALUs at that level
+  for (size_t l = 1; l + 1 < levels_.size(); l++) {
+    Level &Lc = levels_[l];
+    if (Lc.n_nodes < 65536) break;
+    if (ensure_vecs((int)l, 1)) return -1;
+    launch_fill<float>(Lc.ndof, 1.0f, Lc.v32.b.p, st_);
+    const int keep = Lc.tile_nodes;
+    int bsz = keep;
+    double bms = -1.0;
+    AsmArgs ac;
+    ac.y = Lc.v32.r.p;
+    const bool kv = fp32_valu_;
+    fp32_valu_ = true;
+    for (int cand : {128, 256, 512, 1024}) {
+      if ((Lc.n_nodes + cand - 1) / cand < 256) continue;  // at least one tile per CU
+      Lc.tile_nodes = cand;
+      if (build_tiles((int)l)) return -1;
+      const Tiles &tl = Lc.tiles;
+      const size_t lds64 = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
+      if (lds64 + 64 > (size_t)max_lds) continue;
+      apply_level<float>((int)l, Lc.v32.b.p, ASM_STORE, ac);
+      (void)hipEventRecord(e0, st_);
+      for (int r = 0; r < reps; r++) apply_level<float>((int)l, Lc.v32.b.p, ASM_STORE, ac);
+      (void)hipEventRecord(e1, st_);
+      float ms = 0.0f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        fp32_valu_ = kv;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        last_error = "autotune: timing failed";
+        return -1;
+      }
+      if (bms < 0 || ms < bms) {
+        bms = ms;
+        bsz = cand;
+      }
+    }
+    fp32_valu_ = kv;
+    Lc.tile_nodes = bsz;
+    if (build_tiles((int)l)) return -1;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   // tiles moved: reduction plan, partial buffers and captured graphs follow
   if (build_plan()) return -1;
   const int cap = std::max(1, batch_cap_);
