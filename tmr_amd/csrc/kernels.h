@@ -134,6 +134,10 @@ struct TransferCSR {  // rows = output nodes
   const int *col = nullptr;
   const float *w = nullptr;
   const int *run = nullptr;  // as Tiles::run
+  // per fine node: 1 + e when its prolongation row has 2^e equal weights
+  // 2^-e (the weights are then not read), 0 otherwise; indexed by row for
+  // P and by column for R = P^T
+  const uint8_t *wexp = nullptr;
 };
 
 // y-side operation performed by the assembly pass

@@ -319,16 +319,19 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
                                                      const uint8_t *__restrict__ mask,
                                                      const T *__restrict__ xc, T *__restrict__ xf,
                                                      int64_t vsc, int64_t vsf,
-                                                     const int *__restrict__ run) {
+                                                     const int *__restrict__ run,
+                                                     const uint8_t *__restrict__ wexp) {
   if (run && *run == 0) return;
   const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (i >= n) return;
   xc += (int64_t)blockIdx.y * vsc;
   xf += (int64_t)blockIdx.y * vsf;
+  const int e = wexp ? (int)wexp[i] : 0;
+  const T wu = e ? (T)ldexpf(1.0f, 1 - e) : T(0);  // equal weights 2^-(e-1)
   T a0 = T(0), a1 = T(0), a2 = T(0);
   for (int j = ptr[i]; j < ptr[i + 1]; j++) {
     const int c = col[j];
-    const T ww = (T)w[j];
+    const T ww = e ? wu : (T)w[j];
     a0 += ww * xc[3 * c];
     a1 += ww * xc[3 * c + 1];
     a2 += ww * xc[3 * c + 2];
@@ -350,7 +353,8 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
                                                   int64_t vsf, int64_t vsc,
                                                   const int *__restrict__ run,
                                                   const T *__restrict__ dinv_c, double c0,
-                                                  T *__restrict__ x1) {
+                                                  T *__restrict__ x1,
+                                                  const uint8_t *__restrict__ wexp) {
   if (run && *run == 0) return;
   const int g = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   rf += (int64_t)blockIdx.y * vsf;
@@ -361,7 +365,8 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   if (i < n) {
     for (int j = ptr[i] + sub; j < ptr[i + 1]; j += 4) {
       const int c = col[j];
-      const T ww = (T)w[j];
+      const int e = wexp ? (int)wexp[c] : 0;
+      const T ww = e ? (T)ldexpf(1.0f, 1 - e) : (T)w[j];
       a0 += ww * rf[3 * c];
       a1 += ww * rf[3 * c + 1];
       a2 += ww * rf[3 * c + 2];
@@ -392,7 +397,7 @@ void launch_prolong_add(const TransferCSR &P, const uint8_t *mask_f, const T *xc
                         hipStream_t st, int nrhs, int64_t vsc, int64_t vsf) {
   if (P.n_rows == 0) return;
   hipLaunchKernelGGL(k_prolong_add<T>, dim3((P.n_rows + 255) / 256, nrhs), dim3(256), 0, st,
-                     P.n_rows, P.ptr, P.col, P.w, mask_f, xc, xf, vsc, vsf, P.run);
+                     P.n_rows, P.ptr, P.col, P.w, mask_f, xc, xf, vsc, vsf, P.run, P.wexp);
   TMRX_LAUNCH_CHECK();
 }
 
@@ -404,7 +409,7 @@ void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T
   const int64_t threads = 4 * (int64_t)R.n_rows;
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256), nrhs), dim3(256), 0,
                      st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc, R.run, dinv_c, c0,
-                     x1);
+                     x1, R.wexp);
   TMRX_LAUNCH_CHECK();
 }
 

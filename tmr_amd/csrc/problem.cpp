@@ -769,6 +769,19 @@ int Problem::build_transfer(int l) {
     return -1;
   }
   std::vector<float> pw(P.w.begin(), P.w.end());
+  // rows of 2^e equal weights 2^-e (every row of a uniformly refined region)
+  // carry 1 + e instead of being read weight by weight
+  std::vector<uint8_t> wexp(P.nrows, 0);
+  for (int r = 0; r < P.nrows; r++) {
+    const int len = P.rowp[r + 1] - P.rowp[r];
+    int e = 0;
+    while ((1 << e) < len) e++;
+    if (len < 1 || (1 << e) != len || e > 30) continue;
+    bool eq = true;
+    for (int j = P.rowp[r]; j < P.rowp[r + 1]; j++)
+      if (P.w[j] != std::ldexp(1.0, -e)) eq = false;
+    if (eq) wexp[r] = (uint8_t)(1 + e);
+  }
   // transpose (restriction), rows in coarse order, entries in fine order
   std::vector<int> rptr(Cc.n_nodes + 1, 0);
   for (int c : P.cols) rptr[c + 1]++;
@@ -808,14 +821,14 @@ int Problem::build_transfer(int l) {
       vw[fill[E]++] = (float)std::ldexp(1.0, -3 * (F.lev[e] - Cc.lev[E]));
     }
   }
-  if (F.P_ptr.upload(P.rowp) || F.P_col.upload(P.cols) || F.P_w.upload(pw) ||
+  if (F.P_ptr.upload(P.rowp) || F.P_col.upload(P.cols) || F.P_w.upload(pw) || F.P_wexp.upload(wexp) ||
       F.R_ptr.upload(rptr) || F.R_col.upload(rcol) || F.R_w.upload(rw) ||
       F.sc_ptr.upload(cnt) || F.sc_child.upload(child) || F.sc_w.upload(vw)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
-  F.P = {F.n_nodes, F.P_ptr.p, F.P_col.p, F.P_w.p};
-  F.R = {Cc.n_nodes, F.R_ptr.p, F.R_col.p, F.R_w.p};
+  F.P = {F.n_nodes, F.P_ptr.p, F.P_col.p, F.P_w.p, nullptr, F.P_wexp.p};
+  F.R = {Cc.n_nodes, F.R_ptr.p, F.R_col.p, F.R_w.p, nullptr, F.P_wexp.p};
   return 0;
 }
 

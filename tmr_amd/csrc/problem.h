@@ -137,6 +137,7 @@ struct Level {
   // transfer to the next coarser level (P: rows = this level's nodes)
   DevArray<int> P_ptr, P_col, R_ptr, R_col;
   DevArray<float> P_w, R_w;
+  DevArray<uint8_t> P_wexp;
   TransferCSR P, R;
   // element-scale restriction from this level into the next coarser one
   // (CSR over the coarse elements)
