@@ -1335,6 +1335,13 @@ __device__ __forceinline__ void lds_add3(T *p, T a, T b, T c) {
     v.y += b;
     v.z += c;
     *reinterpret_cast<float4 *>(p) = v;
+  } else if constexpr (sizeof(T) == 8) {
+    // one writer per slot within a colour phase: a no-return LDS atomic add
+    // is the plain read-add-write in one instruction (measured faster for
+    // FP64; the FP32 LDS atomic is several times slower than read + write)
+    __hip_atomic_fetch_add(p, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p + 2, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   } else {
     p[0] += a;
     p[1] += b;
