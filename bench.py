@@ -106,7 +106,7 @@ def bench_multirhs(args):
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
     if not args.no_autotune:
-        prob.autotune(TILE_CANDIDATES, reps=3)
+        prob.autotune(args.tiles, reps=3)
     rng = np.random.default_rng(0)
     F = np.stack([prob.traction_load("load", rng.standard_normal(3)) for _ in range(nrhs)])
     Fd = torch.from_numpy(F).cuda()
@@ -158,7 +158,7 @@ def bench_batch(args):
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
     if not args.no_autotune:
-        prob.autotune(TILE_CANDIDATES, reps=3)
+        prob.autotune(args.tiles, reps=3)
     f = torch.from_numpy(prob.traction_load("load", [0.0, 0.0, -1.0])).cuda()
     u = torch.zeros_like(f)
     opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=500, no_graph=int(args.no_graph))
@@ -229,7 +229,10 @@ def main():
                     help="after the solves, copy MIB MiB device to device (a known-size "
                          "reference for memory-traffic counters)")
     ap.add_argument("--count", type=int, default=96)
+    ap.add_argument("--tiles", default=",".join(map(str, TILE_CANDIDATES)),
+                    help="owner-tile sizes (nodes) the autotuner tries, comma-separated")
     args = ap.parse_args()
+    args.tiles = tuple(int(v) for v in args.tiles.split(","))
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
@@ -258,7 +261,7 @@ def main():
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        tune = prob.autotune(TILE_CANDIDATES, reps=3)
+        tune = prob.autotune(args.tiles, reps=3)
         log("autotuned owner tiles: %d nodes (ms per 4 FP32 + 1 FP64 products: %s) in %.1f s"
             % (tune[0], tune[1], time.perf_counter() - t0))
 

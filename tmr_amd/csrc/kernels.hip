@@ -1047,6 +1047,21 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
 // hanging-node interpolation, clamped dofs zero) and zero the accumulators;
 // S floats per node (S = 4 pads each node to 16 bytes with a zero)
+// the three components of one node as a single 12-byte (FP32) or
+// 24-byte (FP64) load
+template <class T>
+struct __attribute__((packed, aligned(sizeof(T)))) Vec3 {
+  T a, b, c;
+};
+template <class T>
+__device__ __forceinline__ Vec3<T> load_vec3(const T *__restrict__ p) {
+  return *reinterpret_cast<const Vec3<T> *>(p);
+}
+template <class T>
+__device__ __forceinline__ void store_vec3(T *__restrict__ p, const T v[3]) {
+  *reinterpret_cast<Vec3<T> *>(p) = Vec3<T>{v[0], v[1], v[2]};
+}
+
 template <class T, int S, int NT = TILE_THREADS, int SY = S>
 __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
                                            int nacc, const int *__restrict__ dep_ptr,
@@ -1055,25 +1070,27 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
                                            const uint8_t *__restrict__ bcmask,
                                            const T *__restrict__ x, T *xs, T *ys) {
   const int tid = threadIdx.x;
-  const int n3 = 3 * nown;
-  // owned values: eight loads in flight per thread, issued before any use
-  // (indices clamped into the tile's range, so the loads are unconditional)
-  constexpr int VO = 8;
-  for (int base = 0; base < n3; base += VO * NT) {
-    T v[VO];
-    uint8_t mk[VO];
+  // owned nodes: one 3-component load and one mask byte per node, two nodes
+  // in flight per thread (indices clamped into the tile, so the loads are
+  // unconditional and issued before any is used)
+  constexpr int VN = 2;
+  for (int j0 = tid; j0 < nown; j0 += VN * NT) {
+    Vec3<T> v[VN];
+    uint8_t mk[VN];
 #pragma unroll
-    for (int k = 0; k < VO; k++) {
-      const int i = min(base + k * NT + tid, n3 - 1);
-      v[k] = x[3 * (size_t)n0 + i];
-      mk[k] = bcmask[n0 + i / 3];
+    for (int k = 0; k < VN; k++) {
+      const int j = min(j0 + k * NT, nown - 1);
+      v[k] = load_vec3(x + 3 * (size_t)(n0 + j));
+      mk[k] = bcmask[n0 + j];
     }
 #pragma unroll
-    for (int k = 0; k < VO; k++) {
-      const int i = base + k * NT + tid;
-      if (i < n3) {
-        const int j = i / 3, c = i - 3 * j;
-        xs[S * tile_pos(j) + c] = ((mk[k] >> c) & 1) ? T(0) : v[k];
+    for (int k = 0; k < VN; k++) {
+      const int j = j0 + k * NT;
+      if (j < nown) {
+        const int q = S * tile_pos(j);
+        xs[q] = (mk[k] & 1) ? T(0) : v[k].a;
+        xs[q + 1] = (mk[k] & 2) ? T(0) : v[k].b;
+        xs[q + 2] = (mk[k] & 4) ? T(0) : v[k].c;
       }
     }
   }
@@ -1084,31 +1101,25 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
     int g[VH];
 #pragma unroll
     for (int k = 0; k < VH; k++) g[k] = ext_l[min(i0 + k * NT, next - 1)];
-    T v[VH][3];
+    Vec3<T> w[VH];
     uint8_t mk[VH];
 #pragma unroll
     for (int k = 0; k < VH; k++) {
       const int gg = g[k] >= 0 ? g[k] : 0;
-      v[k][0] = x[3 * (size_t)gg];
-      v[k][1] = x[3 * (size_t)gg + 1];
-      v[k][2] = x[3 * (size_t)gg + 2];
+      w[k] = load_vec3(x + 3 * (size_t)gg);
       mk[k] = bcmask[gg];
     }
 #pragma unroll
     for (int k = 0; k < VH; k++) {
       const int i = i0 + k * NT;
       if (i < next) {
-        if (g[k] < 0) {
-          load_node<T>(g[k], x, dep_ptr, dep_conn, dep_w, bcmask, v[k]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 3; c++)
-            if ((mk[k] >> c) & 1) v[k][c] = T(0);
-        }
+        T v[3] = {(mk[k] & 1) ? T(0) : w[k].a, (mk[k] & 2) ? T(0) : w[k].b,
+                  (mk[k] & 4) ? T(0) : w[k].c};
+        if (g[k] < 0) load_node<T>(g[k], x, dep_ptr, dep_conn, dep_w, bcmask, v);
         const int q = S * tile_pos(nown + i);
-        xs[q] = v[k][0];
-        xs[q + 1] = v[k][1];
-        xs[q + 2] = v[k][2];
+        xs[q] = v[0];
+        xs[q + 1] = v[1];
+        xs[q + 2] = v[2];
       }
     }
   }
@@ -1146,19 +1157,18 @@ __device__ __forceinline__ void tile_finish(
     const int node = n0 + i;
     const int ti = tile_pos(i);
     const int pi = S * ti;
-    // the epilogue's vector loads go out before the hanging-node sums
-    T bv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)}, wv[3] = {T(0), T(0), T(0)};
-    T xv[3] = {T(0), T(0), T(0)};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const size_t k = 3 * (size_t)node + c;
-      if (MODE == ASM_RESID || MODE == ASM_CHEB) bv[c] = b[k];
-      if (MODE == ASM_CHEB) {
-        if (cflags & CHEB_D_READ) dv[c] = d[k];
-        wv[c] = dinv[k];
-      }
-      if (MODE == ASM_STORE || MODE == ASM_DOT) xv[c] = x[k];
+    // the epilogue's vector loads (one per node and vector) go out before
+    // the hanging-node sums
+    const size_t k3 = 3 * (size_t)node;
+    Vec3<T> bl{T(0), T(0), T(0)}, dl{T(0), T(0), T(0)}, wl{T(0), T(0), T(0)}, xl{T(0), T(0), T(0)};
+    if (MODE == ASM_RESID || MODE == ASM_CHEB) bl = load_vec3(b + k3);
+    if (MODE == ASM_CHEB) {
+      if (cflags & CHEB_D_READ) dl = load_vec3(d + k3);
+      wl = load_vec3(dinv + k3);
     }
+    if (MODE == ASM_STORE || MODE == ASM_DOT) xl = load_vec3(x + k3);
+    const T bv[3] = {bl.a, bl.b, bl.c}, dv[3] = {dl.a, dl.b, dl.c}, wv[3] = {wl.a, wl.b, wl.c};
+    const T xv[3] = {xl.a, xl.b, xl.c};
     const uint8_t m = bcmask[node];
     T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
     for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
@@ -1168,32 +1178,39 @@ __device__ __forceinline__ void tile_finish(
       a[1] += w * ys[S * sl + 1];
       a[2] += w * ys[S * sl + 2];
     }
+    T yo[3], dno[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const size_t k = 3 * (size_t)node + c;
       const bool bc = (m >> c) & 1;
       if (MODE == ASM_STORE) {
-        y[k] = bc ? xv[c] : a[c];
+        yo[c] = bc ? xv[c] : a[c];
       } else if (MODE == ASM_DOT) {
         const T xx = bc ? xv[c] : xat(ti, c);
         const T v = bc ? xx : a[c];
-        y[k] = v;
+        yo[c] = v;
         dot += (double)xx * (double)v;
       } else if (MODE == ASM_RESID) {
-        y[k] = bc ? T(0) : bv[c] - a[c];
+        yo[c] = bc ? T(0) : bv[c] - a[c];
       } else if (MODE == ASM_CHEB) {
         if (bc) {
-          if (cflags & CHEB_D_STORE) d[k] = T(0);
-          x_out[k] = T(0);
+          dno[c] = T(0);
+          yo[c] = T(0);
         } else {
           const T r = bv[c] - a[c];
           const T xc = xat(ti, c);
           const T dp = (cflags & CHEB_D_IS_X) ? xc : dv[c];
           const T dn = (T)ca * dp + (T)cc * wv[c] * r;
-          if (cflags & CHEB_D_STORE) d[k] = dn;
-          x_out[k] = xc + dn;
+          dno[c] = dn;
+          yo[c] = xc + dn;
         }
       }
+    }
+    // one store per node and vector
+    if (MODE == ASM_CHEB) {
+      if (cflags & CHEB_D_STORE) store_vec3(d + k3, dno);
+      store_vec3(x_out + k3, yo);
+    } else {
+      store_vec3(y + k3, yo);
     }
   }
   if (MODE == ASM_DOT) {
