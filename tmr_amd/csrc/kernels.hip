@@ -27,6 +27,21 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 // ---------------------------------------------------------------------------
 // element pass: stage batch nodes in LDS, y_e = s_e K0 x_e, colored LDS
 // accumulation, one partial per slot.
+// the three components of one node as a single 12-byte (FP32) or
+// 24-byte (FP64) load
+template <class T>
+struct __attribute__((packed, aligned(sizeof(T)))) Vec3 {
+  T a, b, c;
+};
+template <class T>
+__device__ __forceinline__ Vec3<T> load_vec3(const T *__restrict__ p) {
+  return *reinterpret_cast<const Vec3<T> *>(p);
+}
+template <class T>
+__device__ __forceinline__ void store_vec3(T *__restrict__ p, const T v[3]) {
+  *reinterpret_cast<Vec3<T> *>(p) = Vec3<T>{v[0], v[1], v[2]};
+}
+
 template <class T>
 __device__ __forceinline__ void load_node(int e, const T *__restrict__ x,
                                           const int *__restrict__ dep_ptr,
@@ -35,9 +50,10 @@ __device__ __forceinline__ void load_node(int e, const T *__restrict__ x,
                                           const uint8_t *__restrict__ bcmask, T v[3]) {
   if (e >= 0) {
     const uint8_t m = bcmask[e];
-    v[0] = (m & 1) ? T(0) : x[3 * e];
-    v[1] = (m & 2) ? T(0) : x[3 * e + 1];
-    v[2] = (m & 4) ? T(0) : x[3 * e + 2];
+    const Vec3<T> u = load_vec3(x + 3 * (size_t)e);
+    v[0] = (m & 1) ? T(0) : u.a;
+    v[1] = (m & 2) ? T(0) : u.b;
+    v[2] = (m & 4) ? T(0) : u.c;
   } else {
     const int d = -e - 1;
     v[0] = v[1] = v[2] = T(0);
@@ -45,9 +61,10 @@ __device__ __forceinline__ void load_node(int e, const T *__restrict__ x,
       const int p = dep_conn[j];
       const T w = (T)dep_w[j];
       const uint8_t m = bcmask[p];
-      if (!(m & 1)) v[0] += w * x[3 * p];
-      if (!(m & 2)) v[1] += w * x[3 * p + 1];
-      if (!(m & 4)) v[2] += w * x[3 * p + 2];
+      const Vec3<T> u = load_vec3(x + 3 * (size_t)p);
+      if (!(m & 1)) v[0] += w * u.a;
+      if (!(m & 2)) v[1] += w * u.b;
+      if (!(m & 4)) v[2] += w * u.c;
     }
   }
 }
@@ -329,17 +346,18 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
   const int e = wexp ? (int)wexp[i] : 0;
   const T wu = e ? (T)ldexpf(1.0f, 1 - e) : T(0);  // equal weights 2^-(e-1)
   T a0 = T(0), a1 = T(0), a2 = T(0);
+  const Vec3<T> f = load_vec3(xf + 3 * (size_t)i);
   for (int j = ptr[i]; j < ptr[i + 1]; j++) {
     const int c = col[j];
     const T ww = e ? wu : (T)w[j];
-    a0 += ww * xc[3 * c];
-    a1 += ww * xc[3 * c + 1];
-    a2 += ww * xc[3 * c + 2];
+    const Vec3<T> u = load_vec3(xc + 3 * (size_t)c);
+    a0 += ww * u.a;
+    a1 += ww * u.b;
+    a2 += ww * u.c;
   }
   const uint8_t m = mask[i];
-  if (!(m & 1)) xf[3 * i] += a0;
-  if (!(m & 2)) xf[3 * i + 1] += a1;
-  if (!(m & 4)) xf[3 * i + 2] += a2;
+  const T o[3] = {(m & 1) ? f.a : f.a + a0, (m & 2) ? f.b : f.b + a1, (m & 4) ? f.c : f.c + a2};
+  store_vec3(xf + 3 * (size_t)i, o);
 }
 
 // 4 lanes per coarse row: rows gather ~27 scattered fine nodes, so spread
@@ -367,9 +385,10 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
       const int c = col[j];
       const int e = wexp ? (int)wexp[c] : 0;
       const T ww = e ? (T)ldexpf(1.0f, 1 - e) : (T)w[j];
-      a0 += ww * rf[3 * c];
-      a1 += ww * rf[3 * c + 1];
-      a2 += ww * rf[3 * c + 2];
+      const Vec3<T> u = load_vec3(rf + 3 * (size_t)c);
+      a0 += ww * u.a;
+      a1 += ww * u.b;
+      a2 += ww * u.c;
     }
   }
   a0 += __shfl_xor(a0, 1, 64);
@@ -380,14 +399,12 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   a2 += __shfl_xor(a2, 2, 64);
   if (i < n && sub == 0) {
     const uint8_t m = mask[i];
-    const T v0 = (m & 1) ? T(0) : a0, v1 = (m & 2) ? T(0) : a1, v2 = (m & 4) ? T(0) : a2;
-    rc[3 * i] = v0;
-    rc[3 * i + 1] = v1;
-    rc[3 * i + 2] = v2;
+    const T v[3] = {(m & 1) ? T(0) : a0, (m & 2) ? T(0) : a1, (m & 4) ? T(0) : a2};
+    store_vec3(rc + 3 * (size_t)i, v);
     if (x1) {  // first Chebyshev step of the coarse level from zero
-      x1[3 * i] = (T)c0 * dinv_c[3 * i] * v0;
-      x1[3 * i + 1] = (T)c0 * dinv_c[3 * i + 1] * v1;
-      x1[3 * i + 2] = (T)c0 * dinv_c[3 * i + 2] * v2;
+      const Vec3<T> di = load_vec3(dinv_c + 3 * (size_t)i);
+      const T u[3] = {(T)c0 * di.a * v[0], (T)c0 * di.b * v[1], (T)c0 * di.c * v[2]};
+      store_vec3(x1 + 3 * (size_t)i, u);
     }
   }
 }
@@ -1047,21 +1064,6 @@ void launch_dinv_dots(int64_t n, const double *dinv, double *y, const double *x,
 // stage x of the tile's slots into LDS (owned nodes in place, halo nodes with
 // hanging-node interpolation, clamped dofs zero) and zero the accumulators;
 // S floats per node (S = 4 pads each node to 16 bytes with a zero)
-// the three components of one node as a single 12-byte (FP32) or
-// 24-byte (FP64) load
-template <class T>
-struct __attribute__((packed, aligned(sizeof(T)))) Vec3 {
-  T a, b, c;
-};
-template <class T>
-__device__ __forceinline__ Vec3<T> load_vec3(const T *__restrict__ p) {
-  return *reinterpret_cast<const Vec3<T> *>(p);
-}
-template <class T>
-__device__ __forceinline__ void store_vec3(T *__restrict__ p, const T v[3]) {
-  *reinterpret_cast<Vec3<T> *>(p) = Vec3<T>{v[0], v[1], v[2]};
-}
-
 template <class T, int S, int NT = TILE_THREADS, int SY = S>
 __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restrict__ ext_l, int next,
                                            int nacc, const int *__restrict__ dep_ptr,
@@ -1125,7 +1127,11 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
   }
   if (S == 4)
     for (int i = tid; i < ((nown + next + 7) & ~7); i += NT) xs[4 * i + 3] = T(0);
-  for (int i = tid; i < SY * ((nacc + 7) & ~7); i += NT) ys[i] = T(0);
+  // accumulators: SY * round8(nacc) values, a multiple of 16 bytes starting
+  // 16-byte aligned, cleared 16 bytes at a time
+  uint4 *y16 = reinterpret_cast<uint4 *>(ys);
+  const int n16 = (int)((SY * ((nacc + 7) & ~7) * sizeof(T)) / 16);
+  for (int i = tid; i < n16; i += NT) y16[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // owned rows: add the hanging-node sums, then the mode's epilogue
@@ -1142,7 +1148,7 @@ __device__ __forceinline__ void tile_finish(
     const float *__restrict__ comb_w, const uint8_t *__restrict__ bcmask, const T *xs, const T *ys,
     const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ y, T *__restrict__ x_out,
     T *__restrict__ d, const T *__restrict__ dinv, double ca, double cc, int cflags,
-    double *__restrict__ partials, double *red, int pstride = 0) {
+    double *__restrict__ partials, double *red, int pstride = 0, bool hang = true) {
   auto xat = [&](int pos, int c) -> T {
     if constexpr (SPLIT) {
       const uint2 *ph = reinterpret_cast<const uint2 *>(xs);
@@ -1171,7 +1177,8 @@ __device__ __forceinline__ void tile_finish(
     const T xv[3] = {xl.a, xl.b, xl.c};
     const uint8_t m = bcmask[node];
     T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
-    for (int j = comb_ptr[node]; j < comb_ptr[node + 1]; j++) {
+    const int j0 = hang ? comb_ptr[node] : 0, j1 = hang ? comb_ptr[node + 1] : 0;
+    for (int j = j0; j < j1; j++) {
       const int sl = comb_slot[j];
       const T w = (T)comb_w[j];
       a[0] += w * ys[S * sl];
@@ -1456,7 +1463,7 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
   }
   tile_finish<T, MODE, S, NT, false, SX>(t, n0, nown, comb_ptr, comb_slot, comb_w,
                                                    bcmask, xs, ys, x, b, y, x_out, d, dinv, ca,
-                                                   cc, cflags, partials, red);
+                                                   cc, cflags, partials, red, 0, nacc > nown);
 }
 
 // ---------------------------------------------------------------------------
