@@ -119,6 +119,13 @@ constexpr int NODAL_TABLE = 196;
 // coefficients X_0[s^1], X_1[s^2], X_2[s^4], scaled by 1/64.  The
 // element-density tile apply takes this table in place of K0.
 constexpr int REFL_TABLE = 72;
+// Structural non-zeros of those blocks (bit 3 r + c for row r, column c of
+// block s) for a box element of an isotropic material, any aspect ratio:
+// the rigid translations X_0[0], X_1[0], X_2[0] span the null rows and
+// columns of blocks 1, 2 and 4, and blocks 3, 5 and 6 decouple one of their
+// three coefficients.  45 of the 72 entries remain.
+constexpr unsigned short REFL_NZ[8] = {0x1FF, 0x1B0, 0x145, 0x11B, 0x01B, 0x155, 0x1B1, 0x1FF};
+__host__ __device__ constexpr bool refl_nz(int s, int k) { return (REFL_NZ[s] >> k) & 1; }
 
 
 struct AsmCSR {

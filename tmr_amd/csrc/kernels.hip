@@ -1248,8 +1248,9 @@ __device__ __forceinline__ void wht8(T v[8]) {
 // block-diagonal -- for every character s in {0..7} one 3x3 block coupling
 // X_0[s^1], X_1[s^2], X_2[s^4].  M holds the 8 blocks (72 entries, the
 // transform normalisation 1/64 folded in): 72 FMAs and 144 add/subtracts
-// in place of the 576 FMAs of the dense product.
-template <class T>
+// in place of the 576 FMAs of the dense product.  With SCALED false the
+// product is K0 x_e, and the caller applies s in its accumulation FMAs.
+template <class T, bool SCALED = true>
 __device__ __forceinline__ void refl_elem(const T *__restrict__ M, T sc, const T xe[24], T ye[24]) {
   T X0[8], X1[8], X2[8];
 #pragma unroll
@@ -1265,10 +1266,21 @@ __device__ __forceinline__ void refl_elem(const T *__restrict__ M, T sc, const T
 #pragma unroll
   for (int s = 0; s < 8; s++) {
     const T *__restrict__ m = M + 9 * s;
-    const T v0 = sc * X0[s ^ 1], v1 = sc * X1[s ^ 2], v2 = sc * X2[s ^ 4];
-    Y0[s ^ 1] = m[0] * v0 + m[1] * v1 + m[2] * v2;
-    Y1[s ^ 2] = m[3] * v0 + m[4] * v1 + m[5] * v2;
-    Y2[s ^ 4] = m[6] * v0 + m[7] * v1 + m[8] * v2;
+    const T v[3] = {SCALED ? sc * X0[s ^ 1] : X0[s ^ 1], SCALED ? sc * X1[s ^ 2] : X1[s ^ 2],
+                    SCALED ? sc * X2[s ^ 4] : X2[s ^ 4]};
+    T o[3];
+    // only the structural non-zeros (compile-time pattern, 45 of 72)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      T acc = T(0);
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        if (refl_nz(s, 3 * r + c)) acc += m[3 * r + c] * v[c];
+      o[r] = acc;
+    }
+    Y0[s ^ 1] = o[0];
+    Y1[s ^ 2] = o[1];
+    Y2[s ^ 4] = o[2];
   }
   wht8(Y0);
   wht8(Y1);
@@ -1351,6 +1363,13 @@ __device__ __forceinline__ void lds_get3(const T *p, T &a, T &b, T &c) {
   }
 }
 
+// p[0..2] += s * (a, b, c), one FMA per value (FP32 vector-ALU accumulation)
+__device__ __forceinline__ void lds_fma3(float *p, float s, float a, float b, float c) {
+  p[0] = __builtin_fmaf(s, a, p[0]);
+  p[1] = __builtin_fmaf(s, b, p[1]);
+  p[2] = __builtin_fmaf(s, c, p[2]);
+}
+
 template <class T, int S>
 __device__ __forceinline__ void lds_add3(T *p, T a, T b, T c) {
   if constexpr (S == 4 && sizeof(T) == 4) {
@@ -1388,6 +1407,10 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
   if (run && *run == 0) return;
   constexpr int S = tile_stride<T>();
   constexpr int SX = tile_xstride<T>();
+  // FP32 element-density products scale in the accumulation FMAs (24 fewer
+  // multiplies per record); FP64 accumulates by LDS atomic adds, so it
+  // scales inside the product
+  constexpr bool FMA_ACC = sizeof(T) == 4 && !NODAL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[NT / 64];
   const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -1421,6 +1444,7 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
   for (int r0 = 0; r0 < nel; r0 += NT) {
     const int k = r0 + tid;
     T ye[24];
+    T sc = T(0);
     int lc[8];
     int mask = 0, col = -1;
     if (k < nel) {
@@ -1439,7 +1463,8 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
       if constexpr (NODAL) {
         nodal_elem<T>(K0 + NODAL_TABLE * cls, s_elem + 8 * (size_t)elem, xe, mask, ye);
       } else {
-        refl_elem<T>(K0 + REFL_TABLE * cls, s_elem[elem], xe, ye);
+        sc = s_elem[elem];
+        refl_elem<T, !FMA_ACC>(K0 + REFL_TABLE * cls, sc, xe, ye);
       }
     }
     // next round's records load while this round accumulates
@@ -1453,10 +1478,16 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
     const int c_lo = (int)((el[3 * (size_t)(q0 + r0) + 2].y >> 8) & 0xff);
     const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + NT, nel) - 1) + 2].y >> 8) & 0xff);
     for (int c = c_lo; c <= c_hi; c++) {
+      auto acc = [&](int a) {
+        if constexpr (FMA_ACC)
+          lds_fma3(ys + S * lc[a], sc, ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
+        else
+          lds_add3<T, S>(ys + S * lc[a], ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
+      };
       if (col == c) {
 #pragma unroll
         for (int a = 0; a < 8; a++)
-          if (mask & (1 << a)) lds_add3<T, S>(ys + S * lc[a], ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
+          if (mask & (1 << a)) acc(a);
       }
       __syncthreads();
     }

@@ -178,9 +178,9 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
               for (int a = 0; a < 8; a++)
                 for (int b = 0; b < 8; b++) v += Hs(t, a) * K[24 * (3 * a + c) + 3 * b + cp] * Hs(tp, b);
               const int sv = t ^ (1 << c);
-              if (sv == (tp ^ (1 << cp)))
+              if (sv == (tp ^ (1 << cp)) && refl_nz(sv, 3 * c + cp))
                 rt[REFL_TABLE * (size_t)cl + 9 * sv + 3 * c + cp] = v / 64.0;
-              else
+              else  // off the blocks, or a structural zero the kernels skip
                 off = std::max(off, std::fabs(v));
             }
       if (off > 1e-12 * 64.0 * kmax) {
