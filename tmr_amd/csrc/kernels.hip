@@ -1467,11 +1467,15 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
         refl_elem<T, !FMA_ACC>(K0 + REFL_TABLE * cls, sc, xe, ye);
       }
     }
-    // next round's records load while this round accumulates
-    if (k + NT < nel) {
-      w0 = el[3 * (size_t)(q0 + k + NT)];
-      w1 = el[3 * (size_t)(q0 + k + NT) + 1];
-      w2 = el[3 * (size_t)(q0 + k + NT) + 2];
+    // next round's records load while this round accumulates; the index is
+    // clamped to the tile's last record (nel >= 1 inside the loop), so the
+    // loads are unconditional and the waits for this round's scale need not
+    // drain them
+    {
+      const size_t kn = 3 * (size_t)(q0 + min(k + NT, nel - 1));
+      w0 = el[kn];
+      w1 = el[kn + 1];
+      w2 = el[kn + 2];
     }
     // elements are sorted by colour: this round only spans the colours of
     // its first and last element, and most waves hold a single colour
@@ -1479,10 +1483,11 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
     const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + NT, nel) - 1) + 2].y >> 8) & 0xff);
     for (int c = c_lo; c <= c_hi; c++) {
       auto acc = [&](int a) {
+        T *p = ys + __umul24(lc[a], S);  // slots are below 2^16
         if constexpr (FMA_ACC)
-          lds_fma3(ys + S * lc[a], sc, ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
+          lds_fma3(p, sc, ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
         else
-          lds_add3<T, S>(ys + S * lc[a], ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
+          lds_add3<T, S>(p, ye[3 * a], ye[3 * a + 1], ye[3 * a + 2]);
       };
       if (col == c) {
 #pragma unroll
