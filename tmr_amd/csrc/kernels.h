@@ -83,6 +83,11 @@ struct Tiles {
   const int *nacc_dep = nullptr;  // [n_tiles] accumulated hanging slots
   const int *el_ptr = nullptr;    // [n_tiles + 1]
   const TileElem *el = nullptr;
+  // the same records for the FP32 element-density apply, with the element
+  // word holding the element's FP32 scale (refreshed with the design)
+  TileElem *el32 = nullptr;
+  const int *el_elem = nullptr;  // element of each record
+  int n_rec = 0;
   const int *comb_ptr = nullptr;  // [n_nodes + 1] owned node <- hanging slots
   const int *comb_slot = nullptr; // local slot in the owning tile
   const float *comb_w = nullptr;

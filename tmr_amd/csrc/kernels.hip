@@ -1463,7 +1463,8 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
       if constexpr (NODAL) {
         nodal_elem<T>(K0 + NODAL_TABLE * cls, s_elem + 8 * (size_t)elem, xe, mask, ye);
       } else {
-        sc = s_elem[elem];
+        // FP32: the record's element word holds the scale itself
+        sc = FMA_ACC ? __uint_as_float(w2.x) : s_elem[elem];
         refl_elem<T, !FMA_ACC>(K0 + REFL_TABLE * cls, sc, xe, ye);
       }
     }
@@ -1793,10 +1794,16 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
       return;
     }
   }
+  // FP32 element-density records carry the scale in their element word
+  const TileElem *recs = (sizeof(T) == 4 && !tl.nodal) ? tl.el32 : tl.el;
+  if (!recs) {  // tiles not built: report like a failed launch
+    note_launch(hipErrorInvalidValue);
+    return;
+  }
 #define TMRX_TILE_NT(M, S, N, NTH)                                                              \
   hipLaunchKernelGGL((k_tile_apply<T, M, S, N, NTH>), grid, dim3(NTH), sh, st, tl.node0,          \
                      tl.ext_ptr, tl.ext, tl.nacc_dep, tl.el_ptr,                                  \
-                     reinterpret_cast<const uint2 *>(tl.el), tl.ncolors, tl.comb_ptr,             \
+                     reinterpret_cast<const uint2 *>(recs), tl.ncolors, tl.comb_ptr,              \
                      tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask, K0,   \
                      s_elem, x, b, y, xo, d, dinv, g.ca, g.cc, g.cflags, g.partials, vs, tl.run)
 #define TMRX_TILE(M, S)                    \

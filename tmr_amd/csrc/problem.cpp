@@ -714,9 +714,10 @@ int Problem::build_tiles(int l) {
     return -1;
   }
   {
-    std::vector<int> me(mel.size());
+    std::vector<int> me(mel.size()), ee(els.size());
     for (size_t k = 0; k < mel.size(); k++) me[k] = mel[k].elem;
-    if (L.t_mel_elem.upload(me)) {
+    for (size_t k = 0; k < els.size(); k++) ee[k] = els[k].elem;
+    if (L.t_mel_elem.upload(me) || L.t_el32.upload(els) || L.t_el_elem.upload(ee)) {
       last_error = "problem: device allocation failed";
       return -1;
     }
@@ -749,8 +750,13 @@ int Problem::build_tiles(int l) {
   T.mel = L.t_mel.p;
   T.mel_elem = L.t_mel_elem.p;
   T.n_mrec = (int)mel.size();
-  if (have_design_)
+  T.el32 = L.t_el32.p;
+  T.el_elem = L.t_el_elem.p;
+  T.n_rec = (int)els.size();
+  if (have_design_) {
     launch_rec_scale(T.n_mrec, T.mel_elem, L.s32.p, reinterpret_cast<uint32_t *>(T.mel), st_);
+    launch_rec_scale(T.n_rec, T.el_elem, L.s32.p, reinterpret_cast<uint32_t *>(T.el32), st_);
+  }
   T.afrag = reinterpret_cast<const uint4 *>(afrag_.p);
   T.use_mfma = true;
   return 0;
@@ -934,9 +940,12 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
     launch_restrict_scale(Cc.n_elem, F.sc_ptr.p, F.sc_child.p, F.sc_w.p, F.s64.p, F.d_lev.p,
                           Cc.d_lev.p, Cc.s64.p, Cc.s32.p, st_);
   }
-  for (auto &L : levels_)
+  for (auto &L : levels_) {
     launch_rec_scale(L.tiles.n_mrec, L.tiles.mel_elem, L.s32.p,
                      reinterpret_cast<uint32_t *>(L.tiles.mel), st_);
+    launch_rec_scale(L.tiles.n_rec, L.tiles.el_elem, L.s32.p,
+                     reinterpret_cast<uint32_t *>(L.tiles.el32), st_);
+  }
   have_design_ = true;
   have_setup_ = false;
   return 0;

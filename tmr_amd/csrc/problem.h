@@ -131,7 +131,8 @@ struct Level {
   DevArray<TileElem> t_mel;
   DevArray<int> t_mel_elem;
   DevArray<int> t_node0, t_ext_ptr, t_ext, t_nacc, t_el_ptr, t_comb_ptr, t_comb_slot;
-  DevArray<TileElem> t_el;
+  DevArray<TileElem> t_el, t_el32;
+  DevArray<int> t_el_elem;
   DevArray<float> t_comb_w;
   Tiles tiles;
   // transfer to the next coarser level (P: rows = this level's nodes)
