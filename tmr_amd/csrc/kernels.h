@@ -271,7 +271,8 @@ void launch_nodal_grad(const ElemBatches &eb, const int16_t *lev, const double *
 void launch_take_comp0(int n, const double *in, double *out, hipStream_t st);
 
 // FP64 PCG helpers (scalars live on the device)
-enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_TRR, SC_N };
+// SC_XIT: the iteration whose alpha p is still to be added to x (-1: none)
+enum { SC_RZ = 0, SC_PQ, SC_ALPHA, SC_RR, SC_ZQ, SC_BETA, SC_BB, SC_RZNEW, SC_TRR, SC_XIT, SC_N };
 int reduce_blocks(int64_t n);
 void launch_pcg_update_xr(int64_t ndof, double *x, double *r, const double *p, const double *q,
                           const double *sc, double *partials, float *r32, hipStream_t st);
@@ -302,18 +303,20 @@ struct SegPlan {
 enum { FIN_BB = 0, FIN_RZ, FIN_PQ, FIN_RR, FIN_BETA, FIN_TRUE };
 void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const int *act,
                    double *partials, hipStream_t st);
-// x += alpha p, r -= alpha q, the FP32 copy r32 of r and, with x1, the
-// finest level's first Chebyshev step from zero x1 = c0 Dinv32 r32
-void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
-                        const double *q, float *r32, const double *sc, const int *act,
-                        double *partials, hipStream_t st, const float *dinv32 = nullptr,
-                        double c0 = 0.0, float *x1 = nullptr);
+// r -= alpha q, the FP32 copy r32 of r and, with x1, the finest level's
+// first Chebyshev step from zero x1 = c0 Dinv32 r32 (x += alpha p follows in
+// the same iteration's p update, which reads p anyway)
+void launch_b_update_xr(const SegPlan &pl, double *r, const double *q, float *r32,
+                        const double *sc, const int *act, double *partials, hipStream_t st,
+                        const float *dinv32 = nullptr, double c0 = 0.0, float *x1 = nullptr);
 template <class TZ>
 void launch_b_dot2(const SegPlan &pl, const TZ *z, const double *r, const double *q,
                    const int *act, double *partials, hipStream_t st);
+// x += alpha p for segments whose alpha was formed in the iteration before
+// *iter (SC_XIT), then p = z + beta p for running segments
 template <class TZ>
 void launch_b_update_p(const SegPlan &pl, double *p, const TZ *z, const double *sc,
-                       const int *act, hipStream_t st);
+                       const int *act, double *x, const int *iter, hipStream_t st);
 void launch_b_copy(const SegPlan &pl, const double *a, double *b, const int *act,
                    hipStream_t st);
 // FIN_BB also sets the stop threshold max(rtol2 |b|^2, atol2) and the flags;

@@ -42,10 +42,9 @@ __global__ __launch_bounds__(BB) void k_b_dot1(SegPlan pl, const double *__restr
   if (threadIdx.x == 0) partials[(int64_t)rhs * pl.nblk + blk] = t;
 }
 
-// x += alpha p, r -= alpha q (and its FP32 copy), partial |r|^2
-__global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restrict__ x,
-                                                    double *__restrict__ r,
-                                                    const double *__restrict__ p,
+// r -= alpha q (and its FP32 copy), partial |r|^2; x += alpha p is left to
+// the p update of the same iteration (it reads p there anyway)
+__global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restrict__ r,
                                                     const double *__restrict__ q,
                                                     float *__restrict__ r32,
                                                     const double *__restrict__ sc,
@@ -62,7 +61,6 @@ __global__ __launch_bounds__(BB) void k_b_update_xr(SegPlan pl, double *__restri
     const double alpha = sc[(int64_t)sidx * SC_N + SC_ALPHA];
     for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB) {
       const int64_t k = off + i;
-      x[k] += alpha * p[k];
       const double rn = r[k] - alpha * q[k];
       r[k] = rn;
       if (r32) {
@@ -109,19 +107,29 @@ __global__ __launch_bounds__(BB) void k_b_dot2(SegPlan pl, const TZ *__restrict_
   }
 }
 
-// p = z + beta p
+// x += alpha p for a segment whose alpha was formed in the iteration before
+// *iter (also the iteration at which it stopped), then p = z + beta p while
+// it runs
 template <class TZ>
 __global__ __launch_bounds__(BB) void k_b_update_p(SegPlan pl, double *__restrict__ p,
                                                    const TZ *__restrict__ z,
                                                    const double *__restrict__ sc,
-                                                   const int *__restrict__ act) {
+                                                   const int *__restrict__ act,
+                                                   double *__restrict__ x,
+                                                   const int *__restrict__ iter) {
   const int blk = blockIdx.x, rhs = blockIdx.y;
   const int sidx = rhs * pl.nseg + pl.blk_seg[blk];
-  if (!act[sidx]) return;
-  const double beta = sc[(int64_t)sidx * SC_N + SC_BETA];
+  const double *s = sc + (int64_t)sidx * SC_N;
+  const bool upx = s[SC_XIT] == (double)(*iter - 1);
+  const bool upp = act[sidx] != 0;
+  if (!upx && !upp) return;
+  const double alpha = s[SC_ALPHA], beta = s[SC_BETA];
   const int64_t off = (int64_t)rhs * pl.n;
-  for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB)
-    p[off + i] = (double)z[off + i] + beta * p[off + i];
+  for (int64_t i = pl.blk_lo[blk] + threadIdx.x; i < pl.blk_lo[blk + 1]; i += BB) {
+    const double pv = p[off + i];
+    if (upx) x[off + i] += alpha * pv;
+    if (upp) p[off + i] = (double)z[off + i] + beta * pv;
+  }
 }
 
 // copy a into b for running segments (b = a everywhere when act is null)
@@ -176,6 +184,7 @@ __global__ __launch_bounds__(256) void k_b_finalize(SegPlan pl, int mode,
     case FIN_BB:  // |b|^2 of the clamped right-hand side: threshold, start or stop
       s[SC_BB] = sa;
       s[SC_RR] = sa;
+      s[SC_XIT] = -1.0;
       thr2[sidx] = fmax(rtol2 * sa, atol2);
       act[sidx] = (sa > 0.0 && sa > atol2) ? 1 : 0;
       itconv[sidx] = 0;
@@ -187,6 +196,7 @@ __global__ __launch_bounds__(256) void k_b_finalize(SegPlan pl, int mode,
       if (act[sidx]) {
         s[SC_PQ] = sa;
         s[SC_ALPHA] = s[SC_RZ] / sa;
+        s[SC_XIT] = (double)*iter;  // x += alpha p is due in this iteration
       }
       break;
     case FIN_RR:
@@ -235,12 +245,11 @@ void launch_b_dot1(const SegPlan &pl, const double *a, const double *b, const in
   hipLaunchKernelGGL(k_b_dot1, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, a, b, act, partials);
   TMRX_LAUNCH_CHECK();
 }
-void launch_b_update_xr(const SegPlan &pl, double *x, double *r, const double *p,
-                        const double *q, float *r32, const double *sc, const int *act,
-                        double *partials, hipStream_t st, const float *dinv32, double c0,
-                        float *x1) {
-  hipLaunchKernelGGL(k_b_update_xr, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, x, r, p, q, r32,
-                     sc, act, partials, dinv32, c0, x1);
+void launch_b_update_xr(const SegPlan &pl, double *r, const double *q, float *r32,
+                        const double *sc, const int *act, double *partials, hipStream_t st,
+                        const float *dinv32, double c0, float *x1) {
+  hipLaunchKernelGGL(k_b_update_xr, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, r, q, r32, sc,
+                     act, partials, dinv32, c0, x1);
   TMRX_LAUNCH_CHECK();
 }
 template <class TZ>
@@ -252,9 +261,9 @@ void launch_b_dot2(const SegPlan &pl, const TZ *z, const double *r, const double
 }
 template <class TZ>
 void launch_b_update_p(const SegPlan &pl, double *p, const TZ *z, const double *sc,
-                       const int *act, hipStream_t st) {
+                       const int *act, double *x, const int *iter, hipStream_t st) {
   hipLaunchKernelGGL(k_b_update_p<TZ>, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, p, z, sc,
-                     act);
+                     act, x, iter);
   TMRX_LAUNCH_CHECK();
 }
 template void launch_b_dot2<float>(const SegPlan &, const float *, const double *, const double *,
@@ -262,9 +271,10 @@ template void launch_b_dot2<float>(const SegPlan &, const float *, const double 
 template void launch_b_dot2<double>(const SegPlan &, const double *, const double *,
                                     const double *, const int *, double *, hipStream_t);
 template void launch_b_update_p<float>(const SegPlan &, double *, const float *, const double *,
-                                       const int *, hipStream_t);
+                                       const int *, double *, const int *, hipStream_t);
 template void launch_b_update_p<double>(const SegPlan &, double *, const double *,
-                                        const double *, const int *, hipStream_t);
+                                        const double *, const int *, double *, const int *,
+                                        hipStream_t);
 void launch_b_copy(const SegPlan &pl, const double *a, double *b, const int *act,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_b_copy, dim3(pl.nblk, pl.nrhs), dim3(BB), 0, st, pl, a, b, act);

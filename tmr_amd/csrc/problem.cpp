@@ -1455,7 +1455,7 @@ int Problem::solve_multi(int nrhs, const double *f, double *u, bool on_device, c
 }
 
 // one flexible-PCG iteration for every running (right-hand side, component):
-// q = A p, alpha, x and r, stop test, z = M r, beta, p; then the running
+// q = A p, alpha, r, stop test, z = M r, beta, x and p; then the running
 // count into pinned slot `slot` and the iteration counter advanced.  Kernels
 // and one copy only, so that it can be captured.
 void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const SolveOpts &o,
@@ -1475,7 +1475,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   // mixed precision: the finest level's first smoothing step rides on the
   // residual update (into the spare buffer where the V-cycle starts)
   Level &L0u = levels_[0];
-  launch_b_update_xr(pl, xsol, r_.p, p_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_,
+  launch_b_update_xr(pl, r_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_,
                      mixed ? L0u.v32.dinv.p : nullptr, mixed ? cheb_c0<float>(0, o) : 0.0,
                      mixed ? first_target<float>(0, L0u.v32.x.p, o) : nullptr);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
@@ -1491,13 +1491,13 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
     launch_b_dot2<float>(pl, L0.v32.x.p, r_.p, q_.p, act_.p, parts_.p, st_);
     launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
-    launch_b_update_p<float>(pl, p_.p, L0.v32.x.p, sc_.p, act_.p, st_);
+    launch_b_update_p<float>(pl, p_.p, L0.v32.x.p, sc_.p, act_.p, xsol, iter_.p, st_);
   } else {
     precond(nrhs, o);
     launch_b_dot2<double>(pl, z_.p, r_.p, q_.p, act_.p, parts_.p, st_);
     launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
-    launch_b_update_p<double>(pl, p_.p, z_.p, sc_.p, act_.p, st_);
+    launch_b_update_p<double>(pl, p_.p, z_.p, sc_.p, act_.p, xsol, iter_.p, st_);
   }
   run_ = nullptr;
   (void)hipMemcpyAsync(nact_host_ + slot, nact_.p, sizeof(int), hipMemcpyDeviceToHost, st_);
