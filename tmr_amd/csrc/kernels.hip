@@ -347,13 +347,24 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
   const T wu = e ? (T)ldexpf(1.0f, 1 - e) : T(0);  // equal weights 2^-(e-1)
   T a0 = T(0), a1 = T(0), a2 = T(0);
   const Vec3<T> f = load_vec3(xf + 3 * (size_t)i);
-  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
-    const int c = col[j];
-    const T ww = e ? wu : (T)w[j];
-    const Vec3<T> u = load_vec3(xc + 3 * (size_t)c);
-    a0 += ww * u.a;
-    a1 += ww * u.b;
-    a2 += ww * u.c;
+  const int j0 = ptr[i], j1 = ptr[i + 1];
+  // four entries at a time: their index and value loads go out together
+  // (indices clamped to the row's last entry; the extra terms are skipped)
+  for (int j = j0; j < j1; j += 4) {
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) c[u] = col[min(j + u, j1 - 1)];
+    Vec3<T> v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = load_vec3(xc + 3 * (size_t)c[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (j + u < j1) {
+        const T ww = e ? wu : (T)w[j + u];
+        a0 += ww * v[u].a;
+        a1 += ww * v[u].b;
+        a2 += ww * v[u].c;
+      }
   }
   const uint8_t m = mask[i];
   const T o[3] = {(m & 1) ? f.a : f.a + a0, (m & 2) ? f.b : f.b + a1, (m & 4) ? f.c : f.c + a2};
@@ -381,14 +392,24 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   const int i = g >> 2, sub = g & 3;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   if (i < n) {
-    for (int j = ptr[i] + sub; j < ptr[i + 1]; j += 4) {
-      const int c = col[j];
-      const int e = wexp ? (int)wexp[c] : 0;
-      const T ww = e ? (T)ldexpf(1.0f, 1 - e) : (T)w[j];
-      const Vec3<T> u = load_vec3(rf + 3 * (size_t)c);
-      a0 += ww * u.a;
-      a1 += ww * u.b;
-      a2 += ww * u.c;
+    const int j1 = ptr[i + 1];
+    // two entries per lane at a time (8 per row), loads issued together;
+    // the second index is clamped to the row's last entry and skipped
+    for (int j = ptr[i] + sub; j < j1; j += 8) {
+      const int ja = j, jb = min(j + 4, j1 - 1);
+      const int ca = col[ja], cb = col[jb];
+      const int ea = wexp ? (int)wexp[ca] : 0, eb = wexp ? (int)wexp[cb] : 0;
+      const Vec3<T> ua = load_vec3(rf + 3 * (size_t)ca), ub = load_vec3(rf + 3 * (size_t)cb);
+      const T wa = ea ? (T)ldexpf(1.0f, 1 - ea) : (T)w[ja];
+      a0 += wa * ua.a;
+      a1 += wa * ua.b;
+      a2 += wa * ua.c;
+      if (j + 4 < j1) {
+        const T wb = eb ? (T)ldexpf(1.0f, 1 - eb) : (T)w[jb];
+        a0 += wb * ub.a;
+        a1 += wb * ub.b;
+        a2 += wb * ub.c;
+      }
     }
   }
   a0 += __shfl_xor(a0, 1, 64);
