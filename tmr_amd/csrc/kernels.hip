@@ -1163,6 +1163,29 @@ __device__ __forceinline__ float plane_get(uint2 w, int c) {
   return __uint_as_float(u);
 }
 
+// the global inputs of one owned node's epilogue
+template <class T>
+struct EpiIn {
+  Vec3<T> b, d, w, x;
+  uint8_t m;
+};
+template <class T, int MODE>
+__device__ __forceinline__ EpiIn<T> epi_load(int node, const uint8_t *__restrict__ bcmask,
+                                             const T *__restrict__ x, const T *__restrict__ b,
+                                             const T *__restrict__ d, const T *__restrict__ dinv,
+                                             int cflags) {
+  const size_t k3 = 3 * (size_t)node;
+  EpiIn<T> e{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, 0};
+  if (MODE == ASM_RESID || MODE == ASM_CHEB) e.b = load_vec3(b + k3);
+  if (MODE == ASM_CHEB) {
+    if (cflags & CHEB_D_READ) e.d = load_vec3(d + k3);
+    e.w = load_vec3(dinv + k3);
+  }
+  if (MODE == ASM_STORE || MODE == ASM_DOT) e.x = load_vec3(x + k3);
+  e.m = bcmask[node];
+  return e;
+}
+
 template <class T, int MODE, int S, int NT = TILE_THREADS, bool SPLIT = false, int SX = S>
 __device__ __forceinline__ void tile_finish(
     int t, int n0, int nown, const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
@@ -1180,65 +1203,68 @@ __device__ __forceinline__ void tile_finish(
     }
   };
   double dot = 0.0;
-  for (int i = threadIdx.x; i < nown; i += NT) {
-    const int node = n0 + i;
-    const int ti = tile_pos(i);
-    const int pi = S * ti;
-    // the epilogue's vector loads (one per node and vector) go out before
-    // the hanging-node sums
-    const size_t k3 = 3 * (size_t)node;
-    Vec3<T> bl{T(0), T(0), T(0)}, dl{T(0), T(0), T(0)}, wl{T(0), T(0), T(0)}, xl{T(0), T(0), T(0)};
-    if (MODE == ASM_RESID || MODE == ASM_CHEB) bl = load_vec3(b + k3);
-    if (MODE == ASM_CHEB) {
-      if (cflags & CHEB_D_READ) dl = load_vec3(d + k3);
-      wl = load_vec3(dinv + k3);
-    }
-    if (MODE == ASM_STORE || MODE == ASM_DOT) xl = load_vec3(x + k3);
-    const T bv[3] = {bl.a, bl.b, bl.c}, dv[3] = {dl.a, dl.b, dl.c}, wv[3] = {wl.a, wl.b, wl.c};
-    const T xv[3] = {xl.a, xl.b, xl.c};
-    const uint8_t m = bcmask[node];
-    T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
-    const int j0 = hang ? comb_ptr[node] : 0, j1 = hang ? comb_ptr[node + 1] : 0;
-    for (int j = j0; j < j1; j++) {
-      const int sl = comb_slot[j];
-      const T w = (T)comb_w[j];
-      a[0] += w * ys[S * sl];
-      a[1] += w * ys[S * sl + 1];
-      a[2] += w * ys[S * sl + 2];
-    }
-    T yo[3], dno[3];
+  // two owned nodes per thread at a time: both nodes' loads (one per node
+  // and vector, indices clamped into the tile) go out before either is used
+  for (int i0 = threadIdx.x; i0 < nown; i0 += 2 * NT) {
+    EpiIn<T> ein[2];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const bool bc = (m >> c) & 1;
-      if (MODE == ASM_STORE) {
-        yo[c] = bc ? xv[c] : a[c];
-      } else if (MODE == ASM_DOT) {
-        const T xx = bc ? xv[c] : xat(ti, c);
-        const T v = bc ? xx : a[c];
-        yo[c] = v;
-        dot += (double)xx * (double)v;
-      } else if (MODE == ASM_RESID) {
-        yo[c] = bc ? T(0) : bv[c] - a[c];
-      } else if (MODE == ASM_CHEB) {
-        if (bc) {
-          dno[c] = T(0);
-          yo[c] = T(0);
-        } else {
-          const T r = bv[c] - a[c];
-          const T xc = xat(ti, c);
-          const T dp = (cflags & CHEB_D_IS_X) ? xc : dv[c];
-          const T dn = (T)ca * dp + (T)cc * wv[c] * r;
-          dno[c] = dn;
-          yo[c] = xc + dn;
+    for (int u = 0; u < 2; u++)
+      ein[u] = epi_load<T, MODE>(n0 + min(i0 + u * NT, nown - 1), bcmask, x, b, d, dinv, cflags);
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int i = i0 + u * NT;
+      if (i >= nown) break;
+      const int node = n0 + i;
+      const int ti = tile_pos(i);
+      const int pi = S * ti;
+      const size_t k3 = 3 * (size_t)node;
+      const EpiIn<T> &e = ein[u];
+      const T bv[3] = {e.b.a, e.b.b, e.b.c}, dv[3] = {e.d.a, e.d.b, e.d.c};
+      const T wv[3] = {e.w.a, e.w.b, e.w.c}, xv[3] = {e.x.a, e.x.b, e.x.c};
+      const uint8_t m = e.m;
+      T a[3] = {ys[pi], ys[pi + 1], ys[pi + 2]};
+      const int j0 = hang ? comb_ptr[node] : 0, j1 = hang ? comb_ptr[node + 1] : 0;
+      for (int j = j0; j < j1; j++) {
+        const int sl = comb_slot[j];
+        const T w = (T)comb_w[j];
+        a[0] += w * ys[S * sl];
+        a[1] += w * ys[S * sl + 1];
+        a[2] += w * ys[S * sl + 2];
+      }
+      T yo[3], dno[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const bool bc = (m >> c) & 1;
+        if (MODE == ASM_STORE) {
+          yo[c] = bc ? xv[c] : a[c];
+        } else if (MODE == ASM_DOT) {
+          const T xx = bc ? xv[c] : xat(ti, c);
+          const T v = bc ? xx : a[c];
+          yo[c] = v;
+          dot += (double)xx * (double)v;
+        } else if (MODE == ASM_RESID) {
+          yo[c] = bc ? T(0) : bv[c] - a[c];
+        } else if (MODE == ASM_CHEB) {
+          if (bc) {
+            dno[c] = T(0);
+            yo[c] = T(0);
+          } else {
+            const T r = bv[c] - a[c];
+            const T xc = xat(ti, c);
+            const T dp = (cflags & CHEB_D_IS_X) ? xc : dv[c];
+            const T dn = (T)ca * dp + (T)cc * wv[c] * r;
+            dno[c] = dn;
+            yo[c] = xc + dn;
+          }
         }
       }
-    }
-    // one store per node and vector
-    if (MODE == ASM_CHEB) {
-      if (cflags & CHEB_D_STORE) store_vec3(d + k3, dno);
-      store_vec3(x_out + k3, yo);
-    } else {
-      store_vec3(y + k3, yo);
+      // one store per node and vector
+      if (MODE == ASM_CHEB) {
+        if (cflags & CHEB_D_STORE) store_vec3(d + k3, dno);
+        store_vec3(x_out + k3, yo);
+      } else {
+        store_vec3(y + k3, yo);
+      }
     }
   }
   if (MODE == ASM_DOT) {
@@ -1500,7 +1526,9 @@ __global__ __launch_bounds__(NT) void k_tile_apply(
       w2 = el[kn + 2];
     }
     // elements are sorted by colour: this round only spans the colours of
-    // its first and last element, and most waves hold a single colour
+    // its first and last element, and most waves hold a single colour (the
+    // loads follow the round's arithmetic: issued earlier, they measured
+    // slower)
     const int c_lo = (int)((el[3 * (size_t)(q0 + r0) + 2].y >> 8) & 0xff);
     const int c_hi = (int)((el[3 * (size_t)(q0 + min(r0 + NT, nel) - 1) + 2].y >> 8) & 0xff);
     for (int c = c_lo; c <= c_hi; c++) {
