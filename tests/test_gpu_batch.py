@@ -107,3 +107,21 @@ def test_autotune_keeps_results(adapt):
     assert chosen in (256, 512, 1024)
     u1 = prob.solve(F[0], opts=opts)
     assert np.linalg.norm(u1 - u0) / np.linalg.norm(u0) < 1e-8
+
+
+@pytest.mark.parametrize("precision,no_graph", [(1, 0), (0, 1)])
+def test_iteration_cap_keeps_last_update(adapt, precision, no_graph):
+    """Stopped by the iteration cap, x holds every iteration's alpha p (the
+    last one is added by that iteration's p update): the true residual
+    |f - K x| / |f| matches the PCG recurrence residual."""
+    forests, rho, prob = adapt
+    F = _loads(prob, 2)
+    opts = solver.default_opts(rtol=1e-14, precision=precision, no_graph=no_graph, max_iters=3)
+    print("iteration cap, precision", precision, "no_graph", no_graph, flush=True)
+    prob.solve_multi(F, opts=opts, allow_unconverged=True)
+    for info in prob.infos:
+        assert info.iters == 3 and not info.converged
+    rec = prob.infos[0].rnorm / prob.infos[0].rnorm0
+    tru = prob.true_residual()
+    print("recurrence %.6e true %.6e" % (rec, tru), flush=True)
+    assert abs(tru - rec) < 1e-6 * rec
