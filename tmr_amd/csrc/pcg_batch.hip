@@ -167,14 +167,24 @@ __global__ __launch_bounds__(256) void k_b_finalize(SegPlan pl, int mode,
     hi = pl.seg_blk[seg + 1];
     part = partials + (two ? 2 : 1) * (int64_t)rhs * pl.nblk;
   }
+  // one block does the whole sum: eight partials per thread are loaded
+  // before any is added (same order of additions as one at a time)
   double a = 0.0, b = 0.0;
-  for (int i = lo + threadIdx.x; i < hi; i += 256) {
-    if (two) {
-      a += part[2 * i];
-      b += part[2 * i + 1];
-    } else {
-      a += part[i];
+  constexpr int U = 8;
+  for (int i0 = lo + threadIdx.x; i0 < hi; i0 += U * 256) {
+    double va[U], vb[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      const int i = min(i0 + k * 256, hi - 1);
+      va[k] = two ? part[2 * i] : part[i];
+      vb[k] = two ? part[2 * i + 1] : 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+      if (i0 + k * 256 < hi) {
+        a += va[k];
+        b += vb[k];
+      }
   }
   const double sa = bsum(a, red);
   const double sb = two ? bsum(b, red) : 0.0;
