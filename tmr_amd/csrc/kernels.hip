@@ -495,7 +495,20 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__rest
   const double *a = Ainv + aoff[row];
   const T *bb = b + col0[row];
   const int m = nb[row];
-  for (int j = lane; j < m; j += 64) s += a[j] * (double)bb[j];
+  // eight entries per lane in flight, added in the order of a one-at-a-time
+  // loop (indices clamped to the row, extra terms skipped)
+  for (int j0 = lane; j0 < m; j0 += 8 * 64) {
+    double av[8], bv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int j = min(j0 + 64 * k, m - 1);
+      av[k] = a[j];
+      bv[k] = (double)bb[j];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (j0 + 64 * k < m) s += av[k] * bv[k];
+  }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) x[row] = (T)s;
 }
