@@ -347,6 +347,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
   const T wu = e ? (T)ldexpf(1.0f, 1 - e) : T(0);  // equal weights 2^-(e-1)
   T a0 = T(0), a1 = T(0), a2 = T(0);
   const Vec3<T> f = load_vec3(xf + 3 * (size_t)i);
+  const uint8_t m = mask[i];
   const int j0 = ptr[i], j1 = ptr[i + 1];
   // four entries at a time: their index and value loads go out together
   // (indices clamped to the row's last entry; the extra terms are skipped)
@@ -366,7 +367,6 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
         a2 += ww * v[u].c;
       }
   }
-  const uint8_t m = mask[i];
   const T o[3] = {(m & 1) ? f.a : f.a + a0, (m & 2) ? f.b : f.b + a1, (m & 4) ? f.c : f.c + a2};
   store_vec3(xf + 3 * (size_t)i, o);
 }
@@ -391,6 +391,13 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   if (x1) x1 += (int64_t)blockIdx.y * vsc;
   const int i = g >> 2, sub = g & 3;
   T a0 = T(0), a1 = T(0), a2 = T(0);
+  // the row's own inputs (mask, coarse diagonal) load before the gathers
+  uint8_t m = 0;
+  Vec3<T> di{T(0), T(0), T(0)};
+  if (i < n && sub == 0) {
+    m = mask[i];
+    if (x1) di = load_vec3(dinv_c + 3 * (size_t)i);
+  }
   if (i < n) {
     const int j1 = ptr[i + 1];
     // two entries per lane at a time (8 per row), loads issued together;
@@ -419,11 +426,9 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   a1 += __shfl_xor(a1, 2, 64);
   a2 += __shfl_xor(a2, 2, 64);
   if (i < n && sub == 0) {
-    const uint8_t m = mask[i];
     const T v[3] = {(m & 1) ? T(0) : a0, (m & 2) ? T(0) : a1, (m & 4) ? T(0) : a2};
     store_vec3(rc + 3 * (size_t)i, v);
     if (x1) {  // first Chebyshev step of the coarse level from zero
-      const Vec3<T> di = load_vec3(dinv_c + 3 * (size_t)i);
       const T u[3] = {(T)c0 * di.a * v[0], (T)c0 * di.b * v[1], (T)c0 * di.c * v[2]};
       store_vec3(x1 + 3 * (size_t)i, u);
     }
