@@ -1111,6 +1111,12 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
                                            const uint8_t *__restrict__ bcmask,
                                            const T *__restrict__ x, T *xs, T *ys) {
   const int tid = threadIdx.x;
+  // the first halo round's node indices load first: the halo values they
+  // address can then go out right after the owned loads
+  constexpr int VH = 2;
+  int gfirst[VH];
+#pragma unroll
+  for (int k = 0; k < VH; k++) gfirst[k] = next > 0 ? ext_l[min(tid + k * NT, next - 1)] : 0;
   // owned nodes: one 3-component load and one mask byte per node, two nodes
   // in flight per thread (indices clamped into the tile, so the loads are
   // unconditional and issued before any is used)
@@ -1137,11 +1143,10 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
   }
   // halo nodes: two per thread in flight; independent nodes load directly,
   // hanging ones are interpolated from their parents afterwards
-  constexpr int VH = 2;
   for (int i0 = tid; i0 < next; i0 += VH * NT) {
     int g[VH];
 #pragma unroll
-    for (int k = 0; k < VH; k++) g[k] = ext_l[min(i0 + k * NT, next - 1)];
+    for (int k = 0; k < VH; k++) g[k] = i0 == tid ? gfirst[k] : ext_l[min(i0 + k * NT, next - 1)];
     Vec3<T> w[VH];
     uint8_t mk[VH];
 #pragma unroll
