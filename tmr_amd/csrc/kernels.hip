@@ -1111,63 +1111,76 @@ __device__ __forceinline__ void tile_stage(int n0, int nown, const int *__restri
                                            const uint8_t *__restrict__ bcmask,
                                            const T *__restrict__ x, T *xs, T *ys) {
   const int tid = threadIdx.x;
-  // the first halo round's node indices load first: the halo values they
-  // address can then go out right after the owned loads
-  constexpr int VH = 2;
-  int gfirst[VH];
+  constexpr int VN = 2, VH = 2;
+  auto put = [&](int pos, T a, T b, T c) {
+    const int q = S * tile_pos(pos);
+    xs[q] = a;
+    xs[q + 1] = b;
+    xs[q + 2] = c;
+  };
+  // halo node g of slot nown + i: independent nodes load directly (masked),
+  // hanging ones are interpolated from their parents
+  auto put_halo = [&](int i, int g, const Vec3<T> &w, uint8_t m) {
+    T v[3] = {(m & 1) ? T(0) : w.a, (m & 2) ? T(0) : w.b, (m & 4) ? T(0) : w.c};
+    if (g < 0) load_node<T>(g, x, dep_ptr, dep_conn, dep_w, bcmask, v);
+    put(nown + i, v[0], v[1], v[2]);
+  };
+  // First rounds: the halo node indices, then the owned values (one 12-byte
+  // load and one mask byte per node) and the halo values all go out before
+  // any is used; indices are clamped into the tile, so the loads are
+  // unconditional.
+  int gf[VH];
 #pragma unroll
-  for (int k = 0; k < VH; k++) gfirst[k] = next > 0 ? ext_l[min(tid + k * NT, next - 1)] : 0;
-  // owned nodes: one 3-component load and one mask byte per node, two nodes
-  // in flight per thread (indices clamped into the tile, so the loads are
-  // unconditional and issued before any is used)
-  constexpr int VN = 2;
-  for (int j0 = tid; j0 < nown; j0 += VN * NT) {
-    Vec3<T> v[VN];
-    uint8_t mk[VN];
+  for (int k = 0; k < VH; k++) gf[k] = next > 0 ? ext_l[min(tid + k * NT, next - 1)] : 0;
+  Vec3<T> vo[VN], vh[VH];
+  uint8_t mo[VN], mh[VH];
+#pragma unroll
+  for (int k = 0; k < VN; k++) {
+    const int j = min(tid + k * NT, nown - 1);
+    vo[k] = load_vec3(x + 3 * (size_t)(n0 + j));
+    mo[k] = bcmask[n0 + j];
+  }
+#pragma unroll
+  for (int k = 0; k < VH; k++) {
+    const int gg = gf[k] >= 0 ? gf[k] : 0;
+    vh[k] = load_vec3(x + 3 * (size_t)gg);
+    mh[k] = bcmask[gg];
+  }
+#pragma unroll
+  for (int k = 0; k < VN; k++)
+    if (tid + k * NT < nown)
+      put(tid + k * NT, (mo[k] & 1) ? T(0) : vo[k].a, (mo[k] & 2) ? T(0) : vo[k].b,
+          (mo[k] & 4) ? T(0) : vo[k].c);
+#pragma unroll
+  for (int k = 0; k < VH; k++)
+    if (tid + k * NT < next) put_halo(tid + k * NT, gf[k], vh[k], mh[k]);
+  // further rounds (tiles of more than two nodes or halo slots per thread)
+  for (int j0 = tid + VN * NT; j0 < nown; j0 += VN * NT) {
 #pragma unroll
     for (int k = 0; k < VN; k++) {
       const int j = min(j0 + k * NT, nown - 1);
-      v[k] = load_vec3(x + 3 * (size_t)(n0 + j));
-      mk[k] = bcmask[n0 + j];
+      vo[k] = load_vec3(x + 3 * (size_t)(n0 + j));
+      mo[k] = bcmask[n0 + j];
     }
 #pragma unroll
-    for (int k = 0; k < VN; k++) {
-      const int j = j0 + k * NT;
-      if (j < nown) {
-        const int q = S * tile_pos(j);
-        xs[q] = (mk[k] & 1) ? T(0) : v[k].a;
-        xs[q + 1] = (mk[k] & 2) ? T(0) : v[k].b;
-        xs[q + 2] = (mk[k] & 4) ? T(0) : v[k].c;
-      }
-    }
+    for (int k = 0; k < VN; k++)
+      if (j0 + k * NT < nown)
+        put(j0 + k * NT, (mo[k] & 1) ? T(0) : vo[k].a, (mo[k] & 2) ? T(0) : vo[k].b,
+            (mo[k] & 4) ? T(0) : vo[k].c);
   }
-  // halo nodes: two per thread in flight; independent nodes load directly,
-  // hanging ones are interpolated from their parents afterwards
-  for (int i0 = tid; i0 < next; i0 += VH * NT) {
+  for (int i0 = tid + VH * NT; i0 < next; i0 += VH * NT) {
     int g[VH];
 #pragma unroll
-    for (int k = 0; k < VH; k++) g[k] = i0 == tid ? gfirst[k] : ext_l[min(i0 + k * NT, next - 1)];
-    Vec3<T> w[VH];
-    uint8_t mk[VH];
+    for (int k = 0; k < VH; k++) g[k] = ext_l[min(i0 + k * NT, next - 1)];
 #pragma unroll
     for (int k = 0; k < VH; k++) {
       const int gg = g[k] >= 0 ? g[k] : 0;
-      w[k] = load_vec3(x + 3 * (size_t)gg);
-      mk[k] = bcmask[gg];
+      vh[k] = load_vec3(x + 3 * (size_t)gg);
+      mh[k] = bcmask[gg];
     }
 #pragma unroll
-    for (int k = 0; k < VH; k++) {
-      const int i = i0 + k * NT;
-      if (i < next) {
-        T v[3] = {(mk[k] & 1) ? T(0) : w[k].a, (mk[k] & 2) ? T(0) : w[k].b,
-                  (mk[k] & 4) ? T(0) : w[k].c};
-        if (g[k] < 0) load_node<T>(g[k], x, dep_ptr, dep_conn, dep_w, bcmask, v);
-        const int q = S * tile_pos(nown + i);
-        xs[q] = v[0];
-        xs[q + 1] = v[1];
-        xs[q + 2] = v[2];
-      }
-    }
+    for (int k = 0; k < VH; k++)
+      if (i0 + k * NT < next) put_halo(i0 + k * NT, g[k], vh[k], mh[k]);
   }
   if (S == 4)
     for (int i = tid; i < ((nown + next + 7) & ~7); i += NT) xs[4 * i + 3] = T(0);
