@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n, const int *__restric
   store_vec3(xf + 3 * (size_t)i, o);
 }
 
-// 4 lanes per coarse row: rows gather ~27 scattered fine nodes, so spread
+// 8 lanes per coarse row: rows gather ~27 scattered fine nodes, so spread
 // the loads over lanes and reduce with shuffles (fixed order).
 template <class T>
 __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__ ptr,
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   rf += (int64_t)blockIdx.y * vsf;
   rc += (int64_t)blockIdx.y * vsc;
   if (x1) x1 += (int64_t)blockIdx.y * vsc;
-  const int i = g >> 2, sub = g & 3;
+  const int i = g >> 3, sub = g & 7;
   T a0 = T(0), a1 = T(0), a2 = T(0);
   // the row's own inputs (mask, coarse diagonal) load before the gathers
   uint8_t m = 0;
@@ -400,10 +400,10 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   }
   if (i < n) {
     const int j1 = ptr[i + 1];
-    // two entries per lane at a time (8 per row), loads issued together;
+    // two entries per lane at a time (16 per row), loads issued together;
     // the second index is clamped to the row's last entry and skipped
-    for (int j = ptr[i] + sub; j < j1; j += 8) {
-      const int ja = j, jb = min(j + 4, j1 - 1);
+    for (int j = ptr[i] + sub; j < j1; j += 16) {
+      const int ja = j, jb = min(j + 8, j1 - 1);
       const int ca = col[ja], cb = col[jb];
       const int ea = wexp ? (int)wexp[ca] : 0, eb = wexp ? (int)wexp[cb] : 0;
       const Vec3<T> ua = load_vec3(rf + 3 * (size_t)ca), ub = load_vec3(rf + 3 * (size_t)cb);
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
       a0 += wa * ua.a;
       a1 += wa * ua.b;
       a2 += wa * ua.c;
-      if (j + 4 < j1) {
+      if (j + 8 < j1) {
         const T wb = eb ? (T)ldexpf(1.0f, 1 - eb) : (T)w[jb];
         a0 += wb * ub.a;
         a1 += wb * ub.b;
@@ -425,6 +425,9 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
   a0 += __shfl_xor(a0, 2, 64);
   a1 += __shfl_xor(a1, 2, 64);
   a2 += __shfl_xor(a2, 2, 64);
+  a0 += __shfl_xor(a0, 4, 64);
+  a1 += __shfl_xor(a1, 4, 64);
+  a2 += __shfl_xor(a2, 4, 64);
   if (i < n && sub == 0) {
     const T v[3] = {(m & 1) ? T(0) : a0, (m & 2) ? T(0) : a1, (m & 4) ? T(0) : a2};
     store_vec3(rc + 3 * (size_t)i, v);
@@ -449,7 +452,7 @@ void launch_restrict(const TransferCSR &R, const uint8_t *mask_c, const T *rf, T
                      hipStream_t st, int nrhs, int64_t vsf, int64_t vsc, const T *dinv_c,
                      double c0, T *x1) {
   if (R.n_rows == 0) return;
-  const int64_t threads = 4 * (int64_t)R.n_rows;
+  const int64_t threads = 8 * (int64_t)R.n_rows;
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((threads + 255) / 256), nrhs), dim3(256), 0,
                      st, R.n_rows, R.ptr, R.col, R.w, mask_c, rf, rc, vsf, vsc, R.run, dinv_c, c0,
                      x1, R.wexp);
