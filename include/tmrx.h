@@ -142,7 +142,10 @@ typedef struct {
   double cheb_lower, cheb_upper; /* interval as fractions of lambda_max */
   int eig_iters;    /* power iterations for lambda_max(D^-1 A) */
   int monitor;      /* print the residual history to stderr */
-  int zero_guess;   /* start from u = 0 */
+  int zero_guess;   /* 1: start from u = 0; 0: u holds the initial guess u0 on
+                       entry, the PCG solves for the correction and rtol is
+                       relative to the initial residual |f - K u0| (the Krylov
+                       convention of TACS's GMRES) */
   int fp32_valu;    /* FP32 operator on the vector ALUs instead of the matrix cores */
   int no_graph;     /* enqueue each PCG iteration instead of replaying a captured HIP graph */
   int cheb_pre_degree; /* pre-smoothing degree (0: cheb_degree); post-smoothing uses cheb_degree */

@@ -93,3 +93,51 @@ def test_autograd_functions(adapt):
     # same options as the autograd path (rtol 1e-10 for both solves)
     g_ref = prob.stress_pnorm_grad(8.0, opts=solver.default_opts(rtol=1e-10))
     assert np.allclose(g_s, g_ref, rtol=1e-12, atol=0)
+
+
+def test_autograd_interleaved_forwards(adapt):
+    """Two forwards on one problem (different loads, then different designs)
+    before their backwards: each gradient is the one of its own state."""
+    import torch
+    from tmr_amd import autograd as ag
+    forests, rho, prob = adapt
+    opts = solver.default_opts(rtol=1e-10)
+    f1 = prob.traction_load("load", [0.0, 0.0, -1.0])
+    f2 = prob.traction_load("load", [0.0, 1.0, 0.0])
+    rho2 = np.clip(rho * 0.8 + 0.1, 0.05, 1.0)
+
+    def separate(r_np, f):
+        prob.set_design(r_np)
+        prob.solve(f, opts=opts)
+        return prob.compliance(), prob.compliance_grad().copy()
+
+    c1, g1 = separate(rho, f1)
+    c2, g2 = separate(rho, f2)
+    c3, g3 = separate(rho2, f1)
+    r = torch.tensor(rho, dtype=torch.float64, device="cuda", requires_grad=True)
+    (ag.compliance(prob, r, f1, opts) + ag.compliance(prob, r, f2, opts)).backward()
+    g = r.grad.detach().cpu().numpy()
+    print("interleaved loads: rel err %.2e" % (np.linalg.norm(g - g1 - g2) / np.linalg.norm(g1 + g2)),
+          flush=True)
+    assert np.allclose(g, g1 + g2, rtol=1e-8, atol=1e-14 * np.abs(g1 + g2).max())
+    ra = torch.tensor(rho, dtype=torch.float64, device="cuda", requires_grad=True)
+    rb = torch.tensor(rho2, dtype=torch.float64, device="cuda", requires_grad=True)
+    ca = ag.compliance(prob, ra, f1, opts)
+    cb = ag.compliance(prob, rb, f1, opts)
+    assert float(ca) == pytest.approx(c1, rel=1e-9) and float(cb) == pytest.approx(c3, rel=1e-9)
+    ca.backward()
+    cb.backward()
+    assert np.allclose(ra.grad.cpu().numpy(), g1, rtol=1e-8, atol=1e-14 * np.abs(g1).max())
+    assert np.allclose(rb.grad.cpu().numpy(), g3, rtol=1e-8, atol=1e-14 * np.abs(g3).max())
+    ss = ag.stress_pnorm(prob, ra, f2, P=8.0, opts=opts)
+    sc = ag.compliance(prob, rb, f1, opts)
+    ra.grad = None
+    ss.backward()
+    prob.set_design(rho)
+    prob.solve(f2, opts=opts)
+    g_ref = prob.stress_pnorm_grad(8.0, opts=opts)
+    assert np.allclose(ra.grad.cpu().numpy(), g_ref, rtol=1e-7, atol=1e-12 * np.abs(g_ref).max())
+    rb.grad = None
+    sc.backward()
+    assert np.allclose(rb.grad.cpu().numpy(), g3, rtol=1e-8, atol=1e-14 * np.abs(g3).max())
+    prob.set_design(rho)

@@ -74,9 +74,41 @@ void tmrx_solve_opts_default(tmrx_solve_opts *o) {
 int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material *mat, int nbcs,
                         const char **bc_names, const int *bc_dof_masks, int device,
                         tmrx_problem **out) {
-  if (!out || !forests || nlevels < 1) {
+  if (!out || !forests || nlevels < 1 || nbcs < 0 || (nbcs > 0 && !bc_names)) {
     set_error("tmrx_problem_create: invalid arguments");
     return -1;
+  }
+  // host-side checks first (no device needed to find these mistakes)
+  if (mat && (!(mat->E > 0.0) || !(mat->nu > -1.0 && mat->nu < 0.5) || !(mat->k0 >= 0.0) ||
+              !(mat->q >= 0.0) || !(mat->ys > 0.0) || !(mat->eps >= 0.0) ||
+              (mat->penalty_type != 0 && mat->penalty_type != 1))) {
+    set_error("tmrx_problem_create: material needs E > 0, -1 < nu < 0.5, ys > 0, q >= 0, "
+              "k0 >= 0, eps >= 0 and penalty_type 0 (SIMP) or 1 (RAMP)");
+    return -1;
+  }
+  {
+    tmrx::Forest *f0 = tmrx_forest_impl(forests[0]);
+    if (!f0) {
+      set_error("tmrx_problem_create: null forest");
+      return -1;
+    }
+    for (int k = 0; k < nbcs; k++) {
+      if (!bc_names[k]) {
+        set_error("tmrx_problem_create: null boundary-condition name");
+        return -1;
+      }
+      if (f0->create_nodes()) {
+        set_error("tmrx_problem_create: " + f0->last_error);
+        return -1;
+      }
+      std::vector<int> nodes;
+      f0->nodes_with_name(bc_names[k], nodes);
+      if (nodes.empty()) {
+        set_error(std::string("tmrx_problem_create: boundary condition '") + bc_names[k] +
+                  "' selects no node of the finest forest");
+        return -1;
+      }
+    }
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {

@@ -228,6 +228,9 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st);
 // v[r][i] += alpha y[i] for every right-hand side r (vectors of n values)
 void launch_axpy_bcast(int64_t n, int nrhs, double alpha, const double *y, double *v,
                        hipStream_t st);
+// *bad |= 1 if any of v[0..n) is not finite or outside [lo, hi]
+void launch_check_range(int64_t n, const double *v, double lo, double hi, int *bad,
+                        hipStream_t st);
 // zero the clamped components of a node-major 3-dof vector
 void launch_zero_clamped(int n_nodes, const uint8_t *bcmask, double *v, hipStream_t st,
                          int nrhs = 1, int64_t vs = 0);

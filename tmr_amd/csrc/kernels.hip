@@ -554,6 +554,27 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
   TMRX_LAUNCH_CHECK();
 }
 
+// *bad |= 1 when an entry is not finite or lies outside [lo, hi] (a NaN
+// fails both comparisons)
+__global__ void k_check_range(int64_t n, const double *__restrict__ v, double lo, double hi,
+                              int *bad) {
+  bool b = false;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = v[i];
+    if (!(x >= lo && x <= hi)) b = true;
+  }
+  if (b) atomicOr(bad, 1);
+}
+
+void launch_check_range(int64_t n, const double *v, double lo, double hi, int *bad,
+                        hipStream_t st) {
+  if (n == 0) return;
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_check_range, dim3((unsigned)nb), dim3(256), 0, st, n, v, lo, hi, bad);
+  TMRX_LAUNCH_CHECK();
+}
+
 __global__ void k_zero_clamped(int n_nodes, const uint8_t *__restrict__ bcmask,
                                double *__restrict__ v, int64_t vs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

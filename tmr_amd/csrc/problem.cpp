@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <numeric>
 
 #include "fe.h"
@@ -921,6 +922,37 @@ int Problem::build_plan() {
 }
 
 // ---------------------------------------------------------------------------
+int Problem::check_design_values(const double *v, int64_t n, bool on_device, const char *what) {
+  // SIMP rho^q is defined for rho >= 0; RAMP rho / (1 + q (1 - rho)) needs
+  // rho < 1 + 1/q (TMROctConstitutive, src/topology/TMROctConstitutive.cpp:541-546)
+  const double lo = 0.0;
+  const double hi = mat_.penalty_type == 1 && mat_.q > 0.0
+                        ? (1.0 + 1.0 / mat_.q) * (1.0 - 1e-12)
+                        : std::numeric_limits<double>::max();
+  bool bad = false;
+  if (!on_device) {
+    for (int64_t i = 0; i < n && !bad; i++) bad = !(v[i] >= lo && v[i] <= hi);
+  } else {
+    if (chk_.n == 0 && chk_.alloc(1)) {
+      last_error = std::string(what) + ": device allocation failed";
+      return -1;
+    }
+    int h = 0;
+    HIPCHK(hipMemsetAsync(chk_.p, 0, sizeof(int), st_));
+    launch_check_range(n, v, lo, hi, chk_.p, st_);
+    HIPCHK(hipMemcpyAsync(&h, chk_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    bad = h != 0;
+  }
+  if (bad) {
+    last_error = std::string(what) +
+                 (mat_.penalty_type == 1 ? ": design values must be finite and in [0, 1 + 1/q)"
+                                         : ": design values must be finite and non-negative");
+    return -1;
+  }
+  return 0;
+}
+
 int Problem::set_design(const double *rho, int n, bool on_device) {
   if (levels_.empty()) {
     last_error = "problem: not initialised";
@@ -930,6 +962,11 @@ int Problem::set_design(const double *rho, int n, bool on_device) {
     last_error = "set_design: expected one density per finest-level element";
     return -1;
   }
+  if (!rho) {
+    last_error = "set_design: null densities";
+    return -1;
+  }
+  if (check_design_values(rho, n, on_device, "set_design")) return -1;
   HIPCHK(hipMemcpyAsync(rho_.p, rho, n * sizeof(double),
                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
   design_mode_ = 0;
@@ -966,6 +1003,11 @@ int Problem::set_design_nodal(const double *x, int n, bool on_device) {
     last_error = "set_design_nodal: expected one design value per finest-level node";
     return -1;
   }
+  if (!x) {
+    last_error = "set_design_nodal: null design values";
+    return -1;
+  }
+  if (check_design_values(x, n, on_device, "set_design_nodal")) return -1;
   for (auto &L : levels_) {
     if (L.xn.n < (size_t)L.n_nodes && L.xn.alloc(L.n_nodes)) goto oom;
     if (L.sg64.n < 8 * (size_t)L.n_elem && (L.sg64.alloc(8 * (size_t)L.n_elem) ||
@@ -1283,6 +1325,19 @@ int Problem::setup(const SolveOpts &o) {
     last_error = "setup: set the design first";
     return -1;
   }
+  {
+    // a component with no clamped dof keeps its rigid-body modes: singular
+    const Level &F = levels_[0];
+    for (int sg = 0; sg < nseg_; sg++) {
+      bool any = false;
+      for (int i = F.seg_node0[sg]; i < F.seg_node0[sg + 1] && !any; i++) any = F.bcmask[i] != 0;
+      if (!any) {
+        last_error = "setup: component " + std::to_string(sg) +
+                     " has no clamped dof (missing boundary conditions?): the stiffness is singular";
+        return -1;
+      }
+    }
+  }
   auto t0 = std::chrono::steady_clock::now();
   const int prec = o.precision ? 1 : 0;
   for (size_t l = 0; l < levels_.size(); l++) {
@@ -1440,10 +1495,18 @@ int Problem::solve_multi(int nrhs, const double *f, double *u, bool on_device, c
   }
   if (ensure_batch(nrhs)) return -1;
   const int64_t m = levels_[0].ndof * nrhs;
+  const bool warm = !o.zero_guess;
+  if (warm && !u) {
+    last_error = "solve: zero_guess = 0 needs the initial guess in u";
+    return -1;
+  }
   if (f)
     HIPCHK(hipMemcpyAsync(f_.p, f, m * sizeof(double),
                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
-  const int r = solve_batch(nrhs, f_.p, u_.p, o, infos, has_ubc_);
+  if (warm)
+    HIPCHK(hipMemcpyAsync(u_.p, u, m * sizeof(double),
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+  const int r = solve_batch(nrhs, f_.p, u_.p, o, infos, has_ubc_, warm);
   if (r < 0) return r;
   if (u) {
     HIPCHK(hipMemcpyAsync(u, u_.p, m * sizeof(double),
@@ -1504,7 +1567,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
 }
 
 int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o,
-                         SolveInfo *infos, bool lift) {
+                         SolveInfo *infos, bool lift, bool warm) {
   if (!have_setup_ || setup_precision_ != (o.precision ? 1 : 0)) {
     if (setup(o)) return -1;
   }
@@ -1524,6 +1587,16 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
   const int nidx = nrhs * nseg_;
   const double rtol2 = o.rtol * o.rtol, atol2 = o.atol * o.atol;
   float *r32 = mixed ? L0.v32.b.p : nullptr;
+  if (warm) {
+    // initial guess u0 (the entry contents of xsol): the PCG solves for the
+    // correction against b - A u0 from zero, u0 is added afterwards
+    if ((w0_.n < (size_t)m && w0_.alloc(m)) || (wrhs_.n < (size_t)m && wrhs_.alloc(m))) {
+      last_error = "solve: device allocation failed";
+      return -1;
+    }
+    HIPCHK(hipMemcpyAsync(w0_.p, xsol, m * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, w0_.p, st_, nrhs, n);
+  }
   // x = 0; clamped rows of the right-hand side are zero (TACSAssembler::applyBCs)
   launch_fill<double>(m, 0.0, xsol, st_);
   launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, rhs, st_, nrhs, n);
@@ -1547,6 +1620,14 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
     launch_axpy_bcast(n, nrhs, -1.0, q_.p, lrhs_.p, st_);
     launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, lrhs_.p, st_, nrhs, n);
     rhs = lrhs_.p;
+  }
+  if (warm) {
+    AsmArgs a;
+    a.b = rhs;
+    a.y = wrhs_.p;
+    apply_level<double>(0, w0_.p, ASM_RESID, a, nrhs);
+    launch_zero_clamped(L0.n_nodes, L0.d_bcmask.p, wrhs_.p, st_, nrhs, n);
+    rhs = wrhs_.p;
   }
   HIPCHK(hipMemcpyAsync(r_.p, rhs, m * sizeof(double), hipMemcpyDeviceToDevice, st_));
   launch_b_dot1(pl, r_.p, r_.p, nullptr, parts_.p, st_);
@@ -1631,6 +1712,8 @@ int Problem::solve_batch(int nrhs, double *rhs, double *xsol, const SolveOpts &o
     launch_b_finalize(pl, FIN_TRUE, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
   }
+  if (warm)
+    for (int r = 0; r < nrhs; r++) launch_axpy_bcast(n, 1, 1.0, w0_.p + r * n, xsol + r * n, st_);
   if (lift) launch_axpy_bcast(n, nrhs, 1.0, ubc_.p, xsol, st_);
   seg_sc_.resize((size_t)nidx * SC_N);
   seg_itc_.resize(nidx);
@@ -1844,7 +1927,9 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
   HIPCHK(hipStreamSynchronize(st_));
   // adjoint K psi = df/du (K symmetric); u_ and f_ are left untouched
   const double tr = last_true_residual;
-  int r = solve_batch(1, adjf_.p, psi_.p, o, info);
+  SolveOpts oa = o;
+  oa.zero_guess = 1;  // the adjoint starts from zero
+  int r = solve_batch(1, adjf_.p, psi_.p, oa, info);
   last_adjoint_residual = last_true_residual;
   last_true_residual = tr;
   if (r != 0) return r < 0 ? -1 : 1;

@@ -229,6 +229,9 @@ class Problem {
   template <class T>
   const T *gscale(int l);
   int nodal_gather(const double *ex_gp, double *g, bool on_device);
+  // design values must be finite, >= 0 and (RAMP) below 1 + 1/q, where the
+  // penalty's denominator vanishes; what names the caller in the error
+  int check_design_values(const double *v, int64_t n, bool on_device, const char *what);
   template <class T>
   void apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrhs = 1);
   template <class T>
@@ -248,8 +251,9 @@ class Problem {
   int find_components();
   // batched FP64 flexible PCG from x = 0 on rhs ([nrhs][ndof], clamped rows
   // zeroed in place); work vectors r_, p_, q_, z_
+  // warm: x holds the initial guess on entry (else x = 0)
   int solve_batch(int nrhs, double *rhs, double *x, const SolveOpts &o, SolveInfo *infos,
-                  bool lift = false);
+                  bool lift = false, bool warm = false);
   int estimate_eigs(int l, const SolveOpts &o);
 
   std::vector<Level> levels_;
@@ -270,6 +274,8 @@ class Problem {
   DevArray<uint8_t> nomask_;  // all dofs free (full K, unmasked state loads)
   // non-zero prescribed values on clamped dofs
   DevArray<double> ubc_, lrhs_;
+  // warm start: the initial guess and the corrected right-hand side
+  DevArray<double> w0_, wrhs_;
   bool has_ubc_ = false;
   const uint8_t *state_mask() const { return has_ubc_ ? nomask_.p : levels_[0].d_bcmask.p; }
   DevArray<double> rho_;
@@ -285,6 +291,7 @@ class Problem {
   SegPlan plan_;
   DevArray<int> pl_blk_lo_, pl_blk_seg_, pl_seg_blk_, pl_seg_tile_;
   DevArray<int> act_, itconv_, iter_, nact_;
+  DevArray<int> chk_;  // flag of the design-value check
   DevArray<double> thr_;
   int *nact_host_ = nullptr;  // pinned, two slots (iterations alternate)
   // PCG iteration bodies captured as HIP graphs (two: they write alternate

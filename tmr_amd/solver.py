@@ -16,13 +16,42 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
-def _ptr(a, dtype=np.double):
-    """(pointer, on_device) for a numpy array or a contiguous torch tensor."""
+def _ptr(a, n=None, name="array", out=False):
+    """(pointer, on_device) for a contiguous float64 numpy array or torch
+    tensor of n entries (n None: any length).  Anything else raises
+    ValueError: the C-ABI reads and writes n doubles from the pointer."""
     if _is_torch(a):
+        if str(a.dtype) != "torch.float64":
+            raise ValueError("%s: expected a float64 tensor, got %s" % (name, a.dtype))
         if not a.is_contiguous():
-            raise ValueError("tensor must be contiguous")
+            raise ValueError("%s: tensor must be contiguous" % name)
+        if n is not None and a.numel() != n:
+            raise ValueError("%s: expected %d entries, got %d" % (name, n, a.numel()))
         return C.c_void_p(a.data_ptr()), 1 if a.is_cuda else 0
+    if not isinstance(a, np.ndarray):
+        raise ValueError("%s: expected a numpy array or a torch tensor" % name)
+    if a.dtype != np.float64:
+        raise ValueError("%s: expected float64, got %s" % (name, a.dtype))
+    if not a.flags.c_contiguous:
+        raise ValueError("%s: array must be C-contiguous" % name)
+    if out and not a.flags.writeable:
+        raise ValueError("%s: output array is read-only" % name)
+    if n is not None and a.size != n:
+        raise ValueError("%s: expected %d entries, got %d" % (name, n, a.size))
     return C.c_void_p(a.ctypes.data), 0
+
+
+def _input(a):
+    """Host inputs may be any float sequence (converted to contiguous
+    float64); tensors are passed as they are and checked by _ptr."""
+    if _is_torch(a):
+        return a
+    return np.ascontiguousarray(a, dtype=np.double)
+
+
+def _same_side(dev_a, dev_b, what):
+    if dev_a != dev_b:
+        raise ValueError("%s must both be host arrays or both device tensors" % what)
 
 
 def default_opts(**kw):
@@ -56,6 +85,8 @@ class Problem:
         self.ndof, self.nelem, self.nlevels = nd.value, ne.value, nl.value
         self.material = dict(E=E, nu=nu, ys=ys, q=q, k0=k0, eps=eps, penalty_type=penalty_type)
         self.info = _lib.SolveInfo()
+        self.device = int(device)
+        self.generation = 0
 
     def __del__(self):
         try:
@@ -68,29 +99,42 @@ class Problem:
     def set_stream(self, stream_ptr):
         check(lib().tmrx_problem_set_stream(self.ptr, C.c_void_p(stream_ptr)))
 
+    def _bump(self):
+        """Advance the state generation: every call that changes the design,
+        the solution or the installed load case does this, so that a saved
+        generation tells whether the state is still the one it saw."""
+        self.generation += 1
+
+    def _check_device(self, a):
+        if _is_torch(a) and a.is_cuda and a.device.index not in (None, self.device):
+            raise ValueError("tensor on cuda:%d, problem on device %d" % (a.device.index, self.device))
+
     def set_design(self, rho):
-        if not _is_torch(rho):
-            rho = np.ascontiguousarray(rho, dtype=np.double)
-        p, dev = _ptr(rho)
+        rho = _input(rho)
+        self._check_device(rho)
+        p, dev = _ptr(rho, self.nelem, "rho")
         check(lib().tmrx_problem_set_design(self.ptr, p, int(self.nelem), dev), "set_design")
+        self._bump()
 
     def set_design_nodal(self, x):
         """One design value per independent node of the finest forest
         (TMROctConformTACSTopoCreator + TMRConformFilter); afterwards the
         gradients are per node."""
-        if not _is_torch(x):
-            x = np.ascontiguousarray(x, dtype=np.double)
-        p, dev = _ptr(x)
-        n = len(x)
+        x = _input(x)
+        self._check_device(x)
+        p, dev = _ptr(x, None, "x")
+        n = x.numel() if _is_torch(x) else x.size
         check(lib().tmrx_problem_set_design_nodal(self.ptr, p, int(n), dev), "set_design_nodal")
+        self._bump()
 
     def set_bc_values(self, ubc):
         """Prescribed values of the clamped dofs (one entry per dof; entries of
         free dofs are ignored)."""
-        if not _is_torch(ubc):
-            ubc = np.ascontiguousarray(ubc, dtype=np.double)
-        p, dev = _ptr(ubc)
+        ubc = _input(ubc)
+        self._check_device(ubc)
+        p, dev = _ptr(ubc, self.ndof, "ubc")
         check(lib().tmrx_problem_set_bc_values(self.ptr, p, int(self.ndof), dev), "set_bc_values")
+        self._bump()
 
     def design_info(self):
         """(mode, gradient length): mode 0 element densities, 1 nodal."""
@@ -112,17 +156,21 @@ class Problem:
         check(lib().tmrx_problem_setup(self.ptr, C.byref(opts)), "setup")
 
     def solve(self, f, u=None, opts=None, allow_unconverged=False):
+        """u = K(rho)^-1 f.  With opts.zero_guess = 0, u holds the initial
+        guess on entry."""
         opts = opts or default_opts()
+        f = _input(f)
         if u is None:
-            if _is_torch(f):
-                import torch
-                u = torch.empty_like(f)
-            else:
-                f = np.ascontiguousarray(f, dtype=np.double)
-                u = np.zeros(self.ndof)
-        fp, dev = _ptr(f)
-        up, _ = _ptr(u)
+            if not opts.zero_guess:
+                raise ValueError("solve: zero_guess = 0 needs the initial guess in u")
+            u = f.new_empty(f.shape) if _is_torch(f) else np.zeros(self.ndof)
+        self._check_device(f)
+        self._check_device(u)
+        fp, dev = _ptr(f, self.ndof, "f")
+        up, udev = _ptr(u, self.ndof, "u", out=True)
+        _same_side(dev, udev, "f and u")
         r = lib().tmrx_problem_solve(self.ptr, fp, up, dev, C.byref(opts), C.byref(self.info))
+        self._bump()
         check(r, "solve")
         if r == 1 and not allow_unconverged:
             raise _lib.TmrxError("solve: PCG did not converge (%d iterations, |r|/|b| = %.3e)"
@@ -134,23 +182,24 @@ class Problem:
         batched PCG; returns U of the same shape; per-case infos in
         self.infos.  Load case 0 becomes the current state."""
         opts = opts or default_opts()
-        if _is_torch(F):
-            import torch
-            if F.dim() != 2:
-                raise ValueError("F must be [nrhs, ndof]")
-            U = torch.empty_like(F) if U is None else U
-        else:
-            F = np.ascontiguousarray(F, dtype=np.double)
-            if F.ndim != 2:
-                raise ValueError("F must be [nrhs, ndof]")
-            U = np.zeros_like(F) if U is None else U
+        F = _input(F)
+        if len(F.shape) != 2:
+            raise ValueError("F must be [nrhs, ndof]")
+        if U is None:
+            if not opts.zero_guess:
+                raise ValueError("solve_multi: zero_guess = 0 needs the initial guesses in U")
+            U = F.new_empty(F.shape) if _is_torch(F) else np.zeros_like(F)
         nrhs = F.shape[0]
         if F.shape[1] != self.ndof:
             raise ValueError("F has %d columns, expected %d" % (F.shape[1], self.ndof))
+        self._check_device(F)
+        self._check_device(U)
         infos = (_lib.SolveInfo * nrhs)()
-        fp, dev = _ptr(F)
-        up, _ = _ptr(U)
+        fp, dev = _ptr(F, nrhs * self.ndof, "F")
+        up, udev = _ptr(U, nrhs * self.ndof, "U", out=True)
+        _same_side(dev, udev, "F and U")
         r = lib().tmrx_problem_solve_multi(self.ptr, nrhs, fp, up, dev, C.byref(opts), infos)
+        self._bump()
         check(r, "solve_multi")
         self.infos = list(infos)
         self.info = self.infos[0]
@@ -195,14 +244,14 @@ class Problem:
 
     def set_state(self, f, u):
         """Install a stored load case (f, u) as the current state."""
-        if not _is_torch(f):
-            f = np.ascontiguousarray(f, dtype=np.double)
-            u = np.ascontiguousarray(u, dtype=np.double)
-        fp, dev = _ptr(f)
-        up, dev2 = _ptr(u)
-        if dev != dev2:
-            raise ValueError("f and u must both be host or both device arrays")
+        f, u = _input(f), _input(u)
+        self._check_device(f)
+        self._check_device(u)
+        fp, dev = _ptr(f, self.ndof, "f")
+        up, dev2 = _ptr(u, self.ndof, "u")
+        _same_side(dev, dev2, "f and u")
         check(lib().tmrx_problem_set_state(self.ptr, fp, up, dev), "set_state")
+        self._bump()
 
     def compliance(self):
         v = C.c_double()
@@ -210,9 +259,11 @@ class Problem:
         return v.value
 
     def compliance_grad(self, g=None):
+        n = self.design_info()[1]
         if g is None:
-            g = np.zeros(self.design_info()[1])
-        p, dev = _ptr(g)
+            g = np.zeros(n)
+        self._check_device(g)
+        p, dev = _ptr(g, n, "g", out=True)
         check(lib().tmrx_problem_compliance_grad(self.ptr, p, dev), "compliance_grad")
         return g
 
@@ -223,9 +274,11 @@ class Problem:
 
     def stress_pnorm_grad(self, P=8.0, g=None, opts=None):
         opts = opts or default_opts()
+        n = self.design_info()[1]
         if g is None:
-            g = np.zeros(self.design_info()[1])
-        p, dev = _ptr(g)
+            g = np.zeros(n)
+        self._check_device(g)
+        p, dev = _ptr(g, n, "g", out=True)
         info = _lib.SolveInfo()
         r = lib().tmrx_problem_stress_pnorm_grad(self.ptr, float(P), p, dev, C.byref(opts),
                                                  C.byref(info))
@@ -236,12 +289,14 @@ class Problem:
         return g
 
     def apply(self, x, precision=0, y=None):
-        if not _is_torch(x):
-            x = np.ascontiguousarray(x, dtype=np.double)
+        x = _input(x)
         if y is None:
             y = np.zeros(self.ndof) if not _is_torch(x) else x.new_empty(x.shape)
-        xp, dev = _ptr(x)
-        yp, _ = _ptr(y)
+        self._check_device(x)
+        self._check_device(y)
+        xp, dev = _ptr(x, self.ndof, "x")
+        yp, ydev = _ptr(y, self.ndof, "y", out=True)
+        _same_side(dev, ydev, "x and y")
         check(lib().tmrx_problem_apply(self.ptr, xp, yp, dev, int(precision)), "apply")
         return y
 

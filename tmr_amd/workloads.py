@@ -101,45 +101,64 @@ def gaussian_field(pts, seed, L=(2.0, 1.0, 1.0), ngauss=32):
     return np.clip((amp[None, :] * np.exp(-0.5 * d2 / sig[None, :] ** 2)).sum(1), 1e-3, 1.0)
 
 
-def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), max_sweeps=None):
+def _centres(f, L):
+    octs = f.getOctants()
+    X = f._Xv
+    bc = f.getConnectivity()["block_conn"]
+    ext = X[bc[:, 7]] - X[bc[:, 0]]
+    h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
+    u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
+    return X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h, h, octs
+
+
+def _field_gradient(c, h, seed, L):
+    """|grad rho| at the points c by central differences of step h/4."""
+    eps = 0.25 * h
+    g = np.zeros(len(c))
+    for a in range(3):
+        dp = c.copy(); dm = c.copy()
+        dp[:, a] += eps[:, a]; dm[:, a] -= eps[:, a]
+        g += ((gaussian_field(dp, seed, L) - gaussian_field(dm, seed, L)) / (2 * eps[:, a])) ** 2
+    return np.sqrt(g)
+
+
+def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), max_sweeps=None,
+             base_median=False):
     """Adaptive forest on 2x2x2 blocks: octants where the seeded density has a
     gradient above its median are refined towards a seeded level in
-    [base_level, max_level]; balance(1) after each sweep."""
+    [base_level, max_level]; balance(1) after each sweep.  The median is the
+    one of the gradient over the current octants (base_median False) or over
+    the centres of the uniform base-level octants, fixed for every sweep
+    (base_median True: a median of the field itself, the C2 rule)."""
     rng = np.random.default_rng(seed + 1000)
     f = TMR.createBoxForest(2, 2, 2, L, face_names=FACES)
     f.createTrees(base_level)
     sweeps = max_level - base_level if max_sweeps is None else max_sweeps
+    c, h, _ = _centres(f, L)
+    med0 = np.median(_field_gradient(c, h, seed, L))
     for _ in range(sweeps):
-        octs = f.getOctants()
-        X = f._Xv
-        bc = f.getConnectivity()["block_conn"]
-        ext = X[bc[:, 7]] - X[bc[:, 0]]
-        h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
-        u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
-        c = X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h
-        eps = 0.25 * h
-        g = np.zeros(len(octs))
-        for a in range(3):
-            dp = c.copy(); dm = c.copy()
-            dp[:, a] += eps[:, a]; dm[:, a] -= eps[:, a]
-            g += ((gaussian_field(dp, seed, L) - gaussian_field(dm, seed, L)) / (2 * eps[:, a])) ** 2
-        g = np.sqrt(g)
+        c, h, octs = _centres(f, L)
+        g = _field_gradient(c, h, seed, L)
         target = rng.integers(base_level, max_level + 1, size=len(octs))
-        ref = ((g > np.median(g)) & (octs["level"] < target)).astype(np.intc)
+        med = med0 if base_median else np.median(g)
+        ref = ((g > med) & (octs["level"] < target)).astype(np.intc)
         if not ref.any():
             break
         f.refine(ref, 0, max_level)
         f.balance(1)
     forests = hierarchy(f, nlevels)
-    octs = forests[0].getOctants()
-    X = forests[0]._Xv
-    bc = forests[0].getConnectivity()["block_conn"]
-    ext = X[bc[:, 7]] - X[bc[:, 0]]
-    h = ext[octs["block"]] * np.ldexp(1.0, -octs["level"].astype(np.int64))[:, None]
-    u = np.stack([octs["x"], octs["y"], octs["z"]], 1) / float(1 << 30)
-    cen = X[bc[octs["block"], 0]] + u * ext[octs["block"]] + 0.5 * h
+    cen, _, _ = _centres(forests[0], L)
     rho = gaussian_field(cen, seed, L)
     return forests, rho, L
+
+
+def c2_adaptive(seed=0, nlevels=5):
+    """The adaptive cantilever of about a million hex (1,227,458 at seed 0):
+    2 x 2 x 2 blocks on a 2 x 1 x 1 box, base level 3, per-octant target
+    levels drawn from the seed in 3..7, refined where the gradient of the
+    32-Gaussian density exceeds its median, balance(1) after each sweep;
+    `nlevels` multigrid forests."""
+    return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True)
 
 
 def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
