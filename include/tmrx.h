@@ -175,6 +175,20 @@ int tmrx_problem_create(int nlevels, tmrx_forest **forests, const tmrx_material 
                         int nbcs, const char **bc_names, const int *bc_dof_masks, int device,
                         tmrx_problem **out);
 void tmrx_problem_free(tmrx_problem *p);
+/* TMR_CreateTACSMg (src/TMR_RefinementTools.h:32-35, .cpp:37-107) on the
+   problem's level hierarchy.
+   use_galerkin: coarse operators P^T A P (TACSMg's Galerkin mode), formed on
+     the GPU at every setup from the finer level's element matrices; 0 keeps
+     rediscretised coarse elements (volume-averaged material factors).
+   use_chebyshev_smoother: must be 1.  TMR's alternative, symmetric SOR
+     (TACSGaussSeidel, :93-94), is not available: an error is returned.
+   use_coarse_direct_solve: must be 1 (FP64 Cholesky of the coarsest level,
+     TACSMg's direct solve, :83-86); 0 returns an error.
+   omega: TMR's SOR relaxation factor, checked to lie in (0, 2) and otherwise
+     unused by Chebyshev smoothing.
+   Invalidates the setup; the next solve redoes it. */
+int tmrx_mg_create(tmrx_problem *p, double omega, int use_galerkin, int use_coarse_direct_solve,
+                   int use_chebyshev_smoother);
 /* Sizes of the finest level. */
 int tmrx_problem_get_sizes(tmrx_problem *p, int *ndof, int *nelem, int *nlevels);
 /* HIP stream (hipStream_t) used by every later call; NULL = default stream. */
@@ -254,6 +268,15 @@ int tmrx_problem_autotune(tmrx_problem *p, const int *tile_sizes, int n, int rep
                           double *ms);
 int tmrx_problem_component_info(tmrx_problem *p, int rhs, int comp, int *iters, int *converged,
                                 double *rel_residual);
+/* The Galerkin operator P^T A P of multigrid level `level` (1 <= level <
+   nlevels - 1) after a setup with use_galerkin: the matrix TACSMg would form
+   there (src/TMR_RefinementTools.cpp:76-77), as 3 x 3 blocks over the
+   independent nodes of that level, rows and columns of clamped dofs
+   identity.  n receives the block rows, nnzb the blocks; ptr[n + 1],
+   col[nnzb] and val[9 nnzb] (row-major blocks) are filled when not NULL
+   (call with NULL arrays to size). */
+int tmrx_problem_level_matrix(tmrx_problem *p, int level, int *n, int *nnzb, int *ptr, int *col,
+                              double *val);
 /* Relative true residual |f - K u| / |f| of the last solve (FP64 operator). */
 int tmrx_problem_true_residual(tmrx_problem *p, double *rel);
 /* Make a stored load case (load f, displacement u; ndof each) the state read by

@@ -297,6 +297,7 @@ int orc_get_csr(int *rowp, int *cols, double *vals) {
    depend on the host) */
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int orc_get_threads(void) { return g_threads; }
 
 void orc_matvec(const double *x, double *y) {
   const csr_t *A = g_A;
@@ -378,13 +379,39 @@ int orc_solve(const double *b, double *x, double rtol, int maxit, double *relres
   return conv ? it : -it;
 }
 
+/* Element stiffness of box extents h, memoised: a forest has few distinct
+   element sizes (levels x block shapes).  Returns NULL past 256 sizes. */
+static const double *k_cached(const double h[3], double E, double nu) {
+  static double keys[256][5];
+  static double *vals[256];
+  static int nk = 0;
+  for (int i = 0; i < nk; i++)
+    if (keys[i][0] == h[0] && keys[i][1] == h[1] && keys[i][2] == h[2] && keys[i][3] == E &&
+        keys[i][4] == nu)
+      return vals[i];
+  if (nk == 256) return NULL;
+  double *K = (double *)malloc(sizeof(double) * 576);
+  orc_hex8_stiffness(h[0], h[1], h[2], E, nu, K);
+  keys[nk][0] = h[0];
+  keys[nk][1] = h[1];
+  keys[nk][2] = h[2];
+  keys[nk][3] = E;
+  keys[nk][4] = nu;
+  vals[nk] = K;
+  return vals[nk++];
+}
+
 /* element energies u_e^T K_e u_e / s_e (unpenalised) */
 void orc_elem_energy(int nelem, const int *conn, const int *dep_ptr, const int *dep_conn,
                      const double *dep_w, const double *hdim, const uint8_t *bcmask, double E,
                      double nu, const double *u, double *energy) {
-  double K[576], ue[24];
+  double Kbuf[576], ue[24];
   for (int e = 0; e < nelem; e++) {
-    orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, K);
+    const double *K = k_cached(&hdim[3 * e], E, nu);
+    if (!K) {
+      orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, Kbuf);
+      K = Kbuf;
+    }
     for (int a = 0; a < 8; a++) {
       int g = conn[8 * e + a];
       for (int i = 0; i < 3; i++) {
@@ -594,4 +621,45 @@ void orc_elem_energy_gp(int nelem, const int *conn, const int *dep_ptr, const in
       out[8 * (size_t)e + q] = t;
     }
   }
+}
+
+/* Matrix-free y = A x with A = Z^T (sum_e s_e K_e) Z, identity rows and
+   columns on clamped dofs (the operator orc_assemble stores), element by
+   element without assembly: the residual check of solutions too large to
+   assemble.  s has one factor per element. */
+void orc_apply_free(int nnodes, int nelem, const int *conn, const int *dep_ptr,
+                    const int *dep_conn, const double *dep_w, const double *hdim, const double *s,
+                    const uint8_t *bcmask, double E, double nu, const double *x, double *y) {
+  double Kbuf[576], ue[24], ye[24];
+  memset(y, 0, sizeof(double) * 3 * (size_t)nnodes);
+  for (int e = 0; e < nelem; e++) {
+    const double *K = k_cached(&hdim[3 * e], E, nu);
+    if (!K) {
+      orc_hex8_stiffness(hdim[3 * e], hdim[3 * e + 1], hdim[3 * e + 2], E, nu, Kbuf);
+      K = Kbuf;
+    }
+    gather_ue(conn, e, dep_ptr, dep_conn, dep_w, bcmask, x, ue);
+    for (int i = 0; i < 24; i++) {
+      double t = 0.0;
+      for (int j = 0; j < 24; j++) t += K[24 * i + j] * ue[j];
+      ye[i] = s[e] * t;
+    }
+    for (int a = 0; a < 8; a++) {
+      const int g = conn[8 * e + a];
+      for (int i = 0; i < 3; i++) {
+        if (g >= 0) {
+          if (!((bcmask[g] >> i) & 1)) y[3 * (size_t)g + i] += ye[3 * a + i];
+        } else {
+          const int d = -g - 1;
+          for (int k = dep_ptr[d]; k < dep_ptr[d + 1]; k++) {
+            const int p = dep_conn[k];
+            if (!((bcmask[p] >> i) & 1)) y[3 * (size_t)p + i] += dep_w[k] * ye[3 * a + i];
+          }
+        }
+      }
+    }
+  }
+  for (int n = 0; n < nnodes; n++)
+    for (int i = 0; i < 3; i++)
+      if ((bcmask[n] >> i) & 1) y[3 * (size_t)n + i] = x[3 * (size_t)n + i];
 }

@@ -125,6 +125,33 @@ class Oracle:
             raise RuntimeError("oracle CG did not converge")
         return x
 
+    def apply_free(self, s, x):
+        """y = A x element by element (no assembly): A = Z^T (sum_e s_e K_e) Z
+        with identity rows and columns on clamped dofs."""
+        L = lib()
+        if not hasattr(L, "_free_sigs"):
+            P, I, D = C.c_void_p, C.c_int, C.c_double
+            L.orc_apply_free.argtypes = [I, I, P, P, P, P, P, P, P, D, D, P, P]
+            L._free_sigs = True
+        s = np.ascontiguousarray(s, dtype=np.double)
+        x = np.ascontiguousarray(x, dtype=np.double)
+        y = np.zeros(3 * self.nnodes)
+        L.orc_apply_free(self.nnodes, self.nelem, _p(self.conn), _p(self.dep_ptr),
+                         _p(self.dep_conn), _p(self.dep_w), _p(self.hdim), _p(s),
+                         _p(self.bcmask), self.E, self.nu, _p(x), _p(y))
+        return y
+
+    def free_mask(self):
+        """True on the free dofs."""
+        return ~(((self.bcmask[:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))
+
+    def true_residual(self, s, f, u):
+        """|f - A u| / |f| over the free dofs, f zeroed on clamped dofs (the
+        oracle's check of a solution, any size)."""
+        fr = np.where(self.free_mask(), f, 0.0)
+        r = fr - self.apply_free(s, u)
+        return float(np.linalg.norm(r) / np.linalg.norm(fr))
+
     def energy2(self, u, v):
         u = np.ascontiguousarray(u, dtype=np.double)
         v = np.ascontiguousarray(v, dtype=np.double)
@@ -449,3 +476,96 @@ def nodal_stress_gradient(orc, x, b, q=3.0, k0=1e-6, ptype=0, P=8.0, ys=1.0, eps
     e = orc.energy_gp(u, psi)
     t = (dfdr - _dpen(rg, q, ptype) * e) @ gauss_shape()
     return f, orc.gather_corners(t), u, psi
+
+
+# ---------------------------------------------------------------------------
+# multigrid (oracle/mg_oracle.c): Galerkin hierarchy, Chebyshev or symmetric
+# SOR smoothing, dense Cholesky at the coarsest level; flexible PCG or
+# GMRES(m) around it
+def _mglib():
+    L = lib()
+    if not hasattr(L, "_mg_sigs"):
+        P, I, D = C.c_void_p, C.c_int, C.c_double
+        L.orc_mg_set_level.argtypes = [I, I, P]
+        L.orc_mg_set_matrix.argtypes = [I, P, P, P]
+        L.orc_mg_set_interp.argtypes = [I, I, I, P, P, P]
+        L.orc_mg_galerkin.restype = I
+        L.orc_mg_galerkin.argtypes = [I]
+        L.orc_mg_get_matrix.argtypes = [I, P, P, P, P, P]
+        L.orc_mg_setup.restype = I
+        L.orc_mg_setup.argtypes = [I, I, D, D, D]
+        L.orc_mg_pcg.restype = I
+        L.orc_mg_pcg.argtypes = [P, P, D, I, P]
+        L.orc_mg_gmres.restype = I
+        L.orc_mg_gmres.argtypes = [P, P, I, D, I, P]
+        L._mg_sigs = True
+    return L
+
+
+class Multigrid:
+    """Galerkin multigrid on the oracle's assembled fine operator: levels
+    from a finest-first forest list (their interpolation CSRs and clamped
+    dofs), A_{l+1} = P~^T A_l P~.  smoother 'chebyshev' (degree, interval as
+    fractions of lambda_max(D^-1 A)) or 'sor' (symmetric, omega)."""
+
+    def __init__(self, orc, forests, bcs, interps=None):
+        L = _mglib()
+        self.orc, self.nl = orc, len(forests)
+        rp, cl, v = orc.csr()
+        self.masks = [bcmask_from_forest(f, bcs) for f in forests]
+        for l, f in enumerate(forests):
+            L.orc_mg_set_level(l, int(f.getNodeRange()[1]), _p(self.masks[l]))
+        L.orc_mg_set_matrix(0, _p(rp), _p(cl), _p(v))
+        for l in range(self.nl - 1):
+            if interps is not None:
+                prp, pcl, pw = interps[l]
+            else:
+                prp, pcl, pw = forests[l].createInterpolation(forests[l + 1])
+            prp = np.ascontiguousarray(prp, dtype=np.intc)
+            pcl = np.ascontiguousarray(pcl, dtype=np.intc)
+            pw = np.ascontiguousarray(pw, dtype=np.double)
+            L.orc_mg_set_interp(l, len(prp) - 1, int(forests[l + 1].getNodeRange()[1]),
+                                _p(prp), _p(pcl), _p(pw))
+            L.orc_mg_galerkin(l)
+
+    def setup(self, smoother="chebyshev", degree=2, lower=1.0 / 6.0, upper=1.1, omega=1.0):
+        r = _mglib().orc_mg_setup(0 if smoother == "chebyshev" else 1, int(degree), lower, upper,
+                                  omega)
+        if r != 0:
+            raise RuntimeError("oracle multigrid setup failed (%d)" % r)
+
+    def matrix(self, l):
+        import scipy.sparse as sp
+        L = _mglib()
+        n, nnz = C.c_int(), C.c_int()
+        L.orc_mg_get_matrix(l, C.byref(n), C.byref(nnz), None, None, None)
+        rp = np.zeros(n.value + 1, np.intc)
+        cl = np.zeros(nnz.value, np.intc)
+        v = np.zeros(nnz.value)
+        L.orc_mg_get_matrix(l, None, None, _p(rp), _p(cl), _p(v))
+        return sp.csr_matrix((v, cl, rp), shape=(n.value, n.value))
+
+    def _rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.double).copy()
+        b[~self.orc.free_mask()] = 0.0
+        return b
+
+    def pcg(self, b, rtol=1e-8, maxit=500):
+        b = self._rhs(b)
+        x = np.zeros_like(b)
+        rr = C.c_double()
+        it = _mglib().orc_mg_pcg(_p(b), _p(x), rtol, maxit, C.byref(rr))
+        self.iters, self.relres = abs(it), rr.value
+        if it < 0:
+            raise RuntimeError("oracle MG-PCG did not converge")
+        return x
+
+    def gmres(self, b, m=50, rtol=1e-8, maxit=1000):
+        b = self._rhs(b)
+        x = np.zeros_like(b)
+        rr = C.c_double()
+        it = _mglib().orc_mg_gmres(_p(b), _p(x), int(m), rtol, maxit, C.byref(rr))
+        self.iters, self.relres = abs(it), rr.value
+        if it < 0:
+            raise RuntimeError("oracle MG-GMRES did not converge")
+        return x

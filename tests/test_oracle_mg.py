@@ -1,0 +1,95 @@
+"""The oracle's multigrid (oracle/mg_oracle.c), pinned on the CPU: its
+Galerkin operators equal the triple product P~^T A P~ formed with scipy from
+the oracle's assembled operator and interpolation; MG-PCG (Chebyshev) and
+GMRES(50) with symmetric-SOR MG (TMRTopoProblem's solver) reach the
+Jacobi-PCG solution; one V-cycle on a uniform two-level problem matches a
+dense restatement of the cycle."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import pyoracle as po
+from tmr_amd import workloads
+
+BCS = [("fixed", 7)]
+
+
+def traction_z(forest, orc):
+    """Consistent load of a unit traction -z on the faces named 'load'
+    (exact face integral t A / 4 per face corner, hanging corners to their
+    parents)."""
+    conn = forest.getMeshConn()
+    ext = forest.getElementExtents()
+    dp, dc, dw = forest.getDepNodeConn()
+    f = np.zeros(3 * orc.nnodes)
+    for o in forest.getOctsWithName("load"):
+        e, face = o["tag"], o["info"]
+        ax = face // 2
+        area = ext[e][(ax + 1) % 3] * ext[e][(ax + 2) % 3]
+        for k in range(8):
+            if [(k & 1), (k >> 1) & 1, (k >> 2)][ax] == face % 2:
+                g = conn[e][k]
+                if g >= 0:
+                    f[3 * g + 2] += -0.25 * area
+                else:
+                    d = -g - 1
+                    for j in range(dp[d], dp[d + 1]):
+                        f[3 * dc[j] + 2] += -0.25 * area * dw[j]
+    return f
+
+
+@pytest.fixture(scope="module")
+def adapt():
+    forests, rho, L = workloads.adaptive(seed=3, base_level=2, max_level=4, nlevels=4)
+    orc = po.from_forest(forests[0], BCS)
+    orc.assemble(rho ** 3 + 1e-6)
+    return forests, rho, orc
+
+
+def test_galerkin_matches_scipy_triple_product(adapt):
+    forests, rho, orc = adapt
+    mg = po.Multigrid(orc, forests, BCS)
+    rp, cl, v = orc.csr()
+    A = sp.csr_matrix((v, cl, rp), shape=(3 * orc.nnodes, 3 * orc.nnodes))
+    for l in range(1, len(forests)):
+        _, _, (prp, pcl, pw) = po.interpolation((2, 2, 2), forests[l - 1].getOctants(),
+                                                forests[l].getOctants())
+        nf, nc = len(prp) - 1, int(forests[l].getNodeRange()[1])
+        P = sp.kron(sp.csr_matrix((pw, pcl, prp), shape=(nf, nc)), sp.identity(3), format="csr")
+        ff = orc.free_mask() if l == 1 else free
+        free = ~(((mg.masks[l][:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))
+        Pm = sp.diags(ff.astype(float)) @ P @ sp.diags(free.astype(float))
+        A = (Pm.T @ A @ Pm + sp.diags((~free).astype(float))).tocsr()
+        G = mg.matrix(l)
+        err = abs(G - A).max() / abs(A).max()
+        assert err < 1e-13, (l, err)
+
+
+@pytest.mark.parametrize("mode", ["pcg-chebyshev", "gmres-sor"])
+def test_multigrid_solves_match_jacobi_pcg(adapt, mode):
+    forests, rho, orc = adapt
+    f = traction_z(forests[0], orc)
+    u_ref = orc.solve(f, rtol=1e-13)
+    mg = po.Multigrid(orc, forests, BCS)
+    if mode == "pcg-chebyshev":
+        mg.setup("chebyshev", degree=2)
+        u = mg.pcg(f, rtol=1e-11, maxit=500)
+    else:
+        mg.setup("sor", omega=1.0)
+        u = mg.gmres(f, m=50, rtol=1e-11, maxit=1000)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-9
+    s = rho ** 3 + 1e-6
+    assert orc.true_residual(s, f, u) < 1e-8
+
+
+def test_uniform_density_needs_few_iterations():
+    """A uniform cube with uniform density: the Galerkin V-cycle is a strong
+    preconditioner (iteration count pinned as a regression bound)."""
+    forests, rho, L = workloads.cube_l4(nlevels=3)
+    orc = po.from_forest(forests[0], BCS)
+    orc.assemble(np.full(len(rho), 1.0))
+    f = traction_z(forests[0], orc)
+    mg = po.Multigrid(orc, forests, BCS)
+    mg.setup("chebyshev", degree=2)
+    mg.pcg(f, rtol=1e-10)
+    assert mg.iters <= 14, mg.iters

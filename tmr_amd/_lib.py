@@ -76,6 +76,8 @@ _SIGS = {
     "tmrx_problem_create": (I, [I, C.POINTER(P), C.POINTER(Material), I, C.POINTER(S), IP, I,
                                 C.POINTER(P)]),
     "tmrx_problem_free": (None, [P]),
+    "tmrx_mg_create": (I, [P, D, I, I, I]),
+    "tmrx_problem_level_matrix": (I, [P, I, IP, IP, P, P, P]),
     "tmrx_problem_get_sizes": (I, [P, IP, IP, IP]),
     "tmrx_problem_set_stream": (I, [P, P]),
     "tmrx_problem_set_design": (I, [P, P, I, I]),

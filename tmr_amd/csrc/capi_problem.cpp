@@ -275,6 +275,41 @@ int tmrx_problem_tuning(tmrx_problem *h, int *tile_nodes, int *fp32_matrix_cores
   return 0;
 }
 
+int tmrx_mg_create(tmrx_problem *h, double omega, int use_galerkin, int use_coarse_direct_solve,
+                   int use_chebyshev_smoother) {
+  CHECK_P(h);
+  if (!(omega > 0.0 && omega < 2.0)) {
+    set_error("tmrx_mg_create: omega must lie in (0, 2)");
+    return -1;
+  }
+  if (!use_chebyshev_smoother) {
+    set_error("tmrx_mg_create: symmetric SOR smoothing is not available; the GPU hierarchy "
+              "smooths with Chebyshev (use_chebyshev_smoother = 1)");
+    return -1;
+  }
+  if (!use_coarse_direct_solve) {
+    set_error("tmrx_mg_create: the coarsest level is always solved directly (FP64 Cholesky); "
+              "pass use_coarse_direct_solve = 1");
+    return -1;
+  }
+  if (h->p->set_galerkin(use_galerkin != 0)) return fail(h);
+  return 0;
+}
+
+int tmrx_problem_level_matrix(tmrx_problem *h, int level, int *n, int *nnzb, int *ptr, int *col,
+                              double *val) {
+  CHECK_P(h);
+  std::vector<int> p, c;
+  std::vector<double> v;
+  if (h->p->level_matrix(level, p, c, v)) return fail(h);
+  if (n) *n = (int)p.size() - 1;
+  if (nnzb) *nnzb = (int)c.size();
+  if (ptr) std::copy(p.begin(), p.end(), ptr);
+  if (col) std::copy(c.begin(), c.end(), col);
+  if (val) std::copy(v.begin(), v.end(), val);
+  return 0;
+}
+
 int tmrx_problem_num_components(tmrx_problem *h, int *ncomp) {
   CHECK_P(h);
   if (ncomp) *ncomp = h->p->num_segments();

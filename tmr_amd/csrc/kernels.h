@@ -221,6 +221,42 @@ struct CoarseBlocks {
 template <class T>
 void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs = 1);
 
+// Galerkin coarse operator of one level: 3 x 3 blocks over the independent
+// nodes (block row i = node i), row-major blocks, clamped rows and columns
+// identity
+template <class T>
+struct BSR {
+  int n = 0;     // block rows
+  int nnzb = 0;  // blocks
+  const int *ptr = nullptr;  // [n + 1]
+  const int *col = nullptr;  // [nnzb] column node, ascending within a row
+  const T *val = nullptr;    // [nnzb][9]
+  const uint8_t *bcmask = nullptr;
+  const int *run = nullptr;  // as Tiles::run
+};
+using BSR64 = BSR<double>;
+// coarse element matrices K_E = sum_e (Q_e x I3)^T K_e (Q_e x I3) over the
+// children e of every coarse element (CSR ptr/child over coarse elements;
+// place = level difference << 24 | z offset << 16 | y offset << 8 | x
+// offset of the child inside E, in child widths).  mode 0: K_e = s_e
+// Ktab[cls_e] (finest level, element densities); 1: K_e = Kin[e]; 2:
+// K_e = sum_g s_eg Ktab[cls_e][g] (finest level, nodal design)
+void launch_galerkin_elem(int mode, int nE, const int *ptr, const int *child, const int *place,
+                          const int *cls, const double *s, const double *Ktab, const double *Kin,
+                          double *Kout, hipStream_t st);
+// BSR values from the element matrices: block k collects the terms
+// [tptr[k], tptr[k+1]) (code 64 E + 8 a + b, weight w: w K_E[a][b]); brow[k]
+// is its row
+void launch_bsr_assemble(const BSR64 &A, const int *tptr, const int *tcode, const double *tw,
+                         const int *brow, const double *KE, double *val, hipStream_t st);
+// dinv = 1 / diagonal of the blocks at diag[i]
+template <class T>
+void launch_bsr_dinv(int n, const int *diag, const double *val, T *dinv, hipStream_t st);
+// y = A x with the tile apply's epilogues (ASM_STORE, ASM_RESID, ASM_CHEB)
+template <class T>
+void launch_bsr_apply(const BSR<T> &A, const T *x, int mode, const AsmArgs &g, hipStream_t st,
+                      int nrhs = 1, int64_t vs = 0);
+
 template <class TI, class TO>
 void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);
 template <class T>

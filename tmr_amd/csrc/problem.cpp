@@ -818,19 +818,34 @@ int Problem::build_transfer(int l) {
     cnt[E + 1]++;
   }
   for (int i = 0; i < Cc.n_elem; i++) cnt[i + 1] += cnt[i];
-  std::vector<int> child(F.n_elem);
+  std::vector<int> child(F.n_elem), place(F.n_elem);
   std::vector<float> vw(F.n_elem);
   {
+    const auto &co = Cc.forest->octants();
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (int e = 0; e < F.n_elem; e++) {
       const int E = owner[e];
+      const int dl = F.lev[e] - Cc.lev[E];
+      if (dl > 8) {
+        last_error = "problem: a fine element lies more than 8 levels below its coarse element";
+        return -1;
+      }
+      // offset of e inside E in units of e's width
+      const int sh = MAX_LEVEL - F.lev[e];
+      const int ox = (fo[e].x - co[E].x) >> sh, oy = (fo[e].y - co[E].y) >> sh,
+                oz = (fo[e].z - co[E].z) >> sh;
+      place[fill[E]] = (dl << 24) | (oz << 16) | (oy << 8) | ox;
       child[fill[E]] = e;
-      vw[fill[E]++] = (float)std::ldexp(1.0, -3 * (F.lev[e] - Cc.lev[E]));
+      vw[fill[E]++] = (float)std::ldexp(1.0, -3 * dl);
     }
   }
+  F.sc_ptr_h = cnt;
+  F.sc_child_h = child;
+  F.place_h = place;
   if (F.P_ptr.upload(P.rowp) || F.P_col.upload(P.cols) || F.P_w.upload(pw) || F.P_wexp.upload(wexp) ||
       F.R_ptr.upload(rptr) || F.R_col.upload(rcol) || F.R_w.upload(rw) ||
-      F.sc_ptr.upload(cnt) || F.sc_child.upload(child) || F.sc_w.upload(vw)) {
+      F.sc_ptr.upload(cnt) || F.sc_child.upload(child) || F.sc_w.upload(vw) ||
+      F.g_place.upload(place)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -1082,6 +1097,12 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   }
   AsmArgs g = a;
   if (!g.x) g.x = (void *)x;
+  if (l > 0 && galerkin_) {
+    BSR<T> A = bsr<T>(l);
+    A.run = run_;
+    launch_bsr_apply<T>(A, x, mode, g, st_, nrhs, L.ndof);
+    return;
+  }
   Tiles tl = L.tiles;
   tl.run = run_;
   tl.nodal = design_mode_ == 1;
@@ -1207,9 +1228,16 @@ int Problem::coarse_factor() {
   Level &L = levels_.back();
   const int n = (int)L.ndof;
   const bool nodal = design_mode_ == 1;
-  std::vector<double> s(nodal ? 8 * (size_t)L.n_elem : L.n_elem);
+  // Galerkin: the coarsest level's element matrices K_E from the device
+  const bool gal = galerkin_ && levels_.size() > 1;
+  std::vector<double> s(nodal ? 8 * (size_t)L.n_elem : L.n_elem), KEh;
   HIPCHK(hipMemcpy(s.data(), nodal ? L.sg64.p : L.s64.p, s.size() * sizeof(double),
                    hipMemcpyDeviceToHost));
+  if (gal) {
+    KEh.resize(576 * (size_t)L.n_elem);
+    HIPCHK(hipStreamSynchronize(st_));
+    HIPCHK(hipMemcpy(KEh.data(), L.KE.p, KEh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
   std::vector<double> Ke(576);
   const auto &dptr = L.forest->dep_ptr();
   const auto &dconn = L.forest->dep_conn();
@@ -1236,6 +1264,10 @@ int Problem::coarse_factor() {
           Ke[i] = t;
         }
         K = Ke.data();
+        se = 1.0;
+      }
+      if (gal) {
+        K = &KEh[576 * (size_t)e];
         se = 1.0;
       }
       std::vector<std::pair<int, double>> ex[8];
@@ -1346,8 +1378,16 @@ int Problem::setup(const SolveOpts &o) {
       return -1;
     }
   }
+  if (galerkin_ && galerkin_setup(prec)) return -1;
   for (size_t l = 0; l + 1 < levels_.size(); l++) {
     Level &L = levels_[l];
+    if (galerkin_ && l > 0) {
+      // Jacobi diagonal of the Galerkin operator
+      launch_bsr_dinv<double>(L.n_nodes, L.bsr_diag.p, L.bsr64.p, L.v64.dinv.p, st_);
+      if (prec == 1) launch_bsr_dinv<float>(L.n_nodes, L.bsr_diag.p, L.bsr64.p, L.v32.dinv.p, st_);
+      if (estimate_eigs((int)l, o)) return -1;
+      continue;
+    }
     // Jacobi diagonal in both precisions
     if (design_mode_ == 1)
       launch_elem_diag<double>(L.eb, G64_.p, L.sg64.p, L.v64.partial.p, st_, true);

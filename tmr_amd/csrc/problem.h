@@ -141,9 +141,20 @@ struct Level {
   DevArray<uint8_t> P_wexp;
   TransferCSR P, R;
   // element-scale restriction from this level into the next coarser one
-  // (CSR over the coarse elements)
-  DevArray<int> sc_ptr, sc_child;
+  // (CSR over the coarse elements); g_place: where each child sits inside
+  // its coarse element (level difference, offsets; see launch_galerkin_elem)
+  DevArray<int> sc_ptr, sc_child, g_place;
   DevArray<float> sc_w;
+  std::vector<int> sc_ptr_h, sc_child_h, place_h;
+  // Galerkin coarse operator of this level (levels >= 1, when enabled): the
+  // element matrices K_E, and the BSR matrix with its term lists (levels
+  // that are smoothed; the coarsest level is factored from K_E)
+  DevArray<double> KE;
+  DevArray<int> bsr_ptr, bsr_col, bsr_row, bsr_diag, bt_ptr, bt_code;
+  DevArray<double> bt_w, bsr64;
+  DevArray<float> bsr32;
+  int nnzb = 0;
+  bool has_bsr = false;
   // stiffness scales
   DevArray<double> s64;
   DevArray<float> s32;
@@ -185,6 +196,15 @@ class Problem {
   int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
   bool fp32_matrix_cores() const { return fp32_mfma_; }
   int set_state(const double *f, const double *u, bool on_device);
+  // multigrid options of TMR_CreateTACSMg (src/TMR_RefinementTools.cpp:37-107):
+  // Galerkin coarse operators P^T A P on the coarse levels (else
+  // rediscretised coarse elements)
+  int set_galerkin(bool on);
+  bool galerkin() const { return galerkin_; }
+  // the BSR Galerkin operator of level l (1 <= l < nlevels - 1): block row
+  // pointers, columns and FP64 values (n + 1, nnzb, 9 nnzb entries)
+  int level_matrix(int l, std::vector<int> &ptr, std::vector<int> &col,
+                   std::vector<double> &val);
   // prescribed values of the clamped dofs (ubc[ndof], entries of free dofs
   // ignored); solutions take them, the free dofs solve K_ff u_f = f_f - K_fc u_c
   int set_bc_values(const double *ubc, int n, bool on_device);
@@ -216,6 +236,13 @@ class Problem {
   int ensure_vecs(int l, int precision, int nrhs = 1);
   int ensure_batch(int nrhs);
   int build_plan();
+  // Galerkin: check that the prolongation is element-wise trilinear and
+  // build the BSR structures and term lists of the coarse levels
+  int build_galerkin();
+  int build_bsr(int l);
+  int galerkin_setup(int precision);
+  template <class T>
+  BSR<T> bsr(int l);
   template <class T>
   LevelVecs<T> &vecs(int l);
   template <class T>
@@ -270,6 +297,8 @@ class Problem {
   DevArray<double> G64_;
   DevArray<float> G32_;
   int design_mode_ = 0;      // 0 element densities, 1 nodal
+  bool galerkin_ = false, galerkin_built_ = false;
+  DevArray<double> Kgd_;  // Kgh_ on the device (Galerkin with a nodal design)
   DevArray<double> rho_gp_, egp_, exgp_, gtmp_;
   DevArray<uint8_t> nomask_;  // all dofs free (full K, unmasked state loads)
   // non-zero prescribed values on clamped dofs
@@ -325,5 +354,10 @@ class Problem {
   size_t ev_used_ = 0;
   int ev_prec_mask_ = 0;
 };
+
+template <>
+BSR<double> Problem::bsr<double>(int l);
+template <>
+BSR<float> Problem::bsr<float>(int l);
 
 }  // namespace tmrx

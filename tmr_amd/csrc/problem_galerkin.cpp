@@ -1,0 +1,285 @@
+// Galerkin coarse operators for the multigrid hierarchy: TACSMg with
+// use_galerkin (TMR_CreateTACSMg, src/TMR_RefinementTools.cpp:37-107) forms
+// A_c = P^T A_f P.  Host side: the check that makes the element-wise form
+// exact, the BSR structures and term lists, and the per-design setup that
+// runs the kernels of galerkin.hip.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "problem.h"
+
+namespace tmrx {
+
+#define GCHK(x)                                                          \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ == hipSuccess) e_ = take_launch_error();                      \
+    if (e_ != hipSuccess) {                                              \
+      last_error = std::string("HIP error: ") + hipGetErrorString(e_) + \
+                   " at " + #x;                                          \
+      return -1;                                                         \
+    }                                                                    \
+  } while (0)
+
+namespace {
+
+// independent nodes and weights of element corner value g (>= 0: node g,
+// < 0: hanging node -(g+1) through its constraint)
+template <class F>
+void expand_node(int g, const std::vector<int> &dptr, const std::vector<int> &dconn,
+                 const std::vector<double> &dw, F fn) {
+  if (g >= 0) {
+    fn(g, 1.0);
+  } else {
+    for (int j = dptr[-g - 1]; j < dptr[-g]; j++) fn(dconn[j], dw[j]);
+  }
+}
+
+// trilinear weight of coarse corner c at fine corner a of a child placed at
+// `place` (level difference << 24 | z << 16 | y << 8 | x)
+double child_weight(int place, int a, int c) {
+  const int dl = (place >> 24) & 15;
+  const int off[3] = {place & 255, (place >> 8) & 255, (place >> 16) & 255};
+  double w = 1.0;
+  for (int d = 0; d < 3; d++) {
+    const double xi = std::ldexp((double)(off[d] + ((a >> d) & 1)), -dl);
+    w *= ((c >> d) & 1) ? xi : 1.0 - xi;
+  }
+  return w;
+}
+
+}  // namespace
+
+template <>
+BSR<double> Problem::bsr<double>(int l) {
+  Level &L = levels_[l];
+  BSR<double> A;
+  A.n = L.n_nodes;
+  A.nnzb = L.nnzb;
+  A.ptr = L.bsr_ptr.p;
+  A.col = L.bsr_col.p;
+  A.val = L.bsr64.p;
+  A.bcmask = L.d_bcmask.p;
+  return A;
+}
+template <>
+BSR<float> Problem::bsr<float>(int l) {
+  Level &L = levels_[l];
+  BSR<float> A;
+  A.n = L.n_nodes;
+  A.nnzb = L.nnzb;
+  A.ptr = L.bsr_ptr.p;
+  A.col = L.bsr_col.p;
+  A.val = L.bsr32.p;
+  A.bcmask = L.d_bcmask.p;
+  return A;
+}
+
+// The element-wise Galerkin product equals P^T A P only if, on every fine
+// element e inside coarse element E, the prolongation of each corner of e
+// (a hanging corner through its constraint) is the trilinear interpolation
+// of E's corners (each expanded through E's hanging constraints).  Checked
+// here once, exactly (the weights are dyadic).
+int Problem::build_galerkin() {
+  const int nl = (int)levels_.size();
+  for (int l = 0; l + 1 < nl; l++) {
+    Level &F = levels_[l], &Cc = levels_[l + 1];
+    InterpCSR P;
+    if (F.forest->create_interpolation(Cc.forest, P)) {
+      last_error = "galerkin: interpolation " + std::to_string(l) + ": " + F.forest->last_error;
+      return -1;
+    }
+    const auto &fdp = F.forest->dep_ptr();
+    const auto &fdc = F.forest->dep_conn();
+    const auto &fdw = F.forest->dep_weights();
+    const auto &cdp = Cc.forest->dep_ptr();
+    const auto &cdc = Cc.forest->dep_conn();
+    const auto &cdw = Cc.forest->dep_weights();
+    std::vector<double> acc(Cc.n_nodes, 0.0);
+    std::vector<int> touched;
+    auto add = [&](int node, double w) {
+      if (acc[node] == 0.0) touched.push_back(node);
+      acc[node] += w;
+    };
+    for (int E = 0; E < Cc.n_elem; E++) {
+      for (int k = F.sc_ptr_h[E]; k < F.sc_ptr_h[E + 1]; k++) {
+        const int e = F.sc_child_h[k], pl = F.place_h[k];
+        for (int a = 0; a < 8; a++) {
+          touched.clear();
+          // + prolongation of fine corner a
+          expand_node(F.conn[8 * e + a], fdp, fdc, fdw, [&](int r, double wr) {
+            for (int j = P.rowp[r]; j < P.rowp[r + 1]; j++) add(P.cols[j], wr * P.w[j]);
+          });
+          // - trilinear interpolation of E's corners
+          for (int c = 0; c < 8; c++) {
+            const double q = child_weight(pl, a, c);
+            if (q == 0.0) continue;
+            expand_node(Cc.conn[8 * E + c], cdp, cdc, cdw,
+                        [&](int node, double w) { add(node, -q * w); });
+          }
+          bool ok = true;
+          for (int node : touched) {
+            if (std::fabs(acc[node]) > 1e-12) ok = false;
+            acc[node] = 0.0;
+          }
+          if (!ok) {
+            last_error = "galerkin: the prolongation of level " + std::to_string(l) +
+                         " is not the trilinear interpolation inside coarse element " +
+                         std::to_string(E);
+            return -1;
+          }
+        }
+      }
+    }
+  }
+  for (int l = 1; l < nl; l++) {
+    Level &L = levels_[l];
+    if (L.KE.alloc(576 * (size_t)L.n_elem)) {
+      last_error = "galerkin: device allocation failed";
+      return -1;
+    }
+    if (l + 1 < nl && build_bsr(l)) return -1;
+  }
+  if (Kgd_.upload(Kgh_)) {
+    last_error = "galerkin: device allocation failed";
+    return -1;
+  }
+  galerkin_built_ = true;
+  return 0;
+}
+
+// BSR pattern of level l and, per block, the list of (element, corner a,
+// corner b, weight) terms w K_E[a][b] that sum to it: row i takes corner a
+// of every element whose expanded corner a holds i, column j likewise for
+// corner b; terms sorted by (column, element, a, b), so the sums run in a
+// fixed order.
+int Problem::build_bsr(int l) {
+  Level &L = levels_[l];
+  const auto &dp = L.forest->dep_ptr();
+  const auto &dc = L.forest->dep_conn();
+  const auto &dw = L.forest->dep_weights();
+  const int n = L.n_nodes;
+  std::vector<int64_t> rcnt(n + 1, 0);
+  for (int E = 0; E < L.n_elem; E++) {
+    int nexp = 0;
+    for (int b = 0; b < 8; b++) expand_node(L.conn[8 * E + b], dp, dc, dw, [&](int, double) { nexp++; });
+    for (int a = 0; a < 8; a++)
+      expand_node(L.conn[8 * E + a], dp, dc, dw, [&](int i, double) { rcnt[i + 1] += nexp; });
+  }
+  for (int i = 0; i < n; i++) rcnt[i + 1] += rcnt[i];
+  const int64_t nt = rcnt[n];
+  if (nt >= ((int64_t)1 << 31) || 64 * (int64_t)L.n_elem >= ((int64_t)1 << 31)) {
+    last_error = "galerkin: level too large for 32-bit term lists";
+    return -1;
+  }
+  struct Term {
+    int j, code;
+    double w;
+  };
+  std::vector<Term> terms(nt);
+  {
+    std::vector<int64_t> fill(rcnt.begin(), rcnt.end() - 1);
+    for (int E = 0; E < L.n_elem; E++)
+      for (int a = 0; a < 8; a++)
+        expand_node(L.conn[8 * E + a], dp, dc, dw, [&](int i, double wa) {
+          for (int b = 0; b < 8; b++)
+            expand_node(L.conn[8 * E + b], dp, dc, dw, [&](int j, double wb) {
+              terms[fill[i]++] = Term{j, 64 * E + 8 * a + b, wa * wb};
+            });
+        });
+  }
+  std::vector<int> ptr(n + 1, 0), col, brow, diag(n, -1), tptr{0}, tcode(nt);
+  std::vector<double> tw(nt);
+  int64_t t_out = 0;
+  for (int i = 0; i < n; i++) {
+    auto b0 = terms.begin() + rcnt[i], b1 = terms.begin() + rcnt[i + 1];
+    std::sort(b0, b1, [](const Term &x, const Term &y) {
+      return x.j != y.j ? x.j < y.j : x.code < y.code;
+    });
+    for (auto it = b0; it != b1; ++it) {
+      if (it == b0 || it->j != (it - 1)->j) {
+        if (it != b0) tptr.push_back((int)t_out);
+        if (it->j == i) diag[i] = (int)col.size();
+        col.push_back(it->j);
+        brow.push_back(i);
+      }
+      tcode[t_out] = it->code;
+      tw[t_out++] = it->w;
+    }
+    if (b1 != b0) tptr.push_back((int)t_out);
+    ptr[i + 1] = (int)col.size();
+    if (diag[i] < 0) {
+      last_error = "galerkin: node " + std::to_string(i) + " of level " + std::to_string(l) +
+                   " touches no element";
+      return -1;
+    }
+  }
+  L.nnzb = (int)col.size();
+  if (L.bsr_ptr.upload(ptr) || L.bsr_col.upload(col) || L.bsr_row.upload(brow) ||
+      L.bsr_diag.upload(diag) || L.bt_ptr.upload(tptr) || L.bt_code.upload(tcode) ||
+      L.bt_w.upload(tw) || L.bsr64.alloc(9 * (size_t)L.nnzb) || L.bsr32.alloc(9 * (size_t)L.nnzb)) {
+    last_error = "galerkin: device allocation failed";
+    return -1;
+  }
+  L.has_bsr = true;
+  return 0;
+}
+
+int Problem::set_galerkin(bool on) {
+  if (on && !galerkin_built_ && build_galerkin()) return -1;
+  if (on != galerkin_) {
+    galerkin_ = on;
+    have_setup_ = false;
+    drop_graphs();
+    gen_++;
+  }
+  return 0;
+}
+
+// per design: K_E level by level, then the BSR values (FP64, and FP32 for
+// the mixed-precision V-cycle)
+int Problem::galerkin_setup(int precision) {
+  const int nl = (int)levels_.size();
+  const bool nodal = design_mode_ == 1;
+  for (int l = 1; l < nl; l++) {
+    Level &F = levels_[l - 1], &C = levels_[l];
+    const int mode = l > 1 ? 1 : (nodal ? 2 : 0);
+    const double *s = l > 1 ? nullptr : (nodal ? F.sg64.p : F.s64.p);
+    const double *Ktab = nodal ? Kgd_.p : K0d_.p;
+    launch_galerkin_elem(mode, C.n_elem, F.sc_ptr.p, F.sc_child.p, F.g_place.p, F.d_cls.p, s, Ktab,
+                         l > 1 ? F.KE.p : nullptr, C.KE.p, st_);
+    if (C.has_bsr) {
+      launch_bsr_assemble(bsr<double>(l), C.bt_ptr.p, C.bt_code.p, C.bt_w.p, C.bsr_row.p, C.KE.p,
+                          C.bsr64.p, st_);
+      if (precision == 1)
+        launch_convert<double, float>(9 * (int64_t)C.nnzb, C.bsr64.p, C.bsr32.p, st_);
+    }
+  }
+  GCHK(hipGetLastError());
+  return 0;
+}
+
+int Problem::level_matrix(int l, std::vector<int> &ptr, std::vector<int> &col,
+                          std::vector<double> &val) {
+  if (!galerkin_ || l < 1 || l >= (int)levels_.size() || !levels_[l].has_bsr) {
+    last_error = "level_matrix: no Galerkin operator on that level";
+    return -1;
+  }
+  if (!have_setup_) {
+    last_error = "level_matrix: run setup first";
+    return -1;
+  }
+  Level &L = levels_[l];
+  ptr.resize(L.n_nodes + 1);
+  col.resize(L.nnzb);
+  val.resize(9 * (size_t)L.nnzb);
+  GCHK(hipStreamSynchronize(st_));
+  GCHK(hipMemcpy(ptr.data(), L.bsr_ptr.p, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
+  GCHK(hipMemcpy(col.data(), L.bsr_col.p, col.size() * sizeof(int), hipMemcpyDeviceToHost));
+  GCHK(hipMemcpy(val.data(), L.bsr64.p, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // namespace tmrx

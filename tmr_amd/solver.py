@@ -208,6 +208,32 @@ class Problem:
             raise _lib.TmrxError("solve_multi: load cases %s did not converge" % bad)
         return U
 
+    def set_multigrid(self, omega=1.0, use_galerkin=False, use_coarse_direct_solve=True,
+                      use_chebyshev_smoother=True):
+        """TMR_CreateTACSMg's options on this hierarchy (tmrx_mg_create):
+        Galerkin coarse operators P^T A P or rediscretised coarse levels."""
+        check(lib().tmrx_mg_create(self.ptr, float(omega), int(bool(use_galerkin)),
+                                   int(bool(use_coarse_direct_solve)),
+                                   int(bool(use_chebyshev_smoother))), "set_multigrid")
+        self.galerkin = bool(use_galerkin)
+        self._bump()
+
+    def level_matrix(self, level):
+        """The Galerkin operator of a coarse level as a scipy BSR matrix
+        (3 x 3 blocks; needs set_multigrid(use_galerkin=True) and a setup)."""
+        import scipy.sparse as sp
+        n, nb = C.c_int(), C.c_int()
+        check(lib().tmrx_problem_level_matrix(self.ptr, int(level), C.byref(n), C.byref(nb),
+                                              None, None, None), "level_matrix")
+        ptr = np.zeros(n.value + 1, np.intc)
+        col = np.zeros(nb.value, np.intc)
+        val = np.zeros((nb.value, 3, 3))
+        check(lib().tmrx_problem_level_matrix(self.ptr, int(level), None, None,
+                                              C.c_void_p(ptr.ctypes.data),
+                                              C.c_void_p(col.ctypes.data),
+                                              C.c_void_p(val.ctypes.data)), "level_matrix")
+        return sp.bsr_matrix((val, col, ptr), shape=(3 * n.value, 3 * n.value))
+
     def num_components(self):
         n = C.c_int()
         check(lib().tmrx_problem_num_components(self.ptr, C.byref(n)))
