@@ -168,3 +168,25 @@ def test_element_center_weights_same_forest():
     rp, cl, w, nc = f.createElementCenterWeights(g)
     assert np.all(np.diff(rp) == 8) and np.all(w == 0.125)
     assert np.array_equal(cl.reshape(-1, 8), np.sort(f.getMeshConn(), axis=1))
+
+
+def test_c2_forest_maps_bit_exact():
+    """The adaptive benchmark forest at full size (1,227,458 hex, levels 3-7,
+    2:1 balanced with corners) and its first coarse level: element
+    connectivity, hanging-node pointers, parents and weights, and the
+    interpolation CSR equal the geometric oracle's bit for bit."""
+    from tmr_amd import workloads
+    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=2)
+    fine, coarse = forests
+    assert len(fine.getOctants()) == 1227458
+    fm, cm, (rp, cl, w) = po.interpolation((2, 2, 2), fine.getOctants(), coarse.getOctants())
+    assert np.array_equal(np.asarray(fine.getMeshConn()).reshape(-1, 8), fm.conn)
+    dp, dc, dw = fine.getDepNodeConn()
+    assert len(dp) - 1 == fm.ndep and fm.ndep > 100000
+    assert np.array_equal(dp, fm.dep_ptr)
+    assert np.array_equal(dc, fm.dep_conn)
+    assert np.array_equal(dw, fm.dep_w)
+    prp, pcl, pw = fine.createInterpolation(coarse)
+    assert np.array_equal(prp, rp)
+    assert np.array_equal(pcl, cl)
+    assert np.array_equal(pw, w)

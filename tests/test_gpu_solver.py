@@ -130,3 +130,26 @@ def test_asymmetric_cycles_converge(pre, post, cube):
     print("pre %d post %d iters %d" % (pre, post, prob.info.iters), flush=True)
     assert prob.true_residual() < 1e-9
     assert abs(prob.compliance() - c_ref) / abs(c_ref) < 1e-6
+
+
+@pytest.mark.parametrize("level", [2, 3])
+def test_single_level_is_a_direct_solve(level):
+    """With one level the preconditioner is the coarsest-level solve alone:
+    the explicit inverse from the GPU's blocked Cholesky (375 dofs in 6
+    blocks of 64 at level 2, 2,187 dofs in 35 blocks at level 3, padding
+    included), so PCG converges in one iteration to the oracle's solution."""
+    from tmr_amd import TMR
+    f0 = TMR.createBoxForest(1, 1, 1, (1.0, 1.0, 1.0), face_names=workloads.FACES)
+    f0.createTrees(level)
+    forests = workloads.hierarchy(f0, 1)
+    rho = np.random.default_rng(level).uniform(0.1, 1.0, len(forests[0].getOctants()))
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_design(rho)
+    f = _load(prob)
+    u = prob.solve(f, opts=solver.default_opts(rtol=1e-12, precision=0))
+    orc = _oracle(forests[0], rho)
+    u_ref = orc.solve(f, rtol=1e-14)
+    print("level %d: %d dofs, %d iterations, true res %.2e" % (level, prob.ndof, prob.info.iters,
+          prob.true_residual()), flush=True)
+    assert prob.info.iters <= 2
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-10

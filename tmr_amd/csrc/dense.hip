@@ -1,0 +1,260 @@
+// Coarsest-level direct solve on the GPU: blocked FP64 Cholesky A = L L^T of
+// a dense symmetric matrix and the explicit inverse A^-1 = L^-T L^-1 (the
+// coarse solve of every V-cycle is then one GEMV, launch_dense_solve).
+// Replaces TACSMg's direct coarse solve (src/TMR_RefinementTools.cpp:83-86).
+//
+// Matrices are row-major mp x mp with mp a multiple of DB (padding rows and
+// columns hold the identity), processed in DB x DB blocks:
+//   Cholesky (right-looking): factor block (k,k); solve the panel below it;
+//     update the trailing lower blocks;
+//   L^-1 = X (block forward substitution): X[i][i] = L[i][i]^-1,
+//     X[i][j] = -X[i][i] sum_{k=j}^{i-1} L[i][k] X[k][j];
+//   A^-1 = X^T X, lower blocks then mirrored, written into the m x m result.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace tmrx {
+
+constexpr int DB = 64;  // block size
+constexpr int DT = 256;  // threads per workgroup
+
+// factor the diagonal block k in LDS; *bad = 1 on a non-positive pivot
+__global__ void __launch_bounds__(DT) k_potrf_diag(double *__restrict__ W, int mp, int k,
+                                                   int *bad) {
+  __shared__ double a[DB][DB + 1];
+  const int tid = threadIdx.x;
+  double *blk = W + (size_t)(k * DB) * mp + k * DB;
+  for (int t = tid; t < DB * DB; t += DT) a[t / DB][t % DB] = blk[(size_t)(t / DB) * mp + t % DB];
+  __syncthreads();
+  for (int j = 0; j < DB; j++) {
+    const double d = a[j][j];
+    __syncthreads();
+    if (!(d > 0.0)) {
+      if (tid == 0) atomicOr(bad, 1);
+      return;
+    }
+    const double s = sqrt(d);
+    if (tid == 0) a[j][j] = s;
+    for (int i = j + 1 + tid; i < DB; i += DT) a[i][j] /= s;
+    __syncthreads();
+    // trailing update of the lower part: a[i][l] -= a[i][j] a[l][j], j < l <= i
+    const int rem = DB - 1 - j;
+    for (int t = tid; t < rem * rem; t += DT) {
+      const int i = j + 1 + t / rem, l = j + 1 + t % rem;
+      if (l <= i) a[i][l] -= a[i][j] * a[l][j];
+    }
+    __syncthreads();
+  }
+  for (int t = tid; t < DB * DB; t += DT) {
+    const int i = t / DB, l = t % DB;
+    blk[(size_t)i * mp + l] = l <= i ? a[i][l] : 0.0;
+  }
+}
+
+// rows below block k: x L_kk^T = a, one row per thread
+__global__ void __launch_bounds__(DT) k_trsm_panel(double *__restrict__ W, int mp, int k) {
+  __shared__ double L[DB][DB + 1];
+  const int tid = threadIdx.x;
+  const double *lk = W + (size_t)(k * DB) * mp + k * DB;
+  for (int t = tid; t < DB * DB; t += DT) L[t / DB][t % DB] = lk[(size_t)(t / DB) * mp + t % DB];
+  __syncthreads();
+  const int r = (k + 1) * DB + blockIdx.x * DT + tid;
+  if (r >= mp) return;
+  double *row = W + (size_t)r * mp + k * DB;
+  double x[DB];
+#pragma unroll
+  for (int j = 0; j < DB; j++) x[j] = row[j];
+#pragma unroll
+  for (int j = 0; j < DB; j++) {
+    double v = x[j];
+#pragma unroll
+    for (int t = 0; t < j; t++) v -= x[t] * L[j][t];
+    x[j] = v / L[j][j];
+  }
+#pragma unroll
+  for (int j = 0; j < DB; j++) row[j] = x[j];
+}
+
+// C[I][J] -= A[I][k] A[J][k]^T for the trailing lower blocks I >= J > k
+__global__ void __launch_bounds__(DT) k_syrk_trailing(double *__restrict__ W, int mp, int k) {
+  __shared__ double PI[DB][DB + 1], PJ[DB][DB + 1];
+  // lower-triangle tile index -> (I, J) over the T = nblk - k - 1 trailing blocks
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) I++;
+  while (I * (I + 1) / 2 > t) I--;
+  const int J = t - I * (I + 1) / 2;
+  const int bi = (k + 1 + I) * DB, bj = (k + 1 + J) * DB, bk = k * DB;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < DB * DB; q += DT) {
+    PI[q / DB][q % DB] = W[(size_t)(bi + q / DB) * mp + bk + q % DB];
+    PJ[q / DB][q % DB] = W[(size_t)(bj + q / DB) * mp + bk + q % DB];
+  }
+  __syncthreads();
+  // 4 x 4 outputs per thread
+  const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
+  double acc[4][4] = {};
+  for (int kk = 0; kk < DB; kk++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a[u] = PI[r0 + u][kk];
+      b[u] = PJ[c0 + u][kk];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) W[(size_t)(bi + r0 + u) * mp + bj + c0 + v] -= acc[u][v];
+}
+
+// X[i][i] = L[i][i]^-1 (lower): column c solves L x = e_c
+__global__ void __launch_bounds__(DB) k_trtri_diag(const double *__restrict__ W,
+                                                   double *__restrict__ X, int mp, int i) {
+  __shared__ double L[DB][DB + 1];
+  const int tid = threadIdx.x;
+  const size_t o = (size_t)(i * DB) * mp + i * DB;
+  for (int t = tid; t < DB * DB; t += DB) L[t / DB][t % DB] = W[o + (size_t)(t / DB) * mp + t % DB];
+  __syncthreads();
+  const int c = tid;
+  double x[DB];
+#pragma unroll
+  for (int r = 0; r < DB; r++) {
+    double v = r == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int t = 0; t < r; t++) v -= L[r][t] * x[t];
+    x[r] = r < c ? 0.0 : v / L[r][r];
+  }
+#pragma unroll
+  for (int r = 0; r < DB; r++) X[o + (size_t)r * mp + c] = x[r];
+}
+
+// X[i][j] = -X[i][i] sum_{k=j}^{i-1} L[i][k] X[k][j], one workgroup per j < i
+__global__ void __launch_bounds__(DT) k_trtri_row(const double *__restrict__ W,
+                                                  double *__restrict__ X, int mp, int i) {
+  __shared__ double A[DB][DB + 1], B[DB][DB + 1];
+  const int j = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
+  double acc[4][4] = {};
+  for (int k = j; k < i; k++) {
+    __syncthreads();
+    for (int q = tid; q < DB * DB; q += DT) {
+      A[q / DB][q % DB] = W[(size_t)(i * DB + q / DB) * mp + k * DB + q % DB];
+      B[q / DB][q % DB] = X[(size_t)(k * DB + q / DB) * mp + j * DB + q % DB];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < DB; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = A[r0 + u][kk];
+        b[u] = B[kk][c0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+    }
+  }
+  // T -> LDS, then X[i][j] = -X[i][i] T
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) B[r0 + u][c0 + v] = acc[u][v];
+  for (int q = tid; q < DB * DB; q += DT)
+    A[q / DB][q % DB] = X[(size_t)(i * DB + q / DB) * mp + i * DB + q % DB];
+  __syncthreads();
+  double out[4][4] = {};
+  for (int kk = 0; kk < DB; kk++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a[u] = A[r0 + u][kk];
+      b[u] = B[kk][c0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) out[u][v] -= a[u] * b[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) X[(size_t)(i * DB + r0 + u) * mp + j * DB + c0 + v] = out[u][v];
+}
+
+// A^-1[I][J] = sum_{k >= I} X[k][I]^T X[k][J] for I >= J (X lower); the m x m
+// result (row-major) takes both the block and its mirror
+__global__ void __launch_bounds__(DT) k_gram(const double *__restrict__ X, int mp, int nblk,
+                                             int m, double *__restrict__ out) {
+  __shared__ double A[DB][DB + 1], B[DB][DB + 1];
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) I++;
+  while (I * (I + 1) / 2 > t) I--;
+  const int J = t - I * (I + 1) / 2;
+  const int tid = threadIdx.x;
+  const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
+  double acc[4][4] = {};
+  for (int k = I; k < nblk; k++) {
+    __syncthreads();
+    for (int q = tid; q < DB * DB; q += DT) {
+      A[q / DB][q % DB] = X[(size_t)(k * DB + q / DB) * mp + I * DB + q % DB];
+      B[q / DB][q % DB] = X[(size_t)(k * DB + q / DB) * mp + J * DB + q % DB];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < DB; kk++) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        a[u] = A[kk][r0 + u];
+        b[u] = B[kk][c0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = I * DB + r0 + u, c = J * DB + c0 + v;
+      if (r < m && c < m) {
+        out[(size_t)r * m + c] = acc[u][v];
+        out[(size_t)c * m + r] = acc[u][v];
+      }
+    }
+}
+
+int dense_padded(int m) { return (m + DB - 1) / DB * DB; }
+
+void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st) {
+  const int mp = dense_padded(m), nblk = mp / DB;
+  if (m == 0) return;
+  for (int k = 0; k < nblk; k++) {
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(DT), 0, st, W, mp, k, bad);
+    const int rows = mp - (k + 1) * DB;
+    if (rows > 0) {
+      hipLaunchKernelGGL(k_trsm_panel, dim3((rows + DT - 1) / DT), dim3(DT), 0, st, W, mp, k);
+      const int T = nblk - k - 1;
+      hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(DT), 0, st, W, mp, k);
+    }
+  }
+  for (int i = 0; i < nblk; i++) {
+    hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(DB), 0, st, W, X, mp, i);
+    if (i > 0) hipLaunchKernelGGL(k_trtri_row, dim3(i), dim3(DT), 0, st, W, X, mp, i);
+  }
+  hipLaunchKernelGGL(k_gram, dim3(nblk * (nblk + 1) / 2), dim3(DT), 0, st, X, mp, nblk, m, out);
+  TMRX_LAUNCH_CHECK();
+}
+
+}  // namespace tmrx

@@ -220,6 +220,14 @@ struct CoarseBlocks {
 };
 template <class T>
 void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs = 1);
+// rows and columns of the padded dense matrices of launch_dense_inverse
+int dense_padded(int m);
+// explicit inverse of an SPD matrix by blocked Cholesky on the device: W
+// holds the matrix row-major with dense_padded(m) rows and columns (identity
+// on the padding) and is overwritten by the factor; X (same size) receives
+// L^-1; out (m x m, row-major) receives A^-1; *bad |= 1 on a non-positive
+// pivot
+void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st);
 
 // Galerkin coarse operator of one level: 3 x 3 blocks over the independent
 // nodes (block row i = node i), row-major blocks, clamped rows and columns
@@ -264,6 +272,10 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st);
 // v[r][i] += alpha y[i] for every right-hand side r (vectors of n values)
 void launch_axpy_bcast(int64_t n, int nrhs, double alpha, const double *y, double *v,
                        hipStream_t st);
+// power-iteration start vector: 0.5 + a hash-drawn U[0, 1) per free dof,
+// 0 on clamped dofs
+void launch_start_vector(int64_t n, const uint8_t *bcmask, uint64_t seed, double *x,
+                         hipStream_t st);
 // *bad |= 1 if any of v[0..n) is not finite or outside [lo, hi]
 void launch_check_range(int64_t n, const double *v, double lo, double hi, int *bad,
                         hipStream_t st);

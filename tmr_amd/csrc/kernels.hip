@@ -554,6 +554,28 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
   TMRX_LAUNCH_CHECK();
 }
 
+// power-iteration start: x_i = 0.5 + U_i on free dofs (U_i in [0, 1) from
+// the splitmix64 hash of seed + i, so every dof draws independently), 0 on
+// clamped ones
+__global__ void k_start_vector(int64_t n, const uint8_t *__restrict__ bcmask, uint64_t seed,
+                               double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ULL * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  x[i] = ((bcmask[i / 3] >> (i % 3)) & 1) ? 0.0 : 0.5 + (double)(z >> 11) * 0x1.0p-53;
+}
+
+void launch_start_vector(int64_t n, const uint8_t *bcmask, uint64_t seed, double *x,
+                         hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_start_vector, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     bcmask, seed, x);
+  TMRX_LAUNCH_CHECK();
+}
+
 // *bad |= 1 when an entry is not finite or lies outside [lo, hi] (a NaN
 // fails both comparisons)
 __global__ void k_check_range(int64_t n, const double *__restrict__ v, double lo, double hi,

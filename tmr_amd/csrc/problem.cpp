@@ -1114,21 +1114,14 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
 }
 
 int Problem::estimate_eigs(int l, const SolveOpts &o) {
-  // power iteration on Dinv A (FP64) from a fixed pseudo-random start,
+  // power iteration on Dinv A (FP64) from a fixed pseudo-random start
+  // (drawn on the device),
   // lambda_max ~ |Dinv A x_k| / |x_k| after eig_iters steps (x_{k+1} =
   // Dinv A x_k, no renormalisation: the growth is bounded by lambda^k)
   Level &L = levels_[l];
   auto &v = L.v64;
   const int64_t n = L.ndof;
-  std::vector<double> h(n);
-  uint64_t s = 0x9e3779b97f4a7c15ULL ^ (uint64_t)(l + 1);
-  for (int64_t i = 0; i < n; i++) {
-    s ^= s << 13;
-    s ^= s >> 7;
-    s ^= s << 17;
-    h[i] = ((L.bcmask[i / 3] >> (i % 3)) & 1) ? 0.0 : 0.5 + (double)(s >> 11) * 0x1.0p-53;
-  }
-  HIPCHK(hipMemcpyAsync(v.x.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice, st_));
+  launch_start_vector(n, L.d_bcmask.p, 0x2545f4914f6cdd1dULL * (uint64_t)(l + 1), v.x.p, st_);
   double *a = v.x.p, *b = v.r.p;
   const int iters = std::max(1, o.eig_iters);
   for (int it = 0; it < iters; it++) {
@@ -1246,12 +1239,31 @@ int Problem::coarse_factor() {
   std::vector<int> bseg(seg_block0_.back());
   for (int sg = 0; sg < nseg_; sg++)
     for (int b = seg_block0_[sg]; b < seg_block0_[sg + 1]; b++) bseg[b] = sg;
-  std::vector<double> Aall;
   std::vector<int> col0(n), nb(n);
   std::vector<int64_t> aoff(n);
+  // component blocks of the inverse, one after the other
+  std::vector<size_t> base(nseg_ + 1, 0);
+  size_t wmax = 0;
+  for (int sg = 0; sg < nseg_; sg++) {
+    const size_t m = 3 * (size_t)(L.seg_node0[sg + 1] - L.seg_node0[sg]);
+    base[sg + 1] = base[sg] + m * m;
+    const size_t mp = (size_t)dense_padded((int)m);
+    wmax = std::max(wmax, mp * mp);
+  }
+  if ((Ainv_.n < base[nseg_] && Ainv_.alloc(base[nseg_])) ||
+      (cw_.n < wmax && (cw_.alloc(wmax) || cx_.alloc(wmax))) || (chk_.n == 0 && chk_.alloc(1))) {
+    last_error = "coarse: device allocation failed";
+    return -1;
+  }
+  HIPCHK(hipMemsetAsync(chk_.p, 0, sizeof(int), st_));
+  std::vector<double> A;
   for (int sg = 0; sg < nseg_; sg++) {
     const int d0 = 3 * L.seg_node0[sg], m = 3 * (L.seg_node0[sg + 1] - L.seg_node0[sg]);
-    std::vector<double> A((size_t)m * m, 0.0);
+    const int mp = dense_padded(m);
+    // dense K of the component, row stride mp, identity on clamped dofs and
+    // on the padding
+    HIPCHK(hipStreamSynchronize(st_));  // the previous component's upload buffer is free
+    A.assign((size_t)mp * mp, 0.0);
     for (int e = 0; e < L.n_elem; e++) {
       if (bseg[octs[e].block] != sg) continue;
       const double *K = &K0h_[576 * L.cls[e]];
@@ -1287,60 +1299,32 @@ int Problem::coarse_factor() {
                 if ((L.bcmask[pa.first] >> i) & 1) continue;
                 for (int j = 0; j < 3; j++) {
                   if ((L.bcmask[pb.first] >> j) & 1) continue;
-                  A[(size_t)(3 * pa.first + i - d0) * m + 3 * pb.first + j - d0] +=
+                  A[(size_t)(3 * pa.first + i - d0) * mp + 3 * pb.first + j - d0] +=
                       w * K[(3 * a + i) * 24 + 3 * bb + j];
                 }
               }
             }
     }
-    for (int i = 0; i < m; i++)
-      if ((L.bcmask[(d0 + i) / 3] >> ((d0 + i) % 3)) & 1) A[(size_t)i * m + i] = 1.0;
-    // Cholesky A = L L^T (lower, row-major, in place)
-    for (int j = 0; j < m; j++) {
-      double *Aj = &A[(size_t)j * m];
-      double d = Aj[j];
-      for (int k = 0; k < j; k++) d -= Aj[k] * Aj[k];
-      if (!(d > 0.0)) {
-        last_error = "coarse Cholesky: matrix of component " + std::to_string(sg) +
-                     " is not positive definite (missing boundary conditions?)";
-        return -1;
-      }
-      d = std::sqrt(d);
-      Aj[j] = d;
-      const double inv = 1.0 / d;
-      for (int i = j + 1; i < m; i++) {
-        double *Ai = &A[(size_t)i * m];
-        double t = Ai[j];
-        for (int k = 0; k < j; k++) t -= Ai[k] * Aj[k];
-        Ai[j] = t * inv;
-      }
-    }
-    // Linv (lower), then Ainv = Linv^T Linv
-    std::vector<double> Li((size_t)m * m, 0.0);
-    for (int c = 0; c < m; c++) {
-      Li[(size_t)c * m + c] = 1.0 / A[(size_t)c * m + c];
-      for (int i = c + 1; i < m; i++) {
-        double t = 0.0;
-        for (int k = c; k < i; k++) t -= A[(size_t)i * m + k] * Li[(size_t)k * m + c];
-        Li[(size_t)i * m + c] = t / A[(size_t)i * m + i];
-      }
-    }
-    const size_t base = Aall.size();
-    Aall.resize(base + (size_t)m * m);
-    for (int i = 0; i < m; i++)
-      for (int j = 0; j <= i; j++) {
-        double t = 0.0;
-        for (int k = i; k < m; k++) t += Li[(size_t)k * m + i] * Li[(size_t)k * m + j];
-        Aall[base + (size_t)i * m + j] = t;
-        Aall[base + (size_t)j * m + i] = t;
-      }
+    for (int i = 0; i < mp; i++)
+      if (i >= m || ((L.bcmask[(d0 + i) / 3] >> ((d0 + i) % 3)) & 1)) A[(size_t)i * mp + i] = 1.0;
+    // Cholesky and explicit inverse on the GPU (FP64)
+    HIPCHK(hipMemcpyAsync(cw_.p, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+    launch_dense_inverse(cw_.p, cx_.p, m, Ainv_.p + base[sg], chk_.p, st_);
     for (int i = 0; i < m; i++) {
       col0[d0 + i] = d0;
       nb[d0 + i] = m;
-      aoff[d0 + i] = (int64_t)(base + (size_t)i * m);
+      aoff[d0 + i] = (int64_t)(base[sg] + (size_t)i * m);
     }
   }
-  if (Ainv_.upload(Aall) || cb_col0_.upload(col0) || cb_nb_.upload(nb) || cb_aoff_.upload(aoff)) {
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, chk_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  if (bad) {
+    last_error = "coarse Cholesky: the coarsest-level matrix is not positive definite "
+                 "(missing boundary conditions?)";
+    return -1;
+  }
+  if (cb_col0_.upload(col0) || cb_nb_.upload(nb) || cb_aoff_.upload(aoff)) {
     last_error = "coarse: device allocation failed";
     return -1;
   }

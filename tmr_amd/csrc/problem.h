@@ -309,6 +309,7 @@ class Problem {
   const uint8_t *state_mask() const { return has_ubc_ ? nomask_.p : levels_[0].d_bcmask.p; }
   DevArray<double> rho_;
   DevArray<double> Ainv_;
+  DevArray<double> cw_, cx_;  // coarse factorisation work (padded matrix, L^-1)
   DevArray<int> cb_col0_, cb_nb_;
   DevArray<int64_t> cb_aoff_;
   CoarseBlocks coarse_;

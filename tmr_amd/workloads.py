@@ -123,19 +123,20 @@ def _field_gradient(c, h, seed, L):
 
 
 def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), max_sweeps=None,
-             base_median=False):
+             base_median=False, quantile=0.5, face_names=FACES):
     """Adaptive forest on 2x2x2 blocks: octants where the seeded density has a
     gradient above its median are refined towards a seeded level in
     [base_level, max_level]; balance(1) after each sweep.  The median is the
     one of the gradient over the current octants (base_median False) or over
     the centres of the uniform base-level octants, fixed for every sweep
-    (base_median True: a median of the field itself, the C2 rule)."""
+    (base_median True: a median of the field itself, the C2 rule); with
+    base_median, `quantile` replaces the median by another quantile."""
     rng = np.random.default_rng(seed + 1000)
-    f = TMR.createBoxForest(2, 2, 2, L, face_names=FACES)
+    f = TMR.createBoxForest(2, 2, 2, L, face_names=face_names)
     f.createTrees(base_level)
     sweeps = max_level - base_level if max_sweeps is None else max_sweeps
     c, h, _ = _centres(f, L)
-    med0 = np.median(_field_gradient(c, h, seed, L))
+    med0 = np.quantile(_field_gradient(c, h, seed, L), quantile)
     for _ in range(sweeps):
         c, h, octs = _centres(f, L)
         g = _field_gradient(c, h, seed, L)
@@ -159,6 +160,85 @@ def c2_adaptive(seed=0, nlevels=5):
     32-Gaussian density exceeds its median, balance(1) after each sweep;
     `nlevels` multigrid forests."""
     return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True)
+
+
+ALL_FACES = {"x-": "fixed", "x+": "load", "y-": "y-", "y+": "y+", "z-": "z-", "z+": "z+"}
+
+
+def c5a_adaptive(seed=1, nlevels=5):
+    """The multi-load-case forest: the C2 generator at the 0.1 quantile of the
+    field gradient (2,160,152 hex at seed 1), every outer face named
+    (x- clamped)."""
+    return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True,
+                    quantile=0.1, face_names=ALL_FACES)
+
+
+def face_load(forest, faces, tr):
+    """Consistent nodal load of a constant traction tr on element faces
+    [(element, local face), ...]: the exact bilinear face integral t * area / 4
+    per face corner, hanging corners passing their share to their parents
+    (the rule of TopOptUtils.compute3DTractionLoad on box faces)."""
+    conn = forest.getMeshConn()
+    ext = forest.getElementExtents()
+    dp, dc, dw = forest.getDepNodeConn()
+    n = int(forest.getNodeRange()[1])
+    f = np.zeros((n, 3))
+    tr = np.asarray(tr, dtype=np.double)
+    for e, face in faces:
+        ax = face // 2
+        area = ext[e][(ax + 1) % 3] * ext[e][(ax + 2) % 3]
+        for k in range(8):
+            if [(k & 1), (k >> 1) & 1, (k >> 2)][ax] != face % 2:
+                continue
+            g = conn[e][k]
+            if g >= 0:
+                f[g] += 0.25 * area * tr
+            else:
+                d = -g - 1
+                for j in range(dp[d], dp[d + 1]):
+                    f[dc[j]] += dw[j] * 0.25 * area * tr
+    return f.reshape(-1)
+
+
+def patch_loads(forest, count, seed, L=(2.0, 1.0, 1.0), names=("x+", "y-", "y+", "z-", "z+")):
+    """`count` load cases, each a unit traction in a seeded direction on a
+    seeded disc (radius 0.1 .. 0.3) of one seeded outer face (never the
+    clamped one).  Returns F [count, ndof] and the patch parameters."""
+    rng = np.random.default_rng(seed)
+    octs = forest.getOctants()
+    cen, h, _ = _centres(forest, L)
+    side = {"x-": (0, 0), "x+": (0, 1), "y-": (1, 0), "y+": (1, 1), "z-": (2, 0), "z+": (2, 1)}
+    faces_of = {nm: [(int(o["tag"]), int(o["info"])) for o in forest.getOctsWithName(nm)]
+                for nm in names}
+    F, params = [], []
+    for _ in range(count):
+        while True:
+            nm = names[rng.integers(len(names))]
+            ax, hi = side[nm]
+            c = rng.uniform(0.0, 1.0, 3) * np.array(L)
+            c[ax] = L[ax] if hi else 0.0
+            r = rng.uniform(0.1, 0.3)
+            t = rng.standard_normal(3)
+            t /= np.linalg.norm(t)
+            sel = []
+            for e, fi in faces_of[nm]:
+                p = cen[e].copy()
+                p[ax] = c[ax]
+                if np.linalg.norm(p - c) <= r:
+                    sel.append((e, fi))
+            if sel:
+                break
+        F.append(face_load(forest, sel, t))
+        params.append({"face": nm, "centre": c.tolist(), "radius": r, "traction": t.tolist(),
+                       "element_faces": len(sel)})
+    return np.array(F), params
+
+
+def c5b_edges(count=96, seed=0):
+    """Edge lengths (elements) of the uneven batch, drawn from the seed among
+    8, 16, ..., 48: every cube is k x k x k blocks at level 3 (edge 8k), so
+    each has a four-level nested hierarchy down to k^3 level-0 blocks."""
+    return 8 * np.random.default_rng(seed + 77).integers(1, 7, size=count)
 
 
 def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
@@ -211,7 +291,9 @@ def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
 
 
 def cube_batch(count=96, level=3, nlevels=4, seed=0):
-    """`count` cubes of edge 8k elements (k = 1..6 in turn: k x k x k blocks
-    at level 3), the uneven-size batch of independent problems."""
-    ks = [1 + (i % 6) for i in range(count)]
-    return box_batch([(k, k, k) for k in ks], level, nlevels, seed=seed)
+    """`count` cubes with seeded edges 8k elements (c5b_edges: k x k x k
+    blocks at level 3), the uneven-size batch of independent problems;
+    returns (forests, rho, edges)."""
+    ks = [int(n) // 8 for n in c5b_edges(count, seed)]
+    forests, rho = box_batch([(k, k, k) for k in ks], level, nlevels, seed=seed)
+    return forests, rho, [8 * k for k in ks]
