@@ -41,77 +41,108 @@ TILE_CANDIDATES = (256, 384, 512, 768, 1024)  # owner-tile sizes the autotuner t
 COPY_FETCH_FACTOR = 1048576.0 / 524318.25
 COPY_WRITE_FACTOR = 1048576.0 / 1048576.0
 MEASURED_TRAFFIC_KIB = {("valu", 1024): (173205.2, 66824.0)}
+# the kernel source the counters were measured on (sha256 of
+# tmr_amd/csrc/kernels.hip, first 16 hex digits): after any edit of it the
+# stored traffic no longer describes the kernel and is not reported
+MEASURED_TRAFFIC_SRC = "unmeasured"
+
+
+def kernel_source_hash():
+    import hashlib
+    with open(os.path.join(ROOT, "tmr_amd", "csrc", "kernels.hip"), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(level=5, iters=None):
-    """FP64 oracle (assembled CSR + Jacobi PCG) on the same box cantilever at
-    a smaller level, on all host cores (OpenMP, at most 16 threads: the GPU
-    host's share) and on one core; reports MDOF*iter/s."""
+def cpu_baseline(level=5):
+    """The FP64 oracle's multigrid (oracle/mg_oracle.c) on the same box
+    cantilever at a smaller level (bounded CPU sample): Galerkin coarse
+    operators, flexible PCG with a Chebyshev degree-2 V-cycle (the GPU's FP64
+    leg), and GMRES(50) with a symmetric-SOR V-cycle (TMRTopoProblem's
+    solver), to 1e-8, on the host's cores (OpenMP, at most 16 threads: the
+    GPU host's share) and on one core."""
     from oracle import pyoracle as po
     from tmr_amd import workloads
-    forests, rho, L = workloads.box(level=level, nlevels=1, seed=0)
+    forests, rho, L = workloads.box(level=level, nlevels=level - 1, seed=0)
     f0 = forests[0]
     orc = po.from_forest(f0, [("fixed", 7)])
+    t0 = time.perf_counter()
     orc.assemble(rho ** 3.0 + 1e-6)
-    conn = f0.getMeshConn()
-    ext = f0.getElementExtents()
-    f = np.zeros(3 * orc.nnodes)
-    for o in f0.getOctsWithName("load"):
-        e, face = o["tag"], o["info"]
-        ax = face // 2
-        area = ext[e][(ax + 1) % 3] * ext[e][(ax + 2) % 3]
-        for k in range(8):
-            bit = [(k & 1), (k >> 1) & 1, (k >> 2)][ax]
-            if bit == face % 2:
-                f[3 * conn[e][k] + 2] += -0.25 * area
+    t_asm = time.perf_counter() - t0
+    f = workloads.face_load(f0, [(int(o["tag"]), int(o["info"])) for o in f0.getOctsWithName("load")],
+                            [0.0, 0.0, -1.0])
     ndof = 3 * orc.nnodes
     nthreads = max(1, min(16, os.cpu_count() or 1))
-    rates = {}
+    out = {}
     for th in (nthreads, 1):
         po.set_threads(th)
         t0 = time.perf_counter()
-        try:
-            orc.solve(f, rtol=1e-8, maxit=iters or 20000)
-            it = orc.iters
-        except RuntimeError:
-            it = iters
+        mg = po.Multigrid(orc, forests, [("fixed", 7)])
+        mg.setup("chebyshev", degree=2, lower=1.0 / 6.0, upper=1.1)
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        mg.pcg(f, rtol=1e-8)
         dt = time.perf_counter() - t0
-        rates[th] = (ndof * it / dt / 1e6, it, dt)
+        out[th] = dict(pcg_s=dt, pcg_iters=mg.iters, setup_s=t_setup)
+        if th == nthreads:
+            mg.setup("sor", omega=1.0)
+            t0 = time.perf_counter()
+            mg.gmres(f, m=50, rtol=1e-8)
+            out[th].update(gmres_s=time.perf_counter() - t0, gmres_iters=mg.iters)
     po.set_threads(1)
-    v, it, dt = rates[nthreads]
-    return {"value": v, "unit": "MDOF*iter/s", "cores": nthreads, "kind": "port",
-            "value_1core": rates[1][0],
-            "sample": "oracle FP64 Jacobi-PCG (assembled CSR, OpenMP) on the same box cantilever at "
-                      "level %d (%d hex, %d dofs), %d iterations to 1e-8 in %.2f s on %d threads "
-                      "(%.1f s on 1)" % (level, conn.shape[0], ndof, it, dt, nthreads, rates[1][2])}
+    a = out[nthreads]
+    return {"value": 1.0 / a["pcg_s"], "unit": "solves/s", "cores": nthreads, "kind": "port",
+            "value_1core": 1.0 / out[1]["pcg_s"],
+            "mdof_iter_per_s": ndof * a["pcg_iters"] / a["pcg_s"] / 1e6,
+            "pcg_iters": a["pcg_iters"],
+            "gmres_sor_solves_per_s": 1.0 / a["gmres_s"], "gmres_sor_iters": a["gmres_iters"],
+            "sample": "oracle FP64 Galerkin MG on the same box cantilever at level %d (%d hex, %d "
+                      "dofs, %d levels, coarsest 8x4x4 by dense Cholesky): flexible PCG + "
+                      "Chebyshev-2 V-cycle %d iterations to 1e-8 in %.2f s on %d threads (%.2f s on "
+                      "1); GMRES(50) + symmetric-SOR V-cycle %d iterations in %.2f s; assembly "
+                      "%.1f s and MG setup %.2f s not timed"
+                      % (level, orc.nelem, ndof, len(forests), a["pcg_iters"], a["pcg_s"],
+                         nthreads, out[1]["pcg_s"], a["gmres_iters"], a["gmres_s"], t_asm,
+                         a["setup_s"])}
+
+
+def oracle_error(forest, rho, f, u, u64=None):
+    """The oracle's element-by-element check of a GPU solution: true
+    residual |f - K u| / |f| and the energy identity f^T u = u^T K u."""
+    from oracle import pyoracle as po
+    orc = po.from_forest(forest, [("fixed", 7)])
+    s = rho ** 3.0 + 1e-6
+    res = orc.true_residual(s, f, u)
+    fr = np.where(orc.free_mask(), f, 0.0)
+    c = float(fr @ u)
+    out = {"true_residual": res, "energy_identity": abs(c - float(s @ orc.energy(u))) / abs(c)}
+    if u64 is not None:
+        out["u_rel_diff_mixed_vs_fp64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+    return out
 
 
 def bench_multirhs(args):
-    """16 load cases on a 2,097,152-hex cube (1 block, level 7): one batched
-    solve of all cases against 16 single solves."""
+    """16 load cases (seeded traction patches) on the 2,160,152-hex adaptive
+    forest: one batched solve of all cases against the 16 solved one by one."""
     import torch
-    from tmr_amd import TMR, solver, workloads
+    from tmr_amd import solver, workloads
     nrhs = args.nrhs
-    f0 = TMR.createBoxForest(1, 1, 1, L=(1.0, 1.0, 1.0), face_names=workloads.FACES)
-    f0.createTrees(args.level)
-    forests = workloads.hierarchy(f0, args.nlevels)
-    n = 1 << args.level
-    rho = workloads.element_density_from_grid(forests[0], workloads.smoothed_density((n, n, n), 0),
-                                              (1.0, 1.0, 1.0))
+    t0 = time.perf_counter()
+    forests, rho, L = workloads.c5a_adaptive(seed=1, nlevels=5)
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
+    F, params = workloads.patch_loads(forests[0], nrhs, seed=5)
+    log("multi-load-case problem built in %.1f s: %d hex, %d dofs"
+        % (time.perf_counter() - t0, prob.nelem, prob.ndof))
     if not args.no_autotune:
         prob.autotune(args.tiles, reps=3)
-    rng = np.random.default_rng(0)
-    F = np.stack([prob.traction_load("load", rng.standard_normal(3)) for _ in range(nrhs)])
     Fd = torch.from_numpy(F).cuda()
     Ud = torch.zeros_like(Fd)
-    opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=500, no_graph=int(args.no_graph))
+    opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=400, no_graph=int(args.no_graph))
     prob.setup(opts)
     prob.solve_multi(Fd, Ud, opts=opts)
     torch.cuda.synchronize()
@@ -136,12 +167,17 @@ def bench_multirhs(args):
            "value": nrhs / dt_b, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": 1, "ms_per_step": 1e3 * dt_b, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
-           "config": {"workload": "cube 128^3 hex8 (1 block, level %d), %d traction load cases, "
-                                  "%d-level MG, PCG to 1e-8" % (args.level, nrhs, prob.nlevels),
+           "config": {"workload": "adaptive cantilever (2x2x2 blocks on a 2x1x1 box, levels 3-7, "
+                                  "seed 1), %d seeded traction patches on its free outer faces, "
+                                  "5-level MG, PCG to 1e-8" % nrhs,
                       "elements": prob.nelem, "dofs": ndof},
            "sequential_solves_per_s": nrhs / dt_s, "speedup_batched": dt_s / dt_b,
            "iters_batched": iters_b, "iters_sequential": iters_s,
            "mdof_iter_per_s": ndof * sum(iters_b) / dt_b / 1e6}
+    if not args.no_oracle:
+        U = Ud.cpu().numpy()
+        errs = [oracle_error(forests[0], rho, F[k], U[k])["true_residual"] for k in (0, nrhs - 1)]
+        out["err_vs_oracle"] = {"true_residual_cases_0_and_last": errs}
     print(json.dumps(out), flush=True)
 
 
@@ -151,7 +187,7 @@ def bench_batch(args):
     import torch
     from tmr_amd import solver, workloads
     t0 = time.perf_counter()
-    forests, rho = workloads.cube_batch(args.count, level=3, nlevels=4)
+    forests, rho, edges = workloads.cube_batch(args.count, level=3, nlevels=4)
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
     log("batch of %d problems built in %.1f s: %d elements, %d dofs"
         % (prob.num_components(), time.perf_counter() - t0, prob.nelem, prob.ndof))
@@ -172,10 +208,13 @@ def bench_batch(args):
     dt = (time.perf_counter() - ts) / args.steps
     nc = prob.num_components()
     its = [prob.component_info(k)[0] for k in range(nc)]
-    # the same problems one at a time (first 6: one of each size)
+    # the same problems one at a time: one of each size, scaled by how often
+    # the size was drawn
     seq = 0.0
-    for k in range(min(6, nc)):
-        d = 1 + (k % 6)
+    sizes = sorted(set(int(e) for e in edges))
+    for e in sizes:
+        k = [int(x) for x in edges].index(e)
+        d = e // 8
         fk, rk = workloads.box_batch([(d, d, d)], 3, 4, seed=k)
         pk = solver.Problem(fk, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
         pk.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -188,20 +227,139 @@ def bench_batch(args):
         ts = time.perf_counter()
         pk.solve(gk, uk, opts=opts)
         torch.cuda.synchronize()
-        seq += time.perf_counter() - ts
-    seq_all = seq * nc / min(6, nc)
+        seq += (time.perf_counter() - ts) * sum(1 for x in edges if int(x) == e)
+    seq_all = seq
     out = {"metric": "independent problems solved/s, %d uneven cantilevers in one launch per "
                      "stage vs one by one" % nc,
            "value": nc / dt, "unit": "problems/s", "n_gpus": 1, "steps": args.steps,
            "warmup": 1, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
-           "config": {"workload": "%d cubes of edge 8..48 hex8 (k^3 blocks at level 3, k = 1..6), "
-                                  "4-level MG, PCG to 1e-8 per problem" % nc,
+           "config": {"workload": "%d cubes of seeded edges 8..48 hex8 (k^3 blocks at level 3, "
+                                  "k = 1..6 drawn from the seed), 4-level MG, PCG to 1e-8 per "
+                                  "problem" % nc,
+                      "edge_counts": {str(e): sum(1 for x in edges if int(x) == e)
+                                      for e in sorted(set(int(x) for x in edges))},
                       "elements": prob.nelem, "dofs": prob.ndof},
            "sequential_problems_per_s": nc / seq_all,
            "speedup_batched": seq_all / dt,
-           "sequential_sample": "the 6 sizes solved alone once each, scaled to %d problems" % nc,
+           "sequential_sample": "each drawn size solved alone once, times its count",
            "iters_min_max": [min(its), max(its)]}
+    print(json.dumps(out), flush=True)
+
+
+def bench_adaptive(args):
+    """The adaptive cantilever (1,227,458 hex, levels 3-7, hanging nodes,
+    32-Gaussian density, 5-level MG): FP64 end to end, rediscretised and
+    Galerkin coarse levels, and the FP32 V-cycle, to 1e-8."""
+    import torch
+    from tmr_amd import solver, workloads
+    t0 = time.perf_counter()
+    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5)
+    log("adaptive forest hierarchy %s built in %.1f s" % ([len(x.getOctants()) for x in forests],
+                                                           time.perf_counter() - t0))
+    prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
+    prob.set_stream(torch.cuda.current_stream().cuda_stream)
+    prob.set_design(rho)
+    f_host = prob.traction_load("load", [0.0, 0.0, -1.0])
+    f = torch.from_numpy(f_host).cuda()
+    u = torch.zeros_like(f)
+    if not args.no_autotune:
+        prob.autotune(args.tiles, reps=3)
+    legs = {}
+    for name, prec, gal in (("fp64", 0, False), ("fp64_galerkin", 0, True),
+                            ("mixed", 1, False), ("mixed_galerkin", 1, True)):
+        prob.set_multigrid(use_galerkin=gal)
+        opts = solver.default_opts(rtol=1e-8, precision=prec, max_iters=400,
+                                   no_graph=int(args.no_graph))
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        prob.setup(opts)
+        torch.cuda.synchronize()
+        setup_ms = 1e3 * (time.perf_counter() - ts)
+        for _ in range(args.warmup):
+            prob.solve(f, u, opts=opts)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        its = []
+        for _ in range(args.steps):
+            prob.solve(f, u, opts=opts)
+            its.append(prob.info.iters)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - ts) / args.steps
+        legs[name] = {"solves_per_s": 1.0 / dt, "ms_per_solve": 1e3 * dt, "pcg_iters": its[-1],
+                      "setup_ms": setup_ms, "true_residual": prob.true_residual(),
+                      "compliance": prob.compliance(),
+                      "mdof_iter_per_s": prob.ndof * its[-1] / dt / 1e6}
+        log("%s: %s" % (name, legs[name]))
+    u_host = u.cpu().numpy()
+    best = max(legs, key=lambda k: legs[k]["solves_per_s"])
+    out = {"metric": "adaptive-forest solves/s to 1e-8 (FP64 end to end and mixed), 1.2M-hex "
+                     "SIMP", "value": legs["fp64"]["solves_per_s"], "unit": "solves/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": legs["fp64"]["ms_per_solve"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "adaptive cantilever, 2x2x2 blocks on a 2x1x1 box, levels 3-7 "
+                                  "(seed 0), 32-Gaussian density in [1e-3, 1], 5-level MG",
+                      "elements": prob.nelem, "dofs": prob.ndof,
+                      "hex_per_level": [len(x.getOctants()) for x in forests]},
+           "legs": legs, "fastest_leg": best}
+    if not args.no_oracle:
+        out["err_vs_oracle"] = oracle_error(forests[0], rho, f_host, u_host)
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """Backward of the adaptive cantilever (C2 mesh, FP32 V-cycle): the
+    compliance gradient and the p-norm (P = 8) stress gradient with its
+    adjoint solve; time and extra device memory of each backward."""
+    import torch
+    from tmr_amd import solver, workloads
+    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5)
+    prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
+    prob.set_stream(torch.cuda.current_stream().cuda_stream)
+    prob.set_design(rho)
+    f = torch.from_numpy(prob.traction_load("load", [0.0, 0.0, -1.0])).cuda()
+    u = torch.zeros_like(f)
+    opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=400, no_graph=int(args.no_graph))
+    prob.setup(opts)
+    g = torch.empty(prob.nelem, dtype=torch.float64, device="cuda")
+    times = {"forward_solve": [], "compliance_backward": [], "stress_value": [],
+             "stress_backward": []}
+    mem = {}
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        marks = [time.perf_counter()]
+        prob.solve(f, u, opts=opts)
+        torch.cuda.synchronize()
+        marks.append(time.perf_counter())
+        prob.compliance_grad(g)
+        torch.cuda.synchronize()
+        marks.append(time.perf_counter())
+        prob.stress_pnorm(8.0)
+        marks.append(time.perf_counter())
+        prob.stress_pnorm_grad(8.0, g, opts=opts)
+        torch.cuda.synchronize()
+        marks.append(time.perf_counter())
+        mem["extra_device_mib_first_step" if step == 0 else "extra_device_mib"] = \
+            (free0 - torch.cuda.mem_get_info()[0]) / 2 ** 20
+        if step >= args.warmup:
+            for k, a, b in zip(times, marks[:-1], marks[1:]):
+                times[k].append(1e3 * (b - a))
+    ms = {k: float(np.mean(v)) for k, v in times.items()}
+    step_ms = sum(ms.values())
+    out = {"metric": "differentiation step (solve + compliance and p-norm gradients) per second, "
+                     "1.2M-hex adaptive", "value": 1e3 / step_ms, "unit": "steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": step_ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
+           "config": {"workload": "adaptive cantilever (C2 mesh), FP32 V-cycle, compliance and "
+                                  "p-norm (P = 8) von Mises gradients",
+                      "elements": prob.nelem, "dofs": prob.ndof},
+           "ms": ms, "memory": mem, "adjoint_iters": prob.adjoint_info.iters,
+           "forward_iters": prob.info.iters,
+           "device_memory_total_mib": (torch.cuda.mem_get_info()[1]
+                                       - torch.cuda.mem_get_info()[0]) / 2 ** 20}
     print(json.dumps(out), flush=True)
 
 
@@ -214,6 +372,8 @@ def main():
     ap.add_argument("--nlevels", type=int, default=6)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true",
+                    help="skip the oracle's residual check of the solution")
     ap.add_argument("--cpu-level", type=int, default=5)
     ap.add_argument("--fp32-valu", action="store_true",
                     help="FP32 operator on the vector ALUs instead of the matrix cores")
@@ -221,9 +381,11 @@ def main():
                     help="enqueue every PCG iteration instead of replaying HIP graphs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default owner-tile size")
-    ap.add_argument("--workload", default="cantilever", choices=["cantilever", "multirhs", "batch"],
+    ap.add_argument("--workload", default="cantilever",
+                    choices=["cantilever", "multirhs", "batch", "adaptive", "adjoint"],
                     help="cantilever: the 4.2M-hex headline; multirhs: 16 load cases on a "
-                         "2.1M-hex cube; batch: 96 uneven independent cantilevers")
+                         "2.1M-hex cube; batch: 96 uneven independent cantilevers; adaptive: "
+                         "the 1.2M-hex adaptive forest; adjoint: its gradients")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device (a known-size "
@@ -241,6 +403,10 @@ def main():
         return bench_multirhs(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "adaptive":
+        return bench_adaptive(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import torch
     from tmr_amd import solver, workloads
@@ -284,7 +450,8 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - ts
         res = dict(seconds=dt, iters=iters, setup_ms=setup_ms,
-                   true_res=prob.true_residual(), compliance=prob.compliance())
+                   true_res=prob.true_residual(), compliance=prob.compliance(),
+                   u=u.cpu().numpy().copy())
         if timing:
             # kernel durations from one more (untimed, ungraphed) solve: HIP
             # events around every finest-level Chebyshev-step launch
@@ -321,6 +488,9 @@ def main():
     form = "valu" if (args.fp32_valu or not prob.tuning()[1]) else "mfma"
     traffic = None
     pmc = MEASURED_TRAFFIC_KIB.get((form, prob.tuning()[0])) if args.level == 7 else None
+    src = kernel_source_hash()
+    if src != MEASURED_TRAFFIC_SRC:
+        pmc = None
     if pmc is not None:
         traffic = 1024.0 * (pmc[0] * COPY_FETCH_FACTOR + pmc[1] * COPY_WRITE_FACTOR)
     out = {
@@ -353,6 +523,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": ("k_tile_apply<float, ASM_CHEB>" if form == "valu" else
                                 "k_tile_apply_mfma<ASM_CHEB>") + " (finest-level Chebyshev step)",
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE measured on kernels.hip "
+                                       "%s; this run's kernels.hip %s" % (MEASURED_TRAFFIC_SRC, src),
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "avg_launch_us": avg_s * 1e6, "launches": cnt,
                      "mfma_or_valu_tflops": FLOPS_PER_ELEM * prob.nelem / avg_s / 1e12,
@@ -365,6 +537,17 @@ def main():
                        "compliance": fp64["compliance"], "setup_ms": fp64["setup_ms"]}
         out["speedup_mixed_vs_fp64"] = solves_s / out["fp64"]["solves_per_s"]
         out["compliance_rel_diff_mixed_vs_fp64"] = abs(mixed["compliance"] - fp64["compliance"]) / abs(fp64["compliance"])
+    if not args.no_oracle:
+        # the oracle's own check of the mixed leg's solution (not timed)
+        log("oracle check of the solution ...")
+        t0 = time.perf_counter()
+        out["err_vs_oracle"] = oracle_error(prob.forests[0], rho, f_host, mixed["u"],
+                                            fp64["u"] if fp64 is not None else None)
+        out["err_vs_oracle"]["what"] = ("FP32-V-cycle leg's u: true residual |f - K u| / |f| and "
+                                        "|f^T u - u^T K u| / f^T u from the oracle's "
+                                        "element-by-element FP64 operator; u difference to the "
+                                        "FP64 leg")
+        log("oracle check %.1f s: %s" % (time.perf_counter() - t0, out["err_vs_oracle"]))
     if args.calibration_copy > 0:
         n = args.calibration_copy * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")

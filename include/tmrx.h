@@ -81,9 +81,19 @@ int tmrx_forest_create_nodes(tmrx_forest *f);
 /* TMROctForest::getOctants (src/TMROctForest.cpp:1452-1456).  Borrowed,
    valid until the forest is refined/balanced/freed. */
 int tmrx_forest_get_octants(tmrx_forest *f, const tmrx_octant **octs, int *n);
-/* TMROctForest::getNodeConn (src/TMROctForest.cpp:5686-5704).  Borrowed. */
+/* TMROctForest::getNodeConn (src/TMROctForest.cpp:5686-5704).  Borrowed.
+   num_local_nodes receives the number of local nodes (the length of the node
+   numbers, tmrx_forest_get_node_numbers); the reference declares it but never
+   writes it (it assigns num_owned_nodes twice). */
 int tmrx_forest_get_node_conn(tmrx_forest *f, const int **conn, int *num_elements,
-                              int *num_owned_nodes);
+                              int *num_owned_nodes, int *num_local_nodes);
+/* TMROctForest::setMeshOrder (src/TMROctForest.cpp:1389-1427): frees the
+   nodes and sets the element order; orders below 2 become 2 (the reference
+   clamps to [2, MAX_ORDER]).  Only order 2 (trilinear hex8) is built here:
+   a higher order returns an error.  interp_type as TMRInterpolationType
+   (0 uniform, 1 Gauss-Lobatto, 2 Bernstein; order 2 knots are {-1, 1} for
+   all three). */
+int tmrx_forest_set_mesh_order(tmrx_forest *f, int mesh_order, int interp_type);
 /* TMROctForest::getDepNodeConn (src/TMROctForest.cpp:5717-5729).
    Returns the number of dependent nodes (>= 0) or an error (< 0). */
 int tmrx_forest_get_dep_node_conn(tmrx_forest *f, const int **ptr, const int **conn,
@@ -111,6 +121,32 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
                                      int *ncols, const int **rowp, const int **cols,
                                      const double **weights);
 
+/* TopOptUtils.interpolateDesignVec (tmr/TopOptUtils.py:270-307): the nodal
+   vector x_orig (vars_per_node values per independent node of orig_forest)
+   interpolated onto the independent nodes of new_forest,
+   x_new = P x_orig with P = new_forest.createInterpolation(orig_forest), the
+   product on the current HIP device.  on_device: both arrays are device
+   pointers (else host arrays, staged). */
+int tmrx_forest_interpolate_vec(tmrx_forest *new_forest, tmrx_forest *orig_forest,
+                                int vars_per_node, const double *x_orig, double *x_new,
+                                int on_device);
+
+/* TMR_GenerateBinFile (src/TMR_STLTools.h:55-68; Python writeSTLToBin,
+   tmr/TMR.pyx:4610-4631): the surface of the solid region {x >= cutoff} of
+   the nodal field x (vars_per_node values per independent node, component
+   x_offset) on the forest, as the level set inside the elements plus the
+   solid parts of the outer block faces, normals out of the solid.  The
+   level set is traced by marching tetrahedra (six per element around its
+   0-7 diagonal) where the reference uses marching cubes, so the triangles
+   differ while the surface is the same to second order.  File: int32
+   triangle count, then 9 doubles per triangle.  ntris may be NULL. */
+int tmrx_forest_generate_bin_file(tmrx_forest *f, const char *filename, const double *x,
+                                  int vars_per_node, int x_offset, double cutoff, int *ntris);
+/* TMR_ConvertBinToSTL (src/TMR_STLTools.h:70-77): the binary triangle file
+   as an ASCII STL ("solid topology", one facet with its unit normal per
+   triangle). */
+int tmrx_convert_bin_to_stl(const char *binfile, const char *stlfile);
+
 /* TMROctTACSTopoCreator's element densities from a filter forest
    (src/topology/TMR_TACSTopoCreator.cpp:55-186): rows = elements of f, columns
    = independent nodes of the filter; rho_e = sum_j w_ej x_j.  Borrowed arrays
@@ -118,6 +154,41 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
 int tmrx_forest_element_center_weights(tmrx_forest *f, tmrx_forest *filter, int *nrows,
                                        int *ncols, const int **rowp, const int **cols,
                                        const double **weights);
+
+/* ---------------------------------------------------------------------------
+   Design filters on a filter forest's nodal field (TMRTopoFilter's
+   applyFilter / applyTranspose, src/topology/TMRTopoFilter.h:41-66).
+   ------------------------------------------------------------------------- */
+typedef struct tmrx_filter tmrx_filter;
+/* kind 0: TMRHelmholtzFilter (src/topology/TMRHelmholtzFilter.cpp:31-131):
+     x_f = (M + r^2 K)^-1 M x with M the consistent mass and K the Laplacian
+     of the trilinear element (TMRHexaHelmholtzModel,
+     src/topology/TMRHelmholtzModel.cpp:85-99); the solve is a
+     Jacobi-preconditioned CG to 1e-12 on the GPU (the reference: GMRES with
+     multigrid to 1e-12).
+   kind 1: TMRMatrixFilter (src/topology/TMRMatrixFilter.cpp:50-168):
+     nterms terms of out = a o x + b o (M out) (Horner), D = M 1,
+     a = 1 / (1 + r^2 / D^(2/3)), b = a r^2 / (D^(2/3) D), normalised by
+     1 / filter(1).
+   Hanging nodes enter through their constraints; the operators are over the
+   forest's independent nodes.  device = HIP device index. */
+int tmrx_filter_create(tmrx_forest *filter_forest, int kind, double r, int nterms, int device,
+                       tmrx_filter **out);
+void tmrx_filter_free(tmrx_filter *f);
+/* applyFilter (TMRHelmholtzFilter::applyFilter, :152-255;
+   TMRMatrixFilter::applyFilter, :191-211): in and out hold vars_per_node
+   values per independent node, interleaved; on_device selects device
+   pointers. */
+int tmrx_filter_apply(tmrx_filter *f, int vars_per_node, const double *in, double *out,
+                      int on_device);
+/* applyTranspose (TMRHelmholtzFilter::applyTranspose, :257-361;
+   TMRMatrixFilter::applyTranspose, :216-238): the adjoint of the filter, for
+   gradients. */
+int tmrx_filter_apply_transpose(tmrx_filter *f, int vars_per_node, const double *in, double *out,
+                                int on_device);
+/* Independent nodes of the filter forest and the CG iterations of the last
+   Helmholtz solve. */
+int tmrx_filter_info(tmrx_filter *f, int *num_nodes, int *last_iters);
 
 /* ---------------------------------------------------------------------------
    Problem: the finite-element hierarchy and solver that TMR hands to TACS.

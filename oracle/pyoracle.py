@@ -569,3 +569,94 @@ class Multigrid:
         if it < 0:
             raise RuntimeError("oracle MG-GMRES did not converge")
         return x
+
+
+# ---------------------------------------------------------------------------
+# design filters (TMRHelmholtzFilter, TMRMatrixFilter) on assembled scalar
+# operators; element matrices as tensor products of the 1D linear element
+# (mass h/6 [[2, 1], [1, 2]], stiffness 1/h [[1, -1], [-1, 1]]), a derivation
+# independent of the product's quadrature
+def scalar_mass_laplacian(forest):
+    """(M, K) over the independent nodes, hanging nodes through their
+    constraints, scipy CSR."""
+    import scipy.sparse as sp
+    conn = np.asarray(forest.getMeshConn()).reshape(-1, 8)
+    ext = forest.getElementExtents()
+    dp, dc, dw = forest.getDepNodeConn()
+    n = int(forest.getNodeRange()[1])
+
+    def m1(h):
+        return h / 6.0 * np.array([[2.0, 1.0], [1.0, 2.0]])
+
+    def k1(h):
+        return 1.0 / h * np.array([[1.0, -1.0], [-1.0, 1.0]])
+
+    rows, cols, mv, kv = [], [], [], []
+    idx = [(a & 1, (a >> 1) & 1, a >> 2) for a in range(8)]
+    for e in range(len(conn)):
+        hx, hy, hz = ext[e]
+        mx, my, mz, kx, ky, kz = m1(hx), m1(hy), m1(hz), k1(hx), k1(hy), k1(hz)
+        Me = np.zeros((8, 8))
+        Ke = np.zeros((8, 8))
+        for a, (ia, ja, ka) in enumerate(idx):
+            for b, (ib, jb, kb) in enumerate(idx):
+                Me[a, b] = mx[ia, ib] * my[ja, jb] * mz[ka, kb]
+                Ke[a, b] = (kx[ia, ib] * my[ja, jb] * mz[ka, kb] + mx[ia, ib] * ky[ja, jb] * mz[ka, kb]
+                            + mx[ia, ib] * my[ja, jb] * kz[ka, kb])
+        ex = []
+        for a in range(8):
+            g = conn[e, a]
+            if g >= 0:
+                ex.append([(g, 1.0)])
+            else:
+                d = -g - 1
+                ex.append([(dc[j], dw[j]) for j in range(dp[d], dp[d + 1])])
+        for a in range(8):
+            for (i, wa) in ex[a]:
+                for b in range(8):
+                    for (j, wb) in ex[b]:
+                        rows.append(i)
+                        cols.append(j)
+                        mv.append(wa * wb * Me[a, b])
+                        kv.append(wa * wb * Ke[a, b])
+    M = sp.csr_matrix((mv, (rows, cols)), shape=(n, n))
+    K = sp.csr_matrix((kv, (rows, cols)), shape=(n, n))
+    return M, K
+
+
+def helmholtz_filter(M, K, r, x, transpose=False):
+    """TMRHelmholtzFilter: x_f = (M + r^2 K)^-1 M x; transpose M (M + r^2 K)^-1 g."""
+    import scipy.sparse.linalg as spl
+    A = (M + r * r * K).tocsc()
+    if not transpose:
+        return spl.spsolve(A, M @ x)
+    return M @ spl.spsolve(A, x)
+
+
+def matrix_filter(M, r, N, x, transpose=False):
+    """TMRMatrixFilter (src/topology/TMRMatrixFilter.cpp:118-238): D = M 1,
+    a = 1 / (1 + r^2 / D^(2/3)), b = a r^2 / (D^(2/3) D), N Horner terms,
+    normalised by 1 / F(1)."""
+    D = np.asarray(M @ np.ones(M.shape[0])).ravel()
+    t = D ** (2.0 / 3.0)
+    a = 1.0 / (1.0 + r * r / t)
+    b = a * r * r / (t * D)
+
+    def raw(v, tinv):
+        t1 = a * v
+        out = t1.copy()
+        for _ in range(N):
+            out = b * (M @ out) + t1
+        return tinv * out
+
+    def raw_t(v, tinv):
+        t1 = tinv * v
+        out = t1.copy()
+        for _ in range(N):
+            out = M @ (b * out) + t1
+        return a * out
+
+    ones = np.ones_like(D)
+    y = raw(ones, ones)
+    tinv = np.where(y != 0.0, 1.0 / np.where(y != 0.0, y, 1.0), 0.0)
+    return raw_t(x, tinv) if transpose else raw(x, tinv)

@@ -190,3 +190,117 @@ def test_c2_forest_maps_bit_exact():
     assert np.array_equal(prp, rp)
     assert np.array_equal(pcl, cl)
     assert np.array_equal(pw, w)
+
+
+def test_set_mesh_order_and_local_node_count():
+    """setMeshOrder frees the nodes (they are rebuilt by createNodes) and
+    only order 2 is built; getNodeConn's local-node count is the length of
+    the node numbers (owned independent plus dependent nodes here)."""
+    import ctypes as C
+    f, pre = build((2, 2, 2), 1, seed=3, maxlev=4)
+    conn0 = f.getMeshConn().copy()
+    f.setMeshOrder(2)
+    with pytest.raises(_lib.TmrxError):
+        f.getMeshConn()
+    f.createNodes()
+    assert np.array_equal(f.getMeshConn(), conn0)
+    with pytest.raises(_lib.TmrxError, match="order 2"):
+        f.setMeshOrder(3)
+    f.setMeshOrder(1)  # clamped to 2, as the reference does
+    f.createNodes()
+    p = C.POINTER(C.c_int)()
+    ne, no, nl = C.c_int(), C.c_int(), C.c_int()
+    assert _lib.lib().tmrx_forest_get_node_conn(f.ptr, C.byref(p), C.byref(ne), C.byref(no),
+                                                C.byref(nl)) == 0
+    dp, dc, dw = f.getDepNodeConn()
+    assert ne.value == len(f.getOctants())
+    assert no.value == f.getNodeRange()[1]
+    assert nl.value == len(f.getNodeNumbers()) == no.value + len(dp) - 1
+
+
+# Block connectivities with rotated faces (examples/parallel/octant_test.cpp:
+# the two-block rectangle, whose second block has its own axes, and the
+# seven-block box, a cube inside a cube).  The geometric oracle covers
+# equally oriented lattices only; these fixtures are pinned by closed forms.
+RECT_X = np.array([0, 0, 0, 1, 0, 0, 2, 0, 0, 0, 1, 0, 1, 1, 0, 2, 1, 0,
+                   0, 0, 1, 1, 0, 1, 2, 0, 1, 0, 1, 1, 1, 1, 1, 2, 1, 1], float).reshape(-1, 3)
+RECT_CONN = np.array([[0, 1, 3, 4, 6, 7, 9, 10], [8, 11, 2, 5, 7, 10, 1, 4]], np.intc)
+BOX_X = np.array([-.5, -.5, -.5, .5, -.5, -.5, -.5, .5, -.5, .5, .5, -.5, -.5, -.5, .5, .5,
+                  -.5, .5, -.5, .5, .5, .5, .5, .5, -1, -1, -1, -1, -1, 1, 1, -1,
+                  -1, 1, -1, 1, -1, 1, -1, -1, 1, 1, 1, 1, -1, 1, 1, 1]).reshape(-1, 3)
+BOX_CONN = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 0, 1, 9, 11, 4, 5, 5, 11, 1, 10, 7, 15, 3,
+                     14, 7, 15, 3, 14, 6, 13, 2, 12, 9, 13, 4, 6, 8, 12, 0, 2, 10, 14, 8, 12,
+                     1, 3, 0, 2, 4, 5, 6, 7, 9, 11, 13, 15], np.intc).reshape(-1, 8)
+
+
+def _rotated_forest(X, conn, level, refine_seed=None):
+    f = TMR.OctForest()
+    f.setConnectivity(conn)
+    f.setNodePoints(X)
+    f.createTrees(level)
+    if refine_seed is not None:
+        rng = np.random.default_rng(refine_seed)
+        f.refine((rng.random(len(f.getOctants())) < 0.2).astype(np.intc), 0, level + 2)
+    f.balance(1)
+    f.createNodes()
+    return f
+
+
+def _indep_points(f):
+    """Coordinates of the independent nodes 0..n-1 (getPoints follows the
+    sorted node numbers, dependent nodes first)."""
+    nums = f.getNodeNumbers()
+    X = f.getPoints()
+    n = int(f.getNodeRange()[1])
+    out = np.zeros((n, 3))
+    sel = nums >= 0
+    out[nums[sel]] = X[sel]
+    return out, X[~sel], nums[~sel]
+
+
+@pytest.mark.parametrize("which", ["rectangle", "box"])
+@pytest.mark.parametrize("level", [1, 2])
+def test_rotated_blocks_uniform_node_count(which, level):
+    """Conforming uniform mesh: V + E m + F m^2 + C m^3 nodes, m = 2^L - 1,
+    with V, E, F, C the block vertices, edges, faces and blocks (rectangle
+    12, 20, 11, 2; box 16, 32, 24, 7: Euler's V - E + F - C = 1 and
+    F = (6 C + 6) / 2 for a cube-shaped complex); no hanging nodes."""
+    X, conn = (RECT_X, RECT_CONN) if which == "rectangle" else (BOX_X, BOX_CONN)
+    f = _rotated_forest(X, conn, level)
+    V, E, F, Cb = (12, 20, 11, 2) if which == "rectangle" else (16, 32, 24, 7)
+    assert V - E + F - Cb == 1
+    m = 2 ** level - 1
+    assert f.getNodeRange()[1] == V + E * m + F * m * m + Cb * m ** 3
+    assert len(f.getDepNodeConn()[0]) == 1
+
+
+@pytest.mark.parametrize("which", ["rectangle", "box"])
+def test_rotated_blocks_hanging_nodes_and_interpolation(which):
+    """Adaptive refinement across rotated faces: every hanging node sits at
+    the weighted mean of its parents (weights 1/2 or 1/4 summing to 1), and
+    the prolongation from the coarsened forest reproduces an affine field of
+    the physical coordinates exactly (trilinear block maps keep affine fields
+    trilinear in every element); on the rectangle, whose blocks are boxes, a
+    global trilinear field with cross terms too."""
+    X, conn = (RECT_X, RECT_CONN) if which == "rectangle" else (BOX_X, BOX_CONN)
+    f = _rotated_forest(X, conn, 2, refine_seed=4)
+    P_ind, P_dep, dep_nums = _indep_points(f)
+    dp, dc, dw = f.getDepNodeConn()
+    assert len(dp) - 1 > 0
+    for k, d in enumerate(-dep_nums - 1):
+        w = dw[dp[d]:dp[d + 1]]
+        assert np.all((w == 0.5) | (w == 0.25)) and abs(w.sum() - 1.0) < 1e-15
+        mean = (w[:, None] * P_ind[dc[dp[d]:dp[d + 1]]]).sum(0)
+        assert np.allclose(P_dep[k], mean, atol=1e-14)
+    c = f.coarsen()
+    c.balance(1)
+    c.createNodes()
+    rp, cl, w = f.createInterpolation(c)
+    Pc, _, _ = _indep_points(c)
+    fields = [lambda p: 0.3 - 1.25 * p[:, 0] + 2.0 * p[:, 1] + 0.5 * p[:, 2]]
+    if which == "rectangle":
+        fields.append(lambda p: (0.3 - p[:, 0] + 0.7 * p[:, 1] * p[:, 2] +
+                                 1.1 * p[:, 0] * p[:, 1] - 0.6 * p[:, 0] * p[:, 1] * p[:, 2]))
+    for fld in fields:
+        Pu = np.add.reduceat(w * fld(Pc)[cl], rp[:-1])
+        assert np.max(np.abs(Pu - fld(P_ind))) < 1e-13

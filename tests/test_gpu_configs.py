@@ -123,10 +123,14 @@ def _fd_elements(n, count=64, seed=11):
 def test_c4_central_differences(c2):
     """Central differences of compliance and of the p-norm (P = 8) at 64
     seeded elements, step 1e-6, FP64 solves to rtol 1e-12.  A one-element
-    change moves these functionals by about 1e-12 of their value, which is
-    the accuracy the solves reach; the agreement is reported per element and
-    asserted in the median and for the elements whose change stands above
-    that floor."""
+    change of most elements moves these functionals by less than the
+    accuracy of the solves (about 1e-13 of their value; the p-norm is
+    dominated by a few highly stressed elements), so every element is held
+    to |fd - g| <= 1e-3 |g| + 100 x that noise floor: elements above the
+    floor agree to the relative bound (measured: compliance 2e-5, p-norm
+    1e-6), the rest to the absolute one.  The same 64 elements moved
+    together along a random direction (step 1e-4) give a change far above
+    the floor, held to 1e-4."""
     forests, rho, prob, f = c2
     opts = solver.default_opts(rtol=1e-12, precision=0, max_iters=600)
     prob.set_design(rho)
@@ -164,12 +168,9 @@ def test_c4_central_differences(c2):
           % (time.time() - t, np.median(ec), ec.max(), big_c.sum(),
              ec[big_c].max() if big_c.any() else 0.0, np.median(es), es.max(), big_s.sum(),
              es[big_s].max() if big_s.any() else 0.0), flush=True)
-    assert np.median(ec) < 1e-3
-    assert np.median(es) < 1e-2
-    if big_c.any():
-        assert ec[big_c].max() < 1e-3
-    if big_s.any():
-        assert es[big_s].max() < 1e-2
+    assert np.all(np.abs(fdc - gc[els]) <= 1e-3 * np.abs(gc[els]) + 100 * floor_c)
+    assert np.all(np.abs(fds - gs[els]) <= 1e-3 * np.abs(gs[els]) + 100 * floor_s)
+    assert big_c.sum() >= 1 and big_s.sum() >= 1
     # all 64 at once along a random direction: a change far above the floor
     d = np.zeros(prob.nelem)
     d[els] = np.random.default_rng(3).standard_normal(len(els))

@@ -208,8 +208,16 @@ def test_interpolation_reproduces_trilinear_fields():
     fm, cm, (rowp, cols, w) = po.interpolation(dims, fine_in, np.array(coarse, dtype=po.OCT_DT))
     sums = np.add.reduceat(w, rowp[:-1])
     assert np.all(sums == 1.0)
-    a = np.array([0.3, -1.25, 2.0, 0.5])
-    # evaluate the linear field at independent nodes via the mesh conn/leaves
+    a = np.array([0.3, -1.25, 2.0, 0.5, 0.75, -0.4, 1.1, -0.6])
+
+    # a trilinear field (cross terms xy, yz, xz and xyz included) at the
+    # independent nodes via the mesh conn/leaves: trilinear in every box
+    # element, so the interpolation must reproduce it exactly
+    def field(p):
+        x, y, z = p / float(H)
+        return (a[0] + a[1] * x + a[2] * y + a[3] * z + a[4] * x * y + a[5] * y * z +
+                a[6] * x * z + a[7] * x * y * z)
+
     def field_at_indep(mesh):
         lo, h = _leaf_boxes(mesh)
         vals = {}
@@ -219,7 +227,7 @@ def test_interpolation_reproduces_trilinear_fields():
                 if n < 0:
                     continue
                 p = lo[e] + np.array([c & 1, (c >> 1) & 1, c >> 2]) * h[e]
-                vals[n] = a[0] + (a[1:] * p / float(H)).sum()
+                vals[n] = field(p)
         return np.array([vals[i] for i in range(mesh.nindep)])
     uc = field_at_indep(cm)
     uf = field_at_indep(fm)

@@ -190,12 +190,18 @@ def test_vertex_load():
         TopOptUtils.computeVertexLoad("load", forest, a, [0.0, 1.0])
 
 
-def test_oc_update_meets_volume():
-    rng = np.random.default_rng(3)
-    n = 200
-    x = np.full(n, 0.5)
-    g = -rng.random(n)
-    gcon = np.ones(n) / n
-    vol = lambda z: z.mean() - 0.4
-    xn = TopOptUtils.oc_update(x, g, gcon, vol, np.zeros(n), np.ones(n))
-    assert abs(vol(xn)) < 1e-6 and np.all(np.abs(xn - x) <= 0.2 + 1e-15)
+def test_density_based_refine_rule():
+    """TopOptUtils.densityBasedRefine (tmr/TopOptUtils.py:397-457): refine at
+    or above upper, coarsen at or below lower (max of an element's values;
+    reversed: min, and the roles swap)."""
+    f = TMR.createBoxForest(1, 1, 1, face_names={"x-": "fixed"})
+    f.createTrees(1)
+    d = np.array([0.0, 0.05, 0.2, 0.5, 0.9, 0.04, 0.51, 0.3])
+    flags = TopOptUtils.densityBasedRefine(f, d, lower=0.05, upper=0.5)
+    assert flags.tolist() == [-1, -1, 0, 1, 1, -1, 1, 0]
+    f2 = TMR.createBoxForest(1, 1, 1, face_names={"x-": "fixed"})
+    f2.createTrees(1)
+    dd = np.stack([d, np.ones_like(d)], 1)
+    flags2 = TopOptUtils.densityBasedRefine(f2, dd, lower=0.05, upper=0.5, reverse=True)
+    assert flags2.tolist() == [1, 1, 0, -1, -1, 1, -1, 0]
+    assert f.getOctants()["level"].max() == 2

@@ -66,9 +66,9 @@ class OctForest:
         return self._interp
 
     def setMeshOrder(self, order, interp=GAUSS_LOBATTO_POINTS):
-        if order != 2:
-            raise ValueError("only mesh order 2 (trilinear hex8) is supported")
-        self._order, self._interp = order, interp
+        """TMROctForest::setMeshOrder: frees the nodes; only order 2 is built."""
+        check(lib().tmrx_forest_set_mesh_order(self.ptr, int(order), int(interp)), "setMeshOrder")
+        self._order, self._interp = max(int(order), 2), interp
 
     def setConnectivity(self, conn):
         conn = np.ascontiguousarray(conn, dtype=np.intc)
@@ -175,8 +175,8 @@ class OctForest:
     def getMeshConn(self):
         p = C.POINTER(C.c_int)()
         ne, no = C.c_int(), C.c_int()
-        check(lib().tmrx_forest_get_node_conn(self.ptr, C.byref(p), C.byref(ne), C.byref(no)),
-              "getMeshConn")
+        check(lib().tmrx_forest_get_node_conn(self.ptr, C.byref(p), C.byref(ne), C.byref(no),
+                                              None), "getMeshConn")
         if ne.value == 0:
             return np.zeros((0, 8), dtype=np.intc)
         return np.ctypeslib.as_array(p, (8 * ne.value,)).reshape(ne.value, 8).copy()
@@ -290,8 +290,41 @@ def createBoxForest(nx, ny, nz, L=(1.0, 1.0, 1.0), face_names=None, comm=None):
     return f
 
 
+def writeSTLToBin(fname, forest, x, index=0, cutoff=0.5, vars_per_node=None):
+    """TMR.writeSTLToBin (tmr/TMR.pyx:4610-4631, TMR_GenerateBinFile): the
+    surface of {x >= cutoff} of a nodal design field as a binary triangle
+    file; returns the triangle count.  x holds vars_per_node values per
+    independent node (inferred from its length), component `index`."""
+    x = np.ascontiguousarray(x, dtype=np.double)
+    n = int(forest.getNodeRange()[1])
+    if vars_per_node is None:
+        if n == 0 or x.size % n:
+            raise ValueError("x must hold a whole number of values per node")
+        vars_per_node = x.size // n
+    nt = C.c_int()
+    check(lib().tmrx_forest_generate_bin_file(forest.ptr, str(fname).encode(),
+                                              x.ctypes.data_as(C.POINTER(C.c_double)),
+                                              int(vars_per_node), int(index), float(cutoff),
+                                              C.byref(nt)), "writeSTLToBin")
+    return nt.value
+
+
+def convertBinToSTL(binfile, stlfile):
+    """TMR_ConvertBinToSTL: the binary triangle file as ASCII STL."""
+    check(lib().tmrx_convert_bin_to_stl(str(binfile).encode(), str(stlfile).encode()),
+          "convertBinToSTL")
+
+
+def readSTLBin(binfile):
+    """The triangles of a binary triangle file, shape (ntris, 3, 3)."""
+    with open(binfile, "rb") as fh:
+        nt = int(np.frombuffer(fh.read(4), dtype=np.int32)[0])
+        return np.frombuffer(fh.read(72 * nt), dtype=np.double).reshape(nt, 3, 3).copy()
+
+
 # topology-optimisation classes (tmr/TMR.pyx:3877-6005) live in topo.py
 from .topo import (BoundaryConditions, Compliance, ConformFilter, LagrangeFilter,  # noqa: E402,F401
+                   HelmholtzFilter, MatrixFilter,
                    MaterialProperties, OctConformTopoCreator, OctConstitutive, OctTopoCreator,
                    PNormFailure, StiffnessProperties, StructuralMass, TopoProblem, createMg,
                    RAMP_PENALTY, SIMP_PENALTY)

@@ -1,5 +1,6 @@
-"""TMR's TopOptUtils helpers (tmr/TopOptUtils.py:14-267) over the native
-problem: the level-hierarchy builder and the load vectors.
+"""TMR's TopOptUtils helpers (tmr/TopOptUtils.py:14-307, 397-457) over the
+native problem: the level-hierarchy builder, the load vectors, the design
+transfer between adapted meshes and the density-based refinement rule.
 
 Loads are host-side setup: consistent nodal forces of a constant traction on
 the faces of axis-aligned box elements (exact for bilinear faces: a quarter
@@ -20,10 +21,11 @@ def createTopoProblem(forest, callback, filter_type, nlevels=2, repartition=True
     filter per level from ``creator, filter = callback(forest)``, the
     multigrid object and the TopoProblem (TopOptUtils.py:14-128).
 
-    Only order-2 meshes and the "lagrange" (element densities interpolated
-    from the filter forest, OctTopoCreator) and "conform" (nodal design
-    variables on the analysis forests, OctConformTopoCreator) filters are
-    built."""
+    Only order-2 meshes.  Filters: "lagrange" (element densities
+    interpolated from the filter forest, OctTopoCreator), "conform" (nodal
+    design variables on the analysis forests, OctConformTopoCreator), and
+    "matrix" / "helmholtz" (conform levels whose analysis design is the
+    matrix- or Helmholtz-filtered design, radius r0, N terms)."""
     if design_vars_per_node != 1:
         raise NotImplementedError("one design variable per node only")
     if scale_coordinate_factor != 1.0:
@@ -55,6 +57,10 @@ def createTopoProblem(forest, callback, filter_type, nlevels=2, repartition=True
         filter_obj = topo.LagrangeFilter(assemblers, filters)
     elif filter_type == "conform":
         filter_obj = topo.ConformFilter(assemblers, filters)
+    elif filter_type == "matrix":
+        filter_obj = topo.MatrixFilter(r0, N, assemblers, filters)
+    elif filter_type == "helmholtz":
+        filter_obj = topo.HelmholtzFilter(r0, assemblers, filters)
     else:
         raise NotImplementedError("filter type %r is not supported" % (filter_type,))
     return topo.TopoProblem(filter_obj, mg)
@@ -130,21 +136,53 @@ def computeVertexLoad(name, forest, assembler, point_force):
     return f
 
 
-def oc_update(x, g, gcon, vol_frac_excess, lb, ub, move=0.2, eta=0.5):
-    """One optimality-criteria step for min compliance s.t. a volume bound,
-    with bisection on the multiplier: x_new = clip(x (-g / (lam gcon))^eta).
-    A plain driver for examples; the reference hands the problem to ParOpt.
-    ``vol_frac_excess(x)`` is the constraint in the form c(x) <= 0."""
-    gneg = np.maximum(-g, 1e-30)
-    lo_l, hi_l = 1e-12, 1e12
-    for _ in range(200):
-        lam = np.sqrt(lo_l * hi_l)
-        xn = np.clip(x * (gneg / (lam * np.maximum(gcon, 1e-30))) ** eta,
-                     np.maximum(lb, x - move), np.minimum(ub, x + move))
-        if vol_frac_excess(xn) > 0:
-            lo_l = lam
-        else:
-            hi_l = lam
-        if hi_l / lo_l < 1 + 1e-10:
-            break
-    return xn
+def interpolateDesignVec(orig_filter, orig_vec, new_filter, new_vec=None):
+    """Design variables of an adapted mesh (TopOptUtils.interpolateDesignVec,
+    tmr/TopOptUtils.py:270-307): new_vec = P orig_vec with
+    P = new_filter.createInterpolation(orig_filter), nodal values
+    interpolated at the new forest's independent nodes; the product runs on
+    the GPU (tmrx_forest_interpolate_vec).  Vectors are numpy arrays (TMR
+    passes ParOpt vectors); several values per node are interleaved.
+    Returns new_vec."""
+    import ctypes as C
+    from ._lib import check, lib
+    orig_vec = np.ascontiguousarray(orig_vec, dtype=np.double)
+    n_orig = int(orig_filter.getNodeRange()[1])
+    n_new = int(new_filter.getNodeRange()[1])
+    if n_orig == 0 or orig_vec.size % n_orig:
+        raise ValueError("original vector length %d is not a multiple of the %d nodes"
+                         % (orig_vec.size, n_orig))
+    nv = orig_vec.size // n_orig
+    if new_vec is None:
+        new_vec = np.zeros(n_new * nv)
+    if new_vec.size != n_new * nv or new_vec.dtype != np.double:
+        raise ValueError("new vector must hold %d float64 values per node" % nv)
+    check(lib().tmrx_forest_interpolate_vec(new_filter.ptr, orig_filter.ptr, nv,
+                                            C.c_void_p(orig_vec.ctypes.data),
+                                            C.c_void_p(new_vec.ctypes.data), 0),
+          "interpolateDesignVec")
+    return new_vec
+
+
+def densityBasedRefine(forest, density, lower=0.05, upper=0.5, reverse=False, min_lev=0,
+                       max_lev=TMR.MAX_LEVEL):
+    """TopOptUtils.densityBasedRefine (tmr/TopOptUtils.py:397-457): elements
+    whose density reaches ``upper`` are refined and those at or below
+    ``lower`` coarsened (reversed with ``reverse``), then forest.refine.
+    ``density`` holds one value per element, or (nelem, k) values per element
+    (TMR takes the max over an element's design values, the min when
+    reversed)."""
+    d = np.asarray(density, dtype=np.double)
+    if d.ndim == 2:
+        value = d.min(axis=1) if reverse else d.max(axis=1)
+    else:
+        value = d
+    refine = np.zeros(len(value), dtype=np.intc)
+    if reverse:
+        refine[value >= upper] = -1
+        refine[value <= lower] = 1
+    else:
+        refine[value >= upper] = 1
+        refine[value <= lower] = -1
+    forest.refine(refine, min_lev, max_lev)
+    return refine

@@ -63,7 +63,8 @@ _SIGS = {
     "tmrx_forest_repartition": (I, [P]),
     "tmrx_forest_create_nodes": (I, [P]),
     "tmrx_forest_get_octants": (I, [P, C.POINTER(C.POINTER(Octant)), IP]),
-    "tmrx_forest_get_node_conn": (I, [P, CIP, IP, IP]),
+    "tmrx_forest_get_node_conn": (I, [P, CIP, IP, IP, IP]),
+    "tmrx_forest_set_mesh_order": (I, [P, I, I]),
     "tmrx_forest_get_dep_node_conn": (I, [P, CIP, CIP, CDP]),
     "tmrx_forest_get_owned_node_range": (I, [P, IP]),
     "tmrx_forest_get_node_numbers": (I, [P, CIP, IP]),
@@ -72,6 +73,14 @@ _SIGS = {
     "tmrx_forest_get_octs_with_name": (I, [P, S, C.POINTER(Octant), I]),
     "tmrx_forest_create_interpolation": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_forest_element_center_weights": (I, [P, P, IP, IP, CIP, CIP, CDP]),
+    "tmrx_forest_interpolate_vec": (I, [P, P, I, P, P, I]),
+    "tmrx_forest_generate_bin_file": (I, [P, S, P, I, I, D, IP]),
+    "tmrx_convert_bin_to_stl": (I, [S, S]),
+    "tmrx_filter_create": (I, [P, I, D, I, I, C.POINTER(P)]),
+    "tmrx_filter_free": (None, [P]),
+    "tmrx_filter_apply": (I, [P, I, P, P, I]),
+    "tmrx_filter_apply_transpose": (I, [P, I, P, P, I]),
+    "tmrx_filter_info": (I, [P, IP, IP]),
     "tmrx_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "tmrx_problem_create": (I, [I, C.POINTER(P), C.POINTER(Material), I, C.POINTER(S), IP, I,
                                 C.POINTER(P)]),

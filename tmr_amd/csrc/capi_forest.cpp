@@ -150,12 +150,22 @@ static int need_nodes(tmrx_forest *f, const char *who) {
 }
 
 int tmrx_forest_get_node_conn(tmrx_forest *f, const int **conn, int *num_elements,
-                              int *num_owned_nodes) {
+                              int *num_owned_nodes, int *num_local_nodes) {
   CHECK_HANDLE(f);
   if (need_nodes(f, "tmrx_forest_get_node_conn")) return -1;
   if (conn) *conn = f->f->conn().data();
   if (num_elements) *num_elements = f->f->num_elements();
   if (num_owned_nodes) *num_owned_nodes = f->f->num_owned_nodes();
+  if (num_local_nodes) *num_local_nodes = (int)f->f->node_numbers().size();
+  return 0;
+}
+
+int tmrx_forest_set_mesh_order(tmrx_forest *f, int mesh_order, int interp_type) {
+  CHECK_HANDLE(f);
+  if (f->f->set_mesh_order(mesh_order, interp_type)) {
+    set_error(f->f->last_error);
+    return -1;
+  }
   return 0;
 }
 

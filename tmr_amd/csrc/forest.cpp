@@ -1018,6 +1018,25 @@ int Forest::find_node(const Oct &n) const {
   return (int)(it - nkey_.begin());
 }
 
+// TMROctForest::setMeshOrder (src/TMROctForest.cpp:1389-1427): frees the
+// mesh data and sets the order and knots; orders below 2 become 2.  The
+// trilinear element is the only one built here.
+int Forest::set_mesh_order(int order, int interp) {
+  if (order < 2) order = 2;
+  if (order != 2) {
+    last_error = "set_mesh_order: only mesh order 2 (trilinear hex8) is supported";
+    return -1;
+  }
+  if (interp < 0 || interp > 2) {
+    last_error = "set_mesh_order: unknown interpolation type";
+    return -1;
+  }
+  free_mesh();
+  order_ = order;
+  interp_ = interp;
+  return 0;
+}
+
 int Forest::create_nodes() {
   if (octs_.empty()) {
     last_error = "create_nodes: no octants have been created";

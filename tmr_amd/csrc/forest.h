@@ -78,6 +78,10 @@ class Forest {
   Forest *coarsen() const;
   Forest *duplicate() const;
   int create_nodes();
+  // TMROctForest::setMeshOrder: orders below 2 become 2 (the reference
+  // clamps to [2, MAX_ORDER]); only order 2 builds nodes here, higher orders
+  // are an error.  Frees the nodes (createNodes builds them again).
+  int set_mesh_order(int order, int interp);
 
   // queries ------------------------------------------------------------------
   const std::vector<Oct> &octants() const { return octs_; }

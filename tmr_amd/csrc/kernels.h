@@ -272,6 +272,9 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st);
 // v[r][i] += alpha y[i] for every right-hand side r (vectors of n values)
 void launch_axpy_bcast(int64_t n, int nrhs, double alpha, const double *y, double *v,
                        hipStream_t st);
+// y[r][v] = sum_j w_j x[col_j][v] over CSR rows (nv values per node)
+void launch_interp_nodal(int nrows, const int *ptr, const int *col, const double *w, int nv,
+                         const double *x, double *y, hipStream_t st);
 // power-iteration start vector: 0.5 + a hash-drawn U[0, 1) per free dof,
 // 0 on clamped dofs
 void launch_start_vector(int64_t n, const uint8_t *bcmask, uint64_t seed, double *x,

@@ -554,6 +554,28 @@ void launch_fill(int64_t n, T v, T *a, hipStream_t st) {
   TMRX_LAUNCH_CHECK();
 }
 
+// y[r][v] = sum_j w_j x[col_j][v] over the CSR row r (nv values per node):
+// the interpolation of a nodal field from one forest to another
+__global__ void k_interp_nodal(int nrows, const int *__restrict__ ptr, const int *__restrict__ col,
+                               const double *__restrict__ w, int nv, const double *__restrict__ x,
+                               double *__restrict__ y) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nv) return;
+  const int r = (int)(t / nv), v = (int)(t % nv);
+  double s = 0.0;
+  for (int j = ptr[r]; j < ptr[r + 1]; j++) s += w[j] * x[(int64_t)col[j] * nv + v];
+  y[t] = s;
+}
+
+void launch_interp_nodal(int nrows, const int *ptr, const int *col, const double *w, int nv,
+                         const double *x, double *y, hipStream_t st) {
+  const int64_t n = (int64_t)nrows * nv;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_interp_nodal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nrows,
+                     ptr, col, w, nv, x, y);
+  TMRX_LAUNCH_CHECK();
+}
+
 // power-iteration start: x_i = 0.5 + U_i on free dofs (U_i in [0, 1) from
 // the splitmix64 hash of seed + i, so every dof draws independently), 0 on
 // clamped ones

@@ -317,15 +317,22 @@ class OctConformTopoCreator:
 
 class Mg:
     """The multigrid-preconditioned solver of a level hierarchy (TACSMg from
-    TMR_CreateTACSMg).  Smoothing is Chebyshev on the Jacobi-scaled operator
-    and the coarsest level is a direct (Cholesky) solve whatever the flags;
-    ``use_galerkin`` (algebraic coarse operators) is not built."""
+    TMR_CreateTACSMg, src/TMR_RefinementTools.cpp:37-107).  ``use_galerkin``
+    forms the coarse operators P^T A P on the GPU (tmrx_mg_create).  The
+    smoother is Chebyshev on the Jacobi-scaled operator and the coarsest
+    level a direct FP64 Cholesky solve: TMR's default symmetric SOR
+    (``use_chebyshev_smoother=False``) is replaced by Chebyshev smoothing
+    with a warning, and ``use_coarse_direct_solve=False`` is an error."""
 
     def __init__(self, assemblers, forests, omega=1.0, use_galerkin=False,
                  use_coarse_direct_solve=True, use_chebyshev_smoother=False, device=0):
-        if use_galerkin:
-            raise NotImplementedError("Galerkin coarse operators are not supported; "
-                                      "levels are rediscretised")
+        if not use_coarse_direct_solve:
+            raise NotImplementedError("the coarsest level is always solved directly (FP64 "
+                                      "Cholesky); use_coarse_direct_solve=False is not available")
+        if not use_chebyshev_smoother:
+            import warnings
+            warnings.warn("symmetric SOR smoothing is not available on the GPU; the levels "
+                          "are smoothed with Chebyshev", stacklevel=2)
         if len(assemblers) != len(forests) or len(assemblers) < 1:
             raise ValueError("createMg: list length mismatch")
         a0 = assemblers[0]
@@ -338,6 +345,8 @@ class Mg:
         a0.problem = self.problem
         if a0.bcs.has_values():
             self.problem.set_bc_values(a0.bcs.values_vector(self.forests[0]))
+        self.problem.set_multigrid(omega=omega, use_galerkin=use_galerkin,
+                                   use_coarse_direct_solve=True, use_chebyshev_smoother=True)
         self.opts = solver.default_opts()
 
     def getAssembler(self):
@@ -385,6 +394,14 @@ class TopoFilter:
         design)."""
         return self.assemblers[0].to_design_space(g_rho)
 
+    def filter_nodal(self, x):
+        """The nodal design the analysis sees (TMRConformFilter: x itself)."""
+        return x
+
+    def transpose_nodal(self, g):
+        """Adjoint of filter_nodal for a per-node gradient."""
+        return g
+
 
 class LagrangeFilter(TopoFilter):
     """Design variables on the finest filter forest's nodes
@@ -403,6 +420,96 @@ class ConformFilter(TopoFilter):
         if not all(getattr(a, "nodal", False) for a in assemblers):
             raise ValueError("ConformFilter needs OctConformTopoCreator assemblers")
         super().__init__(assemblers, filters)
+
+
+class _NativeFilter(ConformFilter):
+    """A TMRConformFilter whose analysis design is the filtered design
+    variables: the filter and its transpose run natively on the finest
+    filter forest (tmrx_filter_*)."""
+
+    def __init__(self, kind, r, nterms, assemblers, filters, vars_per_node=1, device=0):
+        import ctypes as C
+        from ._lib import check, lib
+        super().__init__(assemblers, filters)
+        if vars_per_node != 1:
+            raise NotImplementedError("one design variable per node only")
+        self.vars_per_node = vars_per_node
+        h = C.c_void_p()
+        check(lib().tmrx_filter_create(self.filters[0].ptr, int(kind), float(r), int(nterms),
+                                       int(device), C.byref(h)), type(self).__name__)
+        self._h = h
+        n = C.c_int()
+        check(lib().tmrx_filter_info(self._h, C.byref(n), None))
+        self.num_nodes = n.value
+
+    def __del__(self):
+        try:
+            from ._lib import lib
+            if getattr(self, "_h", None):
+                lib().tmrx_filter_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _apply(self, x, transpose):
+        import ctypes as C
+        from ._lib import check, lib
+        x = np.ascontiguousarray(x, dtype=np.double)
+        if x.size != self.num_nodes * self.vars_per_node:
+            raise ValueError("expected %d values, got %d" % (self.num_nodes * self.vars_per_node,
+                                                               x.size))
+        out = np.zeros_like(x)
+        fn = lib().tmrx_filter_apply_transpose if transpose else lib().tmrx_filter_apply
+        check(fn(self._h, self.vars_per_node, C.c_void_p(x.ctypes.data),
+                 C.c_void_p(out.ctypes.data), 0), "filter")
+        return out
+
+    def applyFilter(self, vec_in, vec_out=None):
+        """TMRTopoFilter::applyFilter (src/topology/TMRTopoFilter.h:59-62)."""
+        out = self._apply(vec_in, False)
+        if vec_out is not None:
+            vec_out[:] = out
+            return vec_out
+        return out
+
+    def applyTranspose(self, vec_in, vec_out=None):
+        """TMRTopoFilter::applyTranspose (src/topology/TMRTopoFilter.h:63-66)."""
+        out = self._apply(vec_in, True)
+        if vec_out is not None:
+            vec_out[:] = out
+            return vec_out
+        return out
+
+    def filter_nodal(self, x):
+        return self._apply(x, False)
+
+    def transpose_nodal(self, g):
+        return self._apply(g, True)
+
+    def last_iterations(self):
+        import ctypes as C
+        from ._lib import check, lib
+        it = C.c_int()
+        check(lib().tmrx_filter_info(self._h, None, C.byref(it)))
+        return it.value
+
+
+class HelmholtzFilter(_NativeFilter):
+    """TMRHelmholtzFilter (src/topology/TMRHelmholtzFilter.cpp; Python
+    TMR.HelmholtzFilter(r, assemblers, filters, vars_per_node)): the
+    analysis design is (M + r^2 K)^-1 M x on the finest filter forest."""
+
+    def __init__(self, r, assemblers, filters, vars_per_node=1, device=0):
+        super().__init__(0, r, 0, assemblers, filters, vars_per_node, device)
+
+
+class MatrixFilter(_NativeFilter):
+    """TMRMatrixFilter (src/topology/TMRMatrixFilter.cpp; Python
+    TMR.MatrixFilter(r, N, assemblers, filters, vars_per_node)): the N-term
+    polynomial mass-matrix filter."""
+
+    def __init__(self, r, N, assemblers, filters, vars_per_node=1, device=0):
+        super().__init__(1, r, N, assemblers, filters, vars_per_node, device)
 
 
 # ---------------------------------------------------------------------------
@@ -536,6 +643,12 @@ class TopoProblem:
         ub[:] = self.xub if self.xub is not None else ub0
         return x, lb, ub
 
+    def _to_design(self, g_native):
+        """Native gradient (per element or per analysis node) to the design
+        space: the creator's weights, then the filter transpose."""
+        g = self.assembler.to_design_space(g_native)
+        return self.filter.transpose_nodal(g) if self.assembler.nodal else g
+
     def _eval_func(self, func, case):
         p = self.problem
         if isinstance(func, Compliance):
@@ -582,6 +695,8 @@ class TopoProblem:
         self.x_design = np.array(x, dtype=np.double)
         if self.assembler.nodal:
             self.rho = None
+            # the analysis design (filtered by a Helmholtz or matrix filter)
+            self.x_design = np.array(self.filter.filter_nodal(self.x_design), dtype=np.double)
             self.problem.set_design_nodal(self.x_design)
         else:
             self.rho = self.assembler.element_densities(x)
@@ -652,14 +767,14 @@ class TopoProblem:
             if fn is None:
                 continue
             grho += self.obj_weights[i] * self._eval_func_grad(fn, i)
-        g += a.to_design_space(grho)
+        g += self._to_design(grho)
         k = 0
         for v in self.lin_vecs:
             A[k][:] = v
             k += 1
         for case, funcs, offs, scales in self.cons:
             for fn, s in zip(funcs, scales):
-                A[k][:] = s * a.to_design_space(self._eval_func_grad(fn, case))
+                A[k][:] = s * self._to_design(self._eval_func_grad(fn, case))
                 k += 1
         for ncb, _, cgf in self.concb:
             cgf(self.filter, self.mg, A[k:k + ncb])
