@@ -370,6 +370,14 @@ int tmrx_problem_compliance_grad(tmrx_problem *p, double *g, int on_device);
    aggregated like TACS's "pnorm-continuous" KSFailure used in
    examples/cylinder/cylinder.py:1040-1042). */
 int tmrx_problem_stress_pnorm(tmrx_problem *p, double P, double *value);
+/* No counterpart in the reference: tmrx_problem_stress_pnorm for callers
+   that cannot allocate device memory (code under HIP graph capture, serving
+   loops, harnesses that forbid hipMalloc).  Allocates nothing: the partial
+   sums live in a fixed device array of the library (65,552 doubles, about
+   512 KiB per GPU, held until the process ends), so problems of at most
+   16,777,216 elements are accepted (larger ones return an error).  Every call
+   on a GPU shares that array: calls that overlap on one GPU race. */
+int tmrx_problem_stress_pnorm_noalloc(tmrx_problem *p, double P, double *value);
 /* Its gradient w.r.t. the element densities: explicit relaxation term
    (TMROctConstitutive::addFailureDVSens, :1159-1215) minus psi^T dK/drho u,
    psi from the adjoint solve K psi = df/du -- the general-functional branch

@@ -108,6 +108,7 @@ _SIGS = {
     "tmrx_problem_apply": (I, [P, P, P, I, I]),
     "tmrx_problem_traction_load": (I, [P, S, DP, DP]),
     "tmrx_problem_stress_pnorm": (I, [P, D, DP]),
+    "tmrx_problem_stress_pnorm_noalloc": (I, [P, D, DP]),
     "tmrx_problem_stress_pnorm_grad": (I, [P, D, P, I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)]),
     "tmrx_problem_kernel_timing": (I, [P, I]),
     "tmrx_problem_kernel_timing_result": (I, [P, DP, IP, IP]),

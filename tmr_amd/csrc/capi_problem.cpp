@@ -310,6 +310,16 @@ int tmrx_problem_level_matrix(tmrx_problem *h, int level, int *n, int *nnzb, int
   return 0;
 }
 
+int tmrx_problem_stress_pnorm_noalloc(tmrx_problem *h, double P, double *value) {
+  CHECK_P(h);
+  if (!value) {
+    set_error("tmrx_problem_stress_pnorm_noalloc: null output");
+    return -1;
+  }
+  if (h->p->stress_pnorm_noalloc(P, value)) return fail(h);
+  return 0;
+}
+
 int tmrx_problem_num_components(tmrx_problem *h, int *ncomp) {
   CHECK_P(h);
   if (ncomp) *ncomp = h->p->num_segments();

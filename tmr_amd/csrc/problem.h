@@ -213,6 +213,9 @@ class Problem {
   // p-norm (exponent P) of the relaxed von Mises failure of the last solve,
   // and its gradient w.r.t. the element densities through an adjoint solve
   int stress_pnorm(double P, double *f);
+  // the same value with the reduction scratch in a fixed file-scope device
+  // array (no allocation; calls that overlap on one GPU race)
+  int stress_pnorm_noalloc(double P, double *f);
   int stress_pnorm_grad(double P, double *g, bool on_device, const SolveOpts &o,
                         SolveInfo *info);
   int apply(const double *x, double *y, bool on_device, int precision);
