@@ -386,6 +386,21 @@ int tmrx_problem_stress_pnorm_noalloc(tmrx_problem *p, double P, double *value);
    relaxation uses the Gauss-point density and g is per finest-level node. */
 int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_device,
                                    const tmrx_solve_opts *opts, tmrx_solve_info *info);
+/* One component of the multigrid preconditioner on its own, in the
+   V-cycle's precision (opts->precision), from a zero start -- the pieces
+   TACSMg composes from TMR_CreateTACSMg's objects
+   (src/TMR_RefinementTools.cpp:37-107): op 0 y = A_l x (the level operator,
+   rediscretised or Galerkin); 1 y = R_l x (level l to l + 1, the transpose
+   of createInterpolation's P, src/TMROctForest.cpp:6611-6793, clamped rows
+   zero); 2 y = P_l x (level l + 1 to l); 3 y = the pre-smoothing sweep of
+   level l (TACSChebyshevSmoother's role, :64-71) with right-hand side x;
+   4 y = the coarsest level's direct solve (:83-86); 5 y = the V-cycle from
+   level l.  x and y have the sizes of the input and output levels.
+   lam_max (may be null) receives the level's estimate of the largest
+   eigenvalue of Dinv A (0 on the coarsest level).  Sets the preconditioner
+   up first if the design changed. */
+int tmrx_problem_mg_op(tmrx_problem *p, int op, int level, const double *x, double *y,
+                       int on_device, const tmrx_solve_opts *opts, double *lam_max);
 /* y = K(rho) x on the finest level (precision 0 FP64, 1 FP32 on the matrix
    cores, 2 FP32 on the vector ALUs). */
 int tmrx_problem_apply(tmrx_problem *p, const double *x, double *y, int on_device, int precision);

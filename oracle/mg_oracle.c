@@ -448,6 +448,41 @@ static void vcycle(int l, const double *b, double *x) {
     sor_symmetric(l, b, x);
 }
 
+/* level l's estimate of lambda_max(D^-1 A); set replaces it (to run the
+   smoother with another implementation's estimate) */
+double orc_mg_get_lambda(int l) { return (l >= 0 && l < g.nl) ? g.lam[l] : 0.0; }
+int orc_mg_set_lambda(int l, double lam) {
+  if (l < 0 || l >= g.nl) return -1;
+  g.lam[l] = lam;
+  return 0;
+}
+
+/* one component alone: 0 y = A_l x; 1 y = R_l x; 2 y = P_l x (x on level
+   l + 1); 3 y = level l's pre-smoothing from zero with right-hand side x;
+   4 y = the coarsest solve; 5 y = the V-cycle from level l */
+int orc_mg_op(int op, int l, const double *x, double *y) {
+  if (l < 0 || l >= g.nl) return -1;
+  const int last = g.nl - 1;
+  if ((op >= 1 && op <= 3 && l == last) || (op == 4 && l != last)) return -1;
+  switch (op) {
+    case 0: spmv(&g.A[l], x, y); break;
+    case 1: spmv(&g.R[l], x, y); break;
+    case 2: spmv(&g.P[l], x, y); break;
+    case 3:
+      if (g.smoother == 0) {
+        chebyshev(l, x, y, g.degree, 1);
+      } else {
+        memset(y, 0, sizeof(double) * g.A[l].n);
+        sor_symmetric(l, x, y);
+      }
+      break;
+    case 4: coarse_solve(x, y); break;
+    case 5: vcycle(l, x, y); break;
+    default: return -1;
+  }
+  return 0;
+}
+
 /* flexible PCG (Polak-Ribiere beta) with one V-cycle per iteration, from
    x = 0; b zero on clamped dofs.  Returns iterations (negative if not
    converged) and the recursive relative residual. */

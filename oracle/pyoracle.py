@@ -498,6 +498,12 @@ def _mglib():
         L.orc_mg_pcg.argtypes = [P, P, D, I, P]
         L.orc_mg_gmres.restype = I
         L.orc_mg_gmres.argtypes = [P, P, I, D, I, P]
+        L.orc_mg_get_lambda.restype = D
+        L.orc_mg_get_lambda.argtypes = [I]
+        L.orc_mg_set_lambda.restype = I
+        L.orc_mg_set_lambda.argtypes = [I, D]
+        L.orc_mg_op.restype = I
+        L.orc_mg_op.argtypes = [I, I, P, P]
         L._mg_sigs = True
     return L
 
@@ -544,6 +550,24 @@ class Multigrid:
         v = np.zeros(nnz.value)
         L.orc_mg_get_matrix(l, None, None, _p(rp), _p(cl), _p(v))
         return sp.csr_matrix((v, cl, rp), shape=(n.value, n.value))
+
+    def get_lambda(self, l):
+        return float(_mglib().orc_mg_get_lambda(int(l)))
+
+    def set_lambda(self, l, lam):
+        _mglib().orc_mg_set_lambda(int(l), float(lam))
+
+    OPS = {"apply": 0, "restrict": 1, "prolong": 2, "smooth": 3, "coarse": 4, "vcycle": 5}
+
+    def op(self, name, l, x):
+        """One component alone (orc_mg_op), after setup()."""
+        code = self.OPS[name]
+        nout = 3 * len(self.masks[l + 1 if code == 1 else l])
+        x = np.ascontiguousarray(x, dtype=np.double)
+        y = np.zeros(nout)
+        if _mglib().orc_mg_op(code, int(l), _p(x), _p(y)) != 0:
+            raise ValueError("oracle mg op %r on level %d" % (name, l))
+        return y
 
     def _rhs(self, b):
         b = np.ascontiguousarray(b, dtype=np.double).copy()

@@ -60,8 +60,11 @@ def test_filter_matches_oracle(adapt, kind, r, N):
     print("%s r=%g N=%d: %d nodes, filter rel err %.2e, transpose %.2e, CG its %d"
           % ("helmholtz" if kind == 0 else "matrix", r, N, n, e1, e2, iters()), flush=True)
     assert e1 < 1e-10 and e2 < 1e-10
-    # a constant field passes unchanged
-    assert np.allclose(run(np.full(n, 0.7)), 0.7, rtol=1e-11, atol=0)
+    # a constant field passes unchanged (up to the Helmholtz filter's CG
+    # tolerance, 1e-12 in the residual, times the operator's condition number)
+    dev = np.abs(run(np.full(n, 0.7)) - 0.7).max() / 0.7
+    print("constant field: max relative deviation %.2e" % dev, flush=True)
+    assert dev < 1e-9
     # adjoint identity
     assert abs(fx @ y - x @ fty) <= 1e-11 * abs(fx @ y)
     # interleaved values per node: each component filtered alone

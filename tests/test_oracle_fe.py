@@ -157,3 +157,17 @@ def test_stress_gradient_central_differences():
             vals.append(po.stress_pnorm(orc, r2, u2))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-8 * abs(val))
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_element_by_element_product_equals_assembled(adaptive):
+    """orc_apply_free (the full-size residual check) against the assembled
+    CSR product, clamped rows and columns included."""
+    f, orc, rho, s, b = _small_problem(adaptive=adaptive)
+    x = np.random.default_rng(5).standard_normal(3 * orc.nnodes)
+    y_ref = orc.matvec(x)
+    y = orc.apply_free(s, x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-14
+    fr = np.where(orc.free_mask(), b, 0.0)
+    u = orc.solve(fr, rtol=1e-13)
+    assert orc.true_residual(s, fr, u) < 1e-12

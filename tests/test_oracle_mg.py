@@ -93,3 +93,38 @@ def test_uniform_density_needs_few_iterations():
     mg.setup("chebyshev", degree=2)
     mg.pcg(f, rtol=1e-10)
     assert mg.iters <= 14, mg.iters
+
+
+def test_single_components(adapt):
+    """orc_mg_op (the per-component checker of the GPU's multigrid): the
+    level operators and transfers equal the matrices they are built from,
+    the V-cycle is a symmetric operator (pre- and post-smoothing by the
+    same Chebyshev polynomial), and the coarse solve inverts the coarsest
+    Galerkin matrix."""
+    forests, rho, orc = adapt
+    mg = po.Multigrid(orc, forests, BCS)
+    mg.setup("chebyshev", degree=2)
+    rng = np.random.default_rng(4)
+    nl = len(forests)
+    free = [~(((m[:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1)) for m in mg.masks]
+    for l in range(nl - 1):
+        x = rng.standard_normal(len(free[l]))
+        assert np.allclose(mg.op("apply", l, x), mg.matrix(l) @ x, rtol=1e-13, atol=1e-13)
+        _, _, (prp, pcl, pw) = po.interpolation((2, 2, 2), forests[l].getOctants(),
+                                                forests[l + 1].getOctants())
+        nf, nc = len(prp) - 1, int(forests[l + 1].getNodeRange()[1])
+        P = sp.kron(sp.csr_matrix((pw, pcl, prp), shape=(nf, nc)), sp.identity(3), format="csr")
+        Pm = sp.diags(free[l].astype(float)) @ P @ sp.diags(free[l + 1].astype(float))
+        assert np.allclose(mg.op("restrict", l, x), Pm.T @ x, rtol=1e-14, atol=1e-14)
+        xc = rng.standard_normal(len(free[l + 1]))
+        assert np.allclose(mg.op("prolong", l, xc), Pm @ xc, rtol=1e-14, atol=1e-14)
+    a = np.where(free[0], rng.standard_normal(len(free[0])), 0.0)
+    b = np.where(free[0], rng.standard_normal(len(free[0])), 0.0)
+    va, vb = mg.op("vcycle", 0, a), mg.op("vcycle", 0, b)
+    assert abs(b @ va - a @ vb) < 1e-10 * abs(b @ va)
+    lc = nl - 1
+    bc = np.where(free[lc], rng.standard_normal(len(free[lc])), 0.0)
+    yc = mg.op("coarse", lc, bc)
+    assert np.linalg.norm(mg.matrix(lc) @ yc - bc) < 1e-10 * np.linalg.norm(bc)
+    with pytest.raises(ValueError):
+        mg.op("coarse", 0, bc)

@@ -106,6 +106,7 @@ _SIGS = {
     "tmrx_problem_compliance": (I, [P, DP]),
     "tmrx_problem_compliance_grad": (I, [P, P, I]),
     "tmrx_problem_apply": (I, [P, P, P, I, I]),
+    "tmrx_problem_mg_op": (I, [P, I, I, P, P, I, P, P]),
     "tmrx_problem_traction_load": (I, [P, S, DP, DP]),
     "tmrx_problem_stress_pnorm": (I, [P, D, DP]),
     "tmrx_problem_stress_pnorm_noalloc": (I, [P, D, DP]),

@@ -357,6 +357,12 @@ int tmrx_problem_compliance_grad(tmrx_problem *h, double *g, int on_device) {
   return h->p->compliance_grad(g, on_device != 0) ? fail(h) : 0;
 }
 
+int tmrx_problem_mg_op(tmrx_problem *h, int op, int level, const double *x, double *y,
+                       int on_device, const tmrx_solve_opts *opts, double *lam_max) {
+  CHECK_P(h);
+  return h->p->mg_op(op, level, x, y, on_device != 0, to_opts(opts), lam_max) ? fail(h) : 0;
+}
+
 int tmrx_problem_apply(tmrx_problem *h, const double *x, double *y, int on_device,
                        int precision) {
   CHECK_P(h);

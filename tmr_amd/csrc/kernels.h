@@ -190,6 +190,12 @@ void launch_elem_apply(const ElemBatches &eb, const T *K0, const T *s_elem, cons
 template <class T>
 void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *partial,
                       hipStream_t st, bool nodal = false);
+// Dinv of the nodes with corner-pair terms: dinv = 1 / (1 / dinv + sum_k
+// w_k K_e[a][b]) per direction, K_e = s_e K0[cls] (element design) or
+// sum_g s_g K_g (nodal design, K0 the NODAL_TABLE tables); clamped dofs kept
+void launch_diag_cross(int n, const int *ptr, const int *node, const int *code, const double *w,
+                       const int *cls, const double *K0, const double *s, bool nodal,
+                       const uint8_t *bcmask, double *dinv, hipStream_t st);
 template <class T>
 void launch_assemble(const AsmCSR &a, const uint8_t *bcmask, const T *partial, int mode,
                      const AsmArgs &args, hipStream_t st);

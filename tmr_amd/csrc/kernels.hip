@@ -230,6 +230,57 @@ void launch_elem_diag(const ElemBatches &eb, const T *K0, const T *s_elem, T *pa
   TMRX_LAUNCH_CHECK();
 }
 
+// corner-pair diagonal terms (elements with hanging corners), one thread
+// per node that has them; records in a fixed order, so the sums are too
+template <bool NODAL>
+__global__ __launch_bounds__(256) void k_diag_cross(
+    int n, const int *__restrict__ ptr, const int *__restrict__ node, const int *__restrict__ code,
+    const double *__restrict__ w, const int *__restrict__ cls, const double *__restrict__ K0,
+    const double *__restrict__ s, const uint8_t *__restrict__ bcmask, double *__restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int j = ptr[i]; j < ptr[i + 1]; j++) {
+    const int cd = code[j], e = cd >> 6, a = (cd >> 3) & 7, b = cd & 7;
+    const double wj = w[j];
+    if (NODAL) {
+      // K_g[3a+d][3b+d] = |J| w [(lambda + mu) G_ad G_bd + mu G_a . G_b]
+      const double *__restrict__ Gt = K0 + NODAL_TABLE * cls[e];
+      const double lam = Gt[192], mu = Gt[193];
+      for (int g = 0; g < 8; g++) {
+        const double *ga = Gt + 24 * g + 3 * a, *gb = Gt + 24 * g + 3 * b;
+        const double sg = wj * s[8 * (size_t)e + g];
+        const double dt = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+        for (int d = 0; d < 3; d++) c[d] += sg * ((lam + mu) * ga[d] * gb[d] + dt);
+      }
+    } else {
+      const double *__restrict__ K = K0 + 576 * cls[e];
+      const double se = wj * s[e];
+      for (int d = 0; d < 3; d++) c[d] += se * K[(3 * a + d) * 24 + 3 * b + d];
+    }
+  }
+  const int nd = node[i];
+  const uint8_t m = bcmask[nd];
+  for (int d = 0; d < 3; d++)
+    if (!((m >> d) & 1)) {
+      const size_t k = 3 * (size_t)nd + d;
+      dinv[k] = 1.0 / (1.0 / dinv[k] + c[d]);
+    }
+}
+
+void launch_diag_cross(int n, const int *ptr, const int *node, const int *code, const double *w,
+                       const int *cls, const double *K0, const double *s, bool nodal,
+                       const uint8_t *bcmask, double *dinv, hipStream_t st) {
+  if (n == 0) return;
+  if (nodal)
+    hipLaunchKernelGGL(k_diag_cross<true>, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, node,
+                       code, w, cls, K0, s, bcmask, dinv);
+  else
+    hipLaunchKernelGGL(k_diag_cross<false>, dim3((n + 255) / 256), dim3(256), 0, st, n, ptr, node,
+                       code, w, cls, K0, s, bcmask, dinv);
+  TMRX_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // assembly pass: node-parallel weighted gather of slot partials + epilogue
 constexpr int NODE_BLOCK = 256;

@@ -454,6 +454,60 @@ int Problem::build_level(int l, int nbcs, const char **names, const int *masks) 
       }
     }
   }
+  // Diagonal of A = Z^T K Z beyond the per-corner terms w^2 K_e[a][a] that
+  // the slot gather forms: pairs of distinct corners a < b of one element
+  // whose constraint rows share an independent node i add
+  // 2 Z_ai Z_bi K_e[a][b] (per dof direction) to A_ii.  Only elements with
+  // hanging corners have such pairs.  Records grouped by node i:
+  // code = 64 e + 8 a + b, weight 2 Z_ai Z_bi.
+  {
+    std::vector<std::array<int, 3>> rec;  // node, code, index into wt
+    std::vector<double> wt;
+    std::vector<std::pair<int, double>> lists[8];
+    for (int e = 0; e < ne; e++) {
+      bool hang = false;
+      for (int a = 0; a < 8; a++) hang |= L.conn[8 * e + a] < 0;
+      if (!hang) continue;
+      for (int a = 0; a < 8; a++) {
+        lists[a].clear();
+        const int g = L.conn[8 * e + a];
+        if (g >= 0) {
+          lists[a].push_back({g, 1.0});
+        } else {
+          const int d = -g - 1;
+          for (int j = dptr[d]; j < dptr[d + 1]; j++) lists[a].push_back({dconn[j], dw[j]});
+        }
+      }
+      for (int a = 0; a < 8; a++)
+        for (int b2 = a + 1; b2 < 8; b2++)
+          for (const auto &pa : lists[a])
+            for (const auto &pb : lists[b2])
+              if (pa.first == pb.first) {
+                rec.push_back({pa.first, 64 * e + 8 * a + b2, (int)wt.size()});
+                wt.push_back(2.0 * pa.second * pb.second);
+              }
+    }
+    std::stable_sort(rec.begin(), rec.end(),
+                     [](const std::array<int, 3> &x, const std::array<int, 3> &y) { return x[0] < y[0]; });
+    std::vector<int> xptr{0}, xnode, xcode;
+    std::vector<double> xw;
+    for (size_t k = 0; k < rec.size(); k++) {
+      if (k == 0 || rec[k][0] != rec[k - 1][0]) {
+        if (k) xptr.push_back((int)k);
+        xnode.push_back(rec[k][0]);
+      }
+      xcode.push_back(rec[k][1]);
+      xw.push_back(wt[rec[k][2]]);
+    }
+    xptr.push_back((int)rec.size());
+    L.n_xd = (int)xnode.size();
+    if (L.n_xd &&
+        (L.xd_ptr.upload(xptr) || L.xd_node.upload(xnode) || L.xd_code.upload(xcode) ||
+         L.xd_w.upload(xw))) {
+      last_error = "problem: device allocation failed";
+      return -1;
+    }
+  }
   std::vector<int> dp(dptr.begin(), dptr.end());
   if (dp.empty()) dp.push_back(0);
   std::vector<float> dwf(dw.begin(), dw.end());
@@ -1380,6 +1434,10 @@ int Problem::setup(const SolveOpts &o) {
     AsmArgs a;
     a.y = L.v64.dinv.p;
     launch_assemble<double>(L.asmc, L.d_bcmask.p, L.v64.partial.p, ASM_DIAG, a, st_);
+    launch_diag_cross(L.n_xd, L.xd_ptr.p, L.xd_node.p, L.xd_code.p, L.xd_w.p, L.d_cls.p,
+                      design_mode_ == 1 ? G64_.p : K0d_.p,
+                      design_mode_ == 1 ? L.sg64.p : L.s64.p, design_mode_ == 1,
+                      L.d_bcmask.p, L.v64.dinv.p, st_);
     if (prec == 1) launch_convert<double, float>(L.ndof, L.v64.dinv.p, L.v32.dinv.p, st_);
     if (estimate_eigs((int)l, o)) return -1;
   }
@@ -1979,6 +2037,116 @@ int Problem::stress_pnorm_grad(double P, double *g, bool on_device, const SolveO
     HIPCHK(hipMemcpyAsync(g, out, L.n_elem * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   return 0;
+}
+
+// One component of the V-cycle on its own, in the precision of the
+// V-cycle (o.precision): the pieces that the solve only shows through its
+// iteration count, so that each can be compared with an assembled operator.
+// The input is level lin's vector, the output level lout's.
+template <class T>
+int Problem::mg_op_t(int op, int l, const double *x, double *y, bool on_device,
+                     const SolveOpts &o) {
+  const int lin = op == MG_PROLONG ? l + 1 : l;
+  const int lout = op == MG_RESTRICT ? l + 1 : l;
+  Level &Li = levels_[lin], &Lo = levels_[lout];
+  LevelVecs<T> &vi = vecs<T>(lin), &vo = vecs<T>(lout);
+  const bool f32 = sizeof(T) == 4;
+  // the input goes to the b buffer of its level (x for the prolongation,
+  // which reads the coarse level's solution); FP32 through the p_ staging
+  T *in = op == MG_PROLONG ? vi.x.p : vi.b.p;
+  const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (f32) {
+    HIPCHK(hipMemcpyAsync(p_.p, x, Li.ndof * sizeof(double), kin, st_));
+    launch_convert<double, float>(Li.ndof, p_.p, reinterpret_cast<float *>(in), st_);
+  } else {
+    HIPCHK(hipMemcpyAsync(in, x, Li.ndof * sizeof(double), kin, st_));
+  }
+  T *out = nullptr;
+  switch (op) {
+    case MG_APPLY: {
+      AsmArgs a;
+      a.y = vo.r.p;
+      apply_level<T>(l, in, ASM_STORE, a, 1);
+      out = vo.r.p;
+      break;
+    }
+    case MG_RESTRICT:
+      launch_restrict<T>(Li.R, Lo.d_bcmask.p, in, vo.b.p, st_, 1, Li.ndof, Lo.ndof);
+      out = vo.b.p;
+      break;
+    case MG_PROLONG:
+      HIPCHK(hipMemsetAsync(vo.x.p, 0, Lo.ndof * sizeof(T), st_));
+      launch_prolong_add<T>(Lo.P, Lo.d_bcmask.p, in, vo.x.p, st_, 1, Li.ndof, Lo.ndof);
+      out = vo.x.p;
+      break;
+    case MG_SMOOTH: {
+      // the V-cycle's pre-smoothing: c0 Dinv b, then pre - 1 ping-pong steps
+      const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : std::max(1, o.cheb_degree);
+      T *cur = vo.x.p, *oth = vo.x2.p;
+      launch_cheb_first<T>((int)Lo.ndof, in, vo.dinv.p, cheb_c0<T>(l, o), nullptr, cur, st_, 1);
+      for (int k = 1; k < pre; k++) {
+        const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < pre ? CHEB_D_STORE : 0);
+        cheb_step<T>(l, in, cur, oth, k, o, 1, fl);
+        std::swap(cur, oth);
+      }
+      out = cur;
+      break;
+    }
+    case MG_COARSE:
+      launch_dense_solve<T>(coarse_, in, vo.x.p, st_, 1);
+      out = vo.x.p;
+      break;
+    default:  // MG_VCYCLE
+      vcycle<T>(l, in, vo.x.p, o, 1);
+      out = vo.x.p;
+      break;
+  }
+  const hipMemcpyKind kout = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  if (f32) {
+    launch_convert<float, double>(Lo.ndof, reinterpret_cast<const float *>(out), q_.p, st_);
+    HIPCHK(hipMemcpyAsync(y, q_.p, Lo.ndof * sizeof(double), kout, st_));
+  } else {
+    HIPCHK(hipMemcpyAsync(y, out, Lo.ndof * sizeof(double), kout, st_));
+  }
+  HIPCHK(hipStreamSynchronize(st_));
+  return 0;
+}
+
+int Problem::mg_op(int op, int l, const double *x, double *y, bool on_device,
+                   const SolveOpts &o, double *lam) {
+  const int nl = (int)levels_.size();
+  if (!have_design_) {
+    last_error = "mg_op: set the design first";
+    return -1;
+  }
+  if (op < MG_APPLY || op > MG_VCYCLE || l < 0 || l >= nl) {
+    last_error = "mg_op: invalid operation " + std::to_string(op) + " or level " + std::to_string(l);
+    return -1;
+  }
+  if ((op == MG_RESTRICT || op == MG_PROLONG || op == MG_SMOOTH) && l == nl - 1) {
+    last_error = "mg_op: level " + std::to_string(l) + " is the coarsest (no transfer or smoother)";
+    return -1;
+  }
+  if (op == MG_COARSE && l != nl - 1) {
+    last_error = "mg_op: the direct solve is on the coarsest level " + std::to_string(nl - 1);
+    return -1;
+  }
+  if (op == MG_APPLY && galerkin_ && l > 0 && l == nl - 1) {
+    last_error = "mg_op: the coarsest Galerkin operator is kept factored only";
+    return -1;
+  }
+  if (!x || !y) {
+    last_error = "mg_op: null vector";
+    return -1;
+  }
+  const int prec = o.precision ? 1 : 0;
+  if (!have_setup_ || setup_precision_ != prec) {
+    if (setup(o)) return -1;
+  }
+  fp32_valu_ = o.fp32_valu != 0;
+  if (lam) *lam = levels_[l].lam_max;
+  return prec ? mg_op_t<float>(op, l, x, y, on_device, o)
+              : mg_op_t<double>(op, l, x, y, on_device, o);
 }
 
 int Problem::apply(const double *x, double *y, bool on_device, int precision) {

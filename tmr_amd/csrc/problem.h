@@ -87,6 +87,16 @@ struct SolveOpts {
   int cheb_pre_degree = 0;  // pre-smoothing degree (0: cheb_degree)
 };
 
+// multigrid components that Problem::mg_op runs on their own
+enum MgOp {
+  MG_APPLY = 0,     // y = A_l x
+  MG_RESTRICT = 1,  // y = R_l x (level l to l + 1, clamped rows zero)
+  MG_PROLONG = 2,   // y = P_l x (level l + 1 to l, clamped rows zero)
+  MG_SMOOTH = 3,    // y = the pre-smoothing sweep of level l from zero, right-hand side x
+  MG_COARSE = 4,    // y = the coarsest level's direct solve of x
+  MG_VCYCLE = 5,    // y = the V-cycle from level l applied to x
+};
+
 struct SolveInfo {
   int iters = 0;
   int converged = 0;
@@ -155,6 +165,12 @@ struct Level {
   DevArray<float> bsr32;
   int nnzb = 0;
   bool has_bsr = false;
+  // diagonal terms of corner pairs that share an independent node
+  // (elements with hanging corners): per node i, codes 64 e + 8 a + b and
+  // weights 2 Z_ai Z_bi
+  int n_xd = 0;
+  DevArray<int> xd_ptr, xd_node, xd_code;
+  DevArray<double> xd_w;
   // stiffness scales
   DevArray<double> s64;
   DevArray<float> s32;
@@ -219,6 +235,9 @@ class Problem {
   int stress_pnorm_grad(double P, double *g, bool on_device, const SolveOpts &o,
                         SolveInfo *info);
   int apply(const double *x, double *y, bool on_device, int precision);
+  // one multigrid component on its own (MG_* below), level l, from zero
+  int mg_op(int op, int l, const double *x, double *y, bool on_device, const SolveOpts &o,
+            double *lam);
   int traction_load(const char *name, const double tr[3], double *f);
   int num_dof() const { return levels_.empty() ? 0 : (int)levels_[0].ndof; }
   int num_elements() const { return levels_.empty() ? 0 : levels_[0].n_elem; }
@@ -267,6 +286,8 @@ class Problem {
   template <class T>
   void cheb_step(int l, const T *b, const T *x_in, T *x_out, int k, const SolveOpts &o,
                  int nrhs = 1, int cflags = CHEB_D_READ | CHEB_D_STORE);
+  template <class T>
+  int mg_op_t(int op, int l, const double *x, double *y, bool on_device, const SolveOpts &o);
   template <class T>
   // first_done: the first smoothing step from zero (c0 Dinv b) is already
   // in the level's spare buffer (written by the restriction or the PCG update)

@@ -326,6 +326,33 @@ class Problem:
         check(lib().tmrx_problem_apply(self.ptr, xp, yp, dev, int(precision)), "apply")
         return y
 
+    MG_OPS = {"apply": 0, "restrict": 1, "prolong": 2, "smooth": 3, "coarse": 4, "vcycle": 5}
+
+    def mg_op(self, op, level, x, opts=None):
+        """One multigrid component alone on `level` (tmrx_problem_mg_op):
+        'apply' A_l x, 'restrict' R_l x (to level + 1), 'prolong' P_l x
+        (from level + 1), 'smooth' the pre-smoothing sweep with right-hand
+        side x, 'coarse' the coarsest solve, 'vcycle' the V-cycle from
+        level.  Runs in the V-cycle's precision (opts.precision).  Returns
+        (y, lambda_max of the level)."""
+        code = self.MG_OPS[op]
+        nl = len(self.forests)
+        lin = level + 1 if code == 2 else level
+        lout = level + 1 if code == 1 else level
+        if not (0 <= lin < nl and 0 <= lout < nl):
+            raise ValueError("mg_op: level %d out of range for %r" % (level, op))
+        nin = 3 * int(self.forests[lin].getNodeRange()[1])
+        nout = 3 * int(self.forests[lout].getNodeRange()[1])
+        x = np.ascontiguousarray(x, dtype=np.double)
+        if x.shape != (nin,):
+            raise ValueError("mg_op: x must have %d entries" % nin)
+        y = np.zeros(nout)
+        lam = C.c_double()
+        opts = opts or default_opts()
+        check(lib().tmrx_problem_mg_op(self.ptr, code, int(level), x.ctypes.data, y.ctypes.data, 0,
+                                       C.byref(opts), C.byref(lam)), "mg_op")
+        return y, lam.value
+
     def kernel_timing(self, mask=-1):
         """Time the finest-level operator launches whose kind is in `mask`
         (1 product, 2 residual, 4 Chebyshev update, 16 product with dot, -1
