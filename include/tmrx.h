@@ -77,6 +77,13 @@ int tmrx_forest_duplicate(tmrx_forest *f, tmrx_forest **out);
 int tmrx_forest_repartition(tmrx_forest *f);
 /* TMROctForest::createNodes (src/TMROctForest.cpp:4064-4268). */
 int tmrx_forest_create_nodes(tmrx_forest *f);
+/* The same createNodes on GPU `device` (computeDepFacesAndEdges,
+   labelDependentNodes, createDependentConn and the node locations of
+   src/TMROctForest.cpp:3619-3832, 5157-5680 as device passes with radix
+   sorts of the node keys): octant info bits, connectivity, hanging
+   constraints, node numbers and points equal tmrx_forest_create_nodes's bit
+   for bit.  ms (may be null) receives the wall time of the pass. */
+int tmrx_forest_create_nodes_device(tmrx_forest *f, int device, double *ms);
 
 /* TMROctForest::getOctants (src/TMROctForest.cpp:1452-1456).  Borrowed,
    valid until the forest is refined/balanced/freed. */

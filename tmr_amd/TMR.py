@@ -143,8 +143,17 @@ class OctForest:
     def balance(self, btype):
         check(lib().tmrx_forest_balance(self.ptr, int(btype)), "balance")
 
-    def createNodes(self):
-        check(lib().tmrx_forest_create_nodes(self.ptr), "createNodes")
+    def createNodes(self, device=None):
+        """TMROctForest::createNodes; with `device` (a GPU index) the same
+        nodes are built on that GPU.  Returns the device pass's wall time in
+        ms, or None on the host."""
+        if device is None:
+            check(lib().tmrx_forest_create_nodes(self.ptr), "createNodes")
+            return None
+        ms = C.c_double()
+        check(lib().tmrx_forest_create_nodes_device(self.ptr, int(device), C.byref(ms)),
+              "createNodes")
+        return ms.value
 
     # -- queries --------------------------------------------------------------
     def getOctants(self):

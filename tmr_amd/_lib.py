@@ -62,6 +62,7 @@ _SIGS = {
     "tmrx_forest_duplicate": (I, [P, C.POINTER(P)]),
     "tmrx_forest_repartition": (I, [P]),
     "tmrx_forest_create_nodes": (I, [P]),
+    "tmrx_forest_create_nodes_device": (I, [P, I, P]),
     "tmrx_forest_get_octants": (I, [P, C.POINTER(C.POINTER(Octant)), IP]),
     "tmrx_forest_get_node_conn": (I, [P, CIP, IP, IP, IP]),
     "tmrx_forest_set_mesh_order": (I, [P, I, I]),

@@ -133,6 +133,11 @@ int tmrx_forest_create_nodes(tmrx_forest *f) {
   return f->f->create_nodes() ? fail_from(f) : 0;
 }
 
+int tmrx_forest_create_nodes_device(tmrx_forest *f, int device, double *ms) {
+  CHECK_HANDLE(f);
+  return f->f->create_nodes_device(device, ms) ? fail_from(f) : 0;
+}
+
 int tmrx_forest_get_octants(tmrx_forest *f, const tmrx_octant **octs, int *n) {
   CHECK_HANDLE(f);
   const auto &o = f->f->octants();

@@ -78,6 +78,9 @@ class Forest {
   Forest *coarsen() const;
   Forest *duplicate() const;
   int create_nodes();
+  // create_nodes on GPU `device`, the same results bit for bit; ms (may be
+  // null) receives the wall time of the device pass with its copies
+  int create_nodes_device(int device, double *ms = nullptr);
   // TMROctForest::setMeshOrder: orders below 2 become 2 (the reference
   // clamps to [2, MAX_ORDER]); only order 2 builds nodes here, higher orders
   // are an error.  Frees the nodes (createNodes builds them again).
