@@ -68,6 +68,12 @@ int tmrx_forest_create_trees(tmrx_forest *f, int level);
 int tmrx_forest_refine(tmrx_forest *f, const int *refinement, int min_level, int max_level);
 /* TMROctForest::balance (src/TMROctForest.cpp:2917-3089). */
 int tmrx_forest_balance(tmrx_forest *f, int balance_corner);
+/* The same balance on GPU `device` (balanceOctant, addFace/Edge/
+   CornerNeighbors, src/TMROctForest.cpp:2525-3089, one thread per octant per
+   level, radix sorts of the 128-bit position keys): the octants, their order
+   and tags equal tmrx_forest_balance's bit for bit.  ms (may be null)
+   receives the wall time of the pass. */
+int tmrx_forest_balance_device(tmrx_forest *f, int balance_corner, int device, double *ms);
 /* TMROctForest::coarsen (src/TMROctForest.cpp:2119-2164). New handle. */
 int tmrx_forest_coarsen(tmrx_forest *f, tmrx_forest **out);
 /* TMROctForest::duplicate (src/TMROctForest.cpp:2097-2109). New handle. */
@@ -127,6 +133,14 @@ int tmrx_forest_get_octs_with_name(tmrx_forest *f, const char *name, tmrx_octant
 int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int *nrows,
                                      int *ncols, const int **rowp, const int **cols,
                                      const double **weights);
+/* The same createInterpolation on GPU `device` (findEnclosing and
+   computeElemInterp, src/TMROctForest.cpp:6228-6594, one thread per fine
+   independent node; the nodes of either forest are built there first if
+   missing): equal to tmrx_forest_create_interpolation's CSR bit for bit.
+   ms (may be null) receives the wall time of the pass. */
+int tmrx_forest_create_interpolation_device(tmrx_forest *fine, tmrx_forest *coarse, int device,
+                                            int *nrows, int *ncols, const int **rowp,
+                                            const int **cols, const double **weights, double *ms);
 
 /* TopOptUtils.interpolateDesignVec (tmr/TopOptUtils.py:270-307): the nodal
    vector x_orig (vars_per_node values per independent node of orig_forest)

@@ -140,8 +140,16 @@ class OctForest:
         f._Xv = getattr(self, "_Xv", None)
         return f
 
-    def balance(self, btype):
-        check(lib().tmrx_forest_balance(self.ptr, int(btype)), "balance")
+    def balance(self, btype, device=None):
+        """TMROctForest::balance; with `device` (a GPU index) on that GPU.
+        Returns the device pass's wall time in ms, or None on the host."""
+        if device is None:
+            check(lib().tmrx_forest_balance(self.ptr, int(btype)), "balance")
+            return None
+        ms = C.c_double()
+        check(lib().tmrx_forest_balance_device(self.ptr, int(btype), int(device), C.byref(ms)),
+              "balance")
+        return ms.value
 
     def createNodes(self, device=None):
         """TMROctForest::createNodes; with `device` (a GPU index) the same
@@ -214,20 +222,29 @@ class OctForest:
         check(lib().tmrx_forest_get_points(self.ptr, C.byref(p), C.byref(n)), "getPoints")
         return np.ctypeslib.as_array(p, (3 * n.value,)).reshape(n.value, 3).copy()
 
-    def createInterpolation(self, forest, vec=None):
+    def createInterpolation(self, forest, vec=None, device=None):
         """CSR of the prolongation from `forest` (coarse) to this forest.
 
         Returns (rowp, cols, weights); if `vec` is given (an object with a
         `setCSR(rowp, cols, weights, ncols)` method) it is filled instead.
+        With `device` (a GPU index) the same CSR is built on that GPU.
         """
         if forest.ptr.value == self.ptr.value:
             raise ValueError("Cannot interpolate between the same object")
         nr, nc = C.c_int(), C.c_int()
         rp, cl = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
         w = C.POINTER(C.c_double)()
-        check(lib().tmrx_forest_create_interpolation(self.ptr, forest.ptr, C.byref(nr), C.byref(nc),
-                                                     C.byref(rp), C.byref(cl), C.byref(w)),
-              "createInterpolation")
+        if device is None:
+            st = lib().tmrx_forest_create_interpolation(self.ptr, forest.ptr, C.byref(nr),
+                                                         C.byref(nc), C.byref(rp), C.byref(cl),
+                                                         C.byref(w))
+        else:
+            self.last_device_ms = C.c_double()
+            st = lib().tmrx_forest_create_interpolation_device(
+                self.ptr, forest.ptr, int(device), C.byref(nr), C.byref(nc), C.byref(rp),
+                C.byref(cl), C.byref(w), C.byref(self.last_device_ms))
+            self.last_device_ms = self.last_device_ms.value
+        check(st, "createInterpolation")
         rowp = np.ctypeslib.as_array(rp, (nr.value + 1,)).copy()
         nnz = int(rowp[-1])
         cols = np.ctypeslib.as_array(cl, (nnz,)).copy() if nnz else np.zeros(0, np.intc)

@@ -58,11 +58,12 @@ _SIGS = {
     "tmrx_forest_create_trees": (I, [P, I]),
     "tmrx_forest_refine": (I, [P, IP, I, I]),
     "tmrx_forest_balance": (I, [P, I]),
+    "tmrx_forest_balance_device": (I, [P, I, I, DP]),
     "tmrx_forest_coarsen": (I, [P, C.POINTER(P)]),
     "tmrx_forest_duplicate": (I, [P, C.POINTER(P)]),
     "tmrx_forest_repartition": (I, [P]),
     "tmrx_forest_create_nodes": (I, [P]),
-    "tmrx_forest_create_nodes_device": (I, [P, I, P]),
+    "tmrx_forest_create_nodes_device": (I, [P, I, DP]),
     "tmrx_forest_get_octants": (I, [P, C.POINTER(C.POINTER(Octant)), IP]),
     "tmrx_forest_get_node_conn": (I, [P, CIP, IP, IP, IP]),
     "tmrx_forest_set_mesh_order": (I, [P, I, I]),
@@ -73,6 +74,7 @@ _SIGS = {
     "tmrx_forest_get_nodes_with_name": (I, [P, S, IP, I]),
     "tmrx_forest_get_octs_with_name": (I, [P, S, C.POINTER(Octant), I]),
     "tmrx_forest_create_interpolation": (I, [P, P, IP, IP, CIP, CIP, CDP]),
+    "tmrx_forest_create_interpolation_device": (I, [P, P, I, IP, IP, CIP, CIP, CDP, DP]),
     "tmrx_forest_element_center_weights": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_forest_interpolate_vec": (I, [P, P, I, P, P, I]),
     "tmrx_forest_generate_bin_file": (I, [P, S, P, I, I, D, IP]),

@@ -103,6 +103,11 @@ int tmrx_forest_balance(tmrx_forest *f, int balance_corner) {
   return f->f->balance(balance_corner) ? fail_from(f) : 0;
 }
 
+int tmrx_forest_balance_device(tmrx_forest *f, int balance_corner, int device, double *ms) {
+  CHECK_HANDLE(f);
+  return f->f->balance_device(balance_corner, device, ms) ? fail_from(f) : 0;
+}
+
 int tmrx_forest_coarsen(tmrx_forest *f, tmrx_forest **out) {
   CHECK_HANDLE(f);
   if (!out) {
@@ -238,6 +243,20 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
   CHECK_HANDLE(fine);
   CHECK_HANDLE(coarse);
   if (fine->f->create_interpolation(coarse->f, fine->P)) return fail_from(fine);
+  if (nrows) *nrows = fine->P.nrows;
+  if (ncols) *ncols = fine->P.ncols;
+  if (rowp) *rowp = fine->P.rowp.data();
+  if (cols) *cols = fine->P.cols.data();
+  if (weights) *weights = fine->P.w.data();
+  return 0;
+}
+
+int tmrx_forest_create_interpolation_device(tmrx_forest *fine, tmrx_forest *coarse, int device,
+                                            int *nrows, int *ncols, const int **rowp,
+                                            const int **cols, const double **weights, double *ms) {
+  CHECK_HANDLE(fine);
+  CHECK_HANDLE(coarse);
+  if (fine->f->create_interpolation_device(coarse->f, fine->P, device, ms)) return fail_from(fine);
   if (nrows) *nrows = fine->P.nrows;
   if (ncols) *ncols = fine->P.ncols;
   if (rowp) *rowp = fine->P.rowp.data();

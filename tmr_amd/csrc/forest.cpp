@@ -1644,8 +1644,9 @@ int Forest::create_interpolation(Forest *coarse, InterpCSR &P) {
               for (int jp = cptr[d]; jp < cptr[d + 1]; jp++) wts.push_back({cdc[jp], w * cdw[jp]});
             }
           }
-      // TMRIndexWeight::uniqueSort (src/TMRBase.h:112-135)
-      std::sort(wts.begin(), wts.end(),
+      // TMRIndexWeight::uniqueSort (src/TMRBase.h:112-135); stable, so that
+      // entries of one column are summed in the order they were formed
+      std::stable_sort(wts.begin(), wts.end(),
                 [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
       auto &row = rows[c[j]];
       for (size_t k = 0; k < wts.size(); k++) {

@@ -75,6 +75,8 @@ class Forest {
   int create_trees(int level);
   int refine(const int *refinement, int min_level, int max_level);
   int balance(int balance_corner);
+  // the same balance on GPU `device`, bit for bit (octants, tags, order)
+  int balance_device(int balance_corner, int device, double *ms = nullptr);
   Forest *coarsen() const;
   Forest *duplicate() const;
   int create_nodes();
@@ -105,6 +107,8 @@ class Forest {
   int nodes_with_name(const char *name, std::vector<int> &out) const;
   int octs_with_name(const char *name, std::vector<Oct> &out) const;
   int create_interpolation(Forest *coarse, InterpCSR &P);
+  // the same on GPU `device` (nodes of both forests built there if missing)
+  int create_interpolation_device(Forest *coarse, InterpCSR &P, int device, double *ms = nullptr);
   // index of the octant whose region contains the lower corner of o, or -1
   int locate(const Oct &o) const;
   // element-centre interpolation weights from a filter forest: rows =

@@ -729,6 +729,415 @@ __global__ void k_points(int nloc, const Oct *__restrict__ octs, const int *__re
 }
 
 // --------------------------------------------------------------------------
+// createInterpolation (src/TMROctForest.cpp:6396-6793): one row per fine
+// independent node, from the first element corner (in element order) that
+// names it; the coarse octant enclosing that corner (findEnclosing,
+// :6228-6377), its trilinear weights there, coarse hanging nodes expanded,
+// then sorted by column (stable) and merged.
+__device__ __forceinline__ int cmp_pos(const Key &a, const Key &b) {
+  return key_less(a, b) ? -1 : (key_less(b, a) ? 1 : 0);
+}
+
+__device__ int find_enclosing(int size, const Oct *__restrict__ co, const P &node, int info) {
+  const int32_t h = hsize(node.level);
+  const int ii = info % 2, jj = (info % 4) / 2, kk = info / 4;
+  const int32_t xi = node.x + ii * h, yi = node.y + jj * h, zi = node.z + kk * h;
+  const Key kn = make_key(node.block, node.x, node.y, node.z);
+  auto contains = [&](const Oct &a) {
+    const int32_t ha = hsize(a.level);
+    return node.block == a.block && node.x >= a.x && node.x < a.x + ha && node.y >= a.y &&
+           node.y < a.y + ha && node.z >= a.z && node.z < a.z + ha;
+  };
+  int low = 0, high = size - 1;
+  int mid = low + (high - low) / 2;
+  while (mid != low) {
+    const Oct a = co[mid];
+    if (contains(a)) break;
+    const int stat = cmp_pos(make_key(a.block, a.x, a.y, a.z), kn);
+    if (stat == 0) break;
+    if (stat < 0)
+      low = mid + 1;
+    else
+      high = mid - 1;
+    mid = low + (high - low) / 2;
+  }
+  const Key bound = make_key(node.block, node.x + h, node.y + h, node.z + h);
+  while (mid < size) {
+    const Oct a = co[mid];
+    if (cmp_pos(make_key(a.block, a.x, a.y, a.z), bound) > 0) break;
+    if (a.block == node.block) {
+      const int32_t hm = hsize(a.level);
+      if (a.x <= xi && xi <= a.x + hm && a.y <= yi && yi <= a.y + hm && a.z <= zi &&
+          zi <= a.z + hm)
+        return mid;
+    }
+    mid++;
+  }
+  return -1;
+}
+
+struct DInterp {
+  int nf;  // fine octants
+  const Oct *fo;
+  const int *fconn;
+  int nc;  // coarse octants
+  const Oct *co;
+  const int *cconn, *cptr, *cdc;
+  const double *cdw;
+};
+
+// the first corner naming each fine independent node
+__global__ void k_first_visit(int64_t m, const int *__restrict__ conn, int *first) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int r = conn[t];
+  if (r >= 0) atomicMin(&first[r], (int)t);
+}
+
+// row of fine node r: length (WRITE false) or entries at rowp[r]
+template <bool WRITE>
+__global__ void k_interp_rows(DInterp D, int n, const int *__restrict__ first, int *len,
+                              const int *__restrict__ rowp, int *cols, double *w, int *bad) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int t0 = first[r], i = t0 >> 3, j = t0 & 7;
+  const P node = load_oct(D.fo, i);
+  const int tc = find_enclosing(D.nc, D.co, node, j);
+  if (tc < 0) {
+    atomicOr(bad, 1);
+    if (!WRITE) len[r] = 0;
+    return;
+  }
+  const Oct t = D.co[tc];
+  const int ijk[3] = {j % 2, (j % 4) / 2, j / 4};
+  const int32_t h = hsize(node.level), hc = hsize(t.level);
+  const int32_t nx[3] = {node.x, node.y, node.z}, tx[3] = {t.x, t.y, t.z};
+  const double knots[2] = {-1.0, 1.0};
+  double N[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  int st[3], en[3];
+  for (int a = 0; a < 3; a++) {
+    st[a] = 0;
+    en[a] = 2;
+    if ((ijk[a] == 0 && tx[a] == nx[a]) || (ijk[a] == 1 && tx[a] == nx[a] + h)) {
+      st[a] = 0; en[a] = 1; N[a][0] = 1.0;
+    } else if ((ijk[a] == 0 && tx[a] + hc == nx[a]) || (ijk[a] == 1 && tx[a] + hc == nx[a] + h)) {
+      st[a] = 1; en[a] = 2; N[a][1] = 1.0;
+    } else {
+      const double u = -1.0 + 2.0 * (nx[a] + 0.5 * h * (1.0 + knots[ijk[a]]) - tx[a]) / hc;
+      N[a][0] = (u - 1.0) * (1.0 / (-1.0 - 1.0));
+      N[a][1] = (u + 1.0) * (1.0 / (1.0 + 1.0));
+    }
+  }
+  // at most 8 corners, each expanded into at most 4 parents
+  int kc[32];
+  double kw[32];
+  int nk = 0;
+  const int *cc = D.cconn + 8 * (size_t)t.tag;
+  for (int z = st[2]; z < en[2]; z++)
+    for (int y = st[1]; y < en[1]; y++)
+      for (int x = st[0]; x < en[0]; x++) {
+        const double wv = N[0][x] * N[1][y] * N[2][z];
+        const int cn = cc[x + 2 * y + 4 * z];
+        if (cn >= 0) {
+          kc[nk] = cn;
+          kw[nk++] = wv;
+        } else {
+          const int d = -cn - 1;
+          for (int jp = D.cptr[d]; jp < D.cptr[d + 1] && nk < 32; jp++) {
+            kc[nk] = D.cdc[jp];
+            kw[nk++] = wv * D.cdw[jp];
+          }
+        }
+      }
+  // stable insertion sort by column, then merge equal columns in order
+  for (int a = 1; a < nk; a++) {
+    const int c = kc[a];
+    const double v = kw[a];
+    int b = a - 1;
+    while (b >= 0 && kc[b] > c) {
+      kc[b + 1] = kc[b];
+      kw[b + 1] = kw[b];
+      b--;
+    }
+    kc[b + 1] = c;
+    kw[b + 1] = v;
+  }
+  int out = 0;
+  for (int a = 0; a < nk; a++) {
+    if (out > 0 && kc[out - 1] == kc[a]) {
+      kw[out - 1] += kw[a];
+    } else {
+      kc[out] = kc[a];
+      kw[out++] = kw[a];
+    }
+  }
+  if (!WRITE) {
+    len[r] = out;
+    return;
+  }
+  const int base = rowp[r];
+  for (int a = 0; a < out; a++) {
+    cols[base + a] = kc[a];
+    w[base + a] = kw[a];
+  }
+}
+
+// --------------------------------------------------------------------------
+// balance (src/TMROctForest.cpp:2525-3089) as the host's level sweep: the
+// 0-children of every level, sorted and de-duplicated, each adding the
+// 0-children of its parent's face, edge (and corner) neighbours one level
+// coarser (balanceOctant, across block faces, edges and corners through the
+// block connectivity); then all siblings of every level, sorted by position
+// and level, keeping the finest octant per position.
+__device__ __forceinline__ P sibling0(const P &o) {
+  const int32_t h = hsize(o.level);
+  P s = o;
+  s.x = (o.x & h) ? o.x - h : o.x;
+  s.y = (o.y & h) ? o.y - h : o.y;
+  s.z = (o.z & h) ? o.z - h : o.z;
+  return s;
+}
+
+__device__ __forceinline__ P sibling_of(const P &o, int id) {
+  const P s = sibling0(o);
+  const int32_t h = hsize(o.level);
+  return P{o.block, (id & 1) ? s.x + h : s.x, (id & 2) ? s.y + h : s.y, (id & 4) ? s.z + h : s.z,
+           o.level};
+}
+
+__device__ __forceinline__ P corner_neighbor(const P &o, int corner) {
+  const int32_t h = hsize(o.level);
+  P n = o;
+  n.x = o.x + (2 * (corner & 1) - 1) * h;
+  n.y = o.y + ((corner & 2) - 1) * h;
+  n.z = o.z + ((corner & 4) / 2 - 1) * h;
+  return n;
+}
+
+// emits into out[base + k] (when out is not null) and counts
+struct Emit {
+  Oct *out;
+  int base, k;
+  __device__ void operator()(const P &q) {
+    if (out) out[base + k] = Oct{q.block, q.x, q.y, q.z, 0, (int16_t)q.level, 0};
+    k++;
+  }
+};
+
+__device__ void add_face_neighbors(const DBlk &B, int face_index, const P &p, Emit &em) {
+  const int block = p.block;
+  const int face = B.block_face_conn[6 * block + face_index];
+  const int face_id = B.block_face_ids[6 * block + face_index];
+  const int32_t h = hsize(p.level);
+  int32_t u, v;
+  if (face_index < 2)
+    get_face_coords(face_id, HMAX - 2 * h, p.y, p.z, u, v);
+  else if (face_index < 4)
+    get_face_coords(face_id, HMAX - 2 * h, p.x, p.z, u, v);
+  else
+    get_face_coords(face_id, HMAX - 2 * h, p.x, p.y, u, v);
+  for (int ip = B.face_block_ptr[face]; ip < B.face_block_ptr[face + 1]; ip++) {
+    const int adj = B.face_block_conn[ip] / 6;
+    if (adj == block) continue;
+    const int ai = B.face_block_conn[ip] % 6;
+    const int fid = B.block_face_ids[6 * adj + ai];
+    P n{adj, 0, 0, 0, p.level};
+    if (ai < 2) {
+      n.x = (HMAX - 2 * h) * (ai % 2);
+      set_face_coords(fid, HMAX - 2 * h, u, v, n.y, n.z);
+    } else if (ai < 4) {
+      n.y = (HMAX - 2 * h) * (ai % 2);
+      set_face_coords(fid, HMAX - 2 * h, u, v, n.x, n.z);
+    } else {
+      n.z = (HMAX - 2 * h) * (ai % 2);
+      set_face_coords(fid, HMAX - 2 * h, u, v, n.x, n.y);
+    }
+    em(n);
+  }
+}
+
+__device__ void add_edge_neighbors(const DBlk &B, int edge_index, const P &p, Emit &em) {
+  const int block = p.block;
+  const int edge = B.block_edge_conn[12 * block + edge_index];
+  const int32_t h = hsize(p.level);
+  const int32_t ucoord = (edge_index < 4) ? p.x : (edge_index < 8) ? p.y : p.z;
+  const int n1 = B.block_conn[8 * block + cEdgeNodes[edge_index][0]];
+  const int n2 = B.block_conn[8 * block + cEdgeNodes[edge_index][1]];
+  for (int ip = B.edge_block_ptr[edge]; ip < B.edge_block_ptr[edge + 1]; ip++) {
+    const int adj = B.edge_block_conn[ip] / 12;
+    if (adj == block) continue;
+    const int ai = B.edge_block_conn[ip] % 12;
+    const int m1 = B.block_conn[8 * adj + cEdgeNodes[ai][0]];
+    const int m2 = B.block_conn[8 * adj + cEdgeNodes[ai][1]];
+    const int32_t u = (n1 == m2 && n2 == m1) ? HMAX - 2 * h - ucoord : ucoord;
+    P n{adj, 0, 0, 0, p.level};
+    if (ai < 4) {
+      n.x = u;
+      n.y = (HMAX - 2 * h) * (ai % 2);
+      n.z = (HMAX - 2 * h) * (ai / 2);
+    } else if (ai < 8) {
+      n.x = (HMAX - 2 * h) * (ai % 2);
+      n.y = u;
+      n.z = (HMAX - 2 * h) * ((ai - 4) / 2);
+    } else {
+      n.x = (HMAX - 2 * h) * (ai % 2);
+      n.y = (HMAX - 2 * h) * ((ai - 8) / 2);
+      n.z = u;
+    }
+    em(n);
+  }
+}
+
+__device__ void add_corner_neighbors(const DBlk &B, int corner, const P &p, Emit &em) {
+  const int block = p.block;
+  const int node = B.block_conn[8 * block + corner];
+  const int32_t h = hsize(p.level);
+  for (int ip = B.node_block_ptr[node]; ip < B.node_block_ptr[node + 1]; ip++) {
+    const int adj = B.node_block_conn[ip] / 8;
+    if (adj == block) continue;
+    const int ai = B.node_block_conn[ip] % 8;
+    em(P{adj, (HMAX - 2 * h) * (ai % 2), (HMAX - 2 * h) * ((ai % 4) / 2),
+         (HMAX - 2 * h) * (ai / 4), p.level});
+  }
+}
+
+__device__ void balance_octant(const DBlk &B, const P &oct, int balance_corner, Emit &em) {
+  if (oct.level <= 1) return;
+  const P p = parent_of(oct);
+  for (int f = 0; f < 6; f++) {
+    const P q = sibling0(face_neighbor(p, f));
+    if (in_bounds(q))
+      em(q);
+    else
+      add_face_neighbors(B, f, q, em);
+  }
+  for (int e = 0; e < 12; e++) {
+    const P q = sibling0(edge_neighbor(p, e));
+    if (in_bounds(q)) {
+      em(q);
+      continue;
+    }
+    const int ex = (q.x < 0 || q.x >= HMAX), ey = (q.y < 0 || q.y >= HMAX),
+              ez = (q.z < 0 || q.z >= HMAX);
+    if ((ex && ey) || (ex && ez) || (ey && ez)) {
+      add_edge_neighbors(B, e, q, em);
+    } else {
+      const int face = ex ? (q.x < 0 ? 0 : 1) : ey ? (q.y < 0 ? 2 : 3) : (q.z < 0 ? 4 : 5);
+      add_face_neighbors(B, face, q, em);
+    }
+  }
+  if (!balance_corner) return;
+  for (int c = 0; c < 8; c++) {
+    const P q = sibling0(corner_neighbor(p, c));
+    if (in_bounds(q)) {
+      em(q);
+      continue;
+    }
+    const int ex = (q.x < 0 || q.x >= HMAX), ey = (q.y < 0 || q.y >= HMAX),
+              ez = (q.z < 0 || q.z >= HMAX);
+    if (ex && ey && ez) {
+      add_corner_neighbors(B, c, q, em);
+    } else if ((ex && ey) || (ex && ez) || (ey && ez)) {
+      int edge;
+      if (ey && ez)
+        edge = (q.y < 0 ? 0 : 1) + (q.z < 0 ? 0 : 2);
+      else if (ex && ez)
+        edge = (q.x < 0 ? 4 : 5) + (q.z < 0 ? 0 : 2);
+      else
+        edge = (q.x < 0 ? 8 : 9) + (q.y < 0 ? 0 : 2);
+      add_edge_neighbors(B, edge, q, em);
+    } else {
+      const int face = ex ? (q.x < 0 ? 0 : 1) : ey ? (q.y < 0 ? 2 : 3) : (q.z < 0 ? 4 : 5);
+      add_face_neighbors(B, face, q, em);
+    }
+  }
+}
+
+// the 0-child of every octant (tag and info cleared)
+__global__ void k_sibling0(int n, const Oct *__restrict__ in, Oct *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const P s = sibling0(load_oct(in, i));
+  out[i] = Oct{s.block, s.x, s.y, s.z, 0, (int16_t)s.level, 0};
+}
+
+// balanceOctant's additions: counts (out null) or the octants at offs[i]
+__global__ void k_balance_emit(DBlk B, int n, const Oct *__restrict__ in, int corner,
+                               const int *__restrict__ offs, int *cnt, Oct *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Emit em{out, out ? offs[i] : 0, 0};
+  balance_octant(B, load_oct(in, i), corner, em);
+  if (!out) cnt[i] = em.k;
+}
+
+// the 8 siblings of each octant (of one level > 0)
+__global__ void k_siblings(int n, const Oct *__restrict__ in, Oct *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const P o = load_oct(in, i);
+  for (int j = 0; j < 8; j++) {
+    const P s = sibling_of(o, j);
+    out[8 * (size_t)i + j] = Oct{s.block, s.x, s.y, s.z, 0, (int16_t)s.level, 0};
+  }
+}
+
+__global__ void k_sort_keys(int n, const Oct *__restrict__ a, uint64_t *kh, uint64_t *kl,
+                            uint64_t *klev, uint32_t *idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Oct o = a[i];
+  const Key k = make_key(o.block, o.x, o.y, o.z);
+  kh[i] = k.hi;
+  kl[i] = k.lo;
+  if (klev) klev[i] = (uint64_t)o.level;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_gather_oct(int n, const uint32_t *__restrict__ idx, const Oct *__restrict__ a,
+                             Oct *b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[idx[i]];
+}
+
+// keep[i]: first (last) entry of each run of equal keys
+__global__ void k_run_flags(int n, const uint64_t *__restrict__ kh, const uint64_t *__restrict__ kl,
+                            int last, int *keep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = last ? i + 1 : i - 1;
+  const bool edge = last ? (j >= n) : (j < 0);
+  keep[i] = (edge || kh[j] != kh[i] || kl[j] != kl[i]) ? 1 : 0;
+}
+
+__global__ void k_compact_oct(int n, const int *__restrict__ keep, const int *__restrict__ pos,
+                              const Oct *__restrict__ a, Oct *b, int set_tag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  const int j = pos[i] - 1;
+  Oct o = a[i];
+  if (set_tag) {
+    o.tag = j;
+    o.info = 0;
+  }
+  b[j] = o;
+}
+
+__global__ void k_excl(int n, const int *__restrict__ c, const int *__restrict__ ci, int *co) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) co[i] = ci[i] - c[i];
+}
+
+__global__ void k_key_words(int n, const Oct *__restrict__ a, uint64_t *kh, uint64_t *kl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Key k = make_key(a[i].block, a[i].x, a[i].y, a[i].z);
+  kh[i] = k.hi;
+  kl[i] = k.lo;
+}
+
+// --------------------------------------------------------------------------
 // device buffers released on every exit
 struct Pool {
   std::vector<void *> p;
@@ -975,6 +1384,309 @@ int Forest::create_nodes_device(int device, double *ms) {
     X_.swap(X);
     nodes_ready_ = true;
   }
+  if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+namespace {
+
+// device scratch for sorting and scanning, released with the pool
+struct Dev {
+  Pool &pool;
+  std::string err;
+  // out = inclusive prefix sums of in[0..n)
+  bool scan(const int *in, int *out, int n) {
+    if (n <= 0) return true;
+    size_t tb = 0;
+    (void)hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out, n);
+    void *tmp = pool.get<unsigned char>(tb);
+    if (!tmp || hipcub::DeviceScan::InclusiveSum(tmp, tb, in, out, n) != hipSuccess) {
+      err = "scan failed";
+      return false;
+    }
+    return true;
+  }
+  int last(const int *a, int n) {
+    int v = 0;
+    if (n > 0) (void)hipMemcpy(&v, a + (n - 1), sizeof(int), hipMemcpyDeviceToHost);
+    return v;
+  }
+  // b = a sorted by position key (by level first, when by_level: stable
+  // passes level, low word, high word); kh/kl receive the sorted key words
+  bool sort(int n, const Oct *a, Oct *b, bool by_level, uint64_t **khs, uint64_t **kls) {
+    uint64_t *kh = pool.get<uint64_t>(n), *kl = pool.get<uint64_t>(n);
+    uint64_t *klev = by_level ? pool.get<uint64_t>(n) : nullptr;
+    uint64_t *g = pool.get<uint64_t>(n), *ks = pool.get<uint64_t>(n);
+    uint32_t *p1 = pool.get<uint32_t>(n), *p2 = pool.get<uint32_t>(n);
+    if (!kh || !kl || (by_level && !klev) || !g || !ks || !p1 || !p2) {
+      err = "device allocation failed";
+      return false;
+    }
+    size_t tb = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, g, ks, p1, p2, n);
+    void *tmp = pool.get<unsigned char>(tb);
+    if (!tmp) {
+      err = "device allocation failed";
+      return false;
+    }
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks(n)), dim3(256), 0, 0, n, a, kh, kl, klev, p1);
+    auto pass = [&](const uint64_t *key, int end_bit) {
+      hipLaunchKernelGGL(k_gather_u64, dim3(blocks(n)), dim3(256), 0, 0, (int64_t)n, p1, key, g);
+      size_t t = tb;
+      if (hipcub::DeviceRadixSort::SortPairs(tmp, t, g, ks, p1, p2, n, 0, end_bit) != hipSuccess)
+        return false;
+      std::swap(p1, p2);
+      return true;
+    };
+    if ((by_level && !pass(klev, 8)) || !pass(kl, 64) || !pass(kh, 64)) {
+      err = "radix sort failed";
+      return false;
+    }
+    hipLaunchKernelGGL(k_gather_oct, dim3(blocks(n)), dim3(256), 0, 0, n, p1, a, b);
+    hipLaunchKernelGGL(k_gather_u64, dim3(blocks(n)), dim3(256), 0, 0, (int64_t)n, p1, kl, g);
+    *khs = ks;  // the last pass's sorted high words
+    *kls = g;
+    return true;
+  }
+};
+
+}  // namespace
+
+int Forest::balance_device(int balance_corner, int device, double *ms) {
+  if (octs_.empty()) {
+    last_error = "balance: no octants have been created";
+    return -1;
+  }
+  free_mesh();
+  struct DeviceGuard {
+    int prev = 0;
+    DeviceGuard() { (void)hipGetDevice(&prev); }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+  } guard;
+  if (hipSetDevice(device) != hipSuccess) {
+    last_error = "balance_device: cannot select device " + std::to_string(device);
+    return -1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  Pool pool;
+  Dev dv{pool, ""};
+  auto fail = [&](const std::string &what) {
+    last_error = "balance_device: " + what;
+    return -1;
+  };
+  DBlk B;
+  B.block_conn = pool.up(bd_.block_conn);
+  B.block_edge_conn = pool.up(bd_.block_edge_conn);
+  B.block_face_conn = pool.up(bd_.block_face_conn);
+  B.block_face_ids = pool.up(bd_.block_face_ids);
+  B.node_block_ptr = pool.up(bd_.node_block_ptr);
+  B.node_block_conn = pool.up(bd_.node_block_conn);
+  B.edge_block_ptr = pool.up(bd_.edge_block_ptr);
+  B.edge_block_conn = pool.up(bd_.edge_block_conn);
+  B.face_block_ptr = pool.up(bd_.face_block_ptr);
+  B.face_block_conn = pool.up(bd_.face_block_conn);
+  B.face_owner = pool.up(bd_.face_block_owners);
+  B.edge_owner = pool.up(bd_.edge_block_owners);
+  B.node_owner = pool.up(bd_.node_block_owners);
+  const int n0 = (int)octs_.size();
+  Oct *d_in = pool.up(octs_);
+  Oct *s0 = pool.get<Oct>(n0), *s0l = pool.get<Oct>(n0);
+  if (!B.block_conn || !B.block_edge_conn || !B.block_face_conn || !B.block_face_ids ||
+      !B.node_block_ptr || !B.node_block_conn || !B.edge_block_ptr || !B.edge_block_conn ||
+      !B.face_block_ptr || !B.face_block_conn || !B.face_owner || !B.edge_owner ||
+      !B.node_owner || !d_in || !s0 || !s0l)
+    return fail("device allocation failed");
+  // the 0-children, grouped by level (a stable pass on the level alone)
+  hipLaunchKernelGGL(k_sibling0, dim3(blocks(n0)), dim3(256), 0, 0, n0, d_in, s0);
+  std::vector<int> cnt(MAX_LEVEL + 1, 0), off(MAX_LEVEL + 2, 0);
+  for (const Oct &o : octs_) cnt[o.level]++;
+  for (int L = 0; L <= MAX_LEVEL; L++) off[L + 1] = off[L] + cnt[L];
+  {
+    uint64_t *klev = pool.get<uint64_t>(n0), *ks = pool.get<uint64_t>(n0);
+    uint64_t *kh = pool.get<uint64_t>(n0), *kl = pool.get<uint64_t>(n0);
+    uint32_t *p1 = pool.get<uint32_t>(n0), *p2 = pool.get<uint32_t>(n0);
+    if (!klev || !ks || !kh || !kl || !p1 || !p2) return fail("device allocation failed");
+    size_t tb = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, klev, ks, p1, p2, n0, 0, 8);
+    void *tmp = pool.get<unsigned char>(tb);
+    if (!tmp) return fail("device allocation failed");
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks(n0)), dim3(256), 0, 0, n0, s0, kh, kl, klev, p1);
+    if (hipcub::DeviceRadixSort::SortPairs(tmp, tb, klev, ks, p1, p2, n0, 0, 8) != hipSuccess)
+      return fail("radix sort failed");
+    hipLaunchKernelGGL(k_gather_oct, dim3(blocks(n0)), dim3(256), 0, 0, n0, p2, s0, s0l);
+  }
+  std::vector<Oct *> uq(MAX_LEVEL + 1, nullptr);
+  std::vector<int> nu(MAX_LEVEL + 1, 0);
+  Oct *gen = nullptr;
+  int ngen = 0;
+  for (int L = MAX_LEVEL; L >= 0; L--) {
+    const int ncur = cnt[L] + ngen;
+    if (ncur == 0) continue;
+    Oct *cur = pool.get<Oct>(ncur), *srt = pool.get<Oct>(ncur);
+    int *keep = pool.get<int>(ncur), *pos = pool.get<int>(ncur);
+    if (!cur || !srt || !keep || !pos) return fail("device allocation failed");
+    if (cnt[L])
+      (void)hipMemcpy(cur, s0l + off[L], cnt[L] * sizeof(Oct), hipMemcpyDeviceToDevice);
+    if (ngen) (void)hipMemcpy(cur + cnt[L], gen, ngen * sizeof(Oct), hipMemcpyDeviceToDevice);
+    uint64_t *khs = nullptr, *kls = nullptr;
+    if (!dv.sort(ncur, cur, srt, false, &khs, &kls)) return fail(dv.err);
+    hipLaunchKernelGGL(k_run_flags, dim3(blocks(ncur)), dim3(256), 0, 0, ncur, khs, kls, 0, keep);
+    if (!dv.scan(keep, pos, ncur)) return fail(dv.err);
+    nu[L] = dv.last(pos, ncur);
+    uq[L] = pool.get<Oct>(nu[L]);
+    if (!uq[L]) return fail("device allocation failed");
+    hipLaunchKernelGGL(k_compact_oct, dim3(blocks(ncur)), dim3(256), 0, 0, ncur, keep, pos, srt,
+                       uq[L], 0);
+    ngen = 0;
+    gen = nullptr;
+    if (L > 1) {
+      int *c = pool.get<int>(nu[L]), *ci = pool.get<int>(nu[L]), *co = pool.get<int>(nu[L]);
+      if (!c || !ci || !co) return fail("device allocation failed");
+      hipLaunchKernelGGL(k_balance_emit, dim3(blocks(nu[L])), dim3(256), 0, 0, B, nu[L], uq[L],
+                         balance_corner, nullptr, c, nullptr);
+      if (!dv.scan(c, ci, nu[L])) return fail(dv.err);
+      ngen = dv.last(ci, nu[L]);
+      // exclusive offsets: inclusive sums minus the own count
+      hipLaunchKernelGGL(k_excl, dim3(blocks(nu[L])), dim3(256), 0, 0, nu[L], c, ci, co);
+      gen = pool.get<Oct>(ngen);
+      if (!gen) return fail("device allocation failed");
+      if (ngen)
+        hipLaunchKernelGGL(k_balance_emit, dim3(blocks(nu[L])), dim3(256), 0, 0, B, nu[L], uq[L],
+                           balance_corner, co, nullptr, gen);
+    }
+  }
+  // every sibling, then the finest octant per position
+  int nall = 0;
+  for (int L = 0; L <= MAX_LEVEL; L++) nall += (L > 0 ? 8 : 1) * nu[L];
+  Oct *all = pool.get<Oct>(nall), *srt = pool.get<Oct>(nall);
+  int *keep = pool.get<int>(nall), *pos = pool.get<int>(nall);
+  if (!all || !srt || !keep || !pos) return fail("device allocation failed");
+  {
+    size_t at = 0;
+    for (int L = 0; L <= MAX_LEVEL; L++) {
+      if (!nu[L]) continue;
+      if (L == 0) {
+        (void)hipMemcpy(all + at, uq[L], nu[L] * sizeof(Oct), hipMemcpyDeviceToDevice);
+        at += nu[L];
+      } else {
+        hipLaunchKernelGGL(k_siblings, dim3(blocks(nu[L])), dim3(256), 0, 0, nu[L], uq[L],
+                           all + at);
+        at += 8 * (size_t)nu[L];
+      }
+    }
+  }
+  uint64_t *khs = nullptr, *kls = nullptr;
+  if (!dv.sort(nall, all, srt, true, &khs, &kls)) return fail(dv.err);
+  hipLaunchKernelGGL(k_run_flags, dim3(blocks(nall)), dim3(256), 0, 0, nall, khs, kls, 1, keep);
+  if (!dv.scan(keep, pos, nall)) return fail(dv.err);
+  const int nr = dv.last(pos, nall);
+  Oct *res = pool.get<Oct>(nr);
+  uint64_t *rh = pool.get<uint64_t>(nr), *rl = pool.get<uint64_t>(nr);
+  if (!res || !rh || !rl) return fail("device allocation failed");
+  hipLaunchKernelGGL(k_compact_oct, dim3(blocks(nall)), dim3(256), 0, 0, nall, keep, pos, srt, res,
+                     1);
+  hipLaunchKernelGGL(k_key_words, dim3(blocks(nr)), dim3(256), 0, 0, nr, res, rh, rl);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return fail("kernel launch failed");
+  std::vector<Oct> out(nr);
+  std::vector<uint64_t> hh(nr), hl(nr);
+  if (hipMemcpy(out.data(), res, nr * sizeof(Oct), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hh.data(), rh, nr * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hl.data(), rl, nr * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail("copy failed");
+  octs_.swap(out);
+  okey_.resize(nr);
+  for (int i = 0; i < nr; i++) okey_[i] = ((u128)hh[i] << 64) | (u128)hl[i];
+  if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+int Forest::create_interpolation_device(Forest *coarse, InterpCSR &P, int device, double *ms) {
+  if (coarse == this) {
+    last_error = "create_interpolation: cannot interpolate between the same forest";
+    return -1;
+  }
+  if (coarse->interp_ != interp_) {
+    last_error = "create_interpolation: interpolation types differ";
+    return -1;
+  }
+  if (create_nodes_device(device) != 0) return -1;
+  if (coarse->create_nodes_device(device) != 0) {
+    last_error = "create_interpolation: coarse forest: " + coarse->last_error;
+    return -1;
+  }
+  struct DeviceGuard {
+    int prev = 0;
+    DeviceGuard() { (void)hipGetDevice(&prev); }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+  } guard;
+  if (hipSetDevice(device) != hipSuccess) {
+    last_error = "create_interpolation_device: cannot select device " + std::to_string(device);
+    return -1;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  auto fail = [&](const char *what) {
+    last_error = std::string("create_interpolation_device: ") + what;
+    return -1;
+  };
+  Pool pool;
+  const int n = num_owned_;
+  const int64_t m = 8 * (int64_t)octs_.size();
+  DInterp D;
+  D.nf = (int)octs_.size();
+  D.fo = pool.up(octs_);
+  D.fconn = pool.up(conn_);
+  D.nc = (int)coarse->octs_.size();
+  D.co = pool.up(coarse->octs_);
+  D.cconn = pool.up(coarse->conn_);
+  D.cptr = pool.up(coarse->dep_ptr_.empty() ? std::vector<int>{0} : coarse->dep_ptr_);
+  D.cdc = pool.up(coarse->dep_conn_);
+  D.cdw = pool.up(coarse->dep_w_);
+  int *first = pool.get<int>(n), *len = pool.get<int>(n + 1), *rowp = pool.get<int>(n + 1);
+  int *bad = pool.get<int>(1);
+  if (!D.fo || !D.fconn || !D.co || !D.cconn || !D.cptr || !D.cdc || !D.cdw || !first || !len ||
+      !rowp || !bad)
+    return fail("device allocation failed");
+  (void)hipMemset(first, 0x7f, (n ? n : 1) * sizeof(int));
+  (void)hipMemset(len, 0, (n + 1) * sizeof(int));
+  (void)hipMemset(bad, 0, sizeof(int));
+  hipLaunchKernelGGL(k_first_visit, dim3(blocks(m)), dim3(256), 0, 0, m, D.fconn, first);
+  // lengths in len[1..n], so that the inclusive scan is the row pointer
+  if (n > 0)
+    hipLaunchKernelGGL((k_interp_rows<false>), dim3(blocks(n)), dim3(256), 0, 0, D, n, first,
+                       len + 1, nullptr, nullptr, nullptr, bad);
+  size_t tb = 0;
+  (void)hipcub::DeviceScan::InclusiveSum(nullptr, tb, len, rowp, n + 1);
+  void *tmp = pool.get<unsigned char>(tb);
+  if (!tmp) return fail("device allocation failed");
+  if (hipcub::DeviceScan::InclusiveSum(tmp, tb, len, rowp, n + 1) != hipSuccess)
+    return fail("scan failed");
+  int hbad = 0, nnz = 0;
+  if (hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&nnz, rowp + n, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail("copy failed");
+  if (hbad) {
+    last_error = "create_interpolation: no enclosing coarse octant (forests do not nest)";
+    return -1;
+  }
+  int *cols = pool.get<int>(nnz);
+  double *w = pool.get<double>(nnz);
+  if (!cols || !w) return fail("device allocation failed");
+  if (n > 0)
+    hipLaunchKernelGGL((k_interp_rows<true>), dim3(blocks(n)), dim3(256), 0, 0, D, n, first,
+                       nullptr, rowp, cols, w, bad);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return fail("kernel launch failed");
+  P.nrows = n;
+  P.ncols = coarse->num_owned_;
+  P.rowp.resize(n + 1);
+  P.cols.resize(nnz);
+  P.w.resize(nnz);
+  if (hipMemcpy(P.rowp.data(), rowp, (n + 1) * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      (nnz > 0 &&
+       (hipMemcpy(P.cols.data(), cols, nnz * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(P.w.data(), w, nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)))
+    return fail("copy failed");
   if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }

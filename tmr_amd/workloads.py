@@ -16,17 +16,18 @@ from . import TMR
 FACES = {"x-": "fixed", "x+": "load"}
 
 
-def hierarchy(forest, nlevels):
+def hierarchy(forest, nlevels, device=None):
     """Finest-first list of forests: coarsen + balance(1) per level
-    (TopOptUtils.createTopoProblem, tmr/TopOptUtils.py:69-99)."""
-    forest.balance(1)
+    (TopOptUtils.createTopoProblem, tmr/TopOptUtils.py:69-99); with `device`
+    (a GPU index) the balance and node passes run on that GPU."""
+    forest.balance(1, device=device)
     forests = [forest]
     for _ in range(nlevels - 1):
         c = forests[-1].coarsen()
-        c.balance(1)
+        c.balance(1, device=device)
         forests.append(c)
     for f in forests:
-        f.createNodes()
+        f.createNodes(device=device)
     return forests
 
 
@@ -123,7 +124,7 @@ def _field_gradient(c, h, seed, L):
 
 
 def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), max_sweeps=None,
-             base_median=False, quantile=0.5, face_names=FACES):
+             base_median=False, quantile=0.5, face_names=FACES, device=None):
     """Adaptive forest on 2x2x2 blocks: octants where the seeded density has a
     gradient above its median are refined towards a seeded level in
     [base_level, max_level]; balance(1) after each sweep.  The median is the
@@ -146,20 +147,21 @@ def adaptive(seed=0, base_level=3, max_level=7, nlevels=5, L=(2.0, 1.0, 1.0), ma
         if not ref.any():
             break
         f.refine(ref, 0, max_level)
-        f.balance(1)
-    forests = hierarchy(f, nlevels)
+        f.balance(1, device=device)
+    forests = hierarchy(f, nlevels, device=device)
     cen, _, _ = _centres(forests[0], L)
     rho = gaussian_field(cen, seed, L)
     return forests, rho, L
 
 
-def c2_adaptive(seed=0, nlevels=5):
+def c2_adaptive(seed=0, nlevels=5, device=None):
     """The adaptive cantilever of about a million hex (1,227,458 at seed 0):
     2 x 2 x 2 blocks on a 2 x 1 x 1 box, base level 3, per-octant target
     levels drawn from the seed in 3..7, refined where the gradient of the
     32-Gaussian density exceeds its median, balance(1) after each sweep;
     `nlevels` multigrid forests."""
-    return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True)
+    return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True,
+                    device=device)
 
 
 ALL_FACES = {"x-": "fixed", "x+": "load", "y-": "y-", "y+": "y+", "z-": "z-", "z+": "z+"}
