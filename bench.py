@@ -131,7 +131,7 @@ def bench_multirhs(args):
     from tmr_amd import solver, workloads
     nrhs = args.nrhs
     t0 = time.perf_counter()
-    forests, rho, L = workloads.c5a_adaptive(seed=1, nlevels=5)
+    forests, rho, L = workloads.c5a_adaptive(seed=1, nlevels=5, device=0)
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
@@ -254,7 +254,7 @@ def bench_adaptive(args):
     import torch
     from tmr_amd import solver, workloads
     t0 = time.perf_counter()
-    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5)
+    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5, device=0)
     log("adaptive forest hierarchy %s built in %.1f s" % ([len(x.getOctants()) for x in forests],
                                                            time.perf_counter() - t0))
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
@@ -314,7 +314,7 @@ def bench_adjoint(args):
     adjoint solve; time and extra device memory of each backward."""
     import torch
     from tmr_amd import solver, workloads
-    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5)
+    forests, rho, L = workloads.c2_adaptive(seed=0, nlevels=5, device=0)
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
     prob.set_stream(torch.cuda.current_stream().cuda_stream)
     prob.set_design(rho)
@@ -363,6 +363,55 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_forest(args):
+    """Forest setup of the two benchmark meshes with the host passes and with
+    the GPU passes (balance, createNodes, createInterpolation): the 6-level
+    box hierarchy (4.2M hex at the finest level) and the 5-level adaptive
+    hierarchy (1.2M hex, its refinement sweeps included), each checked
+    bit-exact against the host build."""
+    import numpy as np
+
+    from tmr_amd import workloads
+
+    def build(kind, device):
+        t = time.perf_counter()
+        if kind == "box":
+            forests, _, _ = workloads.box(level=7, nlevels=6, seed=0, device=device)
+        else:
+            forests, _, _ = workloads.c2_adaptive(seed=0, nlevels=5, device=device)
+        t_forest = time.perf_counter() - t
+        t = time.perf_counter()
+        P = [forests[l].createInterpolation(forests[l + 1], device=device)
+             for l in range(len(forests) - 1)]
+        return forests, P, t_forest, time.perf_counter() - t
+
+    out = {}
+    for kind in ("box", "adaptive"):
+        fh, Ph, th, tih = build(kind, None)
+        fd, Pd, td, tid = build(kind, 0)
+        same = all(np.array_equal(a.getOctants(), b.getOctants()) and
+                   np.array_equal(a.getMeshConn(), b.getMeshConn()) and
+                   all(np.array_equal(x, y) for x, y in zip(a.getDepNodeConn(), b.getDepNodeConn()))
+                   and np.array_equal(a.getPoints(), b.getPoints()) for a, b in zip(fh, fd))
+        same = same and all(np.array_equal(x, y) for a, b in zip(Ph, Pd) for x, y in zip(a, b))
+        out[kind] = {"hex": len(fh[0].getOctants()), "levels": len(fh),
+                     "host_s": {"forests": th, "interpolation": tih, "total": th + tih},
+                     "gpu_s": {"forests": td, "interpolation": tid, "total": td + tid},
+                     "speedup": (th + tih) / (td + tid), "bit_exact": bool(same)}
+        log(kind, out[kind])
+    res = {"metric": "forest hierarchy setups/s (balance, nodes, interpolation on the GPU), "
+                     "4.2M-hex box", "value": 1.0 / out["box"]["gpu_s"]["total"],
+           "unit": "setups/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": 1e3 * out["box"]["gpu_s"]["total"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32 octants, f64 weights",
+           "data": "synthetic",
+           "config": {"workload": "6-level box hierarchy (256x128x128 hex8 finest) and the "
+                                  "5-level adaptive hierarchy (1,227,458 hex), host vs GPU "
+                                  "forest passes"},
+           "meshes": out}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,10 +431,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default owner-tile size")
     ap.add_argument("--workload", default="cantilever",
-                    choices=["cantilever", "multirhs", "batch", "adaptive", "adjoint"],
+                    choices=["cantilever", "multirhs", "batch", "adaptive", "adjoint", "forest"],
                     help="cantilever: the 4.2M-hex headline; multirhs: 16 load cases on a "
                          "2.1M-hex cube; batch: 96 uneven independent cantilevers; adaptive: "
-                         "the 1.2M-hex adaptive forest; adjoint: its gradients")
+                         "the 1.2M-hex adaptive forest; adjoint: its gradients; forest: host vs "
+                         "GPU forest setup")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device (a known-size "
@@ -407,12 +457,14 @@ def main():
         return bench_adaptive(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "forest":
+        return bench_forest(args)
 
     import torch
     from tmr_amd import solver, workloads
 
     t0 = time.perf_counter()
-    forests, rho, L = workloads.box(level=args.level, nlevels=args.nlevels, seed=0)
+    forests, rho, L = workloads.box(level=args.level, nlevels=args.nlevels, seed=0, device=0)
     log("forest hierarchy built in %.1f s" % (time.perf_counter() - t0))
     t0 = time.perf_counter()
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])

@@ -467,14 +467,19 @@ int Forest::create_trees(int level) {
   free_mesh();
   const int32_t h = 1 << (MAX_LEVEL - level);
   const int64_t n = int64_t(1) << level;
-  octs_.clear();
-  octs_.reserve((size_t)(n * n * n * bd_.num_blocks));
+  // Morton index m of a block's octant: bit 3k + 2 of m is bit k of x / h,
+  // bit 3k + 1 of y, bit 3k of z (the key's x > y > z priority), so the
+  // octants come out already in key order
+  const int64_t nb = n * n * n;
+  octs_.resize((size_t)(nb * bd_.num_blocks));
   for (int b = 0; b < bd_.num_blocks; b++)
-    for (int32_t x = 0; x < HMAX; x += h)
-      for (int32_t y = 0; y < HMAX; y += h)
-        for (int32_t z = 0; z < HMAX; z += h) octs_.push_back({b, x, y, z, 0, (int16_t)level, 0});
-  sort_unique_finest(octs_);
-  for (size_t i = 0; i < octs_.size(); i++) octs_[i].tag = (int32_t)i;
+    for (int64_t m = 0; m < nb; m++) {
+      int32_t c[3] = {0, 0, 0};
+      for (int k = 0; k < level; k++)
+        for (int a = 0; a < 3; a++) c[a] |= (int32_t)((m >> (3 * k + 2 - a)) & 1) << k;
+      const size_t i = (size_t)(b * nb + m);
+      octs_[i] = {b, c[0] * h, c[1] * h, c[2] * h, (int32_t)i, (int16_t)level, 0};
+    }
   reindex(octs_, okey_);
   return 0;
 }

@@ -121,7 +121,11 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
       last_error = "problem: every block must be an axis-aligned box with points set";
       return -1;
     }
-    if (f->create_nodes()) {
+    // nodes and (below) interpolations on the problem's GPU: the same
+    // results as the host passes, bit for bit
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (f->create_nodes_device(dev)) {
       last_error = "problem: level " + std::to_string(l) + ": " + f->last_error;
       return -1;
     }
@@ -821,7 +825,9 @@ int Problem::build_transfer(int l) {
   Level &F = levels_[l];
   Level &Cc = levels_[l + 1];
   InterpCSR P;
-  if (F.forest->create_interpolation(Cc.forest, P)) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (F.forest->create_interpolation_device(Cc.forest, P, dev)) {
     last_error = "problem: interpolation " + std::to_string(l) + ": " + F.forest->last_error;
     return -1;
   }

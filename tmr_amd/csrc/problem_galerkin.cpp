@@ -87,7 +87,9 @@ int Problem::build_galerkin() {
   for (int l = 0; l + 1 < nl; l++) {
     Level &F = levels_[l], &Cc = levels_[l + 1];
     InterpCSR P;
-    if (F.forest->create_interpolation(Cc.forest, P)) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (F.forest->create_interpolation_device(Cc.forest, P, dev)) {
       last_error = "galerkin: interpolation " + std::to_string(l) + ": " + F.forest->last_error;
       return -1;
     }

@@ -82,11 +82,11 @@ def cube_l4(nlevels=3):
     return forests, rho, (1.0, 1.0, 1.0)
 
 
-def box(level=7, nlevels=6, seed=0, passes=3):
+def box(level=7, nlevels=6, seed=0, passes=3, device=None):
     L = (2.0, 1.0, 1.0)
     f = TMR.createBoxForest(2, 1, 1, L, face_names=FACES)
     f.createTrees(level)
-    forests = hierarchy(f, nlevels)
+    forests = hierarchy(f, nlevels, device=device)
     n = 1 << level
     grid = smoothed_density((n, n, 2 * n), seed, passes)
     rho = element_density_from_grid(forests[0], grid, L)
@@ -167,12 +167,12 @@ def c2_adaptive(seed=0, nlevels=5, device=None):
 ALL_FACES = {"x-": "fixed", "x+": "load", "y-": "y-", "y+": "y+", "z-": "z-", "z+": "z+"}
 
 
-def c5a_adaptive(seed=1, nlevels=5):
+def c5a_adaptive(seed=1, nlevels=5, device=None):
     """The multi-load-case forest: the C2 generator at the 0.1 quantile of the
     field gradient (2,160,152 hex at seed 1), every outer face named
     (x- clamped)."""
     return adaptive(seed=seed, base_level=3, max_level=7, nlevels=nlevels, base_median=True,
-                    quantile=0.1, face_names=ALL_FACES)
+                    quantile=0.1, face_names=ALL_FACES, device=device)
 
 
 def face_load(forest, faces, tr):
