@@ -361,10 +361,11 @@ int tmrx_problem_autotune(tmrx_problem *p, const int *tile_sizes, int n, int rep
 int tmrx_problem_component_info(tmrx_problem *p, int rhs, int comp, int *iters, int *converged,
                                 double *rel_residual);
 /* The Galerkin operator P^T A P of multigrid level `level` (1 <= level <
-   nlevels - 1) after a setup with use_galerkin: the matrix TACSMg would form
+   nlevels) after a setup with use_galerkin: the matrix TACSMg would form
    there (src/TMR_RefinementTools.cpp:76-77), as 3 x 3 blocks over the
    independent nodes of that level, rows and columns of clamped dofs
-   identity.  n receives the block rows, nnzb the blocks; ptr[n + 1],
+   identity.  The coarsest level (level = nlevels - 1) is available in
+   either mode: the matrix its direct solve factors (:83-86).  n receives the block rows, nnzb the blocks; ptr[n + 1],
    col[nnzb] and val[9 nnzb] (row-major blocks) are filled when not NULL
    (call with NULL arrays to size). */
 int tmrx_problem_level_matrix(tmrx_problem *p, int level, int *n, int *nnzb, int *ptr, int *col,

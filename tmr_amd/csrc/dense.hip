@@ -7,7 +7,7 @@
 // columns hold the identity), processed in DB x DB blocks:
 //   Cholesky (right-looking): factor block (k,k); solve the panel below it;
 //     update the trailing lower blocks;
-//   L^-1 = X (block forward substitution): X[i][i] = L[i][i]^-1,
+//   L^-1 = X: X[i][i] = L[i][i]^-1 and, right-looking over k,
 //     X[i][j] = -X[i][i] sum_{k=j}^{i-1} L[i][k] X[k][j];
 //   A^-1 = X^T X, lower blocks then mirrored, written into the m x m result.
 #include <hip/hip_runtime.h>
@@ -113,10 +113,12 @@ __global__ void __launch_bounds__(DT) k_syrk_trailing(double *__restrict__ W, in
     for (int v = 0; v < 4; v++) W[(size_t)(bi + r0 + u) * mp + bj + c0 + v] -= acc[u][v];
 }
 
-// X[i][i] = L[i][i]^-1 (lower): column c solves L x = e_c
+// X[i][i] = L[i][i]^-1 (lower) for every diagonal block i = blockIdx.x:
+// column c solves L x = e_c
 __global__ void __launch_bounds__(DB) k_trtri_diag(const double *__restrict__ W,
-                                                   double *__restrict__ X, int mp, int i) {
+                                                   double *__restrict__ X, int mp) {
   __shared__ double L[DB][DB + 1];
+  const int i = blockIdx.x;
   const int tid = threadIdx.x;
   const size_t o = (size_t)(i * DB) * mp + i * DB;
   for (int t = tid; t < DB * DB; t += DB) L[t / DB][t % DB] = W[o + (size_t)(t / DB) * mp + t % DB];
@@ -134,35 +136,58 @@ __global__ void __launch_bounds__(DB) k_trtri_diag(const double *__restrict__ W,
   for (int r = 0; r < DB; r++) X[o + (size_t)r * mp + c] = x[r];
 }
 
-// X[i][j] = -X[i][i] sum_{k=j}^{i-1} L[i][k] X[k][j], one workgroup per j < i
-__global__ void __launch_bounds__(DT) k_trtri_row(const double *__restrict__ W,
-                                                  double *__restrict__ X, int mp, int i) {
+// zero the strictly lower blocks of X (the accumulators of the right-looking
+// inverse below)
+__global__ void k_trtri_zero(int mp, double *__restrict__ X) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)mp * mp) return;
+  const int r = (int)(t / mp), c = (int)(t % mp);
+  if (r / DB > c / DB) X[t] = 0.0;
+}
+
+// Right-looking L^-1, step k: every block (i, j), i > k >= j, takes
+// X[i][j] -= L[i][k] X[k][j]; row k + 1 has then received all its terms and
+// is finished in the same pass, X[k+1][j] = X[k+1][k+1] X[k+1][j].  After
+// step k rows 0..k+1 are final, so each step is one launch of
+// (nblk - 1 - k) (k + 1) independent block products.
+__global__ void __launch_bounds__(DT) k_trtri_step(const double *__restrict__ W,
+                                                   double *__restrict__ X, int mp, int k) {
   __shared__ double A[DB][DB + 1], B[DB][DB + 1];
-  const int j = blockIdx.x;
+  const int t = blockIdx.x;
+  const int i = k + 1 + t / (k + 1), j = t % (k + 1);
   const int tid = threadIdx.x;
   const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
-  double acc[4][4] = {};
-  for (int k = j; k < i; k++) {
-    __syncthreads();
-    for (int q = tid; q < DB * DB; q += DT) {
-      A[q / DB][q % DB] = W[(size_t)(i * DB + q / DB) * mp + k * DB + q % DB];
-      B[q / DB][q % DB] = X[(size_t)(k * DB + q / DB) * mp + j * DB + q % DB];
-    }
-    __syncthreads();
-    for (int kk = 0; kk < DB; kk++) {
-      double a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        a[u] = A[r0 + u][kk];
-        b[u] = B[kk][c0 + u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
-    }
+  for (int q = tid; q < DB * DB; q += DT) {
+    A[q / DB][q % DB] = W[(size_t)(i * DB + q / DB) * mp + k * DB + q % DB];
+    B[q / DB][q % DB] = X[(size_t)(k * DB + q / DB) * mp + j * DB + q % DB];
   }
-  // T -> LDS, then X[i][j] = -X[i][i] T
+  __syncthreads();
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      acc[u][v] = X[(size_t)(i * DB + r0 + u) * mp + j * DB + c0 + v];
+  for (int kk = 0; kk < DB; kk++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a[u] = A[r0 + u][kk];
+      b[u] = B[kk][c0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] -= a[u] * b[v];
+  }
+  if (i != k + 1) {
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) X[(size_t)(i * DB + r0 + u) * mp + j * DB + c0 + v] = acc[u][v];
+    return;
+  }
+  // the last term of row k + 1: X[k+1][j] = X[k+1][k+1] acc
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < 4; u++)
@@ -182,7 +207,7 @@ __global__ void __launch_bounds__(DT) k_trtri_row(const double *__restrict__ W,
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) out[u][v] -= a[u] * b[v];
+      for (int v = 0; v < 4; v++) out[u][v] += a[u] * b[v];
   }
 #pragma unroll
   for (int u = 0; u < 4; u++)
@@ -235,6 +260,26 @@ __global__ void __launch_bounds__(DT) k_gram(const double *__restrict__ X, int m
     }
 }
 
+// zero the dense matrix, identity on the padding rows (r >= m)
+__global__ void k_dense_fill(int64_t total, int mp, int m, double *__restrict__ W) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int r = (int)(t / mp), c = (int)(t % mp);
+  W[t] = (r >= m && r == c) ? 1.0 : 0.0;
+}
+
+// one thread per block of the component's block rows
+__global__ void k_bsr_scatter(const int *__restrict__ col, const double *__restrict__ val, int n0,
+                              int k0, int k1, const int *__restrict__ rowof, int mp,
+                              double *__restrict__ W) {
+  const int k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= k1) return;
+  const int i = rowof[k] - n0, j = col[k] - n0;
+  const double *v = val + 9 * (size_t)k;
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) W[(size_t)(3 * i + p) * mp + 3 * j + q] = v[3 * p + q];
+}
+
 int dense_padded(int m) { return (m + DB - 1) / DB * DB; }
 
 void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st) {
@@ -249,11 +294,25 @@ void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hi
       hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(DT), 0, st, W, mp, k);
     }
   }
-  for (int i = 0; i < nblk; i++) {
-    hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(DB), 0, st, W, X, mp, i);
-    if (i > 0) hipLaunchKernelGGL(k_trtri_row, dim3(i), dim3(DT), 0, st, W, X, mp, i);
-  }
+  // L^-1: the diagonal blocks at once, then nblk - 1 right-looking steps
+  const int64_t tot = (int64_t)mp * mp;
+  hipLaunchKernelGGL(k_trtri_zero, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, mp, X);
+  hipLaunchKernelGGL(k_trtri_diag, dim3(nblk), dim3(DB), 0, st, W, X, mp);
+  for (int k = 0; k + 1 < nblk; k++)
+    hipLaunchKernelGGL(k_trtri_step, dim3((nblk - 1 - k) * (k + 1)), dim3(DT), 0, st, W, X, mp, k);
   hipLaunchKernelGGL(k_gram, dim3(nblk * (nblk + 1) / 2), dim3(DT), 0, st, X, mp, nblk, m, out);
+  TMRX_LAUNCH_CHECK();
+}
+
+void launch_bsr_to_dense(const int *brow, const int *col, const double *val, int n0, int n1,
+                         int k0, int k1, int mp, double *W, hipStream_t st) {
+  const int m = 3 * (n1 - n0);
+  const int64_t total = (int64_t)mp * mp;
+  hipLaunchKernelGGL(k_dense_fill, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total,
+                     mp, m, W);
+  if (k1 > k0)
+    hipLaunchKernelGGL(k_bsr_scatter, dim3((k1 - k0 + 255) / 256), dim3(256), 0, st, col, val, n0,
+                       k0, k1, brow, mp, W);
   TMRX_LAUNCH_CHECK();
 }
 

@@ -109,6 +109,33 @@ void launch_galerkin_elem(int mode, int nE, const int *ptr, const int *child, co
   TMRX_LAUNCH_CHECK();
 }
 
+// element matrices from scales, one thread per (element, entry)
+__global__ void k_elem_matrices(int ne, const int *__restrict__ cls, const double *__restrict__ s,
+                                const double *__restrict__ Ktab, int nodal,
+                                double *__restrict__ KE) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 576 * (int64_t)ne) return;
+  const int e = (int)(t / 576), i = (int)(t % 576);
+  double v;
+  if (nodal) {
+    const double *Kg = Ktab + 576 * 8 * (size_t)cls[e] + i;
+    v = 0.0;
+    for (int g = 0; g < 8; g++) v += s[8 * (size_t)e + g] * Kg[576 * g];
+  } else {
+    v = s[e] * Ktab[576 * (size_t)cls[e] + i];
+  }
+  KE[t] = v;
+}
+
+void launch_elem_matrices(int ne, const int *cls, const double *s, const double *Ktab, bool nodal,
+                          double *KE, hipStream_t st) {
+  if (ne == 0) return;
+  const int64_t n = 576 * (int64_t)ne;
+  hipLaunchKernelGGL(k_elem_matrices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ne, cls,
+                     s, Ktab, nodal ? 1 : 0, KE);
+  TMRX_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // BSR values: block k (row node i = brow[k], column node j = col[k]) is the
 // sum over its terms of w * K_E[(3a+p)][(3b+q)], term code = 64 E + 8 a + b;

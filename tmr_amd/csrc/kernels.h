@@ -234,6 +234,11 @@ int dense_padded(int m);
 // L^-1; out (m x m, row-major) receives A^-1; *bad |= 1 on a non-positive
 // pivot
 void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st);
+// W (mp x mp, row-major) = the BSR block rows [n0, n1) of a component
+// (blocks [k0, k1), block rows brow) as a dense matrix of its 3 (n1 - n0)
+// dofs, identity on the padding rows
+void launch_bsr_to_dense(const int *brow, const int *col, const double *val, int n0, int n1,
+                         int k0, int k1, int mp, double *W, hipStream_t st);
 
 // Galerkin coarse operator of one level: 3 x 3 blocks over the independent
 // nodes (block row i = node i), row-major blocks, clamped rows and columns
@@ -261,6 +266,11 @@ void launch_galerkin_elem(int mode, int nE, const int *ptr, const int *child, co
 // BSR values from the element matrices: block k collects the terms
 // [tptr[k], tptr[k+1]) (code 64 E + 8 a + b, weight w: w K_E[a][b]); brow[k]
 // is its row
+// element matrices of one level from its scales: KE[e] = s_e K0[cls_e]
+// (element design, Ktab 576 per class) or sum_g s_eg K_g[cls_e] (nodal
+// design, Ktab 8 x 576 per class)
+void launch_elem_matrices(int ne, const int *cls, const double *s, const double *Ktab, bool nodal,
+                          double *KE, hipStream_t st);
 void launch_bsr_assemble(const BSR64 &A, const int *tptr, const int *tcode, const double *tw,
                          const int *brow, const double *KE, double *val, hipStream_t st);
 // dinv = 1 / diagonal of the blocks at diag[i]

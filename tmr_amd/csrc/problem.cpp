@@ -1275,30 +1275,28 @@ int Problem::find_components() {
 }
 
 int Problem::coarse_factor() {
-  // dense K on the coarsest level (identity rows for clamped dofs), one
-  // block per independent component, Cholesky in FP64 and the explicit
+  // K of the coarsest level (identity rows and columns on clamped dofs) as
+  // a BSR matrix assembled on the device from its element matrices (K_E of
+  // the Galerkin product, or the scaled element stiffness), then per
+  // independent component a dense block, its FP64 Cholesky and the explicit
   // inverse for a one-GEMV coarse solve
-  Level &L = levels_.back();
+  const int last = (int)levels_.size() - 1;
+  Level &L = levels_[last];
   const int n = (int)L.ndof;
   const bool nodal = design_mode_ == 1;
-  // Galerkin: the coarsest level's element matrices K_E from the device
   const bool gal = galerkin_ && levels_.size() > 1;
-  std::vector<double> s(nodal ? 8 * (size_t)L.n_elem : L.n_elem), KEh;
-  HIPCHK(hipMemcpy(s.data(), nodal ? L.sg64.p : L.s64.p, s.size() * sizeof(double),
-                   hipMemcpyDeviceToHost));
-  if (gal) {
-    KEh.resize(576 * (size_t)L.n_elem);
-    HIPCHK(hipStreamSynchronize(st_));
-    HIPCHK(hipMemcpy(KEh.data(), L.KE.p, KEh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (!L.has_bsr && build_bsr(last)) return -1;
+  if (!gal) {
+    if ((L.KE.n < 576 * (size_t)L.n_elem && L.KE.alloc(576 * (size_t)L.n_elem)) ||
+        (nodal && Kgd_.n == 0 && Kgd_.upload(Kgh_))) {
+      last_error = "coarse: device allocation failed";
+      return -1;
+    }
+    launch_elem_matrices(L.n_elem, L.d_cls.p, nodal ? L.sg64.p : L.s64.p,
+                         nodal ? Kgd_.p : K0d_.p, nodal, L.KE.p, st_);
   }
-  std::vector<double> Ke(576);
-  const auto &dptr = L.forest->dep_ptr();
-  const auto &dconn = L.forest->dep_conn();
-  const auto &dw = L.forest->dep_weights();
-  const auto &octs = L.forest->octants();
-  std::vector<int> bseg(seg_block0_.back());
-  for (int sg = 0; sg < nseg_; sg++)
-    for (int b = seg_block0_[sg]; b < seg_block0_[sg + 1]; b++) bseg[b] = sg;
+  launch_bsr_assemble(bsr<double>(last), L.bt_ptr.p, L.bt_code.p, L.bt_w.p, L.bsr_row.p, L.KE.p,
+                      L.bsr64.p, st_);
   std::vector<int> col0(n), nb(n);
   std::vector<int64_t> aoff(n);
   // component blocks of the inverse, one after the other
@@ -1316,59 +1314,12 @@ int Problem::coarse_factor() {
     return -1;
   }
   HIPCHK(hipMemsetAsync(chk_.p, 0, sizeof(int), st_));
-  std::vector<double> A;
   for (int sg = 0; sg < nseg_; sg++) {
-    const int d0 = 3 * L.seg_node0[sg], m = 3 * (L.seg_node0[sg + 1] - L.seg_node0[sg]);
+    const int n0 = L.seg_node0[sg], n1 = L.seg_node0[sg + 1];
+    const int d0 = 3 * n0, m = 3 * (n1 - n0);
     const int mp = dense_padded(m);
-    // dense K of the component, row stride mp, identity on clamped dofs and
-    // on the padding
-    HIPCHK(hipStreamSynchronize(st_));  // the previous component's upload buffer is free
-    A.assign((size_t)mp * mp, 0.0);
-    for (int e = 0; e < L.n_elem; e++) {
-      if (bseg[octs[e].block] != sg) continue;
-      const double *K = &K0h_[576 * L.cls[e]];
-      double se = s[e];
-      if (nodal) {  // sum_g s_eg K_g, the element scale folded in
-        const double *Kg = &Kgh_[576 * 8 * (size_t)L.cls[e]];
-        for (int i = 0; i < 576; i++) {
-          double t = 0.0;
-          for (int g = 0; g < 8; g++) t += s[8 * (size_t)e + g] * Kg[576 * g + i];
-          Ke[i] = t;
-        }
-        K = Ke.data();
-        se = 1.0;
-      }
-      if (gal) {
-        K = &KEh[576 * (size_t)e];
-        se = 1.0;
-      }
-      std::vector<std::pair<int, double>> ex[8];
-      for (int a = 0; a < 8; a++) {
-        const int g = L.conn[8 * e + a];
-        if (g >= 0)
-          ex[a].push_back({g, 1.0});
-        else
-          for (int j = dptr[-g - 1]; j < dptr[-g]; j++) ex[a].push_back({dconn[j], dw[j]});
-      }
-      for (int a = 0; a < 8; a++)
-        for (auto &pa : ex[a])
-          for (int bb = 0; bb < 8; bb++)
-            for (auto &pb : ex[bb]) {
-              const double w = se * pa.second * pb.second;
-              for (int i = 0; i < 3; i++) {
-                if ((L.bcmask[pa.first] >> i) & 1) continue;
-                for (int j = 0; j < 3; j++) {
-                  if ((L.bcmask[pb.first] >> j) & 1) continue;
-                  A[(size_t)(3 * pa.first + i - d0) * mp + 3 * pb.first + j - d0] +=
-                      w * K[(3 * a + i) * 24 + 3 * bb + j];
-                }
-              }
-            }
-    }
-    for (int i = 0; i < mp; i++)
-      if (i >= m || ((L.bcmask[(d0 + i) / 3] >> ((d0 + i) % 3)) & 1)) A[(size_t)i * mp + i] = 1.0;
-    // Cholesky and explicit inverse on the GPU (FP64)
-    HIPCHK(hipMemcpyAsync(cw_.p, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+    launch_bsr_to_dense(L.bsr_row.p, L.bsr_col.p, L.bsr64.p, n0, n1, L.bsr_ptr_h[n0],
+                        L.bsr_ptr_h[n1], mp, cw_.p, st_);
     launch_dense_inverse(cw_.p, cx_.p, m, Ainv_.p + base[sg], chk_.p, st_);
     for (int i = 0; i < m; i++) {
       col0[d0 + i] = d0;

@@ -163,6 +163,7 @@ struct Level {
   DevArray<int> bsr_ptr, bsr_col, bsr_row, bsr_diag, bt_ptr, bt_code;
   DevArray<double> bt_w, bsr64;
   DevArray<float> bsr32;
+  std::vector<int> bsr_ptr_h;  // host copy of bsr_ptr
   int nnzb = 0;
   bool has_bsr = false;
   // diagonal terms of corner pairs that share an independent node

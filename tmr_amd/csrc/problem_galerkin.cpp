@@ -219,6 +219,7 @@ int Problem::build_bsr(int l) {
     }
   }
   L.nnzb = (int)col.size();
+  L.bsr_ptr_h = ptr;
   if (L.bsr_ptr.upload(ptr) || L.bsr_col.upload(col) || L.bsr_row.upload(brow) ||
       L.bsr_diag.upload(diag) || L.bt_ptr.upload(tptr) || L.bt_code.upload(tcode) ||
       L.bt_w.upload(tw) || L.bsr64.alloc(9 * (size_t)L.nnzb) || L.bsr32.alloc(9 * (size_t)L.nnzb)) {
@@ -265,8 +266,11 @@ int Problem::galerkin_setup(int precision) {
 
 int Problem::level_matrix(int l, std::vector<int> &ptr, std::vector<int> &col,
                           std::vector<double> &val) {
-  if (!galerkin_ || l < 1 || l >= (int)levels_.size() || !levels_[l].has_bsr) {
-    last_error = "level_matrix: no Galerkin operator on that level";
+  const int nl = (int)levels_.size();
+  // smoothed Galerkin levels, and the coarsest level (factored from its BSR
+  // matrix in either mode)
+  if (l < 1 || l >= nl || !levels_[l].has_bsr || (!galerkin_ && l != nl - 1)) {
+    last_error = "level_matrix: no assembled operator on that level";
     return -1;
   }
   if (!have_setup_) {
