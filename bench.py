@@ -31,7 +31,6 @@ CHEB_BYTES_PER_NODE = (49 + 61 + 61) / 3.0
 CHEB_BYTES_PER_ELEM = 4 + 8 * 4
 FLOPS_PER_ELEM = 1152.0      # 2 * 24 * 24
 TIME_CHEB = 1 << 2           # launch-kind mask: Chebyshev update
-TILE_CANDIDATES = (256, 384, 512, 768, 1024)  # owner-tile sizes the autotuner tries
 # HBM traffic of the finest-level Chebyshev launch on the default workload
 # (level 7), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per
 # dispatch, mean over the launches at that level), keyed by (FP32 form, owner
@@ -447,10 +446,11 @@ def main():
                     help="after the solves, copy MIB MiB device to device (a known-size "
                          "reference for memory-traffic counters)")
     ap.add_argument("--count", type=int, default=96)
-    ap.add_argument("--tiles", default=",".join(map(str, TILE_CANDIDATES)),
-                    help="owner-tile sizes (nodes) the autotuner tries, comma-separated")
+    ap.add_argument("--tiles", default="auto",
+                    help="owner-tile sizes (nodes) the autotuner tries, comma-separated "
+                         "(auto: Problem.tile_candidates())")
     args = ap.parse_args()
-    args.tiles = tuple(int(v) for v in args.tiles.split(","))
+    args.tiles = None if args.tiles == "auto" else tuple(int(v) for v in args.tiles.split(","))
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
@@ -486,7 +486,7 @@ def main():
     if not args.no_autotune:
         t0 = time.perf_counter()
         tune = prob.autotune(args.tiles, reps=3)
-        log("autotuned owner tiles: %d nodes (ms per 4 FP32 + 1 FP64 products: %s) in %.1f s"
+        log("autotuned owner tiles: %d nodes (ms per 3 FP32 Chebyshev steps + FP32 and FP64 residuals: %s) in %.1f s"
             % (tune[0], tune[1], time.perf_counter() - t0))
 
     def run(precision, steps, warmup, timing=False):

@@ -219,6 +219,7 @@ void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x
 struct CoarseBlocks {
   int n = 0;
   const double *Ainv = nullptr;
+  const float *Ainv32 = nullptr;  // the same in FP32 (FP32 V-cycles read it when set)
   const int *col0 = nullptr;
   const int *nb = nullptr;
   const int64_t *aoff = nullptr;

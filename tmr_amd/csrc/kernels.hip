@@ -537,8 +537,8 @@ void launch_cheb_first(int ndof, const T *b, const T *dinv, double c, T *d, T *x
 // symmetric block per independent component; one wave per row).  Row i of
 // the coarse vector belongs to the block whose columns start at col0[i]
 // (nb[i] of them) and whose row i is stored at Ainv + aoff[i].
-template <class T>
-__global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__restrict__ Ainv,
+template <class T, class TA>
+__global__ __launch_bounds__(256) void k_dense_solve(int n, const TA *__restrict__ Ainv,
                                                      const int *__restrict__ col0,
                                                      const int *__restrict__ nb,
                                                      const int64_t *__restrict__ aoff,
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__rest
   b += (int64_t)blockIdx.y * n;
   x += (int64_t)blockIdx.y * n;
   double s = 0.0;
-  const double *a = Ainv + aoff[row];
+  const TA *a = Ainv + aoff[row];
   const T *bb = b + col0[row];
   const int m = nb[row];
   // eight entries per lane in flight, added in the order of a one-at-a-time
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__rest
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       const int j = min(j0 + 64 * k, m - 1);
-      av[k] = a[j];
+      av[k] = (double)a[j];
       bv[k] = (double)bb[j];
     }
 #pragma unroll
@@ -575,8 +575,12 @@ __global__ __launch_bounds__(256) void k_dense_solve(int n, const double *__rest
 template <class T>
 void launch_dense_solve(const CoarseBlocks &cb, const T *b, T *x, hipStream_t st, int nrhs) {
   if (cb.n == 0) return;
-  hipLaunchKernelGGL(k_dense_solve<T>, dim3((cb.n + 3) / 4, nrhs), dim3(256), 0, st, cb.n,
-                     cb.Ainv, cb.col0, cb.nb, cb.aoff, b, x, cb.run);
+  if (sizeof(T) == 4 && cb.Ainv32)
+    hipLaunchKernelGGL((k_dense_solve<T, float>), dim3((cb.n + 3) / 4, nrhs), dim3(256), 0, st,
+                       cb.n, cb.Ainv32, cb.col0, cb.nb, cb.aoff, b, x, cb.run);
+  else
+    hipLaunchKernelGGL((k_dense_solve<T, double>), dim3((cb.n + 3) / 4, nrhs), dim3(256), 0, st,
+                       cb.n, cb.Ainv, cb.col0, cb.nb, cb.aoff, b, x, cb.run);
   TMRX_LAUNCH_CHECK();
 }
 

@@ -1341,6 +1341,16 @@ int Problem::coarse_factor() {
   }
   coarse_.n = n;
   coarse_.Ainv = Ainv_.p;
+  // FP32 V-cycles read an FP32 copy (half the bytes of every coarse solve)
+  coarse_.Ainv32 = nullptr;
+  if (setup_prec_pending_ == 1) {
+    if (Ainv32_.n < base[nseg_] && Ainv32_.alloc(base[nseg_])) {
+      last_error = "coarse: device allocation failed";
+      return -1;
+    }
+    launch_convert<double, float>((int64_t)base[nseg_], Ainv_.p, Ainv32_.p, st_);
+    coarse_.Ainv32 = Ainv32_.p;
+  }
   coarse_.col0 = cb_col0_.p;
   coarse_.nb = cb_nb_.p;
   coarse_.aoff = cb_aoff_.p;
@@ -1398,6 +1408,7 @@ int Problem::setup(const SolveOpts &o) {
     if (prec == 1) launch_convert<double, float>(L.ndof, L.v64.dinv.p, L.v32.dinv.p, st_);
     if (estimate_eigs((int)l, o)) return -1;
   }
+  setup_prec_pending_ = prec;
   if (coarse_factor()) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   have_setup_ = true;
@@ -2197,8 +2208,16 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   }
   Level &L0 = levels_[0];
   const int64_t n = L0.ndof;
+  // inputs of the timed launches in the V-cycle's scratch vectors
+  // (Chebyshev steps and residuals need b, x, d and a diagonal, r standing
+  // in for it; the values only have to be finite; the set-up Dinv is not
+  // touched)
   launch_fill<float>(n, 1.0f, L0.v32.b.p, st_);
+  launch_fill<float>(n, 1.0f, L0.v32.x.p, st_);
+  launch_fill<float>(n, 0.5f, L0.v32.d.p, st_);
+  launch_fill<float>(n, 0.25f, L0.v32.r.p, st_);
   launch_fill<double>(n, 1.0, L0.v64.b.p, st_);
+  launch_fill<double>(n, 1.0, L0.v64.x.p, st_);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -2228,17 +2247,30 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       if (ms_out) ms_out[c] = -1.0;
       continue;
     }
-    AsmArgs a32, a64;
-    a32.y = L0.v32.r.p;
+    // the finest-level launches of one mixed PCG iteration: three FP32
+    // Chebyshev steps and an FP32 residual (V-cycle), an FP64 product (PCG)
+    AsmArgs ac, ar, a64;
+    ac.b = L0.v32.b.p;
+    ac.x_out = L0.v32.x2.p;
+    ac.d = L0.v32.d.p;
+    ac.dinv = L0.v32.r.p;
+    ac.ca = 0.5;
+    ac.cc = 0.25;
+    ac.cflags = CHEB_D_READ | CHEB_D_STORE;
+    ar.b = L0.v32.b.p;
+    ar.y = L0.v32.d.p;
+    a64.b = L0.v64.b.p;
     a64.y = L0.v64.r.p;
-    // ms per launch of one operator form
+    // ms per such group of launches of one operator form
     auto time_op = [&](int form) -> double {  // 0 FP32 matrix cores, 1 FP32 VALU, 2 FP64
       fp32_valu_ = form == 1;
       auto once = [&]() {
-        if (form == 2)
-          apply_level<double>(0, L0.v64.b.p, ASM_STORE, a64);
-        else
-          apply_level<float>(0, L0.v32.b.p, ASM_STORE, a32);
+        if (form == 2) {
+          apply_level<double>(0, L0.v64.x.p, ASM_RESID, a64);
+        } else {
+          for (int k = 0; k < 3; k++) apply_level<float>(0, L0.v32.x.p, ASM_CHEB, ac);
+          apply_level<float>(0, L0.v32.x.p, ASM_RESID, ar);
+        }
       };
       once();  // warm
       (void)hipEventRecord(e0, st_);
@@ -2260,7 +2292,7 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       return -1;
     }
     const bool mfma = tm <= tv;
-    const double per = 4.0 * (mfma ? tm : tv) + t64;
+    const double per = (mfma ? tm : tv) + t64;
     if (ms_out) ms_out[c] = per;
     if (best < 0 || per < best_ms) {
       best = c;

@@ -338,6 +338,8 @@ class Problem {
   DevArray<int> cb_col0_, cb_nb_;
   DevArray<int64_t> cb_aoff_;
   CoarseBlocks coarse_;
+  DevArray<float> Ainv32_;       // FP32 copy of the coarse inverse (FP32 V-cycles)
+  int setup_prec_pending_ = 0;   // precision of the setup in progress
   int nseg_ = 1;                 // independent components
   std::vector<int> seg_block0_;  // [nseg + 1] block ranges of the components
   // PCG state: f_, u_ hold the current load case (first of a batch)

@@ -247,9 +247,28 @@ class Problem:
                                                 C.byref(cv), C.byref(rr)), "component_info")
         return it.value, bool(cv.value), rr.value
 
-    def autotune(self, tile_sizes=(256, 384, 512, 768, 1024), reps=5):
-        """Pick the owner-tile size with the fastest finest-level operator;
-        returns (chosen size, {size: ms})."""
+    def tile_candidates(self):
+        """Owner-tile sizes worth timing: the powers of two and their
+        midpoints from 256 to 1024 nodes, and the sizes that make the tile
+        count a whole number of rounds of resident tiles (256 CUs holding 4
+        tiles of 512 threads or 8 of 256 threads), which removes the partly
+        filled last round on meshes whose node count is not a power of two."""
+        n = int(self.forests[0].getNodeRange()[1])
+        c = {256, 384, 512, 768, 1024}
+        for per_round in (1024, 2048):
+            for r in range(1, 9):
+                t = -(-n // (r * per_round))
+                if 192 <= t <= 1100:
+                    c.add(t)
+        return tuple(sorted(c))
+
+    def autotune(self, tile_sizes=None, reps=5):
+        """Pick the owner-tile size (and FP32 operator form) with the fastest
+        finest-level launches of one mixed PCG iteration (three Chebyshev
+        steps, a residual, an FP64 residual); returns (chosen size,
+        {size: ms}).  Default candidates: tile_candidates()."""
+        if tile_sizes is None:
+            tile_sizes = self.tile_candidates()
         n = len(tile_sizes)
         ts = (C.c_int * n)(*tile_sizes)
         ms = (C.c_double * n)()
