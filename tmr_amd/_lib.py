@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtmrx.so")
+# TMRX_LIB selects another build of the same library (the sanitizer build of
+# tmr_amd.build.build_sanitized, for host-code tests)
+LIB_PATH = os.environ.get("TMRX_LIB") or os.path.join(_HERE, "libtmrx.so")
 
 
 class Octant(C.Structure):

@@ -64,6 +64,44 @@ def build(verbose=False):
     return LIB
 
 
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+             "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(out_dir=None, verbose=False):
+    """libtmrx_asan.so in out_dir (default: a directory under the system
+    temporary directory, outside the package): the host C++ (forest,
+    problem set-up, C-ABI) compiled by g++ with AddressSanitizer and
+    UndefinedBehaviorSanitizer, linked with the device objects of the last
+    build().  For CPU tests of the host code, run with the sanitizer runtimes
+    preloaded; no GPU code is instrumented."""
+    import tempfile
+    cpp, hip = _sources()
+    dev_objs = [os.path.join(OUT_DIR, os.path.basename(s) + ".o") for s in hip]
+    if not all(os.path.exists(o) for o in dev_objs):
+        raise RuntimeError("build_sanitized: run build() first (device objects missing)")
+    out_dir = out_dir or os.path.join(tempfile.gettempdir(), "tmrx_asan_%d" % os.getuid())
+    os.makedirs(out_dir, exist_ok=True)
+    jobs = []
+    for s in cpp:
+        o = os.path.join(out_dir, os.path.basename(s) + ".o")
+        jobs.append((["g++"] + CXX_FLAGS + SAN_FLAGS + ["-c", s, "-o", o], o))
+    nproc = min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=nproc) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    lib = os.path.join(out_dir, "libtmrx_asan.so")
+    _run(["g++", "-shared", "-fPIC"] + SAN_FLAGS + ["-o", lib + ".tmp"] +
+         [o for _, o in jobs] + dev_objs +
+         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(lib + ".tmp", lib)
+    if verbose:
+        print("built", lib)
+    return lib
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv:
+        build_sanitized(verbose=True)
+    else:
+        build(verbose=True)
     sys.exit(0)
