@@ -211,6 +211,17 @@ int tmrx_filter_apply_transpose(tmrx_filter *f, int vars_per_node, const double 
    Helmholtz solve. */
 int tmrx_filter_info(tmrx_filter *f, int *num_nodes, int *last_iters);
 
+/* Race check of owner-tile accumulation schedules.  No counterpart in the
+   reference (its element loops run on the host); for callers that build or
+   inspect schedules: the checker every tile build runs before upload.
+   Records (24 bytes each: 8 uint16 slot positions, int32 element, uint8
+   corner mask, colour, class, pad) of tile t are rec_ptr[t]..rec_ptr[t+1];
+   nacc[t] is the tile's count of accumulated slots.  Returns 0 if the
+   records are sorted by colour, no colour writes a slot twice and every
+   written slot is an accumulator; else -1 with the reason in msg. */
+int tmrx_check_tile_schedule(int ntiles, const int *rec_ptr, const void *recs, const int *nacc,
+                             char *msg, int msglen);
+
 /* ---------------------------------------------------------------------------
    Problem: the finite-element hierarchy and solver that TMR hands to TACS.
    One handle owns every multigrid level on one GPU.

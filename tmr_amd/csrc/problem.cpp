@@ -760,6 +760,17 @@ int Problem::build_tiles(int l) {
     }
   }
   node0[ntiles] = n;
+  {
+    // the schedule must be race-free before it reaches the device
+    std::vector<int> na_t(ntiles);
+    for (int t = 0; t < ntiles; t++) na_t[t] = node0[t + 1] - node0[t] + nacc[t];
+    std::string err;
+    if (check_tile_schedule(ntiles, el_ptr.data(), els.data(), na_t.data(), MFMA_WAVES,
+                            mb_ptr.data(), mel.data(), err)) {
+      last_error = "problem: tile schedule check failed: " + err;
+      return -1;
+    }
+  }
   if (comb_slot.empty()) {
     comb_slot.push_back(0);
     comb_w.push_back(0.0f);

@@ -183,6 +183,11 @@ struct Level {
   LevelVecs<double> v64;
 };
 
+// race check of owner-tile schedules (schedule_check.cpp): 0 if the
+// records can accumulate without conflicts, else -1 with the reason in err
+int check_tile_schedule(int ntiles, const int *rec_ptr, const TileElem *recs, const int *nacc,
+                        int waves, const int *wave_ptr, const TileElem *mrecs, std::string &err);
+
 class Problem {
  public:
   Problem() = default;
