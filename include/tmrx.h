@@ -167,6 +167,16 @@ int tmrx_forest_generate_bin_file(tmrx_forest *f, const char *filename, const do
    as an ASCII STL ("solid topology", one facet with its unit normal per
    triangle). */
 int tmrx_convert_bin_to_stl(const char *binfile, const char *stlfile);
+/* Volume output in place of TACS's ToFH5 .f5 file (the OutputCallback of
+   examples/topology_optimization/cantilever/cantilever.py:154-182: mesh
+   connectivity, nodes and element extras; TACS and its format are not in the
+   reference): a legacy-VTK unstructured grid of the hex8 elements, with an
+   optional nodal field (node_vpn values per independent node; dependent
+   nodes interpolated; 3 components are written as VECTORS) and an optional
+   element field (elem_vpe values per element).  NULL skips a field. */
+int tmrx_forest_write_vtk(tmrx_forest *f, const char *filename, const double *node_vals,
+                          int node_vpn, const char *node_name, const double *elem_vals,
+                          int elem_vpe, const char *elem_name);
 
 /* TMROctTACSTopoCreator's element densities from a filter forest
    (src/topology/TMR_TACSTopoCreator.cpp:55-186): rows = elements of f, columns

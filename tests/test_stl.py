@@ -68,3 +68,62 @@ def test_axis_plane_with_hanging_nodes(tmp_path):
         area, vol = _area_volume(TMR.readSTLBin(tmp_path / "a.bin"))
         assert abs(vol - 1.0) < 1e-12 and abs(area - 6.0) < 1e-12
     assert TMR.writeSTLToBin(tmp_path / "e.bin", f, np.zeros(len(P)), cutoff=0.5) == 0
+
+
+def _read_vtk(path):
+    """Minimal reader of the legacy-VTK files written by writeVTK."""
+    tok = open(path).read().split()
+    i = tok.index("POINTS")
+    n = int(tok[i + 1])
+    P = np.array(tok[i + 3:i + 3 + 3 * n], float).reshape(n, 3)
+    i = tok.index("CELLS")
+    ne = int(tok[i + 1])
+    cells = np.array(tok[i + 3:i + 3 + 9 * ne], int).reshape(ne, 9)
+    assert np.all(cells[:, 0] == 8)
+    i = tok.index("CELL_TYPES")
+    assert all(t == "12" for t in tok[i + 2:i + 2 + ne])
+    out = {"P": P, "cells": cells[:, 1:]}
+    if "VECTORS" in tok:
+        i = tok.index("VECTORS")
+        out["u"] = np.array(tok[i + 3:i + 3 + 3 * n], float).reshape(n, 3)
+    if "CELL_DATA" in tok:
+        i = tok.index("CELL_DATA")
+        out["rho"] = np.array(tok[i + 8:i + 8 + ne], float)
+    return out
+
+
+def _hex_volume(V):
+    """Volume of a hexahedron in VTK corner order (five tetrahedra)."""
+    tets = [(0, 1, 3, 4), (1, 2, 3, 6), (1, 4, 5, 6), (3, 4, 6, 7), (1, 3, 4, 6)]
+    vol = 0.0
+    for a, b, c, d in tets:
+        vol += np.dot(V[b] - V[a], np.cross(V[c] - V[a], V[d] - V[a])) / 6.0
+    return vol
+
+
+def test_vtk_volume_output(tmp_path):
+    """writeVTK: every hexahedron in VTK order has the element's volume
+    (positive orientation), hanging points carry the interpolated nodal
+    field (exact for a linear field), element data round trip."""
+    f = TMR.createBoxForest(2, 1, 1, (2.0, 1.0, 1.0), face_names={"x-": "fixed"})
+    f.createTrees(1)
+    rng = np.random.default_rng(3)
+    f.refine(rng.integers(0, 2, len(f.getOctants())).astype(np.intc), 0, 3)
+    f.balance(1)
+    f.createNodes()
+    nums, X = f.getNodeNumbers(), f.getPoints()
+    n = int(f.getNodeRange()[1])
+    Pind = np.zeros((n, 3))
+    Pind[nums[nums >= 0]] = X[nums >= 0]
+    A = np.array([[0.5, -1.0, 2.0], [1.5, 0.25, -0.5], [0.0, 1.0, 1.0]])
+    u = Pind @ A.T + np.array([0.1, 0.2, 0.3])
+    rho = rng.random(len(f.getOctants()))
+    TMR.writeVTK(tmp_path / "m.vtk", f, node_field=u, elem_field=rho)
+    d = _read_vtk(tmp_path / "m.vtk")
+    assert len(d["P"]) == len(nums) and np.allclose(d["P"], X, rtol=0, atol=1e-15)
+    ext = f.getElementExtents()
+    for e, c in enumerate(d["cells"]):
+        assert abs(_hex_volume(d["P"][c]) - np.prod(ext[e])) < 1e-12
+    assert np.allclose(d["u"], d["P"] @ A.T + np.array([0.1, 0.2, 0.3]), atol=1e-13)
+    assert np.array_equal(d["rho"], rho)
+    assert (nums < 0).any()

@@ -341,6 +341,29 @@ def convertBinToSTL(binfile, stlfile):
           "convertBinToSTL")
 
 
+def writeVTK(fname, forest, node_field=None, node_name="u", elem_field=None, elem_name="rho"):
+    """Volume output of the forest's mesh (the role of TACS's ToFH5 .f5 file
+    in the reference's examples): legacy-VTK hex8 grid with an optional
+    nodal field (values per independent node, any number of components;
+    hanging nodes interpolated) and an optional element field."""
+    n, ne = int(forest.getNodeRange()[1]), len(forest.getOctants())
+    nv = ev = 0
+    nptr = eptr = None
+    if node_field is not None:
+        node_field = np.ascontiguousarray(node_field, dtype=np.double).reshape(-1)
+        if n == 0 or node_field.size % n:
+            raise ValueError("node_field must hold a whole number of values per node")
+        nv, nptr = node_field.size // n, node_field.ctypes.data
+    if elem_field is not None:
+        elem_field = np.ascontiguousarray(elem_field, dtype=np.double).reshape(-1)
+        if ne == 0 or elem_field.size % ne:
+            raise ValueError("elem_field must hold a whole number of values per element")
+        ev, eptr = elem_field.size // ne, elem_field.ctypes.data
+    check(lib().tmrx_forest_write_vtk(forest.ptr, str(fname).encode(), nptr, nv,
+                                      node_name.encode(), eptr, ev, elem_name.encode()),
+          "writeVTK")
+
+
 def readSTLBin(binfile):
     """The triangles of a binary triangle file, shape (ntris, 3, 3)."""
     with open(binfile, "rb") as fh:

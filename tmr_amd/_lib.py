@@ -60,6 +60,7 @@ _SIGS = {
     "tmrx_forest_create_trees": (I, [P, I]),
     "tmrx_forest_refine": (I, [P, IP, I, I]),
     "tmrx_forest_balance": (I, [P, I]),
+    "tmrx_forest_write_vtk": (I, [P, C.c_char_p, P, I, C.c_char_p, P, I, C.c_char_p]),
     "tmrx_check_tile_schedule": (I, [I, P, P, P, C.c_char_p, I]),
     "tmrx_forest_balance_device": (I, [P, I, I, DP]),
     "tmrx_forest_coarsen": (I, [P, C.POINTER(P)]),

@@ -9,6 +9,7 @@
 // oriented with normals out of the solid.  File formats as the reference's:
 //   binary: int32 triangle count, then 9 doubles per triangle (3 vertices);
 //   ASCII STL: "solid topology", one facet per triangle with its unit normal.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -227,6 +228,92 @@ extern "C" int tmrx_convert_bin_to_stl(const char *binfile, const char *stlfile)
   std::fprintf(out, "endsolid topology\n");
   if (std::fclose(out) != 0) {
     tmrx::set_error("tmrx_convert_bin_to_stl: write failed");
+    return -1;
+  }
+  return 0;
+}
+
+// Volume output in place of TACS's ToFH5 (examples/topology_optimization/
+// cantilever/cantilever.py:154-182 writes connectivity, nodes and element
+// extras to an .f5 file; TACS and its format are not part of the
+// reference): a legacy-VTK unstructured grid of the forest's hex8 elements
+// with an optional nodal field (node_vpn values per independent node,
+// dependent nodes interpolated from their parents) and an optional element
+// field (elem_vpe values per element).  Points follow getPoints's order
+// (dependent nodes first).  VTK's hexahedron corner order is 0 1 3 2 4 5 7 6
+// in TMR's local numbering (x fastest).
+extern "C" int tmrx_forest_write_vtk(tmrx_forest *forest, const char *filename,
+                                     const double *node_vals, int node_vpn, const char *node_name,
+                                     const double *elem_vals, int elem_vpe,
+                                     const char *elem_name) {
+  tmrx::Forest *f = tmrx_forest_impl(forest);
+  if (!f || !filename || (node_vals && node_vpn < 1) || (elem_vals && elem_vpe < 1)) {
+    tmrx::set_error("tmrx_forest_write_vtk: invalid arguments");
+    return -1;
+  }
+  if (f->create_nodes()) {
+    tmrx::set_error("tmrx_forest_write_vtk: " + f->last_error);
+    return -1;
+  }
+  const auto &conn = f->conn();
+  const auto &dp = f->dep_ptr();
+  const auto &dc = f->dep_conn();
+  const auto &dw = f->dep_weights();
+  const auto &X = f->points();
+  const int ndep = f->num_dep_nodes();
+  const int nloc = (int)f->node_numbers().size();
+  const int ne = f->num_elements();
+  auto index = [&](int g) { return g >= 0 ? g + ndep : ndep - 1 - (-g - 1); };
+  FILE *fp = std::fopen(filename, "w");
+  if (!fp) {
+    tmrx::set_error(std::string("tmrx_forest_write_vtk: cannot open ") + filename);
+    return -1;
+  }
+  std::fprintf(fp, "# vtk DataFile Version 3.0\ntmrx forest\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(fp, "POINTS %d double\n", nloc);
+  for (int i = 0; i < nloc; i++)
+    std::fprintf(fp, "%.17g %.17g %.17g\n", X[3 * (size_t)i], X[3 * (size_t)i + 1], X[3 * (size_t)i + 2]);
+  std::fprintf(fp, "CELLS %d %d\n", ne, 9 * ne);
+  static const int VTK_ORDER[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+  for (int e = 0; e < ne; e++) {
+    std::fprintf(fp, "8");
+    for (int a = 0; a < 8; a++) std::fprintf(fp, " %d", index(conn[8 * (size_t)e + VTK_ORDER[a]]));
+    std::fprintf(fp, "\n");
+  }
+  std::fprintf(fp, "CELL_TYPES %d\n", ne);
+  for (int e = 0; e < ne; e++) std::fprintf(fp, "12\n");
+  if (node_vals) {
+    std::fprintf(fp, "POINT_DATA %d\n", nloc);
+    if (node_vpn == 3)
+      std::fprintf(fp, "VECTORS %s double\n", node_name ? node_name : "u");
+    else
+      std::fprintf(fp, "SCALARS %s double %d\nLOOKUP_TABLE default\n", node_name ? node_name : "x",
+                   node_vpn);
+    std::vector<double> v(node_vpn);
+    for (int i = 0; i < nloc; i++) {
+      const int g = i >= ndep ? i - ndep : -(ndep - i);  // the node number at point i
+      std::fill(v.begin(), v.end(), 0.0);
+      if (g >= 0) {
+        for (int k = 0; k < node_vpn; k++) v[k] = node_vals[(size_t)node_vpn * g + k];
+      } else {
+        for (int j = dp[-g - 1]; j < dp[-g]; j++)
+          for (int k = 0; k < node_vpn; k++) v[k] += dw[j] * node_vals[(size_t)node_vpn * dc[j] + k];
+      }
+      for (int k = 0; k < node_vpn; k++) std::fprintf(fp, k ? " %.17g" : "%.17g", v[k]);
+      std::fprintf(fp, "\n");
+    }
+  }
+  if (elem_vals) {
+    std::fprintf(fp, "CELL_DATA %d\nSCALARS %s double %d\nLOOKUP_TABLE default\n", ne,
+                 elem_name ? elem_name : "rho", elem_vpe);
+    for (int e = 0; e < ne; e++) {
+      for (int k = 0; k < elem_vpe; k++)
+        std::fprintf(fp, k ? " %.17g" : "%.17g", elem_vals[(size_t)elem_vpe * e + k]);
+      std::fprintf(fp, "\n");
+    }
+  }
+  if (std::fclose(fp) != 0) {
+    tmrx::set_error("tmrx_forest_write_vtk: write failed");
     return -1;
   }
   return 0;
