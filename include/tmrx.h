@@ -205,6 +205,16 @@ typedef struct tmrx_filter tmrx_filter;
    forest's independent nodes.  device = HIP device index. */
 int tmrx_filter_create(tmrx_forest *filter_forest, int kind, double r, int nterms, int device,
                        tmrx_filter **out);
+/* TMRHelmholtzPUFilter (src/topology/TMRHelmholtzPUFilter.cpp:407-720; the
+   base of TopOptUtils.Mfilter, tmr/TopOptUtils.py:785-890): a
+   partition-of-unity filter over n nodes from per-node stencils computed by
+   the caller: B (CSR rowp[n + 1], cols, values; off-diagonal stencil
+   weights, column order free) and the stencil diagonal D (> 0).  Applied as
+   F x = T^-1 sum_{k=0}^{nterms} (D^-1 B)^k D^-1 x (Horner), T^-1 = 1 / F(1);
+   tmrx_filter_apply_transpose applies F^T (B^T inside).  Freed with
+   tmrx_filter_free. */
+int tmrx_filter_create_custom(int n, int nterms, const int *rowp, const int *cols, const double *B,
+                              const double *D, int device, tmrx_filter **out);
 void tmrx_filter_free(tmrx_filter *f);
 /* applyFilter (TMRHelmholtzFilter::applyFilter, :152-255;
    TMRMatrixFilter::applyFilter, :191-211): in and out hold vars_per_node

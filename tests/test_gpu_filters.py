@@ -75,7 +75,71 @@ def test_filter_matches_oracle(adapt, kind, r, N):
     free()
 
 
-@pytest.mark.parametrize("filter_type", ["helmholtz", "matrix"])
+def _mfilter(assemblers, filters):
+    mf = TopOptUtils.Mfilter(4, assemblers, filters, dim=3, r=0.15)
+    mf.initialize()
+    return mf
+
+
+def _horner(rowp, cols, B, D, N, x, transpose=False):
+    """The partition-of-unity filter written out with scipy:
+    T^-1 sum_{k<=N} (D^-1 B)^k D^-1 x, T^-1 = 1 / F(1)."""
+    import scipy.sparse as sp
+    n = len(D)
+    Bm = sp.csr_matrix((B, cols, rowp), shape=(n, n))
+    Dinv = 1.0 / D
+
+    def fwd(v):
+        t = Dinv * v
+        y = t.copy()
+        for _ in range(N):
+            y = Dinv * (Bm @ y) + t
+        return y
+    Tinv = 1.0 / fwd(np.ones(n))
+    if not transpose:
+        return Tinv * fwd(x)
+    t = Tinv * x
+    y = t.copy()
+    for _ in range(N):
+        y = Bm.T @ (Dinv * y) + t
+    return Dinv * y
+
+
+def test_mfilter_matches_the_written_out_sum():
+    """TopOptUtils.Mfilter on the GPU (tmrx_filter_create_custom): F x and
+    F^T y equal the Horner sum written out with scipy from the same
+    stencils; constants pass; the stencils meet their moment conditions."""
+    forest = TMR.createBoxForest(2, 1, 1, L=(2.0, 1.0, 1.0), face_names=FACES)
+    forest.createTrees(2)
+    rng = np.random.default_rng(4)
+    forest.refine(rng.integers(0, 2, len(forest.getOctants())).astype(np.intc), 0, 3)
+    mat = TMR.MaterialProperties(rho=2.5, E=1.0, nu=0.3, ys=1.0)
+    props = TMR.StiffnessProperties(mat, q=3.0, k0=1e-6)
+    bcs = TMR.BoundaryConditions()
+    bcs.addBoundaryCondition("fixed")
+
+    def callback(f):
+        c = topo.OctConformTopoCreator(bcs, f, props=props)
+        return c, c.getFilter()
+
+    prob = TopOptUtils.createTopoProblem(forest, callback, _mfilter, nlevels=2)
+    mf = prob.getTopoFilter()
+    rowp, cols, B, D = mf._stencil
+    n = len(D)
+    x, y = rng.random(n), rng.standard_normal(n)
+    fx, fty = mf.applyFilter(x), mf.applyTranspose(y)
+    rx, rty = _horner(rowp, cols, B, D, 4, x), _horner(rowp, cols, B, D, 4, y, True)
+    e1 = np.linalg.norm(fx - rx) / np.linalg.norm(rx)
+    e2 = np.linalg.norm(fty - rty) / np.linalg.norm(rty)
+    print("Mfilter: %d nodes, %d stencil entries, rel err %.2e, transpose %.2e"
+          % (n, len(cols), e1, e2), flush=True)
+    assert e1 < 1e-12 and e2 < 1e-12
+    assert np.allclose(mf.applyFilter(np.full(n, 0.7)), 0.7, rtol=1e-12, atol=0)
+    assert abs(fx @ y - x @ fty) <= 1e-11 * abs(fx @ y)
+    assert B.min() >= 0.0 and D.min() >= 1.0
+
+
+@pytest.mark.parametrize("filter_type", ["helmholtz", "matrix", "mfilter"])
 def test_topo_problem_with_filter(filter_type):
     """createTopoProblem(..., "helmholtz" | "matrix"): the analysis sees the
     filtered nodal design; the design-space gradient is the filter transpose
@@ -93,7 +157,8 @@ def test_topo_problem_with_filter(filter_type):
         c = topo.OctConformTopoCreator(bcs, f, props=props)
         return c, c.getFilter()
 
-    prob = TopOptUtils.createTopoProblem(forest, callback, filter_type, nlevels=2, r0=0.1, N=6)
+    ft = _mfilter if filter_type == "mfilter" else filter_type
+    prob = TopOptUtils.createTopoProblem(forest, callback, ft, nlevels=2, r0=0.1, N=6)
     a = prob.getAssembler()
     f1 = TopOptUtils.compute3DTractionLoad("load", a.forest, a, [0.0, 0.0, -1.0])
     prob.setLoadCases([f1])

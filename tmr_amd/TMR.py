@@ -373,7 +373,7 @@ def readSTLBin(binfile):
 
 # topology-optimisation classes (tmr/TMR.pyx:3877-6005) live in topo.py
 from .topo import (BoundaryConditions, Compliance, ConformFilter, LagrangeFilter,  # noqa: E402,F401
-                   HelmholtzFilter, MatrixFilter,
+                   HelmholtzFilter, HelmholtzPUFilter, MatrixFilter,
                    MaterialProperties, OctConformTopoCreator, OctConstitutive, OctTopoCreator,
                    PNormFailure, StiffnessProperties, StructuralMass, TopoProblem, createMg,
                    RAMP_PENALTY, SIMP_PENALTY)

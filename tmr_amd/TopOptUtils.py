@@ -25,7 +25,9 @@ def createTopoProblem(forest, callback, filter_type, nlevels=2, repartition=True
     interpolated from the filter forest, OctTopoCreator), "conform" (nodal
     design variables on the analysis forests, OctConformTopoCreator), and
     "matrix" / "helmholtz" (conform levels whose analysis design is the
-    matrix- or Helmholtz-filtered design, radius r0, N terms)."""
+    matrix- or Helmholtz-filtered design, radius r0, N terms), or a callable
+    filter_type(assemblers, filters) returning the filter (e.g. an
+    initialised Mfilter, as the reference's examples pass it)."""
     if design_vars_per_node != 1:
         raise NotImplementedError("one design variable per node only")
     if scale_coordinate_factor != 1.0:
@@ -186,3 +188,107 @@ def densityBasedRefine(forest, density, lower=0.05, upper=0.5, reverse=False, mi
         refine[value <= lower] = -1
     forest.refine(refine, min_lev, max_lev)
     return refine
+
+
+# ---------------------------------------------------------------------------
+# M-filter (tmr/TopOptUtils.py:643-890): Helmholtz partition-of-unity filter
+# whose stencils are the least-squares weights that reproduce the second-
+# order Taylor moments of r^2 times the Laplacian with non-negative weights.
+class OptFilterWeights:
+    """Stencil-weight problem of one node (tmr/TopOptUtils.py:643-782):
+    minimise 1/2 |w|^2 over w >= 0 (one weight per neighbour) subject to
+    A w = b, where the columns of A are the neighbour offsets d and the
+    second moments d d^T / 2 (upper triangle, off-diagonal terms d_i d_j),
+    normalised by the stencil radius delta, and b asks for a zero first
+    moment and the second moments of H.  alpha_j = w_j / delta^2 and the
+    diagonal alpha_0 = 1 + sum_j w_j / delta^2."""
+
+    def __init__(self, diag, X, H):
+        X = np.asarray(X, dtype=float)
+        if X.ndim == 1:
+            X = X[:, None]
+        self.diag, self.n = int(diag), X.shape[0]
+        d = np.delete(X - X[self.diag], self.diag, axis=0)
+        self.delta = float(np.sqrt(np.max((d * d).sum(axis=1))))
+        d = d / self.delta
+        dim = X.shape[1]
+        self.dim = dim
+        H = np.asarray(H, dtype=float)
+        if dim == 1:
+            A = np.vstack([d[:, 0], 0.5 * d[:, 0] ** 2])
+            b = np.array([0.0, H[0, 0]])
+        elif dim == 2:
+            A = np.vstack([d[:, 0], d[:, 1], 0.5 * d[:, 0] ** 2, 0.5 * d[:, 1] ** 2,
+                           d[:, 0] * d[:, 1]])
+            b = np.array([0.0, 0.0, H[0, 0], H[1, 1], 2.0 * H[0, 1]])
+        else:
+            A = np.vstack([d[:, 0], d[:, 1], d[:, 2],
+                           0.5 * d[:, 0] ** 2, 0.5 * d[:, 1] ** 2, 0.5 * d[:, 2] ** 2,
+                           d[:, 1] * d[:, 2], d[:, 0] * d[:, 2], d[:, 0] * d[:, 1]])
+            b = np.array([0.0, 0.0, 0.0, H[0, 0], H[1, 1], H[2, 2],
+                          2.0 * H[1, 2], 2.0 * H[0, 2], 2.0 * H[0, 1]])
+        self.A, self.b = A, b
+
+    def obj_func(self, w):
+        return 0.5 * float(np.dot(w, w))
+
+    def obj_func_der(self, w):
+        return w
+
+    def con_func(self, w):
+        return self.A @ w - self.b
+
+    def con_func_der(self, w):
+        return self.A
+
+    def solve(self):
+        from scipy.optimize import minimize
+        m = self.n - 1
+        res = minimize(self.obj_func, np.ones(m), jac=self.obj_func_der, method="SLSQP",
+                       bounds=[(0.0, None)] * m,
+                       constraints={"type": "eq", "fun": self.con_func,
+                                    "jac": self.con_func_der})
+        return res.x
+
+    def set_alphas(self, w, alpha):
+        alpha[:] = 0.0
+        others = [i for i in range(self.n) if i != self.diag]
+        alpha[others] = w / self.delta ** 2
+        alpha[self.diag] = 1.0 + float(np.sum(w)) / self.delta ** 2
+
+
+class Mfilter(topo.HelmholtzPUFilter):
+    """TopOptUtils.Mfilter (tmr/TopOptUtils.py:785-890): a Helmholtz
+    partition-of-unity filter with radius r whose stencils are the
+    OptFilterWeights solutions: H = r^2 I at interior nodes; at boundary
+    nodes the stencil is fitted in the tangent plane (coordinates along two
+    directions orthogonal to the normal, H = r^2 I in 2D).  Call
+    initialize() before use."""
+
+    def __init__(self, N, assemblers, filters, vars_per_node=1, dim=3, r=0.01, device=0):
+        super().__init__(N, assemblers, filters, vars_per_node, device)
+        self.r, self.dim = float(r), int(dim)
+
+    def getInteriorStencil(self, diag, X, alpha):
+        X = np.asarray(X, dtype=float).reshape(-1, 3)
+        if self.dim == 2:
+            X = X[:, :2]
+        opt = OptFilterWeights(diag, X, self.r ** 2 * np.eye(3))
+        opt.set_alphas(opt.solve(), alpha)
+
+    def getBoundaryStencil(self, diag, normal, X, alpha):
+        X = np.asarray(X, dtype=float).reshape(-1, 3)
+        normal = np.asarray(normal, dtype=float)
+        if self.dim == 2:
+            t = np.array([normal[1], -normal[0]])
+            Xt = (X[:, :2] - X[diag, :2]) @ t
+        else:
+            # two in-plane directions from the coordinate axis least aligned
+            # with the normal
+            e = np.zeros(3)
+            e[np.argmin(np.abs(normal))] = 1.0
+            t2 = np.cross(e, normal)
+            t1 = np.cross(normal, t2)
+            Xt = np.stack([(X - X[diag]) @ t1, (X - X[diag]) @ t2], axis=1)
+        opt = OptFilterWeights(diag, Xt, self.r ** 2 * np.eye(2))
+        opt.set_alphas(opt.solve(), alpha)

@@ -84,6 +84,7 @@ _SIGS = {
     "tmrx_forest_generate_bin_file": (I, [P, S, P, I, I, D, IP]),
     "tmrx_convert_bin_to_stl": (I, [S, S]),
     "tmrx_filter_create": (I, [P, I, D, I, I, C.POINTER(P)]),
+    "tmrx_filter_create_custom": (I, [I, I, P, P, P, P, I, P]),
     "tmrx_filter_free": (None, [P]),
     "tmrx_filter_apply": (I, [P, I, P, P, I]),
     "tmrx_filter_apply_transpose": (I, [P, I, P, P, I]),

@@ -162,7 +162,74 @@ int Filter::init(Forest *f, int kind, double r, int nterms) {
   return 0;
 }
 
-// Horner sum of the matrix filter (transpose: the same sum with M and the
+// TMRHelmholtzPUFilter (src/topology/TMRHelmholtzPUFilter.cpp:407-700): the
+// stencil weights come from the caller (getInteriorStencil /
+// getBoundaryStencil); the Horner sum is the matrix filter's with both
+// diagonal scalings D^-1 and M = B, and B^T in the transpose.
+int Filter::init_custom(int n, int nterms, const int *rowp, const int *cols, const double *B,
+                        const double *D) {
+  if (n < 1 || nterms < 0 || !rowp || !cols || !B || !D || rowp[0] != 0) {
+    last_error = "filter: invalid arguments";
+    return -1;
+  }
+  const int nnz = rowp[n];
+  for (int i = 0; i < n; i++) {
+    if (rowp[i + 1] < rowp[i]) {
+      last_error = "filter: row pointers must not decrease";
+      return -1;
+    }
+    if (!(D[i] > 0.0)) {
+      last_error = "filter: the stencil diagonal must be positive";
+      return -1;
+    }
+  }
+  for (int k = 0; k < nnz; k++)
+    if (cols[k] < 0 || cols[k] >= n) {
+      last_error = "filter: column index out of range";
+      return -1;
+    }
+  kind_ = 2;
+  nterms_ = nterms;
+  n_ = n;
+  std::vector<int> ptr(rowp, rowp + n + 1), col(cols, cols + nnz), tptr(n + 1, 0), tcol(nnz);
+  std::vector<double> val(B, B + nnz), tval(nnz), dinv(n);
+  for (int k = 0; k < nnz; k++) tptr[col[k] + 1]++;
+  for (int i = 0; i < n; i++) tptr[i + 1] += tptr[i];
+  {
+    std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+    for (int i = 0; i < n; i++)
+      for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+        tcol[fill[col[k]]] = i;
+        tval[fill[col[k]]++] = val[k];
+      }
+  }
+  for (int i = 0; i < n; i++) dinv[i] = 1.0 / D[i];
+  if (nnz == 0) {
+    col.push_back(0);
+    val.push_back(0.0);
+    tcol.push_back(0);
+    tval.push_back(0.0);
+  }
+  if (ptr_.upload(ptr) || col_.upload(col) || mval_.upload(val) || tptr_.upload(tptr) ||
+      tcol_.upload(tcol) || tval_.upload(tval) || ascale_.upload(dinv) || bscale_.upload(dinv) ||
+      tinv_.alloc(n_) || w1_.alloc(n_) || w2_.alloc(n_) || w5_.alloc(n_) || in_.alloc(n_) ||
+      out_.alloc(n_)) {
+    last_error = "filter: device allocation failed";
+    return -1;
+  }
+  // T^-1 = 1 / filter(1), formed with T^-1 = 1
+  launch_filter_fill(n_, 1.0, tinv_.p, st_);
+  launch_filter_fill(n_, 1.0, in_.p, st_);
+  if (horner(in_.p, out_.p, false)) return -1;
+  launch_filter_recip(n_, out_.p, tinv_.p, st_);
+  if (hipStreamSynchronize(st_) != hipSuccess) {
+    last_error = "filter: device error";
+    return -1;
+  }
+  return 0;
+}
+
+// Horner sum of the matrix filter (transpose: the same sum with M^T and the
 // diagonal scalings in the opposite order)
 int Filter::horner(const double *x, double *y, bool transpose) {
   double *t = w1_.p, *u = w2_.p;
@@ -177,10 +244,12 @@ int Filter::horner(const double *x, double *y, bool transpose) {
   } else {
     launch_filter_mul(n_, tinv_.p, x, t, st_);  // t = T^-1 o x
     launch_filter_copy(n_, t, y, st_);
+    const bool sym = kind_ != 2;  // the matrix filter's M is symmetric
     for (int k = 0; k < nterms_; k++) {
-      launch_filter_mul(n_, bscale_.p, y, u, st_);                  // u = b o y
-      launch_filter_spmv(n_, ptr_.p, col_.p, mval_.p, u, y, st_);   // y = M u
-      launch_filter_add(n_, t, y, st_);                             // y += t
+      launch_filter_mul(n_, bscale_.p, y, u, st_);  // u = b o y
+      launch_filter_spmv(n_, sym ? ptr_.p : tptr_.p, sym ? col_.p : tcol_.p,
+                         sym ? mval_.p : tval_.p, u, y, st_);  // y = M^T u
+      launch_filter_add(n_, t, y, st_);                        // y += t
     }
     launch_filter_mul(n_, ascale_.p, y, y, st_);
   }
@@ -252,7 +321,7 @@ int Filter::apply(int nv, const double *in, double *out, bool on_device, bool tr
   }
   for (int v = 0; v < nv; v++) {
     launch_filter_gather(n_, nv, v, xin, in_.p, st_);
-    if (kind_ == 1) {
+    if (kind_ == 1 || kind_ == 2) {
       if (horner(in_.p, out_.p, transpose)) return -1;
     } else if (!transpose) {
       launch_filter_spmv(n_, ptr_.p, col_.p, mval_.p, in_.p, w5_.p, st_);  // rhs = M x
@@ -302,6 +371,26 @@ int tmrx_filter_create(tmrx_forest *forest, int kind, double r, int nterms, int 
   }
   auto *f = new tmrx::Filter();
   if (f->init(tmrx_forest_impl(forest), kind, r, nterms)) {
+    tmrx::set_error(f->last_error);
+    delete f;
+    return -1;
+  }
+  *out = new tmrx_filter{f, device};
+  return 0;
+}
+
+int tmrx_filter_create_custom(int n, int nterms, const int *rowp, const int *cols, const double *B,
+                              const double *D, int device, tmrx_filter **out) {
+  if (!out) {
+    tmrx::set_error("tmrx_filter_create_custom: null output");
+    return -1;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    tmrx::set_error("tmrx_filter_create_custom: hipSetDevice failed");
+    return -1;
+  }
+  auto *f = new tmrx::Filter();
+  if (f->init_custom(n, nterms, rowp, cols, B, D)) {
     tmrx::set_error(f->last_error);
     delete f;
     return -1;

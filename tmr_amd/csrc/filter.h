@@ -13,6 +13,10 @@ class Filter {
  public:
   // kind 0: Helmholtz radius r; kind 1: matrix filter, radius r, nterms terms
   int init(Forest *f, int kind, double r, int nterms);
+  // kind 2: partition-of-unity filter from given stencils, F = T^-1 sum_k
+  // (D^-1 B)^k D^-1 (nterms + 1 terms), B as CSR over n nodes, D > 0
+  int init_custom(int n, int nterms, const int *rowp, const int *cols, const double *B,
+                  const double *D);
   // out = F in (or F^T in), nv values per independent node interleaved
   int apply(int nv, const double *in, double *out, bool on_device, bool transpose);
   int num_nodes() const { return n_; }
@@ -27,7 +31,8 @@ class Filter {
   int kind_ = 0, nterms_ = 0, n_ = 0, iters_ = 0;
   double r_ = 0.0;
   hipStream_t st_ = nullptr;
-  DevArray<int> ptr_, col_;
+  DevArray<int> ptr_, col_, tptr_, tcol_;  // t*: the transpose (kind 2, B not symmetric)
+  DevArray<double> tval_;
   DevArray<double> mval_, aval_, dinv_, ascale_, bscale_, tinv_;
   DevArray<double> w1_, w2_, w3_, w4_, w5_, in_, out_, parts_;
 };

@@ -503,6 +503,143 @@ class HelmholtzFilter(_NativeFilter):
         super().__init__(0, r, 0, assemblers, filters, vars_per_node, device)
 
 
+def _filter_pattern(forest):
+    """Rows of the filter forest's node adjacency: for every independent node
+    the sorted independent nodes that share an element with it, hanging
+    corners replaced by their parents (the pattern of the assembled
+    matrices TMRHelmholtzPUFilter::initialize stencils over,
+    src/topology/TMRHelmholtzPUFilter.cpp:437-512)."""
+    conn = np.asarray(forest.getMeshConn()).reshape(-1, 8)
+    dp, dc, dw = forest.getDepNodeConn()
+    n = int(forest.getNodeRange()[1])
+    # the parents of every corner, padded with -1 (at most 4)
+    par = np.full((conn.shape[0], 8, 4), -1, dtype=np.int64)
+    ind = conn >= 0
+    par[..., 0] = np.where(ind, conn, -1)
+    d = np.where(ind, 0, -conn - 1)
+    for k in range(4):
+        take = (~ind) & (dp[d] + k < dp[d + 1])
+        par[..., k] = np.where(take, dc[np.minimum(dp[d] + k, len(dc) - 1)], par[..., k])
+    nodes = par.reshape(conn.shape[0], 32)
+    a = np.repeat(nodes, 32, axis=1).ravel()
+    b = np.tile(nodes, (1, 32)).ravel()
+    keep = (a >= 0) & (b >= 0)
+    key = np.unique(a[keep] * n + b[keep])
+    rows, cols = key // n, key % n
+    rowp = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(rowp, rows + 1, 1)
+    return np.cumsum(rowp).astype(np.intc), cols.astype(np.intc)
+
+
+def _boundary_normals(forest, P):
+    """Sum over the element faces on the domain boundary of their outward
+    unit normals at each independent corner node on them (zero inside):
+    computeOctreeBoundaryNormals (src/topology/TMRHelmholtzPUFilter.cpp:
+    183-302), for elements that are parallelepipeds (box blocks)."""
+    octs = forest.getOctants()
+    conn = np.asarray(forest.getMeshConn()).reshape(-1, 8)
+    nums, X = forest.getNodeNumbers(), forest.getPoints()
+    ndep = int((nums < 0).sum())
+    cinfo = forest.getConnectivity()
+    bfc = np.asarray(cinfo["block_face_conn"]).reshape(-1, 6)
+    nblk_face = np.bincount(bfc.ravel(), minlength=int(cinfo["nfaces"]))  # blocks per face
+    hmax = 1 << 30
+    h = np.left_shift(1, 30 - octs["level"].astype(np.int64))
+    coord = np.stack([octs["x"], octs["y"], octs["z"]], 1).astype(np.int64)
+    N = np.zeros((len(P), 3))
+    for e in range(len(octs)):
+        b = octs["block"][e]
+        Xe = X[conn[e] + ndep]  # corner points (getPoints order: dependent first)
+        for surf in range(6):
+            ax, hi = surf // 2, surf % 2
+            if nblk_face[bfc[b, surf]] != 1:
+                continue
+            if coord[e, ax] != (hmax - h[e] if hi else 0):
+                continue
+            # local tangents of the face (d1, d2 as the reference orders them)
+            t1, t2 = [(1, 2), (2, 0), (0, 1)][ax]
+            da = Xe[1 << t1] - Xe[0]
+            db = Xe[1 << t2] - Xe[0]
+            nrm = np.cross(da, db)
+            nrm /= np.linalg.norm(nrm)
+            if hi == 0:
+                nrm = -nrm
+            for c in range(8):
+                if ((c >> ax) & 1) == hi and conn[e, c] >= 0:
+                    N[conn[e, c]] += nrm
+    return N
+
+
+class HelmholtzPUFilter(_NativeFilter):
+    """TMRHelmholtzPUFilter (src/topology/TMRHelmholtzPUFilter.cpp; Python
+    TMR.HelmholtzPUFilter(N, assemblers, filters, vars_per_node)): a
+    partition-of-unity filter from a stencil at every node of the finest
+    filter forest.  Subclasses provide getInteriorStencil(diag, X, alpha) and
+    getBoundaryStencil(diag, normal, X, alpha) (X: the stencil points as 3n
+    values, alpha: the n weights to fill); initialize() calls them row by row
+    and builds F = T^-1 sum_{k=0}^{N} (D^-1 B)^k D^-1 natively (D the
+    diagonal weights, B the others clipped at zero)."""
+
+    def __init__(self, N, assemblers, filters, vars_per_node=1, device=0):
+        ConformFilter.__init__(self, assemblers, filters)
+        if vars_per_node != 1:
+            raise NotImplementedError("one design variable per node only")
+        self.vars_per_node = vars_per_node
+        self.N, self.device = int(N), int(device)
+        self._h = None
+        self.num_nodes = int(self.filters[0].getNodeRange()[1])
+
+    def getInteriorStencil(self, diag, X, alpha):
+        raise NotImplementedError("HelmholtzPUFilter subclasses provide the stencils")
+
+    def getBoundaryStencil(self, diag, normal, X, alpha):
+        raise NotImplementedError("HelmholtzPUFilter subclasses provide the stencils")
+
+    def stencils(self):
+        """(rowp, cols, B, D) of the filter from the subclass's stencils."""
+        f = self.filters[0]
+        rowp, cols = _filter_pattern(f)
+        nums, X = f.getNodeNumbers(), f.getPoints()
+        P = np.zeros((self.num_nodes, 3))
+        P[nums[nums >= 0]] = X[nums >= 0]
+        normals = _boundary_normals(f, P)
+        B = np.zeros(len(cols))
+        D = np.ones(self.num_nodes)
+        for i in range(self.num_nodes):
+            c = cols[rowp[i]:rowp[i + 1]]
+            diag = int(np.nonzero(c == i)[0][0])
+            Xs = P[c].reshape(-1).copy()
+            alpha = np.zeros(len(c))
+            nrm = normals[i]
+            if not nrm.any():
+                self.getInteriorStencil(diag, Xs, alpha)
+            else:
+                self.getBoundaryStencil(diag, nrm / np.linalg.norm(nrm), Xs, alpha)
+            D[i] = alpha[diag] if alpha[diag] > 0.0 else 1.0
+            w = np.maximum(alpha, 0.0)
+            w[diag] = 0.0
+            B[rowp[i]:rowp[i + 1]] = w
+        return rowp, cols, B, D
+
+    def initialize(self):
+        """TMRHelmholtzPUFilter::initialize: the stencils, then the native
+        filter (and its normalisation T^-1 = 1 / F(1))."""
+        import ctypes as C
+        from ._lib import check, lib
+        rowp, cols, B, D = self.stencils()
+        self._stencil = (rowp, cols, B, D)
+        h = C.c_void_p()
+        check(lib().tmrx_filter_create_custom(self.num_nodes, self.N, rowp.ctypes.data,
+                                              cols.ctypes.data, B.ctypes.data, D.ctypes.data,
+                                              self.device, C.byref(h)), type(self).__name__)
+        self._h = h
+
+    def _apply(self, x, transpose):
+        if self._h is None:
+            raise RuntimeError("HelmholtzPUFilter: call initialize() first")
+        return _NativeFilter._apply(self, x, transpose)
+
+
 class MatrixFilter(_NativeFilter):
     """TMRMatrixFilter (src/topology/TMRMatrixFilter.cpp; Python
     TMR.MatrixFilter(r, N, assemblers, filters, vars_per_node)): the N-term
