@@ -192,7 +192,7 @@ def bench_batch(args):
     import torch
     from tmr_amd import solver, workloads
     t0 = time.perf_counter()
-    forests, rho, edges = workloads.cube_batch(args.count, level=3, nlevels=4)
+    forests, rho, edges = workloads.cube_batch(args.count, nlevels=4, device=0)
     prob = solver.Problem(forests, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
     log("batch of %d problems built in %.1f s: %d elements, %d dofs"
         % (prob.num_components(), time.perf_counter() - t0, prob.nelem, prob.ndof))
@@ -219,8 +219,7 @@ def bench_batch(args):
     sizes = sorted(set(int(e) for e in edges))
     for e in sizes:
         k = [int(x) for x in edges].index(e)
-        d = e // 8
-        fk, rk = workloads.box_batch([(d, d, d)], 3, 4, seed=k)
+        fk, rk = workloads.uneven_cube_batch([e], nlevels=4, seed=k, device=0)
         pk = solver.Problem(fk, q=3.0, k0=1e-6, bcs=[("fixed", 7)])
         pk.set_stream(torch.cuda.current_stream().cuda_stream)
         pk.set_design(rk)
@@ -239,8 +238,9 @@ def bench_batch(args):
            "value": nc / dt, "unit": "problems/s", "n_gpus": 1, "steps": args.steps,
            "warmup": 1, "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64 PCG + f32 multigrid", "data": "synthetic",
-           "config": {"workload": "%d cubes of seeded edges 8..48 hex8 (k^3 blocks at level 3, "
-                                  "k = 1..6 drawn from the seed), 4-level MG, PCG to 1e-8 per "
+           "config": {"workload": "%d cubes of seeded edges 8..48 hex8 (every integer equally "
+                                  "likely), 4-level MG per cube (n, ceil(n/2), ...; coarse levels "
+                                  "not nested where n has no coarsen() level), PCG to 1e-8 per "
                                   "problem" % nc,
                       "edge_counts": {str(e): sum(1 for x in edges if int(x) == e)
                                       for e in sorted(set(int(x) for x in edges))},
@@ -249,6 +249,24 @@ def bench_batch(args):
            "speedup_batched": seq_all / dt,
            "sequential_sample": "each drawn size solved alone once, times its count",
            "iters_min_max": [min(its), max(its)]}
+    if not args.no_oracle:
+        # per problem: the oracle's element-by-element true residual over that
+        # cube's dofs (cube k holds (n_k + 1)^3 consecutive nodes)
+        from oracle import pyoracle as po
+        t0 = time.perf_counter()
+        orc = po.from_forest(forests[0], [("fixed", 7)])
+        uh, fh = u.cpu().numpy(), f.cpu().numpy()
+        fr = np.where(orc.free_mask(), fh, 0.0)
+        r = fr - orc.apply_free(rho ** 3.0 + 1e-6, uh)
+        bounds = np.concatenate([[0], np.cumsum([3 * (int(e) + 1) ** 3 for e in edges])])
+        assert bounds[-1] == len(fh)
+        res = [float(np.linalg.norm(r[a:b]) / np.linalg.norm(fr[a:b]))
+               for a, b in zip(bounds[:-1], bounds[1:])]
+        out["err_vs_oracle"] = {"per_problem_true_residual_max": max(res),
+                                "per_problem_true_residual_median": float(np.median(res)),
+                                "what": "|f - K u| / |f| per cube, K the oracle's FP64 "
+                                        "element-by-element operator"}
+        log("oracle check %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(out), flush=True)
 
 

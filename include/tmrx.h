@@ -63,6 +63,11 @@ int tmrx_forest_set_name(tmrx_forest *f, int kind, int index, const char *name);
 
 /* TMROctForest::createTrees (src/TMROctForest.cpp:1744-1833). */
 int tmrx_forest_create_trees(tmrx_forest *f, int level);
+/* No counterpart in the reference (createTrees takes one depth for every
+   block): uniform trees of depth block_levels[b] in block b, so that blocks
+   of different sizes carry elements of one size (nblocks must equal the
+   number of blocks). */
+int tmrx_forest_create_trees_per_block(tmrx_forest *f, const int *block_levels, int nblocks);
 /* TMROctForest::refine (src/TMROctForest.cpp:2169-2329); refinement may be
    NULL (refine everything by one level). */
 int tmrx_forest_refine(tmrx_forest *f, const int *refinement, int min_level, int max_level);
@@ -141,6 +146,19 @@ int tmrx_forest_create_interpolation(tmrx_forest *fine, tmrx_forest *coarse, int
 int tmrx_forest_create_interpolation_device(tmrx_forest *fine, tmrx_forest *coarse, int device,
                                             int *nrows, int *ncols, const int **rowp,
                                             const int **cols, const double **weights, double *ms);
+/* No counterpart in the reference: interpolation between forests whose
+   blocks differ (coarse spaces that are not nested, for meshes with no
+   coarsen() level, e.g. cubes of 13 elements per edge under one of 7).  Each
+   fine independent node is located physically in the coarse forest (the
+   lowest-index block containing it, then its octant) and the row holds the
+   coarse element's trilinear shape functions there, hanging nodes expanded
+   as createInterpolation does (src/TMROctForest.cpp:6396-6594), merged and
+   sorted by column.  Box blocks only.  On a coarsen() pair it equals
+   tmrx_forest_create_interpolation's CSR up to rounding (1e-13).  Same
+   borrowed-array rule. */
+int tmrx_forest_create_interpolation_geometric(tmrx_forest *fine, tmrx_forest *coarse, int *nrows,
+                                               int *ncols, const int **rowp, const int **cols,
+                                               const double **weights);
 
 /* TopOptUtils.interpolateDesignVec (tmr/TopOptUtils.py:270-307): the nodal
    vector x_orig (vars_per_node values per independent node of orig_forest)

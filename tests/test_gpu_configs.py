@@ -279,16 +279,19 @@ def test_c5a_sixteen_load_cases():
 
 
 # ---------------------------------------------------------------------------
-# C5b: 96 independent cubes of seeded edges (8..48, multiples of 8)
+# C5b: 96 independent cubes of seeded edges (every integer in 8..48), each
+# with coarse levels of its own (not nested where the edge has no coarsen()
+# level)
 def test_c5b_uneven_batch_per_problem_error():
     t = time.time()
-    forests, rho, edges = workloads.cube_batch(96, level=3, nlevels=4, seed=0)
+    forests, rho, edges = workloads.cube_batch(96, nlevels=4, seed=0, device=0)
     prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
     prob.set_design(rho)
     f = prob.traction_load("load", [0.0, 0.0, -1.0])
     nc = prob.num_components()
     print("C5b built: %d problems, edges %s, %d hex (%.1f s)"
-          % (nc, np.bincount(edges)[8::8].tolist(), prob.nelem, time.time() - t), flush=True)
+          % (nc, sorted(edges), prob.nelem, time.time() - t), flush=True)
+    assert len(set(edges)) > 20 and any(e % 2 for e in edges)
     assert nc == 96
     opts = solver.default_opts(rtol=1e-8, precision=1, max_iters=300)
     u = prob.solve(f, opts=opts)
@@ -302,7 +305,7 @@ def test_c5b_uneven_batch_per_problem_error():
     r = fr - orc.apply_free(s, u)
     conn = np.asarray(forests[0].getMeshConn()).reshape(-1, 8)
     blocks = forests[0].getOctants()["block"]
-    nb = [(e // 8) ** 3 for e in edges]
+    nb = [workloads.odd_split(e)[0] ** 3 for e in edges]
     bstart = np.concatenate([[0], np.cumsum(nb)])
     comp_of_elem = np.searchsorted(bstart, blocks, side="right") - 1
     node_comp = np.full(orc.nnodes, -1)
@@ -322,8 +325,7 @@ def test_c5b_uneven_batch_per_problem_error():
     order = np.argsort(edges, kind="stable")[:12]
     worst = 0.0
     for k in order:
-        kk = int(edges[k]) // 8
-        fk, rk = workloads.box_batch([(kk, kk, kk)], 3, 1, seed=int(k))
+        fk, rk = workloads.uneven_cube_batch([int(edges[k])], nlevels=1, seed=int(k))
         ok = po.from_forest(fk[0], BCS)
         ok.assemble(rk ** Q + K0)
         fkv = workloads.face_load(fk[0], [(int(o["tag"]), int(o["info"]))

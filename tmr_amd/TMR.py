@@ -113,8 +113,15 @@ class OctForest:
     def repartition(self, max_rank=-1):
         check(lib().tmrx_forest_repartition(self.ptr), "repartition")
 
-    def createTrees(self, depth=0):
-        check(lib().tmrx_forest_create_trees(self.ptr, int(depth)), "createTrees")
+    def createTrees(self, depth=0, block_levels=None):
+        """Uniform trees of `depth`; with `block_levels` (one per block) each
+        block gets its own depth (blocks of different sizes, elements of one
+        size)."""
+        if block_levels is None:
+            check(lib().tmrx_forest_create_trees(self.ptr, int(depth)), "createTrees")
+            return
+        lv = np.ascontiguousarray(block_levels, dtype=np.intc)
+        check(lib().tmrx_forest_create_trees_per_block(self.ptr, _iptr(lv), len(lv)), "createTrees")
 
     def refine(self, refine=None, min_lev=0, max_lev=MAX_LEVEL):
         if refine is not None:
@@ -245,6 +252,25 @@ class OctForest:
                 C.byref(cl), C.byref(w), C.byref(self.last_device_ms))
             self.last_device_ms = self.last_device_ms.value
         check(st, "createInterpolation")
+        return self._csr_out(nr, nc, rp, cl, w, vec)
+
+    def createGeometricInterpolation(self, forest, vec=None):
+        """Interpolation from `forest` (coarse) to this forest by physical
+        position, for forests whose blocks differ (a coarse space that is not
+        a coarsen() of this forest).  Same return convention as
+        createInterpolation; on a coarsen() pair the two agree to 1e-13."""
+        if forest.ptr.value == self.ptr.value:
+            raise ValueError("Cannot interpolate between the same object")
+        nr, nc = C.c_int(), C.c_int()
+        rp, cl = C.POINTER(C.c_int)(), C.POINTER(C.c_int)()
+        w = C.POINTER(C.c_double)()
+        check(lib().tmrx_forest_create_interpolation_geometric(
+            self.ptr, forest.ptr, C.byref(nr), C.byref(nc), C.byref(rp), C.byref(cl),
+            C.byref(w)), "createGeometricInterpolation")
+        return self._csr_out(nr, nc, rp, cl, w, vec)
+
+    @staticmethod
+    def _csr_out(nr, nc, rp, cl, w, vec):
         rowp = np.ctypeslib.as_array(rp, (nr.value + 1,)).copy()
         nnz = int(rowp[-1])
         cols = np.ctypeslib.as_array(cl, (nnz,)).copy() if nnz else np.zeros(0, np.intc)
@@ -276,18 +302,13 @@ class OctForest:
 def box_connectivity(nx, ny, nz, L=(1.0, 1.0, 1.0)):
     """Block connectivity and vertex points of an nx x ny x nz lattice of box
     blocks covering [0,Lx]x[0,Ly]x[0,Lz]; corner ordering i + 2j + 4k."""
-    vid = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
-    conn = []
-    for k in range(nz):
-        for j in range(ny):
-            for i in range(nx):
-                conn.append([vid(i + a, j + b, k + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)])
-    X = np.zeros(((nx + 1) * (ny + 1) * (nz + 1), 3))
-    for k in range(nz + 1):
-        for j in range(ny + 1):
-            for i in range(nx + 1):
-                X[vid(i, j, k)] = (L[0] * i / nx, L[1] * j / ny, L[2] * k / nz)
-    return np.array(conn, dtype=np.intc), X
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()  # block i + nx (j + ny k)
+    vid = lambda a, b, c: (i + a) + (nx + 1) * ((j + b) + (ny + 1) * (k + c))
+    conn = np.stack([vid(a, b, c) for c in (0, 1) for b in (0, 1) for a in (0, 1)], axis=1)
+    vk, vj, vi = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    X = np.stack([L[0] * vi.ravel() / nx, L[1] * vj.ravel() / ny, L[2] * vk.ravel() / nz], axis=1)
+    return np.ascontiguousarray(conn, dtype=np.intc), np.ascontiguousarray(X, dtype=float)
 
 
 def createBoxForest(nx, ny, nz, L=(1.0, 1.0, 1.0), face_names=None, comm=None):

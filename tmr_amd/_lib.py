@@ -58,6 +58,7 @@ _SIGS = {
     "tmrx_forest_set_node_points": (I, [P, DP]),
     "tmrx_forest_set_name": (I, [P, I, I, S]),
     "tmrx_forest_create_trees": (I, [P, I]),
+    "tmrx_forest_create_trees_per_block": (I, [P, IP, I]),
     "tmrx_forest_refine": (I, [P, IP, I, I]),
     "tmrx_forest_balance": (I, [P, I]),
     "tmrx_forest_write_vtk": (I, [P, C.c_char_p, P, I, C.c_char_p, P, I, C.c_char_p]),
@@ -79,6 +80,7 @@ _SIGS = {
     "tmrx_forest_get_octs_with_name": (I, [P, S, C.POINTER(Octant), I]),
     "tmrx_forest_create_interpolation": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_forest_create_interpolation_device": (I, [P, P, I, IP, IP, CIP, CIP, CDP, DP]),
+    "tmrx_forest_create_interpolation_geometric": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_forest_element_center_weights": (I, [P, P, IP, IP, CIP, CIP, CDP]),
     "tmrx_forest_interpolate_vec": (I, [P, P, I, P, P, I]),
     "tmrx_forest_generate_bin_file": (I, [P, S, P, I, I, D, IP]),

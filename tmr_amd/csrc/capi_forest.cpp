@@ -93,6 +93,15 @@ int tmrx_forest_create_trees(tmrx_forest *f, int level) {
   return f->f->create_trees(level) ? fail_from(f) : 0;
 }
 
+int tmrx_forest_create_trees_per_block(tmrx_forest *f, const int *block_levels, int nblocks) {
+  CHECK_HANDLE(f);
+  if (!block_levels || nblocks != f->f->conn_data().num_blocks) {
+    set_error("tmrx_forest_create_trees_per_block: need one level per block");
+    return -1;
+  }
+  return f->f->create_trees(0, block_levels) ? fail_from(f) : 0;
+}
+
 int tmrx_forest_refine(tmrx_forest *f, const int *refinement, int min_level, int max_level) {
   CHECK_HANDLE(f);
   return f->f->refine(refinement, min_level, max_level) ? fail_from(f) : 0;
@@ -257,6 +266,20 @@ int tmrx_forest_create_interpolation_device(tmrx_forest *fine, tmrx_forest *coar
   CHECK_HANDLE(fine);
   CHECK_HANDLE(coarse);
   if (fine->f->create_interpolation_device(coarse->f, fine->P, device, ms)) return fail_from(fine);
+  if (nrows) *nrows = fine->P.nrows;
+  if (ncols) *ncols = fine->P.ncols;
+  if (rowp) *rowp = fine->P.rowp.data();
+  if (cols) *cols = fine->P.cols.data();
+  if (weights) *weights = fine->P.w.data();
+  return 0;
+}
+
+int tmrx_forest_create_interpolation_geometric(tmrx_forest *fine, tmrx_forest *coarse, int *nrows,
+                                               int *ncols, const int **rowp, const int **cols,
+                                               const double **weights) {
+  CHECK_HANDLE(fine);
+  CHECK_HANDLE(coarse);
+  if (fine->f->create_interpolation_geometric(coarse->f, fine->P)) return fail_from(fine);
   if (nrows) *nrows = fine->P.nrows;
   if (ncols) *ncols = fine->P.ncols;
   if (rowp) *rowp = fine->P.rowp.data();

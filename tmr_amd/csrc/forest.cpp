@@ -457,29 +457,34 @@ static inline void get_face_node_coords(int face_id, int32_t hm, int32_t x, int3
 
 // ---------------------------------------------------------------------------
 // createTrees (src/TMROctForest.cpp:1744-1833), single process
-int Forest::create_trees(int level) {
+int Forest::create_trees(int level, const int *block_levels) {
   if (bd_.num_blocks == 0) {
     last_error = "create_trees: set the connectivity first";
     return -1;
   }
-  if (level < 0) level = 0;
-  if (level >= MAX_LEVEL) level = MAX_LEVEL - 1;
+  auto clamp_level = [](int l) { return l < 0 ? 0 : (l >= MAX_LEVEL ? MAX_LEVEL - 1 : l); };
   free_mesh();
-  const int32_t h = 1 << (MAX_LEVEL - level);
-  const int64_t n = int64_t(1) << level;
+  size_t total = 0;
+  for (int b = 0; b < bd_.num_blocks; b++) {
+    const int lb = clamp_level(block_levels ? block_levels[b] : level);
+    total += (size_t)1 << (3 * lb);
+  }
+  octs_.resize(total);
   // Morton index m of a block's octant: bit 3k + 2 of m is bit k of x / h,
   // bit 3k + 1 of y, bit 3k of z (the key's x > y > z priority), so the
   // octants come out already in key order
-  const int64_t nb = n * n * n;
-  octs_.resize((size_t)(nb * bd_.num_blocks));
-  for (int b = 0; b < bd_.num_blocks; b++)
-    for (int64_t m = 0; m < nb; m++) {
+  size_t i = 0;
+  for (int b = 0; b < bd_.num_blocks; b++) {
+    const int lb = clamp_level(block_levels ? block_levels[b] : level);
+    const int32_t h = 1 << (MAX_LEVEL - lb);
+    const int64_t nb = int64_t(1) << (3 * lb);
+    for (int64_t m = 0; m < nb; m++, i++) {
       int32_t c[3] = {0, 0, 0};
-      for (int k = 0; k < level; k++)
+      for (int k = 0; k < lb; k++)
         for (int a = 0; a < 3; a++) c[a] |= (int32_t)((m >> (3 * k + 2 - a)) & 1) << k;
-      const size_t i = (size_t)(b * nb + m);
-      octs_[i] = {b, c[0] * h, c[1] * h, c[2] * h, (int32_t)i, (int16_t)level, 0};
+      octs_[i] = {b, c[0] * h, c[1] * h, c[2] * h, (int32_t)i, (int16_t)lb, 0};
     }
+  }
   reindex(octs_, okey_);
   return 0;
 }

@@ -72,7 +72,8 @@ class Forest {
   const BlockConn &conn_data() const { return bd_; }
 
   // octree operations ------------------------------------------------------
-  int create_trees(int level);
+  // uniform trees of depth `level`, or of depth block_levels[b] in block b
+  int create_trees(int level, const int *block_levels = nullptr);
   int refine(const int *refinement, int min_level, int max_level);
   int balance(int balance_corner);
   // the same balance on GPU `device`, bit for bit (octants, tags, order)
@@ -118,6 +119,21 @@ class Forest {
   // physical size of an element along each axis (box blocks only)
   void element_extent(const Oct &o, double h[3]) const;
   bool blocks_are_boxes() const;
+
+  // Coarse spaces that are not nested (box blocks only; nonnested.cpp).
+  // True if both forests have the same blocks (connectivity and vertex
+  // points), i.e. one can be a coarsen() of the other.
+  bool same_blocks(const Forest *o) const;
+  // Trilinear interpolation from `coarse` to this forest's independent nodes
+  // by physical position: each fine node is located in the coarse forest
+  // (lowest-index block containing it, then the octant), and the coarse
+  // element's shape functions there give the row, coarse hanging nodes
+  // expanded, merged and sorted by column.  The forests may have unrelated
+  // blocks over the same region.
+  int create_interpolation_geometric(Forest *coarse, InterpCSR &P);
+  // index of the coarse element containing each element centre of this
+  // forest (-1: outside the coarse forest)
+  int locate_centres(Forest *coarse, std::vector<int> &owner);
 
   std::string last_error;
 

@@ -132,32 +132,45 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     levels_[l].forest = f;
   }
   if (find_components()) return -1;
-  // stiffness classes: blocks with the same box extents share K0
-  Forest *f0 = forests[0];
-  const BlockConn &bd = f0->conn_data();
+  // stiffness classes: blocks with the same box extents share K0 (over
+  // every level: the levels of a hierarchy that is not nested have blocks
+  // of their own)
   std::vector<std::array<double, 3>> ext;
-  std::vector<int> block_cls(bd.num_blocks);
-  for (int b = 0; b < bd.num_blocks; b++) {
-    Oct root{b, 0, 0, 0, 0, 0, 0};
-    double h[3];
-    f0->element_extent(root, h);
-    int c = -1;
-    for (size_t k = 0; k < ext.size(); k++) {
-      bool eq = true;
-      for (int a = 0; a < 3; a++)
-        if (std::fabs(ext[k][a] - h[a]) > 1e-12 * std::fabs(h[a])) eq = false;
-      if (eq) {
-        c = (int)k;
-        break;
+  std::vector<std::vector<int>> block_cls(nlevels);
+  for (int l = 0; l < nlevels; l++) {
+    Forest *fl = forests[l];
+    const BlockConn &bd = fl->conn_data();
+    if (l > 0 && fl->same_blocks(forests[l - 1])) {
+      block_cls[l] = block_cls[l - 1];
+      continue;
+    }
+    block_cls[l].assign(bd.num_blocks, 0);
+    for (int b = 0; b < bd.num_blocks; b++) {
+      Oct root{b, 0, 0, 0, 0, 0, 0};
+      double h[3];
+      fl->element_extent(root, h);
+      int c = -1;
+      for (size_t k = 0; k < ext.size(); k++) {
+        bool eq = true;
+        for (int a = 0; a < 3; a++)
+          if (std::fabs(ext[k][a] - h[a]) > 1e-12 * std::fabs(h[a])) eq = false;
+        if (eq) {
+          c = (int)k;
+          break;
+        }
       }
+      if (c < 0) {
+        c = (int)ext.size();
+        ext.push_back({h[0], h[1], h[2]});
+      }
+      block_cls[l][b] = c;
     }
-    if (c < 0) {
-      c = (int)ext.size();
-      ext.push_back({h[0], h[1], h[2]});
-    }
-    block_cls[b] = c;
   }
   nclasses_ = (int)ext.size();
+  if (nclasses_ > 256) {  // element records carry the class in one byte
+    last_error = "problem: more than 256 distinct block extents";
+    return -1;
+  }
   double C[36];
   isotropic_C(mat.E, mat.nu, C);
   K0h_.assign(576 * (size_t)nclasses_, 0.0);
@@ -292,7 +305,7 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
     Level &L = levels_[l];
     const auto &octs = L.forest->octants();
     L.cls.resize(octs.size());
-    for (size_t e = 0; e < octs.size(); e++) L.cls[e] = block_cls[octs[e].block];
+    for (size_t e = 0; e < octs.size(); e++) L.cls[e] = block_cls[l][octs[e].block];
     if (build_level(l, nbcs, names, dof_masks)) return -1;
     if (l > 0) L.tile_nodes = coarse_tile_nodes(levels_[0].tile_nodes, L.n_nodes);
     if (build_tiles(l)) return -1;
@@ -838,7 +851,9 @@ int Problem::build_transfer(int l) {
   InterpCSR P;
   int dev = 0;
   (void)hipGetDevice(&dev);
-  if (F.forest->create_interpolation_device(Cc.forest, P, dev)) {
+  F.nested = F.forest->same_blocks(Cc.forest);
+  if (F.nested ? F.forest->create_interpolation_device(Cc.forest, P, dev)
+               : F.forest->create_interpolation_geometric(Cc.forest, P)) {
     last_error = "problem: interpolation " + std::to_string(l) + ": " + F.forest->last_error;
     return -1;
   }
@@ -875,23 +890,38 @@ int Problem::build_transfer(int l) {
         rw[fill[c]++] = (float)P.w[j];
       }
   }
-  // element containment: each fine element lies in one coarse element
+  // element containment: each fine element lies in one coarse element.
+  // Without nesting, each fine element belongs to the coarse element holding
+  // its centre, and a coarse element's material factor is the volume
+  // average over the fine elements it holds.
   const auto &fo = F.forest->octants();
   std::vector<int> owner(F.n_elem);
   std::vector<int> cnt(Cc.n_elem + 1, 0);
-  for (int e = 0; e < F.n_elem; e++) {
-    const int E = Cc.forest->locate(fo[e]);
-    if (E < 0 || Cc.lev[E] > F.lev[e]) {
-      last_error = "problem: coarse forest does not cover the fine forest";
+  if (F.nested) {
+    for (int e = 0; e < F.n_elem; e++) {
+      const int E = Cc.forest->locate(fo[e]);
+      if (E < 0 || Cc.lev[E] > F.lev[e]) {
+        last_error = "problem: coarse forest does not cover the fine forest";
+        return -1;
+      }
+      owner[e] = E;
+    }
+  } else {
+    if (F.forest->locate_centres(Cc.forest, owner)) {
+      last_error = "problem: " + F.forest->last_error;
       return -1;
     }
-    owner[e] = E;
-    cnt[E + 1]++;
+    for (int e = 0; e < F.n_elem; e++)
+      if (owner[e] < 0) {
+        last_error = "problem: coarse forest does not cover the fine forest";
+        return -1;
+      }
   }
+  for (int e = 0; e < F.n_elem; e++) cnt[owner[e] + 1]++;
   for (int i = 0; i < Cc.n_elem; i++) cnt[i + 1] += cnt[i];
-  std::vector<int> child(F.n_elem), place(F.n_elem);
+  std::vector<int> child(F.n_elem), place(F.n_elem, 0);
   std::vector<float> vw(F.n_elem);
-  {
+  if (F.nested) {
     const auto &co = Cc.forest->octants();
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     for (int e = 0; e < F.n_elem; e++) {
@@ -908,6 +938,26 @@ int Problem::build_transfer(int l) {
       place[fill[E]] = (dl << 24) | (oz << 16) | (oy << 8) | ox;
       child[fill[E]] = e;
       vw[fill[E]++] = (float)std::ldexp(1.0, -3 * dl);
+    }
+  } else {
+    std::vector<double> vol(F.n_elem), tot(Cc.n_elem, 0.0);
+    for (int e = 0; e < F.n_elem; e++) {
+      double h[3];
+      F.forest->element_extent(fo[e], h);
+      vol[e] = h[0] * h[1] * h[2];
+      tot[owner[e]] += vol[e];
+    }
+    for (int E = 0; E < Cc.n_elem; E++)
+      if (!(tot[E] > 0.0)) {
+        last_error = "problem: a coarse element holds no fine element centre (coarse elements "
+                     "smaller than the fine ones)";
+        return -1;
+      }
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < F.n_elem; e++) {
+      const int E = owner[e];
+      child[fill[E]] = e;
+      vw[fill[E]++] = (float)(vol[e] / tot[E]);
     }
   }
   F.sc_ptr_h = cnt;
@@ -1213,9 +1263,10 @@ int Problem::estimate_eigs(int l, const SolveOpts &o) {
 // Blocks that share no vertex, directly or through other blocks, are
 // independent problems solved side by side.  Each component must be a
 // contiguous range of block indices, so that its nodes (numbered block by
-// block) are a contiguous range on every level.
-int Problem::find_components() {
-  const BlockConn &bd = levels_[0].forest->conn_data();
+// block) are a contiguous range on every level.  A level with blocks of its
+// own (a hierarchy that is not nested) must hold the same components in the
+// same order.
+static int block_components(const BlockConn &bd, std::vector<int> &seg0, std::string &err) {
   const int nb = bd.num_blocks;
   std::vector<int> parent(nb);
   for (int b = 0; b < nb; b++) parent[b] = b;
@@ -1234,26 +1285,41 @@ int Problem::find_components() {
         if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
       }
     }
-  seg_block0_.assign(1, 0);
+  seg0.assign(1, 0);
   std::vector<int> comp(nb);
   for (int b = 0; b < nb; b++) comp[b] = root(b);
   for (int b = 1; b < nb; b++) {
     if (comp[b] != comp[b - 1]) {
-      for (int c : seg_block0_)
+      for (int c : seg0)
         if (comp[c] == comp[b]) {
-          last_error = "problem: each independent component must be a contiguous block range";
+          err = "problem: each independent component must be a contiguous block range";
           return -1;
         }
-      seg_block0_.push_back(b);
+      seg0.push_back(b);
     }
   }
-  seg_block0_.push_back(nb);
+  seg0.push_back(nb);
+  return 0;
+}
+
+int Problem::find_components() {
+  if (block_components(levels_[0].forest->conn_data(), seg_block0_, last_error)) return -1;
   nseg_ = (int)seg_block0_.size() - 1;
-  std::vector<int> bseg(nb);
-  for (int s = 0; s < nseg_; s++)
-    for (int b = seg_block0_[s]; b < seg_block0_[s + 1]; b++) bseg[b] = s;
-  for (auto &L : levels_) {
+  std::vector<int> seg0 = seg_block0_;
+  for (size_t l = 0; l < levels_.size(); l++) {
+    Level &L = levels_[l];
     Forest *f = L.forest;
+    if (l > 0 && !f->same_blocks(levels_[l - 1].forest)) {
+      if (block_components(f->conn_data(), seg0, last_error)) return -1;
+      if ((int)seg0.size() != nseg_ + 1) {
+        last_error = "problem: level " + std::to_string(l) +
+                     " holds a different number of independent components";
+        return -1;
+      }
+    }
+    std::vector<int> bseg(f->conn_data().num_blocks);
+    for (int s = 0; s < nseg_; s++)
+      for (int b = seg0[s]; b < seg0[s + 1]; b++) bseg[b] = s;
     const auto &octs = f->octants();
     const auto &conn = f->conn();
     const int n = f->num_owned_nodes();

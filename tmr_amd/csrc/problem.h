@@ -115,6 +115,9 @@ struct LevelVecs {
 
 struct Level {
   Forest *forest = nullptr;
+  // the next coarser level is a coarsen() of this one (same blocks); else
+  // its transfers come from physical positions (no element containment)
+  bool nested = true;
   int n_nodes = 0, n_dep = 0, n_elem = 0;
   int64_t ndof = 0;
   // independent components (separate problems sharing the hierarchy):

@@ -86,6 +86,11 @@ int Problem::build_galerkin() {
   const int nl = (int)levels_.size();
   for (int l = 0; l + 1 < nl; l++) {
     Level &F = levels_[l], &Cc = levels_[l + 1];
+    if (!F.nested) {  // the element-wise product needs coarse elements that hold fine ones
+      last_error = "galerkin: level " + std::to_string(l + 1) +
+                   " is not a coarsen() of the level above; Galerkin coarse operators need nested forests";
+      return -1;
+    }
     InterpCSR P;
     int dev = 0;
     (void)hipGetDevice(&dev);

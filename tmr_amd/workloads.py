@@ -236,11 +236,86 @@ def patch_loads(forest, count, seed, L=(2.0, 1.0, 1.0), names=("x+", "y-", "y+",
     return np.array(F), params
 
 
-def c5b_edges(count=96, seed=0):
-    """Edge lengths (elements) of the uneven batch, drawn from the seed among
-    8, 16, ..., 48: every cube is k x k x k blocks at level 3 (edge 8k), so
-    each has a four-level nested hierarchy down to k^3 level-0 blocks."""
-    return 8 * np.random.default_rng(seed + 77).integers(1, 7, size=count)
+def c5b_edges(count=96, seed=0, lo=8, hi=48):
+    """Edge lengths (elements) of the uneven batch: every integer in
+    [lo, hi] equally likely, drawn from the seed."""
+    return np.random.default_rng(seed + 77).integers(lo, hi + 1, size=count)
+
+
+def level_edges(n, nlevels):
+    """Elements per edge on each level of a cube of n elements: n, then
+    halved and rounded up (13, 7, 4, 2)."""
+    ms = [int(n)]
+    for _ in range(nlevels - 1):
+        ms.append(max(1, (ms[-1] + 1) // 2))
+    return ms
+
+
+def odd_split(m):
+    """(b, L) with m = b 2^L and b odd: a lattice of b^3 blocks at depth L
+    meshes a cube with m elements per edge."""
+    m, L = int(m), 0
+    while m % 2 == 0:
+        m //= 2
+        L += 1
+    return m, L
+
+
+def uneven_cube_batch(edges, nlevels=4, h=0.125, gap=1.0, seed=0, passes=2, device=None):
+    """Independent cantilever cubes of `edges` elements per edge (elements of
+    size h on the finest level), side by side along x, each clamped on its
+    x- face ("fixed") and loaded on its x+ face ("load").  Level l meshes a
+    cube of n with level_edges(n)[l] elements per edge as a lattice of b^3
+    blocks at depth L (odd_split), a forest of its own per level: an edge
+    such as 13 has no coarsen() level (13 level-0 blocks), and its coarser
+    levels (7, 4, 2 elements) are not nested in it; the problem builds their
+    transfers from physical positions.  Returns (forests, rho) with seeded
+    smoothed densities (seed + k for cube k)."""
+    edges = [int(n) for n in edges]
+    forests = []
+    for l in range(nlevels):
+        conns, Xs, names, blev = [], [], [], []
+        off_v, off_b, x0 = 0, 0, 0.0
+        for n in edges:
+            b, L = odd_split(level_edges(n, nlevels)[l])
+            E = n * h
+            conn, X = TMR.box_connectivity(b, b, b, L=(E, E, E))
+            X[:, 0] += x0
+            conns.append(conn + off_v)
+            Xs.append(X)
+            q = np.arange(len(conn))
+            names += [(off_b + int(v), 0, "fixed") for v in q[q % b == 0]]
+            names += [(off_b + int(v), 1, "load") for v in q[q % b == b - 1]]
+            blev.append(np.full(len(conn), L, dtype=np.intc))
+            off_v += len(X)
+            off_b += len(conn)
+            x0 += E + gap
+        f = TMR.OctForest()
+        f.setConnectivity(np.vstack(conns).astype(np.intc))
+        f.setNodePoints(np.vstack(Xs))
+        bfc = f.getConnectivity()["block_face_conn"]
+        for q, fi, name in names:
+            f.setName("face", int(bfc[q, fi]), name)
+        f.createTrees(block_levels=np.concatenate(blev))
+        forests.append(f)
+    for f in forests:
+        f.createNodes(device=device)
+    octs = forests[0].getOctants()
+    rho = np.empty(len(octs))
+    off_b = 0
+    for k, n in enumerate(edges):
+        b, L = odd_split(n)
+        nb = b ** 3
+        sel = (octs["block"] >= off_b) & (octs["block"] < off_b + nb)
+        grid = smoothed_density((n, n, n), seed + k, passes=passes)
+        q = octs["block"][sel] - off_b
+        hh = 1 << (30 - L)
+        ix = (q % b) * (1 << L) + octs["x"][sel] // hh
+        iy = ((q // b) % b) * (1 << L) + octs["y"][sel] // hh
+        iz = (q // (b * b)) * (1 << L) + octs["z"][sel] // hh
+        rho[sel] = grid[iz, iy, ix]
+        off_b += nb
+    return forests, rho
 
 
 def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
@@ -292,10 +367,10 @@ def box_batch(dims_list, level, nlevels, h=1.0, gap=1.0, seed=0, passes=2):
     return forests, rho
 
 
-def cube_batch(count=96, level=3, nlevels=4, seed=0):
-    """`count` cubes with seeded edges 8k elements (c5b_edges: k x k x k
-    blocks at level 3), the uneven-size batch of independent problems;
-    returns (forests, rho, edges)."""
-    ks = [int(n) // 8 for n in c5b_edges(count, seed)]
-    forests, rho = box_batch([(k, k, k) for k in ks], level, nlevels, seed=seed)
-    return forests, rho, [8 * k for k in ks]
+def cube_batch(count=96, nlevels=4, seed=0, device=None):
+    """The uneven-size batch of independent problems: `count` cubes with
+    edges drawn from the seed in 8..48 (c5b_edges), each with a hierarchy
+    of its own (uneven_cube_batch); returns (forests, rho, edges)."""
+    edges = [int(n) for n in c5b_edges(count, seed)]
+    forests, rho = uneven_cube_batch(edges, nlevels, seed=seed, device=device)
+    return forests, rho, edges
