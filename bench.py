@@ -449,6 +449,9 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=5)
     ap.add_argument("--fp32-valu", action="store_true",
                     help="FP32 operator on the vector ALUs instead of the matrix cores")
+    ap.add_argument("--fp32-form", type=int, choices=(0, 1, 2), default=None,
+                    help="pin the FP32 operator form after the autotuner: 0 vector ALUs, 1 matrix "
+                         "cores (bf16 split), 2 matrix cores (FP32 operands)")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue every PCG iteration instead of replaying HIP graphs")
     ap.add_argument("--no-autotune", action="store_true",
@@ -504,8 +507,12 @@ def main():
     if not args.no_autotune:
         t0 = time.perf_counter()
         tune = prob.autotune(args.tiles, reps=3)
-        log("autotuned owner tiles: %d nodes (ms per 3 FP32 Chebyshev steps + FP32 and FP64 residuals: %s) in %.1f s"
-            % (tune[0], tune[1], time.perf_counter() - t0))
+        log("autotuned owner tiles: %d nodes, FP32 operator on the %s (ms per 3 FP32 Chebyshev "
+            "steps + FP32 and FP64 residuals: %s) in %.1f s"
+            % (tune[0], solver.Problem.FP32_FORMS[prob.tuning()[1]], tune[1],
+               time.perf_counter() - t0))
+    if args.fp32_form is not None:
+        prob.set_fp32_form(args.fp32_form)
 
     def run(precision, steps, warmup, timing=False):
         opts = solver.default_opts(rtol=1e-8, precision=precision, max_iters=500,
@@ -561,7 +568,7 @@ def main():
     nnodes = ndof // 3
     bytes_launch = int(round(CHEB_BYTES_PER_NODE * nnodes + CHEB_BYTES_PER_ELEM * prob.nelem))
     achieved = bytes_launch / avg_s / 1e9
-    form = "valu" if (args.fp32_valu or not prob.tuning()[1]) else "mfma"
+    form = ("valu", "mfma", "mfma32")[0 if args.fp32_valu else prob.tuning()[1]]
     traffic = None
     pmc = MEASURED_TRAFFIC_KIB.get((form, prob.tuning()[0])) if args.level == 7 else None
     src = kernel_source_hash()
@@ -593,12 +600,15 @@ def main():
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
         "tile_nodes": prob.tuning()[0],
-        "fp32_operator": "vector ALUs" if form == "valu" else "matrix cores",
+        "fp32_operator": {"valu": "vector ALUs", "mfma": "matrix cores (bf16 three-way split)",
+                          "mfma32": "matrix cores (FP32 operands)"}[form],
         "hip_graphs": not args.no_graph,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_tile_apply<float, ASM_CHEB>" if form == "valu" else
-                                "k_tile_apply_mfma<ASM_CHEB>") + " (finest-level Chebyshev step)",
+                     "kernel": {"valu": "k_tile_apply<float, ASM_CHEB>",
+                                "mfma": "k_tile_apply_mfma<ASM_CHEB>",
+                                "mfma32": "k_tile_apply_mfma32<ASM_CHEB>"}[form]
+                               + " (finest-level Chebyshev step)",
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE measured on kernels.hip "
                                        "%s; this run's kernels.hip %s" % (MEASURED_TRAFFIC_SRC, src),
                      "algorithmic_bytes_per_launch": bytes_launch,

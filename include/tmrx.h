@@ -394,14 +394,19 @@ int tmrx_problem_solve_multi(tmrx_problem *p, int nrhs, const double *f, double 
    the last solve: iterations, converged flag and the true relative residual
    |f - K u| / |f| over its dofs. */
 int tmrx_problem_num_components(tmrx_problem *p, int *ncomp);
-/* The tuned choices: finest-level tile size and whether the FP32 operator
-   runs on the matrix cores (1) or the vector ALUs (0). */
+/* The tuned choices: finest-level tile size and the FP32 operator form:
+   0 vector ALUs, 1 matrix cores with bf16 three-way split operands, 2
+   matrix cores with FP32 operands (non-zero: on the matrix cores). */
 int tmrx_problem_tuning(tmrx_problem *p, int *tile_nodes, int *fp32_matrix_cores);
+/* No counterpart in the reference: pin the FP32 operator form (0, 1, 2 as
+   above) until the next autotune. */
+int tmrx_problem_set_fp32_form(tmrx_problem *p, int form);
 /* Operator autotuner (no counterpart in the reference): rebuilds the owner
    tiles for each candidate size (independent nodes per finest-level tile,
    64..4096; coarse levels shrink theirs to keep >= 2048 tiles), times the
-   finest-level FP32 operator on the matrix cores and on the vector ALUs and
-   the FP64 operator, and keeps the size and FP32 form with the least
+   finest-level FP32 operator in its three forms (vector ALUs, matrix cores
+   with bf16-split or FP32 operands) and the FP64 operator, and keeps the
+   size and FP32 form with the least
    4 x FP32 + 1 x FP64 time (one V-cycle's and one PCG iteration's share).
    ms[n] receives that time per candidate (-1: does not fit in LDS; may be
    NULL).  Needs a design; invalidates the current solution. */

@@ -62,6 +62,38 @@ def test_apply_parity(which, cube, adapt):
     assert e32 < 1e-4
 
 
+@pytest.mark.parametrize("which", ["cube", "adapt"])
+def test_fp32_matrix_cores_fp32_operands(which, cube, adapt):
+    """The FP32 operator on the matrix cores with FP32 operands
+    (v_mfma_f32_16x16x4_f32): the product against the oracle's FP64
+    operator at the FP32 vector-ALU bound, and a mixed-precision solve
+    through it (every epilogue: Chebyshev step, residual, product)."""
+    forests, rho, prob = cube if which == "cube" else adapt
+    orc = _oracle(forests[0], rho)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(prob.ndof)
+    y_ref = orc.matvec(x)
+    keep = prob.tuning()[1]
+    prob.set_fp32_form(2)
+    try:
+        assert prob.tuning()[1] == 2
+        y = prob.apply(x, precision=1)
+        e = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+        f = _load(prob)
+        u_ref = orc.solve(f, rtol=1e-13)
+        c_ref = float(f @ u_ref)
+        u = prob.solve(f, opts=solver.default_opts(rtol=1e-10, precision=1))
+        c, tr, it = prob.compliance(), prob.true_residual(), prob.info.iters
+    finally:
+        prob.set_fp32_form(keep)
+    print("%s: FP32-operand matrix-core product rel err %.3e; solve %d iterations, true res "
+          "%.3e, compliance rel err %.2e" % (which, e, it, tr, abs(c - c_ref) / abs(c_ref)),
+          flush=True)
+    assert e < 1e-5
+    assert tr < 1e-8 and abs(c - c_ref) / abs(c_ref) < 1e-6
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-6
+
+
 @pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("which", ["cube", "adapt"])
 def test_solve_compliance_parity(which, precision, cube, adapt):

@@ -271,8 +271,13 @@ int tmrx_problem_autotune(tmrx_problem *h, const int *tile_sizes, int n, int rep
 int tmrx_problem_tuning(tmrx_problem *h, int *tile_nodes, int *fp32_matrix_cores) {
   CHECK_P(h);
   if (tile_nodes) *tile_nodes = h->p->tile_nodes();
-  if (fp32_matrix_cores) *fp32_matrix_cores = h->p->fp32_matrix_cores() ? 1 : 0;
+  if (fp32_matrix_cores) *fp32_matrix_cores = h->p->fp32_form();
   return 0;
+}
+
+int tmrx_problem_set_fp32_form(tmrx_problem *h, int form) {
+  CHECK_P(h);
+  return h->p->set_fp32_form(form) ? fail(h) : 0;
 }
 
 int tmrx_mg_create(tmrx_problem *h, double omega, int use_galerkin, int use_coarse_direct_solve,

@@ -104,7 +104,10 @@ struct Tiles {
   const int *mel_elem = nullptr;  // [16 * batches] element of each record
   int n_mrec = 0;
   const uint4 *afrag = nullptr;   // [n_classes][hi, lo][row block][64 lanes]
+  // FP32-input matrix-core form: K0 fragments [n_classes][row block][step][64]
+  const float *afrag32 = nullptr;
   bool use_mfma = false;
+  bool mfma_f32 = false;  // matrix cores with FP32 inputs (else bf16 three-way split)
   // nodal design: the element stiffness is sum_g s_eg K_g over the 8 Gauss
   // points (K0 holds the gradient tables, s_elem 8 scales per element)
   bool nodal = false;

@@ -1962,6 +1962,171 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma(
       b, y, x_out, d, dinv, ca, cc, cflags, partials, red, np);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 owner-tile apply on the matrix cores with FP32 inputs
+// (v_mfma_f32_16x16x4_f32): x staged once, 3 floats per slot, and K0 as FP32
+// A fragments; Y (32 x 16) = K0 (32 x 24) X (24 x 16) in six K-steps of 4
+// per row block.  K index d = 4 s + g (step s, lane group g = lane / 16) is
+// dof d of element n = lane % 16 (corner d / 3, component d % 3); output row
+// 4 a + j is component j of corner a (j = 3 pads: zero rows of A), so row
+// block b leaves lane group g the three components of corner 4 b + g.
+// Batches, wave ownership of the accumulators, record ring and epilogues are
+// those of k_tile_apply_mfma; products and sums are FP32 throughout.
+template <int MODE, bool SINGLE_CLASS>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile_apply_mfma32(
+    const int *__restrict__ node0, const int *__restrict__ ext_ptr, const int *__restrict__ ext,
+    const int *__restrict__ nacc_dep, const int *__restrict__ mb_ptr,
+    const uint32_t *__restrict__ mel, const float *__restrict__ afrag32,
+    const int *__restrict__ comb_ptr, const int *__restrict__ comb_slot,
+    const float *__restrict__ comb_w, const int *__restrict__ dep_ptr,
+    const int *__restrict__ dep_conn, const float *__restrict__ dep_w,
+    const uint8_t *__restrict__ bcmask, const float *__restrict__ x, const float *__restrict__ b,
+    float *__restrict__ y, float *__restrict__ x_out, float *__restrict__ d,
+    const float *__restrict__ dinv, double ca, double cc, int cflags,
+    double *__restrict__ partials, int64_t vs, const int *__restrict__ run) {
+  if (run && *run == 0) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[TILE_THREADS / 64];
+  const int t = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  {  // right-hand side blockIdx.y of a batch of vectors, stride vs
+    const int64_t off = (int64_t)blockIdx.y * vs;
+    x += off;
+    if (b) b += off;
+    if (y) y += off;
+    if (x_out) x_out += off;
+    if (d) d += off;
+    if (partials) partials += (int64_t)blockIdx.y * gridDim.x;
+  }
+  const int en = lane & 15, g = lane >> 4;
+  const int n0 = node0[t], nown = node0[t + 1] - n0;
+  const int e0 = ext_ptr[t], next = ext_ptr[t + 1] - e0;
+  const int nslot = nown + next;
+  const int nacc = nown + nacc_dep[t];
+  float *xs = reinterpret_cast<float *>(smem_raw);
+  float *ys = xs + 3 * ((nslot + 7) & ~7);
+  float4 *ys4 = reinterpret_cast<float4 *>(ys);
+  const int *cp = mb_ptr + (size_t)t * (MFMA_WAVES + 1);
+  const int bb0 = __builtin_amdgcn_readfirstlane(cp[wv]);
+  const int bb1 = __builtin_amdgcn_readfirstlane(cp[wv + 1]);
+  // record words: 0-3 corner positions (two per word), 4 FP32 stiffness
+  // scale, 5 mask | colour << 8 | class << 16
+  struct Rec {
+    uint32_t p0, p1, p2, p3, m;
+    float s;
+  };
+  auto load_rec = [&](int bt, Rec &r) {
+    const uint2 *rec = reinterpret_cast<const uint2 *>(mel + 6 * ((size_t)bt * 16 + en));
+    const uint2 a = rec[0], c = rec[1], e = rec[2];
+    r.p0 = a.x;
+    r.p1 = a.y;
+    r.p2 = c.x;
+    r.p3 = c.y;
+    r.s = __uint_as_float(e.x);
+    r.m = e.y;
+  };
+  Rec r0{0u, 0u, 0u, 0u, 0u, 0.0f}, r1 = r0, r2 = r0, r3 = r0;
+  if (bb0 < bb1) load_rec(bb0, r0);
+  if (bb0 + 1 < bb1) load_rec(bb0 + 1, r1);
+  if (bb0 + 2 < bb1) load_rec(bb0 + 2, r2);
+  tile_stage<float, 3, TILE_THREADS, 4>(n0, nown, ext + e0, next, nacc, dep_ptr, dep_conn, dep_w,
+                                        bcmask, x, xs, ys);
+  // A fragments: [class][row block][step][lane]
+  float a[2][6];
+  if (SINGLE_CLASS) {
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+#pragma unroll
+      for (int s = 0; s < 6; s++) a[blk][s] = afrag32[(blk * 6 + s) * 64 + lane];
+#pragma unroll
+    for (int blk = 0; blk < 2; blk++)
+#pragma unroll
+      for (int s = 0; s < 6; s++) asm volatile("" : "+v"(a[blk][s]));
+  }
+  // dof 4 s + g of an element: corner c_s or c_s + 1 (up), component comp
+  // (constants of the lane group)
+  bool up[6];
+  int comp[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int dd = 4 * s + g, c0 = (4 * s) / 3;
+    up[s] = dd >= 3 * (c0 + 1);
+    comp[s] = dd - 3 * (up[s] ? c0 + 1 : c0);
+  }
+  __syncthreads();
+  // corner slot a of a record (compile-time a)
+  auto slot_of = [](const Rec &r, int c) -> int {
+    const uint32_t w = c < 2 ? r.p0 : c < 4 ? r.p1 : c < 6 ? r.p2 : r.p3;
+    return (int)((c & 1) ? (w >> 16) : (w & 0xffff));
+  };
+  auto load_b = [&](const Rec &r, float *B) {
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const int c0 = (4 * s) / 3;
+      const int sl = up[s] ? slot_of(r, c0 + 1) : slot_of(r, c0);
+      B[s] = xs[3 * sl + comp[s]];
+    }
+  };
+  auto compute = [&](const Rec &cur, const float *B) {
+    if (!SINGLE_CLASS) {
+      const int cls = __builtin_amdgcn_readfirstlane((int)((cur.m >> 16) & 0xff));
+      const float *af = afrag32 + (size_t)cls * 768;
+#pragma unroll
+      for (int blk = 0; blk < 2; blk++)
+#pragma unroll
+        for (int s = 0; s < 6; s++) a[blk][s] = af[(blk * 6 + s) * 64 + lane];
+    }
+    f32x4_t acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = acc0;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0][s], B[s], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1][s], B[s], acc1, 0, 0, 0);
+    }
+    const int mask = cur.m & 0xff;
+    // corner g (row block 0) and corner 4 + g (row block 1)
+    const uint32_t wlo = (g & 2) ? cur.p1 : cur.p0, whi = (g & 2) ? cur.p3 : cur.p2;
+    const int slo = (int)((g & 1) ? (wlo >> 16) : (wlo & 0xffff));
+    const int shi = (int)((g & 1) ? (whi >> 16) : (whi & 0xffff));
+    if (mask & (1 << g)) {
+      float4 w = ys4[slo];
+      w.x = __builtin_fmaf(cur.s, acc0[0], w.x);
+      w.y = __builtin_fmaf(cur.s, acc0[1], w.y);
+      w.z = __builtin_fmaf(cur.s, acc0[2], w.z);
+      ys4[slo] = w;
+    }
+    if (mask & (16 << g)) {
+      float4 w = ys4[shi];
+      w.x = __builtin_fmaf(cur.s, acc1[0], w.x);
+      w.y = __builtin_fmaf(cur.s, acc1[1], w.y);
+      w.z = __builtin_fmaf(cur.s, acc1[2], w.z);
+      ys4[shi] = w;
+    }
+  };
+  // ring of four records (fetched three batches ahead) and two sets of B
+  // operands (read one batch ahead)
+  float B0[6], B1[6];
+  if (bb0 < bb1) load_b(r0, B0);
+  auto step = [&](int bt, const Rec &cur, const Rec &nxt, Rec &fill, const float *Bc, float *Bn) {
+    if (bt + 3 < bb1) load_rec(bt + 3, fill);
+    if (bt + 1 < bb1) load_b(nxt, Bn);  // x is read-only during this phase
+    compute(cur, Bc);
+  };
+  int bt = bb0;
+  for (; bt + 4 <= bb1; bt += 4) {
+    step(bt, r0, r1, r3, B0, B1);
+    step(bt + 1, r1, r2, r0, B1, B0);
+    step(bt + 2, r2, r3, r1, B0, B1);
+    step(bt + 3, r3, r0, r2, B1, B0);
+  }
+  if (bt < bb1) step(bt, r0, r1, r3, B0, B1);
+  if (bt + 1 < bb1) step(bt + 1, r1, r2, r0, B1, B0);
+  if (bt + 2 < bb1) step(bt + 2, r2, r3, r1, B0, B1);
+  __syncthreads();
+  tile_finish<float, MODE, 4, TILE_THREADS, false, 3>(t, n0, nown, comb_ptr, comb_slot, comb_w,
+                                                      bcmask, xs, ys, x, b, y, x_out, d, dinv, ca,
+                                                      cc, cflags, partials, red);
+}
+
 template <class T>
 void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x, int mode,
                        const AsmArgs &g, hipStream_t st, int nrhs, int64_t vs) {
@@ -1974,6 +2139,32 @@ void launch_tile_apply(const Tiles &tl, const T *K0, const T *s_elem, const T *x
   const T *dinv = (const T *)g.dinv;
   const bool single = tl.single_class;
   if constexpr (sizeof(T) == 4) {
+    if (tl.use_mfma && tl.mfma_f32 && !tl.nodal) {
+      const size_t sh3 = 12 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
+#define TMRX_MTILE32(M, S)                                                                      \
+  hipLaunchKernelGGL((k_tile_apply_mfma32<M, S>), grid, dim3(TILE_THREADS), sh3, st, tl.node0,  \
+                     tl.ext_ptr, tl.ext, tl.nacc_dep, tl.mb_ptr,                                \
+                     reinterpret_cast<const uint32_t *>(tl.mel), tl.afrag32, tl.comb_ptr,       \
+                     tl.comb_slot, tl.comb_w, tl.dep_ptr, tl.dep_conn, tl.dep_w, tl.bcmask,     \
+                     (const float *)x, (const float *)b, (float *)y, (float *)xo, (float *)d,    \
+                     (const float *)dinv, g.ca, g.cc, g.cflags, g.partials, vs, tl.run)
+#define TMRX_MTILE32B(M)    \
+  if (single) {             \
+    TMRX_MTILE32(M, true);  \
+  } else {                  \
+    TMRX_MTILE32(M, false); \
+  }
+      switch (mode) {
+        case ASM_STORE: TMRX_MTILE32B(ASM_STORE); break;
+        case ASM_RESID: TMRX_MTILE32B(ASM_RESID); break;
+        case ASM_CHEB: TMRX_MTILE32B(ASM_CHEB); break;
+        case ASM_DOT: TMRX_MTILE32B(ASM_DOT); break;
+      }
+#undef TMRX_MTILE32B
+#undef TMRX_MTILE32
+      TMRX_LAUNCH_CHECK();
+      return;
+    }
     if (tl.use_mfma && !tl.nodal) {
       const size_t sh4 = 24 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
 #define TMRX_MTILE(M, S)                                                                         \

@@ -297,7 +297,19 @@ int Problem::init(int nlevels, Forest **forests, const Material &mat, int nbcs,
           af[base + 512 * (2 + blk)] = mid;
           af[base + 512 * (4 + blk)] = lo;
         }
-  if (K0d_.upload(K0h_) || K0f_.upload(K0f) || afrag_.upload(af)) {
+  // FP32 A fragments for the FP32-input matrix-core form: lane l of row
+  // block b, step s holds K0[row][dof] with row 16 b + l % 16 = 4 a + j
+  // (component j of corner a; j = 3 pads with zero) and dof 4 s + l / 16
+  std::vector<float> af32(768 * (size_t)nclasses_, 0.0f);
+  for (int c = 0; c < nclasses_; c++)
+    for (int blk = 0; blk < 2; blk++)
+      for (int st = 0; st < 6; st++)
+        for (int l = 0; l < 64; l++) {
+          const int row = 16 * blk + l % 16, a = row / 4, j = row % 4, dof = 4 * st + l / 16;
+          af32[768 * (size_t)c + (blk * 6 + st) * 64 + l] =
+              j < 3 ? K0f[576 * (size_t)c + 24 * (3 * a + j) + dof] : 0.0f;
+        }
+  if (K0d_.upload(K0h_) || K0f_.upload(K0f) || afrag_.upload(af) || afrag32_.upload(af32)) {
     last_error = "problem: device allocation failed";
     return -1;
   }
@@ -841,6 +853,7 @@ int Problem::build_tiles(int l) {
     launch_rec_scale(T.n_rec, T.el_elem, L.s32.p, reinterpret_cast<uint32_t *>(T.el32), st_);
   }
   T.afrag = reinterpret_cast<const uint4 *>(afrag_.p);
+  T.afrag32 = afrag32_.p;
   T.use_mfma = true;
   return 0;
 }
@@ -1228,6 +1241,7 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   tl.run = run_;
   tl.nodal = design_mode_ == 1;
   tl.use_mfma = tl.use_mfma && !fp32_valu_ && fp32_mfma_ && !tl.nodal;
+  tl.mfma_f32 = fp32_mfma_in32_;
   if (ev) (void)hipEventRecord(ev->first, st_);
   launch_tile_apply<T>(tl, tl.nodal ? gtab<T>() : rtab<T>(), tl.nodal ? gscale<T>(l) : scale<T>(l),
                        x, mode, g, st_, nrhs, L.ndof);
@@ -2264,6 +2278,19 @@ int Problem::traction_load(const char *name, const double tr[3], double *f) {
 // every level and time the finest-level operator (FP32 product on the matrix
 // cores 4x, as in a V-cycle's smoothing and residual, plus one FP64 product,
 // as in a PCG iteration); keep the fastest.  Needs a design.
+int Problem::set_fp32_form(int form) {
+  if (form < 0 || form > 2) {
+    last_error = "set_fp32_form: form must be 0 (vector ALUs), 1 (matrix cores, bf16 split) or 2 "
+                 "(matrix cores, FP32 inputs)";
+    return -1;
+  }
+  fp32_mfma_ = form != 0;
+  fp32_mfma_in32_ = form == 2;
+  drop_graphs();
+  gen_++;
+  return 0;
+}
+
 int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double *ms_out) {
   if (!have_design_) {
     last_error = "autotune: set the design first";
@@ -2298,13 +2325,12 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const bool keep_valu = fp32_valu_;
+  const bool keep_valu = fp32_valu_, keep_mfma = fp32_mfma_, keep_in32 = fp32_mfma_in32_;
   fp32_valu_ = false;
   int dev = 0, max_lds = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  int best = -1;
-  bool best_mfma = true;
+  int best = -1, best_form = 1;
   double best_ms = 0.0;
   for (int c = 0; c < ncand; c++) {
     bool fits = true;
@@ -2338,9 +2364,13 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
     ar.y = L0.v32.d.p;
     a64.b = L0.v64.b.p;
     a64.y = L0.v64.r.p;
-    // ms per such group of launches of one operator form
-    auto time_op = [&](int form) -> double {  // 0 FP32 matrix cores, 1 FP32 VALU, 2 FP64
+    // ms per such group of launches of one operator form: 0 FP32 on the
+    // matrix cores (bf16 three-way split), 1 FP32 on the vector ALUs, 2 FP64,
+    // 3 FP32 on the matrix cores with FP32 inputs
+    auto time_op = [&](int form) -> double {
       fp32_valu_ = form == 1;
+      fp32_mfma_ = form == 0 || form == 3;
+      fp32_mfma_in32_ = form == 3;
       auto once = [&]() {
         if (form == 2) {
           apply_level<double>(0, L0.v64.x.p, ASM_RESID, a64);
@@ -2359,25 +2389,40 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
       return ms / reps;
     };
     const double tv = time_op(1), t64 = time_op(2);
-    // the matrix-core form covers element densities only
-    const double tm = design_mode_ == 1 ? tv + 1.0 : time_op(0);
-    if (tm < 0 || tv < 0 || t64 < 0) {
+    // the matrix-core forms cover element densities only
+    const double tm = design_mode_ == 1 ? tv : time_op(0);
+    const double tm32 = design_mode_ == 1 ? tv : time_op(3);
+    if (tm < 0 || tm32 < 0 || tv < 0 || t64 < 0) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      fp32_valu_ = keep_valu;
+      fp32_mfma_ = keep_mfma;
+      fp32_mfma_in32_ = keep_in32;
       HIPCHK(hipGetLastError());
       last_error = "autotune: timing failed";
       return -1;
     }
-    const bool mfma = tm <= tv;
-    const double per = (mfma ? tm : tv) + t64;
+    int form = 1;
+    double tf = tv;
+    if (design_mode_ != 1 && tm < tf) {
+      form = 0;
+      tf = tm;
+    }
+    if (design_mode_ != 1 && tm32 < tf) {
+      form = 3;
+      tf = tm32;
+    }
+    const double per = tf + t64;
     if (ms_out) ms_out[c] = per;
     if (best < 0 || per < best_ms) {
       best = c;
       best_ms = per;
-      best_mfma = mfma;
+      best_form = form;
     }
   }
   fp32_valu_ = keep_valu;
+  fp32_mfma_ = keep_mfma;
+  fp32_mfma_in32_ = keep_in32;
   if (best < 0) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2447,7 +2492,8 @@ int Problem::autotune(const int *cands, int ncand, int reps, int *chosen, double
   batch_cap_ = 0;
   if (ensure_batch(cap)) return -1;
   gen_++;
-  fp32_mfma_ = best_mfma;
+  fp32_mfma_ = best_form != 1;
+  fp32_mfma_in32_ = best_form == 3;
   if (chosen) *chosen = cands[best];
   return 0;
 }

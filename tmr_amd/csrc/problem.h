@@ -220,6 +220,11 @@ class Problem {
   int autotune(const int *cands, int ncand, int reps, int *chosen, double *ms);
   int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
   bool fp32_matrix_cores() const { return fp32_mfma_; }
+  // FP32 operator form: 0 vector ALUs, 1 matrix cores (bf16 three-way
+  // split), 2 matrix cores (FP32 inputs); set_fp32_form overrides the
+  // autotuner's choice until the next autotune
+  int fp32_form() const { return fp32_mfma_ ? (fp32_mfma_in32_ ? 2 : 1) : 0; }
+  int set_fp32_form(int form);
   int set_state(const double *f, const double *u, bool on_device);
   // multigrid options of TMR_CreateTACSMg (src/TMR_RefinementTools.cpp:37-107):
   // Galerkin coarse operators P^T A P on the coarse levels (else
@@ -323,6 +328,7 @@ class Problem {
   DevArray<double> K0d_;
   DevArray<float> K0f_;
   DevArray<uint16_t> afrag_;  // bf16 hi/lo MFMA fragments of K0f_
+  DevArray<float> afrag32_;   // FP32 MFMA fragments of K0f_
   // nodal design: per-class gradient tables, per-Gauss-point stiffness
   DevArray<double> R64_;  // reflection-block tables of K0, REFL_TABLE per class
   DevArray<float> R32_;
@@ -385,6 +391,7 @@ class Problem {
   hipStream_t st_ = nullptr;
   bool fp32_valu_ = false;  // FP32 operator on the vector ALUs
   bool fp32_mfma_ = true;   // autotuned: FP32 operator on the matrix cores
+  bool fp32_mfma_in32_ = false;  // ... with FP32 inputs (else bf16 three-way split)
   int timing_ = 0;  // mask of the timed launch modes (bit m = AsmMode m)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_used_ = 0;

@@ -276,11 +276,21 @@ class Problem:
         check(lib().tmrx_problem_autotune(self.ptr, ts, n, int(reps), C.byref(ch), ms), "autotune")
         return ch.value, {int(t): (m if m >= 0 else None) for t, m in zip(tile_sizes, ms)}
 
+    FP32_FORMS = ("vector ALUs", "matrix cores (bf16 split)", "matrix cores (FP32)")
+
     def tuning(self):
-        """(finest-level tile size, FP32 operator on the matrix cores?)."""
+        """(finest-level tile size, FP32 operator form): 0 vector ALUs, 1
+        matrix cores with bf16 three-way split operands, 2 matrix cores with
+        FP32 operands (non-zero: on the matrix cores)."""
         t, m = C.c_int(), C.c_int()
         check(lib().tmrx_problem_tuning(self.ptr, C.byref(t), C.byref(m)))
-        return t.value, bool(m.value)
+        return t.value, m.value
+
+    def set_fp32_form(self, form):
+        """Pin the FP32 operator form (0, 1, 2 as in tuning()) until the
+        next autotune."""
+        check(lib().tmrx_problem_set_fp32_form(self.ptr, int(form)), "set_fp32_form")
+        self._bump()
 
     def true_residual(self):
         v = C.c_double()
