@@ -473,8 +473,8 @@ def main():
     args = ap.parse_args()
     args.tiles = None if args.tiles == "auto" else tuple(int(v) for v in args.tiles.split(","))
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "multirhs":
         return bench_multirhs(args)
