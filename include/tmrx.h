@@ -401,6 +401,13 @@ int tmrx_problem_tuning(tmrx_problem *p, int *tile_nodes, int *fp32_matrix_cores
 /* No counterpart in the reference: pin the FP32 operator form (0, 1, 2 as
    above) until the next autotune. */
 int tmrx_problem_set_fp32_form(tmrx_problem *p, int form);
+/* No counterpart in the reference: owner tiles of tile_nodes independent
+   nodes at the finest level (64..4096; coarse levels shrink theirs as the
+   autotuner does).  Overrides the autotuner until its next call. */
+int tmrx_problem_set_tiling(tmrx_problem *p, int tile_nodes);
+/* No counterpart in the reference: the owner tiles of one level (count and
+   nominal size in independent nodes). */
+int tmrx_problem_tile_info(tmrx_problem *p, int level, int *ntiles, int *tile_nodes);
 /* Operator autotuner (no counterpart in the reference): rebuilds the owner
    tiles for each candidate size (independent nodes per finest-level tile,
    64..4096; coarse levels shrink theirs to keep >= 2048 tiles), times the

@@ -240,6 +240,18 @@ def test_c3_mixed_vs_fp64():
         assert ident < 1e-7
     assert dc < 1e-6 and du < 1e-6
     assert im <= 12 and i64 <= 12
+    # the same problem on 512-node owner tiles (set_tiling): same answer
+    prob.set_tiling(512)
+    nt, tn = prob.tile_info(0)
+    print("C3 on %d tiles of %d nodes" % (nt, tn), flush=True)
+    assert tn == 512 and nt == 8354
+    us = prob.solve(f, opts=solver.default_opts(rtol=1e-8, precision=1, max_iters=200))
+    its, trs = prob.info.iters, prob.true_residual()
+    res, ident, _ = _oracle_checks(forests[0], rho, f, us, "C3 mixed, 512-node tiles")
+    dus = np.linalg.norm(us - um) / np.linalg.norm(um)
+    print("C3 512-node tiles: %d iterations, GPU true residual %.3e, u vs 1024-node tiles %.2e"
+          % (its, trs, dus), flush=True)
+    assert res < 1e-8 and ident < 1e-7 and dus < 1e-6 and its <= 12
 
 
 # ---------------------------------------------------------------------------

@@ -112,6 +112,8 @@ _SIGS = {
     "tmrx_problem_autotune": (I, [P, IP, I, I, IP, C.POINTER(C.c_double)]),
     "tmrx_problem_tuning": (I, [P, IP, IP]),
     "tmrx_problem_set_fp32_form": (I, [P, I]),
+    "tmrx_problem_tile_info": (I, [P, I, IP, IP]),
+    "tmrx_problem_set_tiling": (I, [P, I]),
     "tmrx_problem_true_residual": (I, [P, DP]),
     "tmrx_problem_set_state": (I, [P, P, P, I]),
     "tmrx_problem_compliance": (I, [P, DP]),

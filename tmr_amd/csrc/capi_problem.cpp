@@ -275,6 +275,20 @@ int tmrx_problem_tuning(tmrx_problem *h, int *tile_nodes, int *fp32_matrix_cores
   return 0;
 }
 
+int tmrx_problem_tile_info(tmrx_problem *h, int level, int *ntiles, int *tile_nodes) {
+  CHECK_P(h);
+  if (h->p->tile_info(level, ntiles, tile_nodes)) {
+    set_error("tmrx_problem_tile_info: no such level");
+    return -1;
+  }
+  return 0;
+}
+
+int tmrx_problem_set_tiling(tmrx_problem *h, int tile_nodes) {
+  CHECK_P(h);
+  return h->p->set_tiling(tile_nodes) ? fail(h) : 0;
+}
+
 int tmrx_problem_set_fp32_form(tmrx_problem *h, int form) {
   CHECK_P(h);
   return h->p->set_fp32_form(form) ? fail(h) : 0;

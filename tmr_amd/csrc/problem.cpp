@@ -2278,6 +2278,39 @@ int Problem::traction_load(const char *name, const double tr[3], double *f) {
 // every level and time the finest-level operator (FP32 product on the matrix
 // cores 4x, as in a V-cycle's smoothing and residual, plus one FP64 product,
 // as in a PCG iteration); keep the fastest.  Needs a design.
+int Problem::set_tiling(int tile_nodes) {
+  if (levels_.empty()) {
+    last_error = "set_tiling: not initialised";
+    return -1;
+  }
+  if (tile_nodes < 64 || tile_nodes > 4096) {
+    last_error = "set_tiling: tile sizes must lie in [64, 4096]";
+    return -1;
+  }
+  int dev = 0, max_lds = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  for (size_t l = 0; l < levels_.size(); l++) {
+    Level &L = levels_[l];
+    L.tile_nodes = l == 0 ? tile_nodes : coarse_tile_nodes(tile_nodes, L.n_nodes);
+    if (build_tiles((int)l)) return -1;
+    const Tiles &tl = L.tiles;
+    const size_t lds64 = 3 * (size_t)(round8(tl.max_slots) + round8(tl.max_acc)) * sizeof(double);
+    const size_t lds32 = 24 * (size_t)round8(tl.max_slots) + 16 * (size_t)round8(tl.max_acc);
+    if (std::max(lds64, lds32) + 64 > (size_t)max_lds) {
+      last_error = "set_tiling: the tiles of level " + std::to_string(l) + " do not fit in LDS";
+      return -1;
+    }
+  }
+  if (build_plan()) return -1;
+  const int cap = std::max(1, batch_cap_);
+  batch_cap_ = 0;
+  if (ensure_batch(cap)) return -1;
+  drop_graphs();
+  gen_++;
+  return 0;
+}
+
 int Problem::set_fp32_form(int form) {
   if (form < 0 || form > 2) {
     last_error = "set_fp32_form: form must be 0 (vector ALUs), 1 (matrix cores, bf16 split) or 2 "

@@ -219,12 +219,22 @@ class Problem {
   // owner-tile size autotuner (ms per candidate, chosen size)
   int autotune(const int *cands, int ncand, int reps, int *chosen, double *ms);
   int tile_nodes() const { return levels_.empty() ? 0 : levels_[0].tile_nodes; }
+  // owner tiles of level l: count and nominal size
+  int tile_info(int l, int *ntiles, int *tile_nodes) const {
+    if (l < 0 || l >= (int)levels_.size()) return -1;
+    if (ntiles) *ntiles = levels_[l].tiles.n_tiles;
+    if (tile_nodes) *tile_nodes = levels_[l].tile_nodes;
+    return 0;
+  }
   bool fp32_matrix_cores() const { return fp32_mfma_; }
   // FP32 operator form: 0 vector ALUs, 1 matrix cores (bf16 three-way
   // split), 2 matrix cores (FP32 inputs); set_fp32_form overrides the
   // autotuner's choice until the next autotune
   int fp32_form() const { return fp32_mfma_ ? (fp32_mfma_in32_ ? 2 : 1) : 0; }
   int set_fp32_form(int form);
+  // owner tiles of tile_nodes at the finest level (coarse levels by the
+  // autotuner's rule); overrides the autotuner until the next autotune
+  int set_tiling(int tile_nodes);
   int set_state(const double *f, const double *u, bool on_device);
   // multigrid options of TMR_CreateTACSMg (src/TMR_RefinementTools.cpp:37-107):
   // Galerkin coarse operators P^T A P on the coarse levels (else

@@ -286,6 +286,19 @@ class Problem:
         check(lib().tmrx_problem_tuning(self.ptr, C.byref(t), C.byref(m)))
         return t.value, m.value
 
+    def tile_info(self, level=0):
+        """(tile count, nominal tile size) of a level's owner tiles."""
+        nt, tn = C.c_int(), C.c_int()
+        check(lib().tmrx_problem_tile_info(self.ptr, int(level), C.byref(nt), C.byref(tn)),
+              "tile_info")
+        return nt.value, tn.value
+
+    def set_tiling(self, tile_nodes):
+        """Owner tiles of `tile_nodes` at the finest level (coarse levels by
+        the autotuner's rule); overrides the autotuner until its next call."""
+        check(lib().tmrx_problem_set_tiling(self.ptr, int(tile_nodes)), "set_tiling")
+        self._bump()
+
     def set_fp32_form(self, form):
         """Pin the FP32 operator form (0, 1, 2 as in tuning()) until the
         next autotune."""
