@@ -62,6 +62,51 @@ def test_apply_parity(which, cube, adapt):
     assert e32 < 1e-4
 
 
+@pytest.mark.parametrize("nlevels", [1, 2])
+def test_large_coarse_direct_solve(nlevels):
+    """A coarsest level of at least 2048 dofs takes the super-block Cholesky
+    and explicit inverse (256-wide blocks, the bulk in FP64 GEMMs): with one
+    level it is the whole 16 x 8 x 8 box (4,131 dofs), so the coarse solve
+    must match the oracle's operator and one PCG step solves the problem;
+    with two levels (Galerkin) the coarse solve of the 32 x 16 x 16 box's
+    coarse level must match the oracle multigrid's own Cholesky solve."""
+    forests, rho, L = workloads.box(level=3 if nlevels == 1 else 4, nlevels=nlevels, seed=2)
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_design(rho)
+    orc = _oracle(forests[0], rho)
+    rng = np.random.default_rng(5)
+    opts = solver.default_opts(rtol=1e-10, precision=0)
+    lc = nlevels - 1
+    if nlevels == 1:
+        free = orc.free_mask()
+        b = np.where(free, rng.standard_normal(prob.ndof), 0.0)
+        y, _ = prob.mg_op("coarse", lc, b, opts)
+        res = np.linalg.norm(orc.matvec(y) - b) / np.linalg.norm(b)
+        print("one level, %d dofs: coarse solve residual in the oracle's operator %.2e"
+              % (prob.ndof, res), flush=True)
+        assert res < 1e-10
+    else:
+        prob.set_multigrid(use_galerkin=True)
+        mg = po.Multigrid(orc, forests, BCS)
+        mg.setup("chebyshev", degree=2, lower=1.0 / 6.0, upper=1.1)
+        fc = mg.masks[lc]
+        freec = ~(((fc[:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))
+        b = np.where(freec, rng.standard_normal(len(freec)), 0.0)
+        y, _ = prob.mg_op("coarse", lc, b, opts)
+        ec = np.linalg.norm(y - mg.op("coarse", lc, b)) / np.linalg.norm(y)
+        print("two levels, coarse %d dofs: against the oracle's Cholesky solve %.2e"
+              % (len(b), ec), flush=True)
+        assert ec < 1e-10
+    f = _load(prob)
+    u_ref = orc.solve(f, rtol=1e-13)
+    u = prob.solve(f, opts=opts)
+    eu = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+    print("solve: %d iterations, u vs oracle %.2e" % (prob.info.iters, eu), flush=True)
+    assert eu < 1e-8 and prob.true_residual() < 1e-10
+    if nlevels == 1:
+        assert prob.info.iters <= 2
+
+
 @pytest.mark.parametrize("which", ["cube", "adapt"])
 def test_fp32_matrix_cores_fp32_operands(which, cube, adapt):
     """The FP32 operator on the matrix cores with FP32 operands

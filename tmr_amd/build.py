@@ -56,7 +56,7 @@ def build(verbose=False):
     objs = [o for _, o in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lrocblas",
           "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     if verbose:
@@ -92,7 +92,8 @@ def build_sanitized(out_dir=None, verbose=False):
     lib = os.path.join(out_dir, "libtmrx_asan.so")
     _run(["g++", "-shared", "-fPIC"] + SAN_FLAGS + ["-o", lib + ".tmp"] +
          [o for _, o in jobs] + dev_objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+         ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-lrocblas",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(lib + ".tmp", lib)
     if verbose:
         print("built", lib)

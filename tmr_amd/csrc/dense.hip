@@ -11,6 +11,7 @@
 //     X[i][j] = -X[i][i] sum_{k=j}^{i-1} L[i][k] X[k][j];
 //   A^-1 = X^T X, lower blocks then mirrored, written into the m x m result.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include "kernels.h"
 
@@ -39,10 +40,13 @@ __global__ void __launch_bounds__(DT) k_potrf_diag(double *__restrict__ W, int m
     for (int i = j + 1 + tid; i < DB; i += DT) a[i][j] /= s;
     __syncthreads();
     // trailing update of the lower part: a[i][l] -= a[i][j] a[l][j], j < l <= i
-    const int rem = DB - 1 - j;
-    for (int t = tid; t < rem * rem; t += DT) {
-      const int i = j + 1 + t / rem, l = j + 1 + t % rem;
-      if (l <= i) a[i][l] -= a[i][j] * a[l][j];
+    // (thread: row i = tid / 4, columns l = j + 1 + tid % 4, step 4)
+    {
+      const int i = tid >> 2;
+      if (i > j) {
+        const double aij = a[i][j];
+        for (int l = j + 1 + (tid & 3); l <= i; l += 4) a[i][l] -= aij * a[l][j];
+      }
     }
     __syncthreads();
   }
@@ -280,7 +284,162 @@ __global__ void k_bsr_scatter(const int *__restrict__ col, const double *__restr
     for (int q = 0; q < 3; q++) W[(size_t)(3 * i + p) * mp + 3 * j + q] = v[3 * p + q];
 }
 
-int dense_padded(int m) { return (m + DB - 1) / DB * DB; }
+// ---------------------------------------------------------------------------
+// Large matrices (m >= DENSE_BIG): the same factorisation by recursive
+// halving at multiples of 256, with 256 x 256 diagonal blocks factored and
+// inverted by the kernels above (as matrices of their own) and everything
+// else in plain FP64 GEMMs (rocBLAS), large enough to fill the device:
+//   cholinv(A) = (L, X = L^-1): for A = [A11 .; A21 A22],
+//     (L11, X11) = cholinv(A11); L21 = A21 X11^T; A22 -= L21 L21^T;
+//     (L22, X22) = cholinv(A22); X21 = -X22 (L21 X11);
+//   gram(c) = the lower part of X[c:, c:]^T X[c:, c:] (X lower), by halving
+//     the block columns: G22 = gram(second half), G21 = X22^T X21 over all
+//     rows below, G11 = gram(first half).
+// The factor and the inverse live in W and X; A21 is parked in the free
+// block of X where X21 goes, and L21 X11 (transposed) in the unused upper
+// block of W, so no GEMM works in place.
+constexpr int DS = 256;          // diagonal block of the recursion
+constexpr int DENSE_BIG = 2048;  // matrices from this size take this path
+constexpr int GRAM_LEAF = 1024;  // gram: blocks up to this size in one GEMM
+
+// out (m x m) = the lower part of G (mp x mp, row-major) and its mirror
+__global__ void k_dense_mirror(const double *__restrict__ G, int mp, int m, double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)m * m) return;
+  const int r = (int)(t / m), c = (int)(t % m);
+  out[t] = r >= c ? G[(size_t)r * mp + c] : G[(size_t)c * mp + r];
+}
+
+namespace {
+
+// row-major C (M x N) = alpha op(A) op(B) + beta C on a column-major GEMM:
+// C^T = op(B)^T op(A)^T
+bool gemm_rm(rocblas_handle h, bool ta, bool tb, int M, int N, int K, double alpha,
+             const double *A, int lda, const double *B, int ldb, double beta, double *C,
+             int ldc) {
+  if (M <= 0 || N <= 0) return true;
+  return rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                       ta ? rocblas_operation_transpose : rocblas_operation_none, N, M, K, &alpha,
+                       B, ldb, A, lda, &beta, C, ldc) == rocblas_status_success;
+}
+
+bool copy2d(double *dst, int ldd, const double *src, int lds, int rows, int cols, hipStream_t st) {
+  return hipMemcpy2DAsync(dst, sizeof(double) * ldd, src, sizeof(double) * lds,
+                          sizeof(double) * cols, rows, hipMemcpyDeviceToDevice, st) == hipSuccess;
+}
+
+struct Dense {
+  rocblas_handle h;
+  hipStream_t st;
+  double *W, *X, *D, *Xs;  // D, Xs: DS x DS work of the diagonal blocks
+  int ld;
+  int *bad;
+  double *at(double *M, int r, int c) const { return M + (size_t)r * ld + c; }
+
+  // L (lower) and L^-1 of the DS x DS block at (o, o)
+  bool leaf(int o) {
+    const int nblk = DS / DB;
+    if (!copy2d(D, DS, at(W, o, o), ld, DS, DS, st)) return false;
+    for (int k = 0; k < nblk; k++) {
+      hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(DT), 0, st, D, DS, k, bad);
+      const int rows = DS - (k + 1) * DB;
+      if (rows > 0) {
+        hipLaunchKernelGGL(k_trsm_panel, dim3((rows + DT - 1) / DT), dim3(DT), 0, st, D, DS, k);
+        const int T = nblk - k - 1;
+        hipLaunchKernelGGL(k_syrk_trailing, dim3(T * (T + 1) / 2), dim3(DT), 0, st, D, DS, k);
+      }
+    }
+    if (!copy2d(at(W, o, o), ld, D, DS, DS, DS, st)) return false;
+    if (hipMemsetAsync(Xs, 0, sizeof(double) * DS * DS, st) != hipSuccess) return false;
+    hipLaunchKernelGGL(k_trtri_diag, dim3(nblk), dim3(DB), 0, st, D, Xs, DS);
+    for (int k = 0; k + 1 < nblk; k++)
+      hipLaunchKernelGGL(k_trtri_step, dim3((nblk - 1 - k) * (k + 1)), dim3(DT), 0, st, D, Xs, DS,
+                         k);
+    return copy2d(at(X, o, o), ld, Xs, DS, DS, DS, st);
+  }
+
+  // L and X = L^-1 of the n x n diagonal block at (o, o), n a multiple of DS
+  bool cholinv(int o, int n) {
+    if (n == DS) return leaf(o);
+    const int n1 = DS * ((n / DS) / 2), n2 = n - n1, o2 = o + n1;
+    if (!cholinv(o, n1)) return false;
+    // L21 = A21 X11^T (A21 parked in X21's place)
+    if (!copy2d(at(X, o2, o), ld, at(W, o2, o), ld, n2, n1, st)) return false;
+    if (!gemm_rm(h, false, true, n2, n1, n1, 1.0, at(X, o2, o), ld, at(X, o, o), ld, 0.0,
+                 at(W, o2, o), ld))
+      return false;
+    // A22 -= L21 L21^T, lower part, in up to four strips of block rows
+    const int hs = DS * (((n2 / DS) + 3) / 4);
+    for (int s0 = o2; s0 < o2 + n2; s0 += hs) {
+      const int s1 = s0 + hs < o2 + n2 ? s0 + hs : o2 + n2;
+      if (!gemm_rm(h, false, true, s1 - s0, s1 - o2, n1, -1.0, at(W, s0, o), ld, at(W, o2, o), ld,
+                   1.0, at(W, s0, o2), ld))
+        return false;
+    }
+    if (!cholinv(o2, n2)) return false;
+    // X21 = -X22 (L21 X11): (L21 X11)^T = X11^T L21^T into the upper block
+    if (!gemm_rm(h, true, true, n1, n2, n1, 1.0, at(X, o, o), ld, at(W, o2, o), ld, 0.0,
+                 at(W, o, o2), ld))
+      return false;
+    return gemm_rm(h, false, true, n2, n1, n2, -1.0, at(X, o2, o2), ld, at(W, o, o2), ld, 0.0,
+                   at(X, o2, o), ld);
+  }
+
+  // lower part of X[o:end, o:o+n]^T X[o:end, o:o+n] into W[o:o+n, o:o+n]
+  bool gram(int o, int n, int end) {
+    if (n <= GRAM_LEAF)
+      return gemm_rm(h, true, false, n, n, end - o, 1.0, at(X, o, o), ld, at(X, o, o), ld, 0.0,
+                     at(W, o, o), ld);
+    const int n1 = DS * ((n / DS) / 2), n2 = n - n1, o2 = o + n1;
+    if (!gram(o2, n2, end)) return false;
+    if (!gemm_rm(h, true, false, n2, n1, end - o2, 1.0, at(X, o2, o2), ld, at(X, o2, o), ld, 0.0,
+                 at(W, o2, o), ld))
+      return false;
+    return gram(o, n1, end);
+  }
+};
+
+// workspace: D (DS^2), Xs (DS^2)
+bool dense_inverse_big(rocblas_handle h, double *W, double *X, double *ws, int m, double *out,
+                       int *bad, hipStream_t st) {
+  const int mp = dense_padded(m);
+  if (rocblas_set_stream(h, st) != rocblas_status_success) return false;
+  if (hipMemsetAsync(X, 0, sizeof(double) * mp * mp, st) != hipSuccess) return false;
+  Dense dn{h, st, W, X, ws, ws + DS * DS, mp, bad};
+  if (!dn.cholinv(0, mp)) return false;
+  // the factor is no longer needed: A^-1 = X^T X into W
+  if (!dn.gram(0, mp, mp)) return false;
+  const int64_t tot = (int64_t)m * m;
+  hipLaunchKernelGGL(k_dense_mirror, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, W, mp,
+                     m, out);
+  return true;
+}
+
+}  // namespace
+
+void *dense_blas_create() {
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+  return h;
+}
+
+void dense_blas_destroy(void *h) {
+  if (h) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(h));
+}
+
+int dense_padded(int m) {
+  return m >= DENSE_BIG ? (m + DS - 1) / DS * DS : (m + DB - 1) / DB * DB;
+}
+
+size_t dense_workspace(int m) { return m < DENSE_BIG ? 0 : 2 * (size_t)DS * DS; }
+
+int launch_dense_inverse_big(void *blas, double *W, double *X, double *ws, int m, double *out,
+                             int *bad, hipStream_t st) {
+  if (m < DENSE_BIG || !blas || !ws) return -1;
+  if (!dense_inverse_big(static_cast<rocblas_handle>(blas), W, X, ws, m, out, bad, st)) return -1;
+  TMRX_LAUNCH_CHECK();
+  return 0;
+}
 
 void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st) {
   const int mp = dense_padded(m), nblk = mp / DB;

@@ -238,6 +238,14 @@ int dense_padded(int m);
 // L^-1; out (m x m, row-major) receives A^-1; *bad |= 1 on a non-positive
 // pivot
 void launch_dense_inverse(double *W, double *X, int m, double *out, int *bad, hipStream_t st);
+// the same for m of at least 2048 over 256-wide super-blocks with the bulk in
+// FP64 GEMMs (rocBLAS handle from dense_blas_create, workspace of
+// dense_workspace(m) doubles); returns -1 if m is below that or a call fails
+int launch_dense_inverse_big(void *blas, double *W, double *X, double *ws, int m, double *out,
+                             int *bad, hipStream_t st);
+size_t dense_workspace(int m);
+void *dense_blas_create();
+void dense_blas_destroy(void *h);
 // W (mp x mp, row-major) = the BSR block rows [n0, n1) of a component
 // (blocks [k0, k1), block rows brow) as a dense matrix of its 3 (n1 - n0)
 // dofs, identity on the padding rows

@@ -45,6 +45,7 @@ void Problem::drop_graphs() {
 
 Problem::~Problem() {
   drop_graphs();
+  dense_blas_destroy(blas_);
   for (int k = 0; k < 2; k++)
     if (gev_[k]) (void)hipEventDestroy(gev_[k]);
   if (cap_st_) (void)hipStreamDestroy(cap_st_);
@@ -1392,16 +1393,22 @@ int Problem::coarse_factor() {
   std::vector<int64_t> aoff(n);
   // component blocks of the inverse, one after the other
   std::vector<size_t> base(nseg_ + 1, 0);
-  size_t wmax = 0;
+  size_t wmax = 0, wsmax = 0;
   for (int sg = 0; sg < nseg_; sg++) {
     const size_t m = 3 * (size_t)(L.seg_node0[sg + 1] - L.seg_node0[sg]);
     base[sg + 1] = base[sg] + m * m;
     const size_t mp = (size_t)dense_padded((int)m);
     wmax = std::max(wmax, mp * mp);
+    wsmax = std::max(wsmax, dense_workspace((int)m));
   }
   if ((Ainv_.n < base[nseg_] && Ainv_.alloc(base[nseg_])) ||
-      (cw_.n < wmax && (cw_.alloc(wmax) || cx_.alloc(wmax))) || (chk_.n == 0 && chk_.alloc(1))) {
+      (cw_.n < wmax && (cw_.alloc(wmax) || cx_.alloc(wmax))) || (chk_.n == 0 && chk_.alloc(1)) ||
+      (cws_.n < wsmax && cws_.alloc(wsmax))) {
     last_error = "coarse: device allocation failed";
+    return -1;
+  }
+  if (wsmax && !blas_ && !(blas_ = dense_blas_create())) {
+    last_error = "coarse: rocBLAS handle creation failed";
     return -1;
   }
   HIPCHK(hipMemsetAsync(chk_.p, 0, sizeof(int), st_));
@@ -1411,7 +1418,15 @@ int Problem::coarse_factor() {
     const int mp = dense_padded(m);
     launch_bsr_to_dense(L.bsr_row.p, L.bsr_col.p, L.bsr64.p, n0, n1, L.bsr_ptr_h[n0],
                         L.bsr_ptr_h[n1], mp, cw_.p, st_);
-    launch_dense_inverse(cw_.p, cx_.p, m, Ainv_.p + base[sg], chk_.p, st_);
+    if (dense_workspace(m)) {
+      if (launch_dense_inverse_big(blas_, cw_.p, cx_.p, cws_.p, m, Ainv_.p + base[sg], chk_.p,
+                                   st_)) {
+        last_error = "coarse: the super-block factorisation failed";
+        return -1;
+      }
+    } else {
+      launch_dense_inverse(cw_.p, cx_.p, m, Ainv_.p + base[sg], chk_.p, st_);
+    }
     for (int i = 0; i < m; i++) {
       col0[d0 + i] = d0;
       nb[d0 + i] = m;

@@ -359,6 +359,8 @@ class Problem {
   DevArray<double> rho_;
   DevArray<double> Ainv_;
   DevArray<double> cw_, cx_;  // coarse factorisation work (padded matrix, L^-1)
+  DevArray<double> cws_;      // workspace of the super-block factorisation
+  void *blas_ = nullptr;      // rocBLAS handle of the super-block factorisation
   DevArray<int> cb_col0_, cb_nb_;
   DevArray<int64_t> cb_aoff_;
   CoarseBlocks coarse_;
