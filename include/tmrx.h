@@ -63,6 +63,13 @@ int tmrx_forest_set_name(tmrx_forest *f, int kind, int index, const char *name);
 
 /* TMROctForest::createTrees (src/TMROctForest.cpp:1744-1833). */
 int tmrx_forest_create_trees(tmrx_forest *f, int level);
+/* No counterpart in the reference (it restarts from design vectors only):
+   the octants of a forest saved earlier (tmrx_forest_get_octants' layout),
+   so that a checkpointed forest is rebuilt exactly.  Every octant must lie
+   inside its block, aligned to its level, and the octants of each block must
+   tile it without overlap or gap; they are sorted into Morton order and
+   renumbered.  Frees the nodes (createNodes builds them again). */
+int tmrx_forest_set_octants(tmrx_forest *f, const tmrx_octant *octs, int n);
 /* No counterpart in the reference (createTrees takes one depth for every
    block): uniform trees of depth block_levels[b] in block b, so that blocks
    of different sizes carry elements of one size (nblocks must equal the

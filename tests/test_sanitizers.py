@@ -41,7 +41,7 @@ def test_host_code_under_asan_and_ubsan(tmp_path):
     assert r.returncode == 0 and r.stdout.strip().endswith(lib), r.stdout + r.stderr
     tests = [os.path.join(HERE, t) for t in
              ("test_forest.py", "test_stl.py", "test_validation.py", "test_topo.py",
-              "test_nonnested.py")]
+              "test_nonnested.py", "test_checkpoint.py")]
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
                         "-m", "not gpu"] + tests, cwd=ROOT, env=env, capture_output=True,
                        text=True, timeout=900)

@@ -26,13 +26,16 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+# TMROctant's layout (src/TMROctant.h:48-53)
+OCT_DTYPE = np.dtype([("block", np.int32), ("x", np.int32), ("y", np.int32), ("z", np.int32),
+                      ("tag", np.int32), ("level", np.int16), ("info", np.int16)])
+
+
 def _octants_to_array(ptr, n):
-    dt = np.dtype([("block", np.int32), ("x", np.int32), ("y", np.int32), ("z", np.int32),
-                   ("tag", np.int32), ("level", np.int16), ("info", np.int16)])
     if n == 0:
-        return np.zeros(0, dtype=dt)
+        return np.zeros(0, dtype=OCT_DTYPE)
     buf = (C.c_char * (24 * n)).from_address(C.addressof(ptr.contents))
-    return np.frombuffer(bytes(buf), dtype=dt).copy()
+    return np.frombuffer(bytes(buf), dtype=OCT_DTYPE).copy()
 
 
 class OctForest:
@@ -49,6 +52,7 @@ class OctForest:
         self._order = order
         self._interp = interp
         self._keep = []
+        self._names = []  # (kind, index, name) as set, for save()
 
     def __del__(self):
         try:
@@ -96,6 +100,7 @@ class OctForest:
         kinds = {"vertex": 0, "edge": 1, "face": 2, "volume": 3, "block": 3}
         k = kinds[kind] if isinstance(kind, str) else int(kind)
         check(lib().tmrx_forest_set_name(self.ptr, k, int(index), name.encode()), "setName")
+        self._names.append((k, int(index), str(name)))
 
     def getConnectivity(self):
         nb, nf, ne, nn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -112,6 +117,29 @@ class OctForest:
 
     def repartition(self, max_rank=-1):
         check(lib().tmrx_forest_repartition(self.ptr), "repartition")
+
+    def setOctants(self, octs):
+        """The octants of a saved forest (getOctants' structured array):
+        validated (inside their blocks, aligned, tiling every block) and
+        sorted into Morton order; frees the nodes."""
+        a = np.ascontiguousarray(octs, dtype=OCT_DTYPE)
+        check(lib().tmrx_forest_set_octants(self.ptr, C.c_void_p(a.ctypes.data), len(a)),
+              "setOctants")
+
+    def save(self, fname):
+        """Checkpoint of the forest (block connectivity and vertices, names,
+        mesh order, octants) as an .npz file; loadForest rebuilds it."""
+        names = np.array([n for _, _, n in self._names], dtype=str)
+        kinds = np.array([(k, i) for k, i, _ in self._names], dtype=np.int64).reshape(-1, 2)
+        np.savez(fname, block_conn=self.getConnectivity()["block_conn"],
+                 points=getattr(self, "_Xv", np.zeros((0, 3))), name_kinds=kinds,
+                 names=names, octants=self.getOctants(),
+                 order=np.array([self._order, self._interp]))
+
+    def writeForestToVTK(self, fname):
+        """TMROctForest::writeForestToVTK (src/TMROctForest.cpp:1317): the
+        octants as hexahedra (legacy VTK), no fields."""
+        writeVTK(fname, self)
 
     def createTrees(self, depth=0, block_levels=None):
         """Uniform trees of `depth`; with `block_levels` (one per block) each
@@ -138,6 +166,7 @@ class OctForest:
         check(lib().tmrx_forest_duplicate(self.ptr, C.byref(h)), "duplicate")
         f = OctForest(self.comm, self._order, self._interp, _handle=h)
         f._Xv = getattr(self, "_Xv", None)
+        f._names = list(self._names)
         return f
 
     def coarsen(self):
@@ -145,6 +174,7 @@ class OctForest:
         check(lib().tmrx_forest_coarsen(self.ptr, C.byref(h)), "coarsen")
         f = OctForest(self.comm, self._order, self._interp, _handle=h)
         f._Xv = getattr(self, "_Xv", None)
+        f._names = list(self._names)
         return f
 
     def balance(self, btype, device=None):
@@ -297,6 +327,21 @@ class OctForest:
         cols = np.ctypeslib.as_array(cl, (nnz,)).copy() if nnz else np.zeros(0, np.intc)
         wts = np.ctypeslib.as_array(w, (nnz,)).copy() if nnz else np.zeros(0)
         return rowp, cols, wts, nc.value
+
+
+def loadForest(fname, comm=None):
+    """The forest saved by OctForest.save: the same blocks, names, mesh order
+    and octants (nodes are created on demand, as for any forest)."""
+    d = np.load(fname, allow_pickle=False)
+    order, interp = (int(v) for v in d["order"])
+    f = OctForest(comm, order=order, interp=interp)
+    f.setConnectivity(d["block_conn"])
+    if len(d["points"]):
+        f.setNodePoints(d["points"])
+    for (k, i), n in zip(d["name_kinds"], d["names"]):
+        f.setName(int(k), int(i), str(n))
+    f.setOctants(d["octants"])
+    return f
 
 
 def box_connectivity(nx, ny, nz, L=(1.0, 1.0, 1.0)):

@@ -59,6 +59,7 @@ _SIGS = {
     "tmrx_forest_set_name": (I, [P, I, I, S]),
     "tmrx_forest_create_trees": (I, [P, I]),
     "tmrx_forest_create_trees_per_block": (I, [P, IP, I]),
+    "tmrx_forest_set_octants": (I, [P, P, I]),
     "tmrx_forest_refine": (I, [P, IP, I, I]),
     "tmrx_forest_balance": (I, [P, I]),
     "tmrx_forest_write_vtk": (I, [P, C.c_char_p, P, I, C.c_char_p, P, I, C.c_char_p]),

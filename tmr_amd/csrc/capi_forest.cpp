@@ -102,6 +102,11 @@ int tmrx_forest_create_trees_per_block(tmrx_forest *f, const int *block_levels, 
   return f->f->create_trees(0, block_levels) ? fail_from(f) : 0;
 }
 
+int tmrx_forest_set_octants(tmrx_forest *f, const tmrx_octant *octs, int n) {
+  CHECK_HANDLE(f);
+  return f->f->set_octants(reinterpret_cast<const tmrx::Oct *>(octs), n) ? fail_from(f) : 0;
+}
+
 int tmrx_forest_refine(tmrx_forest *f, const int *refinement, int min_level, int max_level) {
   CHECK_HANDLE(f);
   return f->f->refine(refinement, min_level, max_level) ? fail_from(f) : 0;

@@ -489,6 +489,65 @@ int Forest::create_trees(int level, const int *block_levels) {
   return 0;
 }
 
+// Octants given by the caller (a forest saved earlier): each inside its
+// block and aligned to its level; sorted into Morton order, the octants of
+// every block must tile it exactly, each starting where the previous one
+// ends on the interleaved index (a level-l octant spans 8^(30 - l) of it).
+int Forest::set_octants(const Oct *o, int n) {
+  if (bd_.num_blocks == 0) {
+    last_error = "set_octants: set the connectivity first";
+    return -1;
+  }
+  if (n <= 0 || !o) {
+    last_error = "set_octants: no octants";
+    return -1;
+  }
+  std::vector<Oct> v(o, o + n);
+  for (auto &a : v) {
+    if (a.block < 0 || a.block >= bd_.num_blocks || a.level < 0 || a.level > MAX_LEVEL) {
+      last_error = "set_octants: octant with a block or level out of range";
+      return -1;
+    }
+    const int32_t h = (int32_t)1 << (MAX_LEVEL - a.level);
+    const int32_t c[3] = {a.x, a.y, a.z};
+    for (int d = 0; d < 3; d++)
+      if (c[d] < 0 || c[d] > HMAX - h || c[d] % h != 0) {
+        last_error = "set_octants: octant outside its block or not aligned to its level";
+        return -1;
+      }
+    a.info = 0;
+  }
+  std::sort(v.begin(), v.end(), oct_less);
+  const u128 low = (((u128)1) << 96) - 1, full = ((u128)1) << (3 * MAX_LEVEL);
+  u128 expect = 0;
+  int cur = -1;
+  for (const Oct &a : v) {
+    if (a.block != cur) {
+      if ((cur >= 0 && expect != full) || a.block != cur + 1) {
+        last_error = "set_octants: block " + std::to_string(cur + 1) + " is not covered";
+        return -1;
+      }
+      cur = a.block;
+      expect = 0;
+    }
+    if ((pos_key(a) & low) != expect) {
+      last_error = "set_octants: octants of block " + std::to_string(cur) +
+                   " overlap or leave a gap";
+      return -1;
+    }
+    expect += ((u128)1) << (3 * (MAX_LEVEL - a.level));
+  }
+  if (cur != bd_.num_blocks - 1 || expect != full) {
+    last_error = "set_octants: block " + std::to_string(cur) + " is not covered";
+    return -1;
+  }
+  free_mesh();
+  octs_.swap(v);
+  for (size_t i = 0; i < octs_.size(); i++) octs_[i].tag = (int32_t)i;
+  reindex(octs_, okey_);
+  return 0;
+}
+
 // refine (src/TMROctForest.cpp:2169-2329), single process
 int Forest::refine(const int *refinement, int min_level, int max_level) {
   if (octs_.empty()) {

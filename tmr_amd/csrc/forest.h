@@ -74,6 +74,9 @@ class Forest {
   // octree operations ------------------------------------------------------
   // uniform trees of depth `level`, or of depth block_levels[b] in block b
   int create_trees(int level, const int *block_levels = nullptr);
+  // octants of a saved forest (validated: aligned, inside their blocks,
+  // tiling every block exactly)
+  int set_octants(const Oct *o, int n);
   int refine(const int *refinement, int min_level, int max_level);
   int balance(int balance_corner);
   // the same balance on GPU `device`, bit for bit (octants, tags, order)
