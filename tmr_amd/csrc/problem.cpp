@@ -1249,14 +1249,21 @@ void Problem::apply_level(int l, const T *x, int mode, const AsmArgs &a, int nrh
   if (ev) (void)hipEventRecord(ev->second, st_);
 }
 
-int Problem::estimate_eigs(int l, const SolveOpts &o) {
+int Problem::estimate_eigs(int l, const SolveOpts &o, bool defer) {
   // power iteration on Dinv A (FP64) from a fixed pseudo-random start
   // (drawn on the device),
   // lambda_max ~ |Dinv A x_k| / |x_k| after eig_iters steps (x_{k+1} =
-  // Dinv A x_k, no renormalisation: the growth is bounded by lambda^k)
+  // Dinv A x_k, no renormalisation: the growth is bounded by lambda^k).
+  // With defer, the sums stay on the device (slot l of eig_sc_) until
+  // collect_eigs reads every level's at once.
   Level &L = levels_[l];
   auto &v = L.v64;
   const int64_t n = L.ndof;
+  const size_t need = SC_N * levels_.size();
+  if (eig_sc_.n < need && eig_sc_.alloc(need)) {
+    last_error = "setup: device allocation failed";
+    return -1;
+  }
   launch_start_vector(n, L.d_bcmask.p, 0x2545f4914f6cdd1dULL * (uint64_t)(l + 1), v.x.p, st_);
   double *a = v.x.p, *b = v.r.p;
   const int iters = std::max(1, o.eig_iters);
@@ -1267,11 +1274,24 @@ int Problem::estimate_eigs(int l, const SolveOpts &o) {
     launch_dinv_dots(n, v.dinv.p, b, a, parts_.p, st_);
     std::swap(a, b);
   }
-  launch_finalize(reduce_blocks(n), parts_.p, 5, sc_.p, st_);
+  double *slot = eig_sc_.p + (size_t)SC_N * l;
+  launch_finalize(reduce_blocks(n), parts_.p, 5, slot, st_);
+  if (defer) return 0;
   double hsc[SC_N];
-  HIPCHK(hipMemcpyAsync(hsc, sc_.p, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipMemcpyAsync(hsc, slot, SC_N * sizeof(double), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   L.lam_max = std::sqrt(hsc[SC_ZQ] / hsc[SC_RZNEW]);
+  return 0;
+}
+
+// lambda_max of levels [0, nl) from the sums estimate_eigs left on the device
+int Problem::collect_eigs(int nl) {
+  if (nl <= 0) return 0;
+  std::vector<double> h((size_t)SC_N * nl);
+  HIPCHK(hipMemcpyAsync(h.data(), eig_sc_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  for (int l = 0; l < nl; l++)
+    levels_[l].lam_max = std::sqrt(h[(size_t)SC_N * l + SC_ZQ] / h[(size_t)SC_N * l + SC_RZNEW]);
   return 0;
 }
 
@@ -1496,7 +1516,7 @@ int Problem::setup(const SolveOpts &o) {
       // Jacobi diagonal of the Galerkin operator
       launch_bsr_dinv<double>(L.n_nodes, L.bsr_diag.p, L.bsr64.p, L.v64.dinv.p, st_);
       if (prec == 1) launch_bsr_dinv<float>(L.n_nodes, L.bsr_diag.p, L.bsr64.p, L.v32.dinv.p, st_);
-      if (estimate_eigs((int)l, o)) return -1;
+      if (estimate_eigs((int)l, o, true)) return -1;
       continue;
     }
     // Jacobi diagonal in both precisions
@@ -1512,10 +1532,13 @@ int Problem::setup(const SolveOpts &o) {
                       design_mode_ == 1 ? L.sg64.p : L.s64.p, design_mode_ == 1,
                       L.d_bcmask.p, L.v64.dinv.p, st_);
     if (prec == 1) launch_convert<double, float>(L.ndof, L.v64.dinv.p, L.v32.dinv.p, st_);
-    if (estimate_eigs((int)l, o)) return -1;
+    if (estimate_eigs((int)l, o, true)) return -1;
   }
   setup_prec_pending_ = prec;
   if (coarse_factor()) return -1;
+  // one read-back for every level's eigenvalue sums (the coarse factor has
+  // synchronised the stream already)
+  if (collect_eigs((int)levels_.size() - 1)) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   have_setup_ = true;
   setup_precision_ = prec;

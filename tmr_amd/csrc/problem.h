@@ -329,7 +329,8 @@ class Problem {
   // warm: x holds the initial guess on entry (else x = 0)
   int solve_batch(int nrhs, double *rhs, double *x, const SolveOpts &o, SolveInfo *infos,
                   bool lift = false, bool warm = false);
-  int estimate_eigs(int l, const SolveOpts &o);
+  int estimate_eigs(int l, const SolveOpts &o, bool defer = false);
+  int collect_eigs(int nl);
 
   std::vector<Level> levels_;
   Material mat_;
@@ -370,6 +371,7 @@ class Problem {
   std::vector<int> seg_block0_;  // [nseg + 1] block ranges of the components
   // PCG state: f_, u_ hold the current load case (first of a batch)
   DevArray<double> f_, u_, r_, p_, q_, z_, parts_, sc_;
+  DevArray<double> eig_sc_;  // power-iteration sums, SC_N per level
   int batch_cap_ = 0;  // right-hand sides the PCG vectors hold
   SegPlan plan_;
   DevArray<int> pl_blk_lo_, pl_blk_seg_, pl_seg_blk_, pl_seg_tile_;
