@@ -599,6 +599,9 @@ def main():
         "true_residual": mixed["true_res"],
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
+        # a design step of an optimisation loop: the density-dependent setup
+        # (diagonals, eigenvalue estimates, coarse factor) plus one solve
+        "designs_per_s": 1.0 / (mixed["setup_ms"] / 1e3 + mixed["seconds"] / args.steps),
         "tile_nodes": prob.tuning()[0],
         "fp32_operator": {"valu": "vector ALUs", "mfma": "matrix cores (bf16 three-way split)",
                           "mfma32": "matrix cores (FP32 operands)"}[form],
