@@ -330,8 +330,10 @@ void tmrx_problem_free(tmrx_problem *p);
      rediscretised coarse elements (volume-averaged material factors).
    use_chebyshev_smoother: must be 1.  TMR's alternative, symmetric SOR
      (TACSGaussSeidel, :93-94), is not available: an error is returned.
-   use_coarse_direct_solve: must be 1 (FP64 Cholesky of the coarsest level,
-     TACSMg's direct solve, :83-86); 0 returns an error.
+   use_coarse_direct_solve: 1 solves the coarsest level directly (FP64
+     Cholesky, TACSMg's direct solve, :83-86); 0 smooths it like the other
+     levels, one Chebyshev sweep of cheb_degree steps from zero (the
+     smoother TMR hands TACSMg there, :87-102).
    omega: TMR's SOR relaxation factor, checked to lie in (0, 2) and otherwise
      unused by Chebyshev smoothing.
    Invalidates the setup; the next solve redoes it. */
@@ -433,7 +435,8 @@ int tmrx_problem_component_info(tmrx_problem *p, int rhs, int comp, int *iters, 
    there (src/TMR_RefinementTools.cpp:76-77), as 3 x 3 blocks over the
    independent nodes of that level, rows and columns of clamped dofs
    identity.  The coarsest level (level = nlevels - 1) is available in
-   either mode: the matrix its direct solve factors (:83-86).  n receives the block rows, nnzb the blocks; ptr[n + 1],
+   either mode when it is solved directly: the matrix that solve factors
+   (:83-86).  n receives the block rows, nnzb the blocks; ptr[n + 1],
    col[nnzb] and val[9 nnzb] (row-major blocks) are filled when not NULL
    (call with NULL arrays to size). */
 int tmrx_problem_level_matrix(tmrx_problem *p, int level, int *n, int *nnzb, int *ptr, int *col,
@@ -484,10 +487,11 @@ int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_
    of createInterpolation's P, src/TMROctForest.cpp:6611-6793, clamped rows
    zero); 2 y = P_l x (level l + 1 to l); 3 y = the pre-smoothing sweep of
    level l (TACSChebyshevSmoother's role, :64-71) with right-hand side x;
-   4 y = the coarsest level's direct solve (:83-86); 5 y = the V-cycle from
+   4 y = the coarsest level's solve (the direct solve, :83-86, or with
+   use_coarse_direct_solve = 0 its Chebyshev sweep); 5 y = the V-cycle from
    level l.  x and y have the sizes of the input and output levels.
    lam_max (may be null) receives the level's estimate of the largest
-   eigenvalue of Dinv A (0 on the coarsest level).  Sets the preconditioner
+   eigenvalue of Dinv A (0 on a coarsest level solved directly).  Sets the preconditioner
    up first if the design changed. */
 int tmrx_problem_mg_op(tmrx_problem *p, int op, int level, const double *x, double *y,
                        int on_device, const tmrx_solve_opts *opts, double *lam_max);

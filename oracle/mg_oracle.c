@@ -9,7 +9,9 @@
     one interpolation per level (createInterpolation), Galerkin coarse
     operators A_{l+1} = P^T A_l P (use_galerkin), symmetric SOR smoothing
     with omega = 1 and one sweep, or Chebyshev smoothing on D^-1 A, and a
-    direct solve of the coarsest level;
+    direct solve of the coarsest level or (use_coarse_direct_solve = 0,
+    :87-102, orc_mg_set_coarse) one application of the level's smoother
+    there;
   * the Krylov method around it: GMRES with a subspace of 50 and restarts
     (TMRTopoProblem, src/topology/TMRTopoProblem.cpp:197-204), or the
     flexible PCG of the GPU path.
@@ -43,6 +45,7 @@ static struct {
   double *Lc; /* dense Cholesky factor of the coarsest level (lower, row-major) */
   double *w1[MG_MAX], *w2[MG_MAX], *w3[MG_MAX], *w4[MG_MAX];
   int smoother; /* 0 Chebyshev, 1 symmetric SOR */
+  int coarse_smooth; /* 1: the coarsest level is smoothed, not solved */
   int degree;
   double lower, upper, omega;
 } g;
@@ -273,8 +276,14 @@ static double dot(int n, const double *a, const double *b) {
   return s;
 }
 
+/* the coarsest level solved directly (direct = 1, the default) or smoothed
+   like the others (TMR_CreateTACSMg, src/TMR_RefinementTools.cpp:83-102);
+   takes effect at the next orc_mg_setup */
+void orc_mg_set_coarse(int direct) { g.coarse_smooth = !direct; }
+
 /* smoother set-up: Jacobi diagonal and lambda_max(D^-1 A) by 10 power
-   iterations; dense Cholesky of the coarsest level */
+   iterations on every smoothed level; dense Cholesky of the coarsest level
+   when it is solved directly */
 int orc_mg_setup(int smoother, int degree, double lower, double upper, double omega) {
   g.smoother = smoother;
   g.degree = degree < 1 ? 1 : degree;
@@ -301,7 +310,7 @@ int orc_mg_setup(int smoother, int degree, double lower, double upper, double om
       if (!(d > 0.0)) return -1 - l;
       g.dinv[l][i] = 1.0 / d;
     }
-    if (l + 1 < g.nl) {
+    if (l + 1 < g.nl || g.coarse_smooth) {
       double *x = g.w1[l], *y = g.w2[l];
       uint64_t s = 0x2545f4914f6cdd1dULL ^ (uint64_t)(l + 1);
       for (int i = 0; i < n; i++) {
@@ -325,6 +334,8 @@ int orc_mg_setup(int smoother, int degree, double lower, double upper, double om
   /* coarsest level: dense Cholesky */
   const int lc = g.nl - 1, m = g.A[lc].n;
   free(g.Lc);
+  g.Lc = NULL;
+  if (g.coarse_smooth) return 0;
   g.Lc = (double *)calloc((size_t)m * m, sizeof(double));
   for (int i = 0; i < m; i++)
     for (int k = g.A[lc].rowp[i]; k < g.A[lc].rowp[i + 1]; k++)
@@ -418,11 +429,25 @@ static void chebyshev(int l, const double *b, double *x, int k, int first) {
   }
 }
 
+/* the coarsest level's solve: the direct solve, or its smoother once from
+   zero */
+static void coarse_apply(const double *b, double *x) {
+  const int l = g.nl - 1;
+  if (!g.coarse_smooth) {
+    coarse_solve(b, x);
+  } else if (g.smoother == 0) {
+    chebyshev(l, b, x, g.degree, 1);
+  } else {
+    memset(x, 0, sizeof(double) * g.A[l].n);
+    sor_symmetric(l, b, x);
+  }
+}
+
 /* x = V-cycle(b) from zero at level l */
 static void vcycle(int l, const double *b, double *x) {
   const int n = g.A[l].n;
   if (l == g.nl - 1) {
-    coarse_solve(b, x);
+    coarse_apply(b, x);
     return;
   }
   if (g.smoother == 0) {
@@ -459,7 +484,8 @@ int orc_mg_set_lambda(int l, double lam) {
 
 /* one component alone: 0 y = A_l x; 1 y = R_l x; 2 y = P_l x (x on level
    l + 1); 3 y = level l's pre-smoothing from zero with right-hand side x;
-   4 y = the coarsest solve; 5 y = the V-cycle from level l */
+   4 y = the coarsest level's solve (direct or smoothed); 5 y = the V-cycle
+   from level l */
 int orc_mg_op(int op, int l, const double *x, double *y) {
   if (l < 0 || l >= g.nl) return -1;
   const int last = g.nl - 1;
@@ -476,7 +502,7 @@ int orc_mg_op(int op, int l, const double *x, double *y) {
         sor_symmetric(l, x, y);
       }
       break;
-    case 4: coarse_solve(x, y); break;
+    case 4: coarse_apply(x, y); break;
     case 5: vcycle(l, x, y); break;
     default: return -1;
   }

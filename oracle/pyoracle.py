@@ -71,8 +71,15 @@ def dpenalty(rho, q, ptype=0):
     return lib().orc_dpenalty(rho, q, ptype)
 
 
+# the Oracle whose matrix the C library holds (it keeps one assembled
+# operator at a time)
+_HOLDER = [None]
+
+
 class Oracle:
-    """Assembled FP64 operator A = Z^T K(rho) Z with clamped dofs."""
+    """Assembled FP64 operator A = Z^T K(rho) Z with clamped dofs.  The C
+    library holds one assembled operator; an Oracle whose matrix another
+    one has replaced assembles its own again before reading it."""
 
     def __init__(self, conn, dep_ptr, dep_conn, dep_w, hdim, bcmask, E=1.0, nu=0.3):
         self.conn = np.ascontiguousarray(conn, dtype=np.intc).reshape(-1)
@@ -97,9 +104,21 @@ class Oracle:
                                  _p(self.dep_conn), _p(self.dep_w), _p(self.hdim), _p(s),
                                  _p(self.bcmask), self.E, self.nu)
         self.nnz = nnz
+        self._made = (self.assemble, s)
+        _HOLDER[0] = self
         return nnz
 
+    def _hold(self):
+        """Make this Oracle's matrix the library's again if another Oracle
+        has assembled since."""
+        if _HOLDER[0] is not self:
+            if not hasattr(self, "_made"):
+                raise RuntimeError("oracle: assemble first")
+            fn, arg = self._made
+            fn(arg)
+
     def csr(self):
+        self._hold()
         n = 3 * self.nnodes
         rowp = np.zeros(n + 1, np.intc)
         cols = np.zeros(self.nnz, np.intc)
@@ -108,12 +127,14 @@ class Oracle:
         return rowp, cols, vals
 
     def matvec(self, x):
+        self._hold()
         x = np.ascontiguousarray(x, dtype=np.double)
         y = np.zeros_like(x)
         lib().orc_matvec(_p(x), _p(y))
         return y
 
     def solve(self, b, rtol=1e-12, maxit=20000):
+        self._hold()
         b = np.ascontiguousarray(b, dtype=np.double).copy()
         b[np.repeat(self.bcmask, 3).reshape(-1, 3).astype(bool).reshape(-1) &
           (((self.bcmask[:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))] = 0.0
@@ -170,6 +191,8 @@ class Oracle:
                                     _p(self.dep_conn), _p(self.dep_w), _p(self.hdim), _p(sgp),
                                     _p(self.bcmask), self.E, self.nu)
         self.nnz = nnz
+        self._made = (self.assemble_gp, sgp)
+        _HOLDER[0] = self
         return nnz
 
     def energy_gp(self, u, v):
@@ -504,6 +527,8 @@ def _mglib():
         L.orc_mg_set_lambda.argtypes = [I, D]
         L.orc_mg_op.restype = I
         L.orc_mg_op.argtypes = [I, I, P, P]
+        L.orc_mg_set_coarse.restype = None
+        L.orc_mg_set_coarse.argtypes = [I]
         L._mg_sigs = True
     return L
 
@@ -512,7 +537,8 @@ class Multigrid:
     """Galerkin multigrid on the oracle's assembled fine operator: levels
     from a finest-first forest list (their interpolation CSRs and clamped
     dofs), A_{l+1} = P~^T A_l P~.  smoother 'chebyshev' (degree, interval as
-    fractions of lambda_max(D^-1 A)) or 'sor' (symmetric, omega)."""
+    fractions of lambda_max(D^-1 A)) or 'sor' (symmetric, omega); the
+    coarsest level solved directly or (coarse_direct=False) smoothed once."""
 
     def __init__(self, orc, forests, bcs, interps=None):
         L = _mglib()
@@ -534,7 +560,9 @@ class Multigrid:
                                 _p(prp), _p(pcl), _p(pw))
             L.orc_mg_galerkin(l)
 
-    def setup(self, smoother="chebyshev", degree=2, lower=1.0 / 6.0, upper=1.1, omega=1.0):
+    def setup(self, smoother="chebyshev", degree=2, lower=1.0 / 6.0, upper=1.1, omega=1.0,
+              coarse_direct=True):
+        _mglib().orc_mg_set_coarse(1 if coarse_direct else 0)
         r = _mglib().orc_mg_setup(0 if smoother == "chebyshev" else 1, int(degree), lower, upper,
                                   omega)
         if r != 0:

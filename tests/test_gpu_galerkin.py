@@ -110,8 +110,6 @@ def test_mg_create_flags(adapt4):
     from tmr_amd import _lib
     with pytest.raises(_lib.TmrxError, match="SOR"):
         prob.set_multigrid(use_chebyshev_smoother=False)
-    with pytest.raises(_lib.TmrxError, match="coarsest"):
-        prob.set_multigrid(use_coarse_direct_solve=False)
     with pytest.raises(_lib.TmrxError, match="omega"):
         prob.set_multigrid(omega=2.5)
     prob.set_multigrid(use_galerkin=True)

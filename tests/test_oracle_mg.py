@@ -128,3 +128,53 @@ def test_single_components(adapt):
     assert np.linalg.norm(mg.matrix(lc) @ yc - bc) < 1e-10 * np.linalg.norm(bc)
     with pytest.raises(ValueError):
         mg.op("coarse", 0, bc)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_coarse_smoothing_is_the_chebyshev_polynomial(adapt, degree):
+    """use_coarse_direct_solve = 0 (TMR_CreateTACSMg hands the coarsest level
+    its smoother, src/TMR_RefinementTools.cpp:87-102): the coarse op is one
+    Chebyshev sweep from zero, whose residual is fixed by the textbook
+    identity D^-1 (b - A x_k) = T_k((theta - M) / delta) / T_k(theta / delta)
+    D^-1 b, M = D^-1 A, theta and delta the centre and half-width of
+    [lower, upper] lambda_max.  The identity is evaluated densely with the
+    three-term recurrence of T_k, not the smoother's update formulas."""
+    forests, rho, orc = adapt
+    lower, upper = 1.0 / 6.0, 1.1
+    mg = po.Multigrid(orc, forests, BCS)
+    mg.setup("chebyshev", degree=degree, lower=lower, upper=upper, coarse_direct=False)
+    lc = len(forests) - 1
+    A = mg.matrix(lc).toarray()
+    free = ~(((mg.masks[lc][:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))
+    lam = mg.get_lambda(lc)
+    assert lam > 1.0
+    rng = np.random.default_rng(11 + degree)
+    b = np.where(free, rng.standard_normal(len(free)), 0.0)
+    x = mg.op("coarse", lc, b)
+    dinv = 1.0 / np.diag(A)
+    M = dinv[:, None] * A
+
+    def residual(theta, delta):
+        S = (theta * np.eye(len(b)) - M) / delta
+        s0 = theta / delta
+        Tp, T = np.eye(len(b)), S.copy()          # T_0, T_1 of S
+        tp, t = 1.0, s0                           # T_0, T_1 of s0
+        for _ in range(degree - 1):
+            Tp, T = T, 2.0 * S @ T - Tp
+            tp, t = t, 2.0 * s0 * t - tp
+        return (T @ (dinv * b)) / t
+
+    theta, delta = 0.5 * (upper + lower) * lam, 0.5 * (upper - lower) * lam
+    got = dinv * (b - A @ x)
+    scale = np.linalg.norm(dinv * b)
+    err = np.linalg.norm(got - residual(theta, delta)) / scale
+    assert err < 1e-12, err
+    # a 1% wrong interval is caught (the check has teeth)
+    assert np.linalg.norm(got - residual(1.01 * theta, delta)) / scale > 1e-5
+    # the hierarchy with a smoothed coarsest level still converges to the
+    # Jacobi-PCG solution
+    f = traction_z(forests[0], orc)
+    u = mg.pcg(f, rtol=1e-11, maxit=800)
+    u_ref = orc.solve(f, rtol=1e-13)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-8
+    mg.setup("chebyshev", degree=2, coarse_direct=True)

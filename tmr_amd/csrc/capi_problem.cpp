@@ -306,12 +306,8 @@ int tmrx_mg_create(tmrx_problem *h, double omega, int use_galerkin, int use_coar
               "smooths with Chebyshev (use_chebyshev_smoother = 1)");
     return -1;
   }
-  if (!use_coarse_direct_solve) {
-    set_error("tmrx_mg_create: the coarsest level is always solved directly (FP64 Cholesky); "
-              "pass use_coarse_direct_solve = 1");
-    return -1;
-  }
   if (h->p->set_galerkin(use_galerkin != 0)) return fail(h);
+  if (h->p->set_coarse_direct(use_coarse_direct_solve != 0)) return fail(h);
   return 0;
 }
 

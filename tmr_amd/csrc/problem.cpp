@@ -1509,8 +1509,15 @@ int Problem::setup(const SolveOpts &o) {
       return -1;
     }
   }
+  // a smoothed Galerkin coarsest level applies its assembled P^T A P
+  if (galerkin_ && !coarse_direct_ && levels_.size() > 1 && !levels_.back().has_bsr &&
+      build_bsr((int)levels_.size() - 1))
+    return -1;
   if (galerkin_ && galerkin_setup(prec)) return -1;
-  for (size_t l = 0; l + 1 < levels_.size(); l++) {
+  // smoothed levels: all but the coarsest, which is solved directly unless
+  // coarse_direct_ is off (then it is smoothed like the others)
+  const size_t nsm = coarse_direct_ ? levels_.size() - 1 : levels_.size();
+  for (size_t l = 0; l < nsm; l++) {
     Level &L = levels_[l];
     if (galerkin_ && l > 0) {
       // Jacobi diagonal of the Galerkin operator
@@ -1535,10 +1542,12 @@ int Problem::setup(const SolveOpts &o) {
     if (estimate_eigs((int)l, o, true)) return -1;
   }
   setup_prec_pending_ = prec;
-  if (coarse_factor()) return -1;
-  // one read-back for every level's eigenvalue sums (the coarse factor has
-  // synchronised the stream already)
-  if (collect_eigs((int)levels_.size() - 1)) return -1;
+  if (coarse_direct_) {
+    levels_.back().lam_max = 0.0;
+    if (coarse_factor()) return -1;
+  }
+  // one read-back for every level's eigenvalue sums
+  if (collect_eigs((int)nsm)) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   have_setup_ = true;
   setup_precision_ = prec;
@@ -1582,7 +1591,9 @@ template <class T>
 T *Problem::first_target(int l, T *x, const SolveOpts &o) {
   const int deg = std::max(1, o.cheb_degree);
   const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : deg;
-  return ((pre - 1 + deg) & 1) ? vecs<T>(l).x2.p : x;
+  // a smoothed coarsest level runs one sweep of deg steps, no post-sweep
+  const int updates = l == (int)levels_.size() - 1 ? deg - 1 : pre - 1 + deg;
+  return (updates & 1) ? vecs<T>(l).x2.p : x;
 }
 
 template <class T>
@@ -1595,10 +1606,29 @@ double Problem::cheb_c0(int l, const SolveOpts &o) const {
 template <class T>
 void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool first_done) {
   const int last = (int)levels_.size() - 1;
-  if (l == last) {
+  if (l == last && coarse_direct_) {
     CoarseBlocks cb = coarse_;
     cb.run = run_;
     launch_dense_solve<T>(cb, b, x, st_, nrhs);
+    return;
+  }
+  if (l == last) {
+    // no direct coarse solve (TMR_CreateTACSMg with use_coarse_direct_solve
+    // = 0, src/TMR_RefinementTools.cpp:87-102): the level's smoother once,
+    // one Chebyshev sweep from zero
+    Level &L = levels_[l];
+    LevelVecs<T> &v = vecs<T>(l);
+    const int deg = std::max(1, o.cheb_degree);
+    T *cur = first_target<T>(l, x, o);
+    T *oth = cur == x ? v.x2.p : x;
+    if (!first_done)
+      launch_cheb_first<T>((int)L.ndof, b, v.dinv.p, cheb_c0<T>(l, o), nullptr, cur, st_, nrhs,
+                           run_);
+    for (int k = 1; k < deg; k++) {
+      const int fl = (k == 1 ? CHEB_D_IS_X : CHEB_D_READ) | (k + 1 < deg ? CHEB_D_STORE : 0);
+      cheb_step<T>(l, b, cur, oth, k, o, nrhs, fl);
+      std::swap(cur, oth);
+    }
     return;
   }
   Level &L = levels_[l];
@@ -1627,9 +1657,9 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool
   TransferCSR R = L.R, P = L.P;
   R.run = P.run = run_;
   // the coarse level's first smoothing step rides on the restriction (its
-  // spare buffer, where that level's vcycle starts); the coarsest level is
-  // solved directly
-  const bool fuse = l + 1 < last;
+  // spare buffer, where that level's vcycle starts), unless the coarsest
+  // level is solved directly
+  const bool fuse = l + 1 < last || !coarse_direct_;
   launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof,
                      fuse ? c.dinv.p : nullptr, fuse ? cheb_c0<T>(l + 1, o) : 0.0,
                      fuse ? first_target<T>(l + 1, c.x.p, o) : nullptr);
@@ -2190,7 +2220,11 @@ int Problem::mg_op_t(int op, int l, const double *x, double *y, bool on_device,
       break;
     }
     case MG_COARSE:
-      launch_dense_solve<T>(coarse_, in, vo.x.p, st_, 1);
+      // the direct solve, or the coarsest level's Chebyshev sweep
+      if (coarse_direct_)
+        launch_dense_solve<T>(coarse_, in, vo.x.p, st_, 1);
+      else
+        vcycle<T>(l, in, vo.x.p, o, 1);
       out = vo.x.p;
       break;
     default:  // MG_VCYCLE
@@ -2225,11 +2259,12 @@ int Problem::mg_op(int op, int l, const double *x, double *y, bool on_device,
     return -1;
   }
   if (op == MG_COARSE && l != nl - 1) {
-    last_error = "mg_op: the direct solve is on the coarsest level " + std::to_string(nl - 1);
+    last_error = "mg_op: the coarse solve is on the coarsest level " + std::to_string(nl - 1);
     return -1;
   }
-  if (op == MG_APPLY && galerkin_ && l > 0 && l == nl - 1) {
-    last_error = "mg_op: the coarsest Galerkin operator is kept factored only";
+  if (op == MG_APPLY && galerkin_ && l > 0 && l == nl - 1 && coarse_direct_) {
+    last_error = "mg_op: the coarsest Galerkin operator of a direct coarse solve is kept "
+                 "factored only";
     return -1;
   }
   if (!x || !y) {
@@ -2316,6 +2351,16 @@ int Problem::traction_load(const char *name, const double tr[3], double *f) {
 // every level and time the finest-level operator (FP32 product on the matrix
 // cores 4x, as in a V-cycle's smoothing and residual, plus one FP64 product,
 // as in a PCG iteration); keep the fastest.  Needs a design.
+int Problem::set_coarse_direct(bool on) {
+  if (on != coarse_direct_) {
+    coarse_direct_ = on;
+    have_setup_ = false;
+    drop_graphs();
+    gen_++;
+  }
+  return 0;
+}
+
 int Problem::set_tiling(int tile_nodes) {
   if (levels_.empty()) {
     last_error = "set_tiling: not initialised";

@@ -235,6 +235,9 @@ class Problem {
   // owner tiles of tile_nodes at the finest level (coarse levels by the
   // autotuner's rule); overrides the autotuner until the next autotune
   int set_tiling(int tile_nodes);
+  // the coarsest level solved directly (default) or smoothed like the others
+  int set_coarse_direct(bool on);
+  bool coarse_direct() const { return coarse_direct_; }
   int set_state(const double *f, const double *u, bool on_device);
   // multigrid options of TMR_CreateTACSMg (src/TMR_RefinementTools.cpp:37-107):
   // Galerkin coarse operators P^T A P on the coarse levels (else
@@ -348,6 +351,7 @@ class Problem {
   DevArray<float> G32_;
   int design_mode_ = 0;      // 0 element densities, 1 nodal
   bool galerkin_ = false, galerkin_built_ = false;
+  bool coarse_direct_ = true;
   DevArray<double> Kgd_;  // Kgh_ on the device (Galerkin with a nodal design)
   DevArray<double> rho_gp_, egp_, exgp_, gtmp_;
   DevArray<uint8_t> nomask_;  // all dofs free (full K, unmasked state loads)

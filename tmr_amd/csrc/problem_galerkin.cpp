@@ -274,7 +274,8 @@ int Problem::level_matrix(int l, std::vector<int> &ptr, std::vector<int> &col,
   const int nl = (int)levels_.size();
   // smoothed Galerkin levels, and the coarsest level (factored from its BSR
   // matrix in either mode)
-  if (l < 1 || l >= nl || !levels_[l].has_bsr || (!galerkin_ && l != nl - 1)) {
+  if (l < 1 || l >= nl || !levels_[l].has_bsr || (!galerkin_ && l != nl - 1) ||
+      (!galerkin_ && !coarse_direct_)) {
     last_error = "level_matrix: no assembled operator on that level";
     return -1;
   }

@@ -211,7 +211,8 @@ class Problem:
     def set_multigrid(self, omega=1.0, use_galerkin=False, use_coarse_direct_solve=True,
                       use_chebyshev_smoother=True):
         """TMR_CreateTACSMg's options on this hierarchy (tmrx_mg_create):
-        Galerkin coarse operators P^T A P or rediscretised coarse levels."""
+        Galerkin coarse operators P^T A P or rediscretised coarse levels; the
+        coarsest level solved directly or smoothed (one Chebyshev sweep)."""
         check(lib().tmrx_mg_create(self.ptr, float(omega), int(bool(use_galerkin)),
                                    int(bool(use_coarse_direct_solve)),
                                    int(bool(use_chebyshev_smoother))), "set_multigrid")
