@@ -289,9 +289,13 @@ def bench_adaptive(args):
     if not args.no_autotune:
         prob.autotune(args.tiles, reps=3)
     legs = {}
-    for name, prec, gal in (("fp64", 0, False), ("fp64_galerkin", 0, True),
-                            ("mixed", 1, False), ("mixed_galerkin", 1, True)):
-        prob.set_multigrid(use_galerkin=gal)
+    plan = [("fp64", 0, False, False), ("fp64_galerkin", 0, True, False),
+            ("mixed", 1, False, False), ("mixed_galerkin", 1, True, False)]
+    if args.sor:  # TMR's default smoother, symmetric SOR on the assembled levels
+        plan += [("fp64_sor", 0, False, True), ("mixed_sor", 1, False, True),
+                 ("mixed_galerkin_sor", 1, True, True)]
+    for name, prec, gal, sor in plan:
+        prob.set_multigrid(use_galerkin=gal, use_chebyshev_smoother=not sor)
         opts = solver.default_opts(rtol=1e-8, precision=prec, max_iters=400,
                                    no_graph=int(args.no_graph))
         torch.cuda.synchronize()
@@ -462,6 +466,8 @@ def main():
                          "2.1M-hex cube; batch: 96 uneven independent cantilevers; adaptive: "
                          "the 1.2M-hex adaptive forest; adjoint: its gradients; forest: host vs "
                          "GPU forest setup")
+    ap.add_argument("--sor", action="store_true",
+                    help="adaptive workload: add legs with symmetric SOR smoothing")
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
                     help="after the solves, copy MIB MiB device to device (a known-size "

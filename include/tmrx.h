@@ -328,17 +328,31 @@ void tmrx_problem_free(tmrx_problem *p);
    use_galerkin: coarse operators P^T A P (TACSMg's Galerkin mode), formed on
      the GPU at every setup from the finer level's element matrices; 0 keeps
      rediscretised coarse elements (volume-averaged material factors).
-   use_chebyshev_smoother: must be 1.  TMR's alternative, symmetric SOR
-     (TACSGaussSeidel, :93-94), is not available: an error is returned.
+   use_chebyshev_smoother: 1 smooths with Chebyshev on D^-1 A
+     (TACSChebyshevSmoother, :64-71); 0 with symmetric point SOR, one
+     forward and one backward sweep before and after the coarse-grid
+     correction, relaxation omega (TACSGaussSeidel, :93-94), on every
+     smoothed level's assembled matrix.  The dofs are swept in colour-major
+     node order (a greedy colouring of the level's node graph, so that a
+     colour updates in parallel), not in TACS's row order: the smoother is
+     the same SOR, its iterates follow that order.
    use_coarse_direct_solve: 1 solves the coarsest level directly (FP64
      Cholesky, TACSMg's direct solve, :83-86); 0 smooths it like the other
      levels, one Chebyshev sweep of cheb_degree steps from zero (the
      smoother TMR hands TACSMg there, :87-102).
-   omega: TMR's SOR relaxation factor, checked to lie in (0, 2) and otherwise
-     unused by Chebyshev smoothing.
+   omega: the SOR relaxation factor, in (0, 2) (unused by Chebyshev).
    Invalidates the setup; the next solve redoes it. */
 int tmrx_mg_create(tmrx_problem *p, double omega, int use_galerkin, int use_coarse_direct_solve,
                    int use_chebyshev_smoother);
+/* No counterpart in the reference: the SOR sweep order of level `level`
+   after a setup with use_chebyshev_smoother = 0, for checking the smoother
+   against a sequential SOR in the same order.  nnodes receives the level's
+   independent nodes, ncolors the colours; nodes[nnodes] (colour-major) and
+   color_ptr[ncolors + 1] (colour c = nodes[color_ptr[c] .. color_ptr[c + 1]))
+   are filled when not NULL.  Within a node the dofs go 0, 1, 2 forward and
+   2, 1, 0 backward. */
+int tmrx_problem_smoother_order(tmrx_problem *p, int level, int *nnodes, int *ncolors, int *nodes,
+                                int *color_ptr);
 /* Sizes of the finest level. */
 int tmrx_problem_get_sizes(tmrx_problem *p, int *ndof, int *nelem, int *nlevels);
 /* HIP stream (hipStream_t) used by every later call; NULL = default stream. */

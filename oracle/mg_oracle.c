@@ -44,6 +44,7 @@ static struct {
   double lam[MG_MAX];
   double *Lc; /* dense Cholesky factor of the coarsest level (lower, row-major) */
   double *w1[MG_MAX], *w2[MG_MAX], *w3[MG_MAX], *w4[MG_MAX];
+  int *order[MG_MAX]; /* SOR node order per level (NULL: natural) */
   int smoother; /* 0 Chebyshev, 1 symmetric SOR */
   int coarse_smooth; /* 1: the coarsest level is smoothed, not solved */
   int degree;
@@ -68,6 +69,8 @@ void orc_mg_free(void) {
     free(g.w2[l]);
     free(g.w3[l]);
     free(g.w4[l]);
+    free(g.order[l]);
+    g.order[l] = NULL;
     g.free_[l] = NULL;
     g.dinv[l] = g.w1[l] = g.w2[l] = g.w3[l] = g.w4[l] = NULL;
   }
@@ -372,13 +375,30 @@ static void coarse_solve(const double *b, double *x) {
   }
 }
 
-/* one forward and one backward SOR sweep on A x = b (x updated in place) */
+/* the order of level l's SOR sweeps: the rows of node order[0], then of
+   order[1], ... (each node's 3 dofs in turn); NULL restores row order */
+int orc_mg_set_order(int l, int nnodes, const int *order) {
+  if (l < 0 || l >= MG_MAX) return -1;
+  free(g.order[l]);
+  g.order[l] = NULL;
+  if (!order) return 0;
+  if (3 * nnodes != g.A[l].n) return -2;
+  g.order[l] = (int *)malloc(sizeof(int) * (size_t)nnodes);
+  memcpy(g.order[l], order, sizeof(int) * (size_t)nnodes);
+  return 0;
+}
+
+/* one forward and one backward SOR sweep on A x = b (x updated in place),
+   rows in the level's order (orc_mg_set_order); the backward sweep visits
+   them in reverse */
 static void sor_symmetric(int l, const double *b, double *x) {
   const mat_t *A = &g.A[l];
   const double w = g.omega;
+  const int *ord = g.order[l];
   for (int pass = 0; pass < 2; pass++)
     for (int t = 0; t < A->n; t++) {
-      const int i = pass == 0 ? t : A->n - 1 - t;
+      const int u = pass == 0 ? t : A->n - 1 - t;
+      const int i = ord ? 3 * ord[u / 3] + u % 3 : u;
       double s = b[i], d = 1.0;
       for (int k = A->rowp[i]; k < A->rowp[i + 1]; k++) {
         const int j = A->cols[k];

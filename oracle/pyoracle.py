@@ -529,6 +529,8 @@ def _mglib():
         L.orc_mg_op.argtypes = [I, I, P, P]
         L.orc_mg_set_coarse.restype = None
         L.orc_mg_set_coarse.argtypes = [I]
+        L.orc_mg_set_order.restype = I
+        L.orc_mg_set_order.argtypes = [I, I, P]
         L._mg_sigs = True
     return L
 
@@ -567,6 +569,17 @@ class Multigrid:
                                   omega)
         if r != 0:
             raise RuntimeError("oracle multigrid setup failed (%d)" % r)
+
+    def set_order(self, l, nodes):
+        """SOR sweep order of level l: node indices (each node's 3 dofs in
+        turn); None restores row order."""
+        if nodes is None:
+            r = _mglib().orc_mg_set_order(int(l), 0, None)
+        else:
+            nodes = np.ascontiguousarray(nodes, dtype=np.intc)
+            r = _mglib().orc_mg_set_order(int(l), len(nodes), _p(nodes))
+        if r != 0:
+            raise ValueError("oracle: bad SOR order for level %d" % l)
 
     def matrix(self, l):
         import scipy.sparse as sp

@@ -108,8 +108,8 @@ def test_galerkin_solve_parity_and_iterations(adapt4, precision):
 def test_mg_create_flags(adapt4):
     forests, rho, prob = adapt4
     from tmr_amd import _lib
-    with pytest.raises(_lib.TmrxError, match="SOR"):
-        prob.set_multigrid(use_chebyshev_smoother=False)
     with pytest.raises(_lib.TmrxError, match="omega"):
         prob.set_multigrid(omega=2.5)
+    with pytest.raises(_lib.TmrxError, match="omega"):
+        prob.set_multigrid(omega=0.0, use_chebyshev_smoother=False)
     prob.set_multigrid(use_galerkin=True)

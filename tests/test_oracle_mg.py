@@ -178,3 +178,47 @@ def test_coarse_smoothing_is_the_chebyshev_polynomial(adapt, degree):
     u_ref = orc.solve(f, rtol=1e-13)
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-8
     mg.setup("chebyshev", degree=2, coarse_direct=True)
+
+
+@pytest.mark.parametrize("omega", [1.0, 0.7])
+def test_ordered_sor_is_the_ssor_formula(adapt, omega):
+    """The oracle's symmetric SOR in a given node order (the check of the
+    GPU's colour-ordered SOR, TACSGaussSeidel's role in TMR_CreateTACSMg,
+    src/TMR_RefinementTools.cpp:93-94) equals the textbook sweeps on the
+    permuted system A' = Q A Q^T, split D' + L' + U': forward
+    (D' + w L') x1 = w b' - (w U' + (w - 1) D') x0 from x0 = 0, backward
+    (D' + w U') x2 = w b' - (w L' + (w - 1) D') x1, by dense triangular
+    solves."""
+    import scipy.linalg as sl
+    forests, rho, orc = adapt
+    mg = po.Multigrid(orc, forests, BCS)
+    mg.setup("sor", omega=omega, coarse_direct=False)
+    rng = np.random.default_rng(int(omega * 10))
+    nl = len(forests)
+    for l, op in [(nl - 2, "smooth"), (nl - 1, "coarse")]:
+        A = mg.matrix(l).toarray()
+        nn = A.shape[0] // 3
+        order = rng.permutation(nn).astype(np.intc)
+        mg.set_order(l, order)
+        free = ~(((mg.masks[l][:, None] >> np.arange(3)) & 1).astype(bool).reshape(-1))
+        b = np.where(free, rng.standard_normal(len(free)), 0.0)
+        y = mg.op(op, l, b)
+        perm = (3 * order[:, None] + np.arange(3)).reshape(-1)
+        Ap, bp = A[np.ix_(perm, perm)], b[perm]
+        D = np.diag(np.diag(Ap))
+        Lo, Up = np.tril(Ap, -1), np.triu(Ap, 1)
+        x1 = sl.solve_triangular(D + omega * Lo, omega * bp, lower=True)
+        x2 = sl.solve_triangular(D + omega * Up, omega * bp - (omega * Lo + (omega - 1) * D) @ x1,
+                                 lower=False)
+        want = np.empty_like(x2)
+        want[perm] = x2
+        err = np.linalg.norm(y - want) / np.linalg.norm(want)
+        assert err < 1e-12, (l, err)
+        # the natural order gives another result (the order is honoured)
+        mg.set_order(l, None)
+        assert np.linalg.norm(mg.op(op, l, b) - want) / np.linalg.norm(want) > 1e-6
+    # SOR-MG with a smoothed coarsest level still solves the system
+    f = traction_z(forests[0], orc)
+    u = mg.pcg(f, rtol=1e-11, maxit=800)
+    u_ref = orc.solve(f, rtol=1e-13)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-8

@@ -97,6 +97,7 @@ _SIGS = {
                                 C.POINTER(P)]),
     "tmrx_problem_free": (None, [P]),
     "tmrx_mg_create": (I, [P, D, I, I, I]),
+    "tmrx_problem_smoother_order": (I, [P, I, IP, IP, IP, IP]),
     "tmrx_problem_level_matrix": (I, [P, I, IP, IP, P, P, P]),
     "tmrx_problem_get_sizes": (I, [P, IP, IP, IP]),
     "tmrx_problem_set_stream": (I, [P, P]),

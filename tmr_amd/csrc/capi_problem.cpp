@@ -301,13 +301,21 @@ int tmrx_mg_create(tmrx_problem *h, double omega, int use_galerkin, int use_coar
     set_error("tmrx_mg_create: omega must lie in (0, 2)");
     return -1;
   }
-  if (!use_chebyshev_smoother) {
-    set_error("tmrx_mg_create: symmetric SOR smoothing is not available; the GPU hierarchy "
-              "smooths with Chebyshev (use_chebyshev_smoother = 1)");
-    return -1;
-  }
   if (h->p->set_galerkin(use_galerkin != 0)) return fail(h);
   if (h->p->set_coarse_direct(use_coarse_direct_solve != 0)) return fail(h);
+  if (h->p->set_smoother(use_chebyshev_smoother == 0, omega)) return fail(h);
+  return 0;
+}
+
+int tmrx_problem_smoother_order(tmrx_problem *h, int level, int *nnodes, int *ncolors, int *nodes,
+                                int *color_ptr) {
+  CHECK_P(h);
+  std::vector<int> nd, cp;
+  if (h->p->smoother_order(level, nd, cp)) return fail(h);
+  if (nnodes) *nnodes = (int)nd.size();
+  if (ncolors) *ncolors = (int)cp.size() - 1;
+  if (nodes) std::copy(nd.begin(), nd.end(), nodes);
+  if (color_ptr) std::copy(cp.begin(), cp.end(), color_ptr);
   return 0;
 }
 

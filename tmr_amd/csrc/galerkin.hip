@@ -298,6 +298,100 @@ void launch_bsr_apply(const BSR<T> &A, const T *x, int mode, const AsmArgs &g, h
   TMRX_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------
+// One colour of a point SOR sweep on the BSR matrix (TACSGaussSeidel's
+// smoother, symmetric SOR with relaxation omega; TMR_CreateTACSMg,
+// src/TMR_RefinementTools.cpp:93-94).  Every node of the colour takes
+// SOR_LANES lanes: they sum the row's off-diagonal blocks (consecutive
+// lanes on consecutive blocks, which read x of other colours only, so the
+// nodes of a colour update side by side) and a butterfly combines them;
+// then the node's three dofs update one after another through the diagonal
+// block, 0, 1, 2 (BACK: 2, 1, 0), each
+// x_p <- (1 - omega) x_p + omega (b_p - sum_{q != p} a_pq x_q) / a_pp --
+// the point SOR of the dofs in colour-major node order.
+constexpr int SOR_LANES = 8;
+
+template <class T, bool BACK>
+__global__ __launch_bounds__(256) void k_sor_colour(int cnt, const int *__restrict__ nodes,
+                                                    const int *__restrict__ ptr,
+                                                    const int *__restrict__ col,
+                                                    const T *__restrict__ val,
+                                                    const T *__restrict__ b, T *__restrict__ x,
+                                                    T omega, const int *__restrict__ run,
+                                                    int64_t vs) {
+  if (run && *run == 0) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = g / SOR_LANES, sub = g % SOR_LANES;
+  const bool valid = k < cnt;
+  const int64_t off = (int64_t)blockIdx.y * vs;
+  x += off;
+  b += off;
+  const int i = valid ? nodes[k] : 0;
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  int kd = -1;
+  if (valid) {
+    const int q1 = ptr[i + 1];
+    for (int q = ptr[i] + sub; q < q1; q += SOR_LANES) {
+      const int j = col[q];
+      if (j == i) {
+        kd = q;
+        continue;
+      }
+      const T *v = val + 9 * (size_t)q;
+      const T x0 = x[3 * (size_t)j], x1 = x[3 * (size_t)j + 1], x2 = x[3 * (size_t)j + 2];
+      s0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+      s1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+      s2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+  }
+#pragma unroll
+  for (int m = 1; m < SOR_LANES; m <<= 1) {
+    s0 += __shfl_xor(s0, m);
+    s1 += __shfl_xor(s1, m);
+    s2 += __shfl_xor(s2, m);
+    kd = max(kd, __shfl_xor(kd, m));
+  }
+  if (!valid || sub != 0 || kd < 0) return;
+  const T *D = val + 9 * (size_t)kd;
+  const size_t i3 = 3 * (size_t)i;
+  T xi[3] = {x[i3], x[i3 + 1], x[i3 + 2]};
+  const T r[3] = {b[i3] - s0, b[i3 + 1] - s1, b[i3 + 2] - s2};
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const int p = BACK ? 2 - t : t;
+    T acc = r[p];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      if (q != p) acc -= D[3 * p + q] * xi[q];
+    xi[p] = (T(1) - omega) * xi[p] + omega * acc / D[4 * p];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) x[i3 + c] = xi[c];
+}
+
+template <class T>
+void launch_sor_sweep(const BSR<T> &A, const int *nodes, const int *cptr, int ncolors, bool back,
+                      const T *b, T *x, double omega, hipStream_t st, int nrhs, int64_t vs) {
+  for (int t = 0; t < ncolors; t++) {
+    const int c = back ? ncolors - 1 - t : t;
+    const int c0 = cptr[c], cnt = cptr[c + 1] - c0;
+    if (cnt == 0) continue;
+    const dim3 grid((unsigned)(((int64_t)cnt * SOR_LANES + 255) / 256), (unsigned)nrhs);
+    if (back)
+      hipLaunchKernelGGL((k_sor_colour<T, true>), grid, dim3(256), 0, st, cnt, nodes + c0, A.ptr,
+                         A.col, A.val, b, x, (T)omega, A.run, vs);
+    else
+      hipLaunchKernelGGL((k_sor_colour<T, false>), grid, dim3(256), 0, st, cnt, nodes + c0, A.ptr,
+                         A.col, A.val, b, x, (T)omega, A.run, vs);
+    TMRX_LAUNCH_CHECK();
+  }
+}
+
+template void launch_sor_sweep<float>(const BSR<float> &, const int *, const int *, int, bool,
+                                      const float *, float *, double, hipStream_t, int, int64_t);
+template void launch_sor_sweep<double>(const BSR<double> &, const int *, const int *, int, bool,
+                                       const double *, double *, double, hipStream_t, int,
+                                       int64_t);
 template void launch_bsr_dinv<float>(int, const int *, const double *, float *, hipStream_t);
 template void launch_bsr_dinv<double>(int, const int *, const double *, double *, hipStream_t);
 template void launch_bsr_apply<float>(const BSR<float> &, const float *, int, const AsmArgs &,

@@ -293,6 +293,14 @@ template <class T>
 void launch_bsr_apply(const BSR<T> &A, const T *x, int mode, const AsmArgs &g, hipStream_t st,
                       int nrhs = 1, int64_t vs = 0);
 
+// One point SOR sweep (forward, or BACK in reverse) over the colours of a
+// node colouring of A's graph: nodes[cptr[c] .. cptr[c + 1]) hold colour c
+// (cptr on the host, ncolors + 1 entries); x updated in place
+template <class T>
+void launch_sor_sweep(const BSR<T> &A, const int *nodes, const int *cptr, int ncolors, bool back,
+                      const T *b, T *x, double omega, hipStream_t st, int nrhs = 1,
+                      int64_t vs = 0);
+
 template <class TI, class TO>
 void launch_convert(int64_t n, const TI *a, TO *b, hipStream_t st);
 template <class T>

@@ -1513,12 +1513,22 @@ int Problem::setup(const SolveOpts &o) {
   if (galerkin_ && !coarse_direct_ && levels_.size() > 1 && !levels_.back().has_bsr &&
       build_bsr((int)levels_.size() - 1))
     return -1;
-  if (galerkin_ && galerkin_setup(prec)) return -1;
   // smoothed levels: all but the coarsest, which is solved directly unless
   // coarse_direct_ is off (then it is smoothed like the others)
   const size_t nsm = coarse_direct_ ? levels_.size() - 1 : levels_.size();
+  // SOR smooths the assembled matrices: patterns and colourings first, so
+  // that galerkin_setup fills the Galerkin levels' values
+  if (sor_)
+    for (size_t l = 0; l < nsm; l++)
+      if (!levels_[l].has_sor && build_sor((int)l)) return -1;
+  if (galerkin_ && galerkin_setup(prec)) return -1;
   for (size_t l = 0; l < nsm; l++) {
     Level &L = levels_[l];
+    if (sor_) {
+      if (sor_assemble((int)l, prec)) return -1;
+      L.lam_max = 0.0;
+      continue;
+    }
     if (galerkin_ && l > 0) {
       // Jacobi diagonal of the Galerkin operator
       launch_bsr_dinv<double>(L.n_nodes, L.bsr_diag.p, L.bsr64.p, L.v64.dinv.p, st_);
@@ -1547,7 +1557,7 @@ int Problem::setup(const SolveOpts &o) {
     if (coarse_factor()) return -1;
   }
   // one read-back for every level's eigenvalue sums
-  if (collect_eigs((int)nsm)) return -1;
+  if (!sor_ && collect_eigs((int)nsm)) return -1;
   HIPCHK(hipStreamSynchronize(st_));
   have_setup_ = true;
   setup_precision_ = prec;
@@ -1612,6 +1622,10 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool
     launch_dense_solve<T>(cb, b, x, st_, nrhs);
     return;
   }
+  if (l == last && sor_) {
+    sor_smooth<T>(l, b, x, nrhs, true);
+    return;
+  }
   if (l == last) {
     // no direct coarse solve (TMR_CreateTACSMg with use_coarse_direct_solve
     // = 0, src/TMR_RefinementTools.cpp:87-102): the level's smoother once,
@@ -1635,6 +1649,24 @@ void Problem::vcycle(int l, const T *b, T *x, const SolveOpts &o, int nrhs, bool
   Level &Lc = levels_[l + 1];
   LevelVecs<T> &v = vecs<T>(l);
   LevelVecs<T> &c = vecs<T>(l + 1);
+  if (sor_) {
+    // TACSMg with TACSGaussSeidel (TMR_CreateTACSMg, src/TMR_RefinementTools.cpp:
+    // 45-52, 93-94): one symmetric SOR sweep before and after the
+    // coarse-grid correction
+    sor_smooth<T>(l, b, x, nrhs, true);
+    AsmArgs a;
+    a.b = b;
+    a.y = v.r.p;
+    apply_level<T>(l, x, ASM_RESID, a, nrhs);
+    TransferCSR R = L.R, P = L.P;
+    R.run = P.run = run_;
+    launch_restrict<T>(R, Lc.d_bcmask.p, v.r.p, c.b.p, st_, nrhs, L.ndof, Lc.ndof, nullptr, 0.0,
+                       nullptr);
+    vcycle<T>(l + 1, c.b.p, c.x.p, o, nrhs, false);
+    launch_prolong_add<T>(P, L.d_bcmask.p, c.x.p, x, st_, nrhs, Lc.ndof, L.ndof);
+    sor_smooth<T>(l, b, x, nrhs, false);
+    return;
+  }
   const int deg = std::max(1, o.cheb_degree);
   const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : deg;
   // pre - 1 + deg ping-pong updates follow the first step: it goes where
@@ -1747,9 +1779,10 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   // mixed precision: the finest level's first smoothing step rides on the
   // residual update (into the spare buffer where the V-cycle starts)
   Level &L0u = levels_[0];
+  const bool fuse0 = mixed && !sor_;
   launch_b_update_xr(pl, r_.p, q_.p, r32, sc_.p, act_.p, parts_.p, st_,
-                     mixed ? L0u.v32.dinv.p : nullptr, mixed ? cheb_c0<float>(0, o) : 0.0,
-                     mixed ? first_target<float>(0, L0u.v32.x.p, o) : nullptr);
+                     fuse0 ? L0u.v32.dinv.p : nullptr, fuse0 ? cheb_c0<float>(0, o) : 0.0,
+                     fuse0 ? first_target<float>(0, L0u.v32.x.p, o) : nullptr);
   launch_b_finalize(pl, FIN_RR, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2, atol2,
                     st_);
   // count the segments still running (and advance the iteration counter)
@@ -1759,7 +1792,7 @@ void Problem::enqueue_iteration(const SegPlan &pl, int nrhs, double *xsol, const
   // directly (no FP64 copy of z)
   if (mixed) {
     Level &L0 = levels_[0];
-    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs, true);
+    vcycle<float>(0, L0.v32.b.p, L0.v32.x.p, o, nrhs, fuse0);
     launch_b_dot2<float>(pl, L0.v32.x.p, r_.p, q_.p, act_.p, parts_.p, st_);
     launch_b_finalize(pl, FIN_BETA, parts_.p, sc_.p, act_.p, thr_.p, iter_.p, itconv_.p, rtol2,
                       atol2, st_);
@@ -2207,6 +2240,11 @@ int Problem::mg_op_t(int op, int l, const double *x, double *y, bool on_device,
       out = vo.x.p;
       break;
     case MG_SMOOTH: {
+      if (sor_) {  // one symmetric SOR sweep from zero
+        sor_smooth<T>(l, in, vo.x.p, 1, true);
+        out = vo.x.p;
+        break;
+      }
       // the V-cycle's pre-smoothing: c0 Dinv b, then pre - 1 ping-pong steps
       const int pre = o.cheb_pre_degree > 0 ? o.cheb_pre_degree : std::max(1, o.cheb_degree);
       T *cur = vo.x.p, *oth = vo.x2.p;

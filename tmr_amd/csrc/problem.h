@@ -166,9 +166,15 @@ struct Level {
   DevArray<int> bsr_ptr, bsr_col, bsr_row, bsr_diag, bt_ptr, bt_code;
   DevArray<double> bt_w, bsr64;
   DevArray<float> bsr32;
-  std::vector<int> bsr_ptr_h;  // host copy of bsr_ptr
+  std::vector<int> bsr_ptr_h, bsr_col_h;  // host copies of bsr_ptr, bsr_col
   int nnzb = 0;
   bool has_bsr = false;
+  // SOR smoothing: the level's nodes in colour-major order (a colouring of
+  // the BSR graph, greedy in node order), colour c in
+  // sor_nodes[sor_cptr[c] .. sor_cptr[c + 1])
+  DevArray<int> sor_nodes;
+  std::vector<int> sor_cptr, sor_nodes_h;
+  bool has_sor = false;
   // diagonal terms of corner pairs that share an independent node
   // (elements with hanging corners): per node i, codes 64 e + 8 a + b and
   // weights 2 Z_ai Z_bi
@@ -237,6 +243,12 @@ class Problem {
   int set_tiling(int tile_nodes);
   // the coarsest level solved directly (default) or smoothed like the others
   int set_coarse_direct(bool on);
+  // smoother of every smoothed level: Chebyshev (default) or symmetric
+  // point SOR with relaxation omega on the assembled level matrices
+  int set_smoother(bool sor, double omega);
+  bool sor() const { return sor_; }
+  // level l's SOR node order (colour-major) and its colour offsets
+  int smoother_order(int l, std::vector<int> &nodes, std::vector<int> &cptr);
   bool coarse_direct() const { return coarse_direct_; }
   int set_state(const double *f, const double *u, bool on_device);
   // multigrid options of TMR_CreateTACSMg (src/TMR_RefinementTools.cpp:37-107):
@@ -290,6 +302,14 @@ class Problem {
   int build_galerkin();
   int build_bsr(int l);
   int galerkin_setup(int precision);
+  // SOR: colouring of level l (BSR pattern built if missing), and the
+  // values of the smoothed levels' matrices at setup
+  int build_sor(int l);
+  int sor_assemble(int l, int precision);
+  // symmetric SOR (forward then backward sweep) of level l on A x = b,
+  // from x = 0 when from_zero
+  template <class T>
+  void sor_smooth(int l, const T *b, T *x, int nrhs, bool from_zero);
   template <class T>
   BSR<T> bsr(int l);
   template <class T>
@@ -352,6 +372,8 @@ class Problem {
   int design_mode_ = 0;      // 0 element densities, 1 nodal
   bool galerkin_ = false, galerkin_built_ = false;
   bool coarse_direct_ = true;
+  bool sor_ = false;
+  double omega_ = 1.0;
   DevArray<double> Kgd_;  // Kgh_ on the device (Galerkin with a nodal design)
   DevArray<double> rho_gp_, egp_, exgp_, gtmp_;
   DevArray<uint8_t> nomask_;  // all dofs free (full K, unmasked state loads)

@@ -225,6 +225,7 @@ int Problem::build_bsr(int l) {
   }
   L.nnzb = (int)col.size();
   L.bsr_ptr_h = ptr;
+  L.bsr_col_h = col;
   if (L.bsr_ptr.upload(ptr) || L.bsr_col.upload(col) || L.bsr_row.upload(brow) ||
       L.bsr_diag.upload(diag) || L.bt_ptr.upload(tptr) || L.bt_code.upload(tcode) ||
       L.bt_w.upload(tw) || L.bsr64.alloc(9 * (size_t)L.nnzb) || L.bsr32.alloc(9 * (size_t)L.nnzb)) {
@@ -291,6 +292,124 @@ int Problem::level_matrix(int l, std::vector<int> &ptr, std::vector<int> &col,
   GCHK(hipMemcpy(ptr.data(), L.bsr_ptr.p, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
   GCHK(hipMemcpy(col.data(), L.bsr_col.p, col.size() * sizeof(int), hipMemcpyDeviceToHost));
   GCHK(hipMemcpy(val.data(), L.bsr64.p, val.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // namespace tmrx
+
+namespace tmrx {
+
+// Greedy colouring of level l's node graph (the BSR pattern, symmetric):
+// nodes in index order take the smallest colour none of their coloured
+// neighbours holds.  Nodes of one colour share no block, so a colour's SOR
+// updates are independent; the order is checked before it is uploaded.
+int Problem::build_sor(int l) {
+  Level &L = levels_[l];
+  if (!L.has_bsr && build_bsr(l)) return -1;
+  const int n = L.n_nodes;
+  const std::vector<int> &ptr = L.bsr_ptr_h, &col = L.bsr_col_h;
+  std::vector<int> colour(n, -1);
+  int nc = 0;
+  for (int i = 0; i < n; i++) {
+    uint64_t used = 0;
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) {
+      const int j = col[q];
+      if (j != i && colour[j] >= 0) used |= uint64_t(1) << colour[j];
+    }
+    if (used == ~uint64_t(0)) {
+      last_error = "sor: level " + std::to_string(l) + " needs more than 64 colours";
+      return -1;
+    }
+    int c = 0;
+    while (used & (uint64_t(1) << c)) c++;
+    colour[i] = c;
+    nc = std::max(nc, c + 1);
+  }
+  for (int i = 0; i < n; i++)
+    for (int q = ptr[i]; q < ptr[i + 1]; q++)
+      if (col[q] != i && colour[col[q]] == colour[i]) {
+        last_error = "sor: colouring of level " + std::to_string(l) + " puts coupled nodes " +
+                     std::to_string(i) + " and " + std::to_string(col[q]) + " in one colour";
+        return -1;
+      }
+  std::vector<int> cptr(nc + 1, 0), nodes(n);
+  for (int i = 0; i < n; i++) cptr[colour[i] + 1]++;
+  for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+  std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+  for (int i = 0; i < n; i++) nodes[fill[colour[i]]++] = i;
+  if (L.sor_nodes.upload(nodes)) {
+    last_error = "sor: device allocation failed";
+    return -1;
+  }
+  L.sor_cptr = cptr;
+  L.sor_nodes_h = nodes;
+  L.has_sor = true;
+  return 0;
+}
+
+// The matrix SOR smooths at level l, assembled at every setup: a Galerkin
+// level's P^T A P (galerkin_setup has formed it), else the scaled element
+// matrices of the level scattered into its BSR pattern; FP32 values for
+// the mixed-precision V-cycle.
+int Problem::sor_assemble(int l, int precision) {
+  Level &L = levels_[l];
+  if (!L.has_sor && build_sor(l)) return -1;
+  if (!(galerkin_ && l > 0)) {
+    const bool nodal = design_mode_ == 1;
+    if ((L.KE.n < 576 * (size_t)L.n_elem && L.KE.alloc(576 * (size_t)L.n_elem)) ||
+        (nodal && Kgd_.n == 0 && Kgd_.upload(Kgh_))) {
+      last_error = "sor: device allocation failed";
+      return -1;
+    }
+    launch_elem_matrices(L.n_elem, L.d_cls.p, nodal ? L.sg64.p : L.s64.p, nodal ? Kgd_.p : K0d_.p,
+                         nodal, L.KE.p, st_);
+    launch_bsr_assemble(bsr<double>(l), L.bt_ptr.p, L.bt_code.p, L.bt_w.p, L.bsr_row.p, L.KE.p,
+                        L.bsr64.p, st_);
+    if (precision == 1)
+      launch_convert<double, float>(9 * (int64_t)L.nnzb, L.bsr64.p, L.bsr32.p, st_);
+  }
+  GCHK(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+void Problem::sor_smooth(int l, const T *b, T *x, int nrhs, bool from_zero) {
+  Level &L = levels_[l];
+  if (from_zero) launch_fill<T>(L.ndof * (int64_t)nrhs, T(0), x, st_);
+  BSR<T> A = bsr<T>(l);
+  A.run = run_;
+  const int nc = (int)L.sor_cptr.size() - 1;
+  launch_sor_sweep<T>(A, L.sor_nodes.p, L.sor_cptr.data(), nc, false, b, x, omega_, st_, nrhs,
+                      L.ndof);
+  launch_sor_sweep<T>(A, L.sor_nodes.p, L.sor_cptr.data(), nc, true, b, x, omega_, st_, nrhs,
+                      L.ndof);
+}
+
+template void Problem::sor_smooth<float>(int, const float *, float *, int, bool);
+template void Problem::sor_smooth<double>(int, const double *, double *, int, bool);
+
+int Problem::set_smoother(bool sor, double omega) {
+  if (!(omega > 0.0 && omega < 2.0)) {
+    last_error = "mg: omega must lie in (0, 2)";
+    return -1;
+  }
+  if (sor != sor_ || omega != omega_) {
+    sor_ = sor;
+    omega_ = omega;
+    have_setup_ = false;
+    drop_graphs();
+    gen_++;
+  }
+  return 0;
+}
+
+int Problem::smoother_order(int l, std::vector<int> &nodes, std::vector<int> &cptr) {
+  if (l < 0 || l >= (int)levels_.size() || !sor_ || !levels_[l].has_sor) {
+    last_error = "smoother_order: no SOR ordering on that level (set SOR smoothing and run a setup)";
+    return -1;
+  }
+  nodes = levels_[l].sor_nodes_h;
+  cptr = levels_[l].sor_cptr;
   return 0;
 }
 

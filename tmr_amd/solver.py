@@ -212,12 +212,28 @@ class Problem:
                       use_chebyshev_smoother=True):
         """TMR_CreateTACSMg's options on this hierarchy (tmrx_mg_create):
         Galerkin coarse operators P^T A P or rediscretised coarse levels; the
-        coarsest level solved directly or smoothed (one Chebyshev sweep)."""
+        coarsest level solved directly or smoothed once; Chebyshev
+        smoothing, or symmetric SOR with relaxation omega
+        (use_chebyshev_smoother=False)."""
         check(lib().tmrx_mg_create(self.ptr, float(omega), int(bool(use_galerkin)),
                                    int(bool(use_coarse_direct_solve)),
                                    int(bool(use_chebyshev_smoother))), "set_multigrid")
         self.galerkin = bool(use_galerkin)
         self._bump()
+
+    def smoother_order(self, level):
+        """(nodes, color_ptr) of level's SOR sweep after a setup with SOR
+        smoothing: the level's nodes in colour-major order, colour c being
+        nodes[color_ptr[c]:color_ptr[c + 1]]."""
+        n, nc = C.c_int(), C.c_int()
+        check(lib().tmrx_problem_smoother_order(self.ptr, int(level), C.byref(n), C.byref(nc),
+                                                None, None), "smoother_order")
+        nodes = np.zeros(n.value, np.intc)
+        cptr = np.zeros(nc.value + 1, np.intc)
+        check(lib().tmrx_problem_smoother_order(self.ptr, int(level), None, None,
+                                                nodes.ctypes.data_as(_lib.IP),
+                                                cptr.ctypes.data_as(_lib.IP)), "smoother_order")
+        return nodes, cptr
 
     def level_matrix(self, level):
         """The Galerkin operator of a coarse level as a scipy BSR matrix

@@ -319,19 +319,15 @@ class Mg:
     """The multigrid-preconditioned solver of a level hierarchy (TACSMg from
     TMR_CreateTACSMg, src/TMR_RefinementTools.cpp:37-107).  ``use_galerkin``
     forms the coarse operators P^T A P on the GPU (tmrx_mg_create).  The
-    smoother is Chebyshev on the Jacobi-scaled operator; the coarsest level
-    is a direct FP64 Cholesky solve, or with ``use_coarse_direct_solve=False``
-    one Chebyshev sweep (TMR hands TACSMg the level's smoother there,
-    src/TMR_RefinementTools.cpp:87-102).  TMR's default symmetric SOR
-    (``use_chebyshev_smoother=False``) is replaced by Chebyshev smoothing
-    with a warning."""
+    smoother is TMR's default symmetric SOR with relaxation ``omega`` on
+    the assembled level matrices (``use_chebyshev_smoother=False``, swept in
+    colour-major node order), or Chebyshev on the Jacobi-scaled operator;
+    the coarsest level is a direct FP64 Cholesky solve, or with
+    ``use_coarse_direct_solve=False`` the level's smoother once (TMR hands
+    TACSMg the smoother there, src/TMR_RefinementTools.cpp:87-102)."""
 
     def __init__(self, assemblers, forests, omega=1.0, use_galerkin=False,
                  use_coarse_direct_solve=True, use_chebyshev_smoother=False, device=0):
-        if not use_chebyshev_smoother:
-            import warnings
-            warnings.warn("symmetric SOR smoothing is not available on the GPU; the levels "
-                          "are smoothed with Chebyshev", stacklevel=2)
         if len(assemblers) != len(forests) or len(assemblers) < 1:
             raise ValueError("createMg: list length mismatch")
         a0 = assemblers[0]
@@ -346,7 +342,7 @@ class Mg:
             self.problem.set_bc_values(a0.bcs.values_vector(self.forests[0]))
         self.problem.set_multigrid(omega=omega, use_galerkin=use_galerkin,
                                    use_coarse_direct_solve=use_coarse_direct_solve,
-                                   use_chebyshev_smoother=True)
+                                   use_chebyshev_smoother=use_chebyshev_smoother)
         self.opts = solver.default_opts()
 
     def getAssembler(self):
