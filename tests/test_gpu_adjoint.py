@@ -81,7 +81,7 @@ def test_autograd_functions(adapt):
     g_c = r.grad.detach().cpu().numpy().copy()
     prob.set_design(rho)
     prob.solve(f, opts=solver.default_opts(rtol=1e-10))
-    assert float(c) == pytest.approx(prob.compliance(), rel=1e-9)
+    assert float(c.detach()) == pytest.approx(prob.compliance(), rel=1e-9)
     assert np.allclose(g_c, prob.compliance_grad(), rtol=1e-8, atol=0)
     r.grad = None
     s = ag.stress_pnorm(prob, r, f, P=8.0)
