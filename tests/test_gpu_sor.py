@@ -135,3 +135,66 @@ def test_sor_mg_solves(hier, galerkin, precision):
     U = prob.solve_multi(F, opts=opts)
     u2 = prob.solve(F[1], opts=opts)
     assert _rel(U[0], u) < 1e-12 and _rel(U[1], u2) < 1e-12
+
+
+def test_sor_coarse_smoothed_galerkin(hier):
+    """SOR on every level, the coarsest included (use_coarse_direct_solve =
+    0: TMR_CreateTACSMg gives that level its TACSGaussSeidel,
+    src/TMR_RefinementTools.cpp:87-102)."""
+    forests, rho, orc = hier
+    nl = len(forests)
+    lc = nl - 1
+    mg = po.Multigrid(orc, forests, BCS)
+    mg.setup("sor", omega=1.0, coarse_direct=False)
+    free = [_free(m) for m in mg.masks]
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_multigrid(use_galerkin=True, use_coarse_direct_solve=False,
+                       use_chebyshev_smoother=False)
+    prob.set_design(rho)
+    opts = solver.default_opts(precision=0)
+    prob.mg_op("apply", 0, np.zeros(len(free[0])), opts)
+    for l in range(nl):
+        nodes, cptr = prob.smoother_order(l)
+        _check_colouring(mg.matrix(l), nodes, cptr)
+        mg.set_order(l, nodes)
+    rng = np.random.default_rng(41)
+    b = np.where(free[lc], rng.standard_normal(len(free[lc])), 0.0)
+    ec = _rel(prob.mg_op("coarse", lc, b, opts)[0], mg.op("coarse", lc, b))
+    b0 = np.where(free[0], rng.standard_normal(len(free[0])), 0.0)
+    ev = _rel(prob.mg_op("vcycle", 0, b0, opts)[0], mg.op("vcycle", 0, b0))
+    print("SOR on the coarsest level: coarse op vs oracle %.2e, V-cycle %.2e" % (ec, ev),
+          flush=True)
+    assert ec < 1e-12 and ev < 1e-10
+
+
+def _nodal_x(forest, seed):
+    """A smooth seeded nodal field in [0.05, 1] on the independent nodes."""
+    nums = np.asarray(forest.getNodeNumbers())
+    X = forest.getPoints()[nums >= 0]
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(1.0, 3.0, 3)
+    v = 0.55 + 0.45 * np.sin(k[0] * X[:, 0]) * np.cos(k[1] * X[:, 1] + k[2] * X[:, 2])
+    return np.clip(v, 0.05, 1.0)
+
+
+@pytest.mark.parametrize("galerkin", [False, True])
+def test_sor_nodal_design(galerkin):
+    """Nodal design variables (per-Gauss-point stiffness): the assembled
+    level matrices SOR smooths come from the Gauss-point element matrices;
+    the SOR-MG solve reaches the oracle's nodal solve."""
+    forests, _, _ = workloads.adaptive(seed=3, base_level=2, max_level=4, nlevels=3)
+    x = _nodal_x(forests[0], 11)
+    prob = solver.Problem(forests, q=Q, k0=K0, bcs=BCS)
+    prob.set_design_nodal(x)
+    prob.set_multigrid(use_galerkin=galerkin, use_chebyshev_smoother=False)
+    f = prob.traction_load("load", (0.0, 0.0, -1.0))
+    orc = po.from_forest(forests[0], BCS)
+    po.nodal_assemble(orc, x, Q, K0, 0)
+    u_ref = orc.solve(f, rtol=1e-13)
+    for precision in (0, 1):
+        u = prob.solve(f, opts=solver.default_opts(precision=precision, rtol=1e-10,
+                                                   max_iters=400))
+        d = _rel(u, u_ref)
+        print("nodal SOR-MG galerkin %d precision %d: %d iterations, vs oracle %.2e"
+              % (galerkin, precision, prob.info.iters, d), flush=True)
+        assert prob.info.converged and d < 1e-7
