@@ -301,7 +301,10 @@ void launch_bsr_apply(const BSR<T> &A, const T *x, int mode, const AsmArgs &g, h
 // ---------------------------------------------------------------------------
 // One colour of a point SOR sweep on the BSR matrix (TACSGaussSeidel's
 // smoother, symmetric SOR with relaxation omega; TMR_CreateTACSMg,
-// src/TMR_RefinementTools.cpp:93-94).  Every node of the colour takes
+// src/TMR_RefinementTools.cpp:93-94).  The matrix is the level's BSR with
+// its rows in the sweep's colour-major node order (row k holds node
+// nodes[k]; columns are node indices), so a colour's rows are contiguous.
+// Every node of the colour takes
 // SOR_LANES lanes: they sum the row's off-diagonal blocks (consecutive
 // lanes on consecutive blocks, which read x of other colours only, so the
 // nodes of a colour update side by side) and a butterfly combines them;
@@ -330,8 +333,8 @@ __global__ __launch_bounds__(256) void k_sor_colour(int cnt, const int *__restri
   T s0 = T(0), s1 = T(0), s2 = T(0);
   int kd = -1;
   if (valid) {
-    const int q1 = ptr[i + 1];
-    for (int q = ptr[i] + sub; q < q1; q += SOR_LANES) {
+    const int q1 = ptr[k + 1];
+    for (int q = ptr[k] + sub; q < q1; q += SOR_LANES) {
       const int j = col[q];
       if (j == i) {
         kd = q;
@@ -369,6 +372,23 @@ __global__ __launch_bounds__(256) void k_sor_colour(int cnt, const int *__restri
   for (int c = 0; c < 3; c++) x[i3 + c] = xi[c];
 }
 
+// val_out[k] = val[map[k]] for every 3 x 3 block (the colour-major copy)
+__global__ void k_bsr_gather(int nnzb, const int *__restrict__ map, const double *__restrict__ val,
+                             double *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 9 * (int64_t)nnzb) return;
+  const int64_t k = t / 9, e = t % 9;
+  out[t] = val[9 * (int64_t)map[k] + e];
+}
+
+void launch_bsr_gather(int nnzb, const int *map, const double *val, double *out, hipStream_t st) {
+  if (nnzb == 0) return;
+  const int64_t n = 9 * (int64_t)nnzb;
+  hipLaunchKernelGGL(k_bsr_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nnzb, map,
+                     val, out);
+  TMRX_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_sor_sweep(const BSR<T> &A, const int *nodes, const int *cptr, int ncolors, bool back,
                       const T *b, T *x, double omega, hipStream_t st, int nrhs, int64_t vs) {
@@ -378,11 +398,11 @@ void launch_sor_sweep(const BSR<T> &A, const int *nodes, const int *cptr, int nc
     if (cnt == 0) continue;
     const dim3 grid((unsigned)(((int64_t)cnt * SOR_LANES + 255) / 256), (unsigned)nrhs);
     if (back)
-      hipLaunchKernelGGL((k_sor_colour<T, true>), grid, dim3(256), 0, st, cnt, nodes + c0, A.ptr,
-                         A.col, A.val, b, x, (T)omega, A.run, vs);
+      hipLaunchKernelGGL((k_sor_colour<T, true>), grid, dim3(256), 0, st, cnt, nodes + c0,
+                         A.ptr + c0, A.col, A.val, b, x, (T)omega, A.run, vs);
     else
-      hipLaunchKernelGGL((k_sor_colour<T, false>), grid, dim3(256), 0, st, cnt, nodes + c0, A.ptr,
-                         A.col, A.val, b, x, (T)omega, A.run, vs);
+      hipLaunchKernelGGL((k_sor_colour<T, false>), grid, dim3(256), 0, st, cnt, nodes + c0,
+                         A.ptr + c0, A.col, A.val, b, x, (T)omega, A.run, vs);
     TMRX_LAUNCH_CHECK();
   }
 }

@@ -295,7 +295,10 @@ void launch_bsr_apply(const BSR<T> &A, const T *x, int mode, const AsmArgs &g, h
 
 // One point SOR sweep (forward, or BACK in reverse) over the colours of a
 // node colouring of A's graph: nodes[cptr[c] .. cptr[c + 1]) hold colour c
-// (cptr on the host, ncolors + 1 entries); x updated in place
+// (cptr on the host, ncolors + 1 entries) and A's block row k is node
+// nodes[k] (colour-major rows, node-index columns); x updated in place
+// out[k] = val[map[k]] per 3 x 3 block
+void launch_bsr_gather(int nnzb, const int *map, const double *val, double *out, hipStream_t st);
 template <class T>
 void launch_sor_sweep(const BSR<T> &A, const int *nodes, const int *cptr, int ncolors, bool back,
                       const T *b, T *x, double omega, hipStream_t st, int nrhs = 1,

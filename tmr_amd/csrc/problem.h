@@ -175,6 +175,11 @@ struct Level {
   DevArray<int> sor_nodes;
   std::vector<int> sor_cptr, sor_nodes_h;
   bool has_sor = false;
+  // the BSR matrix with its rows in that order (row k = node sor_nodes[k]),
+  // sor_map[k'] = the block of the level's BSR that block k' copies
+  DevArray<int> sor_ptr, sor_col, sor_map;
+  DevArray<double> sor_val64;
+  DevArray<float> sor_val32;
   // diagonal terms of corner pairs that share an independent node
   // (elements with hanging corners): per node i, codes 64 e + 8 a + b and
   // weights 2 Z_ai Z_bi

@@ -337,7 +337,21 @@ int Problem::build_sor(int l) {
   for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
   std::vector<int> fill(cptr.begin(), cptr.end() - 1);
   for (int i = 0; i < n; i++) nodes[fill[colour[i]]++] = i;
-  if (L.sor_nodes.upload(nodes)) {
+  // the matrix in sweep order: a colour's rows contiguous
+  std::vector<int> pptr(n + 1, 0), pcol, pmap;
+  pcol.reserve(col.size());
+  pmap.reserve(col.size());
+  for (int k = 0; k < n; k++) {
+    const int i = nodes[k];
+    for (int q = ptr[i]; q < ptr[i + 1]; q++) {
+      pcol.push_back(col[q]);
+      pmap.push_back(q);
+    }
+    pptr[k + 1] = (int)pcol.size();
+  }
+  if (L.sor_nodes.upload(nodes) || L.sor_ptr.upload(pptr) || L.sor_col.upload(pcol) ||
+      L.sor_map.upload(pmap) || L.sor_val64.alloc(9 * (size_t)L.nnzb) ||
+      L.sor_val32.alloc(9 * (size_t)L.nnzb)) {
     last_error = "sor: device allocation failed";
     return -1;
   }
@@ -365,9 +379,11 @@ int Problem::sor_assemble(int l, int precision) {
                          nodal, L.KE.p, st_);
     launch_bsr_assemble(bsr<double>(l), L.bt_ptr.p, L.bt_code.p, L.bt_w.p, L.bsr_row.p, L.KE.p,
                         L.bsr64.p, st_);
-    if (precision == 1)
-      launch_convert<double, float>(9 * (int64_t)L.nnzb, L.bsr64.p, L.bsr32.p, st_);
   }
+  // (Galerkin levels: galerkin_setup has filled bsr64) -> sweep order
+  launch_bsr_gather(L.nnzb, L.sor_map.p, L.bsr64.p, L.sor_val64.p, st_);
+  if (precision == 1)
+    launch_convert<double, float>(9 * (int64_t)L.nnzb, L.sor_val64.p, L.sor_val32.p, st_);
   GCHK(hipGetLastError());
   return 0;
 }
@@ -377,6 +393,12 @@ void Problem::sor_smooth(int l, const T *b, T *x, int nrhs, bool from_zero) {
   Level &L = levels_[l];
   if (from_zero) launch_fill<T>(L.ndof * (int64_t)nrhs, T(0), x, st_);
   BSR<T> A = bsr<T>(l);
+  A.ptr = L.sor_ptr.p;
+  A.col = L.sor_col.p;
+  if constexpr (sizeof(T) == 8)
+    A.val = L.sor_val64.p;
+  else
+    A.val = L.sor_val32.p;
   A.run = run_;
   const int nc = (int)L.sor_cptr.size() - 1;
   launch_sor_sweep<T>(A, L.sor_nodes.p, L.sor_cptr.data(), nc, false, b, x, omega_, st_, nrhs,
