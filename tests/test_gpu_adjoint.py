@@ -89,7 +89,7 @@ def test_autograd_functions(adapt):
     g_s = r.grad.detach().cpu().numpy()
     prob.set_design(rho)
     prob.solve(f, opts=solver.default_opts(rtol=1e-10))
-    assert float(s) == pytest.approx(prob.stress_pnorm(8.0), rel=1e-9)
+    assert float(s.detach()) == pytest.approx(prob.stress_pnorm(8.0), rel=1e-9)
     # same options as the autograd path (rtol 1e-10 for both solves)
     g_ref = prob.stress_pnorm_grad(8.0, opts=solver.default_opts(rtol=1e-10))
     assert np.allclose(g_s, g_ref, rtol=1e-12, atol=0)
