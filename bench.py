@@ -520,11 +520,21 @@ def main():
     if args.fp32_form is not None:
         prob.set_fp32_form(args.fp32_form)
 
+    rho_dev = torch.from_numpy(rho).cuda()
+
     def run(precision, steps, warmup, timing=False):
         opts = solver.default_opts(rtol=1e-8, precision=precision, max_iters=500,
                                    fp32_valu=int(args.fp32_valu), no_graph=int(args.no_graph))
         torch.cuda.synchronize()
         ts = time.perf_counter()
+        prob.setup(opts)
+        torch.cuda.synchronize()
+        first_setup_ms = 1e3 * (time.perf_counter() - ts)
+        # a design step of an optimisation loop: the next design (resident
+        # on the device) installed and the preconditioner set up again,
+        # without the first setup's one-time allocations
+        ts = time.perf_counter()
+        prob.set_design(rho_dev)
         prob.setup(opts)
         torch.cuda.synchronize()
         setup_ms = 1e3 * (time.perf_counter() - ts)
@@ -538,7 +548,7 @@ def main():
             iters.append(prob.info.iters)
         torch.cuda.synchronize()
         dt = time.perf_counter() - ts
-        res = dict(seconds=dt, iters=iters, setup_ms=setup_ms,
+        res = dict(seconds=dt, iters=iters, setup_ms=setup_ms, first_setup_ms=first_setup_ms,
                    true_res=prob.true_residual(), compliance=prob.compliance(),
                    u=u.cpu().numpy().copy())
         if timing:
@@ -605,8 +615,10 @@ def main():
         "true_residual": mixed["true_res"],
         "compliance": mixed["compliance"],
         "setup_ms": mixed["setup_ms"],
-        # a design step of an optimisation loop: the density-dependent setup
-        # (diagonals, eigenvalue estimates, coarse factor) plus one solve
+        "first_setup_ms": mixed["first_setup_ms"],
+        # a design step of an optimisation loop: the design installed and the
+        # density-dependent setup (diagonals, eigenvalue estimates, coarse
+        # factor) redone, plus one solve
         "designs_per_s": 1.0 / (mixed["setup_ms"] / 1e3 + mixed["seconds"] / args.steps),
         "tile_nodes": prob.tuning()[0],
         "fp32_operator": {"valu": "vector ALUs", "mfma": "matrix cores (bf16 three-way split)",
