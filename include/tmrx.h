@@ -500,7 +500,8 @@ int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_
    rediscretised or Galerkin); 1 y = R_l x (level l to l + 1, the transpose
    of createInterpolation's P, src/TMROctForest.cpp:6611-6793, clamped rows
    zero); 2 y = P_l x (level l + 1 to l); 3 y = the pre-smoothing sweep of
-   level l (TACSChebyshevSmoother's role, :64-71) with right-hand side x;
+   level l with right-hand side x (Chebyshev, TACSChebyshevSmoother's role,
+   :64-71, or one symmetric SOR sweep, TACSGaussSeidel's, :93-94);
    4 y = the coarsest level's solve (the direct solve, :83-86, or with
    use_coarse_direct_solve = 0 its Chebyshev sweep); 5 y = the V-cycle from
    level l.  x and y have the sizes of the input and output levels.

@@ -391,7 +391,8 @@ class Problem:
         """One multigrid component alone on `level` (tmrx_problem_mg_op):
         'apply' A_l x, 'restrict' R_l x (to level + 1), 'prolong' P_l x
         (from level + 1), 'smooth' the pre-smoothing sweep with right-hand
-        side x, 'coarse' the coarsest solve, 'vcycle' the V-cycle from
+        side x (Chebyshev or symmetric SOR), 'coarse' the coarsest level's
+        solve (direct or smoothed), 'vcycle' the V-cycle from
         level.  Runs in the V-cycle's precision (opts.precision).  Returns
         (y, lambda_max of the level)."""
         code = self.MG_OPS[op]
