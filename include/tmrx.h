@@ -506,7 +506,8 @@ int tmrx_problem_stress_pnorm_grad(tmrx_problem *p, double P, double *g, int on_
    use_coarse_direct_solve = 0 its Chebyshev sweep); 5 y = the V-cycle from
    level l.  x and y have the sizes of the input and output levels.
    lam_max (may be null) receives the level's estimate of the largest
-   eigenvalue of Dinv A (0 on a coarsest level solved directly).  Sets the preconditioner
+   eigenvalue of Dinv A (0 where no Chebyshev smoother runs: SOR-smoothed
+   levels and a coarsest level solved directly).  Sets the preconditioner
    up first if the design changed. */
 int tmrx_problem_mg_op(tmrx_problem *p, int op, int level, const double *x, double *y,
                        int on_device, const tmrx_solve_opts *opts, double *lam_max);

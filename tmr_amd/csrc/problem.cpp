@@ -2258,7 +2258,7 @@ int Problem::mg_op_t(int op, int l, const double *x, double *y, bool on_device,
       break;
     }
     case MG_COARSE:
-      // the direct solve, or the coarsest level's Chebyshev sweep
+      // the direct solve, or the coarsest level's smoother once
       if (coarse_direct_)
         launch_dense_solve<T>(coarse_, in, vo.x.p, st_, 1);
       else
